@@ -1,0 +1,224 @@
+/* tgx — MI355X-native SOAP descriptor and GAP energy/force path.
+ *
+ * C ABI of libtgx.so. Plain pointers and sizes only; every function returns an
+ * int status (TGX_OK = 0) and tgx_last_error() gives the message. Arrays at this
+ * boundary keep TurboGAP's conventions: Fortran column-major layouts, FP64, and
+ * 1-based atom indices in neighbour lists. Host arrays are caller-owned; device
+ * buffers are owned by the context.
+ *
+ * Reference interfaces replaced (paths relative to the TurboGAP tree):
+ *   tgx_init / tgx_finalize             src/gpu_context.f90:41-58   gpu_context_init / _finalize
+ *   tgx_memory_budget                   src/gpu_context.f90:100-162 gpu_memory_budget_init
+ *   tgx_set_system                      src/turbogap.f90:871-886    positions/species/box handed to the build
+ *   tgx_build_neighbors_list            src/neighbors.f90:243-605   build_neighbors_list (count phase)
+ *   tgx_get_neighbors_list              src/neighbors.f90:544-591   the allocatable outputs of build_neighbors_list
+ *   tgx_soap_model_create               src/read_files.f90:660-724, 4641-5236  the soap_turbo block + sparse set
+ *   tgx_soap_backend_begin / _end       src/gap.f90:49-64           soap_backend_begin / soap_backend_end
+ *   tgx_get_gap_soap                    src/gap_interface.f90:41-480 get_gap_soap (marshal, get_soap, prediction, fold)
+ *   tgx_get_soap                        soap_turbo get_soap, call shape src/gap_interface.f90:320-325
+ *   tgx_get_soap_energy_and_forces      src/gap.f90:66-259          get_soap_energy_and_forces
+ *   tgx_compute                         src/turbogap.f90:1253-1380,1941-1954  the per-step SOAP loop and totals
+ *   tgx_velocity_verlet etc.            src/md.f90:226-345          velocity_verlet / berendsen_thermostat / remove_cm_vel
+ */
+#ifndef TGX_H
+#define TGX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  TGX_OK = 0,
+  TGX_ERR_ARG = 1,         /* bad argument, size or state */
+  TGX_ERR_HIP = 2,         /* a HIP runtime call failed */
+  TGX_ERR_UNSUPPORTED = 3, /* a configuration this build does not implement */
+  TGX_ERR_NOMEM = 4        /* device allocation failed */
+};
+
+/* Evaluation paths for tgx_compute / tgx_get_gap_soap. */
+enum {
+  TGX_PATH_FP64_PLAIN = 0,   /* FP64; dq/dr materialised, then contracted (the reference's data flow) */
+  TGX_PATH_FP64_ADJOINT = 1, /* FP64; forces from dE/dq contracted with recomputed dq/dr */
+  TGX_PATH_MIXED = 2,        /* FP32 descriptors, BF16 MFMA kernel with FP32 accumulation, FP64 sums */
+  TGX_PATH_SURROGATE = 3     /* MLP surrogate per species (FP32 descriptors, BF16 MLP) */
+};
+
+typedef struct tgx_ctx tgx_ctx;
+
+/* Hypers of one soap_turbo block (src/types.f90:75-134). Per-species arrays have
+ * n_species entries. basis: 0 = "poly3", 1 = "poly3gauss". rcut_soft is the soft
+ * cutoff (rcut - buffer, src/read_files.f90:5101-5109). n_quad_soft/n_quad_buffer:
+ * Gauss-Legendre nodes of the radial projection on [0, rcut_soft] and
+ * [rcut_soft, rcut_hard] (48 nodes on [0, rcut_hard] when there is no buffer);
+ * their sum must not exceed 64. */
+typedef struct tgx_soap_hypers {
+  int n_species;
+  int l_max;
+  int basis;
+  int radial_enhancement;
+  int n_quad_soft;
+  int n_quad_buffer;
+  const int* alpha_max;
+  const double* rcut_hard;
+  const double* rcut_soft;
+  const double* nf;
+  const double* global_scaling;
+  const double* atom_sigma_r;
+  const double* atom_sigma_r_scaling;
+  const double* atom_sigma_t;
+  const double* atom_sigma_t_scaling;
+  const double* amplitude_scaling;
+  const double* central_weight;
+} tgx_soap_hypers;
+
+/* ---- context ------------------------------------------------------------ */
+/* gpu_context_init: bind a HIP device, create the context's stream. */
+int tgx_init(int device, tgx_ctx** ctx);
+/* gpu_context_finalize: free every device buffer and the stream. */
+int tgx_finalize(tgx_ctx* ctx);
+/* Message of the last failing call on this thread (never NULL). */
+const char* tgx_last_error(void);
+/* gpu_memory_budget_init: free and total device memory in GiB. */
+int tgx_memory_budget(tgx_ctx* ctx, double* free_gib, double* total_gib);
+/* Name of the device the context is bound to. */
+int tgx_device_name(tgx_ctx* ctx, char* buf, int buflen);
+
+/* ---- system and neighbour list ---------------------------------------- */
+/* Upload positions (3, n_sites_sc) column-major, the integer species code of
+ * every supercell atom, and the supercell box (a, b, c as three 3-vectors). The
+ * first n_sites atoms are the unit cell (src/read_files.f90:320-367 order). */
+int tgx_set_system(tgx_ctx* ctx, int n_sites, int n_sites_sc, const double* positions, const int* species_sc,
+                   const double* box);
+/* Replace positions (host pointer, same shape as in tgx_set_system). */
+int tgx_set_positions(tgx_ctx* ctx, const double* positions);
+/* Replace positions from a device pointer holding (3, n_sites_sc) doubles. */
+int tgx_set_positions_device(tgx_ctx* ctx, const double* d_positions);
+/* build_neighbors_list on the device, in the reference's order. indices is the
+ * supercell repetition count (1,1,1 when the box holds a cutoff sphere);
+ * do_list may be NULL (all sites). Writes the number of pairs. */
+int tgx_build_neighbors_list(tgx_ctx* ctx, double rcut_max, const int* indices, const int* do_list,
+                             int64_t* n_atom_pairs);
+/* Copy the list to host arrays (any may be NULL): n_neigh(n_sites),
+ * neighbors_list(P) 1-based supercell indices, rjs(P), thetas(P), phis(P),
+ * xyz(3,P), neighbor_species(P). */
+int tgx_get_neighbors_list(tgx_ctx* ctx, int* n_neigh, int* neighbors_list, double* rjs, double* thetas,
+                           double* phis, double* xyz, int* neighbor_species);
+
+/* ---- models -------------------------------------------------------------- */
+/* One soap_turbo block and its sparse set: species_types holds the species
+ * codes of the block's species list; central_species is a 1-based entry of it
+ * (0 = every atom is a centre). Qs is (dim, n_sparse) column-major, alphas
+ * (n_sparse). The sparse set is uploaded once and stays resident. */
+int tgx_soap_model_create(tgx_ctx* ctx, const tgx_soap_hypers* hypers, const int* species_types,
+                          int central_species, double delta, double zeta, double e0, int n_sparse, int dim,
+                          const double* Qs, const double* alphas, int* model_id);
+/* soap_backend_begin / soap_backend_end: select the block the next
+ * tgx_get_gap_soap / tgx_get_soap / tgx_get_soap_energy_and_forces call uses. */
+int tgx_soap_backend_begin(tgx_ctx* ctx, int model_id);
+int tgx_soap_backend_end(tgx_ctx* ctx);
+/* Descriptor size of a model (N(N+1)/2 (l_max+1), N = sum alpha_max). */
+int tgx_soap_model_dim(tgx_ctx* ctx, int model_id, int* dim);
+
+/* get_gap_soap for the current block over the resident neighbour list:
+ * energies(n_sites) and forces(3, n_sites) are ASSIGNED for the atoms this
+ * block touches (src/gap_interface.f90:422-431) and left alone elsewhere;
+ * virial(3,3) is written. path: TGX_PATH_FP64_PLAIN or TGX_PATH_FP64_ADJOINT or
+ * TGX_PATH_MIXED. n_sites_block / n_pairs_block may be NULL. */
+int tgx_get_gap_soap(tgx_ctx* ctx, int path, int do_forces, double* energies, double* forces, double* virial,
+                     int* n_sites_block, int64_t* n_pairs_block);
+/* get_soap for the current block: soap(dim, n_sites_block) in the block's
+ * central-site order and, when soap_cart_der is not NULL, soap_cart_der(3, dim,
+ * n_pairs_block) in the block's pair order (self pair first per site). Sizes
+ * come from a preceding tgx_get_gap_soap or tgx_soap_block_size call. */
+int tgx_soap_block_size(tgx_ctx* ctx, int* n_sites_block, int64_t* n_pairs_block);
+int tgx_get_soap(tgx_ctx* ctx, double* soap, double* soap_cart_der);
+/* get_soap_energy_and_forces for the current block, host inputs: soap(n_soap,
+ * n_sites), soap_der(3, n_soap, n_pairs), neighbors_list 1-based local indices,
+ * xyz(3, n_pairs); outputs energies(n_sites), forces(3, n_sites0), virial(3,3). */
+int tgx_get_soap_energy_and_forces(tgx_ctx* ctx, int n_sites, int n_soap, int64_t n_pairs, const double* soap,
+                                   const double* soap_der, double e0, const int* n_neigh,
+                                   const int* neighbors_list, const double* xyz, int n_sites0, int do_forces,
+                                   double* energies, double* forces, double* virial);
+
+/* ---- whole-step evaluation ------------------------------------------------ */
+/* E, F and virial of every model on the resident positions (neighbour list must
+ * have been built). Per-atom energies include each model's e0 for its central
+ * atoms. Results stay on the device until tgx_get_results. */
+int tgx_compute(tgx_ctx* ctx, int path, double* energy, double* virial);
+/* Copy energies(n_sites) and forces(3, n_sites) to the host (either may be NULL). */
+int tgx_get_results(tgx_ctx* ctx, double* energies, double* forces);
+/* Device pointer of the resident forces (3, n_sites) and energies (n_sites). */
+int tgx_results_device(tgx_ctx* ctx, double** d_forces, double** d_energies);
+
+/* ---- surrogate ----------------------------------------------------------- */
+/* One MLP per model (dim -> h -> h -> 1, SiLU), weights given in FP32 as
+ * W1(h, dim) b1(h) W2(h, h) b2(h) w3(h) b3(1), row-major, plus the descriptor
+ * shift/scale it was trained with. Stored as BF16 for the GEMMs. */
+int tgx_surrogate_set(tgx_ctx* ctx, int model_id, int member, int hidden, const float* W1, const float* b1,
+                      const float* W2, const float* b2, const float* w3, const float* b3, double e_shift,
+                      double e_scale);
+/* Ensemble gate: members > 1 per model, spread threshold in eV/atom; atoms over
+ * it are recomputed by the FP64 GAP path. Writes the gated fraction. */
+int tgx_surrogate_gate(tgx_ctx* ctx, int n_members, double threshold);
+int tgx_surrogate_stats(tgx_ctx* ctx, double* gated_fraction, double* max_spread);
+
+/* ---- MLP training kernels (device pointers) ------------------------------ */
+/* Train state for one network; see turbogap_amd/surrogate.py for the loss. */
+typedef struct tgx_mlp tgx_mlp;
+int tgx_mlp_create(tgx_ctx* ctx, int dim, int hidden, int max_batch, uint64_t seed, tgx_mlp** mlp);
+int tgx_mlp_destroy(tgx_mlp* mlp);
+/* One Adam step on a batch held in device memory: q(B, dim) FP32 descriptors,
+ * e_ref(B) energies, g_ref(B, dim) FP32 descriptor-space energy gradients.
+ * Returns the energy and gradient losses of the batch. */
+int tgx_mlp_train_step(tgx_mlp* mlp, int batch, const float* d_q, const float* d_e_ref, const float* d_g_ref,
+                       float lr, float w_energy, float w_grad, double* loss_e, double* loss_g);
+/* Forward (+ input gradient when d_g != NULL) on device data. */
+int tgx_mlp_forward(tgx_mlp* mlp, int batch, const float* d_q, float* d_e, float* d_g);
+/* Copy weights out (FP32 master copy) in tgx_surrogate_set's layout. */
+int tgx_mlp_get_weights(tgx_mlp* mlp, float* W1, float* b1, float* W2, float* b2, float* w3, float* b3);
+int tgx_mlp_set_weights(tgx_mlp* mlp, const float* W1, const float* b1, const float* W2, const float* b2,
+                        const float* w3, const float* b3);
+/* Descriptors (FP32) and GAP descriptor-space gradients of the last tgx_compute
+ * for one model, device pointers valid until the next compute. */
+int tgx_model_training_view(tgx_ctx* ctx, int model_id, int* n_sites, const float** d_q, const float** d_g,
+                            const double** d_e, const int** d_site_atom);
+
+/* ---- MD (src/md.f90:226-345, src/turbogap_md.f90:160-470) --------------- */
+/* Velocities (3, n_sites) and masses (n_sites) in eV fs^2/A^2 live on the device. */
+int tgx_md_set_state(tgx_ctx* ctx, const double* velocities, const double* masses);
+int tgx_md_get_state(tgx_ctx* ctx, double* positions, double* velocities);
+/* velocity_verlet (md.f90:226-286) with the resident forces and forces_prev. */
+int tgx_velocity_verlet(tgx_ctx* ctx, double dt, int first_step);
+/* E_kin and instantaneous T = 2/(3 (N-1) kB) E_kin (turbogap_md.f90:281-295). */
+int tgx_kinetic(tgx_ctx* ctx, double* e_kin, double* temperature);
+/* berendsen_thermostat (md.f90:289-313). */
+int tgx_berendsen(tgx_ctx* ctx, double T0, double T, double tau, double dt);
+/* remove_cm_vel (md.f90:316-345). */
+int tgx_remove_cm_vel(tgx_ctx* ctx);
+/* wrap positions back into the orthorhombic box. */
+int tgx_wrap_pbc(tgx_ctx* ctx);
+
+/* ---- profiling ----------------------------------------------------------- */
+/* Record HIP events around every kernel launch on the context's stream. */
+int tgx_set_profiling(tgx_ctx* ctx, int on);
+/* Launch count and total milliseconds of kernels whose name contains `name`. */
+int tgx_kernel_time(tgx_ctx* ctx, const char* name, int* count, double* total_ms);
+int tgx_reset_profile(tgx_ctx* ctx);
+int tgx_synchronize(tgx_ctx* ctx);
+
+/* ---- synthetic inputs (host only; no GPU needed) -------------------------- */
+/* Random sequential addition of n points in an orthorhombic box with minimum
+ * distance dmin (periodic). Returns TGX_ERR_ARG if the packing fails. */
+int tgx_generate_rsa(int n, const double* box_lengths, double dmin, uint64_t seed, double* positions);
+/* Seeded Fisher-Yates permutation of 0..n-1. */
+int tgx_generate_permutation(int n, uint64_t seed, int* perm);
+/* Seeded normal deviates with the given standard deviation. */
+int tgx_generate_normal(int n, double stddev, uint64_t seed, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TGX_H */

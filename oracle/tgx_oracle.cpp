@@ -1,0 +1,994 @@
+// tgx oracle — FP64 CPU restatement of TurboGAP's SOAP -> GAP energy/force path.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the correctness checker for the HIP
+// library under turbogap_amd/csrc. Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py may load it, and only to check or to time a CPU
+// baseline. Nothing in the product path calls it.
+//
+// Build: oracle/Makefile (g++ -O2 -fopenmp -ffp-contract=off). -ffp-contract=off
+// is required: the neighbour list must come from the same IEEE operation
+// sequence as the reference (no fused multiply-adds).
+//
+// What each part follows (reference paths relative to the TurboGAP tree):
+//   orc_get_distance                 src/neighbors.f90:47-163
+//   orc_number_of_unit_cells         src/neighbors.f90:175-236
+//   orc_build_supercell              src/read_files.f90:320-367
+//   orc_build_neighbors_list         src/neighbors.f90:243-605
+//   orc_soap_batch_memory_model      src/neighbors.f90:637-663
+//   orc_get_number_of_atom_pairs     src/neighbors.f90:667-778
+//   orc_get_soap                     soap_turbo get_soap (source absent from the
+//                                    tree; call shape src/gap_interface.f90:320-325).
+//                                    Restated from M. A. Caro, PRB 100, 024112 (2019)
+//                                    with the conventions listed at soap_conventions below.
+//   orc_get_soap_energy_and_forces   src/gap.f90:66-259
+//   orc_gap_soap                     src/gap_interface.f90:180-431 (marshal + fold)
+//
+// Parity status per function is recorded in the header of tests/test_oracle_*.py
+// and in the design notes: neighbour list, contraction and scatter are pinned by
+// brute force, closed forms and finite differences; the SOAP conventions that only
+// the absent soap_turbo source could fix are "parity unpinned" against soap_turbo
+// and pinned internally (closed forms, invariances, finite differences).
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// Fortran intrinsic semantics used by the reference (gfortran build).
+// MODULO for reals: gfortran evaluates fmod(a,p) and adds p when the result is
+// nonzero and a, p differ in sign; a zero result takes the sign of p.
+static inline double f_modulo(double a, double p) {
+  double r = std::fmod(a, p);
+  if (r != 0.0) {
+    if ((a < 0.0) != (p < 0.0)) r = r + p;
+  } else {
+    r = std::copysign(0.0, p);
+  }
+  return r;
+}
+// MODULO for integers: result carries the sign of p.
+static inline int i_modulo(int a, int p) {
+  int r = a % p;
+  if (r != 0 && ((r < 0) != (p < 0))) r += p;
+  return r;
+}
+
+// src/neighbors.f90:47-163. posi, posj, a, b, c are 3-vectors. PBC all true
+// on every call site of the hot path. Returns the displacement dist = r_j - r_i
+// under the minimum-image convention and its norm d.
+struct tri_cache { double a0[3], b0[3], c0[3], mat_inv[9]; bool valid; };
+static thread_local tri_cache g_tri = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0}, false};
+
+static inline double dot3(const double* u, const double* v) {
+  // Fortran dot_product over 3 elements, summed in index order.
+  double s = u[0] * v[0];
+  s = s + u[1] * v[1];
+  s = s + u[2] * v[2];
+  return s;
+}
+
+void orc_get_distance(const double* posi, const double* posj, const double* a, const double* b,
+                      const double* c, double* dist, double* d_out) {
+  const double d_tol = 1.0e-6;
+  if (std::fabs(a[1]) < d_tol && std::fabs(a[2]) < d_tol && std::fabs(b[0]) < d_tol &&
+      std::fabs(b[2]) < d_tol && std::fabs(c[0]) < d_tol && std::fabs(c[1]) < d_tol) {
+    const double L[3] = {a[0], b[1], c[2]};
+    double d2 = 0.0;
+    for (int i = 0; i < 3; ++i) {
+      double di = f_modulo(posj[i] - posi[i], L[i]);
+      if (di > L[i] / 2.0) di = di - L[i];
+      dist[i] = di;
+      d2 = d2 + di * di;
+    }
+    *d_out = std::sqrt(d2);
+    return;
+  }
+  // Triclinic cell: 27-image search around the real-valued solution.
+  bool changed = !g_tri.valid;
+  for (int i = 0; i < 3; ++i)
+    if (a[i] != g_tri.a0[i] || b[i] != g_tri.b0[i] || c[i] != g_tri.c0[i]) changed = true;
+  if (changed) {
+    double m11 = dot3(a, a), m12 = dot3(a, b), m13 = dot3(a, c);
+    double m22 = dot3(b, b), m23 = dot3(b, c), m33 = dot3(c, c);
+    double md = -(m13 * m13) * m22 + 2.0 * m12 * m13 * m23 - m11 * (m23 * m23) -
+                (m12 * m12) * m33 + m11 * m22 * m33;
+    double* mi = g_tri.mat_inv;  // row-major here: mi[3*i+j] = mat_inv(i+1,j+1)
+    mi[0] = m22 * m33 - m23 * m23;
+    mi[1] = m13 * m23 - m12 * m33;
+    mi[2] = m12 * m23 - m13 * m22;
+    mi[3] = mi[1];
+    mi[4] = m11 * m33 - m13 * m13;
+    mi[5] = m12 * m13 - m11 * m23;
+    mi[6] = mi[2];
+    mi[7] = mi[5];
+    mi[8] = m11 * m22 - m12 * m12;
+    for (int k = 0; k < 9; ++k) mi[k] = mi[k] / md;
+    for (int i = 0; i < 3; ++i) { g_tri.a0[i] = a[i]; g_tri.b0[i] = b[i]; g_tri.c0[i] = c[i]; }
+    g_tri.valid = true;
+  }
+  double dd[3] = {posj[0] - posi[0], posj[1] - posi[1], posj[2] - posi[2]};
+  double v[3] = {-1.0 * dot3(dd, a), -1.0 * dot3(dd, b), -1.0 * dot3(dd, c)};
+  double ir[3];
+  for (int i = 0; i < 3; ++i) {
+    const double* mi = g_tri.mat_inv + 3 * i;
+    double s = mi[0] * v[0];
+    s = s + mi[1] * v[1];
+    s = s + mi[2] * v[2];
+    ir[i] = s;
+  }
+  int ind[3];
+  for (int i = 0; i < 3; ++i) ind[i] = (int)std::lround(ir[i]);
+  double res_opt = 1.0e10;
+  double best[3] = {dd[0], dd[1], dd[2]};
+  for (int i = ind[0] - 1; i <= ind[0] + 1; ++i)
+    for (int j = ind[1] - 1; j <= ind[1] + 1; ++j)
+      for (int k = ind[2] - 1; k <= ind[2] + 1; ++k) {
+        double t[3];
+        for (int q = 0; q < 3; ++q)
+          t[q] = dd[q] + (double)i * a[q] + (double)j * b[q] + (double)k * c[q];
+        double res = dot3(t, t);
+        if (res < res_opt) { res_opt = res; best[0] = t[0]; best[1] = t[1]; best[2] = t[2]; }
+      }
+  dist[0] = best[0]; dist[1] = best[1]; dist[2] = best[2];
+  *d_out = std::sqrt(dot3(best, best));
+}
+
+static inline void cross3(const double* u, const double* v, double* w) {
+  w[0] = u[1] * v[2] - u[2] * v[1];
+  w[1] = u[2] * v[0] - u[0] * v[2];
+  w[2] = u[0] * v[1] - u[1] * v[0];
+}
+
+// src/neighbors.f90:175-236: repetitions per lattice vector so that the
+// supercell's planes are at least 2*rcut apart.
+void orc_number_of_unit_cells(const double* a, const double* b, const double* c, double rcut,
+                              int* indices) {
+  double axb[3], bxc[3], cxa[3];
+  cross3(a, b, axb); cross3(c, a, cxa); cross3(b, c, bxc);
+  double n1 = std::sqrt(dot3(axb, axb)), n2 = std::sqrt(dot3(cxa, cxa)), n3 = std::sqrt(dot3(bxc, bxc));
+  for (int i = 0; i < 3; ++i) { axb[i] /= n1; cxa[i] /= n2; bxc[i] /= n3; }
+  double m[3][3] = {{dot3(axb, a), dot3(axb, b), dot3(axb, c)},
+                    {dot3(cxa, a), dot3(cxa, b), dot3(cxa, c)},
+                    {dot3(bxc, a), dot3(bxc, b), dot3(bxc, c)}};
+  double md = -m[0][2] * m[1][1] * m[2][0] + m[0][1] * m[1][2] * m[2][0] + m[0][2] * m[1][0] * m[2][1] -
+              m[0][0] * m[1][2] * m[2][1] - m[0][1] * m[1][0] * m[2][2] + m[0][0] * m[1][1] * m[2][2];
+  double mi[3][3];
+  mi[0][0] = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+  mi[0][1] = m[0][2] * m[2][1] - m[0][1] * m[2][2];
+  mi[0][2] = m[0][1] * m[1][2] - m[0][2] * m[1][1];
+  mi[1][0] = m[1][2] * m[2][0] - m[1][0] * m[2][2];
+  mi[1][1] = m[0][0] * m[2][2] - m[0][2] * m[2][0];
+  mi[1][2] = m[0][2] * m[1][0] - m[0][0] * m[1][2];
+  mi[2][0] = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+  mi[2][1] = m[0][1] * m[2][0] - m[0][0] * m[2][1];
+  mi[2][2] = m[0][0] * m[1][1] - m[0][1] * m[1][0];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) mi[i][j] /= md;
+  double v[3] = {2.0 * rcut, 2.0 * rcut, 2.0 * rcut};
+  for (int i = 0; i < 3; ++i) {
+    double s = mi[i][0] * v[0] + mi[i][1] * v[1] + mi[i][2] * v[2];
+    s = std::fabs(s);
+    int k = (int)s;
+    if (s > (double)k) k = k + 1;
+    indices[i] = k;
+  }
+}
+
+// src/read_files.f90:320-367: replicate positions/species; the supercell index
+// runs atoms fastest, then the c, b, a repetitions. Box vectors are scaled in place.
+void orc_build_supercell(int n_sites, const double* positions, const int* species, double* a, double* b,
+                         double* c, const int* indices, double* pos_sc, int* species_sc) {
+  long counter = 0;
+  for (int i2 = 0; i2 < indices[0]; ++i2)
+    for (int j2 = 0; j2 < indices[1]; ++j2)
+      for (int k2 = 0; k2 < indices[2]; ++k2)
+        for (int i = 0; i < n_sites; ++i) {
+          for (int q = 0; q < 3; ++q)
+            pos_sc[3 * counter + q] = positions[3 * i + q] + (double)i2 * a[q] + (double)j2 * b[q] + (double)k2 * c[q];
+          species_sc[counter] = species[i];
+          ++counter;
+        }
+  for (int q = 0; q < 3; ++q) { a[q] *= (double)indices[0]; b[q] *= (double)indices[1]; c[q] *= (double)indices[2]; }
+}
+
+// src/neighbors.f90:243-605. Positions are (3, n_sites_sc) column-major, i.e.
+// xyz fastest. Outputs use the reference's 1-based supercell indices.
+// Returns the number of atom pairs; the list arrays are written only when
+// cap >= that number (a first call with cap = 0 sizes them).
+long orc_build_neighbors_list(int n_sites, int n_sites_sc, const double* positions, double* a, double* b,
+                              double* c, const int* species_sc, double rcut_max, const int* do_list,
+                              const int* indices, int* n_neigh, long cap, int* neighbors_list, double* rjs,
+                              double* thetas, double* phis, double* xyz, int* neighbor_species) {
+  const double d_tol = 1.0e-6;
+  bool square = std::fabs(a[1]) < d_tol && std::fabs(a[2]) < d_tol && std::fabs(b[0]) < d_tol &&
+                std::fabs(b[2]) < d_tol && std::fabs(c[0]) < d_tol && std::fabs(c[1]) < d_tol;
+  if (square) { a[1] = a[2] = b[0] = b[2] = c[0] = c[1] = 0.0; }
+  bool small = indices[0] > 1 || indices[1] > 1 || indices[2] > 1;
+  std::vector<int> list;
+  list.reserve((size_t)n_sites * 32);
+  for (int i = 0; i < n_sites; ++i) n_neigh[i] = 0;
+  const double tol = 1.0e-10;
+  if (square && !small) {
+    int mx = (int)(a[0] / rcut_max), my = (int)(b[1] / rcut_max), mz = (int)(c[2] / rcut_max);
+    std::vector<int> head((size_t)mx * my * mz, 0), this_list(n_sites + 1, 0);
+    const double ctr[3] = {a[0] / 2.0, b[1] / 2.0, c[2] / 2.0};
+    for (int i = 1; i <= n_sites; ++i) {
+      double dist[3], d;
+      orc_get_distance(ctr, positions + 3 * (i - 1), a, b, c, dist, &d);
+      for (int q = 0; q < 3; ++q) dist[q] = dist[q] + ctr[q];
+      int j = 1 + i_modulo((int)(dist[0] / (a[0] + tol) * (double)mx), mx) +
+              i_modulo((int)(dist[1] / (b[1] + tol) * (double)my), my) * mx +
+              i_modulo((int)(dist[2] / (c[2] + tol) * (double)mz), mz) * my * mx;
+      this_list[i] = head[j - 1];
+      head[j - 1] = i;
+    }
+    for (int i = 1; i <= n_sites; ++i) {
+      if (do_list && !do_list[i - 1]) continue;
+      n_neigh[i - 1] += 1;
+      list.push_back(i);
+      double dist[3], d;
+      orc_get_distance(ctr, positions + 3 * (i - 1), a, b, c, dist, &d);
+      for (int q = 0; q < 3; ++q) dist[q] = dist[q] + ctr[q];
+      int i2 = 1 + (int)(dist[0] / (a[0] + tol) * (double)mx);
+      int j2 = 1 + (int)(dist[1] / (b[1] + tol) * (double)my);
+      int k2 = 1 + (int)(dist[2] / (c[2] + tol) * (double)mz);
+      for (int k3 = k2 - 1; k3 <= k2 + 1; ++k3) {
+        if (mz == 1 && k3 != 1) continue;
+        if (mz == 2 && k2 == 1 && k3 == 0) continue;
+        if (mz == 2 && k2 == 2 && k3 == 3) continue;
+        for (int j3 = j2 - 1; j3 <= j2 + 1; ++j3) {
+          if (my == 1 && j3 != 1) continue;
+          if (my == 2 && j2 == 1 && j3 == 0) continue;
+          if (my == 2 && j2 == 2 && j3 == 3) continue;
+          for (int i3 = i2 - 1; i3 <= i2 + 1; ++i3) {
+            if (mx == 1 && i3 != 1) continue;
+            if (mx == 2 && i2 == 1 && i3 == 0) continue;
+            if (mx == 2 && i2 == 2 && i3 == 3) continue;
+            int jc = 1 + i_modulo(i3 - 1, mx) + i_modulo(j3 - 1, my) * mx + i_modulo(k3 - 1, mz) * mx * my;
+            int k = head[jc - 1];
+            while (k != 0) {
+              if (k != i) {
+                double dd[3], dk;
+                orc_get_distance(positions + 3 * (i - 1), positions + 3 * (k - 1), a, b, c, dd, &dk);
+                if (dk < rcut_max) { n_neigh[i - 1] += 1; list.push_back(k); }
+              }
+              k = this_list[k];
+            }
+          }
+        }
+      }
+    }
+  } else {
+    for (int i = 1; i <= n_sites; ++i) {
+      if (do_list && !do_list[i - 1]) continue;
+      n_neigh[i - 1] += 1;
+      list.push_back(i);
+      for (int j = 1; j <= n_sites_sc; ++j) {
+        if (j == i) continue;
+        double dd[3], dk;
+        orc_get_distance(positions + 3 * (i - 1), positions + 3 * (j - 1), a, b, c, dd, &dk);
+        if (dk < rcut_max) { n_neigh[i - 1] += 1; list.push_back(j); }
+      }
+    }
+  }
+  long n_pairs = (long)list.size();
+  if (cap < n_pairs) return n_pairs;
+  long k2 = 0;
+  for (int i = 1; i <= n_sites; ++i) {
+    for (int k = 1; k <= n_neigh[i - 1]; ++k) {
+      int j = list[k2];
+      neighbors_list[k2] = j;
+      if (k == 1) {
+        rjs[k2] = 0.0;
+        xyz[3 * k2] = xyz[3 * k2 + 1] = xyz[3 * k2 + 2] = 0.0;
+        thetas[k2] = 0.0;
+        phis[k2] = 0.0;
+        neighbor_species[k2] = species_sc[i - 1];
+      } else {
+        double dd[3], d;
+        orc_get_distance(positions + 3 * (i - 1), positions + 3 * (j - 1), a, b, c, dd, &d);
+        rjs[k2] = d;
+        xyz[3 * k2] = dd[0]; xyz[3 * k2 + 1] = dd[1]; xyz[3 * k2 + 2] = dd[2];
+        if (std::fabs(dd[2]) >= d) thetas[k2] = dd[2] > 0.0 ? 0.0 : std::acos(-1.0);
+        else thetas[k2] = std::acos(dd[2] / d);
+        phis[k2] = std::atan2(dd[1], dd[0]);
+        neighbor_species[k2] = species_sc[j - 1];
+      }
+      ++k2;
+    }
+  }
+  return n_pairs;
+}
+
+// src/neighbors.f90:637-663
+void orc_soap_batch_memory_model(int l_max, int n_max, int n_soap, int n_species, double* bytes_per_pair,
+                                 double* bytes_per_site) {
+  int k_max = 1 + l_max * (l_max + 1) / 2 + l_max;
+  *bytes_per_pair = 48.0 * k_max * n_max + 64.0 * k_max + 48.0 * n_soap + 16.0 * n_max + 4.0 * n_species + 56.0;
+  *bytes_per_site = 16.0 * k_max * n_max + 8.0 * n_soap + 16.0;
+}
+
+// src/neighbors.f90:667-778. Output lists are 1-based like the reference.
+// Returns n_chunks; lists are written only when cap >= n_chunks.
+int orc_get_number_of_atom_pairs(int n_sites, const int* n_neigh, long n_atom_pairs, const double* rjs,
+                                 double rcut, int l_max, int n_max, int n_soap, int n_species,
+                                 double max_gbytes, int cap, int* i_beg, int* i_end, long* j_beg, long* j_end) {
+  long k = 0, n_in = 0;
+  for (int i = 0; i < n_sites; ++i)
+    for (int j = 0; j < n_neigh[i]; ++j, ++k)
+      if (rjs[k] < rcut) ++n_in;
+  double bpp, bps;
+  orc_soap_batch_memory_model(l_max, n_max, n_soap, n_species, &bpp, &bps);
+  double est = 1.10 * ((double)n_in * bpp + (double)n_sites * bps) / (1024.0 * 1024.0 * 1024.0);
+  double ratio = est / max_gbytes;
+  int n_chunks = (int)std::ceil(ratio);
+  if (n_chunks > n_sites) n_chunks = n_sites;
+  if (cap < n_chunks) return n_chunks;
+  if (n_chunks == 0) return 0;
+  double per_chunk = (double)n_in / (double)n_chunks;
+  i_beg[0] = 1; j_beg[0] = 1;
+  i_end[n_chunks - 1] = n_sites; j_end[n_chunks - 1] = n_atom_pairs;
+  if (n_chunks == 1) return 1;
+  k = 0;
+  long k2 = 0;
+  int ic = 1;
+  for (int i = 1; i <= n_sites; ++i) {
+    for (int j = 1; j <= n_neigh[i - 1]; ++j) {
+      ++k;
+      if (rjs[k - 1] < rcut) ++k2;
+    }
+    // int(float(i_chunk)*pairs_per_chunk): float() is single precision in Fortran.
+    if (k2 >= (long)((double)(float)ic * per_chunk)) {
+      i_end[ic - 1] = i; j_end[ic - 1] = k;
+      ++ic;
+      i_beg[ic - 1] = i + 1; j_beg[ic - 1] = k + 1;
+      if (ic == n_chunks) break;
+    }
+  }
+  return n_chunks;
+}
+
+// ===========================================================================
+// SOAP. soap_conventions (the choices soap_turbo would fix, written out):
+//  C1 Radial coordinate x = r / rcut_hard(s) per neighbour species s; the soft
+//     cutoff is xs = rcut_soft(s)/rcut_hard(s) (rcut_soft = rcut - buffer,
+//     src/read_files.f90:5101-5109).
+//  C2 Radial basis on [0,1] with unit weight. "poly3": Gram-Schmidt, in order, of
+//     (1-x)^3, (1-x)^4, ..., (1-x)^(alpha_max+2). "poly3gauss": Gram-Schmidt, in
+//     order, of exp(-x^2/(2 s0^2)) (s0 = atom_sigma_r/rcut_hard) followed by
+//     (1-x)^3 ... (1-x)^(alpha_max+1). The polynomial part is generated by a
+//     Stieltjes recurrence in t = 1-x with weight t^6, which is the same
+//     orthonormal set as ordered Gram-Schmidt but well conditioned.
+//  C3 Density of neighbour j: A(r_j) * exp(-(x - x_j)^2/(2 s_j^2)) * F(x) * x^re
+//     with s_j = atom_sigma_r (1 + atom_sigma_r_scaling r_j) / rcut_hard,
+//     F = 1 for x <= xs and exp(-nf (x-xs)^2/(1-xs)^2) beyond (the radial
+//     filter), re = radial_enhancement, and amplitude
+//     A = global_scaling / s_j * (1 + 2x_j^3 - 3x_j^2)^amplitude_scaling
+//     (scaling_mode "polynomial"; zero when the polynomial is <= 1e-10).
+//  C4 Radial projections are integrals over [0,xs] and [xs,1] by Gauss-Legendre
+//     rules of n_quad_soft and n_quad_buffer nodes (48 nodes on [0,1] with no
+//     buffer). The derivative is the exact derivative of this quadrature.
+//  C5 Angular part: real spherical harmonics (orthonormal, Condon-Shortley
+//     phase), damped by exp(-l(l+1) st^2 / (2 r^2)) with
+//     st = atom_sigma_t (1 + atom_sigma_t_scaling r). Directions come from xyz.
+//  C6 The central atom adds central_weight * global_scaling / s0 * radial
+//     projection of exp(-x^2/(2 s0^2)) to l = 0 of its own species' block.
+//  C7 Blocks are stacked by species in species order, N = sum alpha_max.
+//     p(n,n',l) = w(n,n') * sum_m c(n,l,m) c(n',l,m), w = sqrt(2) for n != n',
+//     ordered n = 1..N, n' = n..N, l = 0..l_max with l fastest
+//     (tools/compress_indices/compress_indices.py:39-57), D = N(N+1)/2 (l_max+1).
+//  C8 q = p / |p|. dq_i/dr_j per pair; the self pair is -sum of the others.
+//  C9 Neighbours with r >= rcut_hard(s) of their species contribute nothing.
+// ===========================================================================
+
+struct orc_soap_hypers {
+  int n_species;
+  int l_max;
+  int basis;               // 0 = poly3, 1 = poly3gauss
+  int radial_enhancement;  // 0, 1, 2
+  int n_quad_soft;         // Gauss-Legendre nodes on [0, xs]
+  int n_quad_buffer;       // nodes on [xs, 1]
+  const int* alpha_max;
+  const double* rcut_hard;
+  const double* rcut_soft;
+  const double* nf;
+  const double* global_scaling;
+  const double* atom_sigma_r;
+  const double* atom_sigma_r_scaling;
+  const double* atom_sigma_t;
+  const double* atom_sigma_t_scaling;
+  const double* amplitude_scaling;
+  const double* central_weight;
+};
+
+// Gauss-Legendre nodes/weights on [-1,1] by Newton iteration on P_n.
+static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(n, 0.0); w.assign(n, 0.0);
+  const double pi = std::acos(-1.0);
+  for (int i = 0; i < (n + 1) / 2; ++i) {
+    double z = std::cos(pi * (i + 0.75) / (n + 0.5)), pp = 0.0;
+    for (int it = 0; it < 100; ++it) {
+      double p1 = 1.0, p2 = 0.0;
+      for (int j = 1; j <= n; ++j) { double p3 = p2; p2 = p1; p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j; }
+      pp = n * (z * p1 - p2) / (z * z - 1.0);
+      double z1 = z;
+      z = z1 - p1 / pp;
+      if (std::fabs(z - z1) < 1e-16) break;
+    }
+    double p1 = 1.0, p2 = 0.0;
+    for (int j = 1; j <= n; ++j) { double p3 = p2; p2 = p1; p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j; }
+    pp = n * (z * p1 - p2) / (z * z - 1.0);
+    x[i] = -z; x[n - 1 - i] = z;
+    w[i] = w[n - 1 - i] = 2.0 / ((1.0 - z * z) * pp * pp);
+  }
+}
+
+struct radial_table {
+  int nb;                  // alpha_max
+  int nq;                  // nodes
+  std::vector<double> xq;  // nodes in x
+  std::vector<double> B;   // nb x nq: w_q psi_n(x_q) F(x_q) x_q^re
+  std::vector<double> psi; // nb x nq: basis values (for tests)
+  std::vector<double> wq;  // weights
+  std::vector<double> cen; // nb: central-atom projection (without Y00, weight or scaling)
+  double rc, xs, s0;
+};
+
+static void build_radial_table(const orc_soap_hypers& h, int s, radial_table& T) {
+  T.nb = h.alpha_max[s];
+  T.rc = h.rcut_hard[s];
+  T.xs = h.rcut_soft[s] / h.rcut_hard[s];
+  T.s0 = h.atom_sigma_r[s] / h.rcut_hard[s];
+  std::vector<double> xa, wa, xb, wb;
+  bool buffer = T.xs < 1.0;
+  int n1 = buffer ? h.n_quad_soft : h.n_quad_soft + h.n_quad_buffer;
+  int n2 = buffer ? h.n_quad_buffer : 0;
+  double x1 = buffer ? T.xs : 1.0;
+  gauss_legendre(n1, xa, wa);
+  T.xq.clear(); T.wq.clear();
+  for (int i = 0; i < n1; ++i) { T.xq.push_back(0.5 * (xa[i] + 1.0) * x1); T.wq.push_back(0.5 * wa[i] * x1); }
+  if (n2 > 0) {
+    gauss_legendre(n2, xb, wb);
+    for (int i = 0; i < n2; ++i) {
+      T.xq.push_back(T.xs + 0.5 * (xb[i] + 1.0) * (1.0 - T.xs));
+      T.wq.push_back(0.5 * wb[i] * (1.0 - T.xs));
+    }
+  }
+  T.nq = (int)T.xq.size();
+  const int nq = T.nq;
+  auto ip = [&](const double* f, const double* g) {
+    double s_ = 0.0;
+    for (int q = 0; q < nq; ++q) s_ += T.wq[q] * f[q] * g[q];
+    return s_;
+  };
+  // Orthonormal polynomial part: psi = t^3 p_k(t), p_k orthonormal for weight t^6.
+  int n_poly = (h.basis == 1) ? T.nb - 1 : T.nb;
+  std::vector<std::vector<double>> P;  // values t^3 p_k(t_q)
+  {
+    std::vector<double> t3(nq);
+    for (int q = 0; q < nq; ++q) { double t = 1.0 - T.xq[q]; t3[q] = t * t * t; }
+    std::vector<double> cur(nq), prev(nq, 0.0);
+    double nrm = std::sqrt(ip(t3.data(), t3.data()));
+    for (int q = 0; q < nq; ++q) cur[q] = t3[q] / nrm;
+    for (int k = 0; k < n_poly; ++k) {
+      P.push_back(cur);
+      if (k + 1 == n_poly) break;
+      std::vector<double> v(nq);
+      for (int q = 0; q < nq; ++q) v[q] = (1.0 - T.xq[q]) * cur[q];
+      for (int pass = 0; pass < 2; ++pass)
+        for (auto& pj : P) {
+          double cj = ip(v.data(), pj.data());
+          for (int q = 0; q < nq; ++q) v[q] -= cj * pj[q];
+        }
+      double nv = std::sqrt(ip(v.data(), v.data()));
+      for (int q = 0; q < nq; ++q) cur[q] = v[q] / nv;
+    }
+  }
+  std::vector<std::vector<double>> basis;
+  if (h.basis == 1) {
+    std::vector<double> g(nq);
+    for (int q = 0; q < nq; ++q) g[q] = std::exp(-T.xq[q] * T.xq[q] / (2.0 * T.s0 * T.s0));
+    double ng = std::sqrt(ip(g.data(), g.data()));
+    for (int q = 0; q < nq; ++q) g[q] /= ng;
+    basis.push_back(g);
+    for (int k = 0; k < n_poly; ++k) {
+      std::vector<double> v = P[k];
+      for (int pass = 0; pass < 2; ++pass)
+        for (auto& bj : basis) {
+          double cj = ip(v.data(), bj.data());
+          for (int q = 0; q < nq; ++q) v[q] -= cj * bj[q];
+        }
+      double nv = std::sqrt(ip(v.data(), v.data()));
+      for (int q = 0; q < nq; ++q) v[q] /= nv;
+      basis.push_back(v);
+    }
+  } else {
+    basis = P;
+  }
+  T.psi.assign((size_t)T.nb * nq, 0.0);
+  T.B.assign((size_t)T.nb * nq, 0.0);
+  T.cen.assign(T.nb, 0.0);
+  for (int n = 0; n < T.nb; ++n)
+    for (int q = 0; q < nq; ++q) {
+      double x = T.xq[q];
+      double F = (x <= T.xs) ? 1.0 : std::exp(-h.nf[s] * (x - T.xs) * (x - T.xs) / ((1.0 - T.xs) * (1.0 - T.xs)));
+      double en = 1.0;
+      for (int e = 0; e < h.radial_enhancement; ++e) en *= x;
+      T.psi[(size_t)n * nq + q] = basis[n][q];
+      T.B[(size_t)n * nq + q] = T.wq[q] * basis[n][q] * F * en;
+    }
+  for (int n = 0; n < T.nb; ++n) {
+    double s_ = 0.0;
+    for (int q = 0; q < nq; ++q) s_ += T.B[(size_t)n * nq + q] * std::exp(-T.xq[q] * T.xq[q] / (2.0 * T.s0 * T.s0));
+    T.cen[n] = s_ / T.s0;
+  }
+}
+
+// Real spherical harmonics and the gradient of the homogeneous solid harmonic
+// r^l Y_lm at a unit vector u. Index lm = l*l + l + m.
+struct dual4 { double v, x, y, z; };
+static inline dual4 dmul(const dual4& a, const dual4& b) {
+  return {a.v * b.v, a.x * b.v + a.v * b.x, a.y * b.v + a.v * b.y, a.z * b.v + a.v * b.z};
+}
+static inline dual4 dscale(const dual4& a, double s) { return {a.v * s, a.x * s, a.y * s, a.z * s}; }
+static inline dual4 dsub(const dual4& a, const dual4& b) { return {a.v - b.v, a.x - b.x, a.y - b.y, a.z - b.z}; }
+static inline dual4 dadd(const dual4& a, const dual4& b) { return {a.v + b.v, a.x + b.x, a.y + b.y, a.z + b.z}; }
+
+static void real_sph_harm(int lmax, const double* u, double* Y, double* dY /* 3 per lm or null */) {
+  const double pi = std::acos(-1.0);
+  dual4 X = {u[0], 1, 0, 0}, Yv = {u[1], 0, 1, 0}, Z = {u[2], 0, 0, 1};
+  dual4 R2 = {u[0] * u[0] + u[1] * u[1] + u[2] * u[2], 2 * u[0], 2 * u[1], 2 * u[2]};
+  std::vector<dual4> C(lmax + 1), S(lmax + 1);
+  C[0] = {1, 0, 0, 0}; S[0] = {0, 0, 0, 0};
+  for (int m = 1; m <= lmax; ++m) {
+    C[m] = dsub(dmul(X, C[m - 1]), dmul(Yv, S[m - 1]));
+    S[m] = dadd(dmul(X, S[m - 1]), dmul(Yv, C[m - 1]));
+  }
+  for (int m = 0; m <= lmax; ++m) {
+    // Q_l^m for l = m..lmax
+    std::vector<dual4> Q(lmax + 1);
+    double dfact = 1.0;
+    for (int k = 1; k <= 2 * m - 1; k += 2) dfact *= k;
+    Q[m] = {((m % 2) ? -1.0 : 1.0) * dfact, 0, 0, 0};
+    if (m + 1 <= lmax) Q[m + 1] = dscale(dmul(Z, Q[m]), 2.0 * m + 1.0);
+    for (int l = m + 2; l <= lmax; ++l)
+      Q[l] = dscale(dsub(dscale(dmul(Z, Q[l - 1]), 2.0 * l - 1.0), dscale(dmul(R2, Q[l - 2]), (double)(l + m - 1))),
+                    1.0 / (double)(l - m));
+    for (int l = m; l <= lmax; ++l) {
+      double ratio = 1.0;  // (l-m)!/(l+m)!
+      for (int k = l - m + 1; k <= l + m; ++k) ratio /= k;
+      double N = std::sqrt((2.0 * l + 1.0) / (4.0 * pi) * ratio);
+      if (m == 0) {
+        dual4 r = dscale(Q[l], N);
+        int idx = l * l + l;
+        Y[idx] = r.v;
+        if (dY) { dY[3 * idx] = r.x; dY[3 * idx + 1] = r.y; dY[3 * idx + 2] = r.z; }
+      } else {
+        dual4 rc = dscale(dmul(Q[l], C[m]), std::sqrt(2.0) * N);
+        dual4 rs = dscale(dmul(Q[l], S[m]), std::sqrt(2.0) * N);
+        int ip_ = l * l + l + m, im = l * l + l - m;
+        Y[ip_] = rc.v; Y[im] = rs.v;
+        if (dY) {
+          dY[3 * ip_] = rc.x; dY[3 * ip_ + 1] = rc.y; dY[3 * ip_ + 2] = rc.z;
+          dY[3 * im] = rs.x; dY[3 * im + 1] = rs.y; dY[3 * im + 2] = rs.z;
+        }
+      }
+    }
+  }
+}
+
+void orc_real_sph_harm(int lmax, const double* u, double* Y, double* dY) { real_sph_harm(lmax, u, Y, dY); }
+
+static int soap_dim(const orc_soap_hypers& h) {
+  int N = 0;
+  for (int s = 0; s < h.n_species; ++s) N += h.alpha_max[s];
+  return N * (N + 1) / 2 * (h.l_max + 1);
+}
+int orc_soap_dim(const orc_soap_hypers* h) { return soap_dim(*h); }
+
+// Radial table export for tests: nodes, weights, basis values and B.
+int orc_radial_table(const orc_soap_hypers* h, int s, double* xq, double* wq, double* psi, double* B, double* cen) {
+  radial_table T;
+  build_radial_table(*h, s, T);
+  if (xq) std::memcpy(xq, T.xq.data(), sizeof(double) * T.nq);
+  if (wq) std::memcpy(wq, T.wq.data(), sizeof(double) * T.nq);
+  if (psi) std::memcpy(psi, T.psi.data(), sizeof(double) * T.psi.size());
+  if (B) std::memcpy(B, T.B.data(), sizeof(double) * T.B.size());
+  if (cen) std::memcpy(cen, T.cen.data(), sizeof(double) * T.nb);
+  return T.nq;
+}
+
+// Per-pair radial values R_n (n over the species block) and dR_n/dr, in Å^-1.
+static void pair_radial(const orc_soap_hypers& h, int s, const radial_table& T, double r, double* R, double* dR) {
+  const double x = r / T.rc;
+  const double sig = h.atom_sigma_r[s] * (1.0 + h.atom_sigma_r_scaling[s] * r) / T.rc;
+  const double dsig = h.atom_sigma_r[s] * h.atom_sigma_r_scaling[s] / T.rc;  // d sigma / dr
+  const double a = h.amplitude_scaling[s];
+  const double P = 1.0 + 2.0 * x * x * x - 3.0 * x * x;
+  const double dP = (6.0 * x * x - 6.0 * x) / T.rc;  // dP/dr
+  double amp, damp;
+  if (a == 0.0) {
+    amp = 1.0 / sig;
+    damp = -dsig / (sig * sig);
+  } else if (P <= 1.0e-10) {
+    amp = 0.0; damp = 0.0;
+  } else {
+    double Pa = std::pow(P, a);
+    amp = Pa / sig;
+    damp = a * std::pow(P, a - 1.0) * dP / sig - Pa * dsig / (sig * sig);
+  }
+  amp *= h.global_scaling[s];
+  damp *= h.global_scaling[s];
+  for (int n = 0; n < T.nb; ++n) { R[n] = 0.0; dR[n] = 0.0; }
+  const double inv2s2 = 1.0 / (2.0 * sig * sig);
+  for (int q = 0; q < T.nq; ++q) {
+    double dx = T.xq[q] - x;
+    double G = std::exp(-dx * dx * inv2s2);
+    // dG/dr = G * [ dx/sig^2 * (1/rc) + dx^2/sig^3 * dsig ]
+    double dG = G * (dx / (sig * sig) / T.rc + dx * dx / (sig * sig * sig) * dsig);
+    for (int n = 0; n < T.nb; ++n) {
+      double b = T.B[(size_t)n * T.nq + q];
+      R[n] += b * G;
+      dR[n] += b * dG;
+    }
+  }
+  for (int n = 0; n < T.nb; ++n) {
+    double I = R[n], dI = dR[n];
+    R[n] = amp * I;
+    dR[n] = damp * I + amp * dI;
+  }
+}
+
+// get_soap for one descriptor block. Pair arrays are in site order with the
+// self pair first for every site. pair_species: 0-based index into the
+// descriptor's species list, or -1 for a neighbour the descriptor does not see.
+// xyz is (3, n_pairs) with xyz fastest. soap is (n_soap, n_sites) column-major
+// (component fastest); soap_cart_der is (3, n_soap, n_pairs), or null.
+int orc_get_soap(const orc_soap_hypers* hp, int n_sites, const int* n_neigh, const int* pair_species,
+                 const double* rjs, const double* xyz, double* soap, double* soap_cart_der) {
+  const orc_soap_hypers& h = *hp;
+  const int L = h.l_max, NLM = (L + 1) * (L + 1);
+  std::vector<int> off(h.n_species + 1, 0);
+  for (int s = 0; s < h.n_species; ++s) off[s + 1] = off[s] + h.alpha_max[s];
+  const int N = off[h.n_species];
+  const int D = N * (N + 1) / 2 * (L + 1);
+  std::vector<radial_table> T(h.n_species);
+  for (int s = 0; s < h.n_species; ++s) build_radial_table(h, s, T[s]);
+  std::vector<long> beg(n_sites + 1, 0);
+  for (int i = 0; i < n_sites; ++i) beg[i + 1] = beg[i] + n_neigh[i];
+  // power-spectrum index tables
+  std::vector<int> pn, pnp;
+  std::vector<double> pw;
+  for (int n = 0; n < N; ++n)
+    for (int np = n; np < N; ++np) { pn.push_back(n); pnp.push_back(np); pw.push_back(n == np ? 1.0 : std::sqrt(2.0)); }
+  const double Y00 = 0.5 / std::sqrt(std::acos(-1.0));
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int i = 0; i < n_sites; ++i) {
+    const long b0 = beg[i];
+    const int nn = n_neigh[i];
+    std::vector<double> c((size_t)N * NLM, 0.0);
+    // per-pair cached quantities
+    std::vector<double> R((size_t)nn * 16 * 4), dR, Y((size_t)nn * NLM), dY((size_t)nn * NLM * 3), A((size_t)nn * (L + 1)), dA((size_t)nn * (L + 1));
+    std::vector<int> blk(nn, -1);
+    int maxnb = 0;
+    for (int s = 0; s < h.n_species; ++s) maxnb = std::max(maxnb, h.alpha_max[s]);
+    R.assign((size_t)nn * maxnb, 0.0);
+    dR.assign((size_t)nn * maxnb, 0.0);
+    std::vector<double> uvec((size_t)nn * 3, 0.0), rr(nn, 0.0);
+    for (int k = 0; k < nn; ++k) {
+      const long p = b0 + k;
+      const int s = pair_species[p];
+      if (k == 0) {
+        // central atom: its own species block, l = 0 only
+        if (s >= 0 && h.central_weight[s] != 0.0) {
+          const double wc = h.central_weight[s] * h.global_scaling[s] * Y00;
+          for (int n = 0; n < T[s].nb; ++n) c[(size_t)(off[s] + n) * NLM + 0] += wc * T[s].cen[n];
+        }
+        continue;
+      }
+      if (s < 0) continue;
+      const double r = rjs[p];
+      if (!(r < h.rcut_hard[s])) continue;
+      blk[k] = s;
+      rr[k] = r;
+      double u[3] = {xyz[3 * p] / r, xyz[3 * p + 1] / r, xyz[3 * p + 2] / r};
+      for (int q = 0; q < 3; ++q) uvec[3 * k + q] = u[q];
+      pair_radial(h, s, T[s], r, &R[(size_t)k * maxnb], &dR[(size_t)k * maxnb]);
+      real_sph_harm(L, u, &Y[(size_t)k * NLM], &dY[(size_t)k * NLM * 3]);
+      const double st = h.atom_sigma_t[s] * (1.0 + h.atom_sigma_t_scaling[s] * r);
+      const double dst = h.atom_sigma_t[s] * h.atom_sigma_t_scaling[s];
+      for (int l = 0; l <= L; ++l) {
+        double ll = (double)(l * (l + 1));
+        double e = std::exp(-ll * st * st / (2.0 * r * r));
+        A[(size_t)k * (L + 1) + l] = e;
+        dA[(size_t)k * (L + 1) + l] = e * (ll * st * st / (r * r * r) - ll * st * dst / (r * r));
+      }
+      for (int n = 0; n < T[s].nb; ++n) {
+        const double Rn = R[(size_t)k * maxnb + n];
+        double* cn = &c[(size_t)(off[s] + n) * NLM];
+        for (int l = 0; l <= L; ++l) {
+          const double ra = Rn * A[(size_t)k * (L + 1) + l];
+          for (int m = -l; m <= l; ++m) cn[l * l + l + m] += ra * Y[(size_t)k * NLM + l * l + l + m];
+        }
+      }
+    }
+    // power spectrum
+    std::vector<double> p(D, 0.0);
+    int cnt = 0;
+    for (size_t pi = 0; pi < pn.size(); ++pi) {
+      const double* c1 = &c[(size_t)pn[pi] * NLM];
+      const double* c2 = &c[(size_t)pnp[pi] * NLM];
+      for (int l = 0; l <= L; ++l) {
+        double s_ = 0.0;
+        for (int m = -l; m <= l; ++m) s_ += c1[l * l + l + m] * c2[l * l + l + m];
+        p[cnt++] = pw[pi] * s_;
+      }
+    }
+    double nrm2 = 0.0;
+    for (int d = 0; d < D; ++d) nrm2 += p[d] * p[d];
+    const double nrm = std::sqrt(nrm2);
+    double* qi = soap + (size_t)i * D;
+    for (int d = 0; d < D; ++d) qi[d] = nrm > 0.0 ? p[d] / nrm : 0.0;
+    if (!soap_cart_der) continue;
+    // derivatives
+    double* self = soap_cart_der + (size_t)b0 * 3 * D;
+    for (int e = 0; e < 3 * D; ++e) self[e] = 0.0;
+    std::vector<double> dc((size_t)N * NLM * 3), dp((size_t)3 * D);
+    for (int k = 1; k < nn; ++k) {
+      double* out = soap_cart_der + (size_t)(b0 + k) * 3 * D;
+      const int s = blk[k];
+      if (s < 0 || nrm == 0.0) {
+        for (int e = 0; e < 3 * D; ++e) out[e] = 0.0;
+        continue;
+      }
+      std::fill(dc.begin(), dc.end(), 0.0);
+      const double r = rr[k];
+      const double* u = &uvec[3 * k];
+      for (int n = 0; n < T[s].nb; ++n) {
+        const double Rn = R[(size_t)k * maxnb + n], dRn = dR[(size_t)k * maxnb + n];
+        for (int l = 0; l <= L; ++l) {
+          const double Al = A[(size_t)k * (L + 1) + l], dAl = dA[(size_t)k * (L + 1) + l];
+          const double rad = dRn * Al + Rn * dAl;  // d(R A)/dr
+          const double ang = Rn * Al;
+          for (int m = -l; m <= l; ++m) {
+            const int lm = l * l + l + m;
+            const double Yv = Y[(size_t)k * NLM + lm];
+            const double* g = &dY[((size_t)k * NLM + lm) * 3];
+            for (int q = 0; q < 3; ++q) {
+              // grad_r Y = (grad R_lm(u) - l Y u) / r
+              double gY = (g[q] - l * Yv * u[q]) / r;
+              dc[((size_t)(off[s] + n) * NLM + lm) * 3 + q] = rad * Yv * u[q] + ang * gY;
+            }
+          }
+        }
+      }
+      cnt = 0;
+      for (size_t pi = 0; pi < pn.size(); ++pi) {
+        const int n1 = pn[pi], n2 = pnp[pi];
+        const double* c1 = &c[(size_t)n1 * NLM];
+        const double* c2 = &c[(size_t)n2 * NLM];
+        const double* d1 = &dc[(size_t)n1 * NLM * 3];
+        const double* d2 = &dc[(size_t)n2 * NLM * 3];
+        for (int l = 0; l <= L; ++l) {
+          double sx = 0, sy = 0, sz = 0;
+          for (int m = -l; m <= l; ++m) {
+            const int lm = l * l + l + m;
+            sx += d1[3 * lm] * c2[lm] + c1[lm] * d2[3 * lm];
+            sy += d1[3 * lm + 1] * c2[lm] + c1[lm] * d2[3 * lm + 1];
+            sz += d1[3 * lm + 2] * c2[lm] + c1[lm] * d2[3 * lm + 2];
+          }
+          dp[3 * cnt] = pw[pi] * sx; dp[3 * cnt + 1] = pw[pi] * sy; dp[3 * cnt + 2] = pw[pi] * sz;
+          ++cnt;
+        }
+      }
+      double qd[3] = {0, 0, 0};
+      for (int d = 0; d < D; ++d)
+        for (int q = 0; q < 3; ++q) qd[q] += qi[d] * dp[3 * d + q];
+      for (int d = 0; d < D; ++d)
+        for (int q = 0; q < 3; ++q) {
+          double v = (dp[3 * d + q] - qi[d] * qd[q]) / nrm;
+          out[3 * d + q] = v;
+          self[3 * d + q] -= v;
+        }
+    }
+  }
+  return D;
+}
+
+// src/gap.f90:66-259. soap (n_soap, n_sites) and soap_der (3, n_soap, n_pairs)
+// column-major; Qs (n_soap, n_sparse) column-major; neighbors_list 1-based
+// local indices; xyz (3, n_pairs); forces (3, n_sites0).
+void orc_get_soap_energy_and_forces(int n_sites, int n_soap, int n_sparse, const double* soap, const double* soap_der,
+                                    const double* Qs, const double* alphas, double delta, double zeta0, double e0,
+                                    const int* n_neigh, const int* neighbors_list, const double* xyz, int n_sites0,
+                                    int do_forces, double* energies, double* forces, double* virial) {
+  bool is_int = std::fabs(zeta0 - (double)(int)zeta0) < 1.0e-5;
+  int zi = (int)zeta0;
+  double zeta = is_int ? (double)zi : zeta0;
+  std::vector<double> K((size_t)n_sites * n_sparse);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n_sites; ++i)
+    for (int s = 0; s < n_sparse; ++s) {
+      const double* qi = soap + (size_t)i * n_soap;
+      const double* qs = Qs + (size_t)s * n_soap;
+      double acc = 0.0;
+      for (int d = 0; d < n_soap; ++d) acc += qi[d] * qs[d];
+      K[(size_t)i * n_sparse + s] = acc;
+    }
+  auto ipow = [](double x, int n) { double r = 1.0; for (int k = 0; k < n; ++k) r *= x; return r; };
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n_sites; ++i) {
+    double e = 0.0;
+    for (int s = 0; s < n_sparse; ++s) {
+      double k = K[(size_t)i * n_sparse + s];
+      double kz = is_int ? ipow(k, zi) : std::pow(k, zeta);
+      e += kz * alphas[s];
+    }
+    energies[i] = delta * delta * e + e0;
+  }
+  if (!do_forces) return;
+  // Qss(i,:) = -zeta delta^2 sum_s K^(zeta-1) alpha_s Qs(:,s)
+  std::vector<double> Qss((size_t)n_sites * n_soap, 0.0);
+  const double pref = -zeta * delta * delta;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n_sites; ++i) {
+    double* out = &Qss[(size_t)i * n_soap];
+    for (int s = 0; s < n_sparse; ++s) {
+      double k = K[(size_t)i * n_sparse + s];
+      double kd = (is_int ? ipow(k, zi - 1) : std::pow(k, zeta - 1.0)) * alphas[s] * pref;
+      const double* qs = Qs + (size_t)s * n_soap;
+      for (int d = 0; d < n_soap; ++d) out[d] += kd * qs[d];
+    }
+  }
+  for (int k = 0; k < 9; ++k) virial[k] = 0.0;
+  for (long k = 0; k < (long)3 * n_sites0; ++k) forces[k] = 0.0;
+  std::vector<long> beg(n_sites + 1, 0);
+  for (int i = 0; i < n_sites; ++i) beg[i + 1] = beg[i] + n_neigh[i];
+  const long n_pairs = beg[n_sites];
+  std::vector<double> fpair((size_t)n_pairs * 3);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < n_sites; ++i) {
+    const double* tq = &Qss[(size_t)i * n_soap];
+    for (long l = beg[i]; l < beg[i + 1]; ++l) {
+      const double* sd = soap_der + (size_t)l * 3 * n_soap;
+      double f0 = 0, f1 = 0, f2 = 0;
+      for (int d = 0; d < n_soap; ++d) { f0 += tq[d] * sd[3 * d]; f1 += tq[d] * sd[3 * d + 1]; f2 += tq[d] * sd[3 * d + 2]; }
+      fpair[3 * l] = f0; fpair[3 * l + 1] = f1; fpair[3 * l + 2] = f2;
+    }
+  }
+  // Serial accumulation in the reference's pair order.
+  for (long l = 0; l < n_pairs; ++l) {
+    int j2 = (neighbors_list[l] - 1) % n_sites0;
+    const double* f = &fpair[3 * l];
+    for (int k = 0; k < 3; ++k) forces[3 * j2 + k] += f[k];
+    for (int k1 = 0; k1 < 3; ++k1)
+      for (int k2 = 0; k2 < 3; ++k2)
+        virial[3 * k2 + k1] += 0.5 * (f[k1] * xyz[3 * l + k2] + f[k2] * xyz[3 * l + k1]);
+  }
+}
+
+// src/gap_interface.f90:180-431 for one soap_turbo block: select central sites
+// and pairs, call get_soap and get_soap_energy_and_forces on the local arrays,
+// and fold the results back to the n_total_sites atoms.
+//   species_index_sc (n_sites_sc): 0-based descriptor species of every supercell
+//     atom, -1 if the descriptor does not see that species (the role of
+//     assign_species_multiplicity with multiplicity 1).
+//   central_species: 1-based entry of the descriptor's species list, 0 = all.
+// energies0, forces0 (3, n_total_sites): assigned (not accumulated) for the
+// atoms this block touches, like the reference.
+// Returns the number of kept pairs.
+long orc_gap_soap(const orc_soap_hypers* h, int n_total_sites, int n_sites0, const int* n_neigh0,
+                  const int* neighbors_list0, const double* rjs0, const double* xyz0, const int* species_index_sc,
+                  int central_species, const double* Qs, const double* alphas, int n_sparse, double delta, double zeta,
+                  int do_forces, double* energies0, double* forces0, double* virial, double* soap_out,
+                  int* n_sites_out) {
+  double rcut_max = 0.0;
+  for (int s = 0; s < h->n_species; ++s) rcut_max = std::max(rcut_max, h->rcut_hard[s]);
+  const int D = soap_dim(*h);
+  // species0(1,i) of the atom at the head of every site list
+  std::vector<long> beg(n_sites0 + 1, 0);
+  for (int i = 0; i < n_sites0; ++i) beg[i + 1] = beg[i] + n_neigh0[i];
+  auto is_central = [&](int i) {
+    if (central_species == 0) return true;
+    if (n_neigh0[i] == 0) return false;
+    int j = (neighbors_list0[beg[i]] - 1) % n_total_sites;
+    return species_index_sc[j] + 1 == central_species;
+  };
+  int n_sites = 0;
+  long n_pairs = 0;
+  std::vector<char> seen(n_total_sites, 0);
+  for (int i = 0; i < n_sites0; ++i) {
+    if (!is_central(i)) continue;
+    ++n_sites;
+    for (long k = beg[i]; k < beg[i + 1]; ++k) {
+      int j2 = (neighbors_list0[k] - 1) % n_total_sites;
+      if (rjs0[k] < rcut_max && species_index_sc[j2] >= 0) { ++n_pairs; seen[j2] = 1; }
+    }
+  }
+  int n_all = 0;
+  for (int i = 0; i < n_total_sites; ++i) n_all += seen[i];
+  std::vector<int> n_neigh(n_sites), nlist(n_pairs), spec(n_pairs), in_to_out(n_all, 0), out_to_in(n_total_sites, 0);
+  std::vector<double> rjs(n_pairs), xyz((size_t)3 * n_pairs);
+  int i2 = 0;
+  for (int i = 0; i < n_sites0; ++i) {
+    if (!is_central(i)) continue;
+    int j = (neighbors_list0[beg[i]] - 1) % n_total_sites;
+    ++i2;
+    in_to_out[i2 - 1] = j + 1;
+    out_to_in[j] = i2;
+  }
+  long k2 = 0;
+  int i4 = 0;
+  i2 = 0;
+  for (int i = 0; i < n_sites0; ++i) {
+    if (!is_central(i)) continue;
+    ++i2;
+    int jj = 0;
+    for (long k = beg[i]; k < beg[i + 1]; ++k) {
+      int j3 = (neighbors_list0[k] - 1) % n_total_sites;
+      if (!(rjs0[k] < rcut_max && species_index_sc[j3] >= 0)) continue;
+      ++jj;
+      rjs[k2] = rjs0[k];
+      for (int q = 0; q < 3; ++q) xyz[3 * k2 + q] = xyz0[3 * k + q];
+      spec[k2] = species_index_sc[j3];
+      bool first = (k == beg[i]);
+      if (!first && out_to_in[j3] == 0) {
+        ++i4;
+        in_to_out[n_sites + i4 - 1] = j3 + 1;
+        out_to_in[j3] = n_sites + i4;
+        nlist[k2] = out_to_in[j3];
+      } else if (!first) {
+        nlist[k2] = out_to_in[j3];
+      } else {
+        nlist[k2] = i2;
+      }
+      ++k2;
+    }
+    n_neigh[i2 - 1] = jj;
+  }
+  std::vector<double> soap((size_t)n_sites * D), soap_der;
+  if (do_forces) soap_der.assign((size_t)n_pairs * 3 * D, 0.0);
+  if (n_sites > 0)
+    orc_get_soap(h, n_sites, n_neigh.data(), spec.data(), rjs.data(), xyz.data(), soap.data(),
+                 do_forces ? soap_der.data() : nullptr);
+  std::vector<double> energies(n_sites, 0.0), forces((size_t)3 * n_all, 0.0);
+  double vir[9] = {0};
+  if (n_sites > 0)
+    orc_get_soap_energy_and_forces(n_sites, D, n_sparse, soap.data(), do_forces ? soap_der.data() : nullptr, Qs, alphas,
+                                   delta, zeta, 0.0, n_neigh.data(), nlist.data(), xyz.data(), n_all, do_forces,
+                                   energies.data(), forces.data(), vir);
+  for (int i = 0; i < n_sites; ++i) energies0[in_to_out[i] - 1] = energies[i];
+  if (do_forces)
+    for (int i = 0; i < n_all; ++i)
+      for (int q = 0; q < 3; ++q) forces0[3 * (in_to_out[i] - 1) + q] = forces[3 * i + q];
+  for (int k = 0; k < 9; ++k) virial[k] = vir[k];
+  if (soap_out) std::memcpy(soap_out, soap.data(), sizeof(double) * soap.size());
+  if (n_sites_out) *n_sites_out = n_sites;
+  return n_pairs;
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+}  // extern "C"

@@ -1,0 +1,154 @@
+"""GPU parity against the FP64 oracle, through the C ABI.
+
+Bar: neighbour lists bit-exact (indices, species, rjs, xyz; thetas/phis within
+a few ulp: acos/atan2 come from different math libraries); descriptors to
+1e-10; energies to 1e-8 relative, forces to 1e-6 eV/Å and virials to 1e-6 eV
+on the FP64 paths.
+"""
+import numpy as np
+import pytest
+
+from tests import helpers as H
+from turbogap_amd import structures as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare_lists(g, o):
+    assert g["n_atom_pairs"] == o["n_atom_pairs"]
+    np.testing.assert_array_equal(g["n_neigh"], o["n_neigh"])
+    np.testing.assert_array_equal(g["neighbors_list"], o["neighbors_list"])
+    np.testing.assert_array_equal(g["neighbor_species"], o["neighbor_species"])
+    assert np.array_equal(g["rjs"], o["rjs"]), "rjs not bit-exact"
+    assert np.array_equal(g["xyz"], o["xyz"]), "xyz not bit-exact"
+    np.testing.assert_allclose(g["thetas"], o["thetas"], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(g["phis"], o["phis"], rtol=0, atol=1e-14)
+
+
+def test_neighbors_config0_bitexact(ctx, orc):
+    sysm, h, _, _ = H.config0(200)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    g = ctx.build_neighbors_list(h.rcut_max)
+    o = orc.build_neighbors_list(sysm.positions, sysm.species, sysm.box, h.rcut_max)
+    _compare_lists(g, o)
+
+
+def test_neighbors_subset_bitexact(ctx, orc):
+    sysm = S.config1_system(4096, seed=2002)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    g = ctx.build_neighbors_list(3.7)
+    o = orc.build_neighbors_list(sysm.positions, sysm.species, sysm.box, 3.7)
+    _compare_lists(g, o)
+
+
+def test_neighbors_small_box_supercell(ctx, orc):
+    sysm = H.small_box()
+    ind = orc.number_of_unit_cells(sysm.box, 3.7)
+    assert ind.max() > 1
+    psc, ssc, bsc = orc.build_supercell(sysm.positions, sysm.species, sysm.box, ind)
+    ctx.set_system(psc, ssc, bsc, n_sites=sysm.n)
+    g = ctx.build_neighbors_list(3.7, indices=ind)
+    o = orc.build_neighbors_list(psc, ssc, bsc, 3.7, n_sites=sysm.n, indices=ind)
+    _compare_lists(g, o)
+
+
+def test_neighbors_do_list(ctx, orc):
+    sysm, h, _, _ = H.config0(200)
+    do = (np.arange(sysm.n) % 3 != 0).astype(np.int32)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    g = ctx.build_neighbors_list(h.rcut_max, do_list=do)
+    o = orc.build_neighbors_list(sysm.positions, sysm.species, sysm.box, h.rcut_max, do_list=do)
+    _compare_lists(g, o)
+
+
+def test_soap_config0(ctx, orc):
+    sysm, h, Qs, al = H.config0(200)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    nl = ctx.build_neighbors_list(h.rcut_max)
+    mid = ctx.soap_model(h, Qs, al)
+    ctx.soap_backend_begin(mid)
+    q, der = ctx.get_soap(derivatives=True)
+    ctx.soap_backend_end()
+    spi = orc.species_index(nl["neighbor_species"], h.species_types)
+    qo, dero = orc.get_soap(h, nl["n_neigh"], spi, nl["rjs"], nl["xyz"], derivatives=True)
+    np.testing.assert_allclose(q, qo, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(der, dero, rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("path", ["fp64_plain", "fp64_adjoint"])
+def test_gap_config0(ctx, orc, path):
+    sysm, h, Qs, al = H.config0(2000)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    ctx.build_neighbors_list(h.rcut_max, fetch=False)
+    mid = ctx.soap_model(h, Qs, al)
+    E, W = ctx.compute(path)
+    en, F = ctx.results()
+    Eo, eno, Fo, Wo, _ = orc.compute(sysm, [(h, Qs, al)])
+    assert abs(E - Eo) <= 1e-8 * abs(Eo)
+    np.testing.assert_allclose(en, eno, rtol=1e-8, atol=1e-12)
+    assert np.abs(F - Fo).max() <= 1e-6
+    assert np.abs(W - Wo).max() <= 1e-6
+    # the FP64 paths agree far below the bar
+    assert np.abs(F - Fo).max() <= 1e-10
+
+
+def test_get_gap_soap_dropin(ctx, orc):
+    """get_gap_soap assigns energies/forces of the atoms a block touches."""
+    sysm, models = H.subset(300, 4096)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    nl = ctx.build_neighbors_list(3.7)
+    ids = H.load_models(ctx, models)
+    for mid, (h, Qs, al) in zip(ids, models):
+        ctx.soap_backend_begin(mid)
+        en, F, W, ns, npb = ctx.get_gap_soap("fp64_plain")
+        ctx.soap_backend_end()
+        eo, fo, wo, soap, P = orc.gap_soap(h, nl, sysm.species, Qs, al, sysm.n)
+        assert ns == soap.shape[0] and npb == P
+        np.testing.assert_allclose(en, eo, rtol=1e-8, atol=1e-12)
+        assert np.abs(F - fo).max() <= 1e-6
+        assert np.abs(W - wo).max() <= 1e-6
+
+
+@pytest.mark.parametrize("path", ["fp64_plain", "fp64_adjoint"])
+def test_gap_subset_two_species(ctx, orc, path):
+    sysm, models = H.subset(500, 4096)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    ctx.build_neighbors_list(3.7, fetch=False)
+    H.load_models(ctx, models)
+    E, W = ctx.compute(path)
+    en, F = ctx.results()
+    Eo, eno, Fo, Wo, _ = orc.compute(sysm, models)
+    assert abs(E - Eo) <= 1e-8 * abs(Eo)
+    np.testing.assert_allclose(en, eno, rtol=1e-8, atol=1e-12)
+    assert np.abs(F - Fo).max() <= 1e-6
+    assert np.abs(W - Wo).max() <= 1e-6
+
+
+def test_soap_energy_and_forces_host_inputs(ctx, orc):
+    """get_soap_energy_and_forces with the oracle's descriptors as host input."""
+    sysm, h, Qs, al = H.config0(2000)
+    nl = orc.build_neighbors_list(sysm.positions, sysm.species, sysm.box, h.rcut_max)
+    spi = orc.species_index(nl["neighbor_species"], h.species_types)
+    q, der = orc.get_soap(h, nl["n_neigh"], spi, nl["rjs"], nl["xyz"])
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    mid = ctx.soap_model(h, Qs, al)
+    ctx.soap_backend_begin(mid)
+    en, F, W = ctx.get_soap_energy_and_forces(q, der, nl["n_neigh"], nl["neighbors_list"], nl["xyz"], sysm.n)
+    ctx.soap_backend_end()
+    eo, fo, wo = orc.get_soap_energy_and_forces(q, der, Qs, al, h.delta, h.zeta, 0.0, nl["n_neigh"],
+                                                nl["neighbors_list"], nl["xyz"], sysm.n)
+    np.testing.assert_allclose(en, eo, rtol=1e-10, atol=1e-13)
+    assert np.abs(F - fo).max() <= 1e-10
+    assert np.abs(W - wo).max() <= 1e-10
+
+
+def test_compute_deterministic(ctx):
+    sysm, models = H.subset(300, 4096)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    ctx.build_neighbors_list(3.7, fetch=False)
+    H.load_models(ctx, models)
+    ctx.compute("fp64_adjoint")
+    e1, f1 = ctx.results()
+    ctx.compute("fp64_adjoint")
+    e2, f2 = ctx.results()
+    assert np.array_equal(e1, e2) and np.array_equal(f1, f2)

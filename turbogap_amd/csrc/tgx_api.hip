@@ -1,0 +1,602 @@
+// C ABI of libtgx (declared in include/tgx.h): context, system, neighbour list,
+// models, and the SOAP/GAP evaluation entry points.
+#include <cmath>
+#include <algorithm>
+#include <cstdio>
+
+#include "tgx_internal.h"
+#include "tgx_util.h"
+
+int tgx_energy_scatter(tgx_ctx* ctx, tgx_model_t* m);
+int tgx_block_seen(tgx_ctx* ctx, tgx_model_t* m, int* d_seen);
+
+#define TGX_CTX_CHECK(ctx)                                      \
+  do {                                                          \
+    if (!(ctx)) {                                               \
+      tgx_set_error("%s: null context", __func__);              \
+      return TGX_ERR_ARG;                                       \
+    }                                                           \
+    TGX_CHECK_HIP(hipSetDevice((ctx)->device));                 \
+  } while (0)
+
+// ---- profiling ----------------------------------------------------------------
+int tgx_prof_begin(tgx_ctx* ctx, const char* name, int* slot) {
+  prof_rec r;
+  r.name = name;
+  for (hipEvent_t* e : {&r.a, &r.b}) {
+    if (!ctx->ev_pool.empty()) {
+      *e = ctx->ev_pool.back();
+      ctx->ev_pool.pop_back();
+    } else {
+      TGX_CHECK_HIP(hipEventCreate(e));
+    }
+  }
+  TGX_CHECK_HIP(hipEventRecord(r.a, ctx->stream));
+  ctx->prof.push_back(r);
+  *slot = (int)ctx->prof.size() - 1;
+  return TGX_OK;
+}
+
+int tgx_prof_end(tgx_ctx* ctx, int slot) {
+  TGX_CHECK_HIP(hipEventRecord(ctx->prof[slot].b, ctx->stream));
+  return TGX_OK;
+}
+
+extern "C" int tgx_set_profiling(tgx_ctx* ctx, int on) {
+  TGX_CTX_CHECK(ctx);
+  ctx->profiling = on != 0;
+  return TGX_OK;
+}
+
+extern "C" int tgx_reset_profile(tgx_ctx* ctx) {
+  TGX_CTX_CHECK(ctx);
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  for (auto& r : ctx->prof) { ctx->ev_pool.push_back(r.a); ctx->ev_pool.push_back(r.b); }
+  ctx->prof.clear();
+  return TGX_OK;
+}
+
+extern "C" int tgx_kernel_time(tgx_ctx* ctx, const char* name, int* count, double* total_ms) {
+  TGX_CTX_CHECK(ctx);
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  int c = 0;
+  double t = 0.0;
+  for (auto& r : ctx->prof) {
+    if (name && name[0] && r.name.find(name) == std::string::npos) continue;
+    float ms = 0.f;
+    TGX_CHECK_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    ++c;
+    t += ms;
+  }
+  if (count) *count = c;
+  if (total_ms) *total_ms = t;
+  return TGX_OK;
+}
+
+extern "C" int tgx_synchronize(tgx_ctx* ctx) {
+  TGX_CTX_CHECK(ctx);
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return TGX_OK;
+}
+
+// ---- context ------------------------------------------------------------------
+extern "C" int tgx_init(int device, tgx_ctx** out) {
+  TGX_REQUIRE(out, "tgx_init: null output");
+  int n = 0;
+  TGX_CHECK_HIP(hipGetDeviceCount(&n));
+  TGX_REQUIRE(device >= 0 && device < n, "tgx_init: device %d not present (%d visible)", device, n);
+  TGX_CHECK_HIP(hipSetDevice(device));
+  tgx_ctx* ctx = new tgx_ctx();
+  ctx->device = device;
+  hipError_t e = hipStreamCreate(&ctx->stream);
+  if (e != hipSuccess) {
+    delete ctx;
+    tgx_set_error("hipStreamCreate failed: %s", hipGetErrorString(e));
+    return TGX_ERR_HIP;
+  }
+  *out = ctx;
+  return TGX_OK;
+}
+
+static void free_model(tgx_model_t* m) {
+  dbuf* bufs[] = {&m->B, &m->xq, &m->cen, &m->dtab, &m->Qs, &m->QsT, &m->alpha, &m->Qs_bf, &m->QsT_bf, &m->alpha_f,
+                  &m->site_atom, &m->site_beg, &m->pr_gidx, &m->pr_nbr, &m->pr_spec, &m->pr_r, &m->pr_xyz, &m->q64,
+                  &m->q32, &m->qbf, &m->pnorm, &m->e_site, &m->qss64, &m->qss32, &m->kd64, &m->kdbf, &m->epart,
+                  &m->soap_der, &m->fpair};
+  for (dbuf* b : bufs) b->release();
+  for (auto& mb : m->mlp) {
+    mb.W1.release(); mb.W2.release(); mb.w3.release(); mb.b1.release(); mb.b2.release(); mb.b3.release();
+  }
+  delete m;
+}
+
+extern "C" int tgx_finalize(tgx_ctx* ctx) {
+  if (!ctx) return TGX_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto* m : ctx->models) free_model(m);
+  dbuf* bufs[] = {&ctx->pos, &ctx->species_sc, &ctx->n_neigh, &ctx->nl_off, &ctx->nlist, &ctx->rjs, &ctx->xyz,
+                  &ctx->thetas, &ctx->phis, &ctx->nspec, &ctx->rev, &ctx->cell_of, &ctx->cell_cnt, &ctx->cell_start,
+                  &ctx->cell_atoms, &ctx->scan_tmp, &ctx->do_list, &ctx->energies, &ctx->forces, &ctx->fglob,
+                  &ctx->virial_part, &ctx->scratch, &ctx->red, &ctx->vel, &ctx->mass, &ctx->forces_prev,
+                  &ctx->pos_prev, &ctx->e_members, &ctx->gate_mask};
+  for (dbuf* b : bufs) b->release();
+  for (auto& r : ctx->prof) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+  for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return TGX_OK;
+}
+
+extern "C" int tgx_memory_budget(tgx_ctx* ctx, double* free_gib, double* total_gib) {
+  TGX_CTX_CHECK(ctx);
+  size_t f = 0, t = 0;
+  TGX_CHECK_HIP(hipMemGetInfo(&f, &t));
+  if (free_gib) *free_gib = (double)f / (1024.0 * 1024.0 * 1024.0);
+  if (total_gib) *total_gib = (double)t / (1024.0 * 1024.0 * 1024.0);
+  return TGX_OK;
+}
+
+extern "C" int tgx_device_name(tgx_ctx* ctx, char* buf, int buflen) {
+  TGX_CTX_CHECK(ctx);
+  hipDeviceProp_t p;
+  TGX_CHECK_HIP(hipGetDeviceProperties(&p, ctx->device));
+  snprintf(buf, (size_t)buflen, "%s (%s)", p.name, p.gcnArchName);
+  return TGX_OK;
+}
+
+// ---- system / neighbours ----------------------------------------------------------
+extern "C" int tgx_set_system(tgx_ctx* ctx, int n_sites, int n_sites_sc, const double* positions,
+                              const int* species_sc, const double* box) {
+  TGX_CTX_CHECK(ctx);
+  TGX_REQUIRE(n_sites > 0 && n_sites_sc >= n_sites && positions && species_sc && box,
+              "tgx_set_system: bad arguments (n_sites=%d, n_sites_sc=%d)", n_sites, n_sites_sc);
+  ctx->n_sites = n_sites;
+  ctx->n_sites_sc = n_sites_sc;
+  for (int k = 0; k < 9; ++k) ctx->box[k] = box[k];
+  ctx->species_sc_h.assign(species_sc, species_sc + n_sites_sc);
+  TGX_TRY(ctx->pos.ensure(sizeof(double) * 3 * (size_t)n_sites_sc));
+  TGX_TRY(ctx->species_sc.ensure(sizeof(int) * (size_t)n_sites_sc));
+  TGX_CHECK_HIP(hipMemcpyAsync(ctx->pos.p, positions, sizeof(double) * 3 * (size_t)n_sites_sc, hipMemcpyHostToDevice,
+                               ctx->stream));
+  TGX_CHECK_HIP(hipMemcpyAsync(ctx->species_sc.p, species_sc, sizeof(int) * (size_t)n_sites_sc, hipMemcpyHostToDevice,
+                               ctx->stream));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->have_list = false;
+  return TGX_OK;
+}
+
+extern "C" int tgx_set_positions(tgx_ctx* ctx, const double* positions) {
+  TGX_CTX_CHECK(ctx);
+  TGX_REQUIRE(ctx->n_sites_sc > 0 && positions, "tgx_set_positions: no system or null positions");
+  TGX_CHECK_HIP(hipMemcpyAsync(ctx->pos.p, positions, sizeof(double) * 3 * (size_t)ctx->n_sites_sc,
+                               hipMemcpyHostToDevice, ctx->stream));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return TGX_OK;
+}
+
+extern "C" int tgx_set_positions_device(tgx_ctx* ctx, const double* d_positions) {
+  TGX_CTX_CHECK(ctx);
+  TGX_REQUIRE(ctx->n_sites_sc > 0 && d_positions, "tgx_set_positions_device: no system or null positions");
+  TGX_CHECK_HIP(hipMemcpyAsync(ctx->pos.p, d_positions, sizeof(double) * 3 * (size_t)ctx->n_sites_sc,
+                               hipMemcpyDeviceToDevice, ctx->stream));
+  return TGX_OK;
+}
+
+extern "C" int tgx_build_neighbors_list(tgx_ctx* ctx, double rcut_max, const int* indices, const int* do_list,
+                                        int64_t* n_atom_pairs) {
+  TGX_CTX_CHECK(ctx);
+  TGX_TRY(tgx_neighbors_run(ctx, rcut_max, indices, do_list));
+  if (n_atom_pairs) *n_atom_pairs = ctx->n_pairs;
+  return TGX_OK;
+}
+
+extern "C" int tgx_get_neighbors_list(tgx_ctx* ctx, int* n_neigh, int* neighbors_list, double* rjs, double* thetas,
+                                      double* phis, double* xyz, int* neighbor_species) {
+  TGX_CTX_CHECK(ctx);
+  TGX_REQUIRE(ctx->have_list, "tgx_get_neighbors_list: no list built");
+  const size_t P = (size_t)ctx->n_pairs;
+  auto cp = [&](void* dst, const dbuf& src, size_t bytes) -> int {
+    if (dst && bytes) TGX_CHECK_HIP(hipMemcpyAsync(dst, src.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    return TGX_OK;
+  };
+  TGX_TRY(cp(n_neigh, ctx->n_neigh, sizeof(int) * (size_t)ctx->n_sites));
+  TGX_TRY(cp(neighbors_list, ctx->nlist, sizeof(int) * P));
+  TGX_TRY(cp(rjs, ctx->rjs, sizeof(double) * P));
+  TGX_TRY(cp(thetas, ctx->thetas, sizeof(double) * P));
+  TGX_TRY(cp(phis, ctx->phis, sizeof(double) * P));
+  TGX_TRY(cp(xyz, ctx->xyz, sizeof(double) * 3 * P));
+  TGX_TRY(cp(neighbor_species, ctx->nspec, sizeof(int) * P));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return TGX_OK;
+}
+
+// ---- models -----------------------------------------------------------------------
+static int copy_h(const double* src, int n, std::vector<double>& dst, const char* what) {
+  TGX_REQUIRE(src, "tgx_soap_model_create: %s is NULL", what);
+  dst.assign(src, src + n);
+  return TGX_OK;
+}
+
+extern "C" int tgx_soap_model_create(tgx_ctx* ctx, const tgx_soap_hypers* h, const int* species_types,
+                                     int central_species, double delta, double zeta, double e0, int n_sparse,
+                                     int dim, const double* Qs, const double* alphas, int* model_id) {
+  TGX_CTX_CHECK(ctx);
+  TGX_REQUIRE(h && species_types && model_id, "tgx_soap_model_create: null argument");
+  TGX_REQUIRE(h->n_species >= 1 && h->n_species <= TGX_MAX_SPECIES, "n_species=%d outside 1..%d", h->n_species,
+              TGX_MAX_SPECIES);
+  TGX_REQUIRE(h->l_max >= 0 && h->l_max <= TGX_MAX_L, "l_max=%d outside 0..%d", h->l_max, TGX_MAX_L);
+  TGX_REQUIRE(h->basis == 0 || h->basis == 1, "basis must be 0 (poly3) or 1 (poly3gauss)");
+  TGX_REQUIRE(h->radial_enhancement >= 0 && h->radial_enhancement <= 2, "radial_enhancement must be 0, 1 or 2");
+  TGX_REQUIRE(h->n_quad_soft >= 2 && h->n_quad_buffer >= 0 && h->n_quad_soft + h->n_quad_buffer <= TGX_MAX_Q,
+              "quadrature nodes %d + %d outside 2..%d", h->n_quad_soft, h->n_quad_buffer, TGX_MAX_Q);
+  TGX_REQUIRE(central_species >= 0 && central_species <= h->n_species, "central_species=%d outside 0..%d",
+              central_species, h->n_species);
+  for (int s = 0; s < h->n_species; ++s)
+    for (int t = s + 1; t < h->n_species; ++t)
+      TGX_REQUIRE(species_types[s] != species_types[t],
+                  "species %d listed twice: species multiplicity above 1 is not supported", species_types[s]);
+  tgx_model_t* m = new tgx_model_t();
+  auto fail = [&](int code) { free_model(m); return code; };
+  m->n_species = h->n_species;
+  m->l_max = h->l_max;
+  m->basis = h->basis;
+  m->radial_enhancement = h->radial_enhancement;
+  m->nq_soft = h->n_quad_soft;
+  m->nq_buffer = h->n_quad_buffer;
+  const int S = h->n_species;
+  m->alpha_max.assign(h->alpha_max, h->alpha_max + S);
+  m->species_types.assign(species_types, species_types + S);
+  int st;
+  if ((st = copy_h(h->rcut_hard, S, m->rcut_hard, "rcut_hard")) ||
+      (st = copy_h(h->rcut_soft, S, m->rcut_soft, "rcut_soft")) || (st = copy_h(h->nf, S, m->nf, "nf")) ||
+      (st = copy_h(h->global_scaling, S, m->global_scaling, "global_scaling")) ||
+      (st = copy_h(h->atom_sigma_r, S, m->sig_r, "atom_sigma_r")) ||
+      (st = copy_h(h->atom_sigma_r_scaling, S, m->sig_r_sc, "atom_sigma_r_scaling")) ||
+      (st = copy_h(h->atom_sigma_t, S, m->sig_t, "atom_sigma_t")) ||
+      (st = copy_h(h->atom_sigma_t_scaling, S, m->sig_t_sc, "atom_sigma_t_scaling")) ||
+      (st = copy_h(h->amplitude_scaling, S, m->amp_a, "amplitude_scaling")) ||
+      (st = copy_h(h->central_weight, S, m->cw, "central_weight")))
+    return fail(st);
+  int N = 0;
+  for (int s = 0; s < S; ++s) {
+    if (m->alpha_max[s] < 1 || m->alpha_max[s] > TGX_MAX_NB || (m->basis == 1 && m->alpha_max[s] < 2)) {
+      tgx_set_error("alpha_max[%d]=%d outside %d..%d", s, m->alpha_max[s], m->basis == 1 ? 2 : 1, TGX_MAX_NB);
+      return fail(TGX_ERR_ARG);
+    }
+    if (!(m->rcut_hard[s] > 0.0) || !(m->rcut_soft[s] > 0.0) || m->rcut_soft[s] > m->rcut_hard[s] ||
+        !(m->sig_r[s] > 0.0) || !(m->sig_t[s] >= 0.0)) {
+      tgx_set_error("species %d: need 0 < rcut_soft <= rcut_hard and atom_sigma_r > 0", s);
+      return fail(TGX_ERR_ARG);
+    }
+    N += m->alpha_max[s];
+  }
+  if (N > TGX_MAX_NRAD) {
+    tgx_set_error("sum of alpha_max = %d exceeds %d", N, TGX_MAX_NRAD);
+    return fail(TGX_ERR_ARG);
+  }
+  m->N = N;
+  m->D = N * (N + 1) / 2 * (m->l_max + 1);
+  if (dim != m->D) {
+    tgx_set_error("sparse set dimension %d does not match the descriptor size %d", dim, m->D);
+    return fail(TGX_ERR_ARG);
+  }
+  if (n_sparse < 1 || !Qs || !alphas) {
+    tgx_set_error("empty sparse set");
+    return fail(TGX_ERR_ARG);
+  }
+  m->central_species = central_species;
+  m->delta = delta;
+  m->zeta = zeta;
+  m->e0 = e0;
+  m->S = n_sparse;
+  m->Dpad = (int)tgx_round_up(m->D, 128);
+  m->Spad = (int)tgx_round_up(n_sparse, 128);
+  m->rcut_max = *std::max_element(m->rcut_hard.begin(), m->rcut_hard.end());
+  // kernel parameters
+  SoapParams& P = m->P;
+  P.n_species = S;
+  P.L = m->l_max;
+  P.NLM = (m->l_max + 1) * (m->l_max + 1);
+  P.N = N;
+  P.D = m->D;
+  P.Dpad = m->Dpad;
+  int off = 0;
+  for (int s = 0; s < S; ++s) {
+    SpeciesParams& sp = P.sp[s];
+    sp.rc = m->rcut_hard[s];
+    sp.inv_rc = 1.0 / sp.rc;
+    sp.sig_r = m->sig_r[s];
+    sp.sig_r_sc = m->sig_r_sc[s];
+    sp.sig_t = m->sig_t[s];
+    sp.sig_t_sc = m->sig_t_sc[s];
+    sp.amp_a = m->amp_a[s];
+    sp.gs = m->global_scaling[s];
+    sp.nb = m->alpha_max[s];
+    sp.off = off;
+    off += sp.nb;
+  }
+  std::vector<double> B, xq, cen;
+  tgx_build_radial_tables(m, B, xq, cen);
+  std::vector<int> dtab;
+  for (int n = 0; n < N; ++n)
+    for (int np = n; np < N; ++np)
+      for (int l = 0; l <= m->l_max; ++l) dtab.push_back(n | (np << 8) | (l << 16));
+  auto up = [&](dbuf& b, const void* src, size_t bytes) -> int {
+    TGX_TRY(b.ensure(bytes));
+    TGX_CHECK_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return TGX_OK;
+  };
+  // sparse set: Qs(dim, n_sparse) column-major is [s][d] row-major
+  const int Dp = m->Dpad, Sp = m->Spad;
+  std::vector<double> qs((size_t)Sp * Dp, 0.0), qst((size_t)Dp * Sp, 0.0), al(Sp, 0.0);
+  std::vector<__bf16> qsb((size_t)Sp * Dp), qstb((size_t)Dp * Sp);
+  std::vector<float> alf(Sp, 0.0f);
+  for (int s = 0; s < n_sparse; ++s) {
+    al[s] = alphas[s];
+    alf[s] = (float)alphas[s];
+    for (int d = 0; d < m->D; ++d) {
+      const double v = Qs[(size_t)s * m->D + d];
+      qs[(size_t)s * Dp + d] = v;
+      qst[(size_t)d * Sp + s] = v;
+    }
+  }
+  for (size_t k = 0; k < qs.size(); ++k) qsb[k] = (__bf16)(float)qs[k];
+  for (size_t k = 0; k < qst.size(); ++k) qstb[k] = (__bf16)(float)qst[k];
+  if ((st = up(m->B, B.data(), sizeof(double) * B.size())) || (st = up(m->xq, xq.data(), sizeof(double) * xq.size())) ||
+      (st = up(m->cen, cen.data(), sizeof(double) * cen.size())) ||
+      (st = up(m->dtab, dtab.data(), sizeof(int) * dtab.size())) ||
+      (st = up(m->Qs, qs.data(), sizeof(double) * qs.size())) ||
+      (st = up(m->QsT, qst.data(), sizeof(double) * qst.size())) ||
+      (st = up(m->alpha, al.data(), sizeof(double) * al.size())) ||
+      (st = up(m->Qs_bf, qsb.data(), sizeof(__bf16) * qsb.size())) ||
+      (st = up(m->QsT_bf, qstb.data(), sizeof(__bf16) * qstb.size())) ||
+      (st = up(m->alpha_f, alf.data(), sizeof(float) * alf.size())))
+    return fail(st);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    tgx_set_error("upload of the sparse set failed");
+    return fail(TGX_ERR_HIP);
+  }
+  P.B = m->B.as<double>();
+  P.xq = m->xq.as<double>();
+  P.cen = m->cen.as<double>();
+  P.dtab = m->dtab.as<int>();
+  ctx->models.push_back(m);
+  *model_id = (int)ctx->models.size() - 1;
+  return TGX_OK;
+}
+
+static int get_model(tgx_ctx* ctx, int id, tgx_model_t** m) {
+  TGX_REQUIRE(id >= 0 && id < (int)ctx->models.size(), "model id %d not defined", id);
+  *m = ctx->models[id];
+  return TGX_OK;
+}
+
+extern "C" int tgx_soap_model_dim(tgx_ctx* ctx, int model_id, int* dim) {
+  TGX_CTX_CHECK(ctx);
+  tgx_model_t* m;
+  TGX_TRY(get_model(ctx, model_id, &m));
+  if (dim) *dim = m->D;
+  return TGX_OK;
+}
+
+extern "C" int tgx_soap_backend_begin(tgx_ctx* ctx, int model_id) {
+  TGX_CTX_CHECK(ctx);
+  tgx_model_t* m;
+  TGX_TRY(get_model(ctx, model_id, &m));
+  ctx->current = model_id;
+  return TGX_OK;
+}
+
+extern "C" int tgx_soap_backend_end(tgx_ctx* ctx) {
+  TGX_CTX_CHECK(ctx);
+  ctx->current = -1;
+  return TGX_OK;
+}
+
+static int current_model(tgx_ctx* ctx, tgx_model_t** m) {
+  TGX_REQUIRE(ctx->current >= 0, "no model selected: call tgx_soap_backend_begin first");
+  *m = ctx->models[ctx->current];
+  return TGX_OK;
+}
+
+// One block's energies and pair forces (pair forces accumulated into fglob).
+static int run_block(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces) {
+  TGX_TRY(tgx_block_select(ctx, m));
+  const bool plain = path == TGX_PATH_FP64_PLAIN;
+  TGX_TRY(tgx_soap_descriptors(ctx, m, path, plain && do_forces));
+  TGX_TRY(tgx_gap_contract(ctx, m, path, do_forces));
+  if (do_forces) {
+    if (plain) TGX_TRY(tgx_plain_pair_forces(ctx, m));
+    else TGX_TRY(tgx_adjoint_pair_forces(ctx, m, path));
+  }
+  return TGX_OK;
+}
+
+static int begin_step(tgx_ctx* ctx) {
+  TGX_REQUIRE(ctx->have_list, "no neighbour list: call tgx_build_neighbors_list first");
+  const int n = ctx->n_sites;
+  const size_t P = (size_t)(ctx->n_pairs > 0 ? ctx->n_pairs : 1);
+  TGX_TRY(ctx->energies.ensure(sizeof(double) * (size_t)n));
+  TGX_TRY(ctx->forces.ensure(sizeof(double) * 3 * (size_t)n));
+  TGX_TRY(ctx->fglob.ensure(sizeof(double) * 3 * P));
+  TGX_TRY(tgx_zero(ctx, ctx->energies.p, sizeof(double) * (size_t)n));
+  TGX_TRY(tgx_zero(ctx, ctx->fglob.p, sizeof(double) * 3 * P));
+  return TGX_OK;
+}
+
+extern "C" int tgx_get_gap_soap(tgx_ctx* ctx, int path, int do_forces, double* energies, double* forces,
+                                double* virial, int* n_sites_block, int64_t* n_pairs_block) {
+  TGX_CTX_CHECK(ctx);
+  TGX_REQUIRE(path == TGX_PATH_FP64_PLAIN || path == TGX_PATH_FP64_ADJOINT || path == TGX_PATH_MIXED,
+              "tgx_get_gap_soap: path %d not available here", path);
+  tgx_model_t* m;
+  TGX_TRY(current_model(ctx, &m));
+  TGX_TRY(begin_step(ctx));
+  TGX_TRY(run_block(ctx, m, path, do_forces != 0));
+  TGX_TRY(tgx_energy_scatter(ctx, m));
+  const int n = ctx->n_sites;
+  double vir[9] = {0};
+  if (do_forces) TGX_TRY(tgx_gather_forces(ctx, nullptr, vir));
+  // fold back like src/gap_interface.f90:422-431: assign, for touched atoms only
+  std::vector<int> sa(m->n_sites_b), seen(n, 0);
+  std::vector<double> es(m->n_sites_b), F;
+  if (m->n_sites_b) {
+    TGX_CHECK_HIP(hipMemcpyAsync(sa.data(), m->site_atom.p, sizeof(int) * sa.size(), hipMemcpyDeviceToHost, ctx->stream));
+    TGX_CHECK_HIP(hipMemcpyAsync(es.data(), m->e_site.p, sizeof(double) * es.size(), hipMemcpyDeviceToHost, ctx->stream));
+  }
+  if (do_forces) {
+    TGX_TRY(ctx->gate_mask.ensure(sizeof(int) * (size_t)n));
+    TGX_TRY(tgx_block_seen(ctx, m, ctx->gate_mask.as<int>()));
+    F.resize(3 * (size_t)n);
+    TGX_CHECK_HIP(hipMemcpyAsync(seen.data(), ctx->gate_mask.p, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    TGX_CHECK_HIP(hipMemcpyAsync(F.data(), ctx->forces.p, sizeof(double) * F.size(), hipMemcpyDeviceToHost, ctx->stream));
+  }
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  if (energies)
+    for (int s = 0; s < m->n_sites_b; ++s) energies[sa[s]] = es[s];
+  if (do_forces) {
+    if (forces)
+      for (int i = 0; i < n; ++i)
+        if (seen[i])
+          for (int k = 0; k < 3; ++k) forces[3 * (size_t)i + k] = F[3 * (size_t)i + k];
+    if (virial)
+      for (int k = 0; k < 9; ++k) virial[k] = vir[k];
+  } else if (virial) {
+    for (int k = 0; k < 9; ++k) virial[k] = 0.0;
+  }
+  if (n_sites_block) *n_sites_block = m->n_sites_b;
+  if (n_pairs_block) *n_pairs_block = m->n_pairs_b;
+  return TGX_OK;
+}
+
+extern "C" int tgx_soap_block_size(tgx_ctx* ctx, int* n_sites_block, int64_t* n_pairs_block) {
+  TGX_CTX_CHECK(ctx);
+  tgx_model_t* m;
+  TGX_TRY(current_model(ctx, &m));
+  TGX_TRY(tgx_block_select(ctx, m));
+  if (n_sites_block) *n_sites_block = m->n_sites_b;
+  if (n_pairs_block) *n_pairs_block = m->n_pairs_b;
+  return TGX_OK;
+}
+
+extern "C" int tgx_get_soap(tgx_ctx* ctx, double* soap, double* soap_cart_der) {
+  TGX_CTX_CHECK(ctx);
+  tgx_model_t* m;
+  TGX_TRY(current_model(ctx, &m));
+  TGX_TRY(tgx_block_select(ctx, m));
+  TGX_TRY(tgx_soap_descriptors(ctx, m, TGX_PATH_FP64_PLAIN, soap_cart_der != nullptr));
+  const int ns = m->n_sites_b;
+  if (soap && ns)
+    TGX_CHECK_HIP(hipMemcpy2DAsync(soap, sizeof(double) * m->D, m->q64.p, sizeof(double) * m->Dpad, sizeof(double) * m->D,
+                                   ns, hipMemcpyDeviceToHost, ctx->stream));
+  if (soap_cart_der && m->n_pairs_b)
+    TGX_CHECK_HIP(hipMemcpyAsync(soap_cart_der, m->soap_der.p, sizeof(double) * 3 * m->D * (size_t)m->n_pairs_b,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return TGX_OK;
+}
+
+extern "C" int tgx_get_soap_energy_and_forces(tgx_ctx* ctx, int n_sites, int n_soap, int64_t n_pairs,
+                                              const double* soap, const double* soap_der, double e0,
+                                              const int* n_neigh, const int* neighbors_list, const double* xyz,
+                                              int n_sites0, int do_forces, double* energies, double* forces,
+                                              double* virial) {
+  TGX_CTX_CHECK(ctx);
+  tgx_model_t* m;
+  TGX_TRY(current_model(ctx, &m));
+  TGX_REQUIRE(n_soap == m->D, "n_soap=%d does not match the model's descriptor size %d", n_soap, m->D);
+  TGX_REQUIRE(n_sites >= 0 && soap && energies, "tgx_get_soap_energy_and_forces: bad arguments");
+  if (do_forces)
+    TGX_REQUIRE(soap_der && n_neigh && neighbors_list && xyz && forces && virial && n_sites0 > 0,
+                "tgx_get_soap_energy_and_forces: forces need soap_der, lists, xyz and outputs");
+  if (n_sites == 0) return TGX_OK;
+  // stage the inputs as a block of the current model
+  std::vector<int64_t> beg(n_sites + 1, 0), gidx;
+  if (do_forces) {
+    for (int i = 0; i < n_sites; ++i) beg[i + 1] = beg[i] + n_neigh[i];
+    TGX_REQUIRE(beg[n_sites] == n_pairs, "sum of n_neigh (%lld) differs from n_pairs (%lld)", (long long)beg[n_sites],
+                (long long)n_pairs);
+    gidx.resize((size_t)n_pairs);
+    for (int64_t p = 0; p < n_pairs; ++p) gidx[(size_t)p] = p;
+  }
+  m->n_sites_b = n_sites;
+  m->n_pairs_b = do_forces ? n_pairs : 0;
+  const size_t nsp = (size_t)tgx_round_up(n_sites, 128);
+  TGX_TRY(m->q64.ensure(sizeof(double) * nsp * m->Dpad));
+  TGX_TRY(tgx_zero(ctx, m->q64.p, sizeof(double) * nsp * m->Dpad));
+  TGX_CHECK_HIP(hipMemcpy2DAsync(m->q64.p, sizeof(double) * m->Dpad, soap, sizeof(double) * m->D, sizeof(double) * m->D,
+                                 n_sites, hipMemcpyHostToDevice, ctx->stream));
+  const double e0_save = m->e0;
+  m->e0 = e0;
+  int st = tgx_gap_contract(ctx, m, TGX_PATH_FP64_PLAIN, do_forces != 0);
+  m->e0 = e0_save;
+  TGX_TRY(st);
+  std::vector<double> fp;
+  if (do_forces && n_pairs > 0) {
+    TGX_TRY(m->site_beg.ensure(sizeof(int64_t) * beg.size()));
+    TGX_TRY(m->pr_gidx.ensure(sizeof(int64_t) * gidx.size()));
+    TGX_TRY(m->soap_der.ensure(sizeof(double) * 3 * (size_t)m->D * (size_t)n_pairs));
+    TGX_TRY(ctx->fglob.ensure(sizeof(double) * 3 * (size_t)n_pairs));
+    TGX_TRY(tgx_zero(ctx, ctx->fglob.p, sizeof(double) * 3 * (size_t)n_pairs));
+    TGX_CHECK_HIP(hipMemcpyAsync(m->site_beg.p, beg.data(), sizeof(int64_t) * beg.size(), hipMemcpyHostToDevice, ctx->stream));
+    TGX_CHECK_HIP(hipMemcpyAsync(m->pr_gidx.p, gidx.data(), sizeof(int64_t) * gidx.size(), hipMemcpyHostToDevice, ctx->stream));
+    TGX_CHECK_HIP(hipMemcpyAsync(m->soap_der.p, soap_der, sizeof(double) * 3 * (size_t)m->D * (size_t)n_pairs,
+                                 hipMemcpyHostToDevice, ctx->stream));
+    TGX_TRY(tgx_plain_pair_forces(ctx, m));
+    fp.resize(3 * (size_t)n_pairs);
+    TGX_CHECK_HIP(hipMemcpyAsync(fp.data(), ctx->fglob.p, sizeof(double) * fp.size(), hipMemcpyDeviceToHost, ctx->stream));
+  }
+  TGX_CHECK_HIP(hipMemcpyAsync(energies, m->e_site.p, sizeof(double) * (size_t)n_sites, hipMemcpyDeviceToHost, ctx->stream));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  if (do_forces) {
+    // src/gap.f90:204-224: accumulate in the reference's pair order
+    for (int64_t k = 0; k < 3 * (int64_t)n_sites0; ++k) forces[k] = 0.0;
+    for (int k = 0; k < 9; ++k) virial[k] = 0.0;
+    for (int64_t l = 0; l < n_pairs; ++l) {
+      const int j2 = (neighbors_list[l] - 1) % n_sites0;
+      const double* f = &fp[3 * (size_t)l];
+      for (int k = 0; k < 3; ++k) forces[3 * (size_t)j2 + k] += f[k];
+      for (int k1 = 0; k1 < 3; ++k1)
+        for (int k2 = 0; k2 < 3; ++k2) virial[3 * k2 + k1] += 0.5 * (f[k1] * xyz[3 * l + k2] + f[k2] * xyz[3 * l + k1]);
+    }
+  }
+  return TGX_OK;
+}
+
+// ---- whole step ------------------------------------------------------------------------
+int tgx_surrogate_run(tgx_ctx* ctx, tgx_model_t* m);
+
+extern "C" int tgx_compute(tgx_ctx* ctx, int path, double* energy, double* virial) {
+  TGX_CTX_CHECK(ctx);
+  TGX_REQUIRE(path >= TGX_PATH_FP64_PLAIN && path <= TGX_PATH_SURROGATE, "tgx_compute: unknown path %d", path);
+  TGX_REQUIRE(!ctx->models.empty(), "tgx_compute: no model defined");
+  TGX_TRY(begin_step(ctx));
+  for (auto* m : ctx->models) {
+    if (path == TGX_PATH_SURROGATE) {
+      TGX_TRY(tgx_surrogate_run(ctx, m));
+    } else {
+      TGX_TRY(run_block(ctx, m, path, true));
+    }
+    TGX_TRY(tgx_energy_scatter(ctx, m));
+  }
+  TGX_TRY(tgx_gather_forces(ctx, energy, virial));
+  return TGX_OK;
+}
+
+extern "C" int tgx_get_results(tgx_ctx* ctx, double* energies, double* forces) {
+  TGX_CTX_CHECK(ctx);
+  const size_t n = (size_t)ctx->n_sites;
+  if (energies) TGX_CHECK_HIP(hipMemcpyAsync(energies, ctx->energies.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+  if (forces) TGX_CHECK_HIP(hipMemcpyAsync(forces, ctx->forces.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return TGX_OK;
+}
+
+extern "C" int tgx_results_device(tgx_ctx* ctx, double** d_forces, double** d_energies) {
+  TGX_CTX_CHECK(ctx);
+  if (d_forces) *d_forces = ctx->forces.as<double>();
+  if (d_energies) *d_energies = ctx->energies.as<double>();
+  return TGX_OK;
+}

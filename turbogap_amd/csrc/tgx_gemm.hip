@@ -1,0 +1,326 @@
+// GAP kernel contraction on the matrix cores (src/gap.f90:115-186):
+//   K    = q . Qs^T                       (n_sites x n_sparse, k = D)
+//   E_i  = delta^2 sum_s alpha_s K_is^zeta + e0
+//   Kd   = -zeta delta^2 K^(zeta-1) o alpha
+//   Qss  = Kd . Qs                        (n_sites x D, k = n_sparse)  = -dE/dq
+// Both products are "NT" GEMMs (both operands contiguous along k): Qs is kept
+// as [S][D] for the first and transposed [D][S] for the second, uploaded once.
+//
+// FP64: v_mfma_f64_16x16x4f64. BF16: v_mfma_f32_16x16x32_bf16 with FP32
+// accumulation; the epilogue keeps the energy row sums in FP64.
+// Workgroup tile 128x128, four wavefronts of 64x64 (4x4 MFMA tiles each).
+// Tiles are handed out XCD-aware: the eight XCDs each walk a contiguous run of
+// tiles, N fastest, so the A row-panel a wavefront group reuses stays in the
+// XCD's L2.
+#include "tgx_internal.h"
+
+namespace {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int BM = 128, BN = 128;
+
+enum { EPI_STORE = 0, EPI_GAP = 1 };
+
+struct EpiArgs {
+  double pref;           // -zeta delta^2
+  double zeta;
+  int zeta_int;          // >= 0 when zeta is an integer
+  const double* alpha;   // [N] (FP64 path)
+  const float* alpha_f;  // [N] (BF16 path)
+  void* out;             // Kd or Qss
+  int ldo;
+  double* epart;         // [M][n_tiles] (EPI_GAP)
+  int n_tiles;
+};
+
+__device__ __forceinline__ void tile_of(int& tm, int& tn) {
+  const int ntn = gridDim.x, ntm = gridDim.y;
+  const int total = ntn * ntm;
+  const int lin = blockIdx.x + ntn * blockIdx.y;
+  const int xcd = lin & 7, local = lin >> 3;
+  // tiles with lin % 8 == x go to XCD x; give each XCD a contiguous range
+  const int base = total / 8, rem = total % 8;
+  const int start = xcd * base + (xcd < rem ? xcd : rem);
+  const int tile = start + local;
+  tm = tile / ntn;
+  tn = tile - tm * ntn;
+}
+
+template <class T>
+__device__ __forceinline__ void kpow(T k, double zeta, int zi, T& kz, T& kzm1) {
+  if (zi >= 1) {
+    T p = 1;
+    for (int i = 0; i < zi - 1; ++i) p *= k;
+    kzm1 = p;
+    kz = p * k;
+  } else {
+    kzm1 = (T)pow((double)k, zeta - 1.0);
+    kz = (T)pow((double)k, zeta);
+  }
+}
+
+// Shared epilogue. acc holds, for tile (ti, tj) and register r, the element
+// (row0 + ti*16 + rr, col0 + tj*16 + lane%16) where the row within the tile is
+//   rr = 4*r + lane/16   for v_mfma_f64_16x16x4f64   (register-major)
+//   rr = 4*(lane/16) + r for the FP32-output MFMAs   (lane-group-major)
+// (checked against a host GEMM on the device).
+template <bool F64> __device__ __forceinline__ int tile_row(int lane, int r) {
+  return F64 ? 4 * r + (lane >> 4) : 4 * (lane >> 4) + r;
+}
+
+template <int EPI, class OutT, class AccT, class VecT, bool F64>
+__device__ void epilogue(const VecT (&acc)[4][4], int m0, int n0, int tn, int wm, int wn, const EpiArgs& e,
+                         double* sE) {
+  const int lane = threadIdx.x & 63;
+  const int rbase = m0 + wm * 64;
+  const int cbase = n0 + wn * 64 + (lane & 15);
+  OutT* out = reinterpret_cast<OutT*>(e.out);
+  if (EPI == EPI_STORE) {
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          out[(size_t)(rbase + ti * 16 + tile_row<F64>(lane, r)) * e.ldo + cbase + tj * 16] = (OutT)acc[ti][tj][r];
+    return;
+  }
+  double erow[4][4];
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) erow[ti][r] = 0.0;
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj) {
+    const int col = cbase + tj * 16;
+    const AccT al = e.alpha_f ? (AccT)e.alpha_f[col] : (AccT)e.alpha[col];
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        AccT kz, kzm1;
+        kpow<AccT>(acc[ti][tj][r], e.zeta, e.zeta_int, kz, kzm1);
+        out[(size_t)(rbase + ti * 16 + tile_row<F64>(lane, r)) * e.ldo + col] = (OutT)((AccT)e.pref * kzm1 * al);
+        erow[ti][r] += (double)(al * kz);
+      }
+  }
+  // reduce over the 16 lanes that share rows, then over the two column waves
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double v = erow[ti][r];
+      v += __shfl_xor(v, 1);
+      v += __shfl_xor(v, 2);
+      v += __shfl_xor(v, 4);
+      v += __shfl_xor(v, 8);
+      erow[ti][r] = v;
+    }
+  if ((lane & 15) == 0) {
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sE[wn * BM + wm * 64 + ti * 16 + tile_row<F64>(lane, r)] = erow[ti][r];
+  }
+  __syncthreads();
+  if (threadIdx.x < BM) {
+    const int row = m0 + threadIdx.x;
+    e.epart[(size_t)row * e.n_tiles + tn] = sE[threadIdx.x] + sE[BM + threadIdx.x];
+  }
+}
+
+// FP64 NT GEMM. M, N multiples of 128; K multiple of 16.
+template <int EPI>
+__global__ __launch_bounds__(256) void k_gemm_f64(int K, const double* __restrict__ A, int lda,
+                                                  const double* __restrict__ B, int ldb, EpiArgs e) {
+  constexpr int BK = 16, LDS_STRIDE = BM + 4;
+  __shared__ double As[BK * LDS_STRIDE];
+  __shared__ double Bs[BK * LDS_STRIDE];
+  __shared__ double sE[2 * BM];
+  int tm, tn;
+  tile_of(tm, tn);
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const int lrow = tid >> 1, lk = (tid & 1) * 8;
+  const double* Ag = A + (size_t)(m0 + lrow) * lda + lk;
+  const double* Bg = B + (size_t)(n0 + lrow) * ldb + lk;
+  f64x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (f64x4){0, 0, 0, 0};
+  double ra[8], rb[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) { ra[j] = Ag[j]; rb[j] = Bg[j]; }
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      As[(lk + j) * LDS_STRIDE + lrow] = ra[j];
+      Bs[(lk + j) * LDS_STRIDE + lrow] = rb[j];
+    }
+    __syncthreads();
+    if (k0 + BK < K) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) { ra[j] = Ag[k0 + BK + j]; rb[j] = Bg[k0 + BK + j]; }
+    }
+#pragma unroll
+    for (int ks = 0; ks < BK / 4; ++ks) {
+      const int kr = (ks * 4 + (lane >> 4)) * LDS_STRIDE;
+      double a[4], b[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        a[t] = As[kr + wm * 64 + t * 16 + (lane & 15)];
+        b[t] = Bs[kr + wn * 64 + t * 16 + (lane & 15)];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  epilogue<EPI, double, double, f64x4, true>(acc, m0, n0, tn, wm, wn, e, sE);
+}
+
+// BF16 NT GEMM with FP32 accumulation. M, N multiples of 128; K multiple of 64.
+template <int EPI, class OutT>
+__global__ __launch_bounds__(256) void k_gemm_bf16(int K, const __bf16* __restrict__ A, int lda,
+                                                   const __bf16* __restrict__ B, int ldb, EpiArgs e) {
+  constexpr int BK = 64, LDS_STRIDE = BK + 8;
+  __shared__ __align__(16) __bf16 As[BM * LDS_STRIDE];
+  __shared__ __align__(16) __bf16 Bs[BN * LDS_STRIDE];
+  __shared__ double sE[2 * BM];
+  int tm, tn;
+  tile_of(tm, tn);
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const int lrow = tid >> 1, lk = (tid & 1) * 32;
+  const uint4* Ag = reinterpret_cast<const uint4*>(A + (size_t)(m0 + lrow) * lda + lk);
+  const uint4* Bg = reinterpret_cast<const uint4*>(B + (size_t)(n0 + lrow) * ldb + lk);
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (f32x4){0, 0, 0, 0};
+  uint4 ra[4], rb[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) { ra[j] = Ag[j]; rb[j] = Bg[j]; }
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    __syncthreads();
+    uint4* as = reinterpret_cast<uint4*>(As + lrow * LDS_STRIDE + lk);
+    uint4* bs = reinterpret_cast<uint4*>(Bs + lrow * LDS_STRIDE + lk);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) { as[j] = ra[j]; bs[j] = rb[j]; }
+    __syncthreads();
+    if (k0 + BK < K) {
+      const int adv = (k0 + BK) / 8;  // uint4 holds 8 bf16
+#pragma unroll
+      for (int j = 0; j < 4; ++j) { ra[j] = Ag[adv + j]; rb[j] = Bg[adv + j]; }
+    }
+#pragma unroll
+    for (int ks = 0; ks < BK / 32; ++ks) {
+      bf16x8 a[4], b[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        a[t] = *reinterpret_cast<const bf16x8*>(As + (wm * 64 + t * 16 + (lane & 15)) * LDS_STRIDE + ks * 32 + 8 * (lane >> 4));
+        b[t] = *reinterpret_cast<const bf16x8*>(Bs + (wn * 64 + t * 16 + (lane & 15)) * LDS_STRIDE + ks * 32 + 8 * (lane >> 4));
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  epilogue<EPI, OutT, float, f32x4, false>(acc, m0, n0, tn, wm, wn, e, sE);
+}
+
+// E_i = delta^2 sum_t epart[i][t] + e0, the partials summed in tile order.
+__global__ void k_energy_finalize(int n, const double* __restrict__ epart, int n_tiles, double d2, double e0,
+                                  double* __restrict__ e_site) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int t = 0; t < n_tiles; ++t) s += epart[(size_t)i * n_tiles + t];
+  e_site[i] = d2 * s + e0;
+}
+
+__global__ void k_f32_to_bf16(int64_t n, const float* __restrict__ x, __bf16* __restrict__ y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) y[i] = (__bf16)x[i];
+}
+
+}  // namespace
+
+static EpiArgs gap_epi(tgx_model_t* m) {
+  EpiArgs e{};
+  const double z = m->zeta;
+  const bool zint = std::fabs(z - (double)(int)z) < 1.0e-5;
+  e.zeta_int = zint ? (int)z : 0;
+  e.zeta = zint ? (double)(int)z : z;
+  e.pref = -e.zeta * m->delta * m->delta;
+  return e;
+}
+
+static int launch_f64(tgx_ctx* ctx, int epi, int M, int N, int K, const double* A, int lda, const double* B, int ldb,
+                      EpiArgs e, const char* name) {
+  TGX_REQUIRE(M % BM == 0 && N % BN == 0 && K % 16 == 0, "gemm_f64: unpadded shape %d x %d x %d", M, N, K);
+  dim3 grid(N / BN, M / BM);
+  if (epi == EPI_GAP) TGX_LAUNCH(ctx, name, k_gemm_f64<EPI_GAP>, grid, dim3(256), 0, K, A, lda, B, ldb, e);
+  else TGX_LAUNCH(ctx, name, k_gemm_f64<EPI_STORE>, grid, dim3(256), 0, K, A, lda, B, ldb, e);
+  return TGX_OK;
+}
+
+int tgx_gap_contract(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces) {
+  const int ns = m->n_sites_b;
+  if (ns == 0) return TGX_OK;
+  const int Mp = (int)tgx_round_up(ns, BM);
+  const int Sp = m->Spad, Dp = m->Dpad;
+  const int n_tiles = Sp / BN;
+  TGX_TRY(m->epart.ensure(sizeof(double) * (size_t)Mp * n_tiles));
+  TGX_TRY(m->e_site.ensure(sizeof(double) * (size_t)Mp));
+  EpiArgs e = gap_epi(m);
+  e.epart = m->epart.as<double>();
+  e.n_tiles = n_tiles;
+  if (path == TGX_PATH_FP64_PLAIN || path == TGX_PATH_FP64_ADJOINT) {
+    TGX_TRY(m->kd64.ensure(sizeof(double) * (size_t)Mp * Sp));
+    e.alpha = m->alpha.as<double>();
+    e.out = m->kd64.p;
+    e.ldo = Sp;
+    TGX_TRY(launch_f64(ctx, EPI_GAP, Mp, Sp, Dp, m->q64.as<double>(), Dp, m->Qs.as<double>(), Dp, e, "gemm_kernel_f64"));
+  } else {
+    TGX_TRY(m->qbf.ensure(sizeof(__bf16) * (size_t)Mp * Dp));
+    TGX_TRY(m->kdbf.ensure(sizeof(__bf16) * (size_t)Mp * Sp));
+    const int64_t nq = (int64_t)Mp * Dp;
+    TGX_LAUNCH(ctx, "q_to_bf16", k_f32_to_bf16, dim3(2048), dim3(256), 0, nq, m->q32.as<float>(), m->qbf.as<__bf16>());
+    e.alpha_f = m->alpha_f.as<float>();
+    e.out = m->kdbf.p;
+    e.ldo = Sp;
+    TGX_REQUIRE(Dp % 64 == 0 && Sp % 64 == 0, "gemm_bf16: unpadded shape");
+    TGX_LAUNCH(ctx, "gemm_kernel_bf16", (k_gemm_bf16<EPI_GAP, __bf16>), dim3(Sp / BN, Mp / BM), dim3(256), 0, Dp,
+               m->qbf.as<__bf16>(), Dp, m->Qs_bf.as<__bf16>(), Dp, e);
+  }
+  TGX_LAUNCH(ctx, "energy_finalize", k_energy_finalize, dim3((ns + 255) / 256), dim3(256), 0, ns,
+             m->epart.as<double>(), n_tiles, m->delta * m->delta, m->e0, m->e_site.as<double>());
+  if (!do_forces) return TGX_OK;
+  EpiArgs s{};
+  if (path == TGX_PATH_FP64_PLAIN || path == TGX_PATH_FP64_ADJOINT) {
+    TGX_TRY(m->qss64.ensure(sizeof(double) * (size_t)Mp * Dp));
+    s.out = m->qss64.p;
+    s.ldo = Dp;
+    TGX_TRY(launch_f64(ctx, EPI_STORE, Mp, Dp, Sp, m->kd64.as<double>(), Sp, m->QsT.as<double>(), Sp, s,
+                       "gemm_qss_f64"));
+  } else {
+    TGX_TRY(m->qss32.ensure(sizeof(float) * (size_t)Mp * Dp));
+    s.out = m->qss32.p;
+    s.ldo = Dp;
+    TGX_LAUNCH(ctx, "gemm_qss_bf16", (k_gemm_bf16<EPI_STORE, float>), dim3(Dp / BN, Mp / BM), dim3(256), 0, Sp,
+               m->kdbf.as<__bf16>(), Sp, m->QsT_bf.as<__bf16>(), Sp, s);
+  }
+  return TGX_OK;
+}

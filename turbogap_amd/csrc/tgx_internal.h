@@ -1,0 +1,212 @@
+// Internal declarations shared by the libtgx translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include <cstring>
+
+#include "../../include/tgx.h"
+
+// Limits of this build. Checked on the host before any launch.
+#define TGX_MAX_SPECIES 4
+#define TGX_MAX_NB 16    // radial functions per species
+#define TGX_MAX_NRAD 32  // stacked radial functions
+#define TGX_MAX_L 10
+#define TGX_MAX_Q 64     // radial quadrature nodes per species
+#define TGX_PAD 32       // descriptor / sparse padding granule
+
+void tgx_set_error(const char* fmt, ...);
+
+#define TGX_CHECK_HIP(call)                                                                     \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess) {                                                                     \
+      tgx_set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return TGX_ERR_HIP;                                                                       \
+    }                                                                                           \
+  } while (0)
+
+#define TGX_REQUIRE(cond, ...)      \
+  do {                              \
+    if (!(cond)) {                  \
+      tgx_set_error(__VA_ARGS__);   \
+      return TGX_ERR_ARG;           \
+    }                               \
+  } while (0)
+
+#define TGX_TRY(call)          \
+  do {                         \
+    int s_ = (call);           \
+    if (s_ != TGX_OK) return s_; \
+  } while (0)
+
+static inline int64_t tgx_round_up(int64_t x, int64_t g) { return (x + g - 1) / g * g; }
+
+// Growable device buffer.
+struct dbuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t n) {
+    if (n <= bytes && p) return TGX_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    size_t want = n < 256 ? 256 : n;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+      tgx_set_error("hipMalloc(%zu bytes) failed: %s", want, hipGetErrorString(e));
+      p = nullptr;
+      return TGX_ERR_NOMEM;
+    }
+    bytes = want;
+    return TGX_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// Per-species SOAP parameters as the kernels see them (lengths in Å unless
+// marked as scaled by rcut_hard).
+struct SpeciesParams {
+  double rc;         // rcut_hard
+  double inv_rc;
+  double sig_r;      // atom_sigma_r (Å)
+  double sig_r_sc;   // atom_sigma_r_scaling (1/Å)
+  double sig_t;      // atom_sigma_t (Å)
+  double sig_t_sc;   // atom_sigma_t_scaling (1/Å)
+  double amp_a;      // amplitude_scaling
+  double gs;         // global_scaling
+  int nb;            // alpha_max
+  int off;           // first stacked radial index
+  int nq;            // quadrature nodes
+  int pad_;
+};
+
+struct SoapParams {
+  int n_species;
+  int L;     // l_max
+  int NLM;   // (L+1)^2
+  int N;     // stacked radial functions
+  int D;     // descriptor size
+  int Dpad;  // padded row stride of q
+  SpeciesParams sp[TGX_MAX_SPECIES];
+  const double* B;    // [species][TGX_MAX_NB][TGX_MAX_Q]
+  const double* xq;   // [species][TGX_MAX_Q]
+  const double* cen;  // [species][TGX_MAX_NB] central-atom coefficients (weights and Y00 folded in)
+  const int* dtab;    // [D] packed n | n' << 8 | l << 16
+};
+
+struct tgx_model_t {
+  // hypers (host copies)
+  int n_species = 0, l_max = 0, basis = 0, radial_enhancement = 0;
+  int nq_soft = 32, nq_buffer = 16;
+  std::vector<int> alpha_max, species_types;
+  std::vector<double> rcut_hard, rcut_soft, nf, global_scaling, sig_r, sig_r_sc, sig_t, sig_t_sc, amp_a, cw;
+  int central_species = 0;
+  double delta = 1.0, zeta = 2.0, e0 = 0.0;
+  int N = 0, D = 0, Dpad = 0, S = 0, Spad = 0;
+  double rcut_max = 0.0;
+  SoapParams P{};
+  // device copies of the tables
+  dbuf B, xq, cen, dtab;
+  // sparse set: Qs[Spad][Dpad], QsT[Dpad][Spad], alphas[Spad] (FP64) and BF16 copies
+  dbuf Qs, QsT, alpha, Qs_bf, QsT_bf, alpha_f;
+  // surrogate members (BF16 weights + FP32 biases)
+  struct member { int hidden = 0; dbuf W1, W2, w3, b1, b2, b3; double e_shift = 0, e_scale = 1; };
+  std::vector<member> mlp;
+  // per-step block state (device)
+  int n_sites_b = 0;
+  int64_t n_pairs_b = 0;
+  dbuf site_atom;   // int[n_sites_b] global 0-based atom of every centre
+  dbuf site_beg;    // int64[n_sites_b+1] pair offsets
+  dbuf pr_gidx;     // int64[n_pairs_b] global pair index of every kept pair
+  dbuf pr_nbr;      // int[n_pairs_b] folded 0-based neighbour atom
+  dbuf pr_spec;     // int[n_pairs_b] descriptor species (0-based) or -1
+  dbuf pr_r;        // double[n_pairs_b]
+  dbuf pr_xyz;      // double[3 n_pairs_b]
+  dbuf q64, q32, qbf;   // descriptors [n_sites_pad][Dpad]
+  dbuf pnorm;       // double[n_sites] |p|
+  dbuf e_site;      // double[n_sites] energies
+  dbuf qss64;       // [n_sites_pad][Dpad] FP64 -dE/dq
+  dbuf qss32;       // FP32 -dE/dq
+  dbuf kd64, kdbf;  // [n_sites_pad][Spad] K^(zeta-1) alpha (scaled)
+  dbuf epart;       // energy partials [n_sites_pad][n_tiles]
+  dbuf soap_der;    // [n_pairs_b][D][3] FP64 (plain path)
+  dbuf fpair;       // [n_pairs_b][3] pair forces
+};
+
+struct prof_rec { std::string name; hipEvent_t a, b; };
+
+struct tgx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // system
+  int n_sites = 0, n_sites_sc = 0;
+  double box[9] = {0};
+  std::vector<int> species_sc_h;
+  dbuf pos, species_sc;
+  // neighbour list
+  double rcut_max = 0.0;
+  int64_t n_pairs = 0;
+  dbuf n_neigh, nl_off, nlist, rjs, xyz, thetas, phis, nspec, rev;
+  dbuf cell_of, cell_cnt, cell_start, cell_atoms, scan_tmp, do_list;
+  bool have_list = false;
+  // models
+  std::vector<tgx_model_t*> models;
+  int current = -1;
+  // results
+  dbuf energies, forces, fglob, virial_part, scratch, red;
+  // md
+  dbuf vel, mass, forces_prev, pos_prev;
+  double dt_prev = 0.0;
+  // surrogate gate
+  int gate_members = 1;
+  double gate_threshold = 1e30;
+  double gated_fraction = 0.0, max_spread = 0.0;
+  dbuf e_members, gate_mask;
+  // profiling
+  bool profiling = false;
+  std::vector<prof_rec> prof;
+  std::vector<hipEvent_t> ev_pool;
+};
+
+// Profiled launch: records events around the launch when profiling is on.
+int tgx_prof_begin(tgx_ctx* ctx, const char* name, int* slot);
+int tgx_prof_end(tgx_ctx* ctx, int slot);
+
+#define TGX_LAUNCH(ctx, name, kernel, grid, block, shmem, ...)                       \
+  do {                                                                              \
+    int slot_ = -1;                                                                 \
+    if ((ctx)->profiling) TGX_TRY(tgx_prof_begin((ctx), (name), &slot_));           \
+    hipLaunchKernelGGL(kernel, grid, block, shmem, (ctx)->stream, __VA_ARGS__);     \
+    hipError_t le_ = hipGetLastError();                                             \
+    if (le_ != hipSuccess) {                                                        \
+      tgx_set_error("launch of %s failed: %s", (name), hipGetErrorString(le_));     \
+      return TGX_ERR_HIP;                                                           \
+    }                                                                               \
+    if (slot_ >= 0) TGX_TRY(tgx_prof_end((ctx), slot_));                            \
+  } while (0)
+
+// Host-side helpers shared between units (tgx_host.cpp).
+int tgx_build_radial_tables(tgx_model_t* m, std::vector<double>& B, std::vector<double>& xq,
+                            std::vector<double>& cen);
+void tgx_gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w);
+
+// Kernels' host entry points (one per translation unit).
+int tgx_neighbors_run(tgx_ctx* ctx, double rcut_max, const int* indices, const int* do_list);
+int tgx_block_select(tgx_ctx* ctx, tgx_model_t* m);
+int tgx_soap_descriptors(tgx_ctx* ctx, tgx_model_t* m, int path, bool with_der);
+int tgx_gap_contract(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces);
+int tgx_plain_pair_forces(tgx_ctx* ctx, tgx_model_t* m);
+int tgx_adjoint_pair_forces(tgx_ctx* ctx, tgx_model_t* m, int path);
+int tgx_gather_forces(tgx_ctx* ctx, double* energy_out, double* virial_out);
+int tgx_zero(tgx_ctx* ctx, void* p, size_t bytes);
+int tgx_surrogate_energy_grad(tgx_ctx* ctx, tgx_model_t* m);
+int tgx_device_sum(tgx_ctx* ctx, const double* d_x, int64_t n, double* out_host);

@@ -1,0 +1,173 @@
+// Molecular-dynamics step on the device, with the semantics of
+// src/md.f90:226-345 (velocity_verlet, berendsen_thermostat, remove_cm_vel)
+// and the temperature of src/turbogap_md.f90:281-295.
+// Units: Å, fs, eV; masses in eV fs^2 / Å^2 (amu * 103.6426965268).
+#include "tgx_internal.h"
+#include "tgx_util.h"
+
+namespace {
+
+// velocities at t-dt -> t (not on the first step), then positions t -> t+dt.
+__global__ void k_vv(int n, double dt, double dt_prev, int first, const double* __restrict__ F,
+                     double* __restrict__ Fprev, const double* __restrict__ mass, double* __restrict__ v,
+                     double* __restrict__ x, double* __restrict__ xprev) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double m = mass[i];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const size_t j = 3 * (size_t)i + k;
+    double vj = v[j];
+    if (!first) vj = vj + 0.5 * (F[j] + Fprev[j]) / m * dt_prev;
+    v[j] = vj;
+    xprev[j] = x[j];
+    Fprev[j] = F[j];
+    x[j] = x[j] + vj * dt + 0.5 * F[j] / m * dt * dt;
+  }
+}
+
+__global__ void k_scale(int n3, double f, double* __restrict__ v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n3) v[i] *= f;
+}
+
+// 0.5 m v^2 per atom and m v per atom (for the centre of mass)
+__global__ void k_ke_terms(int n, const double* __restrict__ mass, const double* __restrict__ v, double* __restrict__ ke,
+                           double* __restrict__ mv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double m = mass[i];
+  const double a = v[3 * (size_t)i], b = v[3 * (size_t)i + 1], c = v[3 * (size_t)i + 2];
+  ke[i] = 0.5 * m * (a * a + b * b + c * c);
+  mv[i] = m * a;
+  mv[n + i] = m * b;
+  mv[2 * (size_t)n + i] = m * c;
+}
+
+__global__ void k_sub_cm(int n, double cx, double cy, double cz, double* __restrict__ v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  v[3 * (size_t)i] -= cx;
+  v[3 * (size_t)i + 1] -= cy;
+  v[3 * (size_t)i + 2] -= cz;
+}
+
+__global__ void k_wrap(int n, double Lx, double Ly, double Lz, double* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double L[3] = {Lx, Ly, Lz};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double p = x[3 * (size_t)i + k];
+    p = p - floor(p / L[k]) * L[k];
+    if (p >= L[k]) p -= L[k];
+    x[3 * (size_t)i + k] = p;
+  }
+}
+
+}  // namespace
+
+#define MD_CTX(ctx)                                                                \
+  do {                                                                             \
+    if (!(ctx)) { tgx_set_error("%s: null context", __func__); return TGX_ERR_ARG; } \
+    TGX_CHECK_HIP(hipSetDevice((ctx)->device));                                    \
+    TGX_REQUIRE((ctx)->n_sites > 0 && (ctx)->n_sites == (ctx)->n_sites_sc,          \
+                "MD needs a system without supercell replicas");                   \
+  } while (0)
+
+extern "C" int tgx_md_set_state(tgx_ctx* ctx, const double* velocities, const double* masses) {
+  MD_CTX(ctx);
+  const size_t n = (size_t)ctx->n_sites;
+  TGX_REQUIRE(velocities && masses, "tgx_md_set_state: null input");
+  TGX_TRY(ctx->vel.ensure(sizeof(double) * 3 * n));
+  TGX_TRY(ctx->mass.ensure(sizeof(double) * n));
+  TGX_TRY(ctx->forces_prev.ensure(sizeof(double) * 3 * n));
+  TGX_TRY(ctx->pos_prev.ensure(sizeof(double) * 3 * n));
+  TGX_CHECK_HIP(hipMemcpyAsync(ctx->vel.p, velocities, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+  TGX_CHECK_HIP(hipMemcpyAsync(ctx->mass.p, masses, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+  TGX_TRY(tgx_zero(ctx, ctx->forces_prev.p, sizeof(double) * 3 * n));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->dt_prev = 0.0;
+  return TGX_OK;
+}
+
+extern "C" int tgx_md_get_state(tgx_ctx* ctx, double* positions, double* velocities) {
+  MD_CTX(ctx);
+  const size_t n = (size_t)ctx->n_sites;
+  if (positions)
+    TGX_CHECK_HIP(hipMemcpyAsync(positions, ctx->pos.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
+  if (velocities) {
+    TGX_REQUIRE(ctx->vel.p, "tgx_md_get_state: no velocities set");
+    TGX_CHECK_HIP(hipMemcpyAsync(velocities, ctx->vel.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return TGX_OK;
+}
+
+extern "C" int tgx_velocity_verlet(tgx_ctx* ctx, double dt, int first_step) {
+  MD_CTX(ctx);
+  TGX_REQUIRE(ctx->vel.p && ctx->mass.p && ctx->forces.p, "tgx_velocity_verlet: state or forces missing");
+  const int n = ctx->n_sites;
+  TGX_LAUNCH(ctx, "md_verlet", k_vv, dim3((n + 255) / 256), dim3(256), 0, n, dt, ctx->dt_prev, first_step ? 1 : 0,
+             ctx->forces.as<double>(), ctx->forces_prev.as<double>(), ctx->mass.as<double>(), ctx->vel.as<double>(),
+             ctx->pos.as<double>(), ctx->pos_prev.as<double>());
+  ctx->dt_prev = dt;
+  return TGX_OK;
+}
+
+static int ke_and_cm(tgx_ctx* ctx, double* ekin, double cm[3], double* mtot) {
+  const int n = ctx->n_sites;
+  TGX_TRY(ctx->scratch.ensure(sizeof(double) * 5 * (size_t)n));
+  double* ke = ctx->scratch.as<double>();
+  double* mv = ke + n;
+  TGX_LAUNCH(ctx, "md_ke", k_ke_terms, dim3((n + 255) / 256), dim3(256), 0, n, ctx->mass.as<double>(),
+             ctx->vel.as<double>(), ke, mv);
+  TGX_TRY(tgx_device_sum(ctx, ke, n, ekin));
+  if (cm) {
+    for (int k = 0; k < 3; ++k) TGX_TRY(tgx_device_sum(ctx, mv + (size_t)k * n, n, &cm[k]));
+    TGX_TRY(tgx_device_sum(ctx, ctx->mass.as<double>(), n, mtot));
+  }
+  return TGX_OK;
+}
+
+extern "C" int tgx_kinetic(tgx_ctx* ctx, double* e_kin, double* temperature) {
+  MD_CTX(ctx);
+  TGX_REQUIRE(ctx->vel.p, "tgx_kinetic: no velocities");
+  double ek = 0.0;
+  TGX_TRY(ke_and_cm(ctx, &ek, nullptr, nullptr));
+  const double kB = 8.6173303e-5;
+  if (e_kin) *e_kin = ek;
+  if (temperature) *temperature = ctx->n_sites > 1 ? 2.0 / 3.0 / (double)(ctx->n_sites - 1) / kB * ek : 0.0;
+  return TGX_OK;
+}
+
+extern "C" int tgx_berendsen(tgx_ctx* ctx, double T0, double T, double tau, double dt) {
+  MD_CTX(ctx);
+  TGX_REQUIRE(ctx->vel.p && tau > 0.0, "tgx_berendsen: no velocities or tau <= 0");
+  if (!(T > 0.0)) return TGX_OK;
+  const double f = std::sqrt(1.0 + dt / tau * (T0 / T - 1.0));
+  const int n3 = 3 * ctx->n_sites;
+  TGX_LAUNCH(ctx, "md_berendsen", k_scale, dim3((n3 + 255) / 256), dim3(256), 0, n3, f, ctx->vel.as<double>());
+  return TGX_OK;
+}
+
+extern "C" int tgx_remove_cm_vel(tgx_ctx* ctx) {
+  MD_CTX(ctx);
+  TGX_REQUIRE(ctx->vel.p, "tgx_remove_cm_vel: no velocities");
+  double ek, cm[3], mt;
+  TGX_TRY(ke_and_cm(ctx, &ek, cm, &mt));
+  const int n = ctx->n_sites;
+  TGX_LAUNCH(ctx, "md_cm", k_sub_cm, dim3((n + 255) / 256), dim3(256), 0, n, cm[0] / mt, cm[1] / mt, cm[2] / mt,
+             ctx->vel.as<double>());
+  return TGX_OK;
+}
+
+extern "C" int tgx_wrap_pbc(tgx_ctx* ctx) {
+  MD_CTX(ctx);
+  const double* b = ctx->box;
+  TGX_REQUIRE(b[1] == 0 && b[2] == 0 && b[3] == 0 && b[5] == 0 && b[6] == 0 && b[7] == 0,
+              "tgx_wrap_pbc: orthorhombic boxes only");
+  const int n = ctx->n_sites;
+  TGX_LAUNCH(ctx, "md_wrap", k_wrap, dim3((n + 255) / 256), dim3(256), 0, n, b[0], b[4], b[8], ctx->pos.as<double>());
+  return TGX_OK;
+}

@@ -1,0 +1,438 @@
+// Neighbour list on the device: the orthorhombic cell-list build and the
+// brute-force build of src/neighbors.f90:243-605, in the reference's order.
+//
+// Compiled with -ffp-contract=off: rjs and xyz must come from the same IEEE
+// operation sequence as the reference's get_distance (src/neighbors.f90:90-101),
+// so no multiply-add may be fused.
+//
+// Layout (device): positions (3, Nsc) as in the reference; cell_atoms holds each
+// cell's atoms in descending index order (the order the reference's head/next
+// linked list walks, src/neighbors.f90:398-408,445-466); the list is CSR with
+// int64 offsets.
+//
+// One wavefront per central atom: lanes test 64 candidates at a time in the
+// reference's visiting order and compact the hits with a ballot, so the output
+// order is the reference's without any per-thread serial loop.
+#include "tgx_internal.h"
+#include "tgx_util.h"
+
+namespace {
+
+struct BoxInfo {
+  double a[3], b[3], c[3];
+  double L[3];
+  double mat_inv[9];  // triclinic branch only, row-major
+  int ortho;
+};
+
+__device__ __forceinline__ double f_modulo(double a, double p) {
+  double r = fmod(a, p);
+  if (r != 0.0) {
+    if ((a < 0.0) != (p < 0.0)) r = r + p;
+  } else {
+    r = copysign(0.0, p);
+  }
+  return r;
+}
+
+__device__ __forceinline__ int i_modulo(int a, int p) {
+  int r = a % p;
+  if (r != 0 && ((r < 0) != (p < 0))) r += p;
+  return r;
+}
+
+__device__ __forceinline__ double dot3(const double* u, const double* v) {
+  double s = u[0] * v[0];
+  s = s + u[1] * v[1];
+  s = s + u[2] * v[2];
+  return s;
+}
+
+// src/neighbors.f90:47-163 with PBC on in every direction.
+__device__ void get_distance(const double* pi, const double* pj, const BoxInfo& bx, double* dist, double& d) {
+  if (bx.ortho) {
+    double d2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double dk = f_modulo(pj[k] - pi[k], bx.L[k]);
+      if (dk > bx.L[k] / 2.0) dk = dk - bx.L[k];
+      dist[k] = dk;
+      d2 = d2 + dk * dk;
+    }
+    d = sqrt(d2);
+    return;
+  }
+  double dd[3] = {pj[0] - pi[0], pj[1] - pi[1], pj[2] - pi[2]};
+  double v[3] = {-1.0 * dot3(dd, bx.a), -1.0 * dot3(dd, bx.b), -1.0 * dot3(dd, bx.c)};
+  int ind[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double s = bx.mat_inv[3 * i] * v[0];
+    s = s + bx.mat_inv[3 * i + 1] * v[1];
+    s = s + bx.mat_inv[3 * i + 2] * v[2];
+    ind[i] = (int)llround(s);
+  }
+  double res_opt = 1.0e10;
+  double best[3] = {dd[0], dd[1], dd[2]};
+  for (int i = ind[0] - 1; i <= ind[0] + 1; ++i)
+    for (int j = ind[1] - 1; j <= ind[1] + 1; ++j)
+      for (int k = ind[2] - 1; k <= ind[2] + 1; ++k) {
+        double t[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) t[q] = dd[q] + (double)i * bx.a[q] + (double)j * bx.b[q] + (double)k * bx.c[q];
+        double res = dot3(t, t);
+        if (res < res_opt) {
+          res_opt = res;
+          best[0] = t[0]; best[1] = t[1]; best[2] = t[2];
+        }
+      }
+  dist[0] = best[0]; dist[1] = best[1]; dist[2] = best[2];
+  d = sqrt(dot3(best, best));
+}
+
+__device__ __forceinline__ void angles(const double* dist, double d, double& th, double& ph) {
+  if (fabs(dist[2]) >= d) th = dist[2] > 0.0 ? 0.0 : acos(-1.0);
+  else th = acos(dist[2] / d);
+  ph = atan2(dist[1], dist[0]);
+}
+
+// Cell of every unit-cell atom (src/neighbors.f90:398-408) and its 0-based
+// cell coordinates (the i2-1, j2-1, k2-1 of :441-443).
+__global__ void k_bin(int n, const double* __restrict__ pos, BoxInfo bx, int mx, int my, int mz,
+                      int* __restrict__ cell_of, int* __restrict__ cell_xyz, int* __restrict__ cell_cnt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double tol = 1.0e-10;
+  const double ctr[3] = {bx.a[0] / 2.0, bx.b[1] / 2.0, bx.c[2] / 2.0};
+  double p[3] = {pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2]};
+  double dist[3], d;
+  get_distance(ctr, p, bx, dist, d);
+  for (int k = 0; k < 3; ++k) dist[k] = dist[k] + ctr[k];
+  int cx = (int)(dist[0] / (bx.a[0] + tol) * (double)mx);
+  int cy = (int)(dist[1] / (bx.b[1] + tol) * (double)my);
+  int cz = (int)(dist[2] / (bx.c[2] + tol) * (double)mz);
+  int cell = i_modulo(cx, mx) + i_modulo(cy, my) * mx + i_modulo(cz, mz) * my * mx;
+  cell_of[i] = cell;
+  cell_xyz[3 * i] = cx;
+  cell_xyz[3 * i + 1] = cy;
+  cell_xyz[3 * i + 2] = cz;
+  atomicAdd(&cell_cnt[cell], 1);
+}
+
+__global__ void k_place(int n, const int* __restrict__ cell_of, const int64_t* __restrict__ cell_start,
+                        int* __restrict__ fill, int* __restrict__ cell_atoms) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c = cell_of[i];
+  int slot = atomicAdd(&fill[c], 1);
+  cell_atoms[cell_start[c] + slot] = i;
+}
+
+// Descending atom index within each cell: the walk order of the linked list.
+__global__ void k_sort_cells(int n_cells, const int64_t* __restrict__ cell_start, const int* __restrict__ cell_cnt,
+                             int* __restrict__ cell_atoms) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cells) return;
+  int* a = cell_atoms + cell_start[c];
+  int m = cell_cnt[c];
+  for (int i = 1; i < m; ++i) {
+    int v = a[i], j = i - 1;
+    while (j >= 0 && a[j] < v) { a[j + 1] = a[j]; --j; }
+    a[j + 1] = v;
+  }
+}
+
+// Cell-list traversal in the reference's order for one central atom, one
+// wavefront per atom. mode 0 counts, mode 1 fills.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_cell_neigh(int n, const double* __restrict__ pos, BoxInfo bx, int mx, int my,
+                                                  int mz, double rcut, const int* __restrict__ do_list,
+                                                  const int* __restrict__ cell_xyz,
+                                                  const int64_t* __restrict__ cell_start,
+                                                  const int* __restrict__ cell_cnt,
+                                                  const int* __restrict__ cell_atoms,
+                                                  const int* __restrict__ species, int* __restrict__ n_neigh,
+                                                  const int64_t* __restrict__ off, int* __restrict__ nlist,
+                                                  double* __restrict__ rjs, double* __restrict__ xyz,
+                                                  double* __restrict__ thetas, double* __restrict__ phis,
+                                                  int* __restrict__ nspec) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (i >= n) return;
+  if (do_list && !do_list[i]) {
+    if (MODE == 0 && lane == 0) n_neigh[i] = 0;
+    return;
+  }
+  const double pi[3] = {pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2]};
+  const int i2 = 1 + cell_xyz[3 * i], j2 = 1 + cell_xyz[3 * i + 1], k2 = 1 + cell_xyz[3 * i + 2];
+  // visited cells in order (src/neighbors.f90:429-446)
+  int vcell[27], vcnt[27];
+  int nv = 0, total = 0;
+  for (int k3 = k2 - 1; k3 <= k2 + 1; ++k3) {
+    if (mz == 1 && k3 != 1) continue;
+    if (mz == 2 && k2 == 1 && k3 == 0) continue;
+    if (mz == 2 && k2 == 2 && k3 == 3) continue;
+    for (int j3 = j2 - 1; j3 <= j2 + 1; ++j3) {
+      if (my == 1 && j3 != 1) continue;
+      if (my == 2 && j2 == 1 && j3 == 0) continue;
+      if (my == 2 && j2 == 2 && j3 == 3) continue;
+      for (int i3 = i2 - 1; i3 <= i2 + 1; ++i3) {
+        if (mx == 1 && i3 != 1) continue;
+        if (mx == 2 && i2 == 1 && i3 == 0) continue;
+        if (mx == 2 && i2 == 2 && i3 == 3) continue;
+        int c = i_modulo(i3 - 1, mx) + i_modulo(j3 - 1, my) * mx + i_modulo(k3 - 1, mz) * mx * my;
+        vcell[nv] = c;
+        vcnt[nv] = cell_cnt[c];
+        total += vcnt[nv];
+        ++nv;
+      }
+    }
+  }
+  int64_t out = 0;
+  if (MODE == 1) {
+    out = off[i];
+    if (lane == 0) {
+      nlist[out] = i + 1;
+      rjs[out] = 0.0;
+      xyz[3 * out] = 0.0; xyz[3 * out + 1] = 0.0; xyz[3 * out + 2] = 0.0;
+      thetas[out] = 0.0;
+      phis[out] = 0.0;
+      nspec[out] = species[i];
+    }
+    out += 1;
+  }
+  int hits = 0;
+  for (int base = 0; base < total; base += 64) {
+    const int cidx = base + lane;
+    bool hit = false;
+    int k = -1;
+    double dist[3] = {0, 0, 0}, d = 0.0;
+    if (cidx < total) {
+      int acc = 0, v = 0;
+      while (acc + vcnt[v] <= cidx) { acc += vcnt[v]; ++v; }
+      k = cell_atoms[cell_start[vcell[v]] + (cidx - acc)];
+      if (k != i) {
+        const double pk[3] = {pos[3 * (size_t)k], pos[3 * (size_t)k + 1], pos[3 * (size_t)k + 2]};
+        get_distance(pi, pk, bx, dist, d);
+        hit = d < rcut;
+      }
+    }
+    const uint64_t mask = __ballot(hit);
+    if (MODE == 1 && hit) {
+      const int64_t p = out + hits + __popcll(mask & ((1ull << lane) - 1ull));
+      nlist[p] = k + 1;
+      rjs[p] = d;
+      xyz[3 * p] = dist[0]; xyz[3 * p + 1] = dist[1]; xyz[3 * p + 2] = dist[2];
+      double th, ph;
+      angles(dist, d, th, ph);
+      thetas[p] = th;
+      phis[p] = ph;
+      nspec[p] = species[k];
+    }
+    hits += __popcll(mask);
+  }
+  if (MODE == 0 && lane == 0) n_neigh[i] = 1 + hits;
+}
+
+// Brute-force build over the supercell (src/neighbors.f90:473-510): small boxes
+// and triclinic cells.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_brute_neigh(int n, int n_sc, const double* __restrict__ pos, BoxInfo bx,
+                                                   double rcut, const int* __restrict__ do_list,
+                                                   const int* __restrict__ species, int* __restrict__ n_neigh,
+                                                   const int64_t* __restrict__ off, int* __restrict__ nlist,
+                                                   double* __restrict__ rjs, double* __restrict__ xyz,
+                                                   double* __restrict__ thetas, double* __restrict__ phis,
+                                                   int* __restrict__ nspec) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (i >= n) return;
+  if (do_list && !do_list[i]) {
+    if (MODE == 0 && lane == 0) n_neigh[i] = 0;
+    return;
+  }
+  const double pi[3] = {pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2]};
+  int64_t out = 0;
+  if (MODE == 1) {
+    out = off[i];
+    if (lane == 0) {
+      nlist[out] = i + 1;
+      rjs[out] = 0.0;
+      xyz[3 * out] = 0.0; xyz[3 * out + 1] = 0.0; xyz[3 * out + 2] = 0.0;
+      thetas[out] = 0.0;
+      phis[out] = 0.0;
+      nspec[out] = species[i];
+    }
+    out += 1;
+  }
+  int hits = 0;
+  for (int base = 0; base < n_sc; base += 64) {
+    const int j = base + lane;
+    bool hit = false;
+    double dist[3] = {0, 0, 0}, d = 0.0;
+    if (j < n_sc && j != i) {
+      const double pj[3] = {pos[3 * (size_t)j], pos[3 * (size_t)j + 1], pos[3 * (size_t)j + 2]};
+      get_distance(pi, pj, bx, dist, d);
+      hit = d < rcut;
+    }
+    const uint64_t mask = __ballot(hit);
+    if (MODE == 1 && hit) {
+      const int64_t p = out + hits + __popcll(mask & ((1ull << lane) - 1ull));
+      nlist[p] = j + 1;
+      rjs[p] = d;
+      xyz[3 * p] = dist[0]; xyz[3 * p + 1] = dist[1]; xyz[3 * p + 2] = dist[2];
+      double th, ph;
+      angles(dist, d, th, ph);
+      thetas[p] = th;
+      phis[p] = ph;
+      nspec[p] = species[j];
+    }
+    hits += __popcll(mask);
+  }
+  if (MODE == 0 && lane == 0) n_neigh[i] = 1 + hits;
+}
+
+// For pair p = (i -> k), the index of the pair in j = fold(k)'s list that points
+// back at i with the opposite displacement; the self pair maps to itself. -1
+// when j has no list. Used by the deterministic force gather.
+__global__ void k_reverse(int n, const int* __restrict__ n_neigh, const int64_t* __restrict__ off,
+                          const int* __restrict__ nlist, const double* __restrict__ xyz, int64_t* __restrict__ rev) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (i >= n) return;
+  const int64_t b = off[i];
+  const int nn = n_neigh[i];
+  for (int t = lane; t < nn; t += 64) {
+    const int64_t p = b + t;
+    if (t == 0) { rev[p] = p; continue; }
+    const int j = (nlist[p] - 1) % n;
+    const int64_t bj = off[j];
+    const int nj = n_neigh[j];
+    int64_t best = -1;
+    double bd = 0.0;
+    for (int u = 1; u < nj; ++u) {
+      const int64_t q = bj + u;
+      if ((nlist[q] - 1) % n != i) continue;
+      double e0 = xyz[3 * q] + xyz[3 * p], e1 = xyz[3 * q + 1] + xyz[3 * p + 1], e2 = xyz[3 * q + 2] + xyz[3 * p + 2];
+      double dd = e0 * e0 + e1 * e1 + e2 * e2;
+      if (best < 0 || dd < bd) { best = q; bd = dd; }
+    }
+    rev[p] = best;
+  }
+}
+
+}  // namespace
+
+static void host_box_info(const double* box, BoxInfo& bx) {
+  for (int k = 0; k < 3; ++k) { bx.a[k] = box[k]; bx.b[k] = box[3 + k]; bx.c[k] = box[6 + k]; }
+  const double tol = 1.0e-6;
+  bx.ortho = fabs(bx.a[1]) < tol && fabs(bx.a[2]) < tol && fabs(bx.b[0]) < tol && fabs(bx.b[2]) < tol &&
+             fabs(bx.c[0]) < tol && fabs(bx.c[1]) < tol;
+  bx.L[0] = bx.a[0]; bx.L[1] = bx.b[1]; bx.L[2] = bx.c[2];
+  auto dot = [](const double* u, const double* v) {
+    double s = u[0] * v[0];
+    s = s + u[1] * v[1];
+    s = s + u[2] * v[2];
+    return s;
+  };
+  double m11 = dot(bx.a, bx.a), m12 = dot(bx.a, bx.b), m13 = dot(bx.a, bx.c);
+  double m22 = dot(bx.b, bx.b), m23 = dot(bx.b, bx.c), m33 = dot(bx.c, bx.c);
+  double md = -(m13 * m13) * m22 + 2.0 * m12 * m13 * m23 - m11 * (m23 * m23) - (m12 * m12) * m33 + m11 * m22 * m33;
+  double* mi = bx.mat_inv;
+  mi[0] = m22 * m33 - m23 * m23;
+  mi[1] = m13 * m23 - m12 * m33;
+  mi[2] = m12 * m23 - m13 * m22;
+  mi[3] = mi[1];
+  mi[4] = m11 * m33 - m13 * m13;
+  mi[5] = m12 * m13 - m11 * m23;
+  mi[6] = mi[2];
+  mi[7] = mi[5];
+  mi[8] = m11 * m22 - m12 * m12;
+  for (int k = 0; k < 9; ++k) mi[k] = mi[k] / md;
+}
+
+int tgx_neighbors_run(tgx_ctx* ctx, double rcut_max, const int* indices, const int* do_list_h) {
+  const int n = ctx->n_sites, n_sc = ctx->n_sites_sc;
+  TGX_REQUIRE(n > 0, "build_neighbors_list: no system set");
+  TGX_REQUIRE(rcut_max > 0.0, "build_neighbors_list: rcut_max must be positive");
+  // src/neighbors.f90:319-330: a box whose off-diagonal entries are below 1e-6 is
+  // treated as orthorhombic and those entries are zeroed.
+  BoxInfo bx;
+  host_box_info(ctx->box, bx);
+  if (bx.ortho) {
+    ctx->box[1] = ctx->box[2] = ctx->box[3] = ctx->box[5] = ctx->box[6] = ctx->box[7] = 0.0;
+    host_box_info(ctx->box, bx);
+  }
+  const bool small = indices && (indices[0] > 1 || indices[1] > 1 || indices[2] > 1);
+  const int* d_do = nullptr;
+  if (do_list_h) {
+    TGX_TRY(ctx->do_list.ensure(sizeof(int) * n));
+    TGX_CHECK_HIP(hipMemcpyAsync(ctx->do_list.p, do_list_h, sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+    d_do = ctx->do_list.as<int>();
+  }
+  TGX_TRY(ctx->n_neigh.ensure(sizeof(int) * n));
+  TGX_TRY(ctx->nl_off.ensure(sizeof(int64_t) * (n + 1)));
+  const dim3 wblk(256);
+  const dim3 wgrid((n + 3) / 4);
+  int mx = 0, my = 0, mz = 0;
+  const bool cells = bx.ortho && !small;
+  if (cells) {
+    mx = (int)(bx.a[0] / rcut_max);
+    my = (int)(bx.b[1] / rcut_max);
+    mz = (int)(bx.c[2] / rcut_max);
+    TGX_REQUIRE(mx >= 1 && my >= 1 && mz >= 1, "build_neighbors_list: box shorter than rcut_max needs a supercell");
+    const int64_t nc = (int64_t)mx * my * mz;
+    TGX_TRY(ctx->cell_of.ensure(sizeof(int) * 4 * (size_t)n));
+    TGX_TRY(ctx->cell_cnt.ensure(sizeof(int) * 2 * (size_t)nc));
+    TGX_TRY(ctx->cell_start.ensure(sizeof(int64_t) * (size_t)(nc + 1)));
+    TGX_TRY(ctx->cell_atoms.ensure(sizeof(int) * (size_t)n));
+    int* cell_of = ctx->cell_of.as<int>();
+    int* cell_xyz = cell_of + n;
+    int* cnt = ctx->cell_cnt.as<int>();
+    int* fill = cnt + nc;
+    TGX_TRY(tgx_zero(ctx, cnt, sizeof(int) * 2 * (size_t)nc));
+    TGX_LAUNCH(ctx, "nbr_bin", k_bin, dim3((n + 255) / 256), dim3(256), 0, n, ctx->pos.as<double>(), bx, mx, my, mz,
+               cell_of, cell_xyz, cnt);
+    TGX_TRY(tgx_scan_i32(ctx, cnt, ctx->cell_start.as<int64_t>(), nc, nullptr));
+    TGX_LAUNCH(ctx, "nbr_place", k_place, dim3((n + 255) / 256), dim3(256), 0, n, cell_of,
+               ctx->cell_start.as<int64_t>(), fill, ctx->cell_atoms.as<int>());
+    TGX_LAUNCH(ctx, "nbr_sort", k_sort_cells, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, (int)nc,
+               ctx->cell_start.as<int64_t>(), cnt, ctx->cell_atoms.as<int>());
+    TGX_LAUNCH(ctx, "nbr_count", k_cell_neigh<0>, wgrid, wblk, 0, n, ctx->pos.as<double>(), bx, mx, my, mz, rcut_max,
+               d_do, cell_xyz, ctx->cell_start.as<int64_t>(), cnt, ctx->cell_atoms.as<int>(), ctx->species_sc.as<int>(),
+               ctx->n_neigh.as<int>(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+  } else {
+    TGX_LAUNCH(ctx, "nbr_count_brute", k_brute_neigh<0>, wgrid, wblk, 0, n, n_sc, ctx->pos.as<double>(), bx, rcut_max,
+               d_do, ctx->species_sc.as<int>(), ctx->n_neigh.as<int>(), nullptr, nullptr, nullptr, nullptr, nullptr,
+               nullptr, nullptr);
+  }
+  int64_t P = 0;
+  TGX_TRY(tgx_scan_i32(ctx, ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(), n, &P));
+  ctx->n_pairs = P;
+  const size_t Pz = (size_t)(P > 0 ? P : 1);
+  TGX_TRY(ctx->nlist.ensure(sizeof(int) * Pz));
+  TGX_TRY(ctx->rjs.ensure(sizeof(double) * Pz));
+  TGX_TRY(ctx->xyz.ensure(sizeof(double) * 3 * Pz));
+  TGX_TRY(ctx->thetas.ensure(sizeof(double) * Pz));
+  TGX_TRY(ctx->phis.ensure(sizeof(double) * Pz));
+  TGX_TRY(ctx->nspec.ensure(sizeof(int) * Pz));
+  TGX_TRY(ctx->rev.ensure(sizeof(int64_t) * Pz));
+  if (cells) {
+    int* cell_xyz = ctx->cell_of.as<int>() + n;
+    TGX_LAUNCH(ctx, "nbr_fill", k_cell_neigh<1>, wgrid, wblk, 0, n, ctx->pos.as<double>(), bx, mx, my, mz, rcut_max,
+               d_do, cell_xyz, ctx->cell_start.as<int64_t>(), ctx->cell_cnt.as<int>(), ctx->cell_atoms.as<int>(),
+               ctx->species_sc.as<int>(), ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(), ctx->nlist.as<int>(),
+               ctx->rjs.as<double>(), ctx->xyz.as<double>(), ctx->thetas.as<double>(), ctx->phis.as<double>(),
+               ctx->nspec.as<int>());
+  } else {
+    TGX_LAUNCH(ctx, "nbr_fill_brute", k_brute_neigh<1>, wgrid, wblk, 0, n, n_sc, ctx->pos.as<double>(), bx, rcut_max,
+               d_do, ctx->species_sc.as<int>(), ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(),
+               ctx->nlist.as<int>(), ctx->rjs.as<double>(), ctx->xyz.as<double>(), ctx->thetas.as<double>(),
+               ctx->phis.as<double>(), ctx->nspec.as<int>());
+  }
+  TGX_LAUNCH(ctx, "nbr_reverse", k_reverse, wgrid, wblk, 0, n, ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(),
+             ctx->nlist.as<int>(), ctx->xyz.as<double>(), ctx->rev.as<int64_t>());
+  ctx->rcut_max = rcut_max;
+  ctx->have_list = true;
+  return TGX_OK;
+}

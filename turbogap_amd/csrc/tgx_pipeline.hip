@@ -1,0 +1,244 @@
+// Per-block marshal (src/gap_interface.f90:204-309), per-atom force gather and
+// virial/energy reductions (src/gap.f90:206-224, src/turbogap.f90:1352-1373,
+// 1941-1954).
+//
+// Forces are gathered, not scattered with atomics: every pair force lives at its
+// global pair index, and atom j sums the entries of the pairs that point at it
+// by walking its own list through the reverse-pair index, in list order. The
+// result is the same bits on every run.
+#include "tgx_internal.h"
+#include "tgx_util.h"
+
+namespace {
+
+struct SpeciesMap {
+  int n;
+  int code[TGX_MAX_SPECIES];
+  __device__ int index(int c) const {
+    for (int s = 0; s < n; ++s)
+      if (code[s] == c) return s;
+    return -1;
+  }
+};
+
+// A site is a centre of the block when its species is the block's central
+// species (or the block takes every atom); a pair is kept when it lies inside
+// the block's largest cutoff and the block sees the neighbour's species.
+__global__ void k_select_count(int n, SpeciesMap sm, int central, double rcut, const int* __restrict__ n_neigh,
+                               const int64_t* __restrict__ off, const int* __restrict__ species,
+                               const double* __restrict__ rjs, const int* __restrict__ nspec, int* __restrict__ flag,
+                               int* __restrict__ npairs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int sc = sm.index(species[i]);
+  const bool sel = n_neigh[i] > 0 && sc >= 0 && (central == 0 || sc == central - 1);
+  int cnt = 0;
+  if (sel) {
+    const int64_t b = off[i];
+    for (int k = 0; k < n_neigh[i]; ++k)
+      if (rjs[b + k] < rcut && sm.index(nspec[b + k]) >= 0) ++cnt;
+  }
+  flag[i] = sel ? 1 : 0;
+  npairs[i] = cnt;
+}
+
+__global__ __launch_bounds__(256) void k_select_fill(int n, SpeciesMap sm, double rcut, const int* __restrict__ flag,
+                                                   const int64_t* __restrict__ site_idx,
+                                                   const int64_t* __restrict__ pair_off,
+                                                   const int* __restrict__ n_neigh, const int64_t* __restrict__ off,
+                                                   const int* __restrict__ nlist, const double* __restrict__ rjs,
+                                                   const double* __restrict__ xyz, const int* __restrict__ nspec,
+                                                   int* __restrict__ site_atom, int64_t* __restrict__ site_beg,
+                                                   int64_t* __restrict__ pr_gidx, int* __restrict__ pr_nbr,
+                                                   int* __restrict__ pr_spec, double* __restrict__ pr_r,
+                                                   double* __restrict__ pr_xyz, int* __restrict__ seen) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n || !flag[i]) return;
+  const int64_t s = site_idx[i];
+  const int64_t o = pair_off[i];
+  if (lane == 0) { site_atom[s] = i; site_beg[s] = o; }
+  const int64_t b = off[i];
+  const int nn = n_neigh[i];
+  int kept = 0;
+  for (int base = 0; base < nn; base += 64) {
+    const int k = base + lane;
+    bool keep = false;
+    int sp = -1;
+    if (k < nn) {
+      sp = sm.index(nspec[b + k]);
+      keep = rjs[b + k] < rcut && sp >= 0;
+    }
+    const uint64_t mask = __ballot(keep);
+    if (keep) {
+      const int64_t p = o + kept + __popcll(mask & ((1ull << lane) - 1ull));
+      const int64_t g = b + k;
+      const int j = (nlist[g] - 1) % n;
+      pr_gidx[p] = g;
+      pr_nbr[p] = j;
+      pr_spec[p] = sp;
+      pr_r[p] = rjs[g];
+      pr_xyz[3 * p] = xyz[3 * g];
+      pr_xyz[3 * p + 1] = xyz[3 * g + 1];
+      pr_xyz[3 * p + 2] = xyz[3 * g + 2];
+      if (seen) seen[j] = 1;
+    }
+    kept += __popcll(mask);
+  }
+}
+
+__global__ void k_set_last(int64_t* __restrict__ site_beg, int ns, const int64_t* __restrict__ pair_off, int n) {
+  site_beg[ns] = pair_off[n];
+}
+
+// F_j = sum over j's list entries q of fglob[rev[q]], in list order.
+__global__ void k_gather(int n, const int* __restrict__ n_neigh, const int64_t* __restrict__ off,
+                         const int64_t* __restrict__ rev, const double* __restrict__ fglob, double* __restrict__ F,
+                         int* __restrict__ bad) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double f0 = 0, f1 = 0, f2 = 0;
+  const int64_t b = off[j];
+  for (int k = 0; k < n_neigh[j]; ++k) {
+    const int64_t r = rev[b + k];
+    if (r < 0) { atomicExch(bad, 1); continue; }
+    f0 += fglob[3 * r];
+    f1 += fglob[3 * r + 1];
+    f2 += fglob[3 * r + 2];
+  }
+  F[3 * j] = f0;
+  F[3 * j + 1] = f1;
+  F[3 * j + 2] = f2;
+}
+
+// Virial partials: 0.5 (f x^T + x f^T) summed over a fixed slice of pairs per
+// block (column-major 3x3).
+__global__ __launch_bounds__(256) void k_virial(int64_t P, const double* __restrict__ fglob,
+                                                const double* __restrict__ xyz, double* __restrict__ part) {
+  __shared__ double sh[9][256];
+  const int64_t per = (P + gridDim.x - 1) / gridDim.x;
+  const int64_t b = (int64_t)blockIdx.x * per, e = b + per < P ? b + per : P;
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t p = b + threadIdx.x; p < e; p += 256) {
+    const double f[3] = {fglob[3 * p], fglob[3 * p + 1], fglob[3 * p + 2]};
+    const double x[3] = {xyz[3 * p], xyz[3 * p + 1], xyz[3 * p + 2]};
+#pragma unroll
+    for (int k1 = 0; k1 < 3; ++k1)
+#pragma unroll
+      for (int k2 = 0; k2 < 3; ++k2) v[3 * k2 + k1] += 0.5 * (f[k1] * x[k2] + f[k2] * x[k1]);
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) sh[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+#pragma unroll
+      for (int k = 0; k < 9; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 9) part[9 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void k_energy_scatter(int ns, const int* __restrict__ site_atom, const double* __restrict__ e_site,
+                                 double* __restrict__ energies) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < ns) energies[site_atom[s]] += e_site[s];
+}
+
+}  // namespace
+
+static SpeciesMap species_map(const tgx_model_t* m) {
+  SpeciesMap sm{};
+  sm.n = m->n_species;
+  for (int s = 0; s < m->n_species; ++s) sm.code[s] = m->species_types[s];
+  return sm;
+}
+
+int tgx_block_select(tgx_ctx* ctx, tgx_model_t* m) {
+  TGX_REQUIRE(ctx->have_list, "no neighbour list: call tgx_build_neighbors_list first");
+  const int n = ctx->n_sites;
+  TGX_TRY(ctx->scratch.ensure(sizeof(int) * 2 * (size_t)n + sizeof(int64_t) * 2 * (size_t)(n + 1) + 256));
+  int* flag = ctx->scratch.as<int>();
+  int* npairs = flag + n;
+  int64_t* site_idx = reinterpret_cast<int64_t*>(ctx->scratch.as<char>() + tgx_round_up(sizeof(int) * 2 * (size_t)n, 16));
+  int64_t* pair_off = site_idx + (n + 1);
+  SpeciesMap sm = species_map(m);
+  TGX_LAUNCH(ctx, "select_count", k_select_count, dim3((n + 255) / 256), dim3(256), 0, n, sm, m->central_species,
+             m->rcut_max, ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(), ctx->species_sc.as<int>(),
+             ctx->rjs.as<double>(), ctx->nspec.as<int>(), flag, npairs);
+  int64_t ns = 0, np = 0;
+  TGX_TRY(tgx_scan_i32(ctx, flag, site_idx, n, &ns));
+  TGX_TRY(tgx_scan_i32(ctx, npairs, pair_off, n, &np));
+  m->n_sites_b = (int)ns;
+  m->n_pairs_b = np;
+  const size_t npz = (size_t)(np > 0 ? np : 1);
+  TGX_TRY(m->site_atom.ensure(sizeof(int) * (size_t)(ns + 1)));
+  TGX_TRY(m->site_beg.ensure(sizeof(int64_t) * (size_t)(ns + 1)));
+  TGX_TRY(m->pr_gidx.ensure(sizeof(int64_t) * npz));
+  TGX_TRY(m->pr_nbr.ensure(sizeof(int) * npz));
+  TGX_TRY(m->pr_spec.ensure(sizeof(int) * npz));
+  TGX_TRY(m->pr_r.ensure(sizeof(double) * npz));
+  TGX_TRY(m->pr_xyz.ensure(sizeof(double) * 3 * npz));
+  TGX_LAUNCH(ctx, "select_fill", k_select_fill, dim3((n + 3) / 4), dim3(256), 0, n, sm, m->rcut_max, flag, site_idx,
+             pair_off, ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(), ctx->nlist.as<int>(), ctx->rjs.as<double>(),
+             ctx->xyz.as<double>(), ctx->nspec.as<int>(), m->site_atom.as<int>(), m->site_beg.as<int64_t>(),
+             m->pr_gidx.as<int64_t>(), m->pr_nbr.as<int>(), m->pr_spec.as<int>(), m->pr_r.as<double>(),
+             m->pr_xyz.as<double>(), (int*)nullptr);
+  TGX_LAUNCH(ctx, "select_last", k_set_last, dim3(1), dim3(1), 0, m->site_beg.as<int64_t>(), (int)ns, pair_off, n);
+  return TGX_OK;
+}
+
+// Forces and virial from the accumulated pair forces; adds every model's site
+// energies was done by the caller. energy_out/virial_out may be NULL.
+int tgx_gather_forces(tgx_ctx* ctx, double* energy_out, double* virial_out) {
+  const int n = ctx->n_sites;
+  const int64_t P = ctx->n_pairs;
+  TGX_TRY(ctx->forces.ensure(sizeof(double) * 3 * (size_t)n));
+  TGX_TRY(ctx->red.ensure(sizeof(double) * 9 * 512 + 64));
+  int* bad = reinterpret_cast<int*>(ctx->red.as<double>() + 9 * 512);
+  TGX_TRY(tgx_zero(ctx, bad, sizeof(int)));
+  TGX_LAUNCH(ctx, "force_gather", k_gather, dim3((n + 255) / 256), dim3(256), 0, n, ctx->n_neigh.as<int>(),
+             ctx->nl_off.as<int64_t>(), ctx->rev.as<int64_t>(), ctx->fglob.as<double>(), ctx->forces.as<double>(), bad);
+  const int nbk = 512;
+  TGX_LAUNCH(ctx, "virial", k_virial, dim3(nbk), dim3(256), 0, P, ctx->fglob.as<double>(), ctx->xyz.as<double>(),
+             ctx->red.as<double>());
+  std::vector<double> part(9 * nbk);
+  int badh = 0;
+  TGX_CHECK_HIP(hipMemcpyAsync(part.data(), ctx->red.p, sizeof(double) * 9 * nbk, hipMemcpyDeviceToHost, ctx->stream));
+  TGX_CHECK_HIP(hipMemcpyAsync(&badh, bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  if (badh) {
+    tgx_set_error("force gather: a pair has no reverse entry (an atom without its own list); build with do_list = all");
+    return TGX_ERR_UNSUPPORTED;
+  }
+  if (virial_out) {
+    for (int k = 0; k < 9; ++k) virial_out[k] = 0.0;
+    for (int b = 0; b < nbk; ++b)
+      for (int k = 0; k < 9; ++k) virial_out[k] += part[9 * b + k];
+  }
+  if (energy_out) TGX_TRY(tgx_device_sum(ctx, ctx->energies.as<double>(), n, energy_out));
+  return TGX_OK;
+}
+
+int tgx_energy_scatter(tgx_ctx* ctx, tgx_model_t* m) {
+  const int ns = m->n_sites_b;
+  if (ns == 0) return TGX_OK;
+  TGX_LAUNCH(ctx, "energy_scatter", k_energy_scatter, dim3((ns + 255) / 256), dim3(256), 0, ns, m->site_atom.as<int>(),
+             m->e_site.as<double>(), ctx->energies.as<double>());
+  return TGX_OK;
+}
+
+// Atoms a block touches: its centres and every kept neighbour.
+__global__ void k_mark_seen(int64_t np, const int* __restrict__ pr_nbr, int* __restrict__ seen) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < np) seen[pr_nbr[p]] = 1;
+}
+
+int tgx_block_seen(tgx_ctx* ctx, tgx_model_t* m, int* d_seen) {
+  TGX_TRY(tgx_zero(ctx, d_seen, sizeof(int) * (size_t)ctx->n_sites));
+  const int64_t np = m->n_pairs_b;
+  if (np == 0) return TGX_OK;
+  TGX_LAUNCH(ctx, "mark_seen", k_mark_seen, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, np, m->pr_nbr.as<int>(),
+             d_seen);
+  return TGX_OK;
+}

@@ -1,0 +1,706 @@
+// SOAP descriptor kernels: radial (poly3/poly3gauss by fixed Gauss-Legendre
+// projection) and real-spherical-harmonic expansion, power spectrum,
+// normalisation, and the two force routes:
+//   plain   — dq_i/dr_j written for every pair, (3, D, P) like soap_cart_der
+//             (src/gap_interface.f90:312-317), then contracted by k_plain_force;
+//   adjoint — dE/dq back-propagated to dE/dc_nlm per site and contracted with
+//             dc_nlm/dr_j recomputed per pair; dq/dr is never stored.
+// The conventions (basis, density, amplitude, angular damping, ordering,
+// multiplicity sqrt(2), normalisation, self pair = -sum) are the ones the
+// FP64 oracle states; soap_turbo's own source is not available to follow.
+//
+// One workgroup (256 threads, 4 wavefronts) per central site. The site's
+// expansion coefficients c(n, lm) live in LDS; lanes of a wavefront split the
+// quadrature nodes, the (l, m) harmonics and the (n, l) channel sums of a pair;
+// the power spectrum and its derivative are split over threads by component.
+#include "tgx_internal.h"
+
+namespace {
+
+constexpr int WG = 256;
+
+template <class T> __device__ __forceinline__ T t_exp(T x);
+template <> __device__ __forceinline__ double t_exp<double>(double x) { return exp(x); }
+template <> __device__ __forceinline__ float t_exp<float>(float x) { return __expf(x); }
+template <class T> __device__ __forceinline__ T t_sqrt(T x);
+template <> __device__ __forceinline__ double t_sqrt<double>(double x) { return sqrt(x); }
+template <> __device__ __forceinline__ float t_sqrt<float>(float x) { return sqrtf(x); }
+template <class T> __device__ __forceinline__ T t_pow(T x, T y);
+template <> __device__ __forceinline__ double t_pow<double>(double x, double y) { return pow(x, y); }
+template <> __device__ __forceinline__ float t_pow<float>(float x, float y) { return powf(x, y); }
+
+// Real spherical harmonic Y_lm(u) (orthonormal, Condon-Shortley phase) and the
+// gradient of the homogeneous solid harmonic r^l Y_lm at the unit vector u.
+template <class T>
+__device__ void ylm_one(int l, int m, T ux, T uy, T uz, T& Y, T& gx, T& gy, T& gz) {
+  const int am = m < 0 ? -m : m;
+  // (x + i y)^am and its x, y derivatives
+  T c = 1, cx = 0, cy = 0, s = 0, sx = 0, sy = 0;
+  for (int k = 0; k < am; ++k) {
+    T nc = c * ux - s * uy, ncx = cx * ux + c - sx * uy, ncy = cy * ux - sy * uy - s;
+    T ns = c * uy + s * ux, nsx = cx * uy + sx * ux + s, nsy = cy * uy + c + sy * ux;
+    c = nc; cx = ncx; cy = ncy; s = ns; sx = nsx; sy = nsy;
+  }
+  // Q_l^am(z, r^2) with Q_am^am = 1 (the constant goes into the normalisation)
+  T q = 1, qx = 0, qy = 0, qz = 0;        // Q_{l'}
+  T p = 0, px = 0, py = 0, pz = 0;        // Q_{l'-1}
+  if (l > am) {
+    p = q; px = qx; py = qy; pz = qz;
+    q = (T)(2 * am + 1) * uz; qx = 0; qy = 0; qz = (T)(2 * am + 1);
+    for (int lp = am + 2; lp <= l; ++lp) {
+      const T a = (T)(2 * lp - 1), b = (T)(lp + am - 1), inv = (T)1 / (T)(lp - am);
+      // r^2 = 1 at the unit vector, d(r^2) = 2u
+      T nq = (a * uz * q - b * p) * inv;
+      T nqx = (a * uz * qx - b * ((T)2 * ux * p + px)) * inv;
+      T nqy = (a * uz * qy - b * ((T)2 * uy * p + py)) * inv;
+      T nqz = (a * (q + uz * qz) - b * ((T)2 * uz * p + pz)) * inv;
+      p = q; px = qx; py = qy; pz = qz;
+      q = nq; qx = nqx; qy = nqy; qz = nqz;
+    }
+  }
+  // normalisation sqrt((2l+1)/(4 pi) (l-am)!/(l+am)!) * (sqrt 2 for m != 0) * (-1)^am (2am-1)!!
+  double ratio = 1.0;
+  for (int k = l - am + 1; k <= l + am; ++k) ratio /= (double)k;
+  double nrm = sqrt((2.0 * l + 1.0) / (4.0 * M_PI) * ratio);
+  double df = 1.0;
+  for (int k = 1; k <= 2 * am - 1; k += 2) df *= (double)k;
+  nrm *= df * ((am & 1) ? -1.0 : 1.0);
+  if (am != 0) nrm *= M_SQRT2;
+  const T N = (T)nrm;
+  if (m == 0) {
+    Y = N * q; gx = N * qx; gy = N * qy; gz = N * qz;
+  } else if (m > 0) {
+    Y = N * q * c; gx = N * (qx * c + q * cx); gy = N * (qy * c + q * cy); gz = N * qz * c;
+  } else {
+    Y = N * q * s; gx = N * (qx * s + q * sx); gy = N * (qy * s + q * sy); gz = N * qz * s;
+  }
+}
+
+__device__ __forceinline__ int lm_l(int lm) {
+  int l = (int)sqrtf((float)lm);
+  while ((l + 1) * (l + 1) <= lm) ++l;
+  while (l * l > lm) --l;
+  return l;
+}
+
+// Scratch of one pair in LDS.
+template <class T> struct PairScratch {
+  T* G;   // [MAX_Q]
+  T* dG;  // [MAX_Q]
+  T* R;   // [MAX_NB]
+  T* dR;  // [MAX_NB]
+  T* A;   // [L+1]
+  T* dA;  // [L+1]
+  T* Y;   // [NLM]
+  T* gY;  // [3 NLM]  gradient of the solid harmonic
+  static constexpr int size(int L, int NLM) { return 2 * TGX_MAX_Q + 2 * TGX_MAX_NB + 2 * (L + 1) + 4 * NLM; }
+  __device__ void bind(T* base, int L, int NLM) {
+    G = base; dG = G + TGX_MAX_Q; R = dG + TGX_MAX_Q; dR = R + TGX_MAX_NB; A = dR + TGX_MAX_NB; dA = A + (L + 1);
+    Y = dA + (L + 1); gY = Y + NLM;
+  }
+};
+
+// Step 1 of a pair, by `nthr` threads with local index t: Gaussians at the
+// quadrature nodes, harmonics, angular damping.
+template <class T>
+__device__ void pair_step1(const SoapParams& P, int s, T r, T ux, T uy, T uz, int t, int nthr, PairScratch<T>& ps,
+                           bool need_grad) {
+  const SpeciesParams& sp = P.sp[s];
+  const T inv_rc = (T)sp.inv_rc;
+  const T x = r * inv_rc;
+  const T sig = (T)sp.sig_r * ((T)1 + (T)sp.sig_r_sc * r) * inv_rc;
+  const T dsig = (T)(sp.sig_r * sp.sig_r_sc) * inv_rc;
+  const T inv_s2 = (T)1 / (sig * sig);
+  const double* xq = P.xq + s * TGX_MAX_Q;
+  for (int q = t; q < sp.nq; q += nthr) {
+    const T dx = (T)xq[q] - x;
+    const T g = t_exp<T>(-(T)0.5 * dx * dx * inv_s2);
+    ps.G[q] = g;
+    ps.dG[q] = g * (dx * inv_s2 * inv_rc + dx * dx * inv_s2 / sig * dsig);
+  }
+  for (int lm = t; lm < P.NLM; lm += nthr) {
+    const int l = lm_l(lm), m = lm - l * l - l;
+    T Y, gx, gy, gz;
+    ylm_one<T>(l, m, ux, uy, uz, Y, gx, gy, gz);
+    ps.Y[lm] = Y;
+    if (need_grad) { ps.gY[3 * lm] = gx; ps.gY[3 * lm + 1] = gy; ps.gY[3 * lm + 2] = gz; }
+  }
+  for (int l = t; l <= P.L; l += nthr) {
+    const T st = (T)sp.sig_t * ((T)1 + (T)sp.sig_t_sc * r);
+    const T dst = (T)(sp.sig_t * sp.sig_t_sc);
+    const T ll = (T)(l * (l + 1));
+    const T e = t_exp<T>(-ll * st * st / ((T)2 * r * r));
+    ps.A[l] = e;
+    ps.dA[l] = e * (ll * st * st / (r * r * r) - ll * st * dst / (r * r));
+  }
+}
+
+// Step 2: radial functions of the pair's species block and their r-derivatives.
+template <class T>
+__device__ void pair_step2(const SoapParams& P, int s, T r, int t, int nthr, PairScratch<T>& ps) {
+  const SpeciesParams& sp = P.sp[s];
+  const T inv_rc = (T)sp.inv_rc;
+  const T x = r * inv_rc;
+  const T sig = (T)sp.sig_r * ((T)1 + (T)sp.sig_r_sc * r) * inv_rc;
+  const T dsig = (T)(sp.sig_r * sp.sig_r_sc) * inv_rc;
+  const T a = (T)sp.amp_a;
+  const T Pp = (T)1 + (T)2 * x * x * x - (T)3 * x * x;
+  const T dPp = ((T)6 * x * x - (T)6 * x) * inv_rc;
+  T amp, damp;
+  if (a == (T)0) {
+    amp = (T)1 / sig;
+    damp = -dsig / (sig * sig);
+  } else if (Pp <= (T)1.0e-10) {
+    amp = 0; damp = 0;
+  } else {
+    const T Pa = t_pow<T>(Pp, a);
+    amp = Pa / sig;
+    damp = a * t_pow<T>(Pp, a - (T)1) * dPp / sig - Pa * dsig / (sig * sig);
+  }
+  amp *= (T)sp.gs;
+  damp *= (T)sp.gs;
+  const double* B = P.B + (size_t)s * TGX_MAX_NB * TGX_MAX_Q;
+  for (int n = t; n < sp.nb; n += nthr) {
+    T I = 0, dI = 0;
+    const double* Bn = B + n * TGX_MAX_Q;
+    for (int q = 0; q < sp.nq; ++q) {
+      const T b = (T)Bn[q];
+      I += b * ps.G[q];
+      dI += b * ps.dG[q];
+    }
+    ps.R[n] = amp * I;
+    ps.dR[n] = damp * I + amp * dI;
+  }
+}
+
+struct SiteArgs {
+  const int* site_atom;
+  const int64_t* site_beg;
+  const int* pr_spec;
+  const double* pr_r;
+  const double* pr_xyz;
+  const int64_t* pr_gidx;
+};
+
+// Phase A: expansion coefficients c(n, lm) of one site into LDS.
+template <class T>
+__device__ void site_coefficients(const SoapParams& P, const SiteArgs& a, int64_t b0, int nn, T* c, T* wscr) {
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int NLM = P.NLM, L = P.L;
+  for (int e = tid; e < P.N * NLM; e += WG) c[e] = 0;
+  PairScratch<T> ps;
+  ps.bind(wscr + w * PairScratch<T>::size(L, NLM), L, NLM);
+  __syncthreads();
+  // central atom (self pair, first of the site): its own species block, l = 0
+  {
+    const int s = a.pr_spec[b0];
+    if (s >= 0)
+      for (int n = tid; n < P.sp[s].nb; n += WG) c[(P.sp[s].off + n) * NLM] += (T)P.cen[s * TGX_MAX_NB + n];
+  }
+  __syncthreads();
+  for (int g = 1; g < nn; g += 4) {
+    const int k = g + w;
+    int s = -1;
+    T r = 0, ux = 0, uy = 0, uz = 0;
+    if (k < nn) {
+      s = a.pr_spec[b0 + k];
+      r = (T)a.pr_r[b0 + k];
+      if (s >= 0 && !(a.pr_r[b0 + k] < P.sp[s].rc)) s = -1;
+      if (s >= 0) {
+        const T inv = (T)1 / r;
+        ux = (T)a.pr_xyz[3 * (b0 + k)] * inv;
+        uy = (T)a.pr_xyz[3 * (b0 + k) + 1] * inv;
+        uz = (T)a.pr_xyz[3 * (b0 + k) + 2] * inv;
+      }
+    }
+    if (s >= 0) pair_step1<T>(P, s, r, ux, uy, uz, lane, 64, ps, false);
+    __syncthreads();
+    if (s >= 0) pair_step2<T>(P, s, r, lane, 64, ps);
+    __syncthreads();
+    // accumulate the (up to) four pairs of this group in pair order
+    for (int ww = 0; ww < 4; ++ww) {
+      const int kk = g + ww;
+      if (kk >= nn) break;
+      int ss = a.pr_spec[b0 + kk];
+      if (ss < 0 || !(a.pr_r[b0 + kk] < P.sp[ss].rc)) continue;
+      PairScratch<T> po;
+      po.bind(wscr + ww * PairScratch<T>::size(L, NLM), L, NLM);
+      const int nb = P.sp[ss].nb, off = P.sp[ss].off;
+      for (int e = tid; e < nb * NLM; e += WG) {
+        const int n = e / NLM, lm = e - n * NLM;
+        c[(off + n) * NLM + lm] += po.R[n] * po.A[lm_l(lm)] * po.Y[lm];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Phase B: power spectrum into X[0..D), |p| returned to every thread; X is
+// normalised in place to q.
+template <class T>
+__device__ T power_spectrum(const SoapParams& P, const T* c, T* X, T* red) {
+  const int tid = threadIdx.x;
+  const int NLM = P.NLM;
+  T acc = 0;
+  for (int d = tid; d < P.D; d += WG) {
+    const int code = P.dtab[d];
+    const int n = code & 0xff, np = (code >> 8) & 0xff, l = code >> 16;
+    const T* c1 = c + n * NLM + l * l;
+    const T* c2 = c + np * NLM + l * l;
+    T s = 0;
+    for (int m = 0; m < 2 * l + 1; ++m) s += c1[m] * c2[m];
+    if (n != np) s *= (T)M_SQRT2;
+    X[d] = s;
+    acc += s * s;
+  }
+  red[tid] = acc;
+  __syncthreads();
+  for (int o = WG / 2; o > 0; o >>= 1) {
+    if (tid < o) red[tid] += red[tid + o];
+    __syncthreads();
+  }
+  const T nrm = t_sqrt<T>(red[0]);
+  __syncthreads();
+  const T inv = nrm > (T)0 ? (T)1 / nrm : (T)0;
+  for (int d = tid; d < P.D; d += WG) X[d] *= inv;
+  __syncthreads();
+  return nrm;
+}
+
+// H(n, lm) = sum_n' S(n, n', l) c(n', lm) with S = sqrt2 V(n,n',l) off the
+// diagonal and 2 V(n,n,l) on it: the gradient of V . p with respect to c.
+template <class T>
+__device__ void contract_channel(const SoapParams& P, const T* V, const T* c, T* H) {
+  const int NLM = P.NLM, N = P.N, L1 = P.L + 1;
+  for (int e = threadIdx.x; e < N * NLM; e += WG) {
+    const int n = e / NLM, lm = e - n * NLM, l = lm_l(lm);
+    T h = 0;
+    for (int np = 0; np < N; ++np) {
+      const int a = n < np ? n : np, b = n < np ? np : n;
+      const int pidx = a * N - a * (a - 1) / 2 + (b - a);
+      const T v = V[pidx * L1 + l];
+      h += (n == np ? (T)2 : (T)M_SQRT2) * v * c[np * NLM + lm];
+    }
+    H[e] = h;
+  }
+}
+
+// Mode 0: descriptors only. Mode 1: descriptors and dq/dr (plain path, FP64).
+template <class T, int MODE, int KMAX>
+__global__ __launch_bounds__(WG) void k_soap_site(SoapParams P, SiteArgs a, int n_sites, T* __restrict__ q_out,
+                                                  T* __restrict__ pnorm_out, double* __restrict__ der_out) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  const int site = blockIdx.x;
+  if (site >= n_sites) return;
+  const int tid = threadIdx.x;
+  const int NLM = P.NLM, N = P.N, L = P.L, L1 = L + 1, D = P.D;
+  T* c = sm;                                   // N*NLM
+  T* X = c + N * NLM;                          // D (p then q)
+  T* red = X + D;                              // WG
+  T* H = red + WG;                             // N*NLM (mode 1)
+  T* wscr = H + (MODE == 1 ? N * NLM : 0);     // 4 pair scratches / phase D scratch
+  const int64_t b0 = a.site_beg[site];
+  const int nn = (int)(a.site_beg[site + 1] - b0);
+  site_coefficients<T>(P, a, b0, nn, c, wscr);
+  const T nrm = power_spectrum<T>(P, c, X, red);
+  T* qrow = q_out + (size_t)site * P.Dpad;
+  for (int d = tid; d < P.Dpad; d += WG) qrow[d] = d < D ? X[d] : (T)0;
+  if (tid == 0) pnorm_out[site] = nrm;
+  if (MODE == 0) return;
+  // ---- dq/dr for every pair of the site (MODE 1) ----
+  contract_channel<T>(P, X, c, H);
+  __syncthreads();
+  PairScratch<T> ps;
+  ps.bind(wscr, L, NLM);
+  T* s_nl = wscr + PairScratch<T>::size(L, NLM);  // N*L1
+  T* vg_nl = s_nl + N * L1;                       // 3*N*L1 (becomes v)
+  T* E_nl = vg_nl + 3 * N * L1;                   // N*L1
+  T* Dd_nl = E_nl + N * L1;                       // N*L1
+  T* sH = Dd_nl + N * L1;                         // MAX_NB*L1
+  T* vgH = sH + TGX_MAX_NB * L1;                  // 3*MAX_NB*L1
+  T* qd = vgH + 3 * TGX_MAX_NB * L1;              // 3 + pad
+  // component ownership: d = tid + WG*k, k < KMAX (KMAX*WG >= D)
+  T self_acc[KMAX][3];
+  T qreg[KMAX];
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k) {
+    self_acc[k][0] = self_acc[k][1] = self_acc[k][2] = 0;
+    const int d = tid + WG * k;
+    qreg[k] = d < D ? X[d] : (T)0;
+  }
+  const T inv_n = nrm > (T)0 ? (T)1 / nrm : (T)0;
+  for (int k = 1; k < nn; ++k) {
+    const int64_t p = b0 + k;
+    int s = a.pr_spec[p];
+    const double rr = a.pr_r[p];
+    if (s >= 0 && !(rr < P.sp[s].rc)) s = -1;
+    double* out = der_out + (size_t)p * 3 * D;
+    if (s < 0 || nrm == (T)0) {
+      // a pair the descriptor does not see: q does not depend on it
+      for (int e = tid; e < 3 * D; e += WG) out[e] = 0.0;
+      continue;
+    }
+    const T r = (T)rr, inv_r = (T)1 / r;
+    const T ux = (T)a.pr_xyz[3 * p] * inv_r, uy = (T)a.pr_xyz[3 * p + 1] * inv_r, uz = (T)a.pr_xyz[3 * p + 2] * inv_r;
+    pair_step1<T>(P, s, r, ux, uy, uz, tid, WG, ps, true);
+    __syncthreads();
+    const int nb = P.sp[s].nb, off = P.sp[s].off;
+    if (tid < 64) {
+      pair_step2<T>(P, s, r, tid, 64, ps);
+    } else {
+      // channel sums over m: s(n,l) = sum_m Y c, vg(n,l) = sum_m gY c, and the
+      // same against H for the pair's block
+      const int tasks = N * L1 + nb * L1;
+      for (int task = tid - 64; task < tasks; task += WG - 64) {
+        const bool withH = task >= N * L1;
+        const int tt = withH ? task - N * L1 : task;
+        const int n = tt / L1, l = tt - n * L1;
+        const T* src = withH ? H + (off + n) * NLM : c + n * NLM;
+        T s0 = 0, g0 = 0, g1 = 0, g2 = 0;
+        for (int lm = l * l; lm < (l + 1) * (l + 1); ++lm) {
+          const T v = src[lm];
+          s0 += ps.Y[lm] * v;
+          g0 += ps.gY[3 * lm] * v;
+          g1 += ps.gY[3 * lm + 1] * v;
+          g2 += ps.gY[3 * lm + 2] * v;
+        }
+        if (withH) {
+          sH[tt] = s0; vgH[3 * tt] = g0; vgH[3 * tt + 1] = g1; vgH[3 * tt + 2] = g2;
+        } else {
+          s_nl[tt] = s0; vg_nl[3 * tt] = g0; vg_nl[3 * tt + 1] = g1; vg_nl[3 * tt + 2] = g2;
+        }
+      }
+    }
+    __syncthreads();
+    // E = R A, Dd = (R A)', v = (vg - l s u)/r; zero outside the block
+    for (int e = tid; e < N * L1; e += WG) {
+      const int n = e / L1, l = e - n * L1;
+      const T sv = s_nl[e];
+      const T lf = (T)l;
+      vg_nl[3 * e] = (vg_nl[3 * e] - lf * sv * ux) * inv_r;
+      vg_nl[3 * e + 1] = (vg_nl[3 * e + 1] - lf * sv * uy) * inv_r;
+      vg_nl[3 * e + 2] = (vg_nl[3 * e + 2] - lf * sv * uz) * inv_r;
+      const int nl = n - off;
+      if (nl >= 0 && nl < nb) {
+        E_nl[e] = ps.R[nl] * ps.A[l];
+        Dd_nl[e] = ps.dR[nl] * ps.A[l] + ps.R[nl] * ps.dA[l];
+      } else {
+        E_nl[e] = 0;
+        Dd_nl[e] = 0;
+      }
+    }
+    // q . dp: sum over the block's channels (one wavefront, fixed order)
+    if (tid < 64) {
+      T a0 = 0, a1 = 0, a2 = 0;
+      for (int tt = tid; tt < nb * L1; tt += 64) {
+        const int n = tt / L1, l = tt - n * L1;
+        const T E = ps.R[n] * ps.A[l];
+        const T Dd = ps.dR[n] * ps.A[l] + ps.R[n] * ps.dA[l];
+        const T sh = sH[tt], lf = (T)l;
+        a0 += Dd * sh * ux + E * (vgH[3 * tt] - lf * sh * ux) * inv_r;
+        a1 += Dd * sh * uy + E * (vgH[3 * tt + 1] - lf * sh * uy) * inv_r;
+        a2 += Dd * sh * uz + E * (vgH[3 * tt + 2] - lf * sh * uz) * inv_r;
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        a0 += __shfl_xor(a0, o);
+        a1 += __shfl_xor(a1, o);
+        a2 += __shfl_xor(a2, o);
+      }
+      if (tid == 0) { qd[0] = a0; qd[1] = a1; qd[2] = a2; }
+    }
+    __syncthreads();
+    const T q0 = qd[0], q1 = qd[1], q2 = qd[2];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      const int d = tid + WG * k;
+      if (d >= D) break;
+      const int code = P.dtab[d];
+      const int n = code & 0xff, np = (code >> 8) & 0xff, l = code >> 16;
+      const int i1 = n * L1 + l, i2 = np * L1 + l;
+      const T w = n == np ? (T)1 : (T)M_SQRT2;
+      const T rad = Dd_nl[i1] * s_nl[i2] + Dd_nl[i2] * s_nl[i1];
+      const T E1 = E_nl[i1], E2 = E_nl[i2];
+      const T dp0 = w * (rad * ux + E1 * vg_nl[3 * i2] + E2 * vg_nl[3 * i1]);
+      const T dp1 = w * (rad * uy + E1 * vg_nl[3 * i2 + 1] + E2 * vg_nl[3 * i1 + 1]);
+      const T dp2 = w * (rad * uz + E1 * vg_nl[3 * i2 + 2] + E2 * vg_nl[3 * i1 + 2]);
+      const T o0 = (dp0 - qreg[k] * q0) * inv_n;
+      const T o1 = (dp1 - qreg[k] * q1) * inv_n;
+      const T o2 = (dp2 - qreg[k] * q2) * inv_n;
+      out[3 * d] = (double)o0;
+      out[3 * d + 1] = (double)o1;
+      out[3 * d + 2] = (double)o2;
+      self_acc[k][0] -= o0;
+      self_acc[k][1] -= o1;
+      self_acc[k][2] -= o2;
+    }
+    __syncthreads();
+  }
+  double* out = der_out + (size_t)b0 * 3 * D;
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k) {
+    const int d = tid + WG * k;
+    if (d >= D) break;
+    out[3 * d] = (double)self_acc[k][0];
+    out[3 * d + 1] = (double)self_acc[k][1];
+    out[3 * d + 2] = (double)self_acc[k][2];
+  }
+}
+
+// Adjoint forces. For site i: g = (Qss - (Qss.q) q)/|p| is dE... contracted with
+// dp/dq; Hg(n, lm) = d(g.p)/dc; for every pair j the force on j is
+//   f_ij = sum_{n in block, l} [ (R A)' u sHg + R A (vgHg - l sHg u)/r ]
+// and the self pair receives -sum_j f_ij. Pair forces are accumulated into the
+// global pair array fglob (one writer per pair, blocks in stream order).
+template <class T>
+__global__ __launch_bounds__(WG) void k_soap_adjoint(SoapParams P, SiteArgs a, int n_sites, const T* __restrict__ q_in,
+                                                     const T* __restrict__ pnorm_in, const T* __restrict__ qss,
+                                                     double* __restrict__ fglob) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  const int site = blockIdx.x;
+  if (site >= n_sites) return;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int NLM = P.NLM, N = P.N, L = P.L, L1 = L + 1, D = P.D;
+  T* c = sm;                       // N*NLM
+  T* X = c + N * NLM;              // D: g
+  T* red = X + D;                  // WG
+  T* H = red + WG;                 // N*NLM
+  T* wscr = H + N * NLM;           // 4 pair scratches
+  const int64_t b0 = a.site_beg[site];
+  const int nn = (int)(a.site_beg[site + 1] - b0);
+  site_coefficients<T>(P, a, b0, nn, c, wscr);
+  const T nrm = pnorm_in[site];
+  const T* qrow = q_in + (size_t)site * P.Dpad;
+  const T* srow = qss + (size_t)site * P.Dpad;
+  // Qss . q
+  T acc = 0;
+  for (int d = tid; d < D; d += WG) acc += srow[d] * qrow[d];
+  red[tid] = acc;
+  __syncthreads();
+  for (int o = WG / 2; o > 0; o >>= 1) {
+    if (tid < o) red[tid] += red[tid + o];
+    __syncthreads();
+  }
+  const T sq = red[0];
+  const T inv_n = nrm > (T)0 ? (T)1 / nrm : (T)0;
+  for (int d = tid; d < D; d += WG) X[d] = (srow[d] - sq * qrow[d]) * inv_n;
+  __syncthreads();
+  contract_channel<T>(P, X, c, H);
+  __syncthreads();
+  PairScratch<T> ps;
+  ps.bind(wscr + w * PairScratch<T>::size(L, NLM), L, NLM);
+  double self0 = 0, self1 = 0, self2 = 0;  // lane 0: this wave's pairs, in order
+  for (int g = 1; g < nn; g += 4) {
+    const int k = g + w;
+    int s = -1;
+    T r = 0, ux = 0, uy = 0, uz = 0;
+    if (k < nn) {
+      s = a.pr_spec[b0 + k];
+      if (s >= 0 && !(a.pr_r[b0 + k] < P.sp[s].rc)) s = -1;
+      if (s >= 0) {
+        r = (T)a.pr_r[b0 + k];
+        const T inv = (T)1 / r;
+        ux = (T)a.pr_xyz[3 * (b0 + k)] * inv;
+        uy = (T)a.pr_xyz[3 * (b0 + k) + 1] * inv;
+        uz = (T)a.pr_xyz[3 * (b0 + k) + 2] * inv;
+      }
+    }
+    if (s >= 0) pair_step1<T>(P, s, r, ux, uy, uz, lane, 64, ps, true);
+    __syncthreads();
+    if (s >= 0) pair_step2<T>(P, s, r, lane, 64, ps);
+    __syncthreads();
+    if (k < nn) {
+      T a0 = 0, a1 = 0, a2 = 0;
+      if (s >= 0) {
+        const int nb = P.sp[s].nb, off = P.sp[s].off;
+        const T inv_r = (T)1 / r;
+        for (int tt = lane; tt < nb * L1; tt += 64) {
+          const int n = tt / L1, l = tt - n * L1;
+          const T* h = H + (off + n) * NLM;
+          T s0 = 0, g0 = 0, g1 = 0, g2 = 0;
+          for (int lm = l * l; lm < (l + 1) * (l + 1); ++lm) {
+            const T v = h[lm];
+            s0 += ps.Y[lm] * v;
+            g0 += ps.gY[3 * lm] * v;
+            g1 += ps.gY[3 * lm + 1] * v;
+            g2 += ps.gY[3 * lm + 2] * v;
+          }
+          const T E = ps.R[n] * ps.A[l];
+          const T Dd = ps.dR[n] * ps.A[l] + ps.R[n] * ps.dA[l];
+          const T lf = (T)l;
+          a0 += Dd * s0 * ux + E * (g0 - lf * s0 * ux) * inv_r;
+          a1 += Dd * s0 * uy + E * (g1 - lf * s0 * uy) * inv_r;
+          a2 += Dd * s0 * uz + E * (g2 - lf * s0 * uz) * inv_r;
+        }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        a0 += __shfl_xor(a0, o);
+        a1 += __shfl_xor(a1, o);
+        a2 += __shfl_xor(a2, o);
+      }
+      if (lane == 0) {
+        self0 += (double)a0; self1 += (double)a1; self2 += (double)a2;
+        const int64_t gi = a.pr_gidx[b0 + k];
+        fglob[3 * gi] += (double)a0;
+        fglob[3 * gi + 1] += (double)a1;
+        fglob[3 * gi + 2] += (double)a2;
+      }
+    }
+    __syncthreads();
+  }
+  double* wsum = reinterpret_cast<double*>(red);  // 4 waves x 3 (red holds >= 12 doubles)
+  if (lane == 0) { wsum[3 * w] = self0; wsum[3 * w + 1] = self1; wsum[3 * w + 2] = self2; }
+  __syncthreads();
+  if (tid == 0) {
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int ww = 0; ww < 4; ++ww) { s0 += wsum[3 * ww]; s1 += wsum[3 * ww + 1]; s2 += wsum[3 * ww + 2]; }
+    const int64_t gi = a.pr_gidx[b0];
+    fglob[3 * gi] -= s0;
+    fglob[3 * gi + 1] -= s1;
+    fglob[3 * gi + 2] -= s2;
+  }
+}
+
+// Plain-path pair force: f_p = Qss_i . soap_cart_der(:, :, p), one wavefront per
+// pair, streamed over the (3, D) slab. This is the scatter of src/gap.f90:206-224.
+__global__ __launch_bounds__(256) void k_plain_force(int n_sites, const int64_t* __restrict__ site_beg, int D,
+                                                    int Dpad, const double* __restrict__ qss,
+                                                    const double* __restrict__ der,
+                                                    const int64_t* __restrict__ pr_gidx, int64_t n_pairs,
+                                                    const int* __restrict__ pair_site, double* __restrict__ fglob) {
+  const int lane = threadIdx.x & 63;
+  const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (p >= n_pairs) return;
+  const int i = pair_site[p];
+  const double* s = qss + (size_t)i * Dpad;
+  const double* g = der + (size_t)p * 3 * D;
+  double a0 = 0, a1 = 0, a2 = 0;
+  for (int d = lane; d < D; d += 64) {
+    const double sv = s[d];
+    a0 += sv * g[3 * d];
+    a1 += sv * g[3 * d + 1];
+    a2 += sv * g[3 * d + 2];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a0 += __shfl_xor(a0, o);
+    a1 += __shfl_xor(a1, o);
+    a2 += __shfl_xor(a2, o);
+  }
+  if (lane == 0) {
+    const int64_t gi = pr_gidx[p];
+    fglob[3 * gi] += a0;
+    fglob[3 * gi + 1] += a1;
+    fglob[3 * gi + 2] += a2;
+  }
+}
+
+__global__ void k_pair_site(int n_sites, const int64_t* __restrict__ site_beg, int* __restrict__ pair_site) {
+  const int i = blockIdx.x;
+  if (i >= n_sites) return;
+  for (int64_t p = site_beg[i] + threadIdx.x; p < site_beg[i + 1]; p += blockDim.x) pair_site[p] = i;
+}
+
+}  // namespace
+
+// LDS bytes of the site kernels.
+static size_t soap_lds_bytes(const SoapParams& P, int mode, size_t elt) {
+  const int N = P.N, NLM = P.NLM, L1 = P.L + 1;
+  const size_t ps = (size_t)(2 * TGX_MAX_Q + 2 * TGX_MAX_NB + 2 * L1 + 4 * NLM);
+  size_t n = (size_t)N * NLM + P.D + WG;
+  if (mode == 0) n += 4 * ps;
+  if (mode == 1) {
+    n += (size_t)N * NLM;
+    size_t d_scr = ps + (size_t)N * L1 * 6 + (size_t)TGX_MAX_NB * L1 * 4 + 4;
+    n += d_scr > 4 * ps ? d_scr : 4 * ps;
+  }
+  if (mode == 2) n += (size_t)N * NLM + 4 * ps;
+  return n * elt;
+}
+
+static SiteArgs site_args(tgx_model_t* m) {
+  SiteArgs a;
+  a.site_atom = m->site_atom.as<int>();
+  a.site_beg = m->site_beg.as<int64_t>();
+  a.pr_spec = m->pr_spec.as<int>();
+  a.pr_r = m->pr_r.as<double>();
+  a.pr_xyz = m->pr_xyz.as<double>();
+  a.pr_gidx = m->pr_gidx.as<int64_t>();
+  return a;
+}
+
+static int check_lds(size_t bytes) {
+  TGX_REQUIRE(bytes <= 160 * 1024, "SOAP kernel needs %zu bytes of LDS (more than 160 KiB): reduce n_max or l_max",
+              bytes);
+  return TGX_OK;
+}
+
+int tgx_soap_descriptors(tgx_ctx* ctx, tgx_model_t* m, int path, bool with_der) {
+  const int ns = m->n_sites_b;
+  if (ns == 0) return TGX_OK;
+  TGX_REQUIRE(m->P.D <= 24 * WG, "descriptor size %d exceeds this build's limit %d", m->P.D, 24 * WG);
+  const size_t nsp = (size_t)tgx_round_up(ns, 128);
+  TGX_TRY(m->pnorm.ensure(sizeof(double) * nsp));
+  SiteArgs a = site_args(m);
+  if (path == TGX_PATH_FP64_PLAIN || path == TGX_PATH_FP64_ADJOINT) {
+    TGX_TRY(m->q64.ensure(sizeof(double) * nsp * m->Dpad));
+    if (with_der) {
+      TGX_TRY(m->soap_der.ensure(sizeof(double) * (size_t)(m->n_pairs_b > 0 ? m->n_pairs_b : 1) * 3 * m->D));
+      const size_t lds = soap_lds_bytes(m->P, 1, sizeof(double));
+      TGX_TRY(check_lds(lds));
+      const int kmax = (m->P.D + WG - 1) / WG;
+#define TGX_SOAP_DER(K)                                                                                 \
+  TGX_LAUNCH(ctx, "soap_site_der_f64", (k_soap_site<double, 1, K>), dim3(ns), dim3(WG), lds, m->P, a, ns, \
+             m->q64.as<double>(), m->pnorm.as<double>(), m->soap_der.as<double>())
+      if (kmax <= 2) TGX_SOAP_DER(2);
+      else if (kmax <= 5) TGX_SOAP_DER(5);
+      else if (kmax <= 8) TGX_SOAP_DER(8);
+      else if (kmax <= 12) TGX_SOAP_DER(12);
+      else TGX_SOAP_DER(24);
+#undef TGX_SOAP_DER
+    } else {
+      const size_t lds = soap_lds_bytes(m->P, 0, sizeof(double));
+      TGX_TRY(check_lds(lds));
+      TGX_LAUNCH(ctx, "soap_site_f64", (k_soap_site<double, 0, 1>), dim3(ns), dim3(WG), lds, m->P, a, ns,
+                 m->q64.as<double>(), m->pnorm.as<double>(), (double*)nullptr);
+    }
+  } else {
+    TGX_TRY(m->q32.ensure(sizeof(float) * nsp * m->Dpad));
+    const size_t lds = soap_lds_bytes(m->P, 0, sizeof(float));
+    TGX_TRY(check_lds(lds));
+    TGX_LAUNCH(ctx, "soap_site_f32", (k_soap_site<float, 0, 1>), dim3(ns), dim3(WG), lds, m->P, a, ns,
+               m->q32.as<float>(), m->pnorm.as<float>(), (double*)nullptr);
+  }
+  return TGX_OK;
+}
+
+int tgx_adjoint_pair_forces(tgx_ctx* ctx, tgx_model_t* m, int path) {
+  const int ns = m->n_sites_b;
+  if (ns == 0) return TGX_OK;
+  SiteArgs a = site_args(m);
+  if (path == TGX_PATH_FP64_ADJOINT || path == TGX_PATH_FP64_PLAIN) {
+    const size_t lds = soap_lds_bytes(m->P, 2, sizeof(double));
+    TGX_TRY(check_lds(lds));
+    TGX_LAUNCH(ctx, "soap_adjoint_f64", k_soap_adjoint<double>, dim3(ns), dim3(WG), lds, m->P, a, ns,
+               m->q64.as<double>(), m->pnorm.as<double>(), m->qss64.as<double>(), ctx->fglob.as<double>());
+  } else {
+    const size_t lds = soap_lds_bytes(m->P, 2, sizeof(float));
+    TGX_TRY(check_lds(lds));
+    TGX_LAUNCH(ctx, "soap_adjoint_f32", k_soap_adjoint<float>, dim3(ns), dim3(WG), lds, m->P, a, ns,
+               m->q32.as<float>(), m->pnorm.as<float>(), m->qss32.as<float>(), ctx->fglob.as<double>());
+  }
+  return TGX_OK;
+}
+
+int tgx_plain_pair_forces(tgx_ctx* ctx, tgx_model_t* m) {
+  const int ns = m->n_sites_b;
+  const int64_t np = m->n_pairs_b;
+  if (ns == 0 || np == 0) return TGX_OK;
+  TGX_TRY(ctx->scratch.ensure(sizeof(int) * (size_t)np));
+  int* pair_site = ctx->scratch.as<int>();
+  TGX_LAUNCH(ctx, "pair_site", k_pair_site, dim3(ns), dim3(64), 0, ns, m->site_beg.as<int64_t>(), pair_site);
+  TGX_LAUNCH(ctx, "plain_force", k_plain_force, dim3((unsigned)((np + 3) / 4)), dim3(256), 0, ns,
+             m->site_beg.as<int64_t>(), m->D, m->Dpad, m->qss64.as<double>(), m->soap_der.as<double>(),
+             m->pr_gidx.as<int64_t>(), np, pair_site, ctx->fglob.as<double>());
+  return TGX_OK;
+}

@@ -1,0 +1,130 @@
+// Device utilities: zero fill, deterministic exclusive scan, deterministic sum.
+#include "tgx_util.h"
+
+namespace {
+
+constexpr int SCAN_BLOCK = 1024;
+
+__global__ void k_fill_zero(uint32_t* __restrict__ p, size_t n_words) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n_words; i += stride) p[i] = 0u;
+}
+
+__global__ void k_fill_zero_bytes(uint8_t* __restrict__ p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// Block-local exclusive scan of 1024 values (one value per thread).
+__device__ int64_t block_exclusive_scan(int64_t v, int64_t* sh, int64_t* total) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int o = 1; o < SCAN_BLOCK; o <<= 1) {
+    int64_t add = t >= o ? sh[t - o] : 0;
+    __syncthreads();
+    sh[t] += add;
+    __syncthreads();
+  }
+  int64_t incl = sh[t];
+  *total = sh[SCAN_BLOCK - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_local(const int* __restrict__ in, int64_t* __restrict__ out,
+                                                          int64_t n, int64_t* __restrict__ block_sums) {
+  __shared__ int64_t sh[SCAN_BLOCK];
+  const int64_t i = (int64_t)blockIdx.x * SCAN_BLOCK + threadIdx.x;
+  int64_t v = i < n ? in[i] : 0;
+  int64_t tot;
+  int64_t ex = block_exclusive_scan(v, sh, &tot);
+  if (i < n) out[i] = ex;
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
+}
+
+// Serial-in-blocks scan of the block sums by one workgroup.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_sums(int64_t* __restrict__ sums, int64_t nb) {
+  __shared__ int64_t sh[SCAN_BLOCK];
+  int64_t carry = 0;
+  for (int64_t b0 = 0; b0 < nb; b0 += SCAN_BLOCK) {
+    const int64_t i = b0 + threadIdx.x;
+    int64_t v = i < nb ? sums[i] : 0;
+    int64_t tot;
+    int64_t ex = block_exclusive_scan(v, sh, &tot);
+    if (i < nb) sums[i] = ex + carry;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sums[nb] = carry;
+}
+
+__global__ void k_scan_add(int64_t* __restrict__ out, int64_t n, const int64_t* __restrict__ sums) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += sums[i / SCAN_BLOCK];
+  if (i == 0) out[n] = sums[(n + SCAN_BLOCK - 1) / SCAN_BLOCK];
+}
+
+// Fixed-order sum: each block reduces a contiguous slice in a fixed tree.
+__global__ __launch_bounds__(256) void k_sum_partial(const double* __restrict__ x, int64_t n, double* __restrict__ part) {
+  __shared__ double sh[256];
+  const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t b = (int64_t)blockIdx.x * per, e = b + per < n ? b + per : n;
+  double acc = 0.0;
+  for (int64_t i = b + threadIdx.x; i < e; i += 256) acc += x[i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+}  // namespace
+
+int tgx_zero(tgx_ctx* ctx, void* p, size_t bytes) {
+  if (bytes == 0) return TGX_OK;
+  if ((bytes & 3) == 0 && ((uintptr_t)p & 3) == 0) {
+    size_t w = bytes / 4;
+    unsigned g = (unsigned)((w + 255) / 256);
+    if (g > 65536) g = 65536;
+    TGX_LAUNCH(ctx, "fill_zero", k_fill_zero, dim3(g), dim3(256), 0, (uint32_t*)p, w);
+  } else {
+    TGX_LAUNCH(ctx, "fill_zero_bytes", k_fill_zero_bytes, dim3((unsigned)((bytes + 255) / 256)), dim3(256), 0,
+               (uint8_t*)p, bytes);
+  }
+  return TGX_OK;
+}
+
+int tgx_scan_i32(tgx_ctx* ctx, const int* in, int64_t* out, int64_t n, int64_t* total) {
+  const int64_t nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+  TGX_TRY(ctx->scan_tmp.ensure(sizeof(int64_t) * (size_t)(nb + 2)));
+  int64_t* sums = ctx->scan_tmp.as<int64_t>();
+  if (n > 0) {
+    TGX_LAUNCH(ctx, "scan_local", k_scan_local, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, in, out, n, sums);
+    TGX_LAUNCH(ctx, "scan_sums", k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, sums, nb);
+    TGX_LAUNCH(ctx, "scan_add", k_scan_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, out, n, sums);
+  } else {
+    TGX_TRY(tgx_zero(ctx, out, sizeof(int64_t)));
+  }
+  if (total) {
+    TGX_CHECK_HIP(hipMemcpyAsync(total, out + n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return TGX_OK;
+}
+
+int tgx_device_sum(tgx_ctx* ctx, const double* d_x, int64_t n, double* out_host) {
+  const int nbk = 256;
+  TGX_TRY(ctx->red.ensure(sizeof(double) * nbk));
+  TGX_LAUNCH(ctx, "sum_partial", k_sum_partial, dim3(nbk), dim3(256), 0, d_x, n, ctx->red.as<double>());
+  double part[nbk];
+  TGX_CHECK_HIP(hipMemcpyAsync(part, ctx->red.p, sizeof(part), hipMemcpyDeviceToHost, ctx->stream));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  double s = 0.0;
+  for (int i = 0; i < nbk; ++i) s += part[i];
+  *out_host = s;
+  return TGX_OK;
+}
