@@ -152,38 +152,50 @@ int tgx_get_results(tgx_ctx* ctx, double* energies, double* forces);
 /* Device pointer of the resident forces (3, n_sites) and energies (n_sites). */
 int tgx_results_device(tgx_ctx* ctx, double** d_forces, double** d_energies);
 
-/* ---- surrogate ----------------------------------------------------------- */
-/* One MLP per model (dim -> h -> h -> 1, SiLU), weights given in FP32 as
- * W1(h, dim) b1(h) W2(h, h) b2(h) w3(h) b3(1), row-major, plus the descriptor
- * shift/scale it was trained with. Stored as BF16 for the GEMMs. */
-int tgx_surrogate_set(tgx_ctx* ctx, int model_id, int member, int hidden, const float* W1, const float* b1,
-                      const float* W2, const float* b2, const float* w3, const float* b3, double e_shift,
-                      double e_scale);
-/* Ensemble gate: members > 1 per model, spread threshold in eV/atom; atoms over
- * it are recomputed by the FP64 GAP path. Writes the gated fraction. */
-int tgx_surrogate_gate(tgx_ctx* ctx, int n_members, double threshold);
-int tgx_surrogate_stats(tgx_ctx* ctx, double* gated_fraction, double* max_spread);
-
-/* ---- MLP training kernels (device pointers) ------------------------------ */
-/* Train state for one network; see turbogap_amd/surrogate.py for the loss. */
+/* ---- surrogate (no counterpart in the reference) ------------------------- */
+/* A per-block MLP, dim -> hidden -> hidden -> 1 with SiLU, trained by the
+ * library's own kernels on (q, E_i, dE_i/dq) samples that an FP64 tgx_compute
+ * produced, and run with BF16 weights and FP32 accumulation by
+ * tgx_compute(TGX_PATH_SURROGATE). Forces go through the same adjoint kernel as
+ * the GAP's with Qss = -dE/dq. */
 typedef struct tgx_mlp tgx_mlp;
+/* hidden and max_batch must be multiples of 128. */
 int tgx_mlp_create(tgx_ctx* ctx, int dim, int hidden, int max_batch, uint64_t seed, tgx_mlp** mlp);
 int tgx_mlp_destroy(tgx_mlp* mlp);
-/* One Adam step on a batch held in device memory: q(B, dim) FP32 descriptors,
- * e_ref(B) energies, g_ref(B, dim) FP32 descriptor-space energy gradients.
- * Returns the energy and gradient losses of the batch. */
-int tgx_mlp_train_step(tgx_mlp* mlp, int batch, const float* d_q, const float* d_e_ref, const float* d_g_ref,
-                       float lr, float w_energy, float w_grad, double* loss_e, double* loss_g);
-/* Forward (+ input gradient when d_g != NULL) on device data. */
-int tgx_mlp_forward(tgx_mlp* mlp, int batch, const float* d_q, float* d_e, float* d_g);
-/* Copy weights out (FP32 master copy) in tgx_surrogate_set's layout. */
+/* Append every stride-th centre (from offset) of the model's last FP64
+ * tgx_compute to the device-resident dataset; writes the dataset size. */
+int tgx_mlp_add_samples(tgx_mlp* mlp, int model_id, int stride, int offset, int* n_total);
+/* Mean and standard deviation of the energies of the first n_use samples. */
+int tgx_mlp_dataset_stats(tgx_mlp* mlp, int n_use, double* e_mean, double* e_std);
+/* Targets are trained as (E - e_shift)/e_scale and g/e_scale. */
+int tgx_mlp_set_normalization(tgx_mlp* mlp, double e_shift, double e_scale);
+int tgx_mlp_get_normalization(tgx_mlp* mlp, double* e_shift, double* e_scale);
+/* Adam (beta 0.9/0.999) over shuffled batches of samples [0, n_train):
+ * loss = w_energy mean (E - E_ref)^2 + w_grad mean |P (g - g_ref)|^2 with
+ * P = 1 - q q^T, in normalised units. Writes the last epoch's mean losses. */
+int tgx_mlp_train(tgx_mlp* mlp, int n_train, int epochs, int batch, float lr, float w_energy, float w_grad,
+                  uint64_t seed, double* loss_e, double* loss_g);
+/* Errors on samples [begin, end): energy MAE and RMSE (eV), RMS of |P (g - g_ref)| (eV). */
+int tgx_mlp_evaluate(tgx_mlp* mlp, int begin, int end, double* e_mae, double* e_rmse, double* g_rmse);
+/* Host-array forms for tests: q (batch, dim) FP32; outputs E (batch) and dE/dq (batch, dim). */
+int tgx_mlp_forward_host(tgx_mlp* mlp, int batch, const float* q, float* e, float* g);
+/* One batch's losses and weight gradients (batch a multiple of 128); targets
+ * in normalised units. Gradient layout as tgx_mlp_get_weights. */
+int tgx_mlp_gradients_host(tgx_mlp* mlp, int batch, const float* q, const float* e_ref, const float* g_ref,
+                           float w_energy, float w_grad, float* dW1, float* db1, float* dW2, float* db2,
+                           float* dw3, float* db3, double* loss_e, double* loss_g);
+/* FP32 master weights, row-major: W1 (hidden, dim), b1 (hidden), W2 (hidden,
+ * hidden), b2 (hidden), w3 (hidden), b3 (1). */
 int tgx_mlp_get_weights(tgx_mlp* mlp, float* W1, float* b1, float* W2, float* b2, float* w3, float* b3);
 int tgx_mlp_set_weights(tgx_mlp* mlp, const float* W1, const float* b1, const float* W2, const float* b2,
                         const float* w3, const float* b3);
-/* Descriptors (FP32) and GAP descriptor-space gradients of the last tgx_compute
- * for one model, device pointers valid until the next compute. */
-int tgx_model_training_view(tgx_ctx* ctx, int model_id, int* n_sites, const float** d_q, const float** d_g,
-                            const double** d_e, const int** d_site_atom);
+/* Copy a network in as ensemble member `member` of model_id's surrogate. */
+int tgx_surrogate_install(tgx_ctx* ctx, int model_id, int member, tgx_mlp* mlp);
+/* Use n_members per model; centres whose member energies spread by more than
+ * threshold eV are re-evaluated by the FP64 GAP (energy and gradient). */
+int tgx_surrogate_gate(tgx_ctx* ctx, int n_members, double threshold);
+/* Of the last surrogate compute: fraction of centres gated, largest spread (eV). */
+int tgx_surrogate_stats(tgx_ctx* ctx, double* gated_fraction, double* max_spread);
 
 /* ---- MD (src/md.f90:226-345, src/turbogap_md.f90:160-470) --------------- */
 /* Velocities (3, n_sites) and masses (n_sites) in eV fs^2/A^2 live on the device. */

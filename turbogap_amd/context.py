@@ -43,12 +43,16 @@ class Context:
         h = C.c_void_p()
         L.check(self._lib.tgx_init(device, C.byref(h)), "gpu_context_init")
         self.h = h
+        self._children = []  # objects (networks) that hold device state of this context
         self.models: Dict[int, SoapTurbo] = {}
         self.n_sites = 0
         self.n_sites_sc = 0
 
     # -- lifetime --------------------------------------------------------------
     def close(self):
+        for c in self._children:
+            c.close()
+        self._children = []
         if self.h is not None and self.h.value:
             self._lib.tgx_finalize(self.h)
         self.h = None

@@ -99,7 +99,7 @@ extern "C" int tgx_init(int device, tgx_ctx** out) {
 }
 
 static void free_model(tgx_model_t* m) {
-  dbuf* bufs[] = {&m->B, &m->xq, &m->cen, &m->dtab, &m->Qs, &m->QsT, &m->alpha, &m->Qs_bf, &m->QsT_bf, &m->alpha_f,
+  dbuf* bufs[] = {&m->B, &m->xq, &m->cen, &m->dtab, &m->ynorm, &m->Qs, &m->QsT, &m->alpha, &m->Qs_bf, &m->QsT_bf, &m->alpha_f,
                   &m->site_atom, &m->site_beg, &m->pr_gidx, &m->pr_nbr, &m->pr_spec, &m->pr_r, &m->pr_xyz, &m->q64,
                   &m->q32, &m->qbf, &m->pnorm, &m->e_site, &m->qss64, &m->qss32, &m->kd64, &m->kdbf, &m->epart,
                   &m->soap_der, &m->fpair};
@@ -110,10 +110,13 @@ static void free_model(tgx_model_t* m) {
   delete m;
 }
 
+void tgx_surrogate_release(tgx_ctx* ctx);
+
 extern "C" int tgx_finalize(tgx_ctx* ctx) {
   if (!ctx) return TGX_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  tgx_surrogate_release(ctx);
   for (auto* m : ctx->models) free_model(m);
   dbuf* bufs[] = {&ctx->pos, &ctx->species_sc, &ctx->n_neigh, &ctx->nl_off, &ctx->nlist, &ctx->rjs, &ctx->xyz,
                   &ctx->thetas, &ctx->phis, &ctx->nspec, &ctx->rev, &ctx->cell_of, &ctx->cell_cnt, &ctx->cell_start,
@@ -318,6 +321,20 @@ extern "C" int tgx_soap_model_create(tgx_ctx* ctx, const tgx_soap_hypers* h, con
   }
   std::vector<double> B, xq, cen;
   tgx_build_radial_tables(m, B, xq, cen);
+  // constants of the real spherical harmonics, lm = l*l + l + m
+  std::vector<double> ynorm((size_t)(m->l_max + 1) * (m->l_max + 1));
+  for (int l = 0; l <= m->l_max; ++l)
+    for (int mm = -l; mm <= l; ++mm) {
+      const int am = mm < 0 ? -mm : mm;
+      double ratio = 1.0;
+      for (int k = l - am + 1; k <= l + am; ++k) ratio /= (double)k;
+      double nrm = std::sqrt((2.0 * l + 1.0) / (4.0 * M_PI) * ratio);
+      double df = 1.0;
+      for (int k = 1; k <= 2 * am - 1; k += 2) df *= (double)k;
+      nrm *= df * ((am & 1) ? -1.0 : 1.0);
+      if (am != 0) nrm *= M_SQRT2;
+      ynorm[(size_t)l * l + l + mm] = nrm;
+    }
   std::vector<int> dtab;
   for (int n = 0; n < N; ++n)
     for (int np = n; np < N; ++np)
@@ -346,6 +363,7 @@ extern "C" int tgx_soap_model_create(tgx_ctx* ctx, const tgx_soap_hypers* h, con
   if ((st = up(m->B, B.data(), sizeof(double) * B.size())) || (st = up(m->xq, xq.data(), sizeof(double) * xq.size())) ||
       (st = up(m->cen, cen.data(), sizeof(double) * cen.size())) ||
       (st = up(m->dtab, dtab.data(), sizeof(int) * dtab.size())) ||
+      (st = up(m->ynorm, ynorm.data(), sizeof(double) * ynorm.size())) ||
       (st = up(m->Qs, qs.data(), sizeof(double) * qs.size())) ||
       (st = up(m->QsT, qst.data(), sizeof(double) * qst.size())) ||
       (st = up(m->alpha, al.data(), sizeof(double) * al.size())) ||
@@ -361,6 +379,7 @@ extern "C" int tgx_soap_model_create(tgx_ctx* ctx, const tgx_soap_hypers* h, con
   P.xq = m->xq.as<double>();
   P.cen = m->cen.as<double>();
   P.dtab = m->dtab.as<int>();
+  P.ynorm = m->ynorm.as<double>();
   ctx->models.push_back(m);
   *model_id = (int)ctx->models.size() - 1;
   return TGX_OK;

@@ -25,7 +25,7 @@ constexpr int BM = 128, BN = 128;
 enum { EPI_STORE = 0, EPI_GAP = 1 };
 
 struct EpiArgs {
-  double pref;           // -zeta delta^2
+  double pref;           // -zeta delta^2 (EPI_GAP) / output scale (EPI_STORE)
   double zeta;
   int zeta_int;          // >= 0 when zeta is an integer
   const double* alpha;   // [N] (FP64 path)
@@ -49,9 +49,9 @@ __device__ __forceinline__ void tile_of(int& tm, int& tn) {
   tn = tile - tm * ntn;
 }
 
-template <class T>
+template <class T, bool ZINT>
 __device__ __forceinline__ void kpow(T k, double zeta, int zi, T& kz, T& kzm1) {
-  if (zi >= 1) {
+  if (ZINT) {
     T p = 1;
     for (int i = 0; i < zi - 1; ++i) p *= k;
     kzm1 = p;
@@ -71,7 +71,7 @@ template <bool F64> __device__ __forceinline__ int tile_row(int lane, int r) {
   return F64 ? 4 * r + (lane >> 4) : 4 * (lane >> 4) + r;
 }
 
-template <int EPI, class OutT, class AccT, class VecT, bool F64>
+template <int EPI, class OutT, class AccT, class VecT, bool F64, bool ZINT>
 __device__ void epilogue(const VecT (&acc)[4][4], int m0, int n0, int tn, int wm, int wn, const EpiArgs& e,
                          double* sE) {
   const int lane = threadIdx.x & 63;
@@ -85,45 +85,36 @@ __device__ void epilogue(const VecT (&acc)[4][4], int m0, int n0, int tn, int wm
       for (int tj = 0; tj < 4; ++tj)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          out[(size_t)(rbase + ti * 16 + tile_row<F64>(lane, r)) * e.ldo + cbase + tj * 16] = (OutT)acc[ti][tj][r];
+          out[(size_t)(rbase + ti * 16 + tile_row<F64>(lane, r)) * e.ldo + cbase + tj * 16] =
+              (OutT)((AccT)e.pref * acc[ti][tj][r]);
     return;
   }
-  double erow[4][4];
+  // one 16-row slab at a time keeps only four row partials live
 #pragma unroll
-  for (int ti = 0; ti < 4; ++ti)
+  for (int ti = 0; ti < 4; ++ti) {
+    double er[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int r = 0; r < 4; ++r) erow[ti][r] = 0.0;
-#pragma unroll
-  for (int tj = 0; tj < 4; ++tj) {
-    const int col = cbase + tj * 16;
-    const AccT al = e.alpha_f ? (AccT)e.alpha_f[col] : (AccT)e.alpha[col];
-#pragma unroll
-    for (int ti = 0; ti < 4; ++ti)
+    for (int tj = 0; tj < 4; ++tj) {
+      const int col = cbase + tj * 16;
+      const AccT al = e.alpha_f ? (AccT)e.alpha_f[col] : (AccT)e.alpha[col];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         AccT kz, kzm1;
-        kpow<AccT>(acc[ti][tj][r], e.zeta, e.zeta_int, kz, kzm1);
+        kpow<AccT, ZINT>(acc[ti][tj][r], e.zeta, e.zeta_int, kz, kzm1);
         out[(size_t)(rbase + ti * 16 + tile_row<F64>(lane, r)) * e.ldo + col] = (OutT)((AccT)e.pref * kzm1 * al);
-        erow[ti][r] += (double)(al * kz);
+        er[r] += (double)(al * kz);
       }
-  }
-  // reduce over the 16 lanes that share rows, then over the two column waves
-#pragma unroll
-  for (int ti = 0; ti < 4; ++ti)
+    }
+    // reduce over the 16 lanes that share rows
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      double v = erow[ti][r];
+      double v = er[r];
       v += __shfl_xor(v, 1);
       v += __shfl_xor(v, 2);
       v += __shfl_xor(v, 4);
       v += __shfl_xor(v, 8);
-      erow[ti][r] = v;
+      if ((lane & 15) == 0) sE[wn * BM + wm * 64 + ti * 16 + tile_row<F64>(lane, r)] = v;
     }
-  if ((lane & 15) == 0) {
-#pragma unroll
-    for (int ti = 0; ti < 4; ++ti)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) sE[wn * BM + wm * 64 + ti * 16 + tile_row<F64>(lane, r)] = erow[ti][r];
   }
   __syncthreads();
   if (threadIdx.x < BM) {
@@ -133,7 +124,7 @@ __device__ void epilogue(const VecT (&acc)[4][4], int m0, int n0, int tn, int wm
 }
 
 // FP64 NT GEMM. M, N multiples of 128; K multiple of 16.
-template <int EPI>
+template <int EPI, bool ZINT>
 __global__ __launch_bounds__(256) void k_gemm_f64(int K, const double* __restrict__ A, int lda,
                                                   const double* __restrict__ B, int ldb, EpiArgs e) {
   constexpr int BK = 16, LDS_STRIDE = BM + 4;
@@ -183,11 +174,11 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int K, const double* __restric
         for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
   }
-  epilogue<EPI, double, double, f64x4, true>(acc, m0, n0, tn, wm, wn, e, sE);
+  epilogue<EPI, double, double, f64x4, true, ZINT>(acc, m0, n0, tn, wm, wn, e, sE);
 }
 
 // BF16 NT GEMM with FP32 accumulation. M, N multiples of 128; K multiple of 64.
-template <int EPI, class OutT>
+template <int EPI, class OutT, bool ZINT>
 __global__ __launch_bounds__(256) void k_gemm_bf16(int K, const __bf16* __restrict__ A, int lda,
                                                    const __bf16* __restrict__ B, int ldb, EpiArgs e) {
   constexpr int BK = 64, LDS_STRIDE = BK + 8;
@@ -236,7 +227,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int K, const __bf16* __restri
         for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
   }
-  epilogue<EPI, OutT, float, f32x4, false>(acc, m0, n0, tn, wm, wn, e, sE);
+  epilogue<EPI, OutT, float, f32x4, false, ZINT>(acc, m0, n0, tn, wm, wn, e, sE);
 }
 
 // E_i = delta^2 sum_t epart[i][t] + e0, the partials summed in tile order.
@@ -271,8 +262,12 @@ static int launch_f64(tgx_ctx* ctx, int epi, int M, int N, int K, const double* 
                       EpiArgs e, const char* name) {
   TGX_REQUIRE(M % BM == 0 && N % BN == 0 && K % 16 == 0, "gemm_f64: unpadded shape %d x %d x %d", M, N, K);
   dim3 grid(N / BN, M / BM);
-  if (epi == EPI_GAP) TGX_LAUNCH(ctx, name, k_gemm_f64<EPI_GAP>, grid, dim3(256), 0, K, A, lda, B, ldb, e);
-  else TGX_LAUNCH(ctx, name, k_gemm_f64<EPI_STORE>, grid, dim3(256), 0, K, A, lda, B, ldb, e);
+  if (epi == EPI_GAP && e.zeta_int >= 1)
+    TGX_LAUNCH(ctx, name, (k_gemm_f64<EPI_GAP, true>), grid, dim3(256), 0, K, A, lda, B, ldb, e);
+  else if (epi == EPI_GAP)
+    TGX_LAUNCH(ctx, name, (k_gemm_f64<EPI_GAP, false>), grid, dim3(256), 0, K, A, lda, B, ldb, e);
+  else
+    TGX_LAUNCH(ctx, name, (k_gemm_f64<EPI_STORE, true>), grid, dim3(256), 0, K, A, lda, B, ldb, e);
   return TGX_OK;
 }
 
@@ -302,13 +297,18 @@ int tgx_gap_contract(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces) {
     e.out = m->kdbf.p;
     e.ldo = Sp;
     TGX_REQUIRE(Dp % 64 == 0 && Sp % 64 == 0, "gemm_bf16: unpadded shape");
-    TGX_LAUNCH(ctx, "gemm_kernel_bf16", (k_gemm_bf16<EPI_GAP, __bf16>), dim3(Sp / BN, Mp / BM), dim3(256), 0, Dp,
-               m->qbf.as<__bf16>(), Dp, m->Qs_bf.as<__bf16>(), Dp, e);
+    if (e.zeta_int >= 1)
+      TGX_LAUNCH(ctx, "gemm_kernel_bf16", (k_gemm_bf16<EPI_GAP, __bf16, true>), dim3(Sp / BN, Mp / BM), dim3(256), 0,
+                 Dp, m->qbf.as<__bf16>(), Dp, m->Qs_bf.as<__bf16>(), Dp, e);
+    else
+      TGX_LAUNCH(ctx, "gemm_kernel_bf16", (k_gemm_bf16<EPI_GAP, __bf16, false>), dim3(Sp / BN, Mp / BM), dim3(256), 0,
+                 Dp, m->qbf.as<__bf16>(), Dp, m->Qs_bf.as<__bf16>(), Dp, e);
   }
   TGX_LAUNCH(ctx, "energy_finalize", k_energy_finalize, dim3((ns + 255) / 256), dim3(256), 0, ns,
              m->epart.as<double>(), n_tiles, m->delta * m->delta, m->e0, m->e_site.as<double>());
   if (!do_forces) return TGX_OK;
   EpiArgs s{};
+  s.pref = 1.0;
   if (path == TGX_PATH_FP64_PLAIN || path == TGX_PATH_FP64_ADJOINT) {
     TGX_TRY(m->qss64.ensure(sizeof(double) * (size_t)Mp * Dp));
     s.out = m->qss64.p;
@@ -319,8 +319,27 @@ int tgx_gap_contract(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces) {
     TGX_TRY(m->qss32.ensure(sizeof(float) * (size_t)Mp * Dp));
     s.out = m->qss32.p;
     s.ldo = Dp;
-    TGX_LAUNCH(ctx, "gemm_qss_bf16", (k_gemm_bf16<EPI_STORE, float>), dim3(Dp / BN, Mp / BM), dim3(256), 0, Sp,
+    TGX_LAUNCH(ctx, "gemm_qss_bf16", (k_gemm_bf16<EPI_STORE, float, true>), dim3(Dp / BN, Mp / BM), dim3(256), 0, Sp,
                m->kdbf.as<__bf16>(), Sp, m->QsT_bf.as<__bf16>(), Sp, s);
   }
+  return TGX_OK;
+}
+
+// C[M][N] (FP32) = scale * A[M][K] . B[N][K]^T in BF16 with FP32 accumulation.
+// M, N multiples of 128, K a multiple of 64, leading dimensions in elements.
+int tgx_gemm_bf16_nt(tgx_ctx* ctx, const char* name, int M, int N, int K, const __bf16* A, int lda, const __bf16* B,
+                     int ldb, float* C, int ldc, double scale) {
+  TGX_REQUIRE(M % BM == 0 && N % BN == 0 && K % 64 == 0 && lda % 8 == 0 && ldb % 8 == 0,
+              "gemm_bf16_nt: unpadded shape %d x %d x %d", M, N, K);
+  EpiArgs s{};
+  s.pref = scale;
+  s.out = C;
+  s.ldo = ldc;
+  TGX_LAUNCH(ctx, name, (k_gemm_bf16<EPI_STORE, float, true>), dim3(N / BN, M / BM), dim3(256), 0, K, A, lda, B, ldb, s);
+  return TGX_OK;
+}
+
+int tgx_f32_to_bf16(tgx_ctx* ctx, const float* x, __bf16* y, int64_t n) {
+  TGX_LAUNCH(ctx, "to_bf16", k_f32_to_bf16, dim3(1024), dim3(256), 0, n, x, y);
   return TGX_OK;
 }

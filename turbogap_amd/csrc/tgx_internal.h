@@ -101,6 +101,7 @@ struct SoapParams {
   const double* xq;   // [species][TGX_MAX_Q]
   const double* cen;  // [species][TGX_MAX_NB] central-atom coefficients (weights and Y00 folded in)
   const int* dtab;    // [D] packed n | n' << 8 | l << 16
+  const double* ynorm;  // [(L+1)^2] real spherical harmonic constants
 };
 
 struct tgx_model_t {
@@ -115,7 +116,7 @@ struct tgx_model_t {
   double rcut_max = 0.0;
   SoapParams P{};
   // device copies of the tables
-  dbuf B, xq, cen, dtab;
+  dbuf B, xq, cen, dtab, ynorm;
   // sparse set: Qs[Spad][Dpad], QsT[Dpad][Spad], alphas[Spad] (FP64) and BF16 copies
   dbuf Qs, QsT, alpha, Qs_bf, QsT_bf, alpha_f;
   // surrogate members (BF16 weights + FP32 biases)
@@ -210,3 +211,6 @@ int tgx_gather_forces(tgx_ctx* ctx, double* energy_out, double* virial_out);
 int tgx_zero(tgx_ctx* ctx, void* p, size_t bytes);
 int tgx_surrogate_energy_grad(tgx_ctx* ctx, tgx_model_t* m);
 int tgx_device_sum(tgx_ctx* ctx, const double* d_x, int64_t n, double* out_host);
+int tgx_gemm_bf16_nt(tgx_ctx* ctx, const char* name, int M, int N, int K, const __bf16* A, int lda, const __bf16* B,
+                     int ldb, float* C, int ldc, double scale);
+int tgx_f32_to_bf16(tgx_ctx* ctx, const float* x, __bf16* y, int64_t n);

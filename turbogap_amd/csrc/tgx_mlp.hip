@@ -1,27 +1,948 @@
-// Surrogate and MLP training entry points (filled in below).
-#include "tgx_internal.h"
+// MLP surrogate of the per-site GAP energy: training and inference kernels.
+//
+// Network per soap_turbo block: q (D) -> W1 (H x D) -> SiLU -> W2 (H x H) ->
+// SiLU -> w3 (H) -> E. Weights are kept as FP32 masters (Adam) and used as
+// BF16 in every matrix product, with FP32 accumulation (v_mfma_f32_16x16x32_bf16
+// through tgx_gemm_bf16_nt). Targets are normalised, E_n = (E - shift)/scale.
+//
+// Loss of a batch of B environments:
+//   L = w_E mean (E_hat - E_n)^2 + w_G mean |P (g_hat - g_n)|^2
+// where g = dE/dq is the descriptor-space gradient (the GAP's -Qss), and
+// P = 1 - q q^T removes the component along q that no force sees (dq/dr is
+// orthogonal to q for a normalised descriptor). The gradient term needs
+// d/dtheta of u . g_hat(q) with u = P(g_hat - g_n) held fixed: a forward-mode
+// pass along u through the network followed by reverse mode (double
+// backpropagation), written out below with SiLU's first and second derivatives.
+//
+// Forces of the surrogate come from the SOAP adjoint kernel with
+// Qss = -dE_hat/dq, the same route as the GAP's.
+#include <cmath>
+#include <algorithm>
+#include <numeric>
 
-int tgx_surrogate_run(tgx_ctx* ctx, tgx_model_t* m) {
-  (void)ctx; (void)m;
-  tgx_set_error("surrogate path not built yet");
-  return TGX_ERR_UNSUPPORTED;
+#include "tgx_internal.h"
+#include "tgx_util.h"
+
+int tgx_soap_descriptors_sites(tgx_ctx* ctx, tgx_model_t* m, const int* d_sites, int n_list, double* q_out,
+                               double* pnorm_out);
+
+namespace {
+
+__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
+__device__ __forceinline__ float silu1(float z) {
+  const float s = sigm(z);
+  return s * (1.0f + z * (1.0f - s));
+}
+__device__ __forceinline__ float silu2(float z) {
+  const float s = sigm(z);
+  return s * (1.0f - s) * (2.0f + z * (1.0f - 2.0f * s));
 }
 
-#define TGX_NOT_YET(sig)                                           \
-  extern "C" int sig {                                             \
-    tgx_set_error("%s: not available in this build", __func__);    \
-    return TGX_ERR_UNSUPPORTED;                                    \
+// layer 1: z1 += b1; h1 = silu(z1) (bf16 and fp32); s1 = silu'(z1); s1pp = silu''(z1)
+__global__ void k_l1(int64_t n, int H, float* __restrict__ z1, const float* __restrict__ b1, __bf16* __restrict__ h1b,
+                     float* __restrict__ h1, float* __restrict__ s1, float* __restrict__ s1pp) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float z = z1[i] + b1[i % H];
+  z1[i] = z;
+  const float h = silu(z);
+  h1b[i] = (__bf16)h;
+  if (h1) h1[i] = h;
+  s1[i] = silu1(z);
+  if (s1pp) s1pp[i] = silu2(z);
+}
+
+// layer 2 and output, one wavefront per row: z2 += b2, E = w3 . silu(z2) + b3,
+// a2 = w3 * silu'(z2); optional training copies.
+__global__ __launch_bounds__(256) void k_l2(int rows, int H, float* __restrict__ z2, const float* __restrict__ b2,
+                                            const float* __restrict__ w3, const float* __restrict__ b3,
+                                            float* __restrict__ E, __bf16* __restrict__ a2b, float* __restrict__ a2,
+                                            float* __restrict__ h2, float* __restrict__ s2p,
+                                            float* __restrict__ s2pp) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= rows) return;
+  float acc = 0.f;
+  for (int j = lane; j < H; j += 64) {
+    const size_t k = (size_t)row * H + j;
+    const float z = z2[k] + b2[j];
+    z2[k] = z;
+    const float h = silu(z), d1 = silu1(z);
+    acc += w3[j] * h;
+    const float a = w3[j] * d1;
+    a2b[k] = (__bf16)a;
+    if (a2) a2[k] = a;
+    if (h2) h2[k] = h;
+    if (s2p) s2p[k] = d1;
+    if (s2pp) s2pp[k] = silu2(z);
   }
-TGX_NOT_YET(tgx_surrogate_set(tgx_ctx*, int, int, int, const float*, const float*, const float*, const float*,
-                              const float*, const float*, double, double))
-TGX_NOT_YET(tgx_surrogate_gate(tgx_ctx*, int, double))
-TGX_NOT_YET(tgx_surrogate_stats(tgx_ctx*, double*, double*))
-TGX_NOT_YET(tgx_mlp_create(tgx_ctx*, int, int, int, uint64_t, tgx_mlp**))
-TGX_NOT_YET(tgx_mlp_destroy(tgx_mlp*))
-TGX_NOT_YET(tgx_mlp_train_step(tgx_mlp*, int, const float*, const float*, const float*, float, float, float, double*,
-                               double*))
-TGX_NOT_YET(tgx_mlp_forward(tgx_mlp*, int, const float*, float*, float*))
-TGX_NOT_YET(tgx_mlp_get_weights(tgx_mlp*, float*, float*, float*, float*, float*, float*))
-TGX_NOT_YET(tgx_mlp_set_weights(tgx_mlp*, const float*, const float*, const float*, const float*, const float*,
-                                const float*))
-TGX_NOT_YET(tgx_model_training_view(tgx_ctx*, int, int*, const float**, const float**, const double**, const int**))
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if (lane == 0) E[row] = acc + b3[0];
+}
+
+// a1 = v1 * silu'(z1)
+__global__ void k_a1(int64_t n, const float* __restrict__ v1, const float* __restrict__ s1, float* __restrict__ a1,
+                     __bf16* __restrict__ a1b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float a = v1[i] * s1[i];
+  if (a1) a1[i] = a;
+  a1b[i] = (__bf16)a;
+}
+
+// Per row: residuals of the normalised targets, u = P(g - g_n), loss partials,
+// dE = 2 w_E (E - E_n)/B. One wavefront per row.
+__global__ __launch_bounds__(256) void k_loss(int rows, int Dp, const float* __restrict__ E, const double* __restrict__ e_ref,
+                                              const float* __restrict__ g, const float* __restrict__ g_ref,
+                                              const float* __restrict__ q, double shift, double inv_scale, float w_e,
+                                              float* __restrict__ u, __bf16* __restrict__ ub, float* __restrict__ dE,
+                                              double* __restrict__ le, double* __restrict__ lg) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= rows) return;
+  const float* gr = g + (size_t)row * Dp;
+  const float* gf = g_ref + (size_t)row * Dp;
+  const float* qr = q + (size_t)row * Dp;
+  float qd = 0.f;
+  for (int j = lane; j < Dp; j += 64) qd += qr[j] * (gr[j] - (float)(gf[j] * inv_scale));
+  for (int o = 32; o > 0; o >>= 1) qd += __shfl_xor(qd, o);
+  float uu = 0.f;
+  for (int j = lane; j < Dp; j += 64) {
+    const size_t k = (size_t)row * Dp + j;
+    const float v = (gr[j] - (float)(gf[j] * inv_scale)) - qr[j] * qd;
+    u[k] = v;
+    ub[k] = (__bf16)v;
+    uu += v * v;
+  }
+  for (int o = 32; o > 0; o >>= 1) uu += __shfl_xor(uu, o);
+  if (lane == 0) {
+    const double r = (double)E[row] - (e_ref[row] - shift) * inv_scale;
+    dE[row] = (float)(2.0 * w_e * r / rows);
+    le[row] = r * r;
+    lg[row] = uu;
+  }
+}
+
+// tau1 = silu'(z1) * tau0
+__global__ void k_t1(int64_t n, const float* __restrict__ s1, const float* __restrict__ t0, float* __restrict__ t1,
+                     __bf16* __restrict__ t1b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float t = s1[i] * t0[i];
+  t1[i] = t;
+  t1b[i] = (__bf16)t;
+}
+
+// gamma2 = silu'(z2) tau2, zeta2 = w3 tau2 silu''(z2)
+__global__ void k_z2(int64_t n, int H, const float* __restrict__ t2, const float* __restrict__ s2p,
+                     const float* __restrict__ s2pp, const float* __restrict__ w3, float* __restrict__ gam2,
+                     float* __restrict__ zet2, __bf16* __restrict__ zet2b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  gam2[i] = s2p[i] * t2[i];
+  const float z = w3[i % H] * t2[i] * s2pp[i];
+  zet2[i] = z;
+  zet2b[i] = (__bf16)z;
+}
+
+// zeta1 = v1 tau0 silu''(z1) + (W2^T zeta2) silu'(z1)
+__global__ void k_z1(int64_t n, const float* __restrict__ v1, const float* __restrict__ t0,
+                     const float* __restrict__ s1pp, const float* __restrict__ wz, const float* __restrict__ s1,
+                     float* __restrict__ zet1) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  zet1[i] = v1[i] * t0[i] * s1pp[i] + wz[i] * s1[i];
+}
+
+// Y = dE_row * X + c * U (row-scaled combination for the weight gradients)
+__global__ void k_rowcomb(int64_t n, int cols, const float* __restrict__ dE, const float* __restrict__ X, float c,
+                          const float* __restrict__ U, float* __restrict__ Y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Y[i] = dE[i / cols] * X[i] + c * U[i];
+}
+
+// out[j] = sum_b (dE_b X[b][j] + c Y[b][j]); one thread per column, rows in order.
+__global__ void k_colsum(int rows, int cols, const float* __restrict__ dE, const float* __restrict__ X, float c,
+                         const float* __restrict__ Y, float* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cols) return;
+  double acc = 0.0;
+  for (int b = 0; b < rows; ++b) {
+    double v = dE ? (double)dE[b] * (X ? X[(size_t)b * cols + j] : 1.0f) : 0.0;
+    if (Y) v += (double)c * Y[(size_t)b * cols + j];
+    acc += v;
+  }
+  out[j] = (float)acc;
+}
+
+// C[m][n] = sum_b X1[b][m] Y1[b][n] + c2 sum_b X2[b][m] Y2[b][n] (FP32), a 64x64 tile
+// per 256-thread block, 4x4 outputs per thread, the batch staged 16 rows at a time.
+__global__ __launch_bounds__(256) void k_tn_gemm(int rows, int Mc, int Nc, const float* __restrict__ X1, const float* __restrict__ Y1,
+                                                 const float* __restrict__ X2, const float* __restrict__ Y2, float c2,
+                                                 float* __restrict__ C) {
+  __shared__ float sx[16][64 + 4];
+  __shared__ float sy[16][64 + 4];
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  float acc[4][4] = {};
+  for (int pass = 0; pass < (X2 ? 2 : 1); ++pass) {
+    const float* X = pass ? X2 : X1;
+    const float* Y = pass ? Y2 : Y1;
+    const float cc = pass ? c2 : 1.0f;
+    for (int b0 = 0; b0 < rows; b0 += 16) {
+      __syncthreads();
+      for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+        const int rb = e >> 6, cidx = e & 63;
+        const int b = b0 + rb;
+        sx[rb][cidx] = b < rows ? cc * X[(size_t)b * Mc + m0 + cidx] : 0.f;
+        sy[rb][cidx] = b < rows ? Y[(size_t)b * Nc + n0 + cidx] : 0.f;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        float xv[4], yv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) { xv[i] = sx[k][ty * 4 + i]; yv[i] = sy[k][tx * 4 + i]; }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] += xv[i] * yv[j];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) C[(size_t)(m0 + ty * 4 + i) * Nc + n0 + tx * 4 + j] = acc[i][j];
+}
+
+__global__ void k_adam(int64_t n, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, float lr, float b1, float b2, float c1, float c2, float eps) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// BF16 copies of W1 [H][Dp], W1^T [Dp][H], W2 [H][H], W2^T [H][H].
+__global__ void k_pack(int H, int Dp, const float* __restrict__ W1, const float* __restrict__ W2, __bf16* __restrict__ W1b,
+                       __bf16* __restrict__ W1Tb, __bf16* __restrict__ W2b, __bf16* __restrict__ W2Tb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n1 = (int64_t)H * Dp;
+  if (i < n1) {
+    const int h = (int)(i / Dp), d = (int)(i % Dp);
+    const __bf16 v = (__bf16)W1[i];
+    W1b[i] = v;
+    W1Tb[(size_t)d * H + h] = v;
+  } else if (i < n1 + (int64_t)H * H) {
+    const int64_t k = i - n1;
+    const int a = (int)(k / H), b = (int)(k % H);
+    const __bf16 v = (__bf16)W2[k];
+    W2b[k] = v;
+    W2Tb[(size_t)b * H + a] = v;
+  }
+}
+
+// Gather dataset rows into the batch buffers.
+__global__ void k_gather_rows(int rows, int Dp, const int* __restrict__ idx, const float* __restrict__ q,
+                              const float* __restrict__ g, const double* __restrict__ e, float* __restrict__ bq,
+                              __bf16* __restrict__ bqb, float* __restrict__ bg, double* __restrict__ be) {
+  const int r = blockIdx.x;
+  if (r >= rows) return;
+  const int src = idx[r];
+  for (int j = threadIdx.x; j < Dp; j += blockDim.x) {
+    const float v = q[(size_t)src * Dp + j];
+    bq[(size_t)r * Dp + j] = v;
+    bqb[(size_t)r * Dp + j] = (__bf16)v;
+    bg[(size_t)r * Dp + j] = g[(size_t)src * Dp + j];
+  }
+  if (threadIdx.x == 0) be[r] = e[src];
+}
+
+// Dataset rows from a model's FP64 results: q, e and g = dE/dq = -Qss.
+__global__ void k_append(int n_take, int stride, int offset, int Dp, const double* __restrict__ q64,
+                         const double* __restrict__ qss, const double* __restrict__ e_site, float* __restrict__ dq,
+                         float* __restrict__ dg, double* __restrict__ de) {
+  const int r = blockIdx.x;
+  if (r >= n_take) return;
+  const int s = offset + r * stride;
+  for (int j = threadIdx.x; j < Dp; j += blockDim.x) {
+    dq[(size_t)r * Dp + j] = (float)q64[(size_t)s * Dp + j];
+    dg[(size_t)r * Dp + j] = (float)(-qss[(size_t)s * Dp + j]);
+  }
+  if (threadIdx.x == 0) de[r] = e_site[s];
+}
+
+// Surrogate output: e_site = shift + scale * mean_k E_k, spread = max - min over
+// members, Qss = -(scale/K) sum_k G_k (G accumulated by the GEMMs).
+__global__ void k_surrogate_out(int n, int K, const float* __restrict__ Em, double shift, double scale,
+                                double* __restrict__ e_site, float* __restrict__ spread) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0, lo = 1e300, hi = -1e300;
+  for (int k = 0; k < K; ++k) {
+    const double v = (double)Em[(size_t)k * n + i];
+    s += v;
+    lo = v < lo ? v : lo;
+    hi = v > hi ? v : hi;
+  }
+  e_site[i] = shift + scale * s / K;
+  spread[i] = (float)(scale * (hi - lo));
+}
+
+__global__ void k_add_f32(int64_t n, const float* __restrict__ x, float* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += x[i];
+}
+
+// Gated sites take the FP64 GAP energy and gradient.
+__global__ void k_gate_replace(int ng, const int* __restrict__ gated, int Dp, const double* __restrict__ e_gap,
+                               const double* __restrict__ qss_gap, double* __restrict__ e_site,
+                               float* __restrict__ qss32) {
+  const int r = blockIdx.x;
+  if (r >= ng) return;
+  const int s = gated[r];
+  for (int j = threadIdx.x; j < Dp; j += blockDim.x) qss32[(size_t)s * Dp + j] = (float)qss_gap[(size_t)r * Dp + j];
+  if (threadIdx.x == 0) e_site[s] = e_gap[r];
+}
+
+inline dim3 g1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+}  // namespace
+
+struct tgx_mlp {
+  tgx_ctx* ctx = nullptr;
+  int D = 0, Dp = 0, H = 0, maxB = 0;
+  int64_t nparam = 0;
+  int64_t oW1 = 0, ob1 = 0, oW2 = 0, ob2 = 0, ow3 = 0, ob3 = 0;
+  dbuf theta, grad, mom, vel, W1b, W1Tb, W2b, W2Tb;
+  int64_t step = 0;
+  double shift = 0.0, scale = 1.0;
+  bool norm_set = false;
+  // dataset
+  int n = 0, cap = 0;
+  dbuf dq, dg, de;
+  // batch work space
+  dbuf idx, bq, bqb, bg, be, z1, h1, h1b, s1, s1pp, z2, h2, a2, a2b, s2p, s2pp, E, v1, a1, a1b, G, u, ub, t0, t1, t1b,
+      t2, gam2, zet2, zet2b, wz, zet1, dE, Y1, Y2, lossb;
+  uint64_t seed = 0;
+};
+
+static int mlp_workspace(tgx_mlp* m, int B) {
+  const size_t H = m->H, Dp = m->Dp, b = (size_t)B;
+  dbuf* f_bh[] = {&m->z1, &m->h1, &m->s1, &m->s1pp, &m->z2, &m->h2, &m->a2, &m->s2p, &m->s2pp, &m->v1, &m->a1,
+                  &m->t0, &m->t1, &m->t2, &m->gam2, &m->zet2, &m->wz, &m->zet1, &m->Y2};
+  for (dbuf* x : f_bh) TGX_TRY(x->ensure(sizeof(float) * b * H));
+  dbuf* h_bh[] = {&m->h1b, &m->a2b, &m->a1b, &m->t1b, &m->zet2b};
+  for (dbuf* x : h_bh) TGX_TRY(x->ensure(sizeof(__bf16) * b * H));
+  dbuf* f_bd[] = {&m->bq, &m->bg, &m->G, &m->u, &m->Y1};
+  for (dbuf* x : f_bd) TGX_TRY(x->ensure(sizeof(float) * b * Dp));
+  TGX_TRY(m->bqb.ensure(sizeof(__bf16) * b * Dp));
+  TGX_TRY(m->ub.ensure(sizeof(__bf16) * b * Dp));
+  TGX_TRY(m->be.ensure(sizeof(double) * b));
+  TGX_TRY(m->E.ensure(sizeof(float) * b));
+  TGX_TRY(m->dE.ensure(sizeof(float) * b));
+  TGX_TRY(m->lossb.ensure(sizeof(double) * 2 * b));
+  TGX_TRY(m->idx.ensure(sizeof(int) * b));
+  return TGX_OK;
+}
+
+static int mlp_pack(tgx_mlp* m) {
+  tgx_ctx* ctx = m->ctx;
+  const float* th = m->theta.as<float>();
+  const int64_t n = (int64_t)m->H * m->Dp + (int64_t)m->H * m->H;
+  TGX_LAUNCH(ctx, "mlp_pack", k_pack, g1(n), dim3(256), 0, m->H, m->Dp, th + m->oW1, th + m->oW2, m->W1b.as<__bf16>(),
+             m->W1Tb.as<__bf16>(), m->W2b.as<__bf16>(), m->W2Tb.as<__bf16>());
+  return TGX_OK;
+}
+
+extern "C" int tgx_mlp_create(tgx_ctx* ctx, int dim, int hidden, int max_batch, uint64_t seed, tgx_mlp** out) {
+  TGX_REQUIRE(ctx && out && dim > 0 && hidden > 0 && hidden % 128 == 0 && max_batch > 0 && max_batch % 128 == 0,
+              "tgx_mlp_create: need dim > 0, hidden and max_batch multiples of 128");
+  TGX_CHECK_HIP(hipSetDevice(ctx->device));
+  tgx_mlp* m = new tgx_mlp();
+  m->ctx = ctx;
+  m->D = dim;
+  m->Dp = (int)tgx_round_up(dim, 128);
+  m->H = hidden;
+  m->maxB = max_batch;
+  m->seed = seed;
+  const int64_t H = hidden, Dp = m->Dp;
+  m->oW1 = 0;
+  m->ob1 = m->oW1 + H * Dp;
+  m->oW2 = m->ob1 + H;
+  m->ob2 = m->oW2 + H * H;
+  m->ow3 = m->ob2 + H;
+  m->ob3 = m->ow3 + H;
+  m->nparam = tgx_round_up(m->ob3 + 1, 64);
+  auto fail = [&](int st) { delete m; return st; };
+  int st;
+  if ((st = m->theta.ensure(sizeof(float) * m->nparam)) || (st = m->grad.ensure(sizeof(float) * m->nparam)) ||
+      (st = m->mom.ensure(sizeof(float) * m->nparam)) || (st = m->vel.ensure(sizeof(float) * m->nparam)) ||
+      (st = m->W1b.ensure(sizeof(__bf16) * H * Dp)) || (st = m->W1Tb.ensure(sizeof(__bf16) * H * Dp)) ||
+      (st = m->W2b.ensure(sizeof(__bf16) * H * H)) || (st = m->W2Tb.ensure(sizeof(__bf16) * H * H)) ||
+      (st = mlp_workspace(m, max_batch)))
+    return fail(st);
+  // initial weights: W1 ~ N(0, 1) over the D real columns (|q| = 1), W2 ~ N(0, 1/H),
+  // w3 ~ N(0, 1/H), biases zero
+  std::vector<double> w((size_t)m->nparam, 0.0), tmp;
+  std::vector<double> r1((size_t)H * dim), r2((size_t)H * H), r3((size_t)H);
+  tgx_generate_normal((int)r1.size(), 1.0, seed, r1.data());
+  tgx_generate_normal((int)r2.size(), 1.0 / std::sqrt((double)H), seed + 1, r2.data());
+  tgx_generate_normal((int)r3.size(), 1.0 / std::sqrt((double)H), seed + 2, r3.data());
+  for (int64_t h = 0; h < H; ++h)
+    for (int d = 0; d < dim; ++d) w[m->oW1 + h * Dp + d] = r1[(size_t)h * dim + d];
+  for (int64_t k = 0; k < H * H; ++k) w[m->oW2 + k] = r2[(size_t)k];
+  for (int64_t k = 0; k < H; ++k) w[m->ow3 + k] = r3[(size_t)k];
+  std::vector<float> wf(w.begin(), w.end());
+  if (hipMemcpyAsync(m->theta.p, wf.data(), sizeof(float) * wf.size(), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+    return fail(TGX_ERR_HIP);
+  if ((st = tgx_zero(ctx, m->mom.p, sizeof(float) * m->nparam)) || (st = tgx_zero(ctx, m->vel.p, sizeof(float) * m->nparam)) ||
+      (st = mlp_pack(m)))
+    return fail(st);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(TGX_ERR_HIP);
+  *out = m;
+  return TGX_OK;
+}
+
+extern "C" int tgx_mlp_destroy(tgx_mlp* m) {
+  if (!m) return TGX_OK;
+  (void)hipSetDevice(m->ctx->device);
+  (void)hipStreamSynchronize(m->ctx->stream);
+  dbuf* all[] = {&m->theta, &m->grad, &m->mom, &m->vel, &m->W1b, &m->W1Tb, &m->W2b, &m->W2Tb, &m->dq, &m->dg, &m->de,
+                 &m->idx, &m->bq, &m->bqb, &m->bg, &m->be, &m->z1, &m->h1, &m->h1b, &m->s1, &m->s1pp, &m->z2, &m->h2,
+                 &m->a2, &m->a2b, &m->s2p, &m->s2pp, &m->E, &m->v1, &m->a1, &m->a1b, &m->G, &m->u, &m->ub, &m->t0,
+                 &m->t1, &m->t1b, &m->t2, &m->gam2, &m->zet2, &m->zet2b, &m->wz, &m->zet1, &m->dE, &m->Y1, &m->Y2,
+                 &m->lossb};
+  for (dbuf* b : all) b->release();
+  delete m;
+  return TGX_OK;
+}
+
+// Forward (energy and input gradient) on B rows of bqb (BF16) held in the batch
+// buffers. With training copies when `train`.
+static int mlp_forward(tgx_mlp* m, int B, const __bf16* xb, bool train, float* E_out, float* G_out, double gscale) {
+  tgx_ctx* ctx = m->ctx;
+  const int H = m->H, Dp = m->Dp;
+  const float* th = m->theta.as<float>();
+  const int64_t nbh = (int64_t)B * H;
+  TGX_TRY(tgx_gemm_bf16_nt(ctx, "mlp_l1", B, H, Dp, xb, Dp, m->W1b.as<__bf16>(), Dp, m->z1.as<float>(), H, 1.0));
+  TGX_LAUNCH(ctx, "mlp_act1", k_l1, g1(nbh), dim3(256), 0, nbh, H, m->z1.as<float>(), th + m->ob1, m->h1b.as<__bf16>(),
+             train ? m->h1.as<float>() : nullptr, m->s1.as<float>(), train ? m->s1pp.as<float>() : nullptr);
+  TGX_TRY(tgx_gemm_bf16_nt(ctx, "mlp_l2", B, H, H, m->h1b.as<__bf16>(), H, m->W2b.as<__bf16>(), H, m->z2.as<float>(), H,
+                           1.0));
+  TGX_LAUNCH(ctx, "mlp_act2", k_l2, dim3((B + 3) / 4), dim3(256), 0, B, H, m->z2.as<float>(), th + m->ob2, th + m->ow3,
+             th + m->ob3, E_out, m->a2b.as<__bf16>(), train ? m->a2.as<float>() : nullptr,
+             train ? m->h2.as<float>() : nullptr, train ? m->s2p.as<float>() : nullptr,
+             train ? m->s2pp.as<float>() : nullptr);
+  TGX_TRY(tgx_gemm_bf16_nt(ctx, "mlp_back2", B, H, H, m->a2b.as<__bf16>(), H, m->W2Tb.as<__bf16>(), H, m->v1.as<float>(),
+                           H, 1.0));
+  TGX_LAUNCH(ctx, "mlp_a1", k_a1, g1(nbh), dim3(256), 0, nbh, m->v1.as<float>(), m->s1.as<float>(),
+             train ? m->a1.as<float>() : nullptr, m->a1b.as<__bf16>());
+  TGX_TRY(tgx_gemm_bf16_nt(ctx, "mlp_back1", B, Dp, H, m->a1b.as<__bf16>(), H, m->W1Tb.as<__bf16>(), H, G_out, Dp,
+                           gscale));
+  return TGX_OK;
+}
+
+// Loss and weight gradients of the batch in the batch buffers (bq, bqb, bg, be).
+static int mlp_backward(tgx_mlp* m, int B, float w_e, float w_g, double* loss_e, double* loss_g) {
+  tgx_ctx* ctx = m->ctx;
+  const int H = m->H, Dp = m->Dp;
+  const int64_t nbh = (int64_t)B * H, nbd = (int64_t)B * Dp;
+  float* th = m->theta.as<float>();
+  float* gr = m->grad.as<float>();
+  TGX_TRY(mlp_forward(m, B, m->bqb.as<__bf16>(), true, m->E.as<float>(), m->G.as<float>(), 1.0));
+  double* le = m->lossb.as<double>();
+  double* lg = le + B;
+  TGX_LAUNCH(ctx, "mlp_loss", k_loss, dim3((B + 3) / 4), dim3(256), 0, B, Dp, m->E.as<float>(), m->be.as<double>(),
+             m->G.as<float>(), m->bg.as<float>(), m->bq.as<float>(), m->shift, 1.0 / m->scale, w_e, m->u.as<float>(),
+             m->ub.as<__bf16>(), m->dE.as<float>(), le, lg);
+  const float cG = 2.0f * w_g / (float)B;
+  // forward mode along u, then reverse
+  TGX_TRY(tgx_gemm_bf16_nt(ctx, "mlp_jvp1", B, H, Dp, m->ub.as<__bf16>(), Dp, m->W1b.as<__bf16>(), Dp, m->t0.as<float>(),
+                           H, 1.0));
+  TGX_LAUNCH(ctx, "mlp_t1", k_t1, g1(nbh), dim3(256), 0, nbh, m->s1.as<float>(), m->t0.as<float>(), m->t1.as<float>(),
+             m->t1b.as<__bf16>());
+  TGX_TRY(tgx_gemm_bf16_nt(ctx, "mlp_jvp2", B, H, H, m->t1b.as<__bf16>(), H, m->W2b.as<__bf16>(), H, m->t2.as<float>(),
+                           H, 1.0));
+  TGX_LAUNCH(ctx, "mlp_z2", k_z2, g1(nbh), dim3(256), 0, nbh, H, m->t2.as<float>(), m->s2p.as<float>(),
+             m->s2pp.as<float>(), th + m->ow3, m->gam2.as<float>(), m->zet2.as<float>(), m->zet2b.as<__bf16>());
+  TGX_TRY(tgx_gemm_bf16_nt(ctx, "mlp_vjp2", B, H, H, m->zet2b.as<__bf16>(), H, m->W2Tb.as<__bf16>(), H,
+                           m->wz.as<float>(), H, 1.0));
+  TGX_LAUNCH(ctx, "mlp_z1", k_z1, g1(nbh), dim3(256), 0, nbh, m->v1.as<float>(), m->t0.as<float>(), m->s1pp.as<float>(),
+             m->wz.as<float>(), m->s1.as<float>(), m->zet1.as<float>());
+  // dW1 = a1^T (dE q + cG u) + cG zeta1^T q
+  TGX_LAUNCH(ctx, "mlp_y1", k_rowcomb, g1(nbd), dim3(256), 0, nbd, Dp, m->dE.as<float>(), m->bq.as<float>(), cG,
+             m->u.as<float>(), m->Y1.as<float>());
+  TGX_LAUNCH(ctx, "mlp_dW1", k_tn_gemm, dim3(Dp / 64, H / 64), dim3(256), 0, B, H, Dp, m->a1.as<float>(),
+             m->Y1.as<float>(), m->zet1.as<float>(), m->bq.as<float>(), cG, gr + m->oW1);
+  // dW2 = a2^T (dE h1 + cG tau1) + cG zeta2^T h1
+  TGX_LAUNCH(ctx, "mlp_y2", k_rowcomb, g1(nbh), dim3(256), 0, nbh, H, m->dE.as<float>(), m->h1.as<float>(), cG,
+             m->t1.as<float>(), m->Y2.as<float>());
+  TGX_LAUNCH(ctx, "mlp_dW2", k_tn_gemm, dim3(H / 64, H / 64), dim3(256), 0, B, H, H, m->a2.as<float>(),
+             m->Y2.as<float>(), m->zet2.as<float>(), m->h1.as<float>(), cG, gr + m->oW2);
+  // biases and output weights
+  TGX_LAUNCH(ctx, "mlp_db1", k_colsum, g1(H), dim3(256), 0, B, H, m->dE.as<float>(), m->a1.as<float>(), cG,
+             m->zet1.as<float>(), gr + m->ob1);
+  TGX_LAUNCH(ctx, "mlp_db2", k_colsum, g1(H), dim3(256), 0, B, H, m->dE.as<float>(), m->a2.as<float>(), cG,
+             m->zet2.as<float>(), gr + m->ob2);
+  TGX_LAUNCH(ctx, "mlp_dw3", k_colsum, g1(H), dim3(256), 0, B, H, m->dE.as<float>(), m->h2.as<float>(), cG,
+             m->gam2.as<float>(), gr + m->ow3);
+  TGX_LAUNCH(ctx, "mlp_db3", k_colsum, dim3(1), dim3(256), 0, B, 1, m->dE.as<float>(), (const float*)nullptr, 0.0f,
+             (const float*)nullptr, gr + m->ob3);
+  if (loss_e || loss_g) {
+    double se = 0.0, sg = 0.0;
+    TGX_TRY(tgx_device_sum(ctx, le, B, &se));
+    TGX_TRY(tgx_device_sum(ctx, lg, B, &sg));
+    if (loss_e) *loss_e = se / B;
+    if (loss_g) *loss_g = sg / B;
+  }
+  return TGX_OK;
+}
+
+extern "C" int tgx_mlp_add_samples(tgx_mlp* m, int model_id, int stride, int offset, int* n_total) {
+  TGX_REQUIRE(m && stride >= 1 && offset >= 0, "tgx_mlp_add_samples: bad arguments");
+  tgx_ctx* ctx = m->ctx;
+  TGX_CHECK_HIP(hipSetDevice(ctx->device));
+  TGX_REQUIRE(model_id >= 0 && model_id < (int)ctx->models.size(), "model %d not defined", model_id);
+  tgx_model_t* md = ctx->models[model_id];
+  TGX_REQUIRE(md->Dpad == m->Dp && md->D == m->D, "descriptor size of model %d does not match the network", model_id);
+  TGX_REQUIRE(md->q64.p && md->qss64.p && md->e_site.p && md->n_sites_b > 0,
+              "tgx_mlp_add_samples: run an FP64 tgx_compute first");
+  const int ns = md->n_sites_b;
+  const int take = offset < ns ? (ns - offset + stride - 1) / stride : 0;
+  if (m->n + take > m->cap) {
+    const int ncap = std::max(m->cap * 2, m->n + take);
+    dbuf q2, g2, e2;
+    TGX_TRY(q2.ensure(sizeof(float) * (size_t)ncap * m->Dp));
+    TGX_TRY(g2.ensure(sizeof(float) * (size_t)ncap * m->Dp));
+    TGX_TRY(e2.ensure(sizeof(double) * (size_t)ncap));
+    if (m->n) {
+      TGX_CHECK_HIP(hipMemcpyAsync(q2.p, m->dq.p, sizeof(float) * (size_t)m->n * m->Dp, hipMemcpyDeviceToDevice, ctx->stream));
+      TGX_CHECK_HIP(hipMemcpyAsync(g2.p, m->dg.p, sizeof(float) * (size_t)m->n * m->Dp, hipMemcpyDeviceToDevice, ctx->stream));
+      TGX_CHECK_HIP(hipMemcpyAsync(e2.p, m->de.p, sizeof(double) * (size_t)m->n, hipMemcpyDeviceToDevice, ctx->stream));
+      TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    m->dq.release(); m->dg.release(); m->de.release();
+    m->dq = q2; m->dg = g2; m->de = e2;
+    q2.p = g2.p = e2.p = nullptr;
+    m->cap = ncap;
+  }
+  if (take > 0)
+    TGX_LAUNCH(ctx, "mlp_append", k_append, dim3(take), dim3(256), 0, take, stride, offset, m->Dp, md->q64.as<double>(),
+               md->qss64.as<double>(), md->e_site.as<double>(), m->dq.as<float>() + (size_t)m->n * m->Dp,
+               m->dg.as<float>() + (size_t)m->n * m->Dp, m->de.as<double>() + m->n);
+  m->n += take;
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  if (n_total) *n_total = m->n;
+  return TGX_OK;
+}
+
+extern "C" int tgx_mlp_set_normalization(tgx_mlp* m, double e_shift, double e_scale) {
+  TGX_REQUIRE(m && e_scale > 0.0, "tgx_mlp_set_normalization: scale must be positive");
+  m->shift = e_shift;
+  m->scale = e_scale;
+  m->norm_set = true;
+  return TGX_OK;
+}
+
+extern "C" int tgx_mlp_get_normalization(tgx_mlp* m, double* e_shift, double* e_scale) {
+  TGX_REQUIRE(m, "null network");
+  if (e_shift) *e_shift = m->shift;
+  if (e_scale) *e_scale = m->scale;
+  return TGX_OK;
+}
+
+extern "C" int tgx_mlp_dataset_stats(tgx_mlp* m, int n_use, double* e_mean, double* e_std) {
+  TGX_REQUIRE(m && n_use > 0 && n_use <= m->n, "tgx_mlp_dataset_stats: bad range");
+  TGX_CHECK_HIP(hipSetDevice(m->ctx->device));
+  std::vector<double> e((size_t)n_use);
+  TGX_CHECK_HIP(hipMemcpy(e.data(), m->de.p, sizeof(double) * n_use, hipMemcpyDeviceToHost));
+  double mu = 0.0;
+  for (double v : e) mu += v;
+  mu /= n_use;
+  double var = 0.0;
+  for (double v : e) var += (v - mu) * (v - mu);
+  if (e_mean) *e_mean = mu;
+  if (e_std) *e_std = std::sqrt(var / n_use);
+  return TGX_OK;
+}
+
+static int adam_step(tgx_mlp* m, float lr) {
+  m->step += 1;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - (float)std::pow(b1, (double)m->step), c2 = 1.f - (float)std::pow(b2, (double)m->step);
+  TGX_LAUNCH(m->ctx, "mlp_adam", k_adam, g1(m->nparam), dim3(256), 0, m->nparam, m->theta.as<float>(),
+             m->grad.as<float>(), m->mom.as<float>(), m->vel.as<float>(), lr, b1, b2, c1, c2, 1e-8f);
+  // padded columns of W1 stay zero: their gradient is zero (q is zero there)
+  return mlp_pack(m);
+}
+
+extern "C" int tgx_mlp_train(tgx_mlp* m, int n_train, int epochs, int batch, float lr, float w_e, float w_g,
+                             uint64_t seed, double* loss_e, double* loss_g) {
+  TGX_REQUIRE(m && n_train > 0 && n_train <= m->n && epochs >= 1 && batch > 0 && batch % 128 == 0 && batch <= m->maxB,
+              "tgx_mlp_train: need 0 < n_train <= dataset size and batch a multiple of 128 within max_batch");
+  TGX_REQUIRE(m->norm_set, "tgx_mlp_train: set the energy normalisation first");
+  tgx_ctx* ctx = m->ctx;
+  TGX_CHECK_HIP(hipSetDevice(ctx->device));
+  std::vector<int> perm((size_t)n_train);
+  std::iota(perm.begin(), perm.end(), 0);
+  double le_sum = 0.0, lg_sum = 0.0;
+  int nb = 0;
+  std::vector<int> pidx(1);
+  for (int ep = 0; ep < epochs; ++ep) {
+    std::vector<int> p((size_t)n_train);
+    TGX_TRY(tgx_generate_permutation(n_train, seed + (uint64_t)ep * 7919u, p.data()));
+    le_sum = lg_sum = 0.0;
+    nb = 0;
+    for (int b0 = 0; b0 + batch <= n_train; b0 += batch) {
+      TGX_CHECK_HIP(hipMemcpyAsync(m->idx.p, p.data() + b0, sizeof(int) * batch, hipMemcpyHostToDevice, ctx->stream));
+      TGX_LAUNCH(ctx, "mlp_gather", k_gather_rows, dim3(batch), dim3(256), 0, batch, m->Dp, m->idx.as<int>(),
+                 m->dq.as<float>(), m->dg.as<float>(), m->de.as<double>(), m->bq.as<float>(), m->bqb.as<__bf16>(),
+                 m->bg.as<float>(), m->be.as<double>());
+      double le = 0.0, lg = 0.0;
+      const bool last = b0 + 2 * batch > n_train;
+      TGX_TRY(mlp_backward(m, batch, w_e, w_g, (ep == epochs - 1) ? &le : nullptr, (ep == epochs - 1) ? &lg : nullptr));
+      TGX_TRY(adam_step(m, lr));
+      le_sum += le;
+      lg_sum += lg;
+      ++nb;
+      (void)last;
+    }
+  }
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  if (loss_e) *loss_e = nb ? le_sum / nb : 0.0;
+  if (loss_g) *loss_g = nb ? lg_sum / nb : 0.0;
+  return TGX_OK;
+}
+
+// Errors on dataset rows [begin, end): energy MAE / RMSE (eV) and the RMS of
+// |P (g_hat - g)| (eV per unit descriptor length).
+extern "C" int tgx_mlp_evaluate(tgx_mlp* m, int begin, int end, double* e_mae, double* e_rmse, double* g_rmse) {
+  TGX_REQUIRE(m && begin >= 0 && end <= m->n && end > begin, "tgx_mlp_evaluate: bad range");
+  tgx_ctx* ctx = m->ctx;
+  TGX_CHECK_HIP(hipSetDevice(ctx->device));
+  double ae = 0.0, se = 0.0, sg = 0.0;
+  std::vector<float> E((size_t)m->maxB);
+  std::vector<double> er((size_t)m->maxB), lg((size_t)m->maxB);
+  for (int b0 = begin; b0 < end; b0 += m->maxB) {
+    const int rows = std::min(m->maxB, end - b0);
+    const int B = (int)tgx_round_up(rows, 128);
+    std::vector<int> idx((size_t)B);
+    for (int r = 0; r < B; ++r) idx[r] = b0 + std::min(r, rows - 1);
+    TGX_CHECK_HIP(hipMemcpyAsync(m->idx.p, idx.data(), sizeof(int) * B, hipMemcpyHostToDevice, ctx->stream));
+    TGX_LAUNCH(ctx, "mlp_gather", k_gather_rows, dim3(B), dim3(256), 0, B, m->Dp, m->idx.as<int>(), m->dq.as<float>(),
+               m->dg.as<float>(), m->de.as<double>(), m->bq.as<float>(), m->bqb.as<__bf16>(), m->bg.as<float>(),
+               m->be.as<double>());
+    TGX_TRY(mlp_forward(m, B, m->bqb.as<__bf16>(), false, m->E.as<float>(), m->G.as<float>(), 1.0));
+    double* le = m->lossb.as<double>();
+    TGX_LAUNCH(ctx, "mlp_loss", k_loss, dim3((B + 3) / 4), dim3(256), 0, B, m->Dp, m->E.as<float>(), m->be.as<double>(),
+               m->G.as<float>(), m->bg.as<float>(), m->bq.as<float>(), m->shift, 1.0 / m->scale, 1.0f,
+               m->u.as<float>(), m->ub.as<__bf16>(), m->dE.as<float>(), le, le + B);
+    TGX_CHECK_HIP(hipMemcpyAsync(er.data(), le, sizeof(double) * B, hipMemcpyDeviceToHost, ctx->stream));
+    TGX_CHECK_HIP(hipMemcpyAsync(lg.data(), le + B, sizeof(double) * B, hipMemcpyDeviceToHost, ctx->stream));
+    TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    for (int r = 0; r < rows; ++r) {
+      const double d = std::sqrt(er[r]) * m->scale;
+      ae += d;
+      se += d * d;
+      sg += lg[r] * m->scale * m->scale;
+    }
+  }
+  const int n = end - begin;
+  if (e_mae) *e_mae = ae / n;
+  if (e_rmse) *e_rmse = std::sqrt(se / n);
+  if (g_rmse) *g_rmse = std::sqrt(sg / n);
+  return TGX_OK;
+}
+
+extern "C" int tgx_mlp_forward_host(tgx_mlp* m, int batch, const float* q, float* e, float* g) {
+  TGX_REQUIRE(m && q && batch > 0 && batch <= m->maxB, "tgx_mlp_forward_host: bad arguments");
+  tgx_ctx* ctx = m->ctx;
+  TGX_CHECK_HIP(hipSetDevice(ctx->device));
+  const int B = (int)tgx_round_up(batch, 128);
+  std::vector<float> qp((size_t)B * m->Dp, 0.f);
+  for (int r = 0; r < batch; ++r)
+    for (int d = 0; d < m->D; ++d) qp[(size_t)r * m->Dp + d] = q[(size_t)r * m->D + d];
+  TGX_CHECK_HIP(hipMemcpyAsync(m->bq.p, qp.data(), sizeof(float) * qp.size(), hipMemcpyHostToDevice, ctx->stream));
+  TGX_TRY(tgx_f32_to_bf16(ctx, m->bq.as<float>(), m->bqb.as<__bf16>(), (int64_t)B * m->Dp));
+  TGX_TRY(mlp_forward(m, B, m->bqb.as<__bf16>(), false, m->E.as<float>(), m->G.as<float>(), 1.0));
+  std::vector<float> E((size_t)B), G((size_t)B * m->Dp);
+  TGX_CHECK_HIP(hipMemcpyAsync(E.data(), m->E.p, sizeof(float) * B, hipMemcpyDeviceToHost, ctx->stream));
+  TGX_CHECK_HIP(hipMemcpyAsync(G.data(), m->G.p, sizeof(float) * G.size(), hipMemcpyDeviceToHost, ctx->stream));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  for (int r = 0; r < batch; ++r) {
+    if (e) e[r] = E[r];
+    if (g)
+      for (int d = 0; d < m->D; ++d) g[(size_t)r * m->D + d] = G[(size_t)r * m->Dp + d];
+  }
+  return TGX_OK;
+}
+
+// Loss and weight gradients of one batch given on the host (normalised units:
+// e_ref and g_ref are already (E - shift)/scale and g/scale when shift = 0 and
+// scale = 1 are set). Gradient layout as tgx_mlp_get_weights.
+extern "C" int tgx_mlp_gradients_host(tgx_mlp* m, int batch, const float* q, const float* e_ref, const float* g_ref,
+                                      float w_e, float w_g, float* dW1, float* db1, float* dW2, float* db2,
+                                      float* dw3, float* db3, double* loss_e, double* loss_g) {
+  TGX_REQUIRE(m && q && e_ref && g_ref && batch > 0 && batch % 128 == 0 && batch <= m->maxB,
+              "tgx_mlp_gradients_host: batch must be a multiple of 128 within max_batch");
+  tgx_ctx* ctx = m->ctx;
+  TGX_CHECK_HIP(hipSetDevice(ctx->device));
+  const int B = batch, Dp = m->Dp, D = m->D, H = m->H;
+  std::vector<float> qp((size_t)B * Dp, 0.f), gp((size_t)B * Dp, 0.f);
+  std::vector<double> ep((size_t)B);
+  for (int r = 0; r < B; ++r) {
+    ep[r] = e_ref[r];
+    for (int d = 0; d < D; ++d) {
+      qp[(size_t)r * Dp + d] = q[(size_t)r * D + d];
+      gp[(size_t)r * Dp + d] = g_ref[(size_t)r * D + d];
+    }
+  }
+  TGX_CHECK_HIP(hipMemcpyAsync(m->bq.p, qp.data(), sizeof(float) * qp.size(), hipMemcpyHostToDevice, ctx->stream));
+  TGX_CHECK_HIP(hipMemcpyAsync(m->bg.p, gp.data(), sizeof(float) * gp.size(), hipMemcpyHostToDevice, ctx->stream));
+  TGX_CHECK_HIP(hipMemcpyAsync(m->be.p, ep.data(), sizeof(double) * ep.size(), hipMemcpyHostToDevice, ctx->stream));
+  TGX_TRY(tgx_f32_to_bf16(ctx, m->bq.as<float>(), m->bqb.as<__bf16>(), (int64_t)B * Dp));
+  TGX_TRY(mlp_backward(m, B, w_e, w_g, loss_e, loss_g));
+  std::vector<float> gr((size_t)m->nparam);
+  TGX_CHECK_HIP(hipMemcpyAsync(gr.data(), m->grad.p, sizeof(float) * gr.size(), hipMemcpyDeviceToHost, ctx->stream));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  for (int h = 0; h < H; ++h)
+    for (int d = 0; d < D; ++d)
+      if (dW1) dW1[(size_t)h * D + d] = gr[m->oW1 + (size_t)h * Dp + d];
+  for (int k = 0; k < H; ++k) {
+    if (db1) db1[k] = gr[m->ob1 + k];
+    if (db2) db2[k] = gr[m->ob2 + k];
+    if (dw3) dw3[k] = gr[m->ow3 + k];
+  }
+  if (dW2) std::memcpy(dW2, gr.data() + m->oW2, sizeof(float) * (size_t)H * H);
+  if (db3) db3[0] = gr[m->ob3];
+  return TGX_OK;
+}
+
+extern "C" int tgx_mlp_get_weights(tgx_mlp* m, float* W1, float* b1, float* W2, float* b2, float* w3, float* b3) {
+  TGX_REQUIRE(m, "null network");
+  TGX_CHECK_HIP(hipSetDevice(m->ctx->device));
+  std::vector<float> th((size_t)m->nparam);
+  TGX_CHECK_HIP(hipMemcpy(th.data(), m->theta.p, sizeof(float) * th.size(), hipMemcpyDeviceToHost));
+  const int H = m->H, D = m->D, Dp = m->Dp;
+  for (int h = 0; h < H; ++h) {
+    if (W1) std::memcpy(W1 + (size_t)h * D, th.data() + m->oW1 + (size_t)h * Dp, sizeof(float) * D);
+    if (b1) b1[h] = th[m->ob1 + h];
+    if (b2) b2[h] = th[m->ob2 + h];
+    if (w3) w3[h] = th[m->ow3 + h];
+  }
+  if (W2) std::memcpy(W2, th.data() + m->oW2, sizeof(float) * (size_t)H * H);
+  if (b3) b3[0] = th[m->ob3];
+  return TGX_OK;
+}
+
+extern "C" int tgx_mlp_set_weights(tgx_mlp* m, const float* W1, const float* b1, const float* W2, const float* b2,
+                                   const float* w3, const float* b3) {
+  TGX_REQUIRE(m && W1 && b1 && W2 && b2 && w3 && b3, "tgx_mlp_set_weights: null input");
+  tgx_ctx* ctx = m->ctx;
+  TGX_CHECK_HIP(hipSetDevice(ctx->device));
+  std::vector<float> th((size_t)m->nparam, 0.f);
+  const int H = m->H, D = m->D, Dp = m->Dp;
+  for (int h = 0; h < H; ++h) {
+    std::memcpy(th.data() + m->oW1 + (size_t)h * Dp, W1 + (size_t)h * D, sizeof(float) * D);
+    th[m->ob1 + h] = b1[h];
+    th[m->ob2 + h] = b2[h];
+    th[m->ow3 + h] = w3[h];
+  }
+  std::memcpy(th.data() + m->oW2, W2, sizeof(float) * (size_t)H * H);
+  th[m->ob3] = b3[0];
+  TGX_CHECK_HIP(hipMemcpyAsync(m->theta.p, th.data(), sizeof(float) * th.size(), hipMemcpyHostToDevice, ctx->stream));
+  TGX_TRY(tgx_zero(ctx, m->mom.p, sizeof(float) * m->nparam));
+  TGX_TRY(tgx_zero(ctx, m->vel.p, sizeof(float) * m->nparam));
+  m->step = 0;
+  TGX_TRY(mlp_pack(m));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return TGX_OK;
+}
+
+// ---- surrogate inside tgx_compute ---------------------------------------------
+// Members are networks installed per model; their BF16 weights and biases are
+// copied so the network can be destroyed afterwards.
+struct SurrogateMember {
+  int H = 0;
+  double shift = 0, scale = 1;
+  dbuf W1b, W1Tb, W2b, W2Tb, bias;  // bias: b1 | b2 | w3 | b3
+};
+static std::vector<std::pair<std::pair<tgx_ctx*, int>, std::vector<SurrogateMember*>>> g_slots;
+
+static std::vector<SurrogateMember*>& slot(tgx_ctx* ctx, int model_id) {
+  for (auto& s : g_slots)
+    if (s.first.first == ctx && s.first.second == model_id) return s.second;
+  g_slots.push_back({{ctx, model_id}, {}});
+  return g_slots.back().second;
+}
+
+void tgx_surrogate_release(tgx_ctx* ctx) {
+  for (auto& s : g_slots)
+    if (s.first.first == ctx) {
+      for (auto* mb : s.second) {
+        if (!mb) continue;
+        mb->W1b.release(); mb->W1Tb.release(); mb->W2b.release(); mb->W2Tb.release(); mb->bias.release();
+        delete mb;
+      }
+      s.second.clear();
+      s.first.first = nullptr;
+    }
+}
+
+extern "C" int tgx_surrogate_install(tgx_ctx* ctx, int model_id, int member, tgx_mlp* m) {
+  TGX_REQUIRE(ctx && m && m->ctx == ctx, "tgx_surrogate_install: network belongs to another context");
+  TGX_CHECK_HIP(hipSetDevice(ctx->device));
+  TGX_REQUIRE(model_id >= 0 && model_id < (int)ctx->models.size(), "model %d not defined", model_id);
+  tgx_model_t* md = ctx->models[model_id];
+  TGX_REQUIRE(md->D == m->D && md->Dpad == m->Dp, "network dimension %d does not match model %d", m->D, model_id);
+  TGX_REQUIRE(member >= 0 && member < 16, "member index %d outside 0..15", member);
+  auto& sl = slot(ctx, model_id);
+  if ((int)sl.size() <= member) sl.resize(member + 1, nullptr);
+  if (!sl[member]) sl[member] = new SurrogateMember();
+  SurrogateMember* s = sl[member];
+  const size_t H = m->H, Dp = m->Dp;
+  s->H = m->H;
+  s->shift = m->shift;
+  s->scale = m->scale;
+  TGX_TRY(s->W1b.ensure(sizeof(__bf16) * H * Dp));
+  TGX_TRY(s->W1Tb.ensure(sizeof(__bf16) * H * Dp));
+  TGX_TRY(s->W2b.ensure(sizeof(__bf16) * H * H));
+  TGX_TRY(s->W2Tb.ensure(sizeof(__bf16) * H * H));
+  TGX_TRY(s->bias.ensure(sizeof(float) * (3 * H + 1)));
+  TGX_CHECK_HIP(hipMemcpyAsync(s->W1b.p, m->W1b.p, sizeof(__bf16) * H * Dp, hipMemcpyDeviceToDevice, ctx->stream));
+  TGX_CHECK_HIP(hipMemcpyAsync(s->W1Tb.p, m->W1Tb.p, sizeof(__bf16) * H * Dp, hipMemcpyDeviceToDevice, ctx->stream));
+  TGX_CHECK_HIP(hipMemcpyAsync(s->W2b.p, m->W2b.p, sizeof(__bf16) * H * H, hipMemcpyDeviceToDevice, ctx->stream));
+  TGX_CHECK_HIP(hipMemcpyAsync(s->W2Tb.p, m->W2Tb.p, sizeof(__bf16) * H * H, hipMemcpyDeviceToDevice, ctx->stream));
+  const float* th = m->theta.as<float>();
+  float* b = s->bias.as<float>();
+  TGX_CHECK_HIP(hipMemcpyAsync(b, th + m->ob1, sizeof(float) * H, hipMemcpyDeviceToDevice, ctx->stream));
+  TGX_CHECK_HIP(hipMemcpyAsync(b + H, th + m->ob2, sizeof(float) * H, hipMemcpyDeviceToDevice, ctx->stream));
+  TGX_CHECK_HIP(hipMemcpyAsync(b + 2 * H, th + m->ow3, sizeof(float) * H, hipMemcpyDeviceToDevice, ctx->stream));
+  TGX_CHECK_HIP(hipMemcpyAsync(b + 3 * H, th + m->ob3, sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return TGX_OK;
+}
+
+extern "C" int tgx_surrogate_gate(tgx_ctx* ctx, int n_members, double threshold) {
+  TGX_REQUIRE(ctx && n_members >= 1 && n_members <= 16, "tgx_surrogate_gate: 1..16 members");
+  ctx->gate_members = n_members;
+  ctx->gate_threshold = threshold > 0.0 ? threshold : 1e30;
+  return TGX_OK;
+}
+
+extern "C" int tgx_surrogate_stats(tgx_ctx* ctx, double* gated_fraction, double* max_spread) {
+  TGX_REQUIRE(ctx, "null context");
+  if (gated_fraction) *gated_fraction = ctx->gated_fraction;
+  if (max_spread) *max_spread = ctx->max_spread;
+  return TGX_OK;
+}
+
+// Surrogate energies and gradients for one block; forces follow through the
+// FP32 adjoint kernel. Gated sites are re-evaluated by the FP64 GAP.
+int tgx_surrogate_run(tgx_ctx* ctx, tgx_model_t* m) {
+  int model_id = -1;
+  for (int k = 0; k < (int)ctx->models.size(); ++k)
+    if (ctx->models[k] == m) model_id = k;
+  auto& sl = slot(ctx, model_id);
+  const int K = ctx->gate_members;
+  TGX_REQUIRE((int)sl.size() >= K, "surrogate: model %d has %zu installed members, %d requested", model_id, sl.size(), K);
+  for (int k = 0; k < K; ++k) TGX_REQUIRE(sl[k], "surrogate: member %d of model %d missing", k, model_id);
+  TGX_TRY(tgx_block_select(ctx, m));
+  const int ns = m->n_sites_b;
+  if (ns == 0) return TGX_OK;
+  TGX_TRY(tgx_soap_descriptors(ctx, m, TGX_PATH_MIXED, false));
+  const int Mp = (int)tgx_round_up(ns, 128), Dp = m->Dpad;
+  const int H = sl[0]->H;
+  for (int k = 1; k < K; ++k) TGX_REQUIRE(sl[k]->H == H, "surrogate members differ in width");
+  TGX_TRY(m->qbf.ensure(sizeof(__bf16) * (size_t)Mp * Dp));
+  TGX_TRY(m->qss32.ensure(sizeof(float) * (size_t)Mp * Dp));
+  TGX_TRY(m->e_site.ensure(sizeof(double) * (size_t)Mp));
+  TGX_TRY(tgx_f32_to_bf16(ctx, m->q32.as<float>(), m->qbf.as<__bf16>(), (int64_t)Mp * Dp));
+  // work space in the context scratch: z1, s1, z2, v1 (FP32 Mp x H), h1b, a2b, a1b (BF16), Em (K x Mp), G (Mp x Dp)
+  const size_t fh = (size_t)Mp * H;
+  const size_t bytes = sizeof(float) * (4 * fh + (size_t)K * Mp + (size_t)Mp * Dp + Mp) + sizeof(__bf16) * 3 * fh + 16 * 256;
+  TGX_TRY(ctx->e_members.ensure(bytes));
+  char* p = ctx->e_members.as<char>();
+  auto take = [&](size_t n) { char* r = p; p += tgx_round_up((int64_t)n, 256); return r; };
+  float* z1 = (float*)take(sizeof(float) * fh);
+  float* s1 = (float*)take(sizeof(float) * fh);
+  float* z2 = (float*)take(sizeof(float) * fh);
+  float* v1 = (float*)take(sizeof(float) * fh);
+  __bf16* h1b = (__bf16*)take(sizeof(__bf16) * fh);
+  __bf16* a2b = (__bf16*)take(sizeof(__bf16) * fh);
+  __bf16* a1b = (__bf16*)take(sizeof(__bf16) * fh);
+  float* Em = (float*)take(sizeof(float) * K * Mp);
+  float* G = (float*)take(sizeof(float) * (size_t)Mp * Dp);
+  float* spread = (float*)take(sizeof(float) * Mp);
+  const int64_t nbh = (int64_t)Mp * H;
+  for (int k = 0; k < K; ++k) {
+    SurrogateMember* s = sl[k];
+    const float* b = s->bias.as<float>();
+    TGX_TRY(tgx_gemm_bf16_nt(ctx, "mlp_l1", Mp, H, Dp, m->qbf.as<__bf16>(), Dp, s->W1b.as<__bf16>(), Dp, z1, H, 1.0));
+    TGX_LAUNCH(ctx, "mlp_act1", k_l1, g1(nbh), dim3(256), 0, nbh, H, z1, b, h1b, (float*)nullptr, s1, (float*)nullptr);
+    TGX_TRY(tgx_gemm_bf16_nt(ctx, "mlp_l2", Mp, H, H, h1b, H, s->W2b.as<__bf16>(), H, z2, H, 1.0));
+    TGX_LAUNCH(ctx, "mlp_act2", k_l2, dim3((Mp + 3) / 4), dim3(256), 0, Mp, H, z2, b + H, b + 2 * H, b + 3 * H,
+               Em + (size_t)k * Mp, a2b, (float*)nullptr, (float*)nullptr, (float*)nullptr, (float*)nullptr);
+    TGX_TRY(tgx_gemm_bf16_nt(ctx, "mlp_back2", Mp, H, H, a2b, H, s->W2Tb.as<__bf16>(), H, v1, H, 1.0));
+    TGX_LAUNCH(ctx, "mlp_a1", k_a1, g1(nbh), dim3(256), 0, nbh, v1, s1, (float*)nullptr, a1b);
+    // Qss = -dE/dq = -(scale/K) sum_k G_k
+    float* dst = k == 0 ? m->qss32.as<float>() : G;
+    TGX_TRY(tgx_gemm_bf16_nt(ctx, "mlp_back1", Mp, Dp, H, a1b, H, s->W1Tb.as<__bf16>(), H, dst, Dp, -s->scale / K));
+    if (k > 0)
+      TGX_LAUNCH(ctx, "mlp_addg", k_add_f32, g1((int64_t)Mp * Dp), dim3(256), 0, (int64_t)Mp * Dp, G, m->qss32.as<float>());
+  }
+  TGX_LAUNCH(ctx, "mlp_out", k_surrogate_out, g1(ns), dim3(256), 0, ns, K, Em, sl[0]->shift, sl[0]->scale,
+             m->e_site.as<double>(), spread);
+  // ensemble gate
+  ctx->gated_fraction = 0.0;
+  ctx->max_spread = 0.0;
+  if (K > 1) {
+    std::vector<float> sp((size_t)ns);
+    TGX_CHECK_HIP(hipMemcpyAsync(sp.data(), spread, sizeof(float) * ns, hipMemcpyDeviceToHost, ctx->stream));
+    TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<int> gated;
+    for (int i = 0; i < ns; ++i) {
+      ctx->max_spread = std::max(ctx->max_spread, (double)sp[i]);
+      if (sp[i] > ctx->gate_threshold) gated.push_back(i);
+    }
+    ctx->gated_fraction = (double)gated.size() / ns;
+    if (!gated.empty()) {
+      const int ng = (int)gated.size(), ngp = (int)tgx_round_up(ng, 128);
+      TGX_TRY(ctx->gate_mask.ensure(sizeof(int) * (size_t)ngp));
+      TGX_CHECK_HIP(hipMemcpyAsync(ctx->gate_mask.p, gated.data(), sizeof(int) * ng, hipMemcpyHostToDevice, ctx->stream));
+      TGX_TRY(m->q64.ensure(sizeof(double) * (size_t)ngp * Dp));
+      TGX_TRY(tgx_zero(ctx, m->q64.p, sizeof(double) * (size_t)ngp * Dp));
+      // FP64 descriptors and GAP of the gated sites only
+      dbuf pn;
+      TGX_TRY(pn.ensure(sizeof(double) * (size_t)ngp));
+      TGX_TRY(tgx_soap_descriptors_sites(ctx, m, ctx->gate_mask.as<int>(), ng, m->q64.as<double>(), pn.as<double>()));
+      const int ns_save = m->n_sites_b;
+      dbuf e_keep;
+      TGX_TRY(e_keep.ensure(sizeof(double) * (size_t)Mp));
+      TGX_CHECK_HIP(hipMemcpyAsync(e_keep.p, m->e_site.p, sizeof(double) * ns, hipMemcpyDeviceToDevice, ctx->stream));
+      m->n_sites_b = ng;
+      int st = tgx_gap_contract(ctx, m, TGX_PATH_FP64_ADJOINT, true);
+      m->n_sites_b = ns_save;
+      if (st == TGX_OK) {
+        // e_site now holds the gated sites' GAP energies in its first ng rows
+        TGX_LAUNCH(ctx, "gate_replace", k_gate_replace, dim3(ng), dim3(256), 0, ng, ctx->gate_mask.as<int>(), Dp,
+                   m->e_site.as<double>(), m->qss64.as<double>(), e_keep.as<double>(), m->qss32.as<float>());
+        TGX_CHECK_HIP(hipMemcpyAsync(m->e_site.p, e_keep.p, sizeof(double) * ns, hipMemcpyDeviceToDevice, ctx->stream));
+      }
+      TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+      pn.release();
+      e_keep.release();
+      TGX_TRY(st);
+    }
+  }
+  // descriptors the adjoint kernel reads must be the FP32 ones of every site
+  return tgx_adjoint_pair_forces(ctx, m, TGX_PATH_SURROGATE);
+}

@@ -31,8 +31,10 @@ template <> __device__ __forceinline__ float t_pow<float>(float x, float y) { re
 
 // Real spherical harmonic Y_lm(u) (orthonormal, Condon-Shortley phase) and the
 // gradient of the homogeneous solid harmonic r^l Y_lm at the unit vector u.
+// nrm is the tabulated constant sqrt((2l+1)/(4 pi) (l-|m|)!/(l+|m|)!)
+// (-1)^|m| (2|m|-1)!! (times sqrt 2 for m != 0).
 template <class T>
-__device__ void ylm_one(int l, int m, T ux, T uy, T uz, T& Y, T& gx, T& gy, T& gz) {
+__device__ void ylm_one(int l, int m, T nrm, T ux, T uy, T uz, T& Y, T& gx, T& gy, T& gz) {
   const int am = m < 0 ? -m : m;
   // (x + i y)^am and its x, y derivatives
   T c = 1, cx = 0, cy = 0, s = 0, sx = 0, sy = 0;
@@ -41,7 +43,7 @@ __device__ void ylm_one(int l, int m, T ux, T uy, T uz, T& Y, T& gx, T& gy, T& g
     T ns = c * uy + s * ux, nsx = cx * uy + sx * ux + s, nsy = cy * uy + c + sy * ux;
     c = nc; cx = ncx; cy = ncy; s = ns; sx = nsx; sy = nsy;
   }
-  // Q_l^am(z, r^2) with Q_am^am = 1 (the constant goes into the normalisation)
+  // Q_l^am(z, r^2) with Q_am^am = 1 (the constant is in nrm)
   T q = 1, qx = 0, qy = 0, qz = 0;        // Q_{l'}
   T p = 0, px = 0, py = 0, pz = 0;        // Q_{l'-1}
   if (l > am) {
@@ -58,15 +60,7 @@ __device__ void ylm_one(int l, int m, T ux, T uy, T uz, T& Y, T& gx, T& gy, T& g
       q = nq; qx = nqx; qy = nqy; qz = nqz;
     }
   }
-  // normalisation sqrt((2l+1)/(4 pi) (l-am)!/(l+am)!) * (sqrt 2 for m != 0) * (-1)^am (2am-1)!!
-  double ratio = 1.0;
-  for (int k = l - am + 1; k <= l + am; ++k) ratio /= (double)k;
-  double nrm = sqrt((2.0 * l + 1.0) / (4.0 * M_PI) * ratio);
-  double df = 1.0;
-  for (int k = 1; k <= 2 * am - 1; k += 2) df *= (double)k;
-  nrm *= df * ((am & 1) ? -1.0 : 1.0);
-  if (am != 0) nrm *= M_SQRT2;
-  const T N = (T)nrm;
+  const T N = nrm;
   if (m == 0) {
     Y = N * q; gx = N * qx; gy = N * qy; gz = N * qz;
   } else if (m > 0) {
@@ -77,10 +71,8 @@ __device__ void ylm_one(int l, int m, T ux, T uy, T uz, T& Y, T& gx, T& gy, T& g
 }
 
 __device__ __forceinline__ int lm_l(int lm) {
-  int l = (int)sqrtf((float)lm);
-  while ((l + 1) * (l + 1) <= lm) ++l;
-  while (l * l > lm) --l;
-  return l;
+  int l = (int)sqrtf((float)lm + 0.5f);
+  return l * l > lm ? l - 1 : l;
 }
 
 // Scratch of one pair in LDS.
@@ -121,17 +113,24 @@ __device__ void pair_step1(const SoapParams& P, int s, T r, T ux, T uy, T uz, in
   for (int lm = t; lm < P.NLM; lm += nthr) {
     const int l = lm_l(lm), m = lm - l * l - l;
     T Y, gx, gy, gz;
-    ylm_one<T>(l, m, ux, uy, uz, Y, gx, gy, gz);
+    ylm_one<T>(l, m, (T)P.ynorm[lm], ux, uy, uz, Y, gx, gy, gz);
     ps.Y[lm] = Y;
     if (need_grad) { ps.gY[3 * lm] = gx; ps.gY[3 * lm + 1] = gy; ps.gY[3 * lm + 2] = gz; }
   }
-  for (int l = t; l <= P.L; l += nthr) {
+  if (t == nthr - 1) {
+    // A_l = exp(-l(l+1) k), k = st^2/(2 r^2): one exponential, A_l = A_{l-1} e^(2l) with e = exp(-k)
     const T st = (T)sp.sig_t * ((T)1 + (T)sp.sig_t_sc * r);
     const T dst = (T)(sp.sig_t * sp.sig_t_sc);
-    const T ll = (T)(l * (l + 1));
-    const T e = t_exp<T>(-ll * st * st / ((T)2 * r * r));
-    ps.A[l] = e;
-    ps.dA[l] = e * (ll * st * st / (r * r * r) - ll * st * dst / (r * r));
+    const T kk = st * st / ((T)2 * r * r);
+    const T dk = st * st / (r * r * r) - st * dst / (r * r);  // -dk/dr
+    const T e1 = t_exp<T>(-kk), e2 = e1 * e1;
+    T Al = 1, pw = 1;
+    for (int l = 0; l <= P.L; ++l) {
+      if (l > 0) { pw *= e2; Al *= pw; }
+      const T ll = (T)(l * (l + 1));
+      ps.A[l] = Al;
+      ps.dA[l] = Al * ll * dk;
+    }
   }
 }
 
@@ -160,6 +159,29 @@ __device__ void pair_step2(const SoapParams& P, int s, T r, int t, int nthr, Pai
   amp *= (T)sp.gs;
   damp *= (T)sp.gs;
   const double* B = P.B + (size_t)s * TGX_MAX_NB * TGX_MAX_Q;
+  if (nthr == 64) {
+    // 64 lanes = nb groups x (64/nb) node slices (nb <= 16 divides into 4-lane slices at least)
+    const int parts = sp.nb <= 8 ? 8 : 4;
+    const int n = t / parts, part = t - n * parts;
+    T I = 0, dI = 0;
+    if (n < sp.nb) {
+      const double* Bn = B + n * TGX_MAX_Q;
+      for (int q = part; q < sp.nq; q += parts) {
+        const T b = (T)Bn[q];
+        I += b * ps.G[q];
+        dI += b * ps.dG[q];
+      }
+    }
+    for (int o = 1; o < parts; o <<= 1) {
+      I += __shfl_xor(I, o);
+      dI += __shfl_xor(dI, o);
+    }
+    if (n < sp.nb && part == 0) {
+      ps.R[n] = amp * I;
+      ps.dR[n] = damp * I + amp * dI;
+    }
+    return;
+  }
   for (int n = t; n < sp.nb; n += nthr) {
     T I = 0, dI = 0;
     const double* Bn = B + n * TGX_MAX_Q;
@@ -288,11 +310,14 @@ __device__ void contract_channel(const SoapParams& P, const T* V, const T* c, T*
 // Mode 0: descriptors only. Mode 1: descriptors and dq/dr (plain path, FP64).
 template <class T, int MODE, int KMAX>
 __global__ __launch_bounds__(WG) void k_soap_site(SoapParams P, SiteArgs a, int n_sites, T* __restrict__ q_out,
-                                                  T* __restrict__ pnorm_out, double* __restrict__ der_out) {
+                                                  T* __restrict__ pnorm_out, double* __restrict__ der_out,
+                                                  const int* __restrict__ site_list) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* sm = reinterpret_cast<T*>(smem_raw);
-  const int site = blockIdx.x;
-  if (site >= n_sites) return;
+  if ((int)blockIdx.x >= n_sites) return;
+  // with a site list (descriptors only), row blockIdx.x of the output holds site_list[blockIdx.x]
+  const int site = site_list ? site_list[blockIdx.x] : (int)blockIdx.x;
+  const int orow = blockIdx.x;
   const int tid = threadIdx.x;
   const int NLM = P.NLM, N = P.N, L = P.L, L1 = L + 1, D = P.D;
   T* c = sm;                                   // N*NLM
@@ -304,9 +329,9 @@ __global__ __launch_bounds__(WG) void k_soap_site(SoapParams P, SiteArgs a, int 
   const int nn = (int)(a.site_beg[site + 1] - b0);
   site_coefficients<T>(P, a, b0, nn, c, wscr);
   const T nrm = power_spectrum<T>(P, c, X, red);
-  T* qrow = q_out + (size_t)site * P.Dpad;
+  T* qrow = q_out + (size_t)orow * P.Dpad;
   for (int d = tid; d < P.Dpad; d += WG) qrow[d] = d < D ? X[d] : (T)0;
-  if (tid == 0) pnorm_out[site] = nrm;
+  if (tid == 0) pnorm_out[orow] = nrm;
   if (MODE == 0) return;
   // ---- dq/dr for every pair of the site (MODE 1) ----
   contract_channel<T>(P, X, c, H);
@@ -651,7 +676,7 @@ int tgx_soap_descriptors(tgx_ctx* ctx, tgx_model_t* m, int path, bool with_der) 
       const int kmax = (m->P.D + WG - 1) / WG;
 #define TGX_SOAP_DER(K)                                                                                 \
   TGX_LAUNCH(ctx, "soap_site_der_f64", (k_soap_site<double, 1, K>), dim3(ns), dim3(WG), lds, m->P, a, ns, \
-             m->q64.as<double>(), m->pnorm.as<double>(), m->soap_der.as<double>())
+             m->q64.as<double>(), m->pnorm.as<double>(), m->soap_der.as<double>(), (const int*)nullptr)
       if (kmax <= 2) TGX_SOAP_DER(2);
       else if (kmax <= 5) TGX_SOAP_DER(5);
       else if (kmax <= 8) TGX_SOAP_DER(8);
@@ -662,14 +687,14 @@ int tgx_soap_descriptors(tgx_ctx* ctx, tgx_model_t* m, int path, bool with_der) 
       const size_t lds = soap_lds_bytes(m->P, 0, sizeof(double));
       TGX_TRY(check_lds(lds));
       TGX_LAUNCH(ctx, "soap_site_f64", (k_soap_site<double, 0, 1>), dim3(ns), dim3(WG), lds, m->P, a, ns,
-                 m->q64.as<double>(), m->pnorm.as<double>(), (double*)nullptr);
+                 m->q64.as<double>(), m->pnorm.as<double>(), (double*)nullptr, (const int*)nullptr);
     }
   } else {
     TGX_TRY(m->q32.ensure(sizeof(float) * nsp * m->Dpad));
     const size_t lds = soap_lds_bytes(m->P, 0, sizeof(float));
     TGX_TRY(check_lds(lds));
     TGX_LAUNCH(ctx, "soap_site_f32", (k_soap_site<float, 0, 1>), dim3(ns), dim3(WG), lds, m->P, a, ns,
-               m->q32.as<float>(), m->pnorm.as<float>(), (double*)nullptr);
+               m->q32.as<float>(), m->pnorm.as<float>(), (double*)nullptr, (const int*)nullptr);
   }
   return TGX_OK;
 }
@@ -702,5 +727,18 @@ int tgx_plain_pair_forces(tgx_ctx* ctx, tgx_model_t* m) {
   TGX_LAUNCH(ctx, "plain_force", k_plain_force, dim3((unsigned)((np + 3) / 4)), dim3(256), 0, ns,
              m->site_beg.as<int64_t>(), m->D, m->Dpad, m->qss64.as<double>(), m->soap_der.as<double>(),
              m->pr_gidx.as<int64_t>(), np, pair_site, ctx->fglob.as<double>());
+  return TGX_OK;
+}
+
+// FP64 descriptors of the listed sites of the current block, row r of q_out
+// holding site d_sites[r] (used to re-evaluate gated sites with the GAP).
+int tgx_soap_descriptors_sites(tgx_ctx* ctx, tgx_model_t* m, const int* d_sites, int n_list, double* q_out,
+                               double* pnorm_out) {
+  if (n_list == 0) return TGX_OK;
+  SiteArgs a = site_args(m);
+  const size_t lds = soap_lds_bytes(m->P, 0, sizeof(double));
+  TGX_TRY(check_lds(lds));
+  TGX_LAUNCH(ctx, "soap_site_list_f64", (k_soap_site<double, 0, 1>), dim3(n_list), dim3(WG), lds, m->P, a, n_list,
+             q_out, pnorm_out, (double*)nullptr, d_sites);
   return TGX_OK;
 }

@@ -87,16 +87,21 @@ _SIGS = {
     "tgx_compute": [_P, _I, _PD, _PD],
     "tgx_get_results": [_P, _PD, _PD],
     "tgx_results_device": [_P, C.POINTER(_P), C.POINTER(_P)],
-    "tgx_surrogate_set": [_P, _I, _I, _I, _PF, _PF, _PF, _PF, _PF, _PF, _D, _D],
-    "tgx_surrogate_gate": [_P, _I, _D],
-    "tgx_surrogate_stats": [_P, _PD, _PD],
     "tgx_mlp_create": [_P, _I, _I, _I, _U64, C.POINTER(_P)],
     "tgx_mlp_destroy": [_P],
-    "tgx_mlp_train_step": [_P, _I, _P, _P, _P, C.c_float, C.c_float, C.c_float, _PD, _PD],
-    "tgx_mlp_forward": [_P, _I, _P, _P, _P],
+    "tgx_mlp_add_samples": [_P, _I, _I, _I, _PI],
+    "tgx_mlp_dataset_stats": [_P, _I, _PD, _PD],
+    "tgx_mlp_set_normalization": [_P, _D, _D],
+    "tgx_mlp_get_normalization": [_P, _PD, _PD],
+    "tgx_mlp_train": [_P, _I, _I, _I, C.c_float, C.c_float, C.c_float, _U64, _PD, _PD],
+    "tgx_mlp_evaluate": [_P, _I, _I, _PD, _PD, _PD],
+    "tgx_mlp_forward_host": [_P, _I, _PF, _PF, _PF],
+    "tgx_mlp_gradients_host": [_P, _I, _PF, _PF, _PF, C.c_float, C.c_float, _PF, _PF, _PF, _PF, _PF, _PF, _PD, _PD],
     "tgx_mlp_get_weights": [_P, _PF, _PF, _PF, _PF, _PF, _PF],
     "tgx_mlp_set_weights": [_P, _PF, _PF, _PF, _PF, _PF, _PF],
-    "tgx_model_training_view": [_P, _I, _PI, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)],
+    "tgx_surrogate_install": [_P, _I, _I, _P],
+    "tgx_surrogate_gate": [_P, _I, _D],
+    "tgx_surrogate_stats": [_P, _PD, _PD],
     "tgx_md_set_state": [_P, _PD, _PD],
     "tgx_md_get_state": [_P, _PD, _PD],
     "tgx_velocity_verlet": [_P, _D, _I],

@@ -1,0 +1,67 @@
+"""Molecular dynamics on the device with the semantics of TurboGAP's md
+(src/md.f90:95-345) and its step order (src/turbogap_md.f90:160-470):
+forces at x(t) -> velocity_verlet -> E_kin, T -> Berendsen rescale, with the
+neighbour list rebuilt every step (neighbors_buffer = 0 rebuilds on any motion,
+src/turbogap_md.f90:452-470)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import lib as L
+from . import structures as S
+
+KB = 8.6173303e-5  # eV/K (src/md.f90:104)
+
+
+def maxwell_velocities(masses: np.ndarray, T: float, seed: int) -> np.ndarray:
+    """randomize_velocities "maxwell" (src/md.f90:126-131): g sqrt(kB T / m), then
+    the centre-of-mass velocity removed."""
+    n = masses.shape[0]
+    g = S.normal(3 * n, 1.0, seed).reshape(n, 3)
+    v = g * np.sqrt(KB * T / masses)[:, None]
+    cm = (masses[:, None] * v).sum(0) / masses.sum()
+    return v - cm
+
+
+class MD:
+    def __init__(self, ctx, masses: np.ndarray, velocities: np.ndarray, rcut: float, path: str = "fp64_plain"):
+        self.ctx, self.rcut, self.path = ctx, rcut, path
+        self.masses = np.ascontiguousarray(masses, dtype=np.float64)
+        v = np.ascontiguousarray(velocities, dtype=np.float64)
+        L.check(L.load().tgx_md_set_state(ctx.h, L.dptr(v), L.dptr(self.masses)), "md_set_state")
+        self.step_count = 0
+        self.energy = None
+
+    def forces(self, path=None):
+        self.ctx.build_neighbors_list(self.rcut, fetch=False)
+        self.energy, self.virial = self.ctx.compute(path or self.path)
+        return self.energy
+
+    def kinetic(self):
+        ek, T = C.c_double(), C.c_double()
+        L.check(L.load().tgx_kinetic(self.ctx.h, C.byref(ek), C.byref(T)), "kinetic")
+        return ek.value, T.value
+
+    def step(self, dt: float, T0: float = None, tau: float = 100.0, path=None):
+        """One step: forces at x(t), velocity Verlet, optional Berendsen to T0, wrap."""
+        lib = L.load()
+        self.forces(path)
+        L.check(lib.tgx_velocity_verlet(self.ctx.h, float(dt), int(self.step_count == 0)), "velocity_verlet")
+        ek, T = self.kinetic()
+        if T0 is not None:
+            L.check(lib.tgx_berendsen(self.ctx.h, float(T0), float(T), float(tau), float(dt)), "berendsen")
+        L.check(lib.tgx_wrap_pbc(self.ctx.h), "wrap_pbc")
+        self.step_count += 1
+        return self.energy, ek, T
+
+    def state(self):
+        n = self.masses.shape[0]
+        x = np.empty((n, 3))
+        v = np.empty((n, 3))
+        L.check(L.load().tgx_md_get_state(self.ctx.h, L.dptr(x), L.dptr(v)), "md_get_state")
+        return x, v
+
+    def remove_cm(self):
+        L.check(L.load().tgx_remove_cm_vel(self.ctx.h), "remove_cm_vel")

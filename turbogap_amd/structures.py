@@ -133,3 +133,30 @@ def masses_for(species: np.ndarray, amu_by_code=None) -> np.ndarray:
     """Masses in eV fs^2/Å^2; carbon (12.011 amu) for every code by default."""
     amu_by_code = amu_by_code or {}
     return np.array([amu_by_code.get(int(s), 12.011) for s in species]) * AMU
+
+
+def gpu_describe(ctx_factory=None):
+    """A DescriptorFn that computes a block's centre descriptors on the GPU."""
+    from .context import Context
+
+    def describe(hypers: SoapTurbo, system: System) -> np.ndarray:
+        with (ctx_factory() if ctx_factory else Context(0)) as ctx:
+            ctx.set_system(system.positions, system.species, system.box)
+            ctx.build_neighbors_list(hypers.rcut_max, fetch=False)
+            mid = ctx.soap_model(hypers, np.zeros((1, hypers.dim)), np.zeros(1))
+            ctx.soap_backend_begin(mid)
+            q, _ = ctx.get_soap(derivatives=False)
+            ctx.soap_backend_end()
+        return q
+
+    return describe
+
+
+def config1_models(describe: DescriptorFn, n_sparse: int = 4000, seed: int = 9001):
+    """Both central-species blocks of config 1 with their sparse sets, taken from
+    seeded 16,384-atom structures at the same density."""
+    models = []
+    for k, h in enumerate(soap_config1()):
+        Qs, al = sparse_set(h, n_sparse, describe, lambda s: config1_system(16384, seed=s), seed=seed + 100 * k)
+        models.append((h, Qs, al))
+    return models

@@ -1,0 +1,115 @@
+"""MLP surrogate of the per-site GAP energy (host side).
+
+``MLP`` wraps one network in libtgx (tgx_mlp_*): its dataset, weights and
+optimiser state live on the device; training, evaluation and inference run on
+the library's own kernels. The energy targets are the GAP local energies E_i and
+the gradient targets the descriptor-space gradients dE_i/dq_i (= -Qss_i of
+src/gap.f90:184), both taken from FP64 tgx_compute runs; the network is then
+installed as an ensemble member of a block's surrogate (TGX_PATH_SURROGATE).
+
+The gradient term stands in for the force loss: F_j = -sum_i (dE_i/dq_i) .
+(dq_i/dr_j), so matching dE_i/dq_i in the directions a force can see (the
+tangent space of the normalised descriptor) matches every force built from it,
+while batches remain single environments.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import lib as L
+
+
+class MLP:
+    def __init__(self, ctx, dim: int, hidden: int = 256, max_batch: int = 4096, seed: int = 1):
+        self.ctx = ctx
+        self.dim, self.hidden, self.max_batch = dim, hidden, max_batch
+        h = C.c_void_p()
+        L.check(L.load().tgx_mlp_create(ctx.h, dim, hidden, max_batch, C.c_uint64(seed), C.byref(h)), "mlp_create")
+        self.h = h
+        ctx._children.append(self)  # the context destroys its networks before itself
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            L.load().tgx_mlp_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_samples(self, model_id: int, stride: int = 1, offset: int = 0) -> int:
+        n = C.c_int()
+        L.check(L.load().tgx_mlp_add_samples(self.h, model_id, stride, offset, C.byref(n)), "mlp_add_samples")
+        return n.value
+
+    def dataset_stats(self, n_use: int):
+        mu, sd = C.c_double(), C.c_double()
+        L.check(L.load().tgx_mlp_dataset_stats(self.h, n_use, C.byref(mu), C.byref(sd)), "mlp_dataset_stats")
+        return mu.value, sd.value
+
+    def set_normalization(self, shift: float, scale: float):
+        L.check(L.load().tgx_mlp_set_normalization(self.h, float(shift), float(scale)), "mlp_set_normalization")
+
+    def train(self, n_train: int, epochs: int, batch: int = 4096, lr: float = 1e-3, w_energy: float = 1.0,
+              w_grad: float = 1.0, seed: int = 7):
+        le, lg = C.c_double(), C.c_double()
+        L.check(L.load().tgx_mlp_train(self.h, n_train, epochs, batch, C.c_float(lr), C.c_float(w_energy),
+                                       C.c_float(w_grad), C.c_uint64(seed), C.byref(le), C.byref(lg)), "mlp_train")
+        return le.value, lg.value
+
+    def evaluate(self, begin: int, end: int):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        L.check(L.load().tgx_mlp_evaluate(self.h, begin, end, C.byref(a), C.byref(b), C.byref(c)), "mlp_evaluate")
+        return {"e_mae": a.value, "e_rmse": b.value, "g_rmse": c.value}
+
+    def forward(self, q: np.ndarray):
+        q = np.ascontiguousarray(q, dtype=np.float32)
+        B = q.shape[0]
+        e = np.empty(B, np.float32)
+        g = np.empty((B, self.dim), np.float32)
+        L.check(L.load().tgx_mlp_forward_host(self.h, B, L.fptr(q), L.fptr(e), L.fptr(g)), "mlp_forward_host")
+        return e, g
+
+    def gradients(self, q, e_ref, g_ref, w_energy=1.0, w_grad=1.0):
+        q = np.ascontiguousarray(q, dtype=np.float32)
+        e_ref = np.ascontiguousarray(e_ref, dtype=np.float32)
+        g_ref = np.ascontiguousarray(g_ref, dtype=np.float32)
+        H, D = self.hidden, self.dim
+        out = {"W1": np.empty((H, D), np.float32), "b1": np.empty(H, np.float32), "W2": np.empty((H, H), np.float32),
+               "b2": np.empty(H, np.float32), "w3": np.empty(H, np.float32), "b3": np.empty(1, np.float32)}
+        le, lg = C.c_double(), C.c_double()
+        L.check(L.load().tgx_mlp_gradients_host(self.h, q.shape[0], L.fptr(q), L.fptr(e_ref), L.fptr(g_ref),
+                                                C.c_float(w_energy), C.c_float(w_grad), L.fptr(out["W1"]),
+                                                L.fptr(out["b1"]), L.fptr(out["W2"]), L.fptr(out["b2"]),
+                                                L.fptr(out["w3"]), L.fptr(out["b3"]), C.byref(le), C.byref(lg)),
+                "mlp_gradients_host")
+        return out, le.value, lg.value
+
+    def get_weights(self):
+        H, D = self.hidden, self.dim
+        w = {"W1": np.empty((H, D), np.float32), "b1": np.empty(H, np.float32), "W2": np.empty((H, H), np.float32),
+             "b2": np.empty(H, np.float32), "w3": np.empty(H, np.float32), "b3": np.empty(1, np.float32)}
+        L.check(L.load().tgx_mlp_get_weights(self.h, *[L.fptr(w[k]) for k in ("W1", "b1", "W2", "b2", "w3", "b3")]),
+                "mlp_get_weights")
+        return w
+
+    def set_weights(self, w):
+        arr = [np.ascontiguousarray(w[k], dtype=np.float32) for k in ("W1", "b1", "W2", "b2", "w3", "b3")]
+        L.check(L.load().tgx_mlp_set_weights(self.h, *[L.fptr(a) for a in arr]), "mlp_set_weights")
+
+    def install(self, model_id: int, member: int = 0):
+        L.check(L.load().tgx_surrogate_install(self.ctx.h, model_id, member, self.h), "surrogate_install")
+
+
+def set_gate(ctx, n_members: int, threshold: float):
+    L.check(L.load().tgx_surrogate_gate(ctx.h, n_members, float(threshold)), "surrogate_gate")
+
+
+def gate_stats(ctx):
+    f, m = C.c_double(), C.c_double()
+    L.check(L.load().tgx_surrogate_stats(ctx.h, C.byref(f), C.byref(m)), "surrogate_stats")
+    return f.value, m.value
