@@ -43,7 +43,7 @@ def kernel_table(ctx, names):
     return out
 
 
-KERNELS = ["nbr_count", "nbr_fill", "nbr_reverse", "select_fill", "soap_site_der_f64", "soap_site_f64",
+KERNELS = ["nbr_count", "nbr_fill", "nbr_reverse", "select_fill", "soap_site_der_f64", "soap_desc_f64", "soap_desc_f32",
            "soap_site_f32", "soap_adjoint_f64", "soap_adjoint_f32", "gemm_kernel_f64", "gemm_qss_f64",
            "gemm_kernel_bf16", "gemm_qss_bf16", "plain_force", "force_gather", "virial", "mlp_"]
 

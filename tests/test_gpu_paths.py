@@ -1,0 +1,91 @@
+"""GPU: the adjoint, mixed-precision and finite-difference checks.
+
+The adjoint kernel's forces are checked against central finite differences of
+the FP64 energy (h = 1e-4 Å, positions away from the cutoff) and the virial
+against a strain finite difference, W = -dE/deps. The mixed path (FP32
+descriptors, BF16 MFMA contraction) is held to error bounds against the FP64
+oracle: per-atom energy MAE <= 0.1 meV/atom and force RMSE <= 1 meV/Å.
+"""
+import numpy as np
+import pytest
+
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _energy(ctx, pos, box, path):
+    ctx.set_positions(pos)
+    ctx.build_neighbors_list(3.7, fetch=False)
+    E, _ = ctx.compute(path)
+    return E
+
+
+def test_adjoint_forces_finite_difference(ctx):
+    sysm, h, Qs, al = H.config0(500)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    ctx.soap_model(h, Qs, al)
+    ctx.build_neighbors_list(3.7, fetch=False)
+    ctx.compute("fp64_adjoint")
+    _, F = ctx.results()
+    hstep = 1e-4
+    rng = np.random.default_rng(3)
+    for i in rng.choice(sysm.n, 4, replace=False):
+        for k in range(3):
+            p = sysm.positions.copy()
+            p[i, k] += hstep
+            ep = _energy(ctx, p, sysm.box, "fp64_adjoint")
+            p[i, k] -= 2 * hstep
+            em = _energy(ctx, p, sysm.box, "fp64_adjoint")
+            fd = -(ep - em) / (2 * hstep)
+            assert abs(fd - F[i, k]) <= 2e-6 + 1e-4 * abs(F[i, k]), (i, k, fd, F[i, k])
+
+
+def test_virial_strain_finite_difference(ctx):
+    sysm, h, Qs, al = H.config0(500)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    ctx.soap_model(h, Qs, al)
+    ctx.build_neighbors_list(3.7, fetch=False)
+    _, W = ctx.compute("fp64_adjoint")
+    eps = 1e-5
+    for k in range(3):
+        vals = []
+        for sgn in (1, -1):
+            S = np.eye(3)
+            S[k, k] += sgn * eps
+            box = sysm.box @ S
+            pos = sysm.positions @ S
+            ctx.set_system(pos, sysm.species, box)
+            ctx.build_neighbors_list(3.7, fetch=False)
+            E, _ = ctx.compute("fp64_adjoint")
+            vals.append(E)
+        fd = -(vals[0] - vals[1]) / (2 * eps)
+        assert abs(fd - W[k, k]) <= 1e-5 + 1e-4 * abs(W[k, k]), (k, fd, W[k, k])
+
+
+def test_mixed_path_error_bounds(ctx, orc):
+    sysm, models = H.subset(500, 4096)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    ctx.build_neighbors_list(3.7, fetch=False)
+    H.load_models(ctx, models)
+    E, W = ctx.compute("mixed")
+    en, F = ctx.results()
+    Eo, eno, Fo, Wo, _ = orc.compute(sysm, models)
+    assert np.mean(np.abs(en - eno)) <= 1e-4
+    assert np.sqrt(np.mean((F - Fo) ** 2)) <= 1e-3
+    assert np.abs(W - Wo).max() <= 0.05
+
+
+def test_fp64_paths_agree_config1_density(ctx):
+    """Plain (materialised dq/dr) and adjoint forces agree to round-off."""
+    sysm, models = H.subset(300, 4096)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    ctx.build_neighbors_list(3.7, fetch=False)
+    H.load_models(ctx, models)
+    Ep, Wp = ctx.compute("fp64_plain")
+    ep, Fp = ctx.results()
+    Ea, Wa = ctx.compute("fp64_adjoint")
+    ea, Fa = ctx.results()
+    assert abs(Ep - Ea) <= 1e-12 * max(1.0, abs(Ep))
+    assert np.abs(Fp - Fa).max() <= 1e-11
+    assert np.abs(Wp - Wa).max() <= 1e-10

@@ -15,6 +15,9 @@
 // the power spectrum and its derivative are split over threads by component.
 #include "tgx_internal.h"
 
+int tgx_desc_wave(tgx_ctx* ctx, tgx_model_t* m, bool fp64, const int* d_sites, int n_rows, void* q_out, void* pn_out);
+int tgx_adjoint_wave(tgx_ctx* ctx, tgx_model_t* m, bool fp64);
+
 namespace {
 
 constexpr int WG = 256;
@@ -472,121 +475,6 @@ __global__ __launch_bounds__(WG) void k_soap_site(SoapParams P, SiteArgs a, int 
   }
 }
 
-// Adjoint forces. For site i: g = (Qss - (Qss.q) q)/|p| is dE... contracted with
-// dp/dq; Hg(n, lm) = d(g.p)/dc; for every pair j the force on j is
-//   f_ij = sum_{n in block, l} [ (R A)' u sHg + R A (vgHg - l sHg u)/r ]
-// and the self pair receives -sum_j f_ij. Pair forces are accumulated into the
-// global pair array fglob (one writer per pair, blocks in stream order).
-template <class T>
-__global__ __launch_bounds__(WG) void k_soap_adjoint(SoapParams P, SiteArgs a, int n_sites, const T* __restrict__ q_in,
-                                                     const T* __restrict__ pnorm_in, const T* __restrict__ qss,
-                                                     double* __restrict__ fglob) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* sm = reinterpret_cast<T*>(smem_raw);
-  const int site = blockIdx.x;
-  if (site >= n_sites) return;
-  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-  const int NLM = P.NLM, N = P.N, L = P.L, L1 = L + 1, D = P.D;
-  T* c = sm;                       // N*NLM
-  T* X = c + N * NLM;              // D: g
-  T* red = X + D;                  // WG
-  T* H = red + WG;                 // N*NLM
-  T* wscr = H + N * NLM;           // 4 pair scratches
-  const int64_t b0 = a.site_beg[site];
-  const int nn = (int)(a.site_beg[site + 1] - b0);
-  site_coefficients<T>(P, a, b0, nn, c, wscr);
-  const T nrm = pnorm_in[site];
-  const T* qrow = q_in + (size_t)site * P.Dpad;
-  const T* srow = qss + (size_t)site * P.Dpad;
-  // Qss . q
-  T acc = 0;
-  for (int d = tid; d < D; d += WG) acc += srow[d] * qrow[d];
-  red[tid] = acc;
-  __syncthreads();
-  for (int o = WG / 2; o > 0; o >>= 1) {
-    if (tid < o) red[tid] += red[tid + o];
-    __syncthreads();
-  }
-  const T sq = red[0];
-  const T inv_n = nrm > (T)0 ? (T)1 / nrm : (T)0;
-  for (int d = tid; d < D; d += WG) X[d] = (srow[d] - sq * qrow[d]) * inv_n;
-  __syncthreads();
-  contract_channel<T>(P, X, c, H);
-  __syncthreads();
-  PairScratch<T> ps;
-  ps.bind(wscr + w * PairScratch<T>::size(L, NLM), L, NLM);
-  double self0 = 0, self1 = 0, self2 = 0;  // lane 0: this wave's pairs, in order
-  for (int g = 1; g < nn; g += 4) {
-    const int k = g + w;
-    int s = -1;
-    T r = 0, ux = 0, uy = 0, uz = 0;
-    if (k < nn) {
-      s = a.pr_spec[b0 + k];
-      if (s >= 0 && !(a.pr_r[b0 + k] < P.sp[s].rc)) s = -1;
-      if (s >= 0) {
-        r = (T)a.pr_r[b0 + k];
-        const T inv = (T)1 / r;
-        ux = (T)a.pr_xyz[3 * (b0 + k)] * inv;
-        uy = (T)a.pr_xyz[3 * (b0 + k) + 1] * inv;
-        uz = (T)a.pr_xyz[3 * (b0 + k) + 2] * inv;
-      }
-    }
-    if (s >= 0) pair_step1<T>(P, s, r, ux, uy, uz, lane, 64, ps, true);
-    __syncthreads();
-    if (s >= 0) pair_step2<T>(P, s, r, lane, 64, ps);
-    __syncthreads();
-    if (k < nn) {
-      T a0 = 0, a1 = 0, a2 = 0;
-      if (s >= 0) {
-        const int nb = P.sp[s].nb, off = P.sp[s].off;
-        const T inv_r = (T)1 / r;
-        for (int tt = lane; tt < nb * L1; tt += 64) {
-          const int n = tt / L1, l = tt - n * L1;
-          const T* h = H + (off + n) * NLM;
-          T s0 = 0, g0 = 0, g1 = 0, g2 = 0;
-          for (int lm = l * l; lm < (l + 1) * (l + 1); ++lm) {
-            const T v = h[lm];
-            s0 += ps.Y[lm] * v;
-            g0 += ps.gY[3 * lm] * v;
-            g1 += ps.gY[3 * lm + 1] * v;
-            g2 += ps.gY[3 * lm + 2] * v;
-          }
-          const T E = ps.R[n] * ps.A[l];
-          const T Dd = ps.dR[n] * ps.A[l] + ps.R[n] * ps.dA[l];
-          const T lf = (T)l;
-          a0 += Dd * s0 * ux + E * (g0 - lf * s0 * ux) * inv_r;
-          a1 += Dd * s0 * uy + E * (g1 - lf * s0 * uy) * inv_r;
-          a2 += Dd * s0 * uz + E * (g2 - lf * s0 * uz) * inv_r;
-        }
-      }
-      for (int o = 32; o > 0; o >>= 1) {
-        a0 += __shfl_xor(a0, o);
-        a1 += __shfl_xor(a1, o);
-        a2 += __shfl_xor(a2, o);
-      }
-      if (lane == 0) {
-        self0 += (double)a0; self1 += (double)a1; self2 += (double)a2;
-        const int64_t gi = a.pr_gidx[b0 + k];
-        fglob[3 * gi] += (double)a0;
-        fglob[3 * gi + 1] += (double)a1;
-        fglob[3 * gi + 2] += (double)a2;
-      }
-    }
-    __syncthreads();
-  }
-  double* wsum = reinterpret_cast<double*>(red);  // 4 waves x 3 (red holds >= 12 doubles)
-  if (lane == 0) { wsum[3 * w] = self0; wsum[3 * w + 1] = self1; wsum[3 * w + 2] = self2; }
-  __syncthreads();
-  if (tid == 0) {
-    double s0 = 0, s1 = 0, s2 = 0;
-    for (int ww = 0; ww < 4; ++ww) { s0 += wsum[3 * ww]; s1 += wsum[3 * ww + 1]; s2 += wsum[3 * ww + 2]; }
-    const int64_t gi = a.pr_gidx[b0];
-    fglob[3 * gi] -= s0;
-    fglob[3 * gi + 1] -= s1;
-    fglob[3 * gi + 2] -= s2;
-  }
-}
-
 // Plain-path pair force: f_p = Qss_i . soap_cart_der(:, :, p), one wavefront per
 // pair, streamed over the (3, D) slab. This is the scatter of src/gap.f90:206-224.
 __global__ __launch_bounds__(256) void k_plain_force(int n_sites, const int64_t* __restrict__ site_beg, int D,
@@ -639,7 +527,6 @@ static size_t soap_lds_bytes(const SoapParams& P, int mode, size_t elt) {
     size_t d_scr = ps + (size_t)N * L1 * 6 + (size_t)TGX_MAX_NB * L1 * 4 + 4;
     n += d_scr > 4 * ps ? d_scr : 4 * ps;
   }
-  if (mode == 2) n += (size_t)N * NLM + 4 * ps;
   return n * elt;
 }
 
@@ -684,37 +571,17 @@ int tgx_soap_descriptors(tgx_ctx* ctx, tgx_model_t* m, int path, bool with_der) 
       else TGX_SOAP_DER(24);
 #undef TGX_SOAP_DER
     } else {
-      const size_t lds = soap_lds_bytes(m->P, 0, sizeof(double));
-      TGX_TRY(check_lds(lds));
-      TGX_LAUNCH(ctx, "soap_site_f64", (k_soap_site<double, 0, 1>), dim3(ns), dim3(WG), lds, m->P, a, ns,
-                 m->q64.as<double>(), m->pnorm.as<double>(), (double*)nullptr, (const int*)nullptr);
+      TGX_TRY(tgx_desc_wave(ctx, m, true, nullptr, ns, m->q64.p, m->pnorm.p));
     }
   } else {
     TGX_TRY(m->q32.ensure(sizeof(float) * nsp * m->Dpad));
-    const size_t lds = soap_lds_bytes(m->P, 0, sizeof(float));
-    TGX_TRY(check_lds(lds));
-    TGX_LAUNCH(ctx, "soap_site_f32", (k_soap_site<float, 0, 1>), dim3(ns), dim3(WG), lds, m->P, a, ns,
-               m->q32.as<float>(), m->pnorm.as<float>(), (double*)nullptr, (const int*)nullptr);
+    TGX_TRY(tgx_desc_wave(ctx, m, false, nullptr, ns, m->q32.p, m->pnorm.p));
   }
   return TGX_OK;
 }
 
 int tgx_adjoint_pair_forces(tgx_ctx* ctx, tgx_model_t* m, int path) {
-  const int ns = m->n_sites_b;
-  if (ns == 0) return TGX_OK;
-  SiteArgs a = site_args(m);
-  if (path == TGX_PATH_FP64_ADJOINT || path == TGX_PATH_FP64_PLAIN) {
-    const size_t lds = soap_lds_bytes(m->P, 2, sizeof(double));
-    TGX_TRY(check_lds(lds));
-    TGX_LAUNCH(ctx, "soap_adjoint_f64", k_soap_adjoint<double>, dim3(ns), dim3(WG), lds, m->P, a, ns,
-               m->q64.as<double>(), m->pnorm.as<double>(), m->qss64.as<double>(), ctx->fglob.as<double>());
-  } else {
-    const size_t lds = soap_lds_bytes(m->P, 2, sizeof(float));
-    TGX_TRY(check_lds(lds));
-    TGX_LAUNCH(ctx, "soap_adjoint_f32", k_soap_adjoint<float>, dim3(ns), dim3(WG), lds, m->P, a, ns,
-               m->q32.as<float>(), m->pnorm.as<float>(), m->qss32.as<float>(), ctx->fglob.as<double>());
-  }
-  return TGX_OK;
+  return tgx_adjoint_wave(ctx, m, path == TGX_PATH_FP64_ADJOINT || path == TGX_PATH_FP64_PLAIN);
 }
 
 int tgx_plain_pair_forces(tgx_ctx* ctx, tgx_model_t* m) {
@@ -734,11 +601,5 @@ int tgx_plain_pair_forces(tgx_ctx* ctx, tgx_model_t* m) {
 // holding site d_sites[r] (used to re-evaluate gated sites with the GAP).
 int tgx_soap_descriptors_sites(tgx_ctx* ctx, tgx_model_t* m, const int* d_sites, int n_list, double* q_out,
                                double* pnorm_out) {
-  if (n_list == 0) return TGX_OK;
-  SiteArgs a = site_args(m);
-  const size_t lds = soap_lds_bytes(m->P, 0, sizeof(double));
-  TGX_TRY(check_lds(lds));
-  TGX_LAUNCH(ctx, "soap_site_list_f64", (k_soap_site<double, 0, 1>), dim3(n_list), dim3(WG), lds, m->P, a, n_list,
-             q_out, pnorm_out, (double*)nullptr, d_sites);
-  return TGX_OK;
+  return tgx_desc_wave(ctx, m, true, d_sites, n_list, q_out, pnorm_out);
 }
