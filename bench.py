@@ -43,9 +43,9 @@ def kernel_table(ctx, names):
     return out
 
 
-KERNELS = ["nbr_count", "nbr_fill", "nbr_reverse", "select_fill", "soap_site_der_f64", "soap_desc_f64", "soap_desc_f32",
+KERNELS = ["mlp_l1", "mlp_back1", "mlp_l2", "mlp_back2", "nbr_count", "nbr_fill", "nbr_reverse", "select_fill", "soap_site_der_f64", "soap_desc_f64", "soap_desc_f32",
            "soap_site_f32", "soap_adjoint_f64", "soap_adjoint_f32", "gemm_kernel_f64", "gemm_qss_f64",
-           "gemm_kernel_bf16", "gemm_qss_bf16", "plain_force", "force_gather", "virial", "mlp_"]
+           "gemm_kernel_bf16", "gemm_qss_bf16", "plain_force", "force_gather", "virial"]
 
 
 def run_path(ctx, path, steps, warmup, rcut):
@@ -94,6 +94,59 @@ def roofline_entries(stats, n_atoms, blocks, n_pairs):
     return rows
 
 
+def prepare_surrogate(ctx, ids, sysm, models, args):
+    """Dataset from FP64 MD at 300/1000/2000/3000 K (one snapshot after
+    args.md_steps steps each), an ensemble of args.members MLPs per block trained
+    by the library's kernels, held-out errors on a 1500 K snapshot."""
+    from turbogap_amd import surrogate as SG
+    from turbogap_amd.md import MD, maxwell_velocities
+    from turbogap_amd import structures as S
+    t0 = time.perf_counter()
+    D = models[0][0].dim
+    nets = [[SG.MLP(ctx, D, 256, 4096, seed=101 + 10 * k + i) for i in range(args.members)] for k in range(len(ids))]
+    SG.md_dataset(ctx, ids, sysm, nets, steps=args.md_steps, log=log)
+    n_train = [nets[k][0].add_samples(ids[k], 1 << 30, 1 << 30) for k in range(len(ids))]
+    # held-out snapshot: 1500 K
+    mass = S.masses_for(sysm.species)
+    md = MD(ctx, mass, maxwell_velocities(mass, 1500.0, seed=99), 3.7, "fp64_plain")
+    for _ in range(args.md_steps):
+        md.step(1.0, T0=1500.0, tau=100.0)
+    md.forces("fp64_plain")
+    for k, mid in enumerate(ids):
+        for net in nets[k]:
+            net.add_samples(mid, 1, 0)
+    held_pos, _ = md.state()
+    n_all = [nets[k][0].add_samples(ids[k], 1 << 30, 1 << 30) for k in range(len(ids))]
+    log(f"dataset: train {n_train}, held-out {[a - b for a, b in zip(n_all, n_train)]} ({time.perf_counter()-t0:.1f} s)")
+    SG.train_ensemble(nets, n_train, epochs=args.epochs, log=log)
+    held = {}
+    for k in range(len(ids)):
+        ev = [nets[k][i].evaluate(n_train[k], n_all[k]) for i in range(args.members)]
+        held[f"block{k}"] = {"e_mae_meV": 1e3 * float(np.mean([e["e_mae"] for e in ev])),
+                             "grad_rmse_meV": 1e3 * float(np.mean([e["g_rmse"] for e in ev]))}
+        for i in range(args.members):
+            nets[k][i].install(ids[k], i)
+    SG.set_gate(ctx, args.members, args.gate)
+    # held-out forces through the full path vs FP64 GAP on the 1500 K snapshot
+    ctx.set_positions(held_pos)
+    ctx.build_neighbors_list(3.7, fetch=False)
+    ctx.compute("fp64_plain")
+    e64, F64 = ctx.results()
+    ctx.compute("surrogate")
+    es, Fs = ctx.results()
+    gated, spread = SG.gate_stats(ctx)
+    ctx.set_positions(sysm.positions)
+    info = {"train_envs": int(sum(n_train)), "heldout_envs": int(sum(n_all) - sum(n_train)),
+            "members": args.members, "epochs": args.epochs, "heldout_per_block": held,
+            "heldout_snapshot_1500K": {"energy_mae_meV_atom": 1e3 * float(np.mean(np.abs(es - e64))),
+                                       "force_rmse_meV_A": 1e3 * float(np.sqrt(np.mean((Fs - F64) ** 2))),
+                                       "force_rms_ref_meV_A": 1e3 * float(np.sqrt(np.mean(F64 ** 2))),
+                                       "gated_fraction": gated, "max_spread_meV": 1e3 * spread},
+            "prep_s": time.perf_counter() - t0}
+    log(f"surrogate ready: {info}")
+    return info
+
+
 def cpu_baseline(models, n_sample, threads):
     """The FP64 oracle on a bounded sample of the workload: a box of n_sample
     atoms at config 1's density with the same blocks and sparse sets."""
@@ -116,7 +169,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--atoms", type=int, default=65536)
     ap.add_argument("--sparse", type=int, default=4000)
-    ap.add_argument("--paths", default="fp64_plain,fp64_adjoint,mixed")
+    ap.add_argument("--paths", default="fp64_plain,fp64_adjoint,mixed,surrogate")
+    ap.add_argument("--members", type=int, default=3)
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--md-steps", type=int, default=50)
+    ap.add_argument("--gate", type=float, default=0.020)
     ap.add_argument("--cpu-sample", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -148,6 +205,9 @@ def main():
             ctx.soap_backend_end()
             blocks.append({"n_sites": ns.value, "n_pairs": npb.value, "D": h.dim, "S": args.sparse})
         log(f"pairs: {nl['n_atom_pairs']}, blocks: {blocks}")
+        surr = None
+        if "surrogate" in paths:
+            surr = prepare_surrogate(ctx, ids, sysm, models, args)
         for p in paths:
             r = run_path(ctx, p, args.steps, args.warmup, 3.7)
             res[p] = r
@@ -203,6 +263,7 @@ def main():
         "paths_atom_steps_per_s": {p: sysm.n / (r["ms_per_step"] * 1e-3) for p, r in res.items()},
         "speedup_vs_fp64_plain": {p: fp64["ms_per_step"] / r["ms_per_step"] for p, r in res.items()} if fp64 else None,
         "accuracy_vs_fp64_gpu": acc,
+        "surrogate": surr,
         "roofline": roof, "subsystems_fp64_plain": sub,
         "cpu_baseline": cpu,
     }

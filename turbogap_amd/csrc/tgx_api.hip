@@ -9,6 +9,7 @@
 
 int tgx_energy_scatter(tgx_ctx* ctx, tgx_model_t* m);
 int tgx_block_seen(tgx_ctx* ctx, tgx_model_t* m, int* d_seen);
+int tgx_radial_table_build(tgx_ctx* ctx, tgx_model_t* m);
 
 #define TGX_CTX_CHECK(ctx)                                      \
   do {                                                          \
@@ -99,7 +100,7 @@ extern "C" int tgx_init(int device, tgx_ctx** out) {
 }
 
 static void free_model(tgx_model_t* m) {
-  dbuf* bufs[] = {&m->B, &m->xq, &m->cen, &m->dtab, &m->ynorm, &m->Qs, &m->QsT, &m->alpha, &m->Qs_bf, &m->QsT_bf, &m->alpha_f,
+  dbuf* bufs[] = {&m->B, &m->xq, &m->cen, &m->dtab, &m->ynorm, &m->rtab, &m->Qs, &m->QsT, &m->alpha, &m->Qs_bf, &m->QsT_bf, &m->alpha_f,
                   &m->site_atom, &m->site_beg, &m->pr_gidx, &m->pr_nbr, &m->pr_spec, &m->pr_r, &m->pr_xyz, &m->q64,
                   &m->q32, &m->qbf, &m->pnorm, &m->e_site, &m->qss64, &m->qss32, &m->kd64, &m->kdbf, &m->epart,
                   &m->soap_der, &m->fpair};
@@ -380,6 +381,11 @@ extern "C" int tgx_soap_model_create(tgx_ctx* ctx, const tgx_soap_hypers* h, con
   P.cen = m->cen.as<double>();
   P.dtab = m->dtab.as<int>();
   P.ynorm = m->ynorm.as<double>();
+  if ((st = tgx_radial_table_build(ctx, m)) != TGX_OK) return fail(st);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    tgx_set_error("radial table build failed");
+    return fail(TGX_ERR_HIP);
+  }
   ctx->models.push_back(m);
   *model_id = (int)ctx->models.size() - 1;
   return TGX_OK;

@@ -116,7 +116,7 @@ struct tgx_model_t {
   double rcut_max = 0.0;
   SoapParams P{};
   // device copies of the tables
-  dbuf B, xq, cen, dtab, ynorm;
+  dbuf B, xq, cen, dtab, ynorm, rtab;  // rtab: FP32 Hermite table of R_n, dR_n/dr
   // sparse set: Qs[Spad][Dpad], QsT[Dpad][Spad], alphas[Spad] (FP64) and BF16 copies
   dbuf Qs, QsT, alpha, Qs_bf, QsT_bf, alpha_f;
   // surrogate members (BF16 weights + FP32 biases)

@@ -17,6 +17,8 @@
 
 int tgx_desc_wave(tgx_ctx* ctx, tgx_model_t* m, bool fp64, const int* d_sites, int n_rows, void* q_out, void* pn_out);
 int tgx_adjoint_wave(tgx_ctx* ctx, tgx_model_t* m, bool fp64);
+int tgx_desc_f32(tgx_ctx* ctx, tgx_model_t* m);
+int tgx_adjoint_f32(tgx_ctx* ctx, tgx_model_t* m);
 
 namespace {
 
@@ -575,13 +577,16 @@ int tgx_soap_descriptors(tgx_ctx* ctx, tgx_model_t* m, int path, bool with_der) 
     }
   } else {
     TGX_TRY(m->q32.ensure(sizeof(float) * nsp * m->Dpad));
-    TGX_TRY(tgx_desc_wave(ctx, m, false, nullptr, ns, m->q32.p, m->pnorm.p));
+    if (m->P.N <= 16) TGX_TRY(tgx_desc_f32(ctx, m));
+    else TGX_TRY(tgx_desc_wave(ctx, m, false, nullptr, ns, m->q32.p, m->pnorm.p));
   }
   return TGX_OK;
 }
 
 int tgx_adjoint_pair_forces(tgx_ctx* ctx, tgx_model_t* m, int path) {
-  return tgx_adjoint_wave(ctx, m, path == TGX_PATH_FP64_ADJOINT || path == TGX_PATH_FP64_PLAIN);
+  const bool fp64 = path == TGX_PATH_FP64_ADJOINT || path == TGX_PATH_FP64_PLAIN;
+  if (!fp64 && m->P.N <= 16) return tgx_adjoint_f32(ctx, m);
+  return tgx_adjoint_wave(ctx, m, fp64);
 }
 
 int tgx_plain_pair_forces(tgx_ctx* ctx, tgx_model_t* m) {

@@ -113,3 +113,42 @@ def gate_stats(ctx):
     f, m = C.c_double(), C.c_double()
     L.check(L.load().tgx_surrogate_stats(ctx.h, C.byref(f), C.byref(m)), "surrogate_stats")
     return f.value, m.value
+
+
+def md_dataset(ctx, model_ids, system, mlps, temperatures=(300.0, 1000.0, 2000.0, 3000.0), steps=50, dt=1.0,
+               tau=100.0, seed=17, path="fp64_plain", log=None):
+    """Environments for training: for each temperature, NVT Berendsen MD from the
+    initial structure with seeded Maxwell velocities, one snapshot after `steps`
+    steps (dt in fs). The snapshot's centres (q, E_i, dE_i/dq_i from the FP64
+    path) of block k are appended to mlps[k][...] (every network of the block)."""
+    from .md import MD, maxwell_velocities
+    from . import structures as S
+    mass = S.masses_for(system.species)
+    for it, T in enumerate(temperatures):
+        ctx.set_positions(system.positions)
+        md = MD(ctx, mass, maxwell_velocities(mass, T, seed=seed + it), 3.7, path)
+        for _ in range(steps):
+            E, ek, Tinst = md.step(dt, T0=T, tau=tau)
+        md.forces(path)  # the snapshot: FP64 results at the final positions
+        for k, mid in enumerate(model_ids):
+            for net in mlps[k]:
+                net.add_samples(mid, 1, 0)
+        if log:
+            log(f"md dataset: T={T:.0f} K, {steps} steps, E={md.energy:.6f} eV, T_inst={Tinst:.1f} K")
+    ctx.set_positions(system.positions)
+
+
+def train_ensemble(mlps, n_samples, epochs=30, batch=4096, lr=1e-3, w_energy=1.0, w_grad=1.0, log=None):
+    """Train every network on samples [0, n_samples[k]) of its block with the
+    energy mean/std of those samples as normalisation; member i shuffles with
+    seed 1000 k + i."""
+    import time
+    for k, nets in enumerate(mlps):
+        for i, net in enumerate(nets):
+            mu, sd = net.dataset_stats(n_samples[k])
+            net.set_normalization(mu, sd if sd > 0 else 1.0)
+            t0 = time.perf_counter()
+            le, lg = net.train(n_samples[k], epochs, batch, lr, w_energy, w_grad, seed=1000 * k + i)
+            if log:
+                log(f"train block {k} member {i}: {n_samples[k]} samples, {epochs} epochs, "
+                    f"loss_E={le:.4g} loss_g={lg:.4g} ({time.perf_counter() - t0:.1f} s)")
