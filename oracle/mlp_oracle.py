@@ -1,0 +1,89 @@
+"""FP64 numpy restatement of the surrogate MLP and its training gradients.
+
+TEST INFRASTRUCTURE ONLY (tests/ use it to check libtgx's MLP kernels).
+
+Network: z1 = W1 q + b1, h1 = silu(z1), z2 = W2 h1 + b2, h2 = silu(z2),
+E = w3 . h2 + b3, g = dE/dq. Loss of a batch of B rows:
+    L = w_E mean_b (E_b - e_b)^2 + w_G mean_b |u_b|^2,  u_b = P_b (g_b - g_ref_b),
+    P_b = 1 - q_b q_b^T.
+The gradient of the second term is the reverse-mode derivative of u . g(q)
+with u fixed (forward mode along u, then backward) -- written here from the
+chain rule independently of the HIP kernels. `bf16` rounds the operands the
+kernels feed to BF16 matrix products (round to nearest even).
+"""
+import numpy as np
+
+
+def bf16(x):
+    x = np.asarray(x, dtype=np.float32)
+    b = x.view(np.uint32).astype(np.uint64)
+    rounded = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16
+    return rounded.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def sig(z):
+    return 1.0 / (1.0 + np.exp(-z))
+
+
+def silu(z):
+    return z * sig(z)
+
+
+def silu1(z):
+    s = sig(z)
+    return s * (1 + z * (1 - s))
+
+
+def silu2(z):
+    s = sig(z)
+    return s * (1 - s) * (2 + z * (1 - 2 * s))
+
+
+def forward(w, q, emulate_bf16=False):
+    r = bf16 if emulate_bf16 else (lambda a: np.asarray(a, np.float64))
+    W1, b1, W2, b2, w3, b3 = (np.asarray(w[k], np.float64) for k in ("W1", "b1", "W2", "b2", "w3", "b3"))
+    z1 = r(q) @ r(W1).T + b1
+    h1 = silu(z1)
+    z2 = r(h1) @ r(W2).T + b2
+    h2 = silu(z2)
+    E = h2 @ w3 + b3[0]
+    a2 = w3 * silu1(z2)
+    v1 = r(a2) @ r(W2)
+    a1 = v1 * silu1(z1)
+    g = r(a1) @ r(W1)
+    return dict(z1=z1, h1=h1, z2=z2, h2=h2, E=E, a2=a2, v1=v1, a1=a1, g=g)
+
+
+def loss_and_grads(w, q, e_ref, g_ref, w_e=1.0, w_g=1.0, emulate_bf16=False):
+    r = bf16 if emulate_bf16 else (lambda a: np.asarray(a, np.float64))
+    q = np.asarray(q, np.float64)
+    B = q.shape[0]
+    W1, W2, w3 = (np.asarray(w[k], np.float64) for k in ("W1", "W2", "w3"))
+    f = forward(w, q, emulate_bf16)
+    res = f["E"] - e_ref
+    d = f["g"] - g_ref
+    u = d - q * np.sum(q * d, axis=1, keepdims=True)
+    loss_e = np.mean(res ** 2)
+    loss_g = np.mean(np.sum(u * u, axis=1))
+    dE = 2 * w_e * res / B
+    cG = 2 * w_g / B
+    s1, s1pp = silu1(f["z1"]), silu2(f["z1"])
+    s2, s2pp = silu1(f["z2"]), silu2(f["z2"])
+    # forward mode along u
+    t0 = r(u) @ r(W1).T
+    t1 = s1 * t0
+    t2 = r(t1) @ r(W2).T
+    gam2 = s2 * t2
+    zet2 = w3 * t2 * s2pp
+    wz = r(zet2) @ r(W2)
+    zet1 = f["v1"] * t0 * s1pp + wz * s1
+    a1, a2, h1, h2 = f["a1"], f["a2"], f["h1"], f["h2"]
+    g = {
+        "W1": a1.T @ (dE[:, None] * q + cG * u) + cG * zet1.T @ q,
+        "b1": (dE[:, None] * a1 + cG * zet1).sum(0),
+        "W2": a2.T @ (dE[:, None] * h1 + cG * t1) + cG * zet2.T @ h1,
+        "b2": (dE[:, None] * a2 + cG * zet2).sum(0),
+        "w3": (dE[:, None] * h2 + cG * gam2).sum(0),
+        "b3": np.array([dE.sum()]),
+    }
+    return loss_e, loss_g, g

@@ -1,0 +1,134 @@
+"""GPU tests of the learned path and the device MD.
+
+  * MLP: energies, input gradients, losses and weight gradients of the library's
+    kernels against oracle/mlp_oracle.py with the same BF16 operand rounding
+    (relative tolerance 1e-2 on each tensor: BF16 roundings of intermediate
+    activations can differ by one BF16 ulp where FP32 and FP64 accumulations
+    straddle a rounding boundary).
+  * Surrogate: trained on FP64-GAP MD environments of a 4,096-atom box, it
+    beats the untrained network on a held-out snapshot; with the ensemble gate
+    forced on every centre the surrogate path returns the FP64 GAP results.
+  * MD: one velocity-Verlet step and a Berendsen rescale against hand
+    arithmetic (src/md.f90:226-313).
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as MO
+from tests import helpers as H
+from turbogap_amd import structures as S
+from turbogap_amd import surrogate as SG
+from turbogap_amd.md import MD, maxwell_velocities, KB
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30)
+
+
+def test_mlp_kernels_match_oracle(ctx):
+    rng = np.random.default_rng(0)
+    D, Hd, B = 300, 128, 128
+    net = SG.MLP(ctx, D, Hd, 256, seed=3)
+    w = {"W1": rng.normal(size=(Hd, D)).astype(np.float32), "b1": (0.1 * rng.normal(size=Hd)).astype(np.float32),
+         "W2": (rng.normal(size=(Hd, Hd)) / np.sqrt(Hd)).astype(np.float32),
+         "b2": (0.1 * rng.normal(size=Hd)).astype(np.float32),
+         "w3": (rng.normal(size=Hd) / np.sqrt(Hd)).astype(np.float32), "b3": np.array([0.02], np.float32)}
+    net.set_weights(w)
+    q = rng.normal(size=(B, D))
+    q /= np.linalg.norm(q, axis=1)[:, None]
+    q = q.astype(np.float32)
+    e_ref = rng.normal(size=B).astype(np.float32)
+    g_ref = rng.normal(size=(B, D)).astype(np.float32) * 0.3
+    e, g = net.forward(q)
+    f = MO.forward(w, q, emulate_bf16=True)
+    assert _rel(e, f["E"]) < 1e-2 and _rel(g, f["g"]) < 1e-2
+    grads, le, lg = net.gradients(q, e_ref, g_ref, 0.7, 1.3)
+    le_o, lg_o, go = MO.loss_and_grads(w, q.astype(np.float64), e_ref.astype(np.float64),
+                                       g_ref.astype(np.float64), 0.7, 1.3, emulate_bf16=True)
+    assert abs(le - le_o) <= 1e-2 * le_o and abs(lg - lg_o) <= 1e-2 * lg_o
+    for k in ("W1", "b1", "W2", "b2", "w3", "b3"):
+        assert _rel(grads[k], go[k]) < 2e-2, (k, _rel(grads[k], go[k]))
+
+
+def _small_models(ctx, n_sparse=300):
+    sysm, models = H.subset(n_sparse, 4096)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    ids = H.load_models(ctx, models)
+    return sysm, models, ids
+
+
+def test_surrogate_learns_and_gate_falls_back(ctx):
+    sysm, models, ids = _small_models(ctx)
+    D = models[0][0].dim
+    nets = [[SG.MLP(ctx, D, 256, 1024, seed=11 + k)] for k in range(2)]
+    untrained = [[SG.MLP(ctx, D, 256, 1024, seed=11 + k)] for k in range(2)]
+    SG.md_dataset(ctx, ids, sysm, nets, temperatures=(500.0, 2500.0), steps=10)
+    n_train = [nets[k][0].add_samples(ids[k], 1 << 30, 1 << 30) for k in range(2)]
+    # held-out snapshot at 1500 K appended after the training samples
+    mass = S.masses_for(sysm.species)
+    md = MD(ctx, mass, maxwell_velocities(mass, 1500.0, seed=5), 3.7, "fp64_plain")
+    for _ in range(10):
+        md.step(1.0, T0=1500.0)
+    md.forces("fp64_plain")
+    for k in range(2):
+        nets[k][0].add_samples(ids[k], 1, 0)
+        untrained[k][0].add_samples(ids[k], 1, 0)
+    n_all = [nets[k][0].add_samples(ids[k], 1 << 30, 1 << 30) for k in range(2)]
+    SG.train_ensemble(nets, n_train, epochs=15, batch=1024)
+    for k in range(2):
+        mu, sd = nets[k][0].dataset_stats(n_train[k])
+        untrained[k][0].set_normalization(mu, sd)
+        ev = nets[k][0].evaluate(n_train[k], n_all[k])
+        ev0 = untrained[k][0].evaluate(0, n_all[k] - n_train[k])
+        assert ev["e_mae"] < 0.5 * ev0["e_mae"] and ev["g_rmse"] < 0.5 * ev0["g_rmse"], (k, ev, ev0)
+        nets[k][0].install(ids[k], 0)
+        untrained[k][0].install(ids[k], 1)
+    held, _ = md.state()
+    ctx.set_positions(held)
+    ctx.build_neighbors_list(3.7, fetch=False)
+    E64, W64 = ctx.compute("fp64_adjoint")
+    e64, F64 = ctx.results()
+    SG.set_gate(ctx, 1, 1e9)
+    Es, Ws = ctx.compute("surrogate")
+    es, Fs = ctx.results()
+    assert np.isfinite(es).all() and np.sqrt(np.mean((Fs - F64) ** 2)) < 0.05
+    # two very different members and a zero threshold: every centre falls back to the FP64 GAP
+    SG.set_gate(ctx, 2, 1e-12)
+    Eg, Wg = ctx.compute("surrogate")
+    eg, Fg = ctx.results()
+    frac, spread = SG.gate_stats(ctx)
+    assert frac == 1.0 and spread > 0
+    np.testing.assert_allclose(eg, e64, atol=1e-10)
+    assert np.abs(Fg - F64).max() < 1e-4
+
+
+def test_velocity_verlet_and_berendsen(ctx):
+    sysm, h, Qs, al = H.config0(300)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    ctx.soap_model(h, Qs, al)
+    mass = S.masses_for(sysm.species)
+    v0 = maxwell_velocities(mass, 800.0, seed=3)
+    md = MD(ctx, mass, v0, 3.7, "fp64_adjoint")
+    dt = 1.0
+    md.forces()
+    _, F0 = ctx.results()
+    from turbogap_amd import lib as L
+    L.check(L.load().tgx_velocity_verlet(ctx.h, dt, 1))
+    x1, v1 = md.state()
+    np.testing.assert_allclose(x1, sysm.positions + v0 * dt + 0.5 * F0 / mass[:, None] * dt * dt, atol=1e-12)
+    np.testing.assert_allclose(v1, v0, atol=0)
+    md.forces()
+    _, F1 = ctx.results()
+    L.check(L.load().tgx_velocity_verlet(ctx.h, dt, 0))
+    x2, v2 = md.state()
+    v1b = v0 + 0.5 * (F0 + F1) / mass[:, None] * dt
+    np.testing.assert_allclose(v2, v1b, atol=1e-13)
+    ek, T = md.kinetic()
+    ek_ref = 0.5 * np.sum(mass[:, None] * v1b ** 2)
+    assert abs(ek - ek_ref) < 1e-12 * ek_ref
+    assert abs(T - 2.0 / 3.0 / (sysm.n - 1) / KB * ek_ref) < 1e-9 * T
+    L.check(L.load().tgx_berendsen(ctx.h, 300.0, T, 50.0, dt))
+    _, v3 = md.state()
+    np.testing.assert_allclose(v3, v1b * np.sqrt(1 + dt / 50.0 * (300.0 / T - 1)), rtol=1e-13)

@@ -1,0 +1,108 @@
+"""CPU pins of the oracle's GAP prediction (src/gap.f90:66-259) and the block
+marshal/fold (src/gap_interface.f90:180-431).
+
+  * energies against numpy's closed form delta^2 sum_s alpha_s (q . Q_s)^zeta
+    (independent summation by BLAS), zeta = 1 linearity, alpha = e_s picking
+    one kernel column;
+  * the scatter against a numpy einsum of Qss . dq/dr per pair;
+  * forces against central finite differences of the total energy
+    (h = 1e-4 Å, src/gap.f90 forces are -dE/dr), the virial against a strain
+    finite difference W = -dE/deps, Newton's third law sum_j F_j = 0 and
+    energy invariance under a rigid translation;
+  * the marshal: a block's centres are exactly the atoms of its central
+    species, and the per-block results add up to the all-block totals.
+"""
+import numpy as np
+
+from tests import helpers as H
+from turbogap_amd import structures as S
+
+
+def _block(orc, h, sysm):
+    nl = orc.build_neighbors_list(sysm.positions, sysm.species, sysm.box, h.rcut_max)
+    spi = orc.species_index(nl["neighbor_species"], h.species_types)
+    q, der = orc.get_soap(h, nl["n_neigh"], spi, nl["rjs"], nl["xyz"])
+    return nl, q, der
+
+
+def test_energy_closed_form_and_special_cases(orc):
+    sysm, h, Qs, al = H.config0(300)
+    nl, q, der = _block(orc, h, sysm)
+    for zeta in (1.0, 2.0, 4.0, 2.5):
+        e, _, _ = orc.get_soap_energy_and_forces(q, der, Qs, al, 1.3, zeta, 0.1, nl["n_neigh"], nl["neighbors_list"],
+                                                 nl["xyz"], sysm.n, do_forces=False)
+        K = q @ Qs.T
+        np.testing.assert_allclose(e, 1.3 ** 2 * (K ** zeta) @ al + 0.1, rtol=1e-12, atol=1e-14)
+    a1, a2 = np.random.default_rng(0).normal(size=(2, len(al)))
+    e1, _, _ = orc.get_soap_energy_and_forces(q, der, Qs, a1, 1.0, 1.0, 0.0, nl["n_neigh"], nl["neighbors_list"],
+                                              nl["xyz"], sysm.n, do_forces=False)
+    e2, _, _ = orc.get_soap_energy_and_forces(q, der, Qs, a2, 1.0, 1.0, 0.0, nl["n_neigh"], nl["neighbors_list"],
+                                              nl["xyz"], sysm.n, do_forces=False)
+    e12, _, _ = orc.get_soap_energy_and_forces(q, der, Qs, a1 + a2, 1.0, 1.0, 0.0, nl["n_neigh"],
+                                               nl["neighbors_list"], nl["xyz"], sysm.n, do_forces=False)
+    np.testing.assert_allclose(e12, e1 + e2, atol=1e-13)
+    onehot = np.zeros(len(al))
+    onehot[7] = 1.0
+    e7, _, _ = orc.get_soap_energy_and_forces(q, der, Qs, onehot, 1.0, 4.0, 0.0, nl["n_neigh"], nl["neighbors_list"],
+                                              nl["xyz"], sysm.n, do_forces=False)
+    np.testing.assert_allclose(e7, (q @ Qs[7]) ** 4, rtol=1e-13)
+
+
+def test_scatter_matches_einsum(orc):
+    sysm, h, Qs, al = H.config0(300)
+    nl, q, der = _block(orc, h, sysm)
+    e, F, W = orc.get_soap_energy_and_forces(q, der, Qs, al, 1.0, 4.0, 0.0, nl["n_neigh"], nl["neighbors_list"],
+                                             nl["xyz"], sysm.n)
+    K = q @ Qs.T
+    Qss = -4.0 * ((K ** 3) * al) @ Qs
+    site = np.repeat(np.arange(sysm.n), nl["n_neigh"])
+    fpair = np.einsum("pd,pdk->pk", Qss[site], der)
+    Fref = np.zeros((sysm.n, 3))
+    np.add.at(Fref, nl["neighbors_list"] - 1, fpair)
+    Wref = 0.5 * (np.einsum("pi,pj->ij", fpair, nl["xyz"]) + np.einsum("pi,pj->ij", nl["xyz"], fpair))
+    np.testing.assert_allclose(F, Fref, atol=1e-14)
+    np.testing.assert_allclose(W, Wref, atol=1e-13)
+
+
+def test_forces_virial_finite_difference(orc):
+    sysm, h, Qs, al = H.config0(300)
+    models = [(h, Qs, al)]
+    E, en, F, W, _ = orc.compute(sysm, models)
+    np.testing.assert_allclose(F.sum(0), 0.0, atol=1e-13)
+    hh = 1e-4
+    for i in (3, 100, 200):
+        for k in range(3):
+            p = sysm.positions.copy()
+            p[i, k] += hh
+            ep = orc.compute(S.System(p, sysm.species, sysm.box), models)[0]
+            p[i, k] -= 2 * hh
+            em = orc.compute(S.System(p, sysm.species, sysm.box), models)[0]
+            assert abs(-(ep - em) / (2 * hh) - F[i, k]) < 1e-7
+    eps = 1e-5
+    for k in range(3):
+        vals = []
+        for sgn in (1, -1):
+            Sm = np.eye(3)
+            Sm[k, k] += sgn * eps
+            vals.append(orc.compute(S.System(sysm.positions @ Sm, sysm.species, sysm.box @ Sm), models)[0])
+        assert abs(-(vals[0] - vals[1]) / (2 * eps) - W[k, k]) < 1e-6
+    shifted = S.System(np.mod(sysm.positions + np.array([0.37, -1.1, 2.9]), 12.91), sysm.species, sysm.box)
+    assert abs(orc.compute(shifted, models)[0] - E) < 1e-11
+
+
+def test_marshal_blocks(orc):
+    sysm, models = H.subset(200, 4096)
+    nl = orc.build_neighbors_list(sysm.positions, sysm.species, sysm.box, 3.7)
+    tot_e = np.zeros(sysm.n)
+    tot_f = np.zeros((sysm.n, 3))
+    for k, (h, Qs, al) in enumerate(models):
+        e0, f0, v, soap, P = orc.gap_soap(h, nl, sysm.species, Qs, al, sysm.n)
+        centres = np.nonzero(sysm.species == h.species_types[h.central_species - 1])[0]
+        assert soap.shape[0] == len(centres)
+        assert np.all(e0[np.setdiff1d(np.arange(sysm.n), centres)] == 0.0)
+        assert P == int(nl["n_neigh"][centres].sum())
+        tot_e += e0
+        tot_f += f0
+    E, en, F, W, _ = orc.compute(sysm, models, nl)
+    np.testing.assert_allclose(tot_e, en, atol=1e-15)
+    np.testing.assert_allclose(tot_f, F, atol=1e-15)
