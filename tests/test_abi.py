@@ -60,3 +60,11 @@ def test_config_shapes():
     h0 = S.soap_config0()
     assert h0.dim == 324 and h0.rcut_soft == [3.2]
     assert S.soap_config1()[0].dim == 1224
+
+
+def test_fortran_shim_binds_declared_symbols():
+    """Every bind(C) name in the Fortran shim is a libtgx export the header declares."""
+    src = open(os.path.join(ROOT, "turbogap_amd", "fortran", "tgx_turbogap.f90")).read()
+    bound = set(re.findall(r'bind\(C,\s*name="(tgx_\w+)"\)', src))
+    assert {"tgx_init", "tgx_build_neighbors_list", "tgx_get_gap_soap", "tgx_soap_backend_begin"} <= bound
+    assert bound <= set(declared()), bound - set(declared())

@@ -1,0 +1,298 @@
+! tgx_turbogap: TurboGAP's SOAP/GAP call surface over the libtgx C ABI.
+!
+! A Fortran host (TurboGAP's driver) keeps calling the routines by their
+! reference names; each forwards to libtgx (include/tgx.h) through
+! iso_c_binding. Device memory, streams and kernels stay behind the C ABI: the
+! Fortran side never holds a device pointer.
+!
+!   gpu_context_init / gpu_context_finalize   src/gpu_context.f90:41-58
+!   build_neighbors_list                      src/neighbors.f90:243-605
+!   soap_model_from_files                     src/read_files.f90:660-724 (alphas / desc_sparse)
+!   soap_backend_begin / soap_backend_end     src/gap.f90:49-64
+!   get_gap_soap                              src/gap_interface.f90:41-480 (results on the host)
+!   get_soap_energy_and_forces                src/gap.f90:66-259
+!
+! Errors: a non-zero status prints tgx_last_error() and stops with code 1, the
+! behaviour of turbogap_abort (src/error.f90:48-61).
+module tgx_turbogap
+   use iso_c_binding
+   implicit none
+   private
+
+   integer, parameter, public :: dp = c_double
+   integer(c_int), parameter, public :: TGX_PATH_FP64_PLAIN = 0, TGX_PATH_FP64_ADJOINT = 1, &
+                                        TGX_PATH_MIXED = 2, TGX_PATH_SURROGATE = 3
+
+   type, bind(C), public :: tgx_soap_hypers
+      integer(c_int) :: n_species, l_max, basis, radial_enhancement, n_quad_soft, n_quad_buffer
+      type(c_ptr) :: alpha_max, rcut_hard, rcut_soft, nf, global_scaling, atom_sigma_r, &
+                     atom_sigma_r_scaling, atom_sigma_t, atom_sigma_t_scaling, amplitude_scaling, central_weight
+   end type
+
+   type(c_ptr), save :: ctx = c_null_ptr
+
+   public :: gpu_context_init, gpu_context_finalize, build_neighbors_list, soap_model_from_files, &
+             soap_backend_begin, soap_backend_end, get_gap_soap, get_soap_energy_and_forces, tgx_check
+
+   interface
+      integer(c_int) function c_tgx_init(device, ctx_out) bind(C, name="tgx_init")
+         import :: c_int, c_ptr
+         integer(c_int), value :: device
+         type(c_ptr) :: ctx_out
+      end function
+      integer(c_int) function c_tgx_finalize(c) bind(C, name="tgx_finalize")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: c
+      end function
+      type(c_ptr) function c_tgx_last_error() bind(C, name="tgx_last_error")
+         import :: c_ptr
+      end function
+      integer(c_int) function c_tgx_set_system(c, n, nsc, pos, spec, box) bind(C, name="tgx_set_system")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c
+         integer(c_int), value :: n, nsc
+         real(c_double) :: pos(*), box(*)
+         integer(c_int) :: spec(*)
+      end function
+      integer(c_int) function c_tgx_build(c, rcut, indices, do_list, npairs) bind(C, name="tgx_build_neighbors_list")
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value :: c
+         real(c_double), value :: rcut
+         integer(c_int) :: indices(*), do_list(*)
+         integer(c_int64_t) :: npairs
+      end function
+      integer(c_int) function c_tgx_get_nl(c, nn, nl, rjs, th, ph, xyz, ns) bind(C, name="tgx_get_neighbors_list")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c
+         integer(c_int) :: nn(*), nl(*), ns(*)
+         real(c_double) :: rjs(*), th(*), ph(*), xyz(*)
+      end function
+      integer(c_int) function c_tgx_model(c, h, st, cs, delta, zeta, e0, ns, dim, qs, al, mid) &
+         bind(C, name="tgx_soap_model_create")
+         import :: c_int, c_ptr, c_double, tgx_soap_hypers
+         type(c_ptr), value :: c
+         type(tgx_soap_hypers) :: h
+         integer(c_int) :: st(*)
+         integer(c_int), value :: cs, ns, dim
+         real(c_double), value :: delta, zeta, e0
+         real(c_double) :: qs(*), al(*)
+         integer(c_int) :: mid
+      end function
+      integer(c_int) function c_tgx_begin(c, mid) bind(C, name="tgx_soap_backend_begin")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: c
+         integer(c_int), value :: mid
+      end function
+      integer(c_int) function c_tgx_end(c) bind(C, name="tgx_soap_backend_end")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: c
+      end function
+      integer(c_int) function c_tgx_gap_soap(c, path, do_forces, en, f, vir, nsb, npb) bind(C, name="tgx_get_gap_soap")
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value :: c
+         integer(c_int), value :: path, do_forces
+         real(c_double) :: en(*), f(*), vir(*)
+         integer(c_int) :: nsb
+         integer(c_int64_t) :: npb
+      end function
+      integer(c_int) function c_tgx_eaf(c, n, nsoap, np, soap, sder, e0, nn, nl, xyz, n0, dof, en, f, vir) &
+         bind(C, name="tgx_get_soap_energy_and_forces")
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value :: c
+         integer(c_int), value :: n, nsoap, n0, dof
+         integer(c_int64_t), value :: np
+         real(c_double), value :: e0
+         real(c_double) :: soap(*), sder(*), xyz(*), en(*), f(*), vir(*)
+         integer(c_int) :: nn(*), nl(*)
+      end function
+      integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
+         import :: c_size_t, c_ptr
+         type(c_ptr), value :: s
+      end function
+   end interface
+
+contains
+
+   subroutine tgx_check(status, what)
+      integer(c_int), intent(in) :: status
+      character(len=*), intent(in) :: what
+      character(kind=c_char), pointer :: msg(:)
+      integer :: n
+      if (status == 0) return
+      n = int(c_strlen(c_tgx_last_error()))
+      call c_f_pointer(c_tgx_last_error(), msg, [n])
+      write (*, '(A,A,A,I0,A,*(A))') 'ERROR in ', what, ' (status ', status, '): ', msg(1:n)
+      error stop 1
+   end subroutine
+
+   subroutine gpu_context_init(device)
+      integer, intent(in) :: device
+      call tgx_check(c_tgx_init(int(device, c_int), ctx), "gpu_context_init")
+   end subroutine
+
+   subroutine gpu_context_finalize()
+      if (c_associated(ctx)) call tgx_check(c_tgx_finalize(ctx), "gpu_context_finalize")
+      ctx = c_null_ptr
+   end subroutine
+
+   ! build_neighbors_list with the reference's 20 arguments in its order
+   ! (src/neighbors.f90:243-246) and the same outputs: callee-allocated arrays,
+   ! 1-based supercell indices in neighbors_list, xyz(3, n_atom_pairs) = r_j - r_i.
+   ! The device always builds the list: with rebuild_neighbors_list = .false.
+   ! the pairs inside the cutoff are the same ones, in the device's order.
+   ! do_timing prints the build time; rank only selects who prints.
+   subroutine build_neighbors_list(positions, a_box, b_box, c_box, do_timing, species_supercell, rcut_max, &
+                                   n_atom_pairs, rjs, thetas, phis, xyz, n_neigh, neighbors_list, neighbor_species, &
+                                   n_sites, indices, rebuild_neighbors_list, do_list, rank)
+      real(dp), intent(in) :: positions(:, :), rcut_max
+      real(dp), intent(inout) :: a_box(1:3), b_box(1:3), c_box(1:3)
+      logical, intent(in) :: do_timing, rebuild_neighbors_list, do_list(:)
+      integer, intent(in) :: species_supercell(:), n_sites, indices(1:3), rank
+      integer, intent(out) :: n_atom_pairs
+      real(dp), allocatable, intent(inout) :: rjs(:), thetas(:), phis(:), xyz(:, :)
+      integer, allocatable, intent(inout) :: n_neigh(:), neighbors_list(:), neighbor_species(:)
+      real(dp) :: box(9)
+      integer(c_int) :: ind(3)
+      integer(c_int), allocatable :: spec(:), dl(:)
+      integer(c_int64_t) :: np
+      integer(8) :: t0, t1, rate
+      if (do_timing) call system_clock(t0, rate)
+      box(1:3) = a_box; box(4:6) = b_box; box(7:9) = c_box
+      ind = int(indices, c_int)
+      allocate (spec(size(species_supercell)), dl(n_sites))
+      spec = int(species_supercell, c_int)
+      dl = merge(1_c_int, 0_c_int, do_list(1:n_sites))
+      call tgx_check(c_tgx_set_system(ctx, int(n_sites, c_int), int(size(positions, 2), c_int), positions, spec, box), &
+                     "build_neighbors_list")
+      call tgx_check(c_tgx_build(ctx, rcut_max, ind, dl, np), "build_neighbors_list")
+      n_atom_pairs = int(np)
+      if (allocated(rjs)) deallocate (rjs)
+      if (allocated(thetas)) deallocate (thetas)
+      if (allocated(phis)) deallocate (phis)
+      if (allocated(xyz)) deallocate (xyz)
+      if (allocated(n_neigh)) deallocate (n_neigh)
+      if (allocated(neighbors_list)) deallocate (neighbors_list)
+      if (allocated(neighbor_species)) deallocate (neighbor_species)
+      allocate (rjs(n_atom_pairs), thetas(n_atom_pairs), phis(n_atom_pairs), xyz(3, n_atom_pairs))
+      allocate (n_neigh(n_sites), neighbors_list(n_atom_pairs), neighbor_species(n_atom_pairs))
+      call tgx_check(c_tgx_get_nl(ctx, n_neigh, neighbors_list, rjs, thetas, phis, xyz, neighbor_species), &
+                     "build_neighbors_list")
+      if (do_timing .and. rank == 0) then
+         call system_clock(t1)
+         write (*, '(A, F9.3, A)') '  *) Neighbors build: ', real(t1 - t0, dp)/real(rate, dp), ' seconds |'
+      end if
+   end subroutine
+
+   ! One soap_turbo block from the reference's sparse-set files
+   ! (src/read_files.f90:660-724: one value per line, alphas then desc_sparse
+   ! with the descriptor component fastest). Returns the model id.
+   subroutine soap_model_from_files(file_alphas, file_desc, species_types, alpha_max, l_max, rcut_hard, rcut_soft, &
+                                    atom_sigma_r, atom_sigma_t, central_weight, basis, central_species, delta, zeta, &
+                                    model_id)
+      character(len=*), intent(in) :: file_alphas, file_desc, basis
+      integer, intent(in) :: species_types(:), alpha_max(:), l_max, central_species
+      real(dp), intent(in) :: rcut_hard(:), rcut_soft(:), atom_sigma_r(:), atom_sigma_t(:), central_weight(:)
+      real(dp), intent(in) :: delta, zeta
+      integer, intent(out) :: model_id
+      real(dp), allocatable :: alphas(:), qs(:, :)
+      real(dp), allocatable, target :: r_h(:), r_s(:), nf(:), gs(:), sr(:), srs(:), stt(:), sts(:), amp(:), cw(:)
+      integer(c_int), allocatable, target :: am(:)
+      integer(c_int), allocatable :: st(:)
+      type(tgx_soap_hypers) :: h
+      integer :: n_sparse, dim, i, j, ios, u, ns
+      integer(c_int) :: mid
+      real(dp) :: junk
+      open (newunit=u, file=file_alphas, status="old")
+      n_sparse = 0
+      do
+         read (u, *, iostat=ios) junk
+         if (ios /= 0) exit
+         n_sparse = n_sparse + 1
+      end do
+      close (u)
+      open (newunit=u, file=file_desc, status="old")
+      i = 0
+      do
+         read (u, *, iostat=ios) junk
+         if (ios /= 0) exit
+         i = i + 1
+      end do
+      close (u)
+      dim = i/n_sparse
+      allocate (alphas(n_sparse), qs(dim, n_sparse))
+      open (newunit=u, file=file_alphas, status="old")
+      do i = 1, n_sparse
+         read (u, *) alphas(i)
+      end do
+      close (u)
+      open (newunit=u, file=file_desc, status="old")
+      do i = 1, n_sparse
+         do j = 1, dim
+            read (u, *) qs(j, i)
+         end do
+      end do
+      close (u)
+      ns = size(species_types)
+      allocate (am(ns), st(ns), r_h(ns), r_s(ns), nf(ns), gs(ns), sr(ns), srs(ns), stt(ns), sts(ns), amp(ns), cw(ns))
+      am = int(alpha_max, c_int); st = int(species_types, c_int)
+      r_h = rcut_hard; r_s = rcut_soft; nf = 4.0_dp; gs = 1.0_dp; sr = atom_sigma_r; srs = 0.0_dp
+      stt = atom_sigma_t; sts = 0.0_dp; amp = 1.0_dp; cw = central_weight
+      h%n_species = ns; h%l_max = l_max; h%radial_enhancement = 0; h%n_quad_soft = 32; h%n_quad_buffer = 16
+      h%basis = merge(1, 0, trim(basis) == "poly3gauss")
+      h%alpha_max = c_loc(am); h%rcut_hard = c_loc(r_h); h%rcut_soft = c_loc(r_s); h%nf = c_loc(nf)
+      h%global_scaling = c_loc(gs); h%atom_sigma_r = c_loc(sr); h%atom_sigma_r_scaling = c_loc(srs)
+      h%atom_sigma_t = c_loc(stt); h%atom_sigma_t_scaling = c_loc(sts); h%amplitude_scaling = c_loc(amp)
+      h%central_weight = c_loc(cw)
+      call tgx_check(c_tgx_model(ctx, h, st, int(central_species, c_int), delta, zeta, 0.0_dp, int(n_sparse, c_int), &
+                                 int(dim, c_int), qs, alphas, mid), "soap_model_from_files")
+      model_id = int(mid)
+   end subroutine
+
+   subroutine soap_backend_begin(model_id)
+      integer, intent(in) :: model_id
+      call tgx_check(c_tgx_begin(ctx, int(model_id, c_int)), "soap_backend_begin")
+   end subroutine
+
+   subroutine soap_backend_end()
+      call tgx_check(c_tgx_end(ctx), "soap_backend_end")
+   end subroutine
+
+   ! get_gap_soap for the current block on the list last built: energies0 and
+   ! forces0(3, n_sites) are ASSIGNED for the atoms the block touches
+   ! (src/gap_interface.f90:422-431); virial(3,3) is written.
+   subroutine get_gap_soap(path, do_forces, energies0, forces0, virial)
+      integer, intent(in) :: path
+      logical, intent(in) :: do_forces
+      real(dp), intent(inout) :: energies0(:), forces0(:, :)
+      real(dp), intent(out) :: virial(3, 3)
+      integer(c_int) :: nsb
+      integer(c_int64_t) :: npb
+      call tgx_check(c_tgx_gap_soap(ctx, int(path, c_int), merge(1_c_int, 0_c_int, do_forces), energies0, forces0, &
+                                    virial, nsb, npb), "get_gap_soap")
+   end subroutine
+
+   ! get_soap_energy_and_forces with the reference's arguments in its order
+   ! (src/gap.f90:66-68) on host arrays, for the current block. delta and zeta0
+   ! belong to the block (given when it was created) and must equal its values.
+   subroutine get_soap_energy_and_forces(soap, soap_der, delta, zeta0, e0, n_neigh, neighbors_list, xyz, do_forces, &
+                                         do_timing, energies, forces, virial)
+      real(dp), intent(in) :: soap(:, :), soap_der(:, :, :), delta, zeta0, e0, xyz(:, :)
+      integer, intent(in) :: n_neigh(:), neighbors_list(:)
+      logical, intent(in) :: do_forces, do_timing
+      real(dp), intent(out) :: energies(:), forces(:, :), virial(3, 3)
+      integer(c_int), allocatable :: nn(:), nl(:)
+      integer(8) :: t0, t1, rate
+      if (do_timing) call system_clock(t0, rate)
+      allocate (nn(size(n_neigh)), nl(size(neighbors_list)))
+      nn = int(n_neigh, c_int); nl = int(neighbors_list, c_int)
+      call tgx_check(c_tgx_eaf(ctx, int(size(soap, 2), c_int), int(size(soap, 1), c_int), &
+                               int(size(soap_der, 3), c_int64_t), soap, soap_der, e0, nn, nl, xyz, &
+                               int(size(forces, 2), c_int), merge(1_c_int, 0_c_int, do_forces), energies, forces, &
+                               virial), "get_soap_energy_and_forces")
+      if (do_timing) then
+         call system_clock(t1)
+         write (*, '(A, F9.3, A)') '  *) GAP prediction: ', real(t1 - t0, dp)/real(rate, dp), ' seconds |'
+      end if
+   end subroutine
+
+end module tgx_turbogap
