@@ -134,3 +134,58 @@ def test_velocity_verlet_and_berendsen(ctx):
     L.check(L.load().tgx_berendsen(ctx.h, 300.0, T, 50.0, dt))
     _, v3 = md.state()
     np.testing.assert_allclose(v3, v1b * np.sqrt(1 + dt / 50.0 * (300.0 / T - 1)), rtol=1e-13)
+
+
+def _config0_block(ctx):
+    sysm, h, Qs, al = H.config0(200)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    ctx.build_neighbors_list(h.rcut_max, fetch=False)
+    return sysm, h, ctx.soap_model(h, Qs, al)
+
+
+def test_surrogate_energies_match_mlp_oracle(ctx):
+    """One installed network: the fused inference (SiLU epilogue on the first
+    GEMM) gives e_i = shift + scale E(q_i) of the oracle network on the FP64
+    descriptors (BF16 operands: 2e-2 of the output range)."""
+    sysm, h, mid = _config0_block(ctx)
+    net = SG.MLP(ctx, h.dim, 128, 256, seed=21)
+    net.set_normalization(-0.5, 0.03)
+    net.install(mid, 0)
+    SG.set_gate(ctx, 1, 1e9)
+    ctx.compute("surrogate")
+    en, F = ctx.results()
+    ctx.soap_backend_begin(mid)
+    q, _ = ctx.get_soap(derivatives=False)
+    ctx.soap_backend_end()
+    f = MO.forward(net.get_weights(), q, emulate_bf16=True)
+    e_ref = -0.5 + 0.03 * f["E"]
+    assert np.isfinite(F).all()
+    assert np.abs(en - e_ref).max() <= 2e-2 * np.abs(e_ref + 0.5).max()
+
+
+def test_surrogate_ensemble_is_member_mean(ctx):
+    """Two members with different normalisations: energies are the mean of the
+    single-member runs, the spread their difference, and forces (linear in
+    dE/dq) the mean of the single-member forces."""
+    sysm, h, mid = _config0_block(ctx)
+    a = SG.MLP(ctx, h.dim, 128, 256, seed=21)
+    b = SG.MLP(ctx, h.dim, 128, 256, seed=22)
+    a.set_normalization(-0.5, 0.03)
+    b.set_normalization(-0.4, 0.05)
+    SG.set_gate(ctx, 1, 1e9)
+    out = []
+    for net in (a, b):
+        net.install(mid, 0)
+        ctx.compute("surrogate")
+        out.append(ctx.results())
+    a.install(mid, 0)
+    b.install(mid, 1)
+    SG.set_gate(ctx, 2, 1e9)
+    ctx.compute("surrogate")
+    e2, F2 = ctx.results()
+    frac, spread = SG.gate_stats(ctx)
+    (ea, Fa), (eb, Fb) = out
+    np.testing.assert_allclose(e2, 0.5 * (ea + eb), rtol=0, atol=1e-12)
+    assert frac == 0.0
+    np.testing.assert_allclose(spread, np.abs(ea - eb).max(), rtol=1e-5)
+    assert np.abs(F2 - 0.5 * (Fa + Fb)).max() <= 1e-4 * np.abs(F2).max()

@@ -152,3 +152,21 @@ def test_compute_deterministic(ctx):
     ctx.compute("fp64_adjoint")
     e2, f2 = ctx.results()
     assert np.array_equal(e1, e2) and np.array_equal(f1, f2)
+
+
+@pytest.mark.parametrize("path", ["fp64_plain", "fp64_adjoint"])
+def test_perfect_crystal(ctx, orc, path):
+    """Perfect diamond (site symmetry Td): oracle parity and the invariants equal energies, zero forces, isotropic virial."""
+    from tests.test_oracle_gap import diamond
+    _, h, Qs, al = H.config0(300)
+    sysm = diamond()
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    ctx.build_neighbors_list(h.rcut_max, fetch=False)
+    ctx.soap_model(h, Qs, al)
+    E, W = ctx.compute(path)
+    en, F = ctx.results()
+    Eo, eno, Fo, Wo, _ = orc.compute(sysm, [(h, Qs, al)])
+    np.testing.assert_allclose(en, eno, rtol=1e-8, atol=1e-12)
+    assert np.abs(F).max() <= 1e-9
+    assert np.abs(W - Wo).max() <= 1e-6
+    assert np.abs(W - np.diag(np.diag(W))).max() <= 1e-9 * np.abs(W).max()

@@ -106,3 +106,27 @@ def test_marshal_blocks(orc):
     E, en, F, W, _ = orc.compute(sysm, models, nl)
     np.testing.assert_allclose(tot_e, en, atol=1e-15)
     np.testing.assert_allclose(tot_f, F, atol=1e-15)
+
+
+def diamond(n_cells=3, a=3.567):
+    """Perfect diamond carbon, n_cells^3 conventional cells (8 atoms each)."""
+    fcc = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0]])
+    basis = np.concatenate([fcc, fcc + 0.25])
+    cells = np.array([[i, j, k] for i in range(n_cells) for j in range(n_cells) for k in range(n_cells)], float)
+    pos = ((cells[:, None, :] + basis[None, :, :]).reshape(-1, 3)) * a
+    n = len(pos)
+    return S.System(pos, np.full(n, S.CARBON, np.int32), S.cubic_box(n_cells * a))
+
+
+def test_perfect_crystal_invariants(orc):
+    """Every atom of perfect diamond is equivalent with site symmetry Td: equal
+    local energies, zero forces, and a cubic (isotropic) virial."""
+    _, h, Qs, al = H.config0(300)
+    sysm = diamond()
+    E, en, F, W, _ = orc.compute(sysm, [(h, Qs, al)])
+    assert np.ptp(en) <= 1e-10 * max(1.0, np.abs(en).max())
+    assert np.abs(F).max() <= 1e-10
+    off = W - np.diag(np.diag(W))
+    assert np.abs(off).max() <= 1e-10 * max(1.0, np.abs(W).max())
+    assert np.ptp(np.diag(W)) <= 1e-10 * max(1.0, np.abs(W).max())
+    assert abs(W[0, 0]) > 1e-6  # the crystal is not stress free at this lattice constant

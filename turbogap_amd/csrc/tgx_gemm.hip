@@ -22,18 +22,23 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int BM = 128, BN = 128;
 
-enum { EPI_STORE = 0, EPI_GAP = 1 };
+enum { EPI_STORE = TGX_EPI_STORE_F32, EPI_GAP = 1, EPI_SILU = TGX_EPI_SILU, EPI_SCALE_AUX = TGX_EPI_SCALE_AUX };
 
 struct EpiArgs {
-  double pref;           // -zeta delta^2 (EPI_GAP) / output scale (EPI_STORE)
+  double pref;           // -zeta delta^2 (EPI_GAP) / output scale (EPI_STORE, EPI_SCALE_AUX)
   double zeta;
   int zeta_int;          // >= 0 when zeta is an integer
   const double* alpha;   // [N] (FP64 path)
   const float* alpha_f;  // [N] (BF16 path)
-  void* out;             // Kd or Qss
+  void* out;             // Kd, Qss or the MLP output
   int ldo;
   double* epart;         // [M][n_tiles] (EPI_GAP)
   int n_tiles;
+  const float* bias;     // [N] (EPI_SILU)
+  float* aux;            // silu'(v) written by EPI_SILU, read by EPI_SCALE_AUX
+  int ldaux;
+  int64_t sA, sB, sOut, sAux, sBias;  // per-blockIdx.z element strides
+  const float* zscale;   // per-z scale (EPI_SCALE_AUX), or null for pref
 };
 
 __device__ __forceinline__ void tile_of(int& tm, int& tn) {
@@ -78,6 +83,27 @@ __device__ void epilogue(const VecT (&acc)[4][4], int m0, int n0, int tn, int wm
   const int rbase = m0 + wm * 64;
   const int cbase = n0 + wn * 64 + (lane & 15);
   OutT* out = reinterpret_cast<OutT*>(e.out);
+  if (EPI == EPI_SILU || EPI == EPI_SCALE_AUX) {
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj) {
+      const int col = cbase + tj * 16;
+      const float bc = EPI == EPI_SILU ? e.bias[col] : 0.0f;
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const size_t row = (size_t)(rbase + ti * 16 + tile_row<F64>(lane, r));
+          if (EPI == EPI_SILU) {
+            const float v = (float)acc[ti][tj][r] + bc;
+            out[row * e.ldo + col] = (OutT)tgx_silu(v);
+            e.aux[row * e.ldaux + col] = tgx_silu1(v);
+          } else {
+            out[row * e.ldo + col] = (OutT)((float)acc[ti][tj][r] * e.aux[row * e.ldaux + col] * (float)e.pref);
+          }
+        }
+    }
+    return;
+  }
   if (EPI == EPI_STORE) {
 #pragma unroll
     for (int ti = 0; ti < 4; ++ti)
@@ -178,54 +204,93 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int K, const double* __restric
 }
 
 // BF16 NT GEMM with FP32 accumulation. M, N multiples of 128; K multiple of 64.
+// Two LDS stages: the global loads of stage k+1 are in flight while the MFMAs
+// of stage k run, and one barrier per stage separates the LDS write of k+1
+// from the reads of k. Each thread moves four 16-byte chunks per operand and
+// stage: chunk t%8 of rows t/8 + 32 j, so a wavefront reads eight full 128-byte
+// row segments per load. The next-stage offset is clamped to the last stage
+// (a redundant, in-bounds reload) so the staging registers are written on
+// every iteration and never demoted to private memory.
 template <int EPI, class OutT, bool ZINT>
 __global__ __launch_bounds__(256) void k_gemm_bf16(int K, const __bf16* __restrict__ A, int lda,
                                                    const __bf16* __restrict__ B, int ldb, EpiArgs e) {
-  constexpr int BK = 64, LDS_STRIDE = BK + 8;
-  __shared__ __align__(16) __bf16 As[BM * LDS_STRIDE];
-  __shared__ __align__(16) __bf16 Bs[BN * LDS_STRIDE];
+  constexpr int BK = 64, LDS_STRIDE = BK + 8;  // 144-byte row pitch: conflict-free 16-byte reads
+  __shared__ __align__(16) __bf16 As[2][BM * LDS_STRIDE];
+  __shared__ __align__(16) __bf16 Bs[2][BN * LDS_STRIDE];
   __shared__ double sE[2 * BM];
   int tm, tn;
   tile_of(tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
-  const int lrow = tid >> 1, lk = (tid & 1) * 32;
-  const uint4* Ag = reinterpret_cast<const uint4*>(A + (size_t)(m0 + lrow) * lda + lk);
-  const uint4* Bg = reinterpret_cast<const uint4*>(B + (size_t)(n0 + lrow) * ldb + lk);
+  const int lr = tid >> 3, lc = (tid & 7) * 8;
+  {
+    const int z = blockIdx.z;  // batch member: offset every operand
+    A += z * e.sA;
+    B += z * e.sB;
+    e.out = reinterpret_cast<OutT*>(e.out) + z * e.sOut;
+    if (e.aux) e.aux += z * e.sAux;
+    if (e.bias) e.bias += z * e.sBias;
+    if (e.zscale) e.pref = e.zscale[z];
+  }
+  const __bf16* Ag = A + (size_t)(m0 + lr) * lda + lc;
+  const __bf16* Bg = B + (size_t)(n0 + lr) * ldb + lc;
+  const size_t a32 = (size_t)32 * lda, b32 = (size_t)32 * ldb;
   f32x4 acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = (f32x4){0, 0, 0, 0};
-  uint4 ra[4], rb[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) { ra[j] = Ag[j]; rb[j] = Bg[j]; }
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    __syncthreads();
-    uint4* as = reinterpret_cast<uint4*>(As + lrow * LDS_STRIDE + lk);
-    uint4* bs = reinterpret_cast<uint4*>(Bs + lrow * LDS_STRIDE + lk);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) { as[j] = ra[j]; bs[j] = rb[j]; }
-    __syncthreads();
-    if (k0 + BK < K) {
-      const int adv = (k0 + BK) / 8;  // uint4 holds 8 bf16
-#pragma unroll
-      for (int j = 0; j < 4; ++j) { ra[j] = Ag[adv + j]; rb[j] = Bg[adv + j]; }
-    }
+  uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
+  auto gload = [&](int k) {
+    ra0 = *reinterpret_cast<const uint4*>(Ag + k);
+    ra1 = *reinterpret_cast<const uint4*>(Ag + a32 + k);
+    ra2 = *reinterpret_cast<const uint4*>(Ag + 2 * a32 + k);
+    ra3 = *reinterpret_cast<const uint4*>(Ag + 3 * a32 + k);
+    rb0 = *reinterpret_cast<const uint4*>(Bg + k);
+    rb1 = *reinterpret_cast<const uint4*>(Bg + b32 + k);
+    rb2 = *reinterpret_cast<const uint4*>(Bg + 2 * b32 + k);
+    rb3 = *reinterpret_cast<const uint4*>(Bg + 3 * b32 + k);
+  };
+  auto lstore = [&](int buf) {
+    __bf16* as = &As[buf][lr * LDS_STRIDE + lc];
+    __bf16* bs = &Bs[buf][lr * LDS_STRIDE + lc];
+    *reinterpret_cast<uint4*>(as) = ra0;
+    *reinterpret_cast<uint4*>(as + 32 * LDS_STRIDE) = ra1;
+    *reinterpret_cast<uint4*>(as + 64 * LDS_STRIDE) = ra2;
+    *reinterpret_cast<uint4*>(as + 96 * LDS_STRIDE) = ra3;
+    *reinterpret_cast<uint4*>(bs) = rb0;
+    *reinterpret_cast<uint4*>(bs + 32 * LDS_STRIDE) = rb1;
+    *reinterpret_cast<uint4*>(bs + 64 * LDS_STRIDE) = rb2;
+    *reinterpret_cast<uint4*>(bs + 96 * LDS_STRIDE) = rb3;
+  };
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  const int n_stages = K / BK;
+  for (int st = 0; st < n_stages; ++st) {
+    const int buf = st & 1;
+    const int kn = min(st + 1, n_stages - 1) * BK;
+    gload(kn);
+    __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the MFMAs they overlap
+    const __bf16* as = &As[buf][(wm * 64 + (lane & 15)) * LDS_STRIDE + 8 * (lane >> 4)];
+    const __bf16* bs = &Bs[buf][(wn * 64 + (lane & 15)) * LDS_STRIDE + 8 * (lane >> 4)];
 #pragma unroll
     for (int ks = 0; ks < BK / 32; ++ks) {
       bf16x8 a[4], b[4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        a[t] = *reinterpret_cast<const bf16x8*>(As + (wm * 64 + t * 16 + (lane & 15)) * LDS_STRIDE + ks * 32 + 8 * (lane >> 4));
-        b[t] = *reinterpret_cast<const bf16x8*>(Bs + (wn * 64 + t * 16 + (lane & 15)) * LDS_STRIDE + ks * 32 + 8 * (lane >> 4));
+        a[t] = *reinterpret_cast<const bf16x8*>(as + t * 16 * LDS_STRIDE + ks * 32);
+        b[t] = *reinterpret_cast<const bf16x8*>(bs + t * 16 * LDS_STRIDE + ks * 32);
       }
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
+    __builtin_amdgcn_sched_barrier(0);
+    lstore(buf ^ 1);
+    __syncthreads();
   }
   epilogue<EPI, OutT, float, f32x4, false, ZINT>(acc, m0, n0, tn, wm, wn, e, sE);
 }
@@ -336,6 +401,43 @@ int tgx_gemm_bf16_nt(tgx_ctx* ctx, const char* name, int M, int N, int K, const 
   s.out = C;
   s.ldo = ldc;
   TGX_LAUNCH(ctx, name, (k_gemm_bf16<EPI_STORE, float, true>), dim3(N / BN, M / BM), dim3(256), 0, K, A, lda, B, ldb, s);
+  return TGX_OK;
+}
+
+int tgx_gemm_bf16(tgx_ctx* ctx, const char* name, const tgx_gemm_desc& d) {
+  TGX_REQUIRE(d.M % BM == 0 && d.N % BN == 0 && d.K % 64 == 0 && d.lda % 8 == 0 && d.ldb % 8 == 0 && d.sA % 8 == 0 &&
+                  d.sB % 8 == 0 && d.batch >= 1 && d.batch <= 65535,
+              "gemm_bf16 %s: unpadded shape %d x %d x %d (batch %d)", name, d.M, d.N, d.K, d.batch);
+  TGX_REQUIRE(d.epi != TGX_EPI_SILU || (d.bias && d.aux), "gemm_bf16 %s: SiLU epilogue needs bias and aux", name);
+  TGX_REQUIRE(d.epi != TGX_EPI_SCALE_AUX || d.aux, "gemm_bf16 %s: scaled epilogue needs aux", name);
+  EpiArgs e{};
+  e.pref = d.scale;
+  e.out = d.C;
+  e.ldo = d.ldc;
+  e.bias = d.bias;
+  e.aux = d.aux;
+  e.ldaux = d.ldaux;
+  e.sA = d.sA;
+  e.sB = d.sB;
+  e.sOut = d.sC;
+  e.sAux = d.sAux;
+  e.sBias = d.sBias;
+  e.zscale = d.zscale;
+  const dim3 grid(d.N / BN, d.M / BM, d.batch);
+  switch (d.epi) {
+    case TGX_EPI_STORE_F32:
+      TGX_LAUNCH(ctx, name, (k_gemm_bf16<EPI_STORE, float, true>), grid, dim3(256), 0, d.K, d.A, d.lda, d.B, d.ldb, e);
+      break;
+    case TGX_EPI_SILU:
+      TGX_LAUNCH(ctx, name, (k_gemm_bf16<EPI_SILU, __bf16, true>), grid, dim3(256), 0, d.K, d.A, d.lda, d.B, d.ldb, e);
+      break;
+    case TGX_EPI_SCALE_AUX:
+      TGX_LAUNCH(ctx, name, (k_gemm_bf16<EPI_SCALE_AUX, __bf16, true>), grid, dim3(256), 0, d.K, d.A, d.lda, d.B, d.ldb,
+                 e);
+      break;
+    default:
+      TGX_REQUIRE(false, "gemm_bf16 %s: unknown epilogue %d", name, d.epi);
+  }
   return TGX_OK;
 }
 

@@ -214,3 +214,46 @@ int tgx_device_sum(tgx_ctx* ctx, const double* d_x, int64_t n, double* out_host)
 int tgx_gemm_bf16_nt(tgx_ctx* ctx, const char* name, int M, int N, int K, const __bf16* A, int lda, const __bf16* B,
                      int ldb, float* C, int ldc, double scale);
 int tgx_f32_to_bf16(tgx_ctx* ctx, const float* x, __bf16* y, int64_t n);
+
+// SiLU and its first two derivatives (FP32), shared by the MLP kernels and the
+// fused GEMM epilogues.
+__device__ __forceinline__ float tgx_sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float tgx_silu(float z) { return z * tgx_sigm(z); }
+__device__ __forceinline__ float tgx_silu1(float z) {
+  const float s = tgx_sigm(z);
+  return s * (1.0f + z * (1.0f - s));
+}
+__device__ __forceinline__ float tgx_silu2(float z) {
+  const float s = tgx_sigm(z);
+  return s * (1.0f - s) * (2.0f + z * (1.0f - 2.0f * s));
+}
+
+// Batched BF16 NT GEMM with a fused epilogue: for z < batch,
+//   acc = A_z[M][K] . B_z[N][K]^T (FP32 accumulation), X_z = X + z * sX, and
+//   TGX_EPI_STORE_F32 : C_z (FP32)  = scale * acc
+//   TGX_EPI_SILU      : v = acc + bias_z[col]; C_z (BF16) = silu(v); aux_z (FP32) = silu'(v)
+//   TGX_EPI_SCALE_AUX : C_z (BF16)  = acc * aux_z * (zscale ? zscale[z] : scale)
+// M, N multiples of 128, K of 64, leading dimensions and strides in elements,
+// multiples of 8 for A and B.
+enum { TGX_EPI_STORE_F32 = 0, TGX_EPI_SILU = 2, TGX_EPI_SCALE_AUX = 3 };
+struct tgx_gemm_desc {
+  int epi = TGX_EPI_STORE_F32;
+  int M = 0, N = 0, K = 0, batch = 1;
+  const __bf16* A = nullptr;
+  int lda = 0;
+  int64_t sA = 0;
+  const __bf16* B = nullptr;
+  int ldb = 0;
+  int64_t sB = 0;
+  void* C = nullptr;
+  int ldc = 0;
+  int64_t sC = 0;
+  double scale = 1.0;
+  const float* bias = nullptr;
+  int64_t sBias = 0;
+  float* aux = nullptr;
+  int ldaux = 0;
+  int64_t sAux = 0;
+  const float* zscale = nullptr;
+};
+int tgx_gemm_bf16(tgx_ctx* ctx, const char* name, const tgx_gemm_desc& d);

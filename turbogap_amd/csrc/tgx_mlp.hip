@@ -18,6 +18,7 @@
 // Qss = -dE_hat/dq, the same route as the GAP's.
 #include <cmath>
 #include <algorithm>
+#include <memory>
 #include <numeric>
 
 #include "tgx_internal.h"
@@ -28,16 +29,9 @@ int tgx_soap_descriptors_sites(tgx_ctx* ctx, tgx_model_t* m, const int* d_sites,
 
 namespace {
 
-__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
-__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
-__device__ __forceinline__ float silu1(float z) {
-  const float s = sigm(z);
-  return s * (1.0f + z * (1.0f - s));
-}
-__device__ __forceinline__ float silu2(float z) {
-  const float s = sigm(z);
-  return s * (1.0f - s) * (2.0f + z * (1.0f - 2.0f * s));
-}
+__device__ __forceinline__ float silu(float z) { return tgx_silu(z); }
+__device__ __forceinline__ float silu1(float z) { return tgx_silu1(z); }
+__device__ __forceinline__ float silu2(float z) { return tgx_silu2(z); }
 
 // layer 1: z1 += b1; h1 = silu(z1) (bf16 and fp32); s1 = silu'(z1); s1pp = silu''(z1)
 __global__ void k_l1(int64_t n, int H, float* __restrict__ z1, const float* __restrict__ b1, __bf16* __restrict__ h1b,
@@ -279,26 +273,46 @@ __global__ void k_append(int n_take, int stride, int offset, int Dp, const doubl
   if (threadIdx.x == 0) de[r] = e_site[s];
 }
 
-// Surrogate output: e_site = shift + scale * mean_k E_k, spread = max - min over
-// members, Qss = -(scale/K) sum_k G_k (G accumulated by the GEMMs).
-__global__ void k_surrogate_out(int n, int K, const float* __restrict__ Em, double shift, double scale,
-                                double* __restrict__ e_site, float* __restrict__ spread) {
+// Surrogate output: member energies e_k = shift_k + scale_k E_k (FP64),
+// e_site = mean_k e_k, spread = max_k e_k - min_k e_k.
+__global__ void k_surrogate_out(int n, int K, const float* __restrict__ Em, const double* __restrict__ shift,
+                                const double* __restrict__ scale, double* __restrict__ e_site,
+                                float* __restrict__ spread) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0.0, lo = 1e300, hi = -1e300;
   for (int k = 0; k < K; ++k) {
-    const double v = (double)Em[(size_t)k * n + i];
+    const double v = shift[k] + scale[k] * (double)Em[(size_t)k * n + i];
     s += v;
     lo = v < lo ? v : lo;
     hi = v > hi ? v : hi;
   }
-  e_site[i] = shift + scale * s / K;
-  spread[i] = (float)(scale * (hi - lo));
+  e_site[i] = s / K;
+  spread[i] = (float)(hi - lo);
 }
 
-__global__ void k_add_f32(int64_t n, const float* __restrict__ x, float* __restrict__ y) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] += x[i];
+// Second layer of every ensemble member, one wavefront per (row, member):
+// v = z2 + b2_k, E_k = w3_k . silu(v) + b3_k, a2 = w3_k silu'(v) (BF16). The
+// members' hidden units sit side by side in rows of K H.
+__global__ __launch_bounds__(256) void k_l2_ens(int rows, int H, int K, const float* __restrict__ z2,
+                                                const float* __restrict__ b2, const float* __restrict__ w3,
+                                                const float* __restrict__ b3, float* __restrict__ Em,
+                                                __bf16* __restrict__ a2b) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int k = blockIdx.y;
+  if (row >= rows) return;
+  const size_t base = (size_t)row * K * H + (size_t)k * H;
+  const float* bk = b2 + (size_t)k * H;
+  const float* wk = w3 + (size_t)k * H;
+  float acc = 0.f;
+  for (int j = lane; j < H; j += 64) {
+    const float v = z2[base + j] + bk[j];
+    acc += wk[j] * silu(v);
+    a2b[base + j] = (__bf16)(wk[j] * silu1(v));
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if (lane == 0) Em[(size_t)k * rows + row] = acc + b3[k];
 }
 
 // Gated sites take the FP64 GAP energy and gradient.
@@ -769,31 +783,55 @@ extern "C" int tgx_mlp_set_weights(tgx_mlp* m, const float* W1, const float* b1,
 
 // ---- surrogate inside tgx_compute ---------------------------------------------
 // Members are networks installed per model; their BF16 weights and biases are
-// copied so the network can be destroyed afterwards.
+// copied so the network can be destroyed afterwards. Before a run the first K
+// members are packed side by side so that one GEMM evaluates every member's
+// first layer (W1 rows stacked, [K H][D]) and one GEMM sums every member's
+// input gradient (W1^T columns stacked, [D][K H]).
 struct SurrogateMember {
   int H = 0;
   double shift = 0, scale = 1;
   dbuf W1b, W1Tb, W2b, W2Tb, bias;  // bias: b1 | b2 | w3 | b3
 };
-static std::vector<std::pair<std::pair<tgx_ctx*, int>, std::vector<SurrogateMember*>>> g_slots;
+struct EnsemblePack {
+  int K = 0, H = 0;
+  bool valid = false;
+  dbuf W1cat, W1Tcat, W2s, W2Ts;  // BF16: [K H][Dp], [Dp][K H], [K][H][H], [K][H][H]
+  dbuf b1, b2, w3, b3, zscale;    // FP32: [K H], [K H], [K H], [K], [K]
+  dbuf shift, scale;              // FP64: [K], [K]
+  void release() {
+    dbuf* all[] = {&W1cat, &W1Tcat, &W2s, &W2Ts, &b1, &b2, &w3, &b3, &zscale, &shift, &scale};
+    for (dbuf* b : all) b->release();
+    valid = false;
+  }
+};
+struct SurrogateSlot {
+  tgx_ctx* ctx = nullptr;
+  int model = -1;
+  std::vector<SurrogateMember*> members;
+  EnsemblePack pack;
+};
+static std::vector<std::unique_ptr<SurrogateSlot>> g_slots;
 
-static std::vector<SurrogateMember*>& slot(tgx_ctx* ctx, int model_id) {
+static SurrogateSlot& slot(tgx_ctx* ctx, int model_id) {
   for (auto& s : g_slots)
-    if (s.first.first == ctx && s.first.second == model_id) return s.second;
-  g_slots.push_back({{ctx, model_id}, {}});
-  return g_slots.back().second;
+    if (s->ctx == ctx && s->model == model_id) return *s;
+  g_slots.emplace_back(new SurrogateSlot());
+  g_slots.back()->ctx = ctx;
+  g_slots.back()->model = model_id;
+  return *g_slots.back();
 }
 
 void tgx_surrogate_release(tgx_ctx* ctx) {
   for (auto& s : g_slots)
-    if (s.first.first == ctx) {
-      for (auto* mb : s.second) {
+    if (s->ctx == ctx) {
+      for (auto* mb : s->members) {
         if (!mb) continue;
         mb->W1b.release(); mb->W1Tb.release(); mb->W2b.release(); mb->W2Tb.release(); mb->bias.release();
         delete mb;
       }
-      s.second.clear();
-      s.first.first = nullptr;
+      s->members.clear();
+      s->pack.release();
+      s->ctx = nullptr;
     }
 }
 
@@ -804,10 +842,11 @@ extern "C" int tgx_surrogate_install(tgx_ctx* ctx, int model_id, int member, tgx
   tgx_model_t* md = ctx->models[model_id];
   TGX_REQUIRE(md->D == m->D && md->Dpad == m->Dp, "network dimension %d does not match model %d", m->D, model_id);
   TGX_REQUIRE(member >= 0 && member < 16, "member index %d outside 0..15", member);
-  auto& sl = slot(ctx, model_id);
-  if ((int)sl.size() <= member) sl.resize(member + 1, nullptr);
-  if (!sl[member]) sl[member] = new SurrogateMember();
-  SurrogateMember* s = sl[member];
+  SurrogateSlot& sl = slot(ctx, model_id);
+  sl.pack.valid = false;
+  if ((int)sl.members.size() <= member) sl.members.resize(member + 1, nullptr);
+  if (!sl.members[member]) sl.members[member] = new SurrogateMember();
+  SurrogateMember* s = sl.members[member];
   const size_t H = m->H, Dp = m->Dp;
   s->H = m->H;
   s->shift = m->shift;
@@ -831,6 +870,53 @@ extern "C" int tgx_surrogate_install(tgx_ctx* ctx, int model_id, int member, tgx
   return TGX_OK;
 }
 
+// Pack the first K members of a slot (device-to-device copies, once per
+// install or change of K).
+static int ensemble_pack(tgx_ctx* ctx, SurrogateSlot& sl, int K, int Dp) {
+  EnsemblePack& pk = sl.pack;
+  if (pk.valid && pk.K == K) return TGX_OK;
+  const size_t H = sl.members[0]->H, KH = (size_t)K * H;
+  TGX_TRY(pk.W1cat.ensure(sizeof(__bf16) * KH * Dp));
+  TGX_TRY(pk.W1Tcat.ensure(sizeof(__bf16) * KH * Dp));
+  TGX_TRY(pk.W2s.ensure(sizeof(__bf16) * KH * H));
+  TGX_TRY(pk.W2Ts.ensure(sizeof(__bf16) * KH * H));
+  TGX_TRY(pk.b1.ensure(sizeof(float) * KH));
+  TGX_TRY(pk.b2.ensure(sizeof(float) * KH));
+  TGX_TRY(pk.w3.ensure(sizeof(float) * KH));
+  TGX_TRY(pk.b3.ensure(sizeof(float) * K));
+  TGX_TRY(pk.zscale.ensure(sizeof(float) * K));
+  TGX_TRY(pk.shift.ensure(sizeof(double) * K));
+  TGX_TRY(pk.scale.ensure(sizeof(double) * K));
+  std::vector<float> zs(K);
+  std::vector<double> sh(K), sc(K);
+  const hipMemcpyKind d2d = hipMemcpyDeviceToDevice;
+  for (int k = 0; k < K; ++k) {
+    SurrogateMember* s = sl.members[k];
+    const float* b = s->bias.as<float>();
+    TGX_CHECK_HIP(hipMemcpyAsync(pk.W1cat.as<__bf16>() + k * H * Dp, s->W1b.p, sizeof(__bf16) * H * Dp, d2d, ctx->stream));
+    TGX_CHECK_HIP(hipMemcpy2DAsync(pk.W1Tcat.as<__bf16>() + k * H, sizeof(__bf16) * KH, s->W1Tb.p, sizeof(__bf16) * H,
+                                   sizeof(__bf16) * H, Dp, d2d, ctx->stream));
+    TGX_CHECK_HIP(hipMemcpyAsync(pk.W2s.as<__bf16>() + k * H * H, s->W2b.p, sizeof(__bf16) * H * H, d2d, ctx->stream));
+    TGX_CHECK_HIP(hipMemcpyAsync(pk.W2Ts.as<__bf16>() + k * H * H, s->W2Tb.p, sizeof(__bf16) * H * H, d2d, ctx->stream));
+    TGX_CHECK_HIP(hipMemcpyAsync(pk.b1.as<float>() + k * H, b, sizeof(float) * H, d2d, ctx->stream));
+    TGX_CHECK_HIP(hipMemcpyAsync(pk.b2.as<float>() + k * H, b + H, sizeof(float) * H, d2d, ctx->stream));
+    TGX_CHECK_HIP(hipMemcpyAsync(pk.w3.as<float>() + k * H, b + 2 * H, sizeof(float) * H, d2d, ctx->stream));
+    TGX_CHECK_HIP(hipMemcpyAsync(pk.b3.as<float>() + k, b + 3 * H, sizeof(float), d2d, ctx->stream));
+    // the input gradient of member k enters the sum -(1/K) sum_k scale_k dE_k/dq
+    zs[k] = (float)(-s->scale / K);
+    sh[k] = s->shift;
+    sc[k] = s->scale;
+  }
+  TGX_CHECK_HIP(hipMemcpyAsync(pk.zscale.p, zs.data(), sizeof(float) * K, hipMemcpyHostToDevice, ctx->stream));
+  TGX_CHECK_HIP(hipMemcpyAsync(pk.shift.p, sh.data(), sizeof(double) * K, hipMemcpyHostToDevice, ctx->stream));
+  TGX_CHECK_HIP(hipMemcpyAsync(pk.scale.p, sc.data(), sizeof(double) * K, hipMemcpyHostToDevice, ctx->stream));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  pk.K = K;
+  pk.H = (int)H;
+  pk.valid = true;
+  return TGX_OK;
+}
+
 extern "C" int tgx_surrogate_gate(tgx_ctx* ctx, int n_members, double threshold) {
   TGX_REQUIRE(ctx && n_members >= 1 && n_members <= 16, "tgx_surrogate_gate: 1..16 members");
   ctx->gate_members = n_members;
@@ -847,60 +933,87 @@ extern "C" int tgx_surrogate_stats(tgx_ctx* ctx, double* gated_fraction, double*
 
 // Surrogate energies and gradients for one block; forces follow through the
 // FP32 adjoint kernel. Gated sites are re-evaluated by the FP64 GAP.
+//   L1    : [h1 | s1] = silu / silu' (q W1cat^T + b1)       one GEMM, N = K H
+//   L2    : z2_k = h1_k W2_k^T                              batched over k
+//   act2  : E_k, a2_k = w3_k silu'(z2_k + b2_k)
+//   back2 : a1_k = -(scale_k/K) (a2_k W2_k) o s1_k           batched over k
+//   back1 : Qss = a1 W1Tcat^T = -(1/K) sum_k scale_k dE_k/dq  one GEMM, depth K H
 int tgx_surrogate_run(tgx_ctx* ctx, tgx_model_t* m) {
   int model_id = -1;
   for (int k = 0; k < (int)ctx->models.size(); ++k)
     if (ctx->models[k] == m) model_id = k;
-  auto& sl = slot(ctx, model_id);
+  SurrogateSlot& sl = slot(ctx, model_id);
   const int K = ctx->gate_members;
-  TGX_REQUIRE((int)sl.size() >= K, "surrogate: model %d has %zu installed members, %d requested", model_id, sl.size(), K);
-  for (int k = 0; k < K; ++k) TGX_REQUIRE(sl[k], "surrogate: member %d of model %d missing", k, model_id);
+  TGX_REQUIRE((int)sl.members.size() >= K, "surrogate: model %d has %zu installed members, %d requested", model_id,
+              sl.members.size(), K);
+  for (int k = 0; k < K; ++k) TGX_REQUIRE(sl.members[k], "surrogate: member %d of model %d missing", k, model_id);
+  const int H = sl.members[0]->H;
+  for (int k = 1; k < K; ++k) TGX_REQUIRE(sl.members[k]->H == H, "surrogate members differ in width");
   TGX_TRY(tgx_block_select(ctx, m));
   const int ns = m->n_sites_b;
   if (ns == 0) return TGX_OK;
   TGX_TRY(tgx_soap_descriptors(ctx, m, TGX_PATH_MIXED, false));
   const int Mp = (int)tgx_round_up(ns, 128), Dp = m->Dpad;
-  const int H = sl[0]->H;
-  for (int k = 1; k < K; ++k) TGX_REQUIRE(sl[k]->H == H, "surrogate members differ in width");
+  TGX_TRY(ensemble_pack(ctx, sl, K, Dp));
+  const EnsemblePack& pk = sl.pack;
+  const int KH = K * H;
   TGX_TRY(m->qbf.ensure(sizeof(__bf16) * (size_t)Mp * Dp));
   TGX_TRY(m->qss32.ensure(sizeof(float) * (size_t)Mp * Dp));
   TGX_TRY(m->e_site.ensure(sizeof(double) * (size_t)Mp));
   TGX_TRY(tgx_f32_to_bf16(ctx, m->q32.as<float>(), m->qbf.as<__bf16>(), (int64_t)Mp * Dp));
-  // work space in the context scratch: z1, s1, z2, v1 (FP32 Mp x H), h1b, a2b, a1b (BF16), Em (K x Mp), G (Mp x Dp)
-  const size_t fh = (size_t)Mp * H;
-  const size_t bytes = sizeof(float) * (4 * fh + (size_t)K * Mp + (size_t)Mp * Dp + Mp) + sizeof(__bf16) * 3 * fh + 16 * 256;
+  // work space in the context scratch: s1, z2 (FP32 Mp x K H), h1b, a2b, a1b (BF16 Mp x K H), Em (K x Mp), spread
+  const size_t fh = (size_t)Mp * KH;
+  const size_t bytes = sizeof(float) * (2 * fh + (size_t)K * Mp + Mp) + sizeof(__bf16) * 3 * fh + 8 * 256;
   TGX_TRY(ctx->e_members.ensure(bytes));
   char* p = ctx->e_members.as<char>();
   auto take = [&](size_t n) { char* r = p; p += tgx_round_up((int64_t)n, 256); return r; };
-  float* z1 = (float*)take(sizeof(float) * fh);
   float* s1 = (float*)take(sizeof(float) * fh);
   float* z2 = (float*)take(sizeof(float) * fh);
-  float* v1 = (float*)take(sizeof(float) * fh);
   __bf16* h1b = (__bf16*)take(sizeof(__bf16) * fh);
   __bf16* a2b = (__bf16*)take(sizeof(__bf16) * fh);
   __bf16* a1b = (__bf16*)take(sizeof(__bf16) * fh);
   float* Em = (float*)take(sizeof(float) * K * Mp);
-  float* G = (float*)take(sizeof(float) * (size_t)Mp * Dp);
   float* spread = (float*)take(sizeof(float) * Mp);
-  const int64_t nbh = (int64_t)Mp * H;
-  for (int k = 0; k < K; ++k) {
-    SurrogateMember* s = sl[k];
-    const float* b = s->bias.as<float>();
-    TGX_TRY(tgx_gemm_bf16_nt(ctx, "mlp_l1", Mp, H, Dp, m->qbf.as<__bf16>(), Dp, s->W1b.as<__bf16>(), Dp, z1, H, 1.0));
-    TGX_LAUNCH(ctx, "mlp_act1", k_l1, g1(nbh), dim3(256), 0, nbh, H, z1, b, h1b, (float*)nullptr, s1, (float*)nullptr);
-    TGX_TRY(tgx_gemm_bf16_nt(ctx, "mlp_l2", Mp, H, H, h1b, H, s->W2b.as<__bf16>(), H, z2, H, 1.0));
-    TGX_LAUNCH(ctx, "mlp_act2", k_l2, dim3((Mp + 3) / 4), dim3(256), 0, Mp, H, z2, b + H, b + 2 * H, b + 3 * H,
-               Em + (size_t)k * Mp, a2b, (float*)nullptr, (float*)nullptr, (float*)nullptr, (float*)nullptr);
-    TGX_TRY(tgx_gemm_bf16_nt(ctx, "mlp_back2", Mp, H, H, a2b, H, s->W2Tb.as<__bf16>(), H, v1, H, 1.0));
-    TGX_LAUNCH(ctx, "mlp_a1", k_a1, g1(nbh), dim3(256), 0, nbh, v1, s1, (float*)nullptr, a1b);
-    // Qss = -dE/dq = -(scale/K) sum_k G_k
-    float* dst = k == 0 ? m->qss32.as<float>() : G;
-    TGX_TRY(tgx_gemm_bf16_nt(ctx, "mlp_back1", Mp, Dp, H, a1b, H, s->W1Tb.as<__bf16>(), H, dst, Dp, -s->scale / K));
-    if (k > 0)
-      TGX_LAUNCH(ctx, "mlp_addg", k_add_f32, g1((int64_t)Mp * Dp), dim3(256), 0, (int64_t)Mp * Dp, G, m->qss32.as<float>());
-  }
-  TGX_LAUNCH(ctx, "mlp_out", k_surrogate_out, g1(ns), dim3(256), 0, ns, K, Em, sl[0]->shift, sl[0]->scale,
-             m->e_site.as<double>(), spread);
+
+  tgx_gemm_desc l1;
+  l1.epi = TGX_EPI_SILU;
+  l1.M = Mp; l1.N = KH; l1.K = Dp;
+  l1.A = m->qbf.as<__bf16>(); l1.lda = Dp;
+  l1.B = pk.W1cat.as<__bf16>(); l1.ldb = Dp;
+  l1.C = h1b; l1.ldc = KH;
+  l1.bias = pk.b1.as<float>();
+  l1.aux = s1; l1.ldaux = KH;
+  TGX_TRY(tgx_gemm_bf16(ctx, "mlp_l1", l1));
+
+  tgx_gemm_desc l2;
+  l2.M = Mp; l2.N = H; l2.K = H; l2.batch = K;
+  l2.A = h1b; l2.lda = KH; l2.sA = H;
+  l2.B = pk.W2s.as<__bf16>(); l2.ldb = H; l2.sB = (int64_t)H * H;
+  l2.C = z2; l2.ldc = KH; l2.sC = H;
+  TGX_TRY(tgx_gemm_bf16(ctx, "mlp_l2", l2));
+
+  TGX_LAUNCH(ctx, "mlp_act2", k_l2_ens, dim3((Mp + 3) / 4, K), dim3(256), 0, Mp, H, K, z2, pk.b2.as<float>(),
+             pk.w3.as<float>(), pk.b3.as<float>(), Em, a2b);
+
+  tgx_gemm_desc b2;
+  b2.epi = TGX_EPI_SCALE_AUX;
+  b2.M = Mp; b2.N = H; b2.K = H; b2.batch = K;
+  b2.A = a2b; b2.lda = KH; b2.sA = H;
+  b2.B = pk.W2Ts.as<__bf16>(); b2.ldb = H; b2.sB = (int64_t)H * H;
+  b2.C = a1b; b2.ldc = KH; b2.sC = H;
+  b2.aux = s1; b2.ldaux = KH; b2.sAux = H;
+  b2.zscale = pk.zscale.as<float>();
+  TGX_TRY(tgx_gemm_bf16(ctx, "mlp_back2", b2));
+
+  tgx_gemm_desc b1;
+  b1.M = Mp; b1.N = Dp; b1.K = KH;
+  b1.A = a1b; b1.lda = KH;
+  b1.B = pk.W1Tcat.as<__bf16>(); b1.ldb = KH;
+  b1.C = m->qss32.p; b1.ldc = Dp;
+  TGX_TRY(tgx_gemm_bf16(ctx, "mlp_back1", b1));
+
+  TGX_LAUNCH(ctx, "mlp_out", k_surrogate_out, g1(ns), dim3(256), 0, ns, K, Em, pk.shift.as<double>(),
+             pk.scale.as<double>(), m->e_site.as<double>(), spread);
   // ensemble gate
   ctx->gated_fraction = 0.0;
   ctx->max_spread = 0.0;
