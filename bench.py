@@ -43,7 +43,7 @@ def kernel_table(ctx, names):
     return out
 
 
-KERNELS = ["mlp_l1", "mlp_back1", "mlp_l2", "mlp_back2", "nbr_count", "nbr_fill", "nbr_reverse", "select_fill", "soap_site_der_f64", "soap_desc_f64", "soap_desc_f32",
+KERNELS = ["mlp_l1", "mlp_back1", "mlp_l2", "mlp_back2", "mlp_act2", "mlp_out", "soap_pair_basis", "nbr_count", "nbr_fill", "nbr_reverse", "select_fill", "soap_site_der_f64", "soap_desc_f64", "soap_desc_f32",
            "soap_site_f32", "soap_adjoint_f64", "soap_adjoint_f32", "gemm_kernel_f64", "gemm_qss_f64",
            "gemm_kernel_bf16", "gemm_qss_bf16", "plain_force", "force_gather", "virial"]
 
@@ -149,16 +149,16 @@ def prepare_surrogate(ctx, ids, sysm, models, args):
 
 def cpu_baseline(models, n_sample, threads):
     """The FP64 oracle on a bounded sample of the workload: a box of n_sample
-    atoms at config 1's density with the same blocks and sparse sets."""
+    atoms at the density of the 65,536-atom box with the same blocks and sparse sets."""
     from oracle import oracle as O
     from turbogap_amd import structures as S
     O.set_threads(threads)
-    sysm = S.config1_system(n_sample, seed=4242)
+    sysm = S.binary_system(n_sample, seed=4242)
     t0 = time.perf_counter()
     O.compute(sysm, models)
     dt = time.perf_counter() - t0
     return {"value": n_sample / dt, "unit": "atom-steps/s", "cores": threads, "kind": "port",
-            "sample": f"one E+F+virial single point of a {n_sample}-atom box at config 1's density "
+            "sample": f"one E+F+virial single point of a {n_sample}-atom box at the density of the 65,536-atom box "
                       f"(same hypers and 2 x 4000 sparse points), FP64 oracle, OpenMP {threads} threads, {dt:.2f} s"}
 
 
@@ -185,9 +185,9 @@ def main():
     from turbogap_amd import structures as S
 
     t0 = time.perf_counter()
-    sysm = S.config1_system(args.atoms, seed=2001)
+    sysm = S.binary_system(args.atoms, seed=2001)
     log(f"system: {sysm.n} atoms, L={sysm.box[0,0]:.3f} A ({time.perf_counter()-t0:.1f} s)")
-    models = S.config1_models(S.gpu_describe(), n_sparse=args.sparse)
+    models = S.binary_models(S.gpu_describe(), n_sparse=args.sparse)
     log(f"sparse sets: 2 x {args.sparse} ({time.perf_counter()-t0:.1f} s)")
     paths = [p for p in args.paths.split(",") if p]
     res = {}

@@ -1,4 +1,4 @@
-"""Writes tests/golden/config0_golden.npz: the FP64 oracle's outputs for the
+"""Writes tests/golden/carbon216_golden.npz: the FP64 oracle's outputs for the
 216-atom configuration (seeded structure, 200 sparse points), used as a
 regression fixture by tests/test_golden.py (oracle) and tests/test_gpu_golden.py
 (GPU). Inputs are not stored: they are regenerated from the same seeds.
@@ -18,9 +18,9 @@ from turbogap_amd import structures as S  # noqa: E402
 
 
 def build():
-    sysm = S.config0_system()
-    h = S.soap_config0()
-    Qs, al = S.sparse_set(h, 200, O.block_descriptors, S.config0_system, seed=5001)
+    sysm = S.carbon216_system()
+    h = S.soap_carbon()
+    Qs, al = S.sparse_set(h, 200, O.block_descriptors, S.carbon216_system, seed=5001)
     E, en, F, W, nl = O.compute(sysm, [(h, Qs, al)])
     spi = O.species_index(nl["neighbor_species"], h.species_types)
     q, _ = O.get_soap(h, nl["n_neigh"], spi, nl["rjs"], nl["xyz"], derivatives=False)
@@ -30,6 +30,6 @@ def build():
 
 
 if __name__ == "__main__":
-    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "config0_golden.npz")
+    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "carbon216_golden.npz")
     np.savez_compressed(out, **build())
     print("wrote", out, os.path.getsize(out), "bytes")

@@ -8,22 +8,22 @@ from turbogap_amd import structures as S
 
 
 @functools.lru_cache(maxsize=None)
-def config0(n_sparse=2000):
+def carbon216(n_sparse=2000):
     from oracle import oracle as O
-    sysm = S.config0_system()
-    h = S.soap_config0()
-    Qs, al = S.sparse_set(h, n_sparse, O.block_descriptors, S.config0_system, seed=5001)
+    sysm = S.carbon216_system()
+    h = S.soap_carbon()
+    Qs, al = S.sparse_set(h, n_sparse, O.block_descriptors, S.carbon216_system, seed=5001)
     return sysm, h, Qs, al
 
 
 @functools.lru_cache(maxsize=None)
 def subset(n_sparse=500, n=4096):
-    """Two-species box at config 1's density with both central-species blocks."""
+    """Two-species box at the density of the 65,536-atom box with both central-species blocks."""
     from oracle import oracle as O
-    sysm = S.config1_system(n, seed=2002)
+    sysm = S.binary_system(n, seed=2002)
     models = []
-    for k, h in enumerate(S.soap_config1()):
-        Qs, al = S.sparse_set(h, n_sparse, O.block_descriptors, lambda s: S.config1_system(1024, seed=s),
+    for k, h in enumerate(S.soap_binary()):
+        Qs, al = S.sparse_set(h, n_sparse, O.block_descriptors, lambda s: S.binary_system(1024, seed=s),
                               seed=7001 + 100 * k)
         models.append((h, Qs, al))
     return sysm, models

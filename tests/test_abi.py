@@ -54,12 +54,12 @@ def test_generators_deterministic_and_valid():
 
 
 def test_config_shapes():
-    s1 = S.config1_system(4096, seed=1)
+    s1 = S.binary_system(4096, seed=1)
     assert s1.n == 4096 and abs(s1.box[0, 0] - 86.8 * (1 / 16) ** (1 / 3)) < 1e-12
     assert (s1.species == S.SPECIES_A).sum() == 2048
-    h0 = S.soap_config0()
+    h0 = S.soap_carbon()
     assert h0.dim == 324 and h0.rcut_soft == [3.2]
-    assert S.soap_config1()[0].dim == 1224
+    assert S.soap_binary()[0].dim == 1224
 
 
 def test_fortran_shim_binds_declared_symbols():

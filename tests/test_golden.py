@@ -1,4 +1,4 @@
-"""The oracle reproduces the committed config-0 fixture (tests/golden):
+"""The oracle reproduces the committed 216-atom carbon fixture (tests/golden):
 neighbour list bit for bit, energies/forces/virial to 1e-12."""
 import os
 
@@ -6,7 +6,7 @@ import numpy as np
 
 from tests.golden import make_golden
 
-GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config0_golden.npz")
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "carbon216_golden.npz")
 
 
 def test_oracle_reproduces_golden(orc):

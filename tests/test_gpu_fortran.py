@@ -16,11 +16,11 @@ from tests import helpers as H
 pytestmark = pytest.mark.gpu
 ROOT = pathlib.Path(__file__).resolve().parents[1]
 EXE = ROOT / "turbogap_amd" / "fortran" / "_build" / "tgx_example"
-GOLDEN = ROOT / "tests" / "golden" / "config0_golden.npz"
+GOLDEN = ROOT / "tests" / "golden" / "carbon216_golden.npz"
 
 
 def _write_inputs(d: pathlib.Path):
-    sysm, h, Qs, al = H.config0(200)
+    sysm, h, Qs, al = H.carbon216(200)
     L = float(sysm.box[0, 0])
     with open(d / "positions.dat", "w") as f:
         f.write(f"{len(sysm.positions)} {L:.17g}\n")

@@ -1,4 +1,4 @@
-"""The GPU path reproduces the committed config-0 fixture (tests/golden):
+"""The GPU path reproduces the committed 216-atom carbon fixture (tests/golden):
 neighbour list bit for bit, energies to 1e-8 relative, forces to 1e-6 eV/Å."""
 import os
 
@@ -8,13 +8,13 @@ import pytest
 from tests import helpers as H
 
 pytestmark = pytest.mark.gpu
-GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config0_golden.npz")
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "carbon216_golden.npz")
 
 
 @pytest.mark.parametrize("path", ["fp64_plain", "fp64_adjoint"])
 def test_gpu_reproduces_golden(ctx, path):
     ref = np.load(GOLDEN)
-    sysm, h, Qs, al = H.config0(200)
+    sysm, h, Qs, al = H.carbon216(200)
     np.testing.assert_allclose([Qs.sum(), al.sum()], ref["Qs_checksum"], rtol=1e-13)
     ctx.set_system(sysm.positions, sysm.species, sysm.box)
     nl = ctx.build_neighbors_list(3.7)

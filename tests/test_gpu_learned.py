@@ -107,7 +107,7 @@ def test_surrogate_learns_and_gate_falls_back(ctx):
 
 
 def test_velocity_verlet_and_berendsen(ctx):
-    sysm, h, Qs, al = H.config0(300)
+    sysm, h, Qs, al = H.carbon216(300)
     ctx.set_system(sysm.positions, sysm.species, sysm.box)
     ctx.soap_model(h, Qs, al)
     mass = S.masses_for(sysm.species)
@@ -136,8 +136,8 @@ def test_velocity_verlet_and_berendsen(ctx):
     np.testing.assert_allclose(v3, v1b * np.sqrt(1 + dt / 50.0 * (300.0 / T - 1)), rtol=1e-13)
 
 
-def _config0_block(ctx):
-    sysm, h, Qs, al = H.config0(200)
+def _carbon216_block(ctx):
+    sysm, h, Qs, al = H.carbon216(200)
     ctx.set_system(sysm.positions, sysm.species, sysm.box)
     ctx.build_neighbors_list(h.rcut_max, fetch=False)
     return sysm, h, ctx.soap_model(h, Qs, al)
@@ -147,7 +147,7 @@ def test_surrogate_energies_match_mlp_oracle(ctx):
     """One installed network: the fused inference (SiLU epilogue on the first
     GEMM) gives e_i = shift + scale E(q_i) of the oracle network on the FP64
     descriptors (BF16 operands: 2e-2 of the output range)."""
-    sysm, h, mid = _config0_block(ctx)
+    sysm, h, mid = _carbon216_block(ctx)
     net = SG.MLP(ctx, h.dim, 128, 256, seed=21)
     net.set_normalization(-0.5, 0.03)
     net.install(mid, 0)
@@ -167,7 +167,7 @@ def test_surrogate_ensemble_is_member_mean(ctx):
     """Two members with different normalisations: energies are the mean of the
     single-member runs, the spread their difference, and forces (linear in
     dE/dq) the mean of the single-member forces."""
-    sysm, h, mid = _config0_block(ctx)
+    sysm, h, mid = _carbon216_block(ctx)
     a = SG.MLP(ctx, h.dim, 128, 256, seed=21)
     b = SG.MLP(ctx, h.dim, 128, 256, seed=22)
     a.set_normalization(-0.5, 0.03)

@@ -25,8 +25,8 @@ def _compare_lists(g, o):
     np.testing.assert_allclose(g["phis"], o["phis"], rtol=0, atol=1e-14)
 
 
-def test_neighbors_config0_bitexact(ctx, orc):
-    sysm, h, _, _ = H.config0(200)
+def test_neighbors_carbon216_bitexact(ctx, orc):
+    sysm, h, _, _ = H.carbon216(200)
     ctx.set_system(sysm.positions, sysm.species, sysm.box)
     g = ctx.build_neighbors_list(h.rcut_max)
     o = orc.build_neighbors_list(sysm.positions, sysm.species, sysm.box, h.rcut_max)
@@ -34,7 +34,7 @@ def test_neighbors_config0_bitexact(ctx, orc):
 
 
 def test_neighbors_subset_bitexact(ctx, orc):
-    sysm = S.config1_system(4096, seed=2002)
+    sysm = S.binary_system(4096, seed=2002)
     ctx.set_system(sysm.positions, sysm.species, sysm.box)
     g = ctx.build_neighbors_list(3.7)
     o = orc.build_neighbors_list(sysm.positions, sysm.species, sysm.box, 3.7)
@@ -53,7 +53,7 @@ def test_neighbors_small_box_supercell(ctx, orc):
 
 
 def test_neighbors_do_list(ctx, orc):
-    sysm, h, _, _ = H.config0(200)
+    sysm, h, _, _ = H.carbon216(200)
     do = (np.arange(sysm.n) % 3 != 0).astype(np.int32)
     ctx.set_system(sysm.positions, sysm.species, sysm.box)
     g = ctx.build_neighbors_list(h.rcut_max, do_list=do)
@@ -61,8 +61,8 @@ def test_neighbors_do_list(ctx, orc):
     _compare_lists(g, o)
 
 
-def test_soap_config0(ctx, orc):
-    sysm, h, Qs, al = H.config0(200)
+def test_soap_carbon(ctx, orc):
+    sysm, h, Qs, al = H.carbon216(200)
     ctx.set_system(sysm.positions, sysm.species, sysm.box)
     nl = ctx.build_neighbors_list(h.rcut_max)
     mid = ctx.soap_model(h, Qs, al)
@@ -76,8 +76,8 @@ def test_soap_config0(ctx, orc):
 
 
 @pytest.mark.parametrize("path", ["fp64_plain", "fp64_adjoint"])
-def test_gap_config0(ctx, orc, path):
-    sysm, h, Qs, al = H.config0(2000)
+def test_gap_carbon216(ctx, orc, path):
+    sysm, h, Qs, al = H.carbon216(2000)
     ctx.set_system(sysm.positions, sysm.species, sysm.box)
     ctx.build_neighbors_list(h.rcut_max, fetch=False)
     mid = ctx.soap_model(h, Qs, al)
@@ -126,7 +126,7 @@ def test_gap_subset_two_species(ctx, orc, path):
 
 def test_soap_energy_and_forces_host_inputs(ctx, orc):
     """get_soap_energy_and_forces with the oracle's descriptors as host input."""
-    sysm, h, Qs, al = H.config0(2000)
+    sysm, h, Qs, al = H.carbon216(2000)
     nl = orc.build_neighbors_list(sysm.positions, sysm.species, sysm.box, h.rcut_max)
     spi = orc.species_index(nl["neighbor_species"], h.species_types)
     q, der = orc.get_soap(h, nl["n_neigh"], spi, nl["rjs"], nl["xyz"])
@@ -158,7 +158,7 @@ def test_compute_deterministic(ctx):
 def test_perfect_crystal(ctx, orc, path):
     """Perfect diamond (site symmetry Td): oracle parity and the invariants equal energies, zero forces, isotropic virial."""
     from tests.test_oracle_gap import diamond
-    _, h, Qs, al = H.config0(300)
+    _, h, Qs, al = H.carbon216(300)
     sysm = diamond()
     ctx.set_system(sysm.positions, sysm.species, sysm.box)
     ctx.build_neighbors_list(h.rcut_max, fetch=False)

@@ -22,7 +22,7 @@ def _energy(ctx, pos, box, path):
 
 
 def test_adjoint_forces_finite_difference(ctx):
-    sysm, h, Qs, al = H.config0(500)
+    sysm, h, Qs, al = H.carbon216(500)
     ctx.set_system(sysm.positions, sysm.species, sysm.box)
     ctx.soap_model(h, Qs, al)
     ctx.build_neighbors_list(3.7, fetch=False)
@@ -42,7 +42,7 @@ def test_adjoint_forces_finite_difference(ctx):
 
 
 def test_virial_strain_finite_difference(ctx):
-    sysm, h, Qs, al = H.config0(500)
+    sysm, h, Qs, al = H.carbon216(500)
     ctx.set_system(sysm.positions, sysm.species, sysm.box)
     ctx.soap_model(h, Qs, al)
     ctx.build_neighbors_list(3.7, fetch=False)
@@ -76,7 +76,7 @@ def test_mixed_path_error_bounds(ctx, orc):
     assert np.abs(W - Wo).max() <= 0.05
 
 
-def test_fp64_paths_agree_config1_density(ctx):
+def test_fp64_paths_agree_binary_density(ctx):
     """Plain (materialised dq/dr) and adjoint forces agree to round-off."""
     sysm, models = H.subset(300, 4096)
     ctx.set_system(sysm.positions, sysm.species, sysm.box)

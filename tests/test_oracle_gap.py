@@ -26,7 +26,7 @@ def _block(orc, h, sysm):
 
 
 def test_energy_closed_form_and_special_cases(orc):
-    sysm, h, Qs, al = H.config0(300)
+    sysm, h, Qs, al = H.carbon216(300)
     nl, q, der = _block(orc, h, sysm)
     for zeta in (1.0, 2.0, 4.0, 2.5):
         e, _, _ = orc.get_soap_energy_and_forces(q, der, Qs, al, 1.3, zeta, 0.1, nl["n_neigh"], nl["neighbors_list"],
@@ -49,7 +49,7 @@ def test_energy_closed_form_and_special_cases(orc):
 
 
 def test_scatter_matches_einsum(orc):
-    sysm, h, Qs, al = H.config0(300)
+    sysm, h, Qs, al = H.carbon216(300)
     nl, q, der = _block(orc, h, sysm)
     e, F, W = orc.get_soap_energy_and_forces(q, der, Qs, al, 1.0, 4.0, 0.0, nl["n_neigh"], nl["neighbors_list"],
                                              nl["xyz"], sysm.n)
@@ -65,7 +65,7 @@ def test_scatter_matches_einsum(orc):
 
 
 def test_forces_virial_finite_difference(orc):
-    sysm, h, Qs, al = H.config0(300)
+    sysm, h, Qs, al = H.carbon216(300)
     models = [(h, Qs, al)]
     E, en, F, W, _ = orc.compute(sysm, models)
     np.testing.assert_allclose(F.sum(0), 0.0, atol=1e-13)
@@ -121,7 +121,7 @@ def diamond(n_cells=3, a=3.567):
 def test_perfect_crystal_invariants(orc):
     """Every atom of perfect diamond is equivalent with site symmetry Td: equal
     local energies, zero forces, and a cubic (isotropic) virial."""
-    _, h, Qs, al = H.config0(300)
+    _, h, Qs, al = H.carbon216(300)
     sysm = diamond()
     E, en, F, W, _ = orc.compute(sysm, [(h, Qs, al)])
     assert np.ptp(en) <= 1e-10 * max(1.0, np.abs(en).max())

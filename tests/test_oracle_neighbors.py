@@ -46,8 +46,8 @@ def _check_against_brute(nl, pos, box, rc, n_fold):
         assert got == exp, f"atom {i}"
 
 
-def test_config0_matches_brute_force(orc):
-    sysm = S.config0_system()
+def test_carbon216_matches_brute_force(orc):
+    sysm = S.carbon216_system()
     nl = orc.build_neighbors_list(sysm.positions, sysm.species, sysm.box, 3.7)
     _check_against_brute(nl, sysm.positions, sysm.box, 3.7, sysm.n)
     # xyz is the minimum-image displacement and |xyz| = rjs
@@ -55,7 +55,7 @@ def test_config0_matches_brute_force(orc):
 
 
 def test_list_properties_and_cell_order(orc):
-    sysm = S.config1_system(2048, seed=77)
+    sysm = S.binary_system(2048, seed=77)
     rc = 3.7
     nl = orc.build_neighbors_list(sysm.positions, sysm.species, sysm.box, rc)
     beg = np.concatenate([[0], np.cumsum(nl["n_neigh"])])
@@ -125,7 +125,7 @@ def test_batch_memory_model_closed_form(orc):
 
 def test_batches_partition_sites(orc):
     import ctypes as C
-    sysm = S.config0_system()
+    sysm = S.carbon216_system()
     nl = orc.build_neighbors_list(sysm.positions, sysm.species, sysm.box, 3.7)
     lib = orc.load()
     n = sysm.n

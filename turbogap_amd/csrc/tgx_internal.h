@@ -117,11 +117,9 @@ struct tgx_model_t {
   SoapParams P{};
   // device copies of the tables
   dbuf B, xq, cen, dtab, ynorm, rtab;  // rtab: FP32 Hermite table of R_n, dR_n/dr
+  dbuf low_idx, low_coef;              // lowering table of the solid harmonics [NLM][3][2]
   // sparse set: Qs[Spad][Dpad], QsT[Dpad][Spad], alphas[Spad] (FP64) and BF16 copies
   dbuf Qs, QsT, alpha, Qs_bf, QsT_bf, alpha_f;
-  // surrogate members (BF16 weights + FP32 biases)
-  struct member { int hidden = 0; dbuf W1, W2, w3, b1, b2, b3; double e_shift = 0, e_scale = 1; };
-  std::vector<member> mlp;
   // per-step block state (device)
   int n_sites_b = 0;
   int64_t n_pairs_b = 0;
@@ -140,6 +138,7 @@ struct tgx_model_t {
   dbuf kd64, kdbf;  // [n_sites_pad][Spad] K^(zeta-1) alpha (scaled)
   dbuf epart;       // energy partials [n_sites_pad][n_tiles]
   dbuf soap_der;    // [n_pairs_b][D][3] FP64 (plain path)
+  dbuf pbasis;      // [n_pairs_b][ldb] FP32 pair basis: A_l Y_lm | R_n | dR_n (FP32 paths)
   dbuf fpair;       // [n_pairs_b][3] pair forces
 };
 

@@ -274,15 +274,15 @@ __global__ void k_append(int n_take, int stride, int offset, int Dp, const doubl
 }
 
 // Surrogate output: member energies e_k = shift_k + scale_k E_k (FP64),
-// e_site = mean_k e_k, spread = max_k e_k - min_k e_k.
-__global__ void k_surrogate_out(int n, int K, const float* __restrict__ Em, const double* __restrict__ shift,
+// e_site = mean_k e_k, spread = max_k e_k - min_k e_k. Em is [K][ldm].
+__global__ void k_surrogate_out(int n, int K, const float* __restrict__ Em, int ldm, const double* __restrict__ shift,
                                 const double* __restrict__ scale, double* __restrict__ e_site,
                                 float* __restrict__ spread) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0.0, lo = 1e300, hi = -1e300;
   for (int k = 0; k < K; ++k) {
-    const double v = shift[k] + scale[k] * (double)Em[(size_t)k * n + i];
+    const double v = shift[k] + scale[k] * (double)Em[(size_t)k * ldm + i];
     s += v;
     lo = v < lo ? v : lo;
     hi = v > hi ? v : hi;
@@ -1012,7 +1012,7 @@ int tgx_surrogate_run(tgx_ctx* ctx, tgx_model_t* m) {
   b1.C = m->qss32.p; b1.ldc = Dp;
   TGX_TRY(tgx_gemm_bf16(ctx, "mlp_back1", b1));
 
-  TGX_LAUNCH(ctx, "mlp_out", k_surrogate_out, g1(ns), dim3(256), 0, ns, K, Em, pk.shift.as<double>(),
+  TGX_LAUNCH(ctx, "mlp_out", k_surrogate_out, g1(ns), dim3(256), 0, ns, K, Em, Mp, pk.shift.as<double>(),
              pk.scale.as<double>(), m->e_site.as<double>(), spread);
   // ensemble gate
   ctx->gated_fraction = 0.0;

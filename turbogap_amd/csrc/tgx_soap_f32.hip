@@ -6,18 +6,30 @@
 // cubic Hermite interpolation (error ~ h^4 R''''/384, below 1e-9 relative at
 // K = 2048); the force uses the derivative of the same interpolant.
 //
-// Work split: one wavefront per site; each neighbour pair is shared by LP = 3
-// lanes that split its real-harmonic recurrence by m (m = part, part+3, ...),
-// so a typical site (about 21 pairs) fills the wavefront. Expansion and
-// adjoint contractions run with lanes over (l, m) as in the FP64 kernels.
+// Three kernels per block and step:
+//   k_pair_basis   one thread per pair: the damped harmonics W_lm = A_l(r) Y_lm
+//                  and R_n, dR_n/dr (stacked by species) as one row per pair,
+//                  staged in LDS and written as contiguous 64-row panels;
+//   k_site_desc    one wavefront per site, lanes over lm: c(n,lm) = sum_j R_n W_lm
+//                  from the site's rows (coalesced row loads, R by broadcast),
+//                  then the power spectrum and q = p/|p|;
+//   k_site_adjoint one wavefront per site: c again, H(n,lm) = sum_n' S c with
+//                  S from g = (Qss - (Qss.q) q)/|p|, and for every pair
+//                  f = u T + V / r with
+//                    T = sum_lm (l(l+1) k' x + y - l x / r) W_lm,
+//                    V = sum_lm x (A_l/A_(l-1)) sum_t C_t W_(l-1, m_t),
+//                  x = sum_n R_n H, y = sum_n dR_n H (k' = -dk/dr, k = st^2/(2r^2));
+//                  the gradients of the solid harmonics come from the degree
+//                  below through the lowering table C (tgx_sph_host.h), so no
+//                  harmonic is recomputed.
 #include "tgx_internal.h"
+#include "tgx_sph_host.h"
 
 namespace {
 
 constexpr int WAVES = 4;
-constexpr int LP = 3;           // lanes per pair
-constexpr int CH = 64 / LP;     // pairs per pass (21)
-constexpr int NMAXF = 16;       // stacked radial functions held per lane
+constexpr int NMAXF = 16;  // stacked radial functions held per lane
+constexpr int KLOW = 2;    // terms per Cartesian component in the lowering table
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -38,6 +50,8 @@ struct F32Args {
   const int64_t* pr_gidx;
   const float* rtab;   // [S][K][2][MAX_NB]: R then dR per node
   int K;
+  float* basis;        // [pairs][ldb]: W (NLM, padded to roff) | R (N) | dR (N)
+  int ldb, roff;
 };
 
 struct SpecLite {
@@ -73,53 +87,6 @@ __device__ __forceinline__ void radial_tab(const F32Args& a, const SpecLite& sp,
   }
 }
 
-// Harmonics with m = part, part + LP, ... streamed to f(l, lm, Y, gx, gy, gz).
-template <bool GRAD, class F>
-__device__ __forceinline__ void ylm_part(int L, int part, const float* ynorm, float ux, float uy, float uz, F&& f) {
-  float c = 1, cx = 0, cy = 0, s = 0, sx = 0, sy = 0;
-  for (int m = 0; m <= L; ++m) {
-    if (m > 0) {
-      const float nc = c * ux - s * uy, ns = c * uy + s * ux;
-      if (GRAD) {
-        const float ncx = cx * ux + c - sx * uy, ncy = cy * ux - sy * uy - s;
-        const float nsx = cx * uy + sx * ux + s, nsy = cy * uy + c + sy * ux;
-        cx = ncx; cy = ncy; sx = nsx; sy = nsy;
-      }
-      c = nc;
-      s = ns;
-    }
-    if ((m % LP) != part) continue;
-    float q = 1, qx = 0, qy = 0, qz = 0, p = 0, px = 0, py = 0, pz = 0;
-    for (int l = m; l <= L; ++l) {
-      if (l == m + 1) {
-        p = q; px = qx; py = qy; pz = qz;
-        q = (float)(2 * m + 1) * uz;
-        qx = 0; qy = 0; qz = (float)(2 * m + 1);
-      } else if (l > m + 1) {
-        const float aa = (float)(2 * l - 1), bb = (float)(l + m - 1), inv = 1.f / (float)(l - m);
-        const float nq = (aa * uz * q - bb * p) * inv;
-        if (GRAD) {
-          const float nqx = (aa * uz * qx - bb * (2.f * ux * p + px)) * inv;
-          const float nqy = (aa * uz * qy - bb * (2.f * uy * p + py)) * inv;
-          const float nqz = (aa * (q + uz * qz) - bb * (2.f * uz * p + pz)) * inv;
-          px = qx; py = qy; pz = qz;
-          qx = nqx; qy = nqy; qz = nqz;
-        }
-        p = q;
-        q = nq;
-      }
-      const int base = l * l + l;
-      const float N = ynorm[base + m];
-      if (m == 0) {
-        f(l, base, N * q, N * qx, N * qy, N * qz);
-      } else {
-        f(l, base + m, N * q * c, N * (qx * c + q * cx), N * (qy * c + q * cy), N * qz * c);
-        f(l, base - m, N * q * s, N * (qx * s + q * sx), N * (qy * s + q * sy), N * qz * s);
-      }
-    }
-  }
-}
-
 struct Shared {
   float* ynorm;   // [NLM]
   SpecLite* sp;   // [S]
@@ -144,22 +111,6 @@ __device__ void load_shared(const SoapParams& P, const F32Args& a, float* base, 
 
 __host__ __device__ inline int shared_floats(int NLM, int S) { return ((NLM + 3) & ~3) + S * (int)(sizeof(SpecLite) / 4); }
 
-// Per-wave LDS (floats): C [N NLM] | H [N NLM] (adjoint) | U
-//   U: W [CH NLM] + Rf [CH NMAXF]  /  X [D]  /  Xa, Xb [CH NLM] + R, dR [CH NMAXF]
-__host__ __device__ inline int u_floats(const SoapParams& P, bool adj) {
-  int u = CH * P.NLM + CH * NMAXF;
-  if (P.D > u) u = P.D;
-  if (adj) {
-    // the pair chunk overlays C and H (both dead by then) and U
-    const int pc = 2 * CH * P.NLM + 2 * CH * NMAXF - 2 * P.N * P.NLM;
-    if (pc > u) u = pc;
-  }
-  return u;
-}
-__host__ __device__ inline int wave_floats(const SoapParams& P, bool adj) {
-  return (adj ? 2 : 1) * P.N * P.NLM + u_floats(P, adj);
-}
-
 // Damping A_l = exp(-l(l+1) k) and -dA_l/dr... as a table over l.
 __device__ __forceinline__ void damping(const SpecLite& sp, float r, float (&A)[TGX_MAX_L + 1],
                                         float (&dA)[TGX_MAX_L + 1]) {
@@ -175,88 +126,6 @@ __device__ __forceinline__ void damping(const SpecLite& sp, float r, float (&A)[
     A[l] = Al;
     dA[l] = Al * (float)(l * (l + 1)) * dk;
   }
-}
-
-// c(n, lm) of the site into C.
-__device__ void coefficients(const SoapParams& P, const F32Args& a, const Shared& sh, int64_t b0, int nn, float* C,
-                             float* W, float* Rf) {
-  const int lane = threadIdx.x & 63;
-  const int NLM = P.NLM, N = P.N, L = P.L;
-  const int j = lane / LP, part = lane - j * LP;
-  float acc[2][NMAXF];
-#pragma unroll
-  for (int h = 0; h < 2; ++h)
-#pragma unroll
-    for (int n = 0; n < NMAXF; ++n) acc[h][n] = 0.f;
-  {
-    const int s = a.pr_spec[b0];
-    if (s >= 0 && lane == 0) {
-#pragma unroll
-      for (int n = 0; n < NMAXF; ++n) {
-        const int nl = n - sh.sp[s].off;
-        if (nl >= 0 && nl < sh.sp[s].nb) acc[0][n] = (float)P.cen[s * TGX_MAX_NB + nl];
-      }
-    }
-  }
-  for (int k0 = 1; k0 < nn; k0 += CH) {
-    const int cnt = min(CH, nn - k0);
-    wave_sync();
-    if (j < cnt) {
-      const int64_t p = b0 + k0 + j;
-      int s = a.pr_spec[p];
-      const float r = (float)a.pr_r[p];
-      if (s >= 0 && !(a.pr_r[p] < (double)P.sp[s].rc)) s = -1;
-      float* w = W + j * NLM;
-      if (s >= 0) {
-        const SpecLite sp = sh.sp[s];
-        const float inv = 1.f / r;
-        const float ux = (float)a.pr_xyz[3 * p] * inv, uy = (float)a.pr_xyz[3 * p + 1] * inv,
-                    uz = (float)a.pr_xyz[3 * p + 2] * inv;
-        if (part == 0) {
-          float R[TGX_MAX_NB], dR[TGX_MAX_NB];
-          radial_tab<false>(a, sp, s, r, R, dR);
-          float* rf = Rf + j * NMAXF;
-#pragma unroll
-          for (int n = 0; n < NMAXF; ++n) rf[n] = 0.f;
-#pragma unroll
-          for (int n = 0; n < TGX_MAX_NB; ++n)
-            if (n < sp.nb) rf[sp.off + n] = R[n];
-        }
-        float A[TGX_MAX_L + 1], dA[TGX_MAX_L + 1];
-        damping(sp, r, A, dA);
-        ylm_part<false>(L, part, sh.ynorm, ux, uy, uz, [&](int l, int lm, float Y, float, float, float) { w[lm] = A[l] * Y; });
-      } else {
-        if (part == 0) {
-          float* rf = Rf + j * NMAXF;
-#pragma unroll
-          for (int n = 0; n < NMAXF; ++n) rf[n] = 0.f;
-        }
-        for (int lm = part; lm < NLM; lm += LP) w[lm] = 0.f;
-      }
-    }
-    wave_sync();
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int lm = lane + 64 * h;
-      if (lm < NLM)
-        for (int jj = 0; jj < cnt; ++jj) {
-          const float wv = W[jj * NLM + lm];
-          const float* rf = Rf + jj * NMAXF;
-#pragma unroll
-          for (int n = 0; n < NMAXF; ++n) acc[h][n] += rf[n] * wv;
-        }
-    }
-  }
-  wave_sync();
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int lm = lane + 64 * h;
-    if (lm < NLM)
-#pragma unroll
-      for (int n = 0; n < NMAXF; ++n)
-        if (n < N) C[n * NLM + lm] = acc[h][n];
-  }
-  wave_sync();
 }
 
 __device__ float power(const SoapParams& P, const float* C, float* X) {
@@ -286,186 +155,328 @@ __device__ float power(const SoapParams& P, const float* C, float* X) {
   return nrm;
 }
 
-__global__ __launch_bounds__(64 * WAVES) void k_desc_f32(SoapParams P, F32Args a, int n_sites, float* __restrict__ q_out,
-                                                         float* __restrict__ pnorm_out) {
+// All damped harmonics of one direction: w[lm] = A[l] Y_lm(u), the recurrences
+// of tgx_sph_host.h at r = 1.
+__device__ __forceinline__ void harmonics_all(int L, const float* ynorm, float ux, float uy, float uz,
+                                              const float (&A)[TGX_MAX_L + 1], float* w) {
+  float c = 1, s = 0;
+  for (int m = 0; m <= L; ++m) {
+    if (m > 0) {
+      const float nc = c * ux - s * uy, ns = c * uy + s * ux;
+      c = nc;
+      s = ns;
+    }
+    float q = 1, p = 0;
+    for (int l = m; l <= L; ++l) {
+      if (l == m + 1) {
+        p = q;
+        q = (float)(2 * m + 1) * uz;
+      } else if (l > m + 1) {
+        const float nq = ((float)(2 * l - 1) * uz * q - (float)(l + m - 1) * p) / (float)(l - m);
+        p = q;
+        q = nq;
+      }
+      const int base = l * l + l;
+      const float v = A[l] * ynorm[base + m] * q;
+      if (m == 0) {
+        w[base] = v;
+      } else {
+        w[base + m] = v * c;
+        w[base - m] = v * s;
+      }
+    }
+  }
+}
+
+constexpr int PB = 64;  // pairs (threads) per k_pair_basis workgroup
+
+__global__ __launch_bounds__(PB) void k_pair_basis(SoapParams P, F32Args a, int64_t n_pairs) {
   extern __shared__ __align__(16) float smf[];
   Shared sh;
   load_shared(P, a, smf, sh);
+  const int LS = a.ldb + 1;  // odd row pitch: the per-thread row writes hit distinct banks
+  float* st = smf + shared_floats(P.NLM, P.n_species);
+  const int t = threadIdx.x;
+  float* row = st + t * LS;
+  for (int i = 0; i < a.ldb; ++i) row[i] = 0.f;
   __syncthreads();
+  const int64_t p0 = (int64_t)blockIdx.x * PB;
+  const int64_t p = p0 + t;
+  if (p < n_pairs) {
+    int s = a.pr_spec[p];
+    const double rd = a.pr_r[p];
+    // the self pair (r = 0) and neighbours at or beyond their species' cutoff carry nothing
+    if (s >= 0 && !(rd > 0.0 && rd < P.sp[s].rc)) s = -1;
+    if (s >= 0) {
+      const SpecLite sp = sh.sp[s];
+      const float r = (float)rd, inv = 1.f / r;
+      const float ux = (float)a.pr_xyz[3 * p] * inv, uy = (float)a.pr_xyz[3 * p + 1] * inv,
+                  uz = (float)a.pr_xyz[3 * p + 2] * inv;
+      float R[TGX_MAX_NB], dR[TGX_MAX_NB];
+      radial_tab<true>(a, sp, s, r, R, dR);
+#pragma unroll
+      for (int n = 0; n < TGX_MAX_NB; ++n)
+        if (n < sp.nb) {
+          row[a.roff + sp.off + n] = R[n];
+          row[a.roff + P.N + sp.off + n] = dR[n];
+        }
+      float A[TGX_MAX_L + 1], dA[TGX_MAX_L + 1];
+      damping(sp, r, A, dA);
+      harmonics_all(P.L, sh.ynorm, ux, uy, uz, A, row);
+    }
+  }
+  __syncthreads();
+  const int64_t left = n_pairs - p0;
+  const int nrows = left < PB ? (int)left : PB;
+  float* dst = a.basis + p0 * a.ldb;
+  for (int rr = 0; rr < nrows; ++rr)
+    for (int cc = t; cc < a.ldb; cc += PB) dst[(size_t)rr * a.ldb + cc] = st[rr * LS + cc];
+}
+
+// c(n, lm) of a site in registers, lanes over lm (lm = lane and lane + 64).
+__device__ __forceinline__ void site_coefficients(const SoapParams& P, const F32Args& a, int64_t b0, int nn, int lane,
+                                                  float (&acc)[2][NMAXF]) {
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int n = 0; n < NMAXF; ++n) acc[h][n] = 0.f;
+  {
+    const int s = a.pr_spec[b0];
+    if (s >= 0 && lane == 0) {
+#pragma unroll
+      for (int n = 0; n < NMAXF; ++n) {
+        const int nl = n - P.sp[s].off;
+        if (nl >= 0 && nl < P.sp[s].nb) acc[0][n] = (float)P.cen[s * TGX_MAX_NB + nl];
+      }
+    }
+  }
+  const bool v0 = lane < P.NLM, v1 = lane + 64 < P.NLM;
+  const float* row = a.basis + (b0 + 1) * a.ldb;
+#pragma unroll 2
+  for (int j = 1; j < nn; ++j, row += a.ldb) {
+    const float w0 = v0 ? row[lane] : 0.f;
+    const float w1 = v1 ? row[lane + 64] : 0.f;
+#pragma unroll
+    for (int n = 0; n < NMAXF; ++n) {
+      const float rn = n < P.N ? row[a.roff + n] : 0.f;
+      acc[0][n] += rn * w0;
+      acc[1][n] += rn * w1;
+    }
+  }
+}
+
+__host__ __device__ inline int desc_wave_floats(const SoapParams& P) { return P.N * P.NLM + P.D; }
+__host__ __device__ inline int adj_wave_floats(const SoapParams& P) { return P.D + 2 * 4 * ((P.NLM + 3) & ~3); }
+
+__global__ __launch_bounds__(64 * WAVES) void k_site_desc(SoapParams P, F32Args a, int n_sites,
+                                                          float* __restrict__ q_out, float* __restrict__ pnorm_out) {
+  extern __shared__ __align__(16) float smf[];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int site = blockIdx.x * WAVES + w;
+  const int site = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES + w);
   if (site >= n_sites) return;
-  float* C = smf + shared_floats(P.NLM, P.n_species) + w * wave_floats(P, false);
-  float* U = C + P.N * P.NLM;
+  float* C = smf + w * desc_wave_floats(P);
+  float* X = C + P.N * P.NLM;
   const int64_t b0 = a.site_beg[site];
   const int nn = (int)(a.site_beg[site + 1] - b0);
-  coefficients(P, a, sh, b0, nn, C, U, U + CH * P.NLM);
-  const float nrm = power(P, C, U);
+  float acc[2][NMAXF];
+  site_coefficients(P, a, b0, nn, lane, acc);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int lm = lane + 64 * h;
+    if (lm < P.NLM)
+#pragma unroll
+      for (int n = 0; n < NMAXF; ++n)
+        if (n < P.N) C[n * P.NLM + lm] = acc[h][n];
+  }
+  wave_sync();
+  const float nrm = power(P, C, X);
   float* qrow = q_out + (size_t)site * P.Dpad;
-  for (int d = lane; d < P.Dpad; d += 64) qrow[d] = d < P.D ? U[d] : 0.f;
+  for (int d = lane; d < P.Dpad; d += 64) qrow[d] = d < P.D ? X[d] : 0.f;
   if (lane == 0) pnorm_out[site] = nrm;
 }
 
-__global__ __launch_bounds__(64 * WAVES) void k_adjoint_f32(SoapParams P, F32Args a, int n_sites,
-                                                            const float* __restrict__ q_in,
-                                                            const float* __restrict__ pnorm_in,
-                                                            const float* __restrict__ qss, double* __restrict__ fglob) {
+// Sum over the wavefront of a value, available in every lane: DPP row shifts
+// build each 16-lane row's sum in the row's last lane, and the four row sums
+// are read out through scalar registers.
+__device__ __forceinline__ float wave_total(float v) {
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x111, 0xf, 0xf, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x112, 0xf, 0xf, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x114, 0xf, 0xf, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x118, 0xf, 0xf, false));
+  const int b = __builtin_bit_cast(int, v);
+  return (__builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 15)) +
+          __builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 31))) +
+         (__builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 47)) +
+          __builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 63)));
+}
+
+constexpr int PCH = 4;  // pairs per chunk of the adjoint's pair loop
+
+__global__ __launch_bounds__(64 * WAVES) void k_site_adjoint(SoapParams P, F32Args a, int n_sites,
+                                                             const int* __restrict__ low_idx,
+                                                             const float* __restrict__ low_coef,
+                                                             const float* __restrict__ q_in,
+                                                             const float* __restrict__ pnorm_in,
+                                                             const float* __restrict__ qss, double* __restrict__ fglob) {
   extern __shared__ __align__(16) float smf[];
-  Shared sh;
-  load_shared(P, a, smf, sh);
-  __syncthreads();
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int site = blockIdx.x * WAVES + w;
+  const int site = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES + w);
   if (site >= n_sites) return;
-  const int NLM = P.NLM, N = P.N, L = P.L, D = P.D, CN = N * NLM;
-  float* C = smf + shared_floats(NLM, P.n_species) + w * wave_floats(P, true);
-  float* H = C + CN;
-  float* U = H + CN;
+  const int NLM = P.NLM, N = P.N, D = P.D, L1 = P.L + 1;
+  const int NLMp = (NLM + 3) & ~3;
+  float* X = smf + w * adj_wave_floats(P);
+  float* Wbuf = X + D;  // [2][PCH][NLMp]
   const int64_t b0 = a.site_beg[site];
   const int nn = (int)(a.site_beg[site + 1] - b0);
-  coefficients(P, a, sh, b0, nn, C, U, U + CH * NLM);
-  float* X = U;
+  float hreg[2][NMAXF];
+  site_coefficients(P, a, b0, nn, lane, hreg);
+  // g = (Qss - (Qss . q) q) / |p|
   const float* qrow = q_in + (size_t)site * P.Dpad;
   const float* srow = qss + (size_t)site * P.Dpad;
-  float acc = 0.f;
-  for (int d = lane; d < D; d += 64) acc += srow[d] * qrow[d];
-  const float sq = wsum(acc);
+  float sacc = 0.f;
+  for (int d = lane; d < D; d += 64) sacc += srow[d] * qrow[d];
+  const float sq = wave_total(sacc);
   const float nrm = pnorm_in[site];
   const float inv_n = nrm > 0.f ? 1.f / nrm : 0.f;
   for (int d = lane; d < D; d += 64) X[d] = (srow[d] - sq * qrow[d]) * inv_n;
   wave_sync();
-  const int L1 = L + 1;
+  // H(n, lm) = sum_n' w(n, n') g(n, n', l) c(n', lm), w = 2 on the diagonal and
+  // sqrt(2) off it; each stored g(n <= n') serves both orders.
+  int lh[2];
+  bool vh[2];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int lm = lane + 64 * h;
-    if (lm < NLM) {
-      int l = (int)sqrtf((float)lm + 0.5f);
-      if (l * l > lm) --l;
-      float cc[NMAXF];
+    vh[h] = lm < NLM;
+    int l = (int)sqrtf((float)lm + 0.5f);
+    if (l * l > lm) --l;
+    lh[h] = l;
+    float cc[NMAXF], hs[NMAXF];
 #pragma unroll
-      for (int n = 0; n < NMAXF; ++n) cc[n] = n < N ? C[n * NLM + lm] : 0.f;
+    for (int n = 0; n < NMAXF; ++n) {
+      cc[n] = hreg[h][n];
+      hs[n] = 0.f;
+    }
+    if (vh[h]) {
+      int pidx = 0;
 #pragma unroll
       for (int n = 0; n < NMAXF; ++n) {
-        if (n >= N) break;
-        float hs = 0.f;
+        if (n < N) {
+          hs[n] += 2.f * X[pidx * L1 + l] * cc[n];
+          ++pidx;
 #pragma unroll
-        for (int np = 0; np < NMAXF; ++np) {
-          if (np >= N) break;
-          const int lo = n < np ? n : np, hi = n < np ? np : n;
-          const int pidx = lo * N - lo * (lo - 1) / 2 + (hi - lo);
-          hs += (n == np ? 2.f : (float)M_SQRT2) * X[pidx * L1 + l] * cc[np];
-        }
-        H[n * NLM + lm] = hs;
-      }
-    }
-  }
-  wave_sync();
-  float hreg[2][NMAXF];
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int lm = lane + 64 * h;
-#pragma unroll
-    for (int n = 0; n < NMAXF; ++n) hreg[h][n] = (lm < NLM && n < N) ? H[n * NLM + lm] : 0.f;
-  }
-  float* Xa = C;
-  float* Xb = Xa + CH * NLM;
-  float* Rr = Xb + CH * NLM;
-  float* dRr = Rr + CH * NMAXF;
-  const int j = lane / LP, part = lane - j * LP;
-  double s0 = 0, s1 = 0, s2 = 0;
-  for (int k0 = 1; k0 < nn; k0 += CH) {
-    const int cnt = min(CH, nn - k0);
-    int s = -1;
-    float r = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
-    int64_t p = 0;
-    wave_sync();
-    if (j < cnt) {
-      p = b0 + k0 + j;
-      s = a.pr_spec[p];
-      if (s >= 0 && !(a.pr_r[p] < (double)P.sp[s].rc)) s = -1;
-      if (part == 0) {
-        float* rr_ = Rr + j * NMAXF;
-        float* dr_ = dRr + j * NMAXF;
-#pragma unroll
-        for (int n = 0; n < NMAXF; ++n) { rr_[n] = 0.f; dr_[n] = 0.f; }
-      }
-      if (s >= 0) {
-        r = (float)a.pr_r[p];
-        const float inv = 1.f / r;
-        ux = (float)a.pr_xyz[3 * p] * inv; uy = (float)a.pr_xyz[3 * p + 1] * inv; uz = (float)a.pr_xyz[3 * p + 2] * inv;
-        if (part == 0) {
-          const SpecLite sp = sh.sp[s];
-          float R[TGX_MAX_NB], dR[TGX_MAX_NB];
-          radial_tab<true>(a, sp, s, r, R, dR);
-          float* rr_ = Rr + j * NMAXF;
-          float* dr_ = dRr + j * NMAXF;
-#pragma unroll
-          for (int n = 0; n < TGX_MAX_NB; ++n)
-            if (n < sp.nb) { rr_[sp.off + n] = R[n]; dr_[sp.off + n] = dR[n]; }
-        }
-      }
-    }
-    wave_sync();
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int lm = lane + 64 * h;
-      if (lm < NLM)
-        for (int jj = 0; jj < cnt; ++jj) {
-          const float* rj = Rr + jj * NMAXF;
-          const float* dj = dRr + jj * NMAXF;
-          float xa = 0.f, xb = 0.f;
-#pragma unroll
-          for (int n = 0; n < NMAXF; ++n) {
-            xa += rj[n] * hreg[h][n];
-            xb += dj[n] * hreg[h][n];
+          for (int np = n + 1; np < NMAXF; ++np) {
+            if (np < N) {
+              const float g = (float)M_SQRT2 * X[pidx * L1 + l];
+              hs[n] += g * cc[np];
+              hs[np] += g * cc[n];
+              ++pidx;
+            }
           }
-          Xa[jj * NLM + lm] = xa;
-          Xb[jj * NLM + lm] = xb;
         }
+      }
     }
-    wave_sync();
-    float f0 = 0.f, f1 = 0.f, f2 = 0.f;
-    if (j < cnt && s >= 0) {
-      const SpecLite sp = sh.sp[s];
-      const float inv_r = 1.f / r;
-      float A[TGX_MAX_L + 1], dA[TGX_MAX_L + 1];
-      damping(sp, r, A, dA);
-      const float* xa = Xa + j * NLM;
-      const float* xb = Xb + j * NLM;
-      ylm_part<true>(L, part, sh.ynorm, ux, uy, uz, [&](int l, int lm, float Y, float gx, float gy, float gz) {
-        const float x = xa[lm], y = xb[lm];
-        const float rad = dA[l] * x + A[l] * y;
-        const float ang = A[l] * x * inv_r;
-        const float lf = (float)l;
-        f0 += rad * Y * ux + ang * (gx - lf * Y * ux);
-        f1 += rad * Y * uy + ang * (gy - lf * Y * uy);
-        f2 += rad * Y * uz + ang * (gz - lf * Y * uz);
-      });
-    }
-    // combine the LP lanes of each pair (lanes j*LP .. j*LP+2)
-    const float g0 = f0 + __shfl(f0, lane + 1) + __shfl(f0, lane + 2);
-    const float g1 = f1 + __shfl(f1, lane + 1) + __shfl(f1, lane + 2);
-    const float g2 = f2 + __shfl(f2, lane + 1) + __shfl(f2, lane + 2);
-    if (j < cnt && part == 0) {
-      const int64_t gi = a.pr_gidx[p];
-      fglob[3 * gi] += (double)g0;
-      fglob[3 * gi + 1] += (double)g1;
-      fglob[3 * gi + 2] += (double)g2;
-      s0 += g0;
-      s1 += g1;
-      s2 += g2;
-    }
+#pragma unroll
+    for (int n = 0; n < NMAXF; ++n) hreg[h][n] = hs[n];
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    s0 += __shfl_xor(s0, o);
-    s1 += __shfl_xor(s1, o);
-    s2 += __shfl_xor(s2, o);
+  // lowering table entries of this lane's lm
+  int li[2][3][KLOW];
+  float lc[2][3][KLOW];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int t = 0; t < KLOW; ++t) {
+        const int k = ((lane + 64 * h) * 3 + c) * KLOW + t;
+        li[h][c][t] = vh[h] ? low_idx[k] : -1;
+        lc[h][c][t] = vh[h] ? low_coef[k] : 0.f;
+      }
+  double f_self0 = 0.0, f_self1 = 0.0, f_self2 = 0.0;
+  int buf = 0;
+  for (int j0 = 1; j0 < nn; j0 += PCH, buf ^= 1) {
+    float* Wb = Wbuf + buf * PCH * NLMp;
+    float Wv[PCH][2], xa[PCH][2], xb[PCH][2];
+    bool ok[PCH];
+#pragma unroll
+    for (int u = 0; u < PCH; ++u) {
+      const int j = j0 + u;
+      const int64_t p = b0 + j;
+      ok[u] = false;
+      Wv[u][0] = Wv[u][1] = xa[u][0] = xa[u][1] = xb[u][0] = xb[u][1] = 0.f;
+      if (j < nn) {
+        const int s = a.pr_spec[p];
+        ok[u] = s >= 0 && a.pr_r[p] < P.sp[s].rc;  // uniform across the wavefront
+      }
+      if (ok[u]) {
+        const float* row = a.basis + p * a.ldb;
+        Wv[u][0] = vh[0] ? row[lane] : 0.f;
+        Wv[u][1] = vh[1] ? row[lane + 64] : 0.f;
+#pragma unroll
+        for (int n = 0; n < NMAXF; ++n) {
+          if (n < N) {
+            const float rn = row[a.roff + n], dn = row[a.roff + N + n];
+            xa[u][0] += rn * hreg[0][n];
+            xa[u][1] += rn * hreg[1][n];
+            xb[u][0] += dn * hreg[0][n];
+            xb[u][1] += dn * hreg[1][n];
+          }
+        }
+      }
+      if (vh[0]) Wb[u * NLMp + lane] = Wv[u][0];
+      if (vh[1]) Wb[u * NLMp + lane + 64] = Wv[u][1];
+    }
+    wave_sync();  // the other buffer is written next, so one wait per chunk suffices
+#pragma unroll
+    for (int u = 0; u < PCH; ++u) {
+      if (!ok[u]) continue;
+      const int64_t p = b0 + j0 + u;
+      const float r = (float)a.pr_r[p], inv_r = 1.f / r;
+      const int s = a.pr_spec[p];
+      const float sgt = (float)P.sp[s].sig_t, sgs = (float)P.sp[s].sig_t_sc;
+      const float stv = sgt * (1.f + sgs * r), dst = sgt * sgs;
+      const float kk = stv * stv * 0.5f * inv_r * inv_r;
+      const float dk = stv * stv * inv_r * inv_r * inv_r - stv * dst * inv_r * inv_r;
+      const float* Wu = Wb + u * NLMp;
+      float T = 0.f, V0 = 0.f, V1 = 0.f, V2 = 0.f;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const float lf = (float)lh[h];
+        T += (lf * (lf + 1.f) * dk * xa[u][h] + xb[u][h] - lf * xa[u][h] * inv_r) * Wv[u][h];
+        const float xr = xa[u][h] * __expf(-2.f * lf * kk);
+        float g[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int t = 0; t < KLOW; ++t)
+            if (li[h][c][t] >= 0) g[c] += lc[h][c][t] * Wu[li[h][c][t]];
+        V0 += xr * g[0];
+        V1 += xr * g[1];
+        V2 += xr * g[2];
+      }
+      T = wave_total(T);
+      V0 = wave_total(V0);
+      V1 = wave_total(V1);
+      V2 = wave_total(V2);
+      if (lane == 0) {
+        const float ux = (float)a.pr_xyz[3 * p] * inv_r, uy = (float)a.pr_xyz[3 * p + 1] * inv_r,
+                    uz = (float)a.pr_xyz[3 * p + 2] * inv_r;
+        const float f0 = ux * T + V0 * inv_r, f1 = uy * T + V1 * inv_r, f2 = uz * T + V2 * inv_r;
+        const int64_t gi = a.pr_gidx[p];
+        fglob[3 * gi] += (double)f0;
+        fglob[3 * gi + 1] += (double)f1;
+        fglob[3 * gi + 2] += (double)f2;
+        f_self0 += f0;
+        f_self1 += f1;
+        f_self2 += f2;
+      }
+    }
   }
   if (lane == 0) {
     const int64_t gi = a.pr_gidx[b0];
-    fglob[3 * gi] -= s0;
-    fglob[3 * gi + 1] -= s1;
-    fglob[3 * gi + 2] -= s2;
+    fglob[3 * gi] -= f_self0;
+    fglob[3 * gi + 1] -= f_self1;
+    fglob[3 * gi + 2] -= f_self2;
   }
 }
 
@@ -526,6 +537,19 @@ int tgx_radial_table_build(tgx_ctx* ctx, tgx_model_t* m) {
   TGX_TRY(m->rtab.ensure(sizeof(float) * (size_t)m->n_species * KTAB * 2 * TGX_MAX_NB));
   const int n = m->n_species * KTAB;
   TGX_LAUNCH(ctx, "radial_table", k_radial_table, dim3((n + 127) / 128), dim3(128), 0, m->P, KTAB, m->rtab.as<float>());
+  // lowering table of the solid harmonics (host, FP64), read by the FP32 adjoint
+  std::vector<double> yn((size_t)m->P.NLM);
+  TGX_CHECK_HIP(hipMemcpy(yn.data(), m->P.ynorm, sizeof(double) * yn.size(), hipMemcpyDeviceToHost));
+  std::vector<int> li;
+  std::vector<float> lc;
+  double resid = 0.0;
+  TGX_REQUIRE(tgx_lowering_table(m->l_max, yn.data(), KLOW, li, lc, &resid),
+              "lowering table of the solid harmonics failed (l_max %d, residual %g)", m->l_max, resid);
+  TGX_TRY(m->low_idx.ensure(sizeof(int) * li.size()));
+  TGX_TRY(m->low_coef.ensure(sizeof(float) * lc.size()));
+  TGX_CHECK_HIP(hipMemcpyAsync(m->low_idx.p, li.data(), sizeof(int) * li.size(), hipMemcpyHostToDevice, ctx->stream));
+  TGX_CHECK_HIP(hipMemcpyAsync(m->low_coef.p, lc.data(), sizeof(float) * lc.size(), hipMemcpyHostToDevice, ctx->stream));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   return TGX_OK;
 }
 
@@ -538,31 +562,46 @@ static F32Args f32_args(tgx_model_t* m) {
   a.pr_gidx = m->pr_gidx.as<int64_t>();
   a.rtab = m->rtab.as<float>();
   a.K = KTAB;
+  a.roff = (m->P.NLM + 3) & ~3;
+  a.ldb = a.roff + 2 * m->P.N;
+  a.basis = m->pbasis.as<float>();
   return a;
 }
 
-static size_t f32_lds(const SoapParams& P, bool adj) {
-  return sizeof(float) * ((size_t)shared_floats(P.NLM, P.n_species) + (size_t)WAVES * wave_floats(P, adj)) + 64;
-}
-
+// Pair basis of the current block, then its descriptors.
 int tgx_desc_f32(tgx_ctx* ctx, tgx_model_t* m) {
   const int ns = m->n_sites_b;
   if (ns == 0) return TGX_OK;
   TGX_REQUIRE(m->P.N <= NMAXF, "FP32 descriptor kernel holds at most %d radial functions", NMAXF);
-  const size_t lds = f32_lds(m->P, false);
+  const int64_t np = m->n_pairs_b;
+  F32Args a = f32_args(m);
+  TGX_TRY(m->pbasis.ensure(sizeof(float) * (size_t)(np > 0 ? np : 1) * a.ldb));
+  a.basis = m->pbasis.as<float>();
+  if (np > 0) {
+    const size_t lds = sizeof(float) * ((size_t)shared_floats(m->P.NLM, m->P.n_species) + (size_t)PB * (a.ldb + 1));
+    TGX_REQUIRE(lds <= 64 * 1024, "pair basis needs %zu bytes of LDS", lds);
+    TGX_LAUNCH(ctx, "soap_pair_basis", k_pair_basis, dim3((unsigned)((np + PB - 1) / PB)), dim3(PB), lds, m->P, a, np);
+  }
+  const size_t lds = sizeof(float) * (size_t)WAVES * desc_wave_floats(m->P);
   TGX_REQUIRE(lds <= 160 * 1024, "FP32 descriptor kernel needs %zu bytes of LDS", lds);
-  TGX_LAUNCH(ctx, "soap_desc_f32", k_desc_f32, dim3((ns + WAVES - 1) / WAVES), dim3(64 * WAVES), lds, m->P, f32_args(m),
-             ns, m->q32.as<float>(), m->pnorm.as<float>());
+  TGX_LAUNCH(ctx, "soap_desc_f32", k_site_desc, dim3((ns + WAVES - 1) / WAVES), dim3(64 * WAVES), lds, m->P, a, ns,
+             m->q32.as<float>(), m->pnorm.as<float>());
   return TGX_OK;
 }
 
+// Pair forces of the current block from its FP32 Qss, reading the pair basis
+// that tgx_desc_f32 left for the block.
 int tgx_adjoint_f32(tgx_ctx* ctx, tgx_model_t* m) {
   const int ns = m->n_sites_b;
   if (ns == 0) return TGX_OK;
   TGX_REQUIRE(m->P.N <= NMAXF, "FP32 adjoint kernel holds at most %d radial functions", NMAXF);
-  const size_t lds = f32_lds(m->P, true);
+  F32Args a = f32_args(m);
+  TGX_REQUIRE(m->pbasis.p && m->pbasis.bytes >= sizeof(float) * (size_t)m->n_pairs_b * a.ldb,
+              "FP32 adjoint: the pair basis of the block was not built");
+  const size_t lds = sizeof(float) * (size_t)WAVES * adj_wave_floats(m->P);
   TGX_REQUIRE(lds <= 160 * 1024, "FP32 adjoint kernel needs %zu bytes of LDS", lds);
-  TGX_LAUNCH(ctx, "soap_adjoint_f32", k_adjoint_f32, dim3((ns + WAVES - 1) / WAVES), dim3(64 * WAVES), lds, m->P,
-             f32_args(m), ns, m->q32.as<float>(), m->pnorm.as<float>(), m->qss32.as<float>(), ctx->fglob.as<double>());
+  TGX_LAUNCH(ctx, "soap_adjoint_f32", k_site_adjoint, dim3((ns + WAVES - 1) / WAVES), dim3(64 * WAVES), lds, m->P, a,
+             ns, m->low_idx.as<int>(), m->low_coef.as<float>(), m->q32.as<float>(), m->pnorm.as<float>(),
+             m->qss32.as<float>(), ctx->fglob.as<double>());
   return TGX_OK;
 }

@@ -1,4 +1,4 @@
-"""Seeded synthetic structures and models for the benchmark configurations.
+"""Seeded synthetic structures and models for the benchmark and test workloads.
 
 Every random draw comes from libtgx's host generators (xoshiro256** seeded by
 splitmix64), so the same seeds give the same structures on any machine; no GPU
@@ -6,10 +6,10 @@ is needed to generate them.
 
   * ``rsa``: random sequential addition in a periodic orthorhombic box with a
     minimum pair distance.
-  * ``config0_system``: 216 C atoms, L = 12.91 Å, d_min = 1.2 Å.
-  * ``config1_system``: 65,536 atoms, L = 86.8 Å, species A/B 50:50 by a seeded
+  * ``carbon216_system``: 216 C atoms, L = 12.91 Å, d_min = 1.2 Å.
+  * ``binary_system``: 65,536 atoms, L = 86.8 Å, species A/B 50:50 by a seeded
     permutation; ``subset_system`` is the 4,096-atom box at the same density.
-  * ``soap_config0`` / ``soap_config1``: poly3gauss, n_max 8 per species,
+  * ``soap_carbon`` / ``soap_binary``: poly3gauss, n_max 8 per species,
     l_max 8, rcut 3.7 Å, buffer 0.5 Å (rcut_soft 3.2 Å), sigma_r = sigma_t =
     0.5 Å, central_weight 1, zeta 4, delta 1 eV.
   * ``sparse_set``: sparse descriptors are the descriptors of the first
@@ -67,7 +67,7 @@ def cubic_box(Lc: float) -> np.ndarray:
     return np.diag([Lc, Lc, Lc]).astype(np.float64)
 
 
-def config0_system(seed: int = 1001) -> System:
+def carbon216_system(seed: int = 1001) -> System:
     Lc = 12.91
     return System(rsa(216, Lc, 1.2, seed), np.full(216, CARBON, np.int32), cubic_box(Lc))
 
@@ -80,15 +80,15 @@ def two_species_system(n: int, Lc: float, seed: int) -> System:
     return System(pos, sp, cubic_box(Lc))
 
 
-def config1_system(n: int = 65536, seed: int = 2001) -> System:
+def binary_system(n: int = 65536, seed: int = 2001) -> System:
     # L = 86.8 Å at 65,536 atoms; other sizes keep the density
     Lc = 86.8 * (n / 65536.0) ** (1.0 / 3.0)
     return two_species_system(n, Lc, seed)
 
 
 def subset_system(seed: int = 2002) -> System:
-    """The 4,096-atom box at config 1's density (L = 86.8 (1/16)^(1/3) Å)."""
-    return config1_system(4096, seed)
+    """The 4,096-atom box at the density of the 65,536-atom box (L = 86.8 (1/16)^(1/3) Å)."""
+    return binary_system(4096, seed)
 
 
 def soap_hypers(species: List[int], central_species: int = 0, n_max: int = 8, l_max: int = 8) -> SoapTurbo:
@@ -99,11 +99,11 @@ def soap_hypers(species: List[int], central_species: int = 0, n_max: int = 8, l_
         central_species=central_species, zeta=4.0, delta=1.0)
 
 
-def soap_config0() -> SoapTurbo:
+def soap_carbon() -> SoapTurbo:
     return soap_hypers([CARBON], central_species=0)
 
 
-def soap_config1() -> List[SoapTurbo]:
+def soap_binary() -> List[SoapTurbo]:
     return [soap_hypers([SPECIES_A, SPECIES_B], central_species=c) for c in (1, 2)]
 
 
@@ -152,11 +152,11 @@ def gpu_describe(ctx_factory=None):
     return describe
 
 
-def config1_models(describe: DescriptorFn, n_sparse: int = 4000, seed: int = 9001):
-    """Both central-species blocks of config 1 with their sparse sets, taken from
+def binary_models(describe: DescriptorFn, n_sparse: int = 4000, seed: int = 9001):
+    """Both central-species blocks of the two-species box with their sparse sets, taken from
     seeded 16,384-atom structures at the same density."""
     models = []
-    for k, h in enumerate(soap_config1()):
-        Qs, al = sparse_set(h, n_sparse, describe, lambda s: config1_system(16384, seed=s), seed=seed + 100 * k)
+    for k, h in enumerate(soap_binary()):
+        Qs, al = sparse_set(h, n_sparse, describe, lambda s: binary_system(16384, seed=s), seed=seed + 100 * k)
         models.append((h, Qs, al))
     return models
