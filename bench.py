@@ -66,32 +66,64 @@ def run_path(ctx, path, steps, warmup, rcut):
     return {"ms_per_step": 1e3 * dt / steps, "energy": E, "virial": W, "energies": en, "forces": F, "kernels": kt}
 
 
-def roofline_entries(stats, n_atoms, blocks, n_pairs):
-    """Algorithmic work per launch for each subsystem kernel (per-unit figures in
-    the design notes): (a) neighbour list bytes, (b) SOAP bytes, (c) GEMM flops,
-    (d) scatter bytes."""
+def roofline_entries(stats, n_atoms, blocks, n_pairs, members=1, hidden=256):
+    """Algorithmic work per launch (averaged over the launches of a step) of
+    every kernel of a path: HBM bytes for the streaming kernels, MFMA flops for
+    the GEMMs. Per-unit figures:
+      neighbours (count + fill)  24 B/atom read + 40 B/pair written + 4 B/atom
+      soap_site_der_f64          3 D 8 B/pair (dq/dr) + D 8 B/site + 40 B/pair
+      plain_force                3 D 8 B/pair + D 8 B/site + 28 B/pair + 24 B/site
+      soap_pair_basis            36 B/pair read + 4 ldb B/pair written
+      soap_desc_f32              4 ldb B/pair + 4 Dpad B/site
+      soap_adjoint_f32           4 ldb + 36 + 48 B/pair + 8 Dpad B/site
+      gemm_*                     2 n S D flops per GEMM
+      mlp_l1 / mlp_back1         2 n (K H) Dpad flops
+      mlp_l2 / mlp_back2         2 n K H^2 flops
+    Returns {kernel: (bound, achieved, peak_key)}."""
     rows = {}
     P = n_pairs
     Nb = [b["n_sites"] for b in blocks]
     Pb = [b["n_pairs"] for b in blocks]
     D, S = blocks[0]["D"], blocks[0]["S"]
+    Dpad = -(-D // 128) * 128
+    N_rad = blocks[0].get("N", 16)
+    nlm = blocks[0].get("NLM", 81)
+    ldb = -(-nlm // 4) * 4 + 2 * N_rad
+    KH = members * hidden
     k = stats["kernels"]
-    if "nbr_fill" in k:
-        byts = 24 * n_atoms + 56 * P + 4 * n_atoms
-        rows["neighbors(a)"] = ("hbm", byts / 1e9 / (k["nbr_fill"]["avg_ms"] + k.get("nbr_count", {"avg_ms": 0})["avg_ms"]) * 1e3)
-    for name, w in (("soap_site_der_f64", 8), ):
+
+    def hbm(name, byts):
         if name in k:
-            byts = np.mean([p * 3 * D * w + n * D * w + 40 * p for n, p in zip(Nb, Pb)])
-            rows["soap_dqdr(b)"] = ("hbm", byts / 1e9 / k[name]["avg_ms"] * 1e3)
+            rows[name] = ("hbm", byts / 1e9 / k[name]["avg_ms"] * 1e3, "hbm_GBs")
+
+    def mfma(name, flops, peak):
+        if name in k:
+            rows[name] = ("mfma", flops / 1e12 / k[name]["avg_ms"] * 1e3, peak)
+
+    if "nbr_fill" in k:
+        byts = 24 * n_atoms + 40 * P + 4 * n_atoms
+        t = k["nbr_fill"]["avg_ms"] + k.get("nbr_count", {"avg_ms": 0.0})["avg_ms"]
+        rows["nbr_fill"] = ("hbm", byts / 1e9 / t * 1e3, "hbm_GBs")
+    hbm("soap_site_der_f64", np.mean([p * 3 * D * 8 + n * D * 8 + 40 * p for n, p in zip(Nb, Pb)]))
+    hbm("plain_force", np.mean([p * 3 * D * 8 + n * D * 8 + 28 * p + 24 * n for n, p in zip(Nb, Pb)]))
+    hbm("soap_pair_basis", np.mean([p * (36 + 4 * ldb) for p in Pb]))
+    hbm("soap_desc_f32", np.mean([p * 4 * ldb + n * 4 * Dpad for n, p in zip(Nb, Pb)]))
+    hbm("soap_adjoint_f32", np.mean([p * (4 * ldb + 36 + 48) + n * 8 * Dpad for n, p in zip(Nb, Pb)]))
     for name, peak in (("gemm_kernel_f64", "fp64_TF"), ("gemm_qss_f64", "fp64_TF"), ("gemm_kernel_bf16", "bf16_TF"),
                        ("gemm_qss_bf16", "bf16_TF")):
-        if name in k:
-            fl = np.mean([2.0 * n * S * D for n in Nb])
-            rows[name + "(c)"] = ("mfma", fl / 1e12 / k[name]["avg_ms"] * 1e3, peak)
-    if "plain_force" in k:
-        byts = np.mean([p * 3 * D * 8 + n * D * 8 + 28 * p + 24 * n for n, p in zip(Nb, Pb)])
-        rows["scatter(d)"] = ("hbm", byts / 1e9 / k["plain_force"]["avg_ms"] * 1e3)
+        mfma(name, np.mean([2.0 * n * S * D for n in Nb]), peak)
+    mfma("mlp_l1", np.mean([2.0 * n * KH * Dpad for n in Nb]), "bf16_TF")
+    mfma("mlp_back1", np.mean([2.0 * n * KH * Dpad for n in Nb]), "bf16_TF")
+    mfma("mlp_l2", np.mean([2.0 * n * KH * hidden for n in Nb]), "bf16_TF")
+    mfma("mlp_back2", np.mean([2.0 * n * KH * hidden for n in Nb]), "bf16_TF")
     return rows
+
+
+def roofline_object(name, entry, traffic=None):
+    bound, ach, peak_key = entry
+    peak = PEAK[peak_key]
+    return {"bound": bound, "kernel": name, "achieved": round(ach, 3), "peak": peak,
+            "unit": "TFLOP/s" if bound == "mfma" else "GB/s", "frac": round(ach / peak, 4), "traffic": traffic}
 
 
 def prepare_surrogate(ctx, ids, sysm, models, args):
@@ -147,6 +179,59 @@ def prepare_surrogate(ctx, ids, sysm, models, args):
     return info
 
 
+def production_md(ctx, sysm, args):
+    """Surrogate MD on the benchmark box: args.nvt_steps of Berendsen NVT at
+    1500 K then NVE, dt = 1 fs, neighbour list rebuilt every step. Every
+    args.check_every-th step takes its forces from the FP64 GAP (the full-system
+    check) after the surrogate has been evaluated on the same positions, and
+    the surrogate's error there is recorded; gated sites fall back to the GAP on
+    every step. Reports the NVE energy drift, the fallback fraction, the force
+    RMSE on the checked steps and the whole run's atom-steps/s."""
+    from turbogap_amd import surrogate as SG
+    from turbogap_amd.md import MD, maxwell_velocities
+    from turbogap_amd import structures as S
+    mass = S.masses_for(sysm.species)
+    ctx.set_positions(sysm.positions)
+    md = MD(ctx, mass, maxwell_velocities(mass, 1500.0, seed=4444), 3.7, "surrogate")
+    n = sysm.n
+    e_nve, t_nve, gated, checks = [], [], [], []
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.production_steps):
+        T0 = 1500.0 if k < args.nvt_steps else None
+        check = args.check_every > 0 and (k + 1) % args.check_every == 0
+        if check:
+            md.forces("surrogate")
+            gated.append(SG.gate_stats(ctx)[0])
+            es, Fs = ctx.results()
+            E, ek, T = md.step(1.0, T0=T0, tau=100.0, path="fp64_plain")
+            e64, F64 = ctx.results()
+            checks.append({"step": k + 1, "force_rmse_meV_A": 1e3 * float(np.sqrt(np.mean((Fs - F64) ** 2))),
+                           "energy_mae_meV_atom": 1e3 * float(np.mean(np.abs(es - e64)))})
+        else:
+            E, ek, T = md.step(1.0, T0=T0, tau=100.0)
+            gated.append(SG.gate_stats(ctx)[0])
+        if k >= args.nvt_steps:
+            e_nve.append((E + ek) / n)
+            t_nve.append(k * 1e-3)  # ps
+        if (k + 1) % 500 == 0:
+            log(f"production md: step {k + 1}, E_pot={E:.4f} eV, T={T:.1f} K")
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    drift = float(np.polyfit(t_nve, e_nve, 1)[0]) * 1e3 if len(e_nve) > 2 else None
+    ctx.set_positions(sysm.positions)
+    out = {"steps": args.production_steps, "nvt_steps_1500K": args.nvt_steps, "dt_fs": 1.0,
+           "atom_steps_per_s": n * args.production_steps / dt, "wall_s": dt,
+           "energy_drift_meV_atom_ps": drift, "fallback_fraction": float(np.mean(gated)) if gated else None,
+           "check_every": args.check_every,
+           "checked_force_rmse_meV_A": float(np.mean([c["force_rmse_meV_A"] for c in checks])) if checks else None,
+           "checked_energy_mae_meV_atom": float(np.mean([c["energy_mae_meV_atom"] for c in checks])) if checks else None,
+           "checks": checks}
+    log(f"production md: {out['atom_steps_per_s']:.4g} atom-steps/s, drift {drift} meV/atom/ps, "
+        f"fallback {out['fallback_fraction']}, checked force RMSE {out['checked_force_rmse_meV_A']} meV/A")
+    return out
+
+
 def cpu_baseline(models, n_sample, threads):
     """The FP64 oracle on a bounded sample of the workload: a box of n_sample
     atoms at the density of the 65,536-atom box with the same blocks and sparse sets."""
@@ -176,6 +261,9 @@ def main():
     ap.add_argument("--gate", type=float, default=0.020)
     ap.add_argument("--cpu-sample", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--production-steps", type=int, default=2000)
+    ap.add_argument("--nvt-steps", type=int, default=200)
+    ap.add_argument("--check-every", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -203,7 +291,8 @@ def main():
             ns, npb = C.c_int(), C.c_int64()
             ctx._lib.tgx_soap_block_size(ctx.h, C.byref(ns), C.byref(npb))
             ctx.soap_backend_end()
-            blocks.append({"n_sites": ns.value, "n_pairs": npb.value, "D": h.dim, "S": args.sparse})
+            blocks.append({"n_sites": ns.value, "n_pairs": npb.value, "D": h.dim, "S": args.sparse, "N": h.n_max,
+                           "NLM": (h.l_max + 1) ** 2})
         log(f"pairs: {nl['n_atom_pairs']}, blocks: {blocks}")
         surr = None
         if "surrogate" in paths:
@@ -214,6 +303,9 @@ def main():
             log(f"{p}: {r['ms_per_step']:.3f} ms/step, E={r['energy']:.8f}")
             for kname, kv in r["kernels"].items():
                 log(f"   {kname:22s} {kv['launches']:4d} x {kv['avg_ms']:.4f} ms")
+        prod = None
+        if surr is not None and args.production_steps > 0:
+            prod = production_md(ctx, sysm, args)
     ref = res.get("fp64_plain") or res.get("fp64_adjoint")
     acc = {}
     for p, r in res.items():
@@ -230,21 +322,14 @@ def main():
     br = res[best]
     value = sysm.n / (br["ms_per_step"] * 1e-3)
     fp64 = res.get("fp64_plain")
-    rl = roofline_entries(br, sysm.n, blocks, nl["n_atom_pairs"])
-    # dominant kernel of the headline path
-    dom = max(br["kernels"].items(), key=lambda kv: kv[1]["total_ms"])[0]
-    roof = None
-    for key, v in rl.items():
-        if key.startswith(dom):
-            bound, ach = v[0], v[1]
-            peak = PEAK[v[2]] if bound == "mfma" else PEAK["hbm_GBs"]
-            roof = {"bound": bound, "kernel": dom, "achieved": ach, "peak": peak,
-                    "unit": "TFLOP/s" if bound == "mfma" else "GB/s", "frac": ach / peak, "traffic": None}
-    sub = {}
-    for key, v in roofline_entries(fp64, sysm.n, blocks, nl["n_atom_pairs"]).items() if fp64 else []:
-        peak = PEAK[v[2]] if v[0] == "mfma" else PEAK["hbm_GBs"]
-        sub[key] = {"bound": v[0], "achieved": round(v[1], 3), "peak": peak, "frac": round(v[1] / peak, 4),
-                    "unit": "TFLOP/s" if v[0] == "mfma" else "GB/s"}
+    rl = roofline_entries(br, sysm.n, blocks, nl["n_atom_pairs"], members=args.members if best == "surrogate" else 1)
+    # dominant kernel of the headline path (largest time per step among the kernels with a roofline)
+    ranked = sorted(rl, key=lambda n: -br["kernels"][n]["total_ms"])
+    roof = roofline_object(ranked[0], rl[ranked[0]]) if ranked else None
+    kernels_roofline = {}
+    for p, r in res.items():
+        ent = roofline_entries(r, sysm.n, blocks, nl["n_atom_pairs"], members=args.members if p == "surrogate" else 1)
+        kernels_roofline[p] = {n: roofline_object(n, e) for n, e in ent.items()}
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -264,7 +349,8 @@ def main():
         "speedup_vs_fp64_plain": {p: fp64["ms_per_step"] / r["ms_per_step"] for p, r in res.items()} if fp64 else None,
         "accuracy_vs_fp64_gpu": acc,
         "surrogate": surr,
-        "roofline": roof, "subsystems_fp64_plain": sub,
+        "production_md": prod,
+        "roofline": roof, "kernels_roofline": kernels_roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out, default=float))

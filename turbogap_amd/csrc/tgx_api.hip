@@ -103,7 +103,7 @@ static void free_model(tgx_model_t* m) {
   dbuf* bufs[] = {&m->B, &m->xq, &m->cen, &m->dtab, &m->ynorm, &m->rtab, &m->Qs, &m->QsT, &m->alpha, &m->Qs_bf, &m->QsT_bf, &m->alpha_f,
                   &m->site_atom, &m->site_beg, &m->pr_gidx, &m->pr_nbr, &m->pr_spec, &m->pr_r, &m->pr_xyz, &m->q64,
                   &m->q32, &m->qbf, &m->pnorm, &m->e_site, &m->qss64, &m->qss32, &m->kd64, &m->kdbf, &m->epart,
-                  &m->soap_der, &m->fpair, &m->pbasis, &m->low_idx, &m->low_coef};
+                  &m->soap_der, &m->fpair, &m->pbasis, &m->low_idx, &m->low_coef, &m->gate_pn, &m->gate_e};
   for (dbuf* b : bufs) b->release();
   delete m;
 }
@@ -120,7 +120,7 @@ extern "C" int tgx_finalize(tgx_ctx* ctx) {
                   &ctx->thetas, &ctx->phis, &ctx->nspec, &ctx->rev, &ctx->cell_of, &ctx->cell_cnt, &ctx->cell_start,
                   &ctx->cell_atoms, &ctx->scan_tmp, &ctx->do_list, &ctx->energies, &ctx->forces, &ctx->fglob,
                   &ctx->virial_part, &ctx->scratch, &ctx->red, &ctx->vel, &ctx->mass, &ctx->forces_prev,
-                  &ctx->pos_prev, &ctx->e_members, &ctx->gate_mask};
+                  &ctx->pos_prev, &ctx->e_members, &ctx->gate_mask, &ctx->cell_pos};
   for (dbuf* b : bufs) b->release();
   for (auto& r : ctx->prof) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -197,6 +197,7 @@ extern "C" int tgx_get_neighbors_list(tgx_ctx* ctx, int* n_neigh, int* neighbors
   TGX_CTX_CHECK(ctx);
   TGX_REQUIRE(ctx->have_list, "tgx_get_neighbors_list: no list built");
   const size_t P = (size_t)ctx->n_pairs;
+  if (thetas || phis) TGX_TRY(tgx_neighbor_angles(ctx));
   auto cp = [&](void* dst, const dbuf& src, size_t bytes) -> int {
     if (dst && bytes) TGX_CHECK_HIP(hipMemcpyAsync(dst, src.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
     return TGX_OK;
@@ -595,6 +596,8 @@ extern "C" int tgx_compute(tgx_ctx* ctx, int path, double* energy, double* viria
   TGX_REQUIRE(path >= TGX_PATH_FP64_PLAIN && path <= TGX_PATH_SURROGATE, "tgx_compute: unknown path %d", path);
   TGX_REQUIRE(!ctx->models.empty(), "tgx_compute: no model defined");
   TGX_TRY(begin_step(ctx));
+  ctx->gate_sites = ctx->gate_gated = 0;
+  ctx->gated_fraction = ctx->max_spread = 0.0;
   for (auto* m : ctx->models) {
     if (path == TGX_PATH_SURROGATE) {
       TGX_TRY(tgx_surrogate_run(ctx, m));

@@ -138,6 +138,7 @@ struct tgx_model_t {
   dbuf kd64, kdbf;  // [n_sites_pad][Spad] K^(zeta-1) alpha (scaled)
   dbuf epart;       // energy partials [n_sites_pad][n_tiles]
   dbuf soap_der;    // [n_pairs_b][D][3] FP64 (plain path)
+  dbuf gate_pn, gate_e;  // surrogate fallback: |p| of the gated sites, saved energies
   dbuf pbasis;      // [n_pairs_b][ldb] FP32 pair basis: A_l Y_lm | R_n | dR_n (FP32 paths)
   dbuf fpair;       // [n_pairs_b][3] pair forces
 };
@@ -158,6 +159,8 @@ struct tgx_ctx {
   dbuf n_neigh, nl_off, nlist, rjs, xyz, thetas, phis, nspec, rev;
   dbuf cell_of, cell_cnt, cell_start, cell_atoms, scan_tmp, do_list;
   bool have_list = false;
+  bool have_angles = false;  // thetas / phis of the resident list computed
+  dbuf cell_pos;             // positions in cell order (cell-list build)
   // models
   std::vector<tgx_model_t*> models;
   int current = -1;
@@ -169,7 +172,8 @@ struct tgx_ctx {
   // surrogate gate
   int gate_members = 1;
   double gate_threshold = 1e30;
-  double gated_fraction = 0.0, max_spread = 0.0;
+  double gated_fraction = 0.0, max_spread = 0.0;  // over every block of the last tgx_compute
+  int64_t gate_sites = 0, gate_gated = 0;
   dbuf e_members, gate_mask;
   // profiling
   bool profiling = false;
@@ -201,6 +205,7 @@ void tgx_gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w);
 
 // Kernels' host entry points (one per translation unit).
 int tgx_neighbors_run(tgx_ctx* ctx, double rcut_max, const int* indices, const int* do_list);
+int tgx_neighbor_angles(tgx_ctx* ctx);
 int tgx_block_select(tgx_ctx* ctx, tgx_model_t* m);
 int tgx_soap_descriptors(tgx_ctx* ctx, tgx_model_t* m, int path, bool with_der);
 int tgx_gap_contract(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces);

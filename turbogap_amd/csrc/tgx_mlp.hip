@@ -1014,9 +1014,7 @@ int tgx_surrogate_run(tgx_ctx* ctx, tgx_model_t* m) {
 
   TGX_LAUNCH(ctx, "mlp_out", k_surrogate_out, g1(ns), dim3(256), 0, ns, K, Em, Mp, pk.shift.as<double>(),
              pk.scale.as<double>(), m->e_site.as<double>(), spread);
-  // ensemble gate
-  ctx->gated_fraction = 0.0;
-  ctx->max_spread = 0.0;
+  // ensemble gate (statistics accumulate over the blocks of one tgx_compute)
   if (K > 1) {
     std::vector<float> sp((size_t)ns);
     TGX_CHECK_HIP(hipMemcpyAsync(sp.data(), spread, sizeof(float) * ns, hipMemcpyDeviceToHost, ctx->stream));
@@ -1026,7 +1024,9 @@ int tgx_surrogate_run(tgx_ctx* ctx, tgx_model_t* m) {
       ctx->max_spread = std::max(ctx->max_spread, (double)sp[i]);
       if (sp[i] > ctx->gate_threshold) gated.push_back(i);
     }
-    ctx->gated_fraction = (double)gated.size() / ns;
+    ctx->gate_sites += ns;
+    ctx->gate_gated += (int64_t)gated.size();
+    ctx->gated_fraction = (double)ctx->gate_gated / (double)ctx->gate_sites;
     if (!gated.empty()) {
       const int ng = (int)gated.size(), ngp = (int)tgx_round_up(ng, 128);
       TGX_TRY(ctx->gate_mask.ensure(sizeof(int) * (size_t)ngp));
@@ -1034,11 +1034,11 @@ int tgx_surrogate_run(tgx_ctx* ctx, tgx_model_t* m) {
       TGX_TRY(m->q64.ensure(sizeof(double) * (size_t)ngp * Dp));
       TGX_TRY(tgx_zero(ctx, m->q64.p, sizeof(double) * (size_t)ngp * Dp));
       // FP64 descriptors and GAP of the gated sites only
-      dbuf pn;
+      dbuf& pn = m->gate_pn;
       TGX_TRY(pn.ensure(sizeof(double) * (size_t)ngp));
       TGX_TRY(tgx_soap_descriptors_sites(ctx, m, ctx->gate_mask.as<int>(), ng, m->q64.as<double>(), pn.as<double>()));
       const int ns_save = m->n_sites_b;
-      dbuf e_keep;
+      dbuf& e_keep = m->gate_e;
       TGX_TRY(e_keep.ensure(sizeof(double) * (size_t)Mp));
       TGX_CHECK_HIP(hipMemcpyAsync(e_keep.p, m->e_site.p, sizeof(double) * ns, hipMemcpyDeviceToDevice, ctx->stream));
       m->n_sites_b = ng;
@@ -1050,9 +1050,6 @@ int tgx_surrogate_run(tgx_ctx* ctx, tgx_model_t* m) {
                    m->e_site.as<double>(), m->qss64.as<double>(), e_keep.as<double>(), m->qss32.as<float>());
         TGX_CHECK_HIP(hipMemcpyAsync(m->e_site.p, e_keep.p, sizeof(double) * ns, hipMemcpyDeviceToDevice, ctx->stream));
       }
-      TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-      pn.release();
-      e_keep.release();
       TGX_TRY(st);
     }
   }

@@ -26,7 +26,9 @@ struct BoxInfo {
 };
 
 __device__ __forceinline__ double f_modulo(double a, double p) {
-  double r = fmod(a, p);
+  // fmod(a, p) is exactly a when |a| < p (a zero quotient): skip the division
+  // for the in-box differences that are almost every call
+  double r = fabs(a) < p ? a : fmod(a, p);
   if (r != 0.0) {
     if ((a < 0.0) != (p < 0.0)) r = r + p;
   } else {
@@ -48,18 +50,24 @@ __device__ __forceinline__ double dot3(const double* u, const double* v) {
   return s;
 }
 
+// The orthorhombic branch of src/neighbors.f90:90-101.
+__device__ __forceinline__ void get_distance_ortho(const double* pi, const double* pj, const BoxInfo& bx, double* dist,
+                                                   double& d) {
+  double d2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double dk = f_modulo(pj[k] - pi[k], bx.L[k]);
+    if (dk > bx.L[k] / 2.0) dk = dk - bx.L[k];
+    dist[k] = dk;
+    d2 = d2 + dk * dk;
+  }
+  d = sqrt(d2);
+}
+
 // src/neighbors.f90:47-163 with PBC on in every direction.
 __device__ void get_distance(const double* pi, const double* pj, const BoxInfo& bx, double* dist, double& d) {
   if (bx.ortho) {
-    double d2 = 0.0;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      double dk = f_modulo(pj[k] - pi[k], bx.L[k]);
-      if (dk > bx.L[k] / 2.0) dk = dk - bx.L[k];
-      dist[k] = dk;
-      d2 = d2 + dk * dk;
-    }
-    d = sqrt(d2);
+    get_distance_ortho(pi, pj, bx, dist, d);
     return;
   }
   double dd[3] = {pj[0] - pi[0], pj[1] - pi[1], pj[2] - pi[2]};
@@ -142,52 +150,86 @@ __global__ void k_sort_cells(int n_cells, const int64_t* __restrict__ cell_start
   }
 }
 
+// Positions in cell order: cell_pos[slot] = pos[cell_atoms[slot]], so the
+// candidates of a cell are read as one contiguous run.
+__global__ void k_cell_pos(int n, const int* __restrict__ cell_atoms, const double* __restrict__ pos,
+                           double* __restrict__ cell_pos) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int k = cell_atoms[s];
+  cell_pos[3 * (size_t)s] = pos[3 * (size_t)k];
+  cell_pos[3 * (size_t)s + 1] = pos[3 * (size_t)k + 1];
+  cell_pos[3 * (size_t)s + 2] = pos[3 * (size_t)k + 2];
+}
+
+__device__ __forceinline__ void wave_sync_nb() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int NBW = 4;  // wavefronts (central atoms) per workgroup
+
 // Cell-list traversal in the reference's order for one central atom, one
-// wavefront per atom. mode 0 counts, mode 1 fills.
+// wavefront per atom. The visited cells (src/neighbors.f90:429-446) and the
+// running count of their atoms go to LDS; lane t of a round takes candidate
+// base + t, finds its cell by bisection and reads the candidate's index and
+// position from the cell-ordered arrays. Hits are compacted with a ballot, so
+// the output order is the reference's. mode 0 counts, mode 1 fills (index,
+// distance, displacement, species; the angles come from k_angles on demand).
 template <int MODE>
-__global__ __launch_bounds__(256) void k_cell_neigh(int n, const double* __restrict__ pos, BoxInfo bx, int mx, int my,
-                                                  int mz, double rcut, const int* __restrict__ do_list,
-                                                  const int* __restrict__ cell_xyz,
-                                                  const int64_t* __restrict__ cell_start,
-                                                  const int* __restrict__ cell_cnt,
-                                                  const int* __restrict__ cell_atoms,
-                                                  const int* __restrict__ species, int* __restrict__ n_neigh,
-                                                  const int64_t* __restrict__ off, int* __restrict__ nlist,
-                                                  double* __restrict__ rjs, double* __restrict__ xyz,
-                                                  double* __restrict__ thetas, double* __restrict__ phis,
-                                                  int* __restrict__ nspec) {
-  const int lane = threadIdx.x & 63;
-  const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+__global__ __launch_bounds__(64 * NBW) void k_cell_neigh(int n, const double* __restrict__ pos, BoxInfo bx, int mx,
+                                                       int my, int mz, double rcut, const int* __restrict__ do_list,
+                                                       const int* __restrict__ cell_xyz,
+                                                       const int64_t* __restrict__ cell_start,
+                                                       const int* __restrict__ cell_cnt,
+                                                       const int* __restrict__ cell_atoms,
+                                                       const double* __restrict__ cell_pos,
+                                                       const int* __restrict__ species, int* __restrict__ n_neigh,
+                                                       const int64_t* __restrict__ off, int* __restrict__ nlist,
+                                                       double* __restrict__ rjs, double* __restrict__ xyz,
+                                                       int* __restrict__ nspec) {
+  __shared__ int s_beg[NBW][29];  // running counts of the (up to 27) visited cells, then their number
+  __shared__ int64_t s_st[NBW][27];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * NBW + w);
   if (i >= n) return;
   if (do_list && !do_list[i]) {
     if (MODE == 0 && lane == 0) n_neigh[i] = 0;
     return;
   }
-  const double pi[3] = {pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2]};
+  int* vbeg = s_beg[w];
+  int64_t* vst = s_st[w];
   const int i2 = 1 + cell_xyz[3 * i], j2 = 1 + cell_xyz[3 * i + 1], k2 = 1 + cell_xyz[3 * i + 2];
-  // visited cells in order (src/neighbors.f90:429-446)
-  int vcell[27], vcnt[27];
-  int nv = 0, total = 0;
-  for (int k3 = k2 - 1; k3 <= k2 + 1; ++k3) {
-    if (mz == 1 && k3 != 1) continue;
-    if (mz == 2 && k2 == 1 && k3 == 0) continue;
-    if (mz == 2 && k2 == 2 && k3 == 3) continue;
-    for (int j3 = j2 - 1; j3 <= j2 + 1; ++j3) {
-      if (my == 1 && j3 != 1) continue;
-      if (my == 2 && j2 == 1 && j3 == 0) continue;
-      if (my == 2 && j2 == 2 && j3 == 3) continue;
-      for (int i3 = i2 - 1; i3 <= i2 + 1; ++i3) {
-        if (mx == 1 && i3 != 1) continue;
-        if (mx == 2 && i2 == 1 && i3 == 0) continue;
-        if (mx == 2 && i2 == 2 && i3 == 3) continue;
-        int c = i_modulo(i3 - 1, mx) + i_modulo(j3 - 1, my) * mx + i_modulo(k3 - 1, mz) * mx * my;
-        vcell[nv] = c;
-        vcnt[nv] = cell_cnt[c];
-        total += vcnt[nv];
-        ++nv;
+  if (lane == 0) {
+    int nv = 0, total = 0;
+    for (int k3 = k2 - 1; k3 <= k2 + 1; ++k3) {
+      if (mz == 1 && k3 != 1) continue;
+      if (mz == 2 && k2 == 1 && k3 == 0) continue;
+      if (mz == 2 && k2 == 2 && k3 == 3) continue;
+      for (int j3 = j2 - 1; j3 <= j2 + 1; ++j3) {
+        if (my == 1 && j3 != 1) continue;
+        if (my == 2 && j2 == 1 && j3 == 0) continue;
+        if (my == 2 && j2 == 2 && j3 == 3) continue;
+        for (int i3 = i2 - 1; i3 <= i2 + 1; ++i3) {
+          if (mx == 1 && i3 != 1) continue;
+          if (mx == 2 && i2 == 1 && i3 == 0) continue;
+          if (mx == 2 && i2 == 2 && i3 == 3) continue;
+          const int c = i_modulo(i3 - 1, mx) + i_modulo(j3 - 1, my) * mx + i_modulo(k3 - 1, mz) * mx * my;
+          vbeg[nv] = total;
+          vst[nv] = cell_start[c];
+          total += cell_cnt[c];
+          ++nv;
+        }
       }
     }
+    vbeg[nv] = total;
+    vbeg[28] = nv;
   }
+  wave_sync_nb();
+  const int nv = vbeg[28];
+  const int total = vbeg[nv];
+  const double pi[3] = {pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2]};
   int64_t out = 0;
   if (MODE == 1) {
     out = off[i];
@@ -195,8 +237,6 @@ __global__ __launch_bounds__(256) void k_cell_neigh(int n, const double* __restr
       nlist[out] = i + 1;
       rjs[out] = 0.0;
       xyz[3 * out] = 0.0; xyz[3 * out + 1] = 0.0; xyz[3 * out + 2] = 0.0;
-      thetas[out] = 0.0;
-      phis[out] = 0.0;
       nspec[out] = species[i];
     }
     out += 1;
@@ -208,12 +248,17 @@ __global__ __launch_bounds__(256) void k_cell_neigh(int n, const double* __restr
     int k = -1;
     double dist[3] = {0, 0, 0}, d = 0.0;
     if (cidx < total) {
-      int acc = 0, v = 0;
-      while (acc + vcnt[v] <= cidx) { acc += vcnt[v]; ++v; }
-      k = cell_atoms[cell_start[vcell[v]] + (cidx - acc)];
+      int lo = 0, hi = nv;  // vbeg[lo] <= cidx < vbeg[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (vbeg[mid] <= cidx) lo = mid;
+        else hi = mid;
+      }
+      const int64_t slot = vst[lo] + (cidx - vbeg[lo]);
+      k = cell_atoms[slot];
       if (k != i) {
-        const double pk[3] = {pos[3 * (size_t)k], pos[3 * (size_t)k + 1], pos[3 * (size_t)k + 2]};
-        get_distance(pi, pk, bx, dist, d);
+        const double pk[3] = {cell_pos[3 * slot], cell_pos[3 * slot + 1], cell_pos[3 * slot + 2]};
+        get_distance_ortho(pi, pk, bx, dist, d);  // the cell list is built for orthorhombic boxes only
         hit = d < rcut;
       }
     }
@@ -223,15 +268,30 @@ __global__ __launch_bounds__(256) void k_cell_neigh(int n, const double* __restr
       nlist[p] = k + 1;
       rjs[p] = d;
       xyz[3 * p] = dist[0]; xyz[3 * p + 1] = dist[1]; xyz[3 * p + 2] = dist[2];
-      double th, ph;
-      angles(dist, d, th, ph);
-      thetas[p] = th;
-      phis[p] = ph;
       nspec[p] = species[k];
     }
     hits += __popcll(mask);
   }
   if (MODE == 0 && lane == 0) n_neigh[i] = 1 + hits;
+}
+
+// thetas / phis of every pair from its displacement (src/neighbors.f90:552-591);
+// the self pair (r = 0) gets 0, 0.
+__global__ void k_angles(int64_t P, const double* __restrict__ rjs, const double* __restrict__ xyz,
+                         double* __restrict__ thetas, double* __restrict__ phis) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const double d = rjs[p];
+  if (d == 0.0) {
+    thetas[p] = 0.0;
+    phis[p] = 0.0;
+    return;
+  }
+  const double dist[3] = {xyz[3 * p], xyz[3 * p + 1], xyz[3 * p + 2]};
+  double th, ph;
+  angles(dist, d, th, ph);
+  thetas[p] = th;
+  phis[p] = ph;
 }
 
 // Brute-force build over the supercell (src/neighbors.f90:473-510): small boxes
@@ -242,7 +302,6 @@ __global__ __launch_bounds__(256) void k_brute_neigh(int n, int n_sc, const doub
                                                    const int* __restrict__ species, int* __restrict__ n_neigh,
                                                    const int64_t* __restrict__ off, int* __restrict__ nlist,
                                                    double* __restrict__ rjs, double* __restrict__ xyz,
-                                                   double* __restrict__ thetas, double* __restrict__ phis,
                                                    int* __restrict__ nspec) {
   const int lane = threadIdx.x & 63;
   const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -259,8 +318,6 @@ __global__ __launch_bounds__(256) void k_brute_neigh(int n, int n_sc, const doub
       nlist[out] = i + 1;
       rjs[out] = 0.0;
       xyz[3 * out] = 0.0; xyz[3 * out + 1] = 0.0; xyz[3 * out + 2] = 0.0;
-      thetas[out] = 0.0;
-      phis[out] = 0.0;
       nspec[out] = species[i];
     }
     out += 1;
@@ -281,10 +338,6 @@ __global__ __launch_bounds__(256) void k_brute_neigh(int n, int n_sc, const doub
       nlist[p] = j + 1;
       rjs[p] = d;
       xyz[3 * p] = dist[0]; xyz[3 * p + 1] = dist[1]; xyz[3 * p + 2] = dist[2];
-      double th, ph;
-      angles(dist, d, th, ph);
-      thetas[p] = th;
-      phis[p] = ph;
       nspec[p] = species[j];
     }
     hits += __popcll(mask);
@@ -305,14 +358,17 @@ __global__ void k_reverse(int n, const int* __restrict__ n_neigh, const int64_t*
   for (int t = lane; t < nn; t += 64) {
     const int64_t p = b + t;
     if (t == 0) { rev[p] = p; continue; }
-    const int j = (nlist[p] - 1) % n;
+    int j = nlist[p] - 1;
+    if (j >= n) j %= n;
     const int64_t bj = off[j];
     const int nj = n_neigh[j];
     int64_t best = -1;
     double bd = 0.0;
     for (int u = 1; u < nj; ++u) {
       const int64_t q = bj + u;
-      if ((nlist[q] - 1) % n != i) continue;
+      int jq = nlist[q] - 1;
+      if (jq >= n) jq %= n;
+      if (jq != i) continue;
       double e0 = xyz[3 * q] + xyz[3 * p], e1 = xyz[3 * q + 1] + xyz[3 * p + 1], e2 = xyz[3 * q + 2] + xyz[3 * p + 2];
       double dd = e0 * e0 + e1 * e1 + e2 * e2;
       if (best < 0 || dd < bd) { best = q; bd = dd; }
@@ -372,8 +428,8 @@ int tgx_neighbors_run(tgx_ctx* ctx, double rcut_max, const int* indices, const i
   }
   TGX_TRY(ctx->n_neigh.ensure(sizeof(int) * n));
   TGX_TRY(ctx->nl_off.ensure(sizeof(int64_t) * (n + 1)));
-  const dim3 wblk(256);
-  const dim3 wgrid((n + 3) / 4);
+  const dim3 wblk(64 * NBW);
+  const dim3 wgrid((n + NBW - 1) / NBW);
   int mx = 0, my = 0, mz = 0;
   const bool cells = bx.ortho && !small;
   if (cells) {
@@ -398,13 +454,16 @@ int tgx_neighbors_run(tgx_ctx* ctx, double rcut_max, const int* indices, const i
                ctx->cell_start.as<int64_t>(), fill, ctx->cell_atoms.as<int>());
     TGX_LAUNCH(ctx, "nbr_sort", k_sort_cells, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, (int)nc,
                ctx->cell_start.as<int64_t>(), cnt, ctx->cell_atoms.as<int>());
+    TGX_TRY(ctx->cell_pos.ensure(sizeof(double) * 3 * (size_t)n));
+    TGX_LAUNCH(ctx, "nbr_cell_pos", k_cell_pos, dim3((n + 255) / 256), dim3(256), 0, n, ctx->cell_atoms.as<int>(),
+               ctx->pos.as<double>(), ctx->cell_pos.as<double>());
     TGX_LAUNCH(ctx, "nbr_count", k_cell_neigh<0>, wgrid, wblk, 0, n, ctx->pos.as<double>(), bx, mx, my, mz, rcut_max,
-               d_do, cell_xyz, ctx->cell_start.as<int64_t>(), cnt, ctx->cell_atoms.as<int>(), ctx->species_sc.as<int>(),
-               ctx->n_neigh.as<int>(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+               d_do, cell_xyz, ctx->cell_start.as<int64_t>(), cnt, ctx->cell_atoms.as<int>(),
+               ctx->cell_pos.as<double>(), ctx->species_sc.as<int>(), ctx->n_neigh.as<int>(), nullptr, nullptr,
+               nullptr, nullptr, nullptr);
   } else {
     TGX_LAUNCH(ctx, "nbr_count_brute", k_brute_neigh<0>, wgrid, wblk, 0, n, n_sc, ctx->pos.as<double>(), bx, rcut_max,
-               d_do, ctx->species_sc.as<int>(), ctx->n_neigh.as<int>(), nullptr, nullptr, nullptr, nullptr, nullptr,
-               nullptr, nullptr);
+               d_do, ctx->species_sc.as<int>(), ctx->n_neigh.as<int>(), nullptr, nullptr, nullptr, nullptr, nullptr);
   }
   int64_t P = 0;
   TGX_TRY(tgx_scan_i32(ctx, ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(), n, &P));
@@ -421,18 +480,28 @@ int tgx_neighbors_run(tgx_ctx* ctx, double rcut_max, const int* indices, const i
     int* cell_xyz = ctx->cell_of.as<int>() + n;
     TGX_LAUNCH(ctx, "nbr_fill", k_cell_neigh<1>, wgrid, wblk, 0, n, ctx->pos.as<double>(), bx, mx, my, mz, rcut_max,
                d_do, cell_xyz, ctx->cell_start.as<int64_t>(), ctx->cell_cnt.as<int>(), ctx->cell_atoms.as<int>(),
-               ctx->species_sc.as<int>(), ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(), ctx->nlist.as<int>(),
-               ctx->rjs.as<double>(), ctx->xyz.as<double>(), ctx->thetas.as<double>(), ctx->phis.as<double>(),
+               ctx->cell_pos.as<double>(), ctx->species_sc.as<int>(), ctx->n_neigh.as<int>(),
+               ctx->nl_off.as<int64_t>(), ctx->nlist.as<int>(), ctx->rjs.as<double>(), ctx->xyz.as<double>(),
                ctx->nspec.as<int>());
   } else {
     TGX_LAUNCH(ctx, "nbr_fill_brute", k_brute_neigh<1>, wgrid, wblk, 0, n, n_sc, ctx->pos.as<double>(), bx, rcut_max,
                d_do, ctx->species_sc.as<int>(), ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(),
-               ctx->nlist.as<int>(), ctx->rjs.as<double>(), ctx->xyz.as<double>(), ctx->thetas.as<double>(),
-               ctx->phis.as<double>(), ctx->nspec.as<int>());
+               ctx->nlist.as<int>(), ctx->rjs.as<double>(), ctx->xyz.as<double>(), ctx->nspec.as<int>());
   }
   TGX_LAUNCH(ctx, "nbr_reverse", k_reverse, wgrid, wblk, 0, n, ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(),
              ctx->nlist.as<int>(), ctx->xyz.as<double>(), ctx->rev.as<int64_t>());
   ctx->rcut_max = rcut_max;
   ctx->have_list = true;
+  ctx->have_angles = false;
+  return TGX_OK;
+}
+
+// thetas / phis of the resident list, computed once per build when asked for.
+int tgx_neighbor_angles(tgx_ctx* ctx) {
+  if (ctx->have_angles || ctx->n_pairs == 0) return TGX_OK;
+  const int64_t P = ctx->n_pairs;
+  TGX_LAUNCH(ctx, "nbr_angles", k_angles, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, P, ctx->rjs.as<double>(),
+             ctx->xyz.as<double>(), ctx->thetas.as<double>(), ctx->phis.as<double>());
+  ctx->have_angles = true;
   return TGX_OK;
 }

@@ -128,28 +128,41 @@ __device__ __forceinline__ void damping(const SpecLite& sp, float r, float (&A)[
   }
 }
 
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+// Power spectrum p(n, n', l) = w sum_m c(n, lm) c(n', lm) (w = sqrt(2) for
+// n != n') and q = p/|p| into X[D]. For every l the 16 x 16 Gram matrix
+// C_l C_l^T is formed on the matrix cores by v_mfma_f32_16x16x4_f32 (k over m
+// in steps of 4): lane t supplies C[t % 16][l^2 + k + t / 16] as both operands,
+// and holds the result rows 4 (t / 16) + r, column t % 16. Needs N <= 16 and
+// the whole wavefront active.
 __device__ float power(const SoapParams& P, const float* C, float* X) {
   const int lane = threadIdx.x & 63;
-  const int NLM = P.NLM;
+  const int NLM = P.NLM, N = P.N, L1 = P.L + 1;
+  const int cn = lane & 15, kq = lane >> 4;
   float acc = 0.f;
-  for (int d = lane; d < P.D; d += 64) {
-    const int code = P.dtab[d];
-    const int n = code & 0xff, np = (code >> 8) & 0xff, l = code >> 16;
-    const float* c1 = C + n * NLM + l * l;
-    const float* c2 = C + np * NLM + l * l;
-    float s0 = 0.f, s1 = 0.f;
-#pragma unroll
-    for (int m = 0; m < 2 * TGX_MAX_L + 1; m += 2) {
-      if (m < 2 * l + 1) s0 += c1[m] * c2[m];
-      if (m + 1 < 2 * l + 1) s1 += c1[m + 1] * c2[m + 1];
+  for (int l = 0; l < L1; ++l) {
+    const int K = 2 * l + 1;
+    f32x4_t g = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += 4) {
+      const int k = k0 + kq;
+      const float v = (cn < N && k < K) ? C[cn * NLM + l * l + k] : 0.f;
+      g = __builtin_amdgcn_mfma_f32_16x16x4f32(v, v, g, 0, 0, 0);
     }
-    float s = s0 + s1;
-    if (n != np) s *= (float)M_SQRT2;
-    X[d] = s;
-    acc += s * s;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n = 4 * kq + r, np = cn;
+      if (n <= np && np < N) {
+        const int pidx = n * N - n * (n - 1) / 2 + (np - n);
+        const float s = n == np ? g[r] : (float)M_SQRT2 * g[r];
+        X[pidx * L1 + l] = s;
+        acc += s * s;
+      }
+    }
   }
   const float nrm = sqrtf(wsum(acc));
   const float inv = nrm > 0.f ? 1.f / nrm : 0.f;
+  wave_sync();
   for (int d = lane; d < P.D; d += 64) X[d] *= inv;
   wave_sync();
   return nrm;
@@ -251,16 +264,28 @@ __device__ __forceinline__ void site_coefficients(const SoapParams& P, const F32
     }
   }
   const bool v0 = lane < P.NLM, v1 = lane + 64 < P.NLM;
-  const float* row = a.basis + (b0 + 1) * a.ldb;
-#pragma unroll 2
-  for (int j = 1; j < nn; ++j, row += a.ldb) {
-    const float w0 = v0 ? row[lane] : 0.f;
-    const float w1 = v1 ? row[lane + 64] : 0.f;
+  // chunks of CCH pairs: all of a chunk's row loads are issued before its FMAs
+  constexpr int CCH = 8;
+  for (int j0 = 1; j0 < nn; j0 += CCH) {
+    float w0[CCH], w1[CCH];
 #pragma unroll
-    for (int n = 0; n < NMAXF; ++n) {
-      const float rn = n < P.N ? row[a.roff + n] : 0.f;
-      acc[0][n] += rn * w0;
-      acc[1][n] += rn * w1;
+    for (int u = 0; u < CCH; ++u) {
+      const float* row = a.basis + (b0 + j0 + u) * a.ldb;
+      const bool ok = j0 + u < nn;
+      w0[u] = ok && v0 ? row[lane] : 0.f;
+      w1[u] = ok && v1 ? row[lane + 64] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < CCH; ++u) {
+      if (j0 + u < nn) {
+        const float* row = a.basis + (b0 + j0 + u) * a.ldb + a.roff;
+#pragma unroll
+        for (int n = 0; n < NMAXF; ++n) {
+          const float rn = n < P.N ? row[n] : 0.f;
+          acc[0][n] += rn * w0[u];
+          acc[1][n] += rn * w1[u];
+        }
+      }
     }
   }
 }
