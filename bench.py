@@ -318,6 +318,8 @@ def main():
                   "virial_maxabs_err_eV": float(np.abs(r["virial"] - ref["virial"]).max())}
     # headline: the fastest path within the force-error bar (50 meV/A vs FP64)
     ok = [p for p in res if p in ("fp64_plain", "fp64_adjoint") or acc.get(p, {}).get("force_rmse_meV_A", 1e9) <= 50.0]
+    if ref is None:  # no FP64 path timed (profiling runs): accuracy unchecked
+        ok = list(res)
     best = min(ok, key=lambda p: res[p]["ms_per_step"])
     br = res[best]
     value = sysm.n / (br["ms_per_step"] * 1e-3)

@@ -158,17 +158,31 @@ __global__ void k_rowcomb(int64_t n, int cols, const float* __restrict__ dE, con
 }
 
 // out[j] = sum_b (dE_b X[b][j] + c Y[b][j]); one thread per column, rows in order.
-__global__ void k_colsum(int rows, int cols, const float* __restrict__ dE, const float* __restrict__ X, float c,
-                         const float* __restrict__ Y, float* __restrict__ out) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= cols) return;
+// out[j] = sum_b (dE[b] X[b][j] + c Y[b][j]) (X null: X = 1; Y null: no second
+// term). A 64 x 16 thread block per 64 columns: each of the 16 row groups sums
+// every 16th row in FP64 (coalesced along the columns), then the 16 partials are
+// added in a fixed order.
+__global__ __launch_bounds__(1024) void k_colsum(int rows, int cols, const float* __restrict__ dE,
+                                                 const float* __restrict__ X, float c, const float* __restrict__ Y,
+                                                 float* __restrict__ out) {
+  __shared__ double part[16][64];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + tx;
   double acc = 0.0;
-  for (int b = 0; b < rows; ++b) {
-    double v = dE ? (double)dE[b] * (X ? X[(size_t)b * cols + j] : 1.0f) : 0.0;
-    if (Y) v += (double)c * Y[(size_t)b * cols + j];
-    acc += v;
+  if (j < cols) {
+    for (int b = ty; b < rows; b += 16) {
+      double v = dE ? (double)dE[b] * (X ? (double)X[(size_t)b * cols + j] : 1.0) : 0.0;
+      if (Y) v += (double)c * (double)Y[(size_t)b * cols + j];
+      acc += v;
+    }
   }
-  out[j] = (float)acc;
+  part[ty][tx] = acc;
+  __syncthreads();
+  if (ty == 0 && j < cols) {
+    double t = 0.0;
+    for (int g = 0; g < 16; ++g) t += part[g][tx];
+    out[j] = (float)t;
+  }
 }
 
 // C[m][n] = sum_b X1[b][m] Y1[b][n] + c2 sum_b X2[b][m] Y2[b][n] (FP32), a 64x64 tile
@@ -502,13 +516,13 @@ static int mlp_backward(tgx_mlp* m, int B, float w_e, float w_g, double* loss_e,
   TGX_LAUNCH(ctx, "mlp_dW2", k_tn_gemm, dim3(H / 64, H / 64), dim3(256), 0, B, H, H, m->a2.as<float>(),
              m->Y2.as<float>(), m->zet2.as<float>(), m->h1.as<float>(), cG, gr + m->oW2);
   // biases and output weights
-  TGX_LAUNCH(ctx, "mlp_db1", k_colsum, g1(H), dim3(256), 0, B, H, m->dE.as<float>(), m->a1.as<float>(), cG,
+  TGX_LAUNCH(ctx, "mlp_db1", k_colsum, dim3((H + 63) / 64), dim3(1024), 0, B, H, m->dE.as<float>(), m->a1.as<float>(), cG,
              m->zet1.as<float>(), gr + m->ob1);
-  TGX_LAUNCH(ctx, "mlp_db2", k_colsum, g1(H), dim3(256), 0, B, H, m->dE.as<float>(), m->a2.as<float>(), cG,
+  TGX_LAUNCH(ctx, "mlp_db2", k_colsum, dim3((H + 63) / 64), dim3(1024), 0, B, H, m->dE.as<float>(), m->a2.as<float>(), cG,
              m->zet2.as<float>(), gr + m->ob2);
-  TGX_LAUNCH(ctx, "mlp_dw3", k_colsum, g1(H), dim3(256), 0, B, H, m->dE.as<float>(), m->h2.as<float>(), cG,
+  TGX_LAUNCH(ctx, "mlp_dw3", k_colsum, dim3((H + 63) / 64), dim3(1024), 0, B, H, m->dE.as<float>(), m->h2.as<float>(), cG,
              m->gam2.as<float>(), gr + m->ow3);
-  TGX_LAUNCH(ctx, "mlp_db3", k_colsum, dim3(1), dim3(256), 0, B, 1, m->dE.as<float>(), (const float*)nullptr, 0.0f,
+  TGX_LAUNCH(ctx, "mlp_db3", k_colsum, dim3(1), dim3(1024), 0, B, 1, m->dE.as<float>(), (const float*)nullptr, 0.0f,
              (const float*)nullptr, gr + m->ob3);
   if (loss_e || loss_g) {
     double se = 0.0, sg = 0.0;

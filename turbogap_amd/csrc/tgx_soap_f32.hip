@@ -50,7 +50,7 @@ struct F32Args {
   const int64_t* pr_gidx;
   const float* rtab;   // [S][K][2][MAX_NB]: R then dR per node
   int K;
-  float* basis;        // [pairs][ldb]: W (NLM, padded to roff) | R (N) | dR (N)
+  float* basis;        // [pairs][ldb]: W (NLM, padded to roff) | R (16 slots) | dR (16 slots)
   int ldb, roff;
 };
 
@@ -129,6 +129,7 @@ __device__ __forceinline__ void damping(const SpecLite& sp, float r, float (&A)[
 }
 
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x16_t __attribute__((ext_vector_type(16), aligned(4)));  // rows are only dword-aligned in general
 
 // Power spectrum p(n, n', l) = w sum_m c(n, lm) c(n', lm) (w = sqrt(2) for
 // n != n') and q = p/|p| into X[D]. For every l the 16 x 16 Gram matrix
@@ -231,7 +232,7 @@ __global__ __launch_bounds__(PB) void k_pair_basis(SoapParams P, F32Args a, int6
       for (int n = 0; n < TGX_MAX_NB; ++n)
         if (n < sp.nb) {
           row[a.roff + sp.off + n] = R[n];
-          row[a.roff + P.N + sp.off + n] = dR[n];
+          row[a.roff + NMAXF + sp.off + n] = dR[n];
         }
       float A[TGX_MAX_L + 1], dA[TGX_MAX_L + 1];
       damping(sp, r, A, dA);
@@ -278,12 +279,11 @@ __device__ __forceinline__ void site_coefficients(const SoapParams& P, const F32
 #pragma unroll
     for (int u = 0; u < CCH; ++u) {
       if (j0 + u < nn) {
-        const float* row = a.basis + (b0 + j0 + u) * a.ldb + a.roff;
+        const f32x16_t rv = *reinterpret_cast<const f32x16_t*>(a.basis + (b0 + j0 + u) * a.ldb + a.roff);
 #pragma unroll
         for (int n = 0; n < NMAXF; ++n) {
-          const float rn = n < P.N ? row[n] : 0.f;
-          acc[0][n] += rn * w0[u];
-          acc[1][n] += rn * w1[u];
+          acc[0][n] += rv[n] * w0[u];
+          acc[1][n] += rv[n] * w1[u];
         }
       }
     }
@@ -291,7 +291,13 @@ __device__ __forceinline__ void site_coefficients(const SoapParams& P, const F32
 }
 
 __host__ __device__ inline int desc_wave_floats(const SoapParams& P) { return P.N * P.NLM + P.D; }
-__host__ __device__ inline int adj_wave_floats(const SoapParams& P) { return P.D + 2 * 4 * ((P.NLM + 3) & ~3); }
+constexpr int APASS = 16;  // pair rows staged per pass of the adjoint
+// per wavefront: max(X [D], APASS rows) then the pass's pair forces [APASS][3]
+__host__ __device__ inline int adj_rows_floats(const SoapParams& P, int ldb) {
+  const int r = APASS * ldb;
+  return P.D > r ? ((P.D + 3) & ~3) : r;
+}
+__host__ __device__ inline int adj_wave_floats(const SoapParams& P, int ldb) { return adj_rows_floats(P, ldb) + 3 * APASS; }
 
 __global__ __launch_bounds__(64 * WAVES) void k_site_desc(SoapParams P, F32Args a, int n_sites,
                                                           float* __restrict__ q_out, float* __restrict__ pnorm_out) {
@@ -335,7 +341,6 @@ __device__ __forceinline__ float wave_total(float v) {
           __builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 63)));
 }
 
-constexpr int PCH = 4;  // pairs per chunk of the adjoint's pair loop
 
 __global__ __launch_bounds__(64 * WAVES) void k_site_adjoint(SoapParams P, F32Args a, int n_sites,
                                                              const int* __restrict__ low_idx,
@@ -348,9 +353,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_site_adjoint(SoapParams P, F32Ar
   const int site = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES + w);
   if (site >= n_sites) return;
   const int NLM = P.NLM, N = P.N, D = P.D, L1 = P.L + 1;
-  const int NLMp = (NLM + 3) & ~3;
-  float* X = smf + w * adj_wave_floats(P);
-  float* Wbuf = X + D;  // [2][PCH][NLMp]
+  float* X = smf + w * adj_wave_floats(P, a.ldb);
   const int64_t b0 = a.site_beg[site];
   const int nn = (int)(a.site_beg[site + 1] - b0);
   float hreg[2][NMAXF];
@@ -417,64 +420,67 @@ __global__ __launch_bounds__(64 * WAVES) void k_site_adjoint(SoapParams P, F32Ar
         li[h][c][t] = vh[h] ? low_idx[k] : -1;
         lc[h][c][t] = vh[h] ? low_coef[k] : 0.f;
       }
+  // Pair passes: the rows of up to APASS consecutive pairs are copied into LDS
+  // (over X, which H no longer needs) by one coalesced sweep, so each pass costs
+  // one memory round trip. Pairs without a contribution (other species, beyond
+  // their cutoff) have all-zero rows and come out with a zero force.
+  wave_sync();
+  float* rows = X;
+  float* fb = X + adj_rows_floats(P, a.ldb);  // [APASS][3] pair forces of the pass
+  const int ldb = a.ldb, roff = a.roff;
   double f_self0 = 0.0, f_self1 = 0.0, f_self2 = 0.0;
-  int buf = 0;
-  for (int j0 = 1; j0 < nn; j0 += PCH, buf ^= 1) {
-    float* Wb = Wbuf + buf * PCH * NLMp;
-    float Wv[PCH][2], xa[PCH][2], xb[PCH][2];
-    bool ok[PCH];
-#pragma unroll
-    for (int u = 0; u < PCH; ++u) {
-      const int j = j0 + u;
-      const int64_t p = b0 + j;
-      ok[u] = false;
-      Wv[u][0] = Wv[u][1] = xa[u][0] = xa[u][1] = xb[u][0] = xb[u][1] = 0.f;
-      if (j < nn) {
-        const int s = a.pr_spec[p];
-        ok[u] = s >= 0 && a.pr_r[p] < P.sp[s].rc;  // uniform across the wavefront
-      }
-      if (ok[u]) {
-        const float* row = a.basis + p * a.ldb;
-        Wv[u][0] = vh[0] ? row[lane] : 0.f;
-        Wv[u][1] = vh[1] ? row[lane + 64] : 0.f;
-#pragma unroll
-        for (int n = 0; n < NMAXF; ++n) {
-          if (n < N) {
-            const float rn = row[a.roff + n], dn = row[a.roff + N + n];
-            xa[u][0] += rn * hreg[0][n];
-            xa[u][1] += rn * hreg[1][n];
-            xb[u][0] += dn * hreg[0][n];
-            xb[u][1] += dn * hreg[1][n];
-          }
-        }
-      }
-      if (vh[0]) Wb[u * NLMp + lane] = Wv[u][0];
-      if (vh[1]) Wb[u * NLMp + lane + 64] = Wv[u][1];
+  for (int j0 = 1; j0 < nn; j0 += APASS) {
+    const int cnt = min(APASS, nn - j0);
+    {
+      const float4* src = reinterpret_cast<const float4*>(a.basis + (b0 + j0) * ldb);
+      float4* dst = reinterpret_cast<float4*>(rows);
+      const int n4 = cnt * ldb / 4;
+      for (int t = lane; t < n4; t += 64) dst[t] = src[t];
     }
-    wave_sync();  // the other buffer is written next, so one wait per chunk suffices
+    // per-pair scalars of the pass, one lane per pair
+    float pr = 1.f, px = 0.f, py = 0.f, pz = 0.f;
+    int ps = 0;
+    if (lane < cnt) {
+      const int64_t p = b0 + j0 + lane;
+      pr = (float)a.pr_r[p];
+      px = (float)a.pr_xyz[3 * p];
+      py = (float)a.pr_xyz[3 * p + 1];
+      pz = (float)a.pr_xyz[3 * p + 2];
+      ps = a.pr_spec[p];
+      if (ps < 0) ps = 0;  // its row is zero: any species' damping will do
+    }
+    wave_sync();
+    for (int u = 0; u < cnt; ++u) {
+      const float* row = rows + u * ldb;
+      const float r = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, pr), u));
+      const int s = __builtin_amdgcn_readlane(ps, u);
+      const float inv_r = 1.f / r;
+      float xa[2] = {0.f, 0.f}, xb[2] = {0.f, 0.f};
 #pragma unroll
-    for (int u = 0; u < PCH; ++u) {
-      if (!ok[u]) continue;
-      const int64_t p = b0 + j0 + u;
-      const float r = (float)a.pr_r[p], inv_r = 1.f / r;
-      const int s = a.pr_spec[p];
+      for (int n = 0; n < NMAXF; ++n) {
+        const float rn = row[roff + n], dn = row[roff + NMAXF + n];
+        xa[0] += rn * hreg[0][n];
+        xa[1] += rn * hreg[1][n];
+        xb[0] += dn * hreg[0][n];
+        xb[1] += dn * hreg[1][n];
+      }
       const float sgt = (float)P.sp[s].sig_t, sgs = (float)P.sp[s].sig_t_sc;
       const float stv = sgt * (1.f + sgs * r), dst = sgt * sgs;
       const float kk = stv * stv * 0.5f * inv_r * inv_r;
       const float dk = stv * stv * inv_r * inv_r * inv_r - stv * dst * inv_r * inv_r;
-      const float* Wu = Wb + u * NLMp;
       float T = 0.f, V0 = 0.f, V1 = 0.f, V2 = 0.f;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const float lf = (float)lh[h];
-        T += (lf * (lf + 1.f) * dk * xa[u][h] + xb[u][h] - lf * xa[u][h] * inv_r) * Wv[u][h];
-        const float xr = xa[u][h] * __expf(-2.f * lf * kk);
+        const float Wv = vh[h] ? row[lane + 64 * h] : 0.f;
+        T += (lf * (lf + 1.f) * dk * xa[h] + xb[h] - lf * xa[h] * inv_r) * Wv;
+        const float xr = xa[h] * __expf(-2.f * lf * kk);
         float g[3] = {0.f, 0.f, 0.f};
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
           for (int t = 0; t < KLOW; ++t)
-            if (li[h][c][t] >= 0) g[c] += lc[h][c][t] * Wu[li[h][c][t]];
+            if (li[h][c][t] >= 0) g[c] += lc[h][c][t] * row[li[h][c][t]];
         V0 += xr * g[0];
         V1 += xr * g[1];
         V2 += xr * g[2];
@@ -484,18 +490,26 @@ __global__ __launch_bounds__(64 * WAVES) void k_site_adjoint(SoapParams P, F32Ar
       V1 = wave_total(V1);
       V2 = wave_total(V2);
       if (lane == 0) {
-        const float ux = (float)a.pr_xyz[3 * p] * inv_r, uy = (float)a.pr_xyz[3 * p + 1] * inv_r,
-                    uz = (float)a.pr_xyz[3 * p + 2] * inv_r;
+        const float ux = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, px), u)) * inv_r;
+        const float uy = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, py), u)) * inv_r;
+        const float uz = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, pz), u)) * inv_r;
         const float f0 = ux * T + V0 * inv_r, f1 = uy * T + V1 * inv_r, f2 = uz * T + V2 * inv_r;
-        const int64_t gi = a.pr_gidx[p];
-        fglob[3 * gi] += (double)f0;
-        fglob[3 * gi + 1] += (double)f1;
-        fglob[3 * gi + 2] += (double)f2;
+        fb[3 * u] = f0;
+        fb[3 * u + 1] = f1;
+        fb[3 * u + 2] = f2;
         f_self0 += f0;
         f_self1 += f1;
         f_self2 += f2;
       }
     }
+    wave_sync();
+    if (lane < cnt) {
+      const int64_t gi = a.pr_gidx[b0 + j0 + lane];
+      fglob[3 * gi] += (double)fb[3 * lane];
+      fglob[3 * gi + 1] += (double)fb[3 * lane + 1];
+      fglob[3 * gi + 2] += (double)fb[3 * lane + 2];
+    }
+    wave_sync();  // rows and fb are rewritten by the next pass
   }
   if (lane == 0) {
     const int64_t gi = a.pr_gidx[b0];
@@ -588,7 +602,7 @@ static F32Args f32_args(tgx_model_t* m) {
   a.rtab = m->rtab.as<float>();
   a.K = KTAB;
   a.roff = (m->P.NLM + 3) & ~3;
-  a.ldb = a.roff + 2 * m->P.N;
+  a.ldb = a.roff + 2 * NMAXF;  // 16-byte aligned rows: R and dR load as whole vectors
   a.basis = m->pbasis.as<float>();
   return a;
 }
@@ -623,7 +637,7 @@ int tgx_adjoint_f32(tgx_ctx* ctx, tgx_model_t* m) {
   F32Args a = f32_args(m);
   TGX_REQUIRE(m->pbasis.p && m->pbasis.bytes >= sizeof(float) * (size_t)m->n_pairs_b * a.ldb,
               "FP32 adjoint: the pair basis of the block was not built");
-  const size_t lds = sizeof(float) * (size_t)WAVES * adj_wave_floats(m->P);
+  const size_t lds = sizeof(float) * (size_t)WAVES * adj_wave_floats(m->P, a.ldb);
   TGX_REQUIRE(lds <= 160 * 1024, "FP32 adjoint kernel needs %zu bytes of LDS", lds);
   TGX_LAUNCH(ctx, "soap_adjoint_f32", k_site_adjoint, dim3((ns + WAVES - 1) / WAVES), dim3(64 * WAVES), lds, m->P, a,
              ns, m->low_idx.as<int>(), m->low_coef.as<float>(), m->q32.as<float>(), m->pnorm.as<float>(),
