@@ -3,6 +3,7 @@
 #include <cmath>
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
 
 #include "tgx_internal.h"
 #include "tgx_util.h"
@@ -103,7 +104,8 @@ static void free_model(tgx_model_t* m) {
   dbuf* bufs[] = {&m->B, &m->xq, &m->cen, &m->dtab, &m->ynorm, &m->rtab, &m->Qs, &m->QsT, &m->alpha, &m->Qs_bf, &m->QsT_bf, &m->alpha_f,
                   &m->site_atom, &m->site_beg, &m->pr_gidx, &m->pr_nbr, &m->pr_spec, &m->pr_r, &m->pr_xyz, &m->q64,
                   &m->q32, &m->qbf, &m->pnorm, &m->e_site, &m->qss64, &m->qss32, &m->kd64, &m->kdbf, &m->epart,
-                  &m->soap_der, &m->fpair, &m->pbasis, &m->low_idx, &m->low_coef, &m->gate_pn, &m->gate_e};
+                  &m->soap_der, &m->fpair, &m->pbasis, &m->low_idx, &m->low_coef, &m->gate_pn, &m->gate_e,
+                  &m->gate_stat, &m->gate_list};
   for (dbuf* b : bufs) b->release();
   delete m;
 }
@@ -122,6 +124,7 @@ extern "C" int tgx_finalize(tgx_ctx* ctx) {
                   &ctx->virial_part, &ctx->scratch, &ctx->red, &ctx->vel, &ctx->mass, &ctx->forces_prev,
                   &ctx->pos_prev, &ctx->e_members, &ctx->gate_mask, &ctx->cell_pos};
   for (dbuf* b : bufs) b->release();
+  if (ctx->gate_host) (void)hipHostFree(ctx->gate_host);
   for (auto& r : ctx->prof) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
@@ -589,7 +592,8 @@ extern "C" int tgx_get_soap_energy_and_forces(tgx_ctx* ctx, int n_sites, int n_s
 }
 
 // ---- whole step ------------------------------------------------------------------------
-int tgx_surrogate_run(tgx_ctx* ctx, tgx_model_t* m);
+int tgx_surrogate_eval(tgx_ctx* ctx, tgx_model_t* m);
+int tgx_surrogate_finish(tgx_ctx* ctx, tgx_model_t* m, int n_gated);
 
 extern "C" int tgx_compute(tgx_ctx* ctx, int path, double* energy, double* virial) {
   TGX_CTX_CHECK(ctx);
@@ -598,13 +602,42 @@ extern "C" int tgx_compute(tgx_ctx* ctx, int path, double* energy, double* viria
   TGX_TRY(begin_step(ctx));
   ctx->gate_sites = ctx->gate_gated = 0;
   ctx->gated_fraction = ctx->max_spread = 0.0;
-  for (auto* m : ctx->models) {
-    if (path == TGX_PATH_SURROGATE) {
-      TGX_TRY(tgx_surrogate_run(ctx, m));
-    } else {
-      TGX_TRY(run_block(ctx, m, path, true));
+  if (path == TGX_PATH_SURROGATE) {
+    // every block's network first, then one readback of the gate counters
+    const int nm = (int)ctx->models.size();
+    if (ctx->gate_host_n < 2 * nm) {
+      if (ctx->gate_host) (void)hipHostFree(ctx->gate_host);
+      ctx->gate_host = nullptr;
+      ctx->gate_host_n = 0;
+      TGX_CHECK_HIP(hipHostMalloc((void**)&ctx->gate_host, sizeof(int) * 2 * nm, hipHostMallocDefault));
+      ctx->gate_host_n = 2 * nm;
     }
-    TGX_TRY(tgx_energy_scatter(ctx, m));
+    for (int k = 0; k < nm; ++k) {
+      tgx_model_t* m = ctx->models[k];
+      TGX_TRY(tgx_surrogate_eval(ctx, m));
+      ctx->gate_host[2 * k] = ctx->gate_host[2 * k + 1] = 0;
+      if (m->n_sites_b > 0 && ctx->gate_members > 1)
+        TGX_CHECK_HIP(hipMemcpyAsync(ctx->gate_host + 2 * k, m->gate_stat.p, sizeof(int) * 2, hipMemcpyDeviceToHost,
+                                     ctx->stream));
+    }
+    TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < nm; ++k) {
+      tgx_model_t* m = ctx->models[k];
+      const int ng = ctx->gate_host[2 * k];
+      float sp;
+      std::memcpy(&sp, &ctx->gate_host[2 * k + 1], sizeof(float));
+      ctx->gate_sites += m->n_sites_b;
+      ctx->gate_gated += ng;
+      ctx->max_spread = std::max(ctx->max_spread, (double)sp);
+      TGX_TRY(tgx_surrogate_finish(ctx, m, ng));
+      TGX_TRY(tgx_energy_scatter(ctx, m));
+    }
+    ctx->gated_fraction = ctx->gate_sites > 0 ? (double)ctx->gate_gated / (double)ctx->gate_sites : 0.0;
+  } else {
+    for (auto* m : ctx->models) {
+      TGX_TRY(run_block(ctx, m, path, true));
+      TGX_TRY(tgx_energy_scatter(ctx, m));
+    }
   }
   TGX_TRY(tgx_gather_forces(ctx, energy, virial));
   return TGX_OK;

@@ -354,10 +354,9 @@ int tgx_gap_contract(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces) {
     e.ldo = Sp;
     TGX_TRY(launch_f64(ctx, EPI_GAP, Mp, Sp, Dp, m->q64.as<double>(), Dp, m->Qs.as<double>(), Dp, e, "gemm_kernel_f64"));
   } else {
-    TGX_TRY(m->qbf.ensure(sizeof(__bf16) * (size_t)Mp * Dp));
+    // qbf (BF16 descriptors) comes from the FP32 descriptor pass
+    TGX_REQUIRE(m->qbf.bytes >= sizeof(__bf16) * (size_t)Mp * Dp, "gemm_bf16: BF16 descriptors missing");
     TGX_TRY(m->kdbf.ensure(sizeof(__bf16) * (size_t)Mp * Sp));
-    const int64_t nq = (int64_t)Mp * Dp;
-    TGX_LAUNCH(ctx, "q_to_bf16", k_f32_to_bf16, dim3(2048), dim3(256), 0, nq, m->q32.as<float>(), m->qbf.as<__bf16>());
     e.alpha_f = m->alpha_f.as<float>();
     e.out = m->kdbf.p;
     e.ldo = Sp;

@@ -139,6 +139,7 @@ struct tgx_model_t {
   dbuf epart;       // energy partials [n_sites_pad][n_tiles]
   dbuf soap_der;    // [n_pairs_b][D][3] FP64 (plain path)
   dbuf gate_pn, gate_e;  // surrogate fallback: |p| of the gated sites, saved energies
+  dbuf gate_stat, gate_list;  // surrogate gate: {count, max spread bits}, gated sites
   dbuf pbasis;      // [n_pairs_b][ldb] FP32 pair basis: A_l Y_lm | R_n | dR_n (FP32 paths)
   dbuf fpair;       // [n_pairs_b][3] pair forces
 };
@@ -174,6 +175,8 @@ struct tgx_ctx {
   double gate_threshold = 1e30;
   double gated_fraction = 0.0, max_spread = 0.0;  // over every block of the last tgx_compute
   int64_t gate_sites = 0, gate_gated = 0;
+  int* gate_host = nullptr;  // pinned: {count, max spread bits} of every block
+  int gate_host_n = 0;
   dbuf e_members, gate_mask;
   // profiling
   bool profiling = false;

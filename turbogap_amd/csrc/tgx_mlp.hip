@@ -288,10 +288,14 @@ __global__ void k_append(int n_take, int stride, int offset, int Dp, const doubl
 }
 
 // Surrogate output: member energies e_k = shift_k + scale_k E_k (FP64),
-// e_site = mean_k e_k, spread = max_k e_k - min_k e_k. Em is [K][ldm].
+// e_site = mean_k e_k, spread = max_k e_k - min_k e_k. Em is [K][ldm]. With more
+// than one member, sites whose spread exceeds the threshold are appended to
+// gated[] (stat[0] counts them; the order of the list does not matter, every
+// listed site is re-evaluated on its own) and stat[1] keeps the largest spread
+// (as the bits of a non-negative float, which order like the values).
 __global__ void k_surrogate_out(int n, int K, const float* __restrict__ Em, int ldm, const double* __restrict__ shift,
-                                const double* __restrict__ scale, double* __restrict__ e_site,
-                                float* __restrict__ spread) {
+                                const double* __restrict__ scale, double* __restrict__ e_site, float threshold,
+                                int* __restrict__ stat, int* __restrict__ gated) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0.0, lo = 1e300, hi = -1e300;
@@ -302,7 +306,11 @@ __global__ void k_surrogate_out(int n, int K, const float* __restrict__ Em, int 
     hi = v > hi ? v : hi;
   }
   e_site[i] = s / K;
-  spread[i] = (float)(hi - lo);
+  if (K > 1) {
+    const float sp = (float)(hi - lo);
+    atomicMax(&stat[1], __float_as_int(sp));
+    if (sp > threshold) gated[atomicAdd(&stat[0], 1)] = i;
+  }
 }
 
 // Second layer of every ensemble member, one wavefront per (row, member):
@@ -945,14 +953,15 @@ extern "C" int tgx_surrogate_stats(tgx_ctx* ctx, double* gated_fraction, double*
   return TGX_OK;
 }
 
-// Surrogate energies and gradients for one block; forces follow through the
-// FP32 adjoint kernel. Gated sites are re-evaluated by the FP64 GAP.
+// First phase of a block: FP32 descriptors, the ensemble's energies and
+// gradients, and the device-side gate marks. tgx_surrogate_finish completes the
+// block once the caller has read the number of gated sites.
 //   L1    : [h1 | s1] = silu / silu' (q W1cat^T + b1)       one GEMM, N = K H
 //   L2    : z2_k = h1_k W2_k^T                              batched over k
 //   act2  : E_k, a2_k = w3_k silu'(z2_k + b2_k)
 //   back2 : a1_k = -(scale_k/K) (a2_k W2_k) o s1_k           batched over k
 //   back1 : Qss = a1 W1Tcat^T = -(1/K) sum_k scale_k dE_k/dq  one GEMM, depth K H
-int tgx_surrogate_run(tgx_ctx* ctx, tgx_model_t* m) {
+int tgx_surrogate_eval(tgx_ctx* ctx, tgx_model_t* m) {
   int model_id = -1;
   for (int k = 0; k < (int)ctx->models.size(); ++k)
     if (ctx->models[k] == m) model_id = k;
@@ -971,13 +980,12 @@ int tgx_surrogate_run(tgx_ctx* ctx, tgx_model_t* m) {
   TGX_TRY(ensemble_pack(ctx, sl, K, Dp));
   const EnsemblePack& pk = sl.pack;
   const int KH = K * H;
-  TGX_TRY(m->qbf.ensure(sizeof(__bf16) * (size_t)Mp * Dp));
+  TGX_REQUIRE(m->qbf.bytes >= sizeof(__bf16) * (size_t)Mp * Dp, "surrogate: BF16 descriptors missing");
   TGX_TRY(m->qss32.ensure(sizeof(float) * (size_t)Mp * Dp));
   TGX_TRY(m->e_site.ensure(sizeof(double) * (size_t)Mp));
-  TGX_TRY(tgx_f32_to_bf16(ctx, m->q32.as<float>(), m->qbf.as<__bf16>(), (int64_t)Mp * Dp));
-  // work space in the context scratch: s1, z2 (FP32 Mp x K H), h1b, a2b, a1b (BF16 Mp x K H), Em (K x Mp), spread
+  // work space in the context scratch: s1, z2 (FP32 Mp x K H), h1b, a2b, a1b (BF16 Mp x K H), Em (K x Mp)
   const size_t fh = (size_t)Mp * KH;
-  const size_t bytes = sizeof(float) * (2 * fh + (size_t)K * Mp + Mp) + sizeof(__bf16) * 3 * fh + 8 * 256;
+  const size_t bytes = sizeof(float) * (2 * fh + (size_t)K * Mp) + sizeof(__bf16) * 3 * fh + 8 * 256;
   TGX_TRY(ctx->e_members.ensure(bytes));
   char* p = ctx->e_members.as<char>();
   auto take = [&](size_t n) { char* r = p; p += tgx_round_up((int64_t)n, 256); return r; };
@@ -987,7 +995,6 @@ int tgx_surrogate_run(tgx_ctx* ctx, tgx_model_t* m) {
   __bf16* a2b = (__bf16*)take(sizeof(__bf16) * fh);
   __bf16* a1b = (__bf16*)take(sizeof(__bf16) * fh);
   float* Em = (float*)take(sizeof(float) * K * Mp);
-  float* spread = (float*)take(sizeof(float) * Mp);
 
   tgx_gemm_desc l1;
   l1.epi = TGX_EPI_SILU;
@@ -1026,46 +1033,40 @@ int tgx_surrogate_run(tgx_ctx* ctx, tgx_model_t* m) {
   b1.C = m->qss32.p; b1.ldc = Dp;
   TGX_TRY(tgx_gemm_bf16(ctx, "mlp_back1", b1));
 
+  TGX_TRY(m->gate_stat.ensure(sizeof(int) * 2));
+  TGX_TRY(m->gate_list.ensure(sizeof(int) * (size_t)Mp));
+  TGX_TRY(tgx_zero(ctx, m->gate_stat.p, sizeof(int) * 2));
   TGX_LAUNCH(ctx, "mlp_out", k_surrogate_out, g1(ns), dim3(256), 0, ns, K, Em, Mp, pk.shift.as<double>(),
-             pk.scale.as<double>(), m->e_site.as<double>(), spread);
-  // ensemble gate (statistics accumulate over the blocks of one tgx_compute)
-  if (K > 1) {
-    std::vector<float> sp((size_t)ns);
-    TGX_CHECK_HIP(hipMemcpyAsync(sp.data(), spread, sizeof(float) * ns, hipMemcpyDeviceToHost, ctx->stream));
-    TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-    std::vector<int> gated;
-    for (int i = 0; i < ns; ++i) {
-      ctx->max_spread = std::max(ctx->max_spread, (double)sp[i]);
-      if (sp[i] > ctx->gate_threshold) gated.push_back(i);
-    }
-    ctx->gate_sites += ns;
-    ctx->gate_gated += (int64_t)gated.size();
-    ctx->gated_fraction = (double)ctx->gate_gated / (double)ctx->gate_sites;
-    if (!gated.empty()) {
-      const int ng = (int)gated.size(), ngp = (int)tgx_round_up(ng, 128);
-      TGX_TRY(ctx->gate_mask.ensure(sizeof(int) * (size_t)ngp));
-      TGX_CHECK_HIP(hipMemcpyAsync(ctx->gate_mask.p, gated.data(), sizeof(int) * ng, hipMemcpyHostToDevice, ctx->stream));
-      TGX_TRY(m->q64.ensure(sizeof(double) * (size_t)ngp * Dp));
-      TGX_TRY(tgx_zero(ctx, m->q64.p, sizeof(double) * (size_t)ngp * Dp));
-      // FP64 descriptors and GAP of the gated sites only
-      dbuf& pn = m->gate_pn;
-      TGX_TRY(pn.ensure(sizeof(double) * (size_t)ngp));
-      TGX_TRY(tgx_soap_descriptors_sites(ctx, m, ctx->gate_mask.as<int>(), ng, m->q64.as<double>(), pn.as<double>()));
-      const int ns_save = m->n_sites_b;
-      dbuf& e_keep = m->gate_e;
-      TGX_TRY(e_keep.ensure(sizeof(double) * (size_t)Mp));
-      TGX_CHECK_HIP(hipMemcpyAsync(e_keep.p, m->e_site.p, sizeof(double) * ns, hipMemcpyDeviceToDevice, ctx->stream));
-      m->n_sites_b = ng;
-      int st = tgx_gap_contract(ctx, m, TGX_PATH_FP64_ADJOINT, true);
-      m->n_sites_b = ns_save;
-      if (st == TGX_OK) {
-        // e_site now holds the gated sites' GAP energies in its first ng rows
-        TGX_LAUNCH(ctx, "gate_replace", k_gate_replace, dim3(ng), dim3(256), 0, ng, ctx->gate_mask.as<int>(), Dp,
-                   m->e_site.as<double>(), m->qss64.as<double>(), e_keep.as<double>(), m->qss32.as<float>());
-        TGX_CHECK_HIP(hipMemcpyAsync(m->e_site.p, e_keep.p, sizeof(double) * ns, hipMemcpyDeviceToDevice, ctx->stream));
-      }
-      TGX_TRY(st);
-    }
+             pk.scale.as<double>(), m->e_site.as<double>(), (float)ctx->gate_threshold, m->gate_stat.as<int>(),
+             m->gate_list.as<int>());
+  return TGX_OK;
+}
+
+// Second phase of a block: the ng gated sites (listed on the device by
+// tgx_surrogate_eval) take the FP64 GAP energy and gradient, then the adjoint
+// kernel turns the gradients into pair forces.
+int tgx_surrogate_finish(tgx_ctx* ctx, tgx_model_t* m, int ng) {
+  const int ns = m->n_sites_b;
+  if (ns == 0) return TGX_OK;
+  const int Mp = (int)tgx_round_up(ns, 128), Dp = m->Dpad;
+  if (ng > 0) {
+    const int ngp = (int)tgx_round_up(ng, 128);
+    TGX_TRY(m->q64.ensure(sizeof(double) * (size_t)ngp * Dp));
+    // FP64 descriptors and GAP of the gated sites only
+    dbuf& pn = m->gate_pn;
+    TGX_TRY(pn.ensure(sizeof(double) * (size_t)ngp));
+    TGX_TRY(tgx_soap_descriptors_sites(ctx, m, m->gate_list.as<int>(), ng, m->q64.as<double>(), pn.as<double>()));
+    dbuf& e_keep = m->gate_e;
+    TGX_TRY(e_keep.ensure(sizeof(double) * (size_t)Mp));
+    TGX_CHECK_HIP(hipMemcpyAsync(e_keep.p, m->e_site.p, sizeof(double) * ns, hipMemcpyDeviceToDevice, ctx->stream));
+    m->n_sites_b = ng;
+    int st = tgx_gap_contract(ctx, m, TGX_PATH_FP64_ADJOINT, true);
+    m->n_sites_b = ns;
+    TGX_TRY(st);
+    // e_site now holds the gated sites' GAP energies in its first ng rows
+    TGX_LAUNCH(ctx, "gate_replace", k_gate_replace, dim3(ng), dim3(256), 0, ng, m->gate_list.as<int>(), Dp,
+               m->e_site.as<double>(), m->qss64.as<double>(), e_keep.as<double>(), m->qss32.as<float>());
+    TGX_CHECK_HIP(hipMemcpyAsync(m->e_site.p, e_keep.p, sizeof(double) * ns, hipMemcpyDeviceToDevice, ctx->stream));
   }
   // descriptors the adjoint kernel reads must be the FP32 ones of every site
   return tgx_adjoint_pair_forces(ctx, m, TGX_PATH_SURROGATE);

@@ -577,8 +577,13 @@ int tgx_soap_descriptors(tgx_ctx* ctx, tgx_model_t* m, int path, bool with_der) 
     }
   } else {
     TGX_TRY(m->q32.ensure(sizeof(float) * nsp * m->Dpad));
-    if (m->P.N <= 16) TGX_TRY(tgx_desc_f32(ctx, m));
-    else TGX_TRY(tgx_desc_wave(ctx, m, false, nullptr, ns, m->q32.p, m->pnorm.p));
+    if (m->P.N <= 16) {
+      TGX_TRY(tgx_desc_f32(ctx, m));  // writes the BF16 copy as well
+    } else {
+      TGX_TRY(tgx_desc_wave(ctx, m, false, nullptr, ns, m->q32.p, m->pnorm.p));
+      TGX_TRY(m->qbf.ensure(sizeof(__bf16) * nsp * m->Dpad));
+      TGX_TRY(tgx_f32_to_bf16(ctx, m->q32.as<float>(), m->qbf.as<__bf16>(), (int64_t)nsp * m->Dpad));
+    }
   }
   return TGX_OK;
 }

@@ -300,7 +300,8 @@ __host__ __device__ inline int adj_rows_floats(const SoapParams& P, int ldb) {
 __host__ __device__ inline int adj_wave_floats(const SoapParams& P, int ldb) { return adj_rows_floats(P, ldb) + 3 * APASS; }
 
 __global__ __launch_bounds__(64 * WAVES) void k_site_desc(SoapParams P, F32Args a, int n_sites,
-                                                          float* __restrict__ q_out, float* __restrict__ pnorm_out) {
+                                                          float* __restrict__ q_out, __bf16* __restrict__ qb_out,
+                                                          float* __restrict__ pnorm_out) {
   extern __shared__ __align__(16) float smf[];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int site = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES + w);
@@ -322,7 +323,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_site_desc(SoapParams P, F32Args 
   wave_sync();
   const float nrm = power(P, C, X);
   float* qrow = q_out + (size_t)site * P.Dpad;
-  for (int d = lane; d < P.Dpad; d += 64) qrow[d] = d < P.D ? X[d] : 0.f;
+  __bf16* qbrow = qb_out + (size_t)site * P.Dpad;
+  for (int d = lane; d < P.Dpad; d += 64) {
+    const float v = d < P.D ? X[d] : 0.f;
+    qrow[d] = v;
+    qbrow[d] = (__bf16)v;  // the BF16 operand of the contraction / network input
+  }
   if (lane == 0) pnorm_out[site] = nrm;
 }
 
@@ -623,8 +629,9 @@ int tgx_desc_f32(tgx_ctx* ctx, tgx_model_t* m) {
   }
   const size_t lds = sizeof(float) * (size_t)WAVES * desc_wave_floats(m->P);
   TGX_REQUIRE(lds <= 160 * 1024, "FP32 descriptor kernel needs %zu bytes of LDS", lds);
+  TGX_TRY(m->qbf.ensure(sizeof(__bf16) * (size_t)tgx_round_up(ns, 128) * m->Dpad));
   TGX_LAUNCH(ctx, "soap_desc_f32", k_site_desc, dim3((ns + WAVES - 1) / WAVES), dim3(64 * WAVES), lds, m->P, a, ns,
-             m->q32.as<float>(), m->pnorm.as<float>());
+             m->q32.as<float>(), m->qbf.as<__bf16>(), m->pnorm.as<float>());
   return TGX_OK;
 }
 
