@@ -52,6 +52,7 @@ struct F32Args {
   int K;
   float* basis;        // [pairs][ldb]: W (NLM, padded to roff) | R (16 slots) | dR (16 slots)
   int ldb, roff;
+  int split8;          // every species has <= 8 radial functions, stacked at 0 or 8
 };
 
 struct SpecLite {
@@ -129,7 +130,8 @@ __device__ __forceinline__ void damping(const SpecLite& sp, float r, float (&A)[
 }
 
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
-typedef float f32x16_t __attribute__((ext_vector_type(16), aligned(4)));  // rows are only dword-aligned in general
+typedef float f32x16_t __attribute__((ext_vector_type(16), aligned(4)));
+typedef float f32x8_t __attribute__((ext_vector_type(8), aligned(4)));  // rows are only dword-aligned in general
 
 // Power spectrum p(n, n', l) = w sum_m c(n, lm) c(n', lm) (w = sqrt(2) for
 // n != n') and q = p/|p| into X[D]. For every l the 16 x 16 Gram matrix
@@ -248,6 +250,8 @@ __global__ __launch_bounds__(PB) void k_pair_basis(SoapParams P, F32Args a, int6
 }
 
 // c(n, lm) of a site in registers, lanes over lm (lm = lane and lane + 64).
+// SPLIT allows the species-split accumulation (fewer FMAs, more registers).
+template <bool SPLIT>
 __device__ __forceinline__ void site_coefficients(const SoapParams& P, const F32Args& a, int64_t b0, int nn, int lane,
                                                   float (&acc)[2][NMAXF]) {
 #pragma unroll
@@ -276,14 +280,37 @@ __device__ __forceinline__ void site_coefficients(const SoapParams& P, const F32
       w0[u] = ok && v0 ? row[lane] : 0.f;
       w1[u] = ok && v1 ? row[lane + 64] : 0.f;
     }
+    if (SPLIT && a.split8) {
+      // at most 8 radial functions per species, stacked at 0 or 8: each pair
+      // feeds only its species' half, selected by an unrolled species loop so
+      // that every accumulator index stays static
+      int off[CCH];
 #pragma unroll
-    for (int u = 0; u < CCH; ++u) {
-      if (j0 + u < nn) {
-        const f32x16_t rv = *reinterpret_cast<const f32x16_t*>(a.basis + (b0 + j0 + u) * a.ldb + a.roff);
+      for (int u = 0; u < CCH; ++u) off[u] = (j0 + u < nn && a.pr_spec[b0 + j0 + u] > 0) ? 8 : 0;
 #pragma unroll
-        for (int n = 0; n < NMAXF; ++n) {
-          acc[0][n] += rv[n] * w0[u];
-          acc[1][n] += rv[n] * w1[u];
+      for (int sh = 0; sh < 2; ++sh) {
+#pragma unroll
+        for (int u = 0; u < CCH; ++u) {
+          if (j0 + u < nn && off[u] == 8 * sh) {
+            const f32x8_t rv = *reinterpret_cast<const f32x8_t*>(a.basis + (b0 + j0 + u) * a.ldb + a.roff + 8 * sh);
+#pragma unroll
+            for (int n = 0; n < 8; ++n) {
+              acc[0][8 * sh + n] += rv[n] * w0[u];
+              acc[1][8 * sh + n] += rv[n] * w1[u];
+            }
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < CCH; ++u) {
+        if (j0 + u < nn) {
+          const f32x16_t rv = *reinterpret_cast<const f32x16_t*>(a.basis + (b0 + j0 + u) * a.ldb + a.roff);
+#pragma unroll
+          for (int n = 0; n < NMAXF; ++n) {
+            acc[0][n] += rv[n] * w0[u];
+            acc[1][n] += rv[n] * w1[u];
+          }
         }
       }
     }
@@ -297,7 +324,7 @@ __host__ __device__ inline int adj_rows_floats(const SoapParams& P, int ldb) {
   const int r = APASS * ldb;
   return P.D > r ? ((P.D + 3) & ~3) : r;
 }
-__host__ __device__ inline int adj_wave_floats(const SoapParams& P, int ldb) { return adj_rows_floats(P, ldb) + 3 * APASS; }
+__host__ __device__ inline int adj_wave_floats(const SoapParams& P, int ldb) { return adj_rows_floats(P, ldb) + 16 * APASS; }
 
 __global__ __launch_bounds__(64 * WAVES) void k_site_desc(SoapParams P, F32Args a, int n_sites,
                                                           float* __restrict__ q_out, __bf16* __restrict__ qb_out,
@@ -311,7 +338,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_site_desc(SoapParams P, F32Args 
   const int64_t b0 = a.site_beg[site];
   const int nn = (int)(a.site_beg[site + 1] - b0);
   float acc[2][NMAXF];
-  site_coefficients(P, a, b0, nn, lane, acc);
+  site_coefficients<false>(P, a, b0, nn, lane, acc);
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int lm = lane + 64 * h;
@@ -330,6 +357,16 @@ __global__ __launch_bounds__(64 * WAVES) void k_site_desc(SoapParams P, F32Args 
     qbrow[d] = (__bf16)v;  // the BF16 operand of the contraction / network input
   }
   if (lane == 0) pnorm_out[site] = nrm;
+}
+
+// Sum over each 16-lane row, left in the row's last lane (lanes 15, 31, 47, 63):
+// prefix sums by DPP row shifts.
+__device__ __forceinline__ float row_sum16(float v) {
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x111, 0xf, 0xf, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x112, 0xf, 0xf, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x114, 0xf, 0xf, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x118, 0xf, 0xf, false));
+  return v;
 }
 
 // Sum over the wavefront of a value, available in every lane: DPP row shifts
@@ -363,7 +400,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_site_adjoint(SoapParams P, F32Ar
   const int64_t b0 = a.site_beg[site];
   const int nn = (int)(a.site_beg[site + 1] - b0);
   float hreg[2][NMAXF];
-  site_coefficients(P, a, b0, nn, lane, hreg);
+  site_coefficients<true>(P, a, b0, nn, lane, hreg);
   // g = (Qss - (Qss . q) q) / |p|
   const float* qrow = q_in + (size_t)site * P.Dpad;
   const float* srow = qss + (size_t)site * P.Dpad;
@@ -429,12 +466,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_site_adjoint(SoapParams P, F32Ar
   // Pair passes: the rows of up to APASS consecutive pairs are copied into LDS
   // (over X, which H no longer needs) by one coalesced sweep, so each pass costs
   // one memory round trip. Pairs without a contribution (other species, beyond
-  // their cutoff) have all-zero rows and come out with a zero force.
+  // their cutoff) have all-zero rows and come out with a zero force. Each pair's
+  // four lane sums (T, V) are reduced within 16-lane rows by DPP and the four
+  // row partials go to LDS; at the end of the pass lane u finishes pair u.
   wave_sync();
   float* rows = X;
-  float* fb = X + adj_rows_floats(P, a.ldb);  // [APASS][3] pair forces of the pass
+  float* part = X + adj_rows_floats(P, a.ldb);  // [APASS][4 values][4 rows]
   const int ldb = a.ldb, roff = a.roff;
-  double f_self0 = 0.0, f_self1 = 0.0, f_self2 = 0.0;
+  float fs0 = 0.f, fs1 = 0.f, fs2 = 0.f;  // this lane's share of the self-pair force
   for (int j0 = 1; j0 < nn; j0 += APASS) {
     const int cnt = min(APASS, nn - j0);
     {
@@ -462,13 +501,37 @@ __global__ __launch_bounds__(64 * WAVES) void k_site_adjoint(SoapParams P, F32Ar
       const int s = __builtin_amdgcn_readlane(ps, u);
       const float inv_r = 1.f / r;
       float xa[2] = {0.f, 0.f}, xb[2] = {0.f, 0.f};
+      if (a.split8) {
+        // at most 8 radial functions per species, stacked at 0 or 8: half the FMAs
+        const int off = P.sp[s].off;
+        const float* rr = row + roff + off;
+        const float* dd = row + roff + NMAXF + off;
+        if (off == 0) {
 #pragma unroll
-      for (int n = 0; n < NMAXF; ++n) {
-        const float rn = row[roff + n], dn = row[roff + NMAXF + n];
-        xa[0] += rn * hreg[0][n];
-        xa[1] += rn * hreg[1][n];
-        xb[0] += dn * hreg[0][n];
-        xb[1] += dn * hreg[1][n];
+          for (int n = 0; n < 8; ++n) {
+            xa[0] += rr[n] * hreg[0][n];
+            xa[1] += rr[n] * hreg[1][n];
+            xb[0] += dd[n] * hreg[0][n];
+            xb[1] += dd[n] * hreg[1][n];
+          }
+        } else {
+#pragma unroll
+          for (int n = 0; n < 8; ++n) {
+            xa[0] += rr[n] * hreg[0][8 + n];
+            xa[1] += rr[n] * hreg[1][8 + n];
+            xb[0] += dd[n] * hreg[0][8 + n];
+            xb[1] += dd[n] * hreg[1][8 + n];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int n = 0; n < NMAXF; ++n) {
+          const float rn = row[roff + n], dn = row[roff + NMAXF + n];
+          xa[0] += rn * hreg[0][n];
+          xa[1] += rn * hreg[1][n];
+          xb[0] += dn * hreg[0][n];
+          xb[1] += dn * hreg[1][n];
+        }
       }
       const float sgt = (float)P.sp[s].sig_t, sgs = (float)P.sp[s].sig_t_sc;
       const float stv = sgt * (1.f + sgs * r), dst = sgt * sgs;
@@ -491,37 +554,46 @@ __global__ __launch_bounds__(64 * WAVES) void k_site_adjoint(SoapParams P, F32Ar
         V1 += xr * g[1];
         V2 += xr * g[2];
       }
-      T = wave_total(T);
-      V0 = wave_total(V0);
-      V1 = wave_total(V1);
-      V2 = wave_total(V2);
-      if (lane == 0) {
-        const float ux = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, px), u)) * inv_r;
-        const float uy = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, py), u)) * inv_r;
-        const float uz = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, pz), u)) * inv_r;
-        const float f0 = ux * T + V0 * inv_r, f1 = uy * T + V1 * inv_r, f2 = uz * T + V2 * inv_r;
-        fb[3 * u] = f0;
-        fb[3 * u + 1] = f1;
-        fb[3 * u + 2] = f2;
-        f_self0 += f0;
-        f_self1 += f1;
-        f_self2 += f2;
+      T = row_sum16(T);
+      V0 = row_sum16(V0);
+      V1 = row_sum16(V1);
+      V2 = row_sum16(V2);
+      if ((lane & 15) == 15) {
+        float* pu = part + 16 * u + (lane >> 4);
+        pu[0] = T;
+        pu[4] = V0;
+        pu[8] = V1;
+        pu[12] = V2;
       }
     }
     wave_sync();
     if (lane < cnt) {
+      const float4* pu = reinterpret_cast<const float4*>(part + 16 * lane);
+      const float4 t4 = pu[0], a4 = pu[1], b4 = pu[2], c4 = pu[3];
+      const float T = (t4.x + t4.y) + (t4.z + t4.w);
+      const float V0 = (a4.x + a4.y) + (a4.z + a4.w);
+      const float V1 = (b4.x + b4.y) + (b4.z + b4.w);
+      const float V2 = (c4.x + c4.y) + (c4.z + c4.w);
+      const float inv_r = 1.f / pr;
+      const float f0 = px * inv_r * T + V0 * inv_r, f1 = py * inv_r * T + V1 * inv_r, f2 = pz * inv_r * T + V2 * inv_r;
       const int64_t gi = a.pr_gidx[b0 + j0 + lane];
-      fglob[3 * gi] += (double)fb[3 * lane];
-      fglob[3 * gi + 1] += (double)fb[3 * lane + 1];
-      fglob[3 * gi + 2] += (double)fb[3 * lane + 2];
+      fglob[3 * gi] += (double)f0;
+      fglob[3 * gi + 1] += (double)f1;
+      fglob[3 * gi + 2] += (double)f2;
+      fs0 += f0;
+      fs1 += f1;
+      fs2 += f2;
     }
-    wave_sync();  // rows and fb are rewritten by the next pass
+    wave_sync();  // rows and part are rewritten by the next pass
   }
+  fs0 = wave_total(fs0);
+  fs1 = wave_total(fs1);
+  fs2 = wave_total(fs2);
   if (lane == 0) {
     const int64_t gi = a.pr_gidx[b0];
-    fglob[3 * gi] -= f_self0;
-    fglob[3 * gi + 1] -= f_self1;
-    fglob[3 * gi + 2] -= f_self2;
+    fglob[3 * gi] -= (double)fs0;
+    fglob[3 * gi + 1] -= (double)fs1;
+    fglob[3 * gi + 2] -= (double)fs2;
   }
 }
 
@@ -610,6 +682,10 @@ static F32Args f32_args(tgx_model_t* m) {
   a.roff = (m->P.NLM + 3) & ~3;
   a.ldb = a.roff + 2 * NMAXF;  // 16-byte aligned rows: R and dR load as whole vectors
   a.basis = m->pbasis.as<float>();
+  const SoapParams& P = m->P;
+  a.split8 = P.N <= NMAXF && P.n_species <= 2;
+  for (int k = 0; k < P.n_species; ++k)
+    if (P.sp[k].nb > 8 || P.sp[k].off != 8 * k) a.split8 = 0;
   return a;
 }
 
