@@ -88,7 +88,7 @@ def roofline_entries(stats, n_atoms, blocks, n_pairs, members=1, hidden=256):
     Dpad = -(-D // 128) * 128
     N_rad = blocks[0].get("N", 16)
     nlm = blocks[0].get("NLM", 81)
-    ldb = -(-nlm // 4) * 4 + 2 * N_rad
+    ldb = -(-nlm // 4) * 4 + 2 * 16  # R and dR take 16 slots each
     KH = members * hidden
     k = stats["kernels"]
 
@@ -119,11 +119,31 @@ def roofline_entries(stats, n_atoms, blocks, n_pairs, members=1, hidden=256):
     return rows
 
 
-def roofline_object(name, entry, traffic=None):
+# bench kernel label -> device kernel name in the rocprofv3 counter summary
+KERNEL_SYMBOL = {"soap_adjoint_f32": "k_site_adjoint", "soap_desc_f32": "k_site_desc", "soap_pair_basis": "k_pair_basis",
+                 "nbr_fill": "k_cell_neigh<1>", "soap_site_der_f64": "k_soap_site<double, 1, 5>",
+                 "plain_force": "k_plain_force", "gemm_kernel_f64": "k_gemm_f64<1, true>",
+                 "gemm_qss_f64": "k_gemm_f64<0, true>"}
+
+
+def measured_traffic(name):
+    """HBM bytes per launch of a kernel from the committed counter summary
+    (profiles/hbm_traffic_65k.json: FETCH_SIZE and WRITE_SIZE runs calibrated on
+    the streaming copy), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_65k.json")
+    if not os.path.exists(path):
+        return None
+    tab = json.load(open(path))["bytes_per_launch"]
+    v = tab.get(KERNEL_SYMBOL.get(name, name))
+    return float(v) if v is not None else None
+
+
+def roofline_object(name, entry):
     bound, ach, peak_key = entry
     peak = PEAK[peak_key]
     return {"bound": bound, "kernel": name, "achieved": round(ach, 3), "peak": peak,
-            "unit": "TFLOP/s" if bound == "mfma" else "GB/s", "frac": round(ach / peak, 4), "traffic": traffic}
+            "unit": "TFLOP/s" if bound == "mfma" else "GB/s", "frac": round(ach / peak, 4),
+            "traffic": measured_traffic(name)}
 
 
 def prepare_surrogate(ctx, ids, sysm, models, args):
@@ -259,11 +279,12 @@ def main():
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--md-steps", type=int, default=50)
     ap.add_argument("--gate", type=float, default=0.020)
-    ap.add_argument("--cpu-sample", type=int, default=2048)
+    ap.add_argument("--cpu-sample", type=int, default=32768)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--production-steps", type=int, default=2000)
     ap.add_argument("--nvt-steps", type=int, default=200)
     ap.add_argument("--check-every", type=int, default=100)
+    ap.add_argument("--hbm-bytes", type=int, default=1 << 30, help="bytes of the streaming-copy reference (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -306,6 +327,9 @@ def main():
         prod = None
         if surr is not None and args.production_steps > 0:
             prod = production_md(ctx, sysm, args)
+        hbm = ctx.hbm_probe(args.hbm_bytes, 5) if args.hbm_bytes > 0 else None
+        if hbm:
+            log(f"streaming copy (k_hbm_copy, {args.hbm_bytes} B): {hbm:.1f} GB/s read + write")
     ref = res.get("fp64_plain") or res.get("fp64_adjoint")
     acc = {}
     for p, r in res.items():
@@ -352,6 +376,7 @@ def main():
         "accuracy_vs_fp64_gpu": acc,
         "surrogate": surr,
         "production_md": prod,
+        "hbm_copy_GBs": hbm,
         "roofline": roof, "kernels_roofline": kernels_roofline,
         "cpu_baseline": cpu,
     }

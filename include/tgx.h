@@ -219,6 +219,11 @@ int tgx_set_profiling(tgx_ctx* ctx, int on);
 int tgx_kernel_time(tgx_ctx* ctx, const char* name, int* count, double* total_ms);
 int tgx_reset_profile(tgx_ctx* ctx);
 int tgx_synchronize(tgx_ctx* ctx);
+/* HBM reference measurement (no counterpart in the reference): copies `bytes`
+ * between two device buffers with a streaming kernel (k_hbm_copy), `reps` times,
+ * and reports the read + write bandwidth of the fastest copy in GB/s. The
+ * kernel also calibrates the FETCH_SIZE / WRITE_SIZE counters. */
+int tgx_hbm_probe(tgx_ctx* ctx, size_t bytes, int reps, double* gbps);
 
 /* ---- synthetic inputs (host only; no GPU needed) -------------------------- */
 /* Random sequential addition of n points in an orthorhombic box with minimum

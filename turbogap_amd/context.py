@@ -229,3 +229,9 @@ class Context:
 
     def synchronize(self):
         L.check(self._lib.tgx_synchronize(self.h))
+
+    def hbm_probe(self, nbytes: int = 1 << 30, reps: int = 5) -> float:
+        """Read + write GB/s of the library's streaming copy kernel (best of reps)."""
+        g = C.c_double()
+        L.check(self._lib.tgx_hbm_probe(self.h, int(nbytes), int(reps), C.byref(g)), "hbm_probe")
+        return g.value

@@ -81,6 +81,38 @@ extern "C" int tgx_synchronize(tgx_ctx* ctx) {
   return TGX_OK;
 }
 
+extern "C" int tgx_hbm_probe(tgx_ctx* ctx, size_t bytes, int reps, double* gbps) {
+  TGX_CTX_CHECK(ctx);
+  TGX_REQUIRE(gbps && reps >= 1 && bytes >= 16 && bytes % 16 == 0 && bytes <= ((size_t)64 << 30),
+              "tgx_hbm_probe: need 16 <= bytes <= 64 GiB, a multiple of 16, and reps >= 1");
+  dbuf a, b;
+  TGX_TRY(a.ensure(bytes));
+  TGX_TRY(b.ensure(bytes));
+  TGX_TRY(tgx_zero(ctx, a.p, bytes));
+  hipEvent_t e0, e1;
+  TGX_CHECK_HIP(hipEventCreate(&e0));
+  TGX_CHECK_HIP(hipEventCreate(&e1));
+  double best = 1e30;
+  int st = TGX_OK;
+  for (int r = 0; r < reps && st == TGX_OK; ++r) {
+    (void)hipEventRecord(e0, ctx->stream);
+    st = tgx_hbm_copy(ctx, a.p, b.p, bytes);
+    (void)hipEventRecord(e1, ctx->stream);
+    if (hipEventSynchronize(e1) != hipSuccess) st = TGX_ERR_HIP;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    best = std::min(best, (double)ms);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(ctx->stream);
+  a.release();
+  b.release();
+  TGX_TRY(st);
+  *gbps = 2.0 * (double)bytes / (best * 1e-3) / 1e9;
+  return TGX_OK;
+}
+
 // ---- context ------------------------------------------------------------------
 extern "C" int tgx_init(int device, tgx_ctx** out) {
   TGX_REQUIRE(out, "tgx_init: null output");
@@ -125,6 +157,7 @@ extern "C" int tgx_finalize(tgx_ctx* ctx) {
                   &ctx->pos_prev, &ctx->e_members, &ctx->gate_mask, &ctx->cell_pos};
   for (dbuf* b : bufs) b->release();
   if (ctx->gate_host) (void)hipHostFree(ctx->gate_host);
+  if (ctx->pin64) (void)hipHostFree(ctx->pin64);
   for (auto& r : ctx->prof) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);

@@ -166,9 +166,14 @@ int tgx_block_select(tgx_ctx* ctx, tgx_model_t* m) {
   TGX_LAUNCH(ctx, "select_count", k_select_count, dim3((n + 255) / 256), dim3(256), 0, n, sm, m->central_species,
              m->rcut_max, ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(), ctx->species_sc.as<int>(),
              ctx->rjs.as<double>(), ctx->nspec.as<int>(), flag, npairs);
-  int64_t ns = 0, np = 0;
-  TGX_TRY(tgx_scan_i32(ctx, flag, site_idx, n, &ns));
-  TGX_TRY(tgx_scan_i32(ctx, npairs, pair_off, n, &np));
+  TGX_TRY(tgx_scan_i32(ctx, flag, site_idx, n, nullptr));
+  TGX_TRY(tgx_scan_i32(ctx, npairs, pair_off, n, nullptr));
+  // both totals in one readback
+  if (!ctx->pin64) TGX_CHECK_HIP(hipHostMalloc((void**)&ctx->pin64, 2 * sizeof(int64_t), hipHostMallocDefault));
+  TGX_CHECK_HIP(hipMemcpyAsync(ctx->pin64, site_idx + n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+  TGX_CHECK_HIP(hipMemcpyAsync(ctx->pin64 + 1, pair_off + n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  const int64_t ns = ctx->pin64[0], np = ctx->pin64[1];
   m->n_sites_b = (int)ns;
   m->n_pairs_b = np;
   const size_t npz = (size_t)(np > 0 ? np : 1);

@@ -66,6 +66,13 @@ __global__ void k_scan_add(int64_t* __restrict__ out, int64_t n, const int64_t* 
   if (i == 0) out[n] = sums[(n + SCAN_BLOCK - 1) / SCAN_BLOCK];
 }
 
+// Streaming copy, 16 bytes per thread and iteration (the HBM reference).
+__global__ __launch_bounds__(256) void k_hbm_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n16; i += stride) dst[i] = src[i];
+}
+
 // Fixed-order sum: each block reduces a contiguous slice in a fixed tree.
 __global__ __launch_bounds__(256) void k_sum_partial(const double* __restrict__ x, int64_t n, double* __restrict__ part) {
   __shared__ double sh[256];
@@ -126,5 +133,12 @@ int tgx_device_sum(tgx_ctx* ctx, const double* d_x, int64_t n, double* out_host)
   double s = 0.0;
   for (int i = 0; i < nbk; ++i) s += part[i];
   *out_host = s;
+  return TGX_OK;
+}
+
+int tgx_hbm_copy(tgx_ctx* ctx, const void* src, void* dst, size_t bytes) {
+  const size_t n16 = bytes / 16;
+  TGX_LAUNCH(ctx, "hbm_copy", k_hbm_copy, dim3(4096), dim3(256), 0, reinterpret_cast<const uint4*>(src),
+             reinterpret_cast<uint4*>(dst), n16);
   return TGX_OK;
 }

@@ -113,6 +113,7 @@ _SIGS = {
     "tgx_kernel_time": [_P, C.c_char_p, _PI, _PD],
     "tgx_reset_profile": [_P],
     "tgx_synchronize": [_P],
+    "tgx_hbm_probe": [_P, C.c_size_t, _I, _PD],
     "tgx_generate_rsa": [_I, _PD, _D, _U64, _PD],
     "tgx_generate_permutation": [_I, _U64, _PI],
     "tgx_generate_normal": [_I, _D, _U64, _PD],
