@@ -162,6 +162,16 @@ __global__ void k_cell_pos(int n, const int* __restrict__ cell_atoms, const doub
   cell_pos[3 * (size_t)s + 2] = pos[3 * (size_t)k + 2];
 }
 
+// Workgroup b is dispatched to XCD b % 8. Handing each XCD a contiguous run of
+// logical workgroups keeps spatially close atoms (when the run is walked in
+// cell order) on one XCD's L2.
+__device__ __forceinline__ int xcd_block() {
+  const int G = gridDim.x, b = blockIdx.x;
+  const int xcd = b & 7, local = b >> 3;
+  const int base = G / 8, rem = G % 8;
+  return xcd * base + (xcd < rem ? xcd : rem) + local;
+}
+
 __device__ __forceinline__ void wave_sync_nb() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -192,8 +202,10 @@ __global__ __launch_bounds__(64 * NBW) void k_cell_neigh(int n, const double* __
   __shared__ int s_beg[NBW][29];  // running counts of the (up to 27) visited cells, then their number
   __shared__ int64_t s_st[NBW][27];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * NBW + w);
-  if (i >= n) return;
+  // central atoms in cell order (slot -> atom): neighbouring waves share candidate cells
+  const int slot_i = __builtin_amdgcn_readfirstlane(xcd_block() * NBW + w);
+  if (slot_i >= n) return;
+  const int i = __builtin_amdgcn_readfirstlane(cell_atoms[slot_i]);
   if (do_list && !do_list[i]) {
     if (MODE == 0 && lane == 0) n_neigh[i] = 0;
     return;
@@ -349,10 +361,12 @@ __global__ __launch_bounds__(256) void k_brute_neigh(int n, int n_sc, const doub
 // back at i with the opposite displacement; the self pair maps to itself. -1
 // when j has no list. Used by the deterministic force gather.
 __global__ void k_reverse(int n, const int* __restrict__ n_neigh, const int64_t* __restrict__ off,
-                          const int* __restrict__ nlist, const double* __restrict__ xyz, int64_t* __restrict__ rev) {
+                          const int* __restrict__ nlist, const double* __restrict__ xyz, const int* __restrict__ order,
+                          int64_t* __restrict__ rev) {
   const int lane = threadIdx.x & 63;
-  const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (i >= n) return;
+  const int w = xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= n) return;
+  const int i = order ? order[w] : w;  // cell order when the cell list was built
   const int64_t b = off[i];
   const int nn = n_neigh[i];
   for (int t = lane; t < nn; t += 64) {
@@ -489,7 +503,8 @@ int tgx_neighbors_run(tgx_ctx* ctx, double rcut_max, const int* indices, const i
                ctx->nlist.as<int>(), ctx->rjs.as<double>(), ctx->xyz.as<double>(), ctx->nspec.as<int>());
   }
   TGX_LAUNCH(ctx, "nbr_reverse", k_reverse, wgrid, wblk, 0, n, ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(),
-             ctx->nlist.as<int>(), ctx->xyz.as<double>(), ctx->rev.as<int64_t>());
+             ctx->nlist.as<int>(), ctx->xyz.as<double>(), cells ? ctx->cell_atoms.as<int>() : (const int*)nullptr,
+             ctx->rev.as<int64_t>());
   ctx->rcut_max = rcut_max;
   ctx->have_list = true;
   ctx->have_angles = false;

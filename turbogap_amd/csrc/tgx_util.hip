@@ -4,6 +4,7 @@
 namespace {
 
 constexpr int SCAN_BLOCK = 1024;
+constexpr int64_t SCAN_SINGLE_MAX = 16 * 1024;  // one workgroup scans up to this many values
 
 __global__ void k_fill_zero(uint32_t* __restrict__ p, size_t n_words) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -66,6 +67,33 @@ __global__ void k_scan_add(int64_t* __restrict__ out, int64_t n, const int64_t* 
   if (i == 0) out[n] = sums[(n + SCAN_BLOCK - 1) / SCAN_BLOCK];
 }
 
+// Whole exclusive scan in one workgroup (n up to SCAN_SINGLE_MAX = 16 per
+// thread): thread t owns the contiguous chunk [t c, (t+1) c), loaded at once into
+// registers; the 1024 chunk sums are scanned in LDS and each thread writes its
+// chunk's prefixes; out[n] is the total.
+constexpr int SCAN_CMAX = 16;
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_single(const int* __restrict__ in, int64_t* __restrict__ out,
+                                                           int64_t n) {
+  __shared__ int64_t sh[SCAN_BLOCK];
+  const int c = (int)((n + SCAN_BLOCK - 1) / SCAN_BLOCK);
+  const int64_t b = (int64_t)threadIdx.x * c;
+  int v[SCAN_CMAX];
+  int64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_CMAX; ++k) {
+    v[k] = (k < c && b + k < n) ? in[b + k] : 0;
+    s += v[k];
+  }
+  int64_t tot;
+  int64_t run = block_exclusive_scan(s, sh, &tot);
+#pragma unroll
+  for (int k = 0; k < SCAN_CMAX; ++k) {
+    if (k < c && b + k < n) out[b + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 0) out[n] = tot;
+}
+
 // Streaming copy, 16 bytes per thread and iteration (the HBM reference).
 __global__ __launch_bounds__(256) void k_hbm_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -109,7 +137,9 @@ int tgx_scan_i32(tgx_ctx* ctx, const int* in, int64_t* out, int64_t n, int64_t* 
   const int64_t nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
   TGX_TRY(ctx->scan_tmp.ensure(sizeof(int64_t) * (size_t)(nb + 2)));
   int64_t* sums = ctx->scan_tmp.as<int64_t>();
-  if (n > 0) {
+  if (n > 0 && n <= SCAN_SINGLE_MAX) {
+    TGX_LAUNCH(ctx, "scan_single", k_scan_single, dim3(1), dim3(SCAN_BLOCK), 0, in, out, n);
+  } else if (n > 0) {
     TGX_LAUNCH(ctx, "scan_local", k_scan_local, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, in, out, n, sums);
     TGX_LAUNCH(ctx, "scan_sums", k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, sums, nb);
     TGX_LAUNCH(ctx, "scan_add", k_scan_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, out, n, sums);
