@@ -213,30 +213,47 @@ __global__ __launch_bounds__(64 * NBW) void k_cell_neigh(int n, const double* __
   int* vbeg = s_beg[w];
   int64_t* vst = s_st[w];
   const int i2 = 1 + cell_xyz[3 * i], j2 = 1 + cell_xyz[3 * i + 1], k2 = 1 + cell_xyz[3 * i + 2];
-  if (lane == 0) {
-    int nv = 0, total = 0;
-    for (int k3 = k2 - 1; k3 <= k2 + 1; ++k3) {
-      if (mz == 1 && k3 != 1) continue;
-      if (mz == 2 && k2 == 1 && k3 == 0) continue;
-      if (mz == 2 && k2 == 2 && k3 == 3) continue;
-      for (int j3 = j2 - 1; j3 <= j2 + 1; ++j3) {
-        if (my == 1 && j3 != 1) continue;
-        if (my == 2 && j2 == 1 && j3 == 0) continue;
-        if (my == 2 && j2 == 2 && j3 == 3) continue;
-        for (int i3 = i2 - 1; i3 <= i2 + 1; ++i3) {
-          if (mx == 1 && i3 != 1) continue;
-          if (mx == 2 && i2 == 1 && i3 == 0) continue;
-          if (mx == 2 && i2 == 2 && i3 == 3) continue;
-          const int c = i_modulo(i3 - 1, mx) + i_modulo(j3 - 1, my) * mx + i_modulo(k3 - 1, mz) * mx * my;
-          vbeg[nv] = total;
-          vst[nv] = cell_start[c];
-          total += cell_cnt[c];
-          ++nv;
-        }
+  {
+    // the 27 candidate cells in visiting order (k3 outer, then j3, then i3), one
+    // lane each; skipped cells are compacted out with a ballot and the running
+    // atom counts come from an inclusive scan over the lanes
+    bool valid = false;
+    int c = 0, cnt = 0;
+    int64_t st = 0;
+    if (lane < 27) {
+      const int k3 = k2 - 1 + lane / 9, j3 = j2 - 1 + (lane / 3) % 3, i3 = i2 - 1 + lane % 3;
+      valid = true;
+      if (mz == 1 && k3 != 1) valid = false;
+      if (mz == 2 && k2 == 1 && k3 == 0) valid = false;
+      if (mz == 2 && k2 == 2 && k3 == 3) valid = false;
+      if (my == 1 && j3 != 1) valid = false;
+      if (my == 2 && j2 == 1 && j3 == 0) valid = false;
+      if (my == 2 && j2 == 2 && j3 == 3) valid = false;
+      if (mx == 1 && i3 != 1) valid = false;
+      if (mx == 2 && i2 == 1 && i3 == 0) valid = false;
+      if (mx == 2 && i2 == 2 && i3 == 3) valid = false;
+      if (valid) {
+        c = i_modulo(i3 - 1, mx) + i_modulo(j3 - 1, my) * mx + i_modulo(k3 - 1, mz) * mx * my;
+        cnt = cell_cnt[c];
+        st = cell_start[c];
       }
     }
-    vbeg[nv] = total;
-    vbeg[28] = nv;
+    const uint64_t vm = __ballot(valid);
+    const int pos = __popcll(vm & ((1ull << lane) - 1ull));
+    int incl = cnt;  // inclusive scan of the counts over lanes 0..26
+    for (int o = 1; o < 32; o <<= 1) {
+      const int t = __shfl_up(incl, o);
+      if (lane >= o) incl += t;
+    }
+    if (valid) {
+      vbeg[pos] = incl - cnt;
+      vst[pos] = st;
+    }
+    const int nvv = __popcll(vm);
+    if (lane == 26) {
+      vbeg[nvv] = incl;
+      vbeg[28] = nvv;
+    }
   }
   wave_sync_nb();
   const int nv = vbeg[28];
@@ -376,16 +393,36 @@ __global__ void k_reverse(int n, const int* __restrict__ n_neigh, const int64_t*
     if (j >= n) j %= n;
     const int64_t bj = off[j];
     const int nj = n_neigh[j];
-    int64_t best = -1;
-    double bd = 0.0;
-    for (int u = 1; u < nj; ++u) {
-      const int64_t q = bj + u;
-      int jq = nlist[q] - 1;
-      if (jq >= n) jq %= n;
-      if (jq != i) continue;
-      double e0 = xyz[3 * q] + xyz[3 * p], e1 = xyz[3 * q + 1] + xyz[3 * p + 1], e2 = xyz[3 * q + 2] + xyz[3 * p + 2];
-      double dd = e0 * e0 + e1 * e1 + e2 * e2;
-      if (best < 0 || dd < bd) { best = q; bd = dd; }
+    // indices first, eight loads in flight; the displacement test only decides
+    // between several images of the same atom (small boxes)
+    int first = -1, nmatch = 0;
+    for (int u0 = 1; u0 < nj; u0 += 8) {
+      int v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] = u0 + k < nj ? nlist[bj + u0 + k] : 0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        int jq = v[k] - 1;
+        if (jq >= n) jq %= n;
+        if (u0 + k < nj && jq == i) {
+          if (first < 0) first = u0 + k;
+          ++nmatch;
+        }
+      }
+    }
+    int64_t best = first >= 0 ? bj + first : -1;
+    if (nmatch > 1) {
+      double bd = 0.0;
+      best = -1;
+      for (int u = 1; u < nj; ++u) {
+        const int64_t q = bj + u;
+        int jq = nlist[q] - 1;
+        if (jq >= n) jq %= n;
+        if (jq != i) continue;
+        double e0 = xyz[3 * q] + xyz[3 * p], e1 = xyz[3 * q + 1] + xyz[3 * p + 1], e2 = xyz[3 * q + 2] + xyz[3 * p + 2];
+        double dd = e0 * e0 + e1 * e1 + e2 * e2;
+        if (best < 0 || dd < bd) { best = q; bd = dd; }
+      }
     }
     rev[p] = best;
   }
