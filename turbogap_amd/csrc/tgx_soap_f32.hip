@@ -412,7 +412,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_site_adjoint(SoapParams P, F32Ar
   for (int d = lane; d < D; d += 64) X[d] = (srow[d] - sq * qrow[d]) * inv_n;
   wave_sync();
   // H(n, lm) = sum_n' w(n, n') g(n, n', l) c(n', lm), w = 2 on the diagonal and
-  // sqrt(2) off it; each stored g(n <= n') serves both orders.
+  // sqrt(2) off it; each stored g(n <= n') serves both orders. With 16 radial
+  // functions the triangle is walked by fully static code.
   int lh[2];
   bool vh[2];
 #pragma unroll
@@ -422,33 +423,62 @@ __global__ __launch_bounds__(64 * WAVES) void k_site_adjoint(SoapParams P, F32Ar
     int l = (int)sqrtf((float)lm + 0.5f);
     if (l * l > lm) --l;
     lh[h] = l;
-    float cc[NMAXF], hs[NMAXF];
+  }
+  if (N == NMAXF) {
 #pragma unroll
-    for (int n = 0; n < NMAXF; ++n) {
-      cc[n] = hreg[h][n];
-      hs[n] = 0.f;
-    }
-    if (vh[h]) {
+    for (int h = 0; h < 2; ++h) {
+      float hs[NMAXF];
+#pragma unroll
+      for (int n = 0; n < NMAXF; ++n) hs[n] = 0.f;
+      const float* Xl = X + (vh[h] ? lh[h] : 0);
       int pidx = 0;
 #pragma unroll
       for (int n = 0; n < NMAXF; ++n) {
-        if (n < N) {
-          hs[n] += 2.f * X[pidx * L1 + l] * cc[n];
-          ++pidx;
+        hs[n] += 2.f * Xl[pidx * L1] * hreg[h][n];
+        ++pidx;
 #pragma unroll
-          for (int np = n + 1; np < NMAXF; ++np) {
-            if (np < N) {
-              const float g = (float)M_SQRT2 * X[pidx * L1 + l];
-              hs[n] += g * cc[np];
-              hs[np] += g * cc[n];
-              ++pidx;
+        for (int np = n + 1; np < NMAXF; ++np) {
+          const float g = (float)M_SQRT2 * Xl[pidx * L1];
+          hs[n] += g * hreg[h][np];
+          hs[np] += g * hreg[h][n];
+          ++pidx;
+        }
+      }
+#pragma unroll
+      for (int n = 0; n < NMAXF; ++n) hreg[h][n] = vh[h] ? hs[n] : 0.f;
+    }
+  } else {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int l = lh[h];
+      float cc[NMAXF], hs[NMAXF];
+#pragma unroll
+      for (int n = 0; n < NMAXF; ++n) {
+        cc[n] = hreg[h][n];
+        hs[n] = 0.f;
+      }
+      if (vh[h]) {
+        int pidx = 0;
+#pragma unroll
+        for (int n = 0; n < NMAXF; ++n) {
+          if (n < N) {
+            hs[n] += 2.f * X[pidx * L1 + l] * cc[n];
+            ++pidx;
+#pragma unroll
+            for (int np = n + 1; np < NMAXF; ++np) {
+              if (np < N) {
+                const float g = (float)M_SQRT2 * X[pidx * L1 + l];
+                hs[n] += g * cc[np];
+                hs[np] += g * cc[n];
+                ++pidx;
+              }
             }
           }
         }
       }
-    }
 #pragma unroll
-    for (int n = 0; n < NMAXF; ++n) hreg[h][n] = hs[n];
+      for (int n = 0; n < NMAXF; ++n) hreg[h][n] = hs[n];
+    }
   }
   // lowering table entries of this lane's lm
   int li[2][3][KLOW];
@@ -459,9 +489,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_site_adjoint(SoapParams P, F32Ar
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int t = 0; t < KLOW; ++t) {
+        // unused terms read W_00 with a zero coefficient, so the sums need no branch
         const int k = ((lane + 64 * h) * 3 + c) * KLOW + t;
-        li[h][c][t] = vh[h] ? low_idx[k] : -1;
-        lc[h][c][t] = vh[h] ? low_coef[k] : 0.f;
+        const int ix = vh[h] ? low_idx[k] : -1;
+        li[h][c][t] = ix >= 0 ? ix : 0;
+        lc[h][c][t] = ix >= 0 ? low_coef[k] : 0.f;
       }
   // Pair passes: the rows of up to APASS consecutive pairs are copied into LDS
   // (over X, which H no longer needs) by one coalesced sweep, so each pass costs
@@ -499,7 +531,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_site_adjoint(SoapParams P, F32Ar
       const float* row = rows + u * ldb;
       const float r = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, pr), u));
       const int s = __builtin_amdgcn_readlane(ps, u);
-      const float inv_r = 1.f / r;
+      const float inv_r = __builtin_amdgcn_rcpf(r);
       float xa[2] = {0.f, 0.f}, xb[2] = {0.f, 0.f};
       if (a.split8) {
         // at most 8 radial functions per species, stacked at 0 or 8: half the FMAs
@@ -548,8 +580,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_site_adjoint(SoapParams P, F32Ar
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
-          for (int t = 0; t < KLOW; ++t)
-            if (li[h][c][t] >= 0) g[c] += lc[h][c][t] * row[li[h][c][t]];
+          for (int t = 0; t < KLOW; ++t) g[c] += lc[h][c][t] * row[li[h][c][t]];
         V0 += xr * g[0];
         V1 += xr * g[1];
         V2 += xr * g[2];
@@ -574,7 +605,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_site_adjoint(SoapParams P, F32Ar
       const float V0 = (a4.x + a4.y) + (a4.z + a4.w);
       const float V1 = (b4.x + b4.y) + (b4.z + b4.w);
       const float V2 = (c4.x + c4.y) + (c4.z + c4.w);
-      const float inv_r = 1.f / pr;
+      const float inv_r = __builtin_amdgcn_rcpf(pr);
       const float f0 = px * inv_r * T + V0 * inv_r, f1 = py * inv_r * T + V1 * inv_r, f2 = pz * inv_r * T + V2 * inv_r;
       const int64_t gi = a.pr_gidx[b0 + j0 + lane];
       fglob[3 * gi] += (double)f0;
