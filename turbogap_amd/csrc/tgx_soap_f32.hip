@@ -204,6 +204,41 @@ __device__ __forceinline__ void harmonics_all(int L, const float* ynorm, float u
   }
 }
 
+// The same with l_max known at compile time: every index is a constant.
+template <int LT>
+__device__ __forceinline__ void harmonics_fixed(const float* ynorm, float ux, float uy, float uz,
+                                                const float (&A)[TGX_MAX_L + 1], float* w) {
+  float c = 1, s = 0;
+#pragma unroll
+  for (int m = 0; m <= LT; ++m) {
+    if (m > 0) {
+      const float nc = c * ux - s * uy, ns = c * uy + s * ux;
+      c = nc;
+      s = ns;
+    }
+    float q = 1, p = 0;
+#pragma unroll
+    for (int l = m; l <= LT; ++l) {
+      if (l == m + 1) {
+        p = q;
+        q = (float)(2 * m + 1) * uz;
+      } else if (l > m + 1) {
+        const float nq = ((float)(2 * l - 1) * uz * q - (float)(l + m - 1) * p) * (1.0f / (float)(l - m));
+        p = q;
+        q = nq;
+      }
+      const int base = l * l + l;
+      const float v = A[l] * ynorm[base + m] * q;
+      if (m == 0) {
+        w[base] = v;
+      } else {
+        w[base + m] = v * c;
+        w[base - m] = v * s;
+      }
+    }
+  }
+}
+
 constexpr int PB = 64;  // pairs (threads) per k_pair_basis workgroup
 
 __global__ __launch_bounds__(PB) void k_pair_basis(SoapParams P, F32Args a, int64_t n_pairs) {
@@ -214,15 +249,20 @@ __global__ __launch_bounds__(PB) void k_pair_basis(SoapParams P, F32Args a, int6
   float* st = smf + shared_floats(P.NLM, P.n_species);
   const int t = threadIdx.x;
   float* row = st + t * LS;
-  for (int i = 0; i < a.ldb; ++i) row[i] = 0.f;
-  __syncthreads();
+  __syncthreads();  // the shared tables are loaded
   const int64_t p0 = (int64_t)blockIdx.x * PB;
   const int64_t p = p0 + t;
+  int s = -1;
+  double rd = 0.0;
   if (p < n_pairs) {
-    int s = a.pr_spec[p];
-    const double rd = a.pr_r[p];
+    s = a.pr_spec[p];
+    rd = a.pr_r[p];
     // the self pair (r = 0) and neighbours at or beyond their species' cutoff carry nothing
     if (s >= 0 && !(rd > 0.0 && rd < P.sp[s].rc)) s = -1;
+  }
+  // zero what this row does not overwrite: the radial slots, and the harmonics of an empty row
+  for (int i = s >= 0 ? a.roff : 0; i < a.ldb; ++i) row[i] = 0.f;
+  if (p < n_pairs) {
     if (s >= 0) {
       const SpecLite sp = sh.sp[s];
       const float r = (float)rd, inv = 1.f / r;
@@ -238,7 +278,9 @@ __global__ __launch_bounds__(PB) void k_pair_basis(SoapParams P, F32Args a, int6
         }
       float A[TGX_MAX_L + 1], dA[TGX_MAX_L + 1];
       damping(sp, r, A, dA);
-      harmonics_all(P.L, sh.ynorm, ux, uy, uz, A, row);
+      if (P.L == 8) harmonics_fixed<8>(sh.ynorm, ux, uy, uz, A, row);
+      else harmonics_all(P.L, sh.ynorm, ux, uy, uz, A, row);
+      for (int i = P.NLM; i < a.roff; ++i) row[i] = 0.f;  // padding between W and R
     }
   }
   __syncthreads();
