@@ -75,6 +75,24 @@ def test_soap_carbon(ctx, orc):
     np.testing.assert_allclose(der, dero, rtol=0, atol=1e-11)
 
 
+@pytest.mark.parametrize("n_max,l_max", [(6, 8), (10, 6), (5, 4)])
+def test_soap_derivative_sizes(ctx, orc, n_max, l_max):
+    """dq/dr of the plain path at other basis sizes: the l_max = 8 kernel with fewer rows
+    than its 16, and the general kernel (more than eight radial functions, other l_max)."""
+    sysm, _, _, _ = H.carbon216(200)
+    h = S.soap_hypers([S.CARBON], n_max=n_max, l_max=l_max)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    nl = ctx.build_neighbors_list(h.rcut_max)
+    mid = ctx.soap_model(h, np.zeros((1, h.dim)), np.zeros(1))
+    ctx.soap_backend_begin(mid)
+    q, der = ctx.get_soap(derivatives=True)
+    ctx.soap_backend_end()
+    spi = orc.species_index(nl["neighbor_species"], h.species_types)
+    qo, dero = orc.get_soap(h, nl["n_neigh"], spi, nl["rjs"], nl["xyz"], derivatives=True)
+    np.testing.assert_allclose(q, qo, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(der, dero, rtol=0, atol=1e-11)
+
+
 @pytest.mark.parametrize("path", ["fp64_plain", "fp64_adjoint"])
 def test_gap_carbon216(ctx, orc, path):
     sysm, h, Qs, al = H.carbon216(2000)

@@ -76,20 +76,23 @@ template <bool F64> __device__ __forceinline__ int tile_row(int lane, int r) {
   return F64 ? 4 * r + (lane >> 4) : 4 * (lane >> 4) + r;
 }
 
-template <int EPI, class OutT, class AccT, class VecT, bool F64, bool ZINT>
-__device__ void epilogue(const VecT (&acc)[4][4], int m0, int n0, int tn, int wm, int wn, const EpiArgs& e,
+// TI x TJ MFMA tiles of 16 x 16 per wavefront (a WM x WN wave tile), the
+// workgroup tile BMT rows tall with two wavefronts along N.
+template <int EPI, class OutT, class AccT, class VecT, bool F64, bool ZINT, int TI = 4, int TJ = 4, int WM = 64,
+          int WN = 64, int BMT = BM>
+__device__ void epilogue(const VecT (&acc)[TI][TJ], int m0, int n0, int tn, int wm, int wn, const EpiArgs& e,
                          double* sE) {
   const int lane = threadIdx.x & 63;
-  const int rbase = m0 + wm * 64;
-  const int cbase = n0 + wn * 64 + (lane & 15);
+  const int rbase = m0 + wm * WM;
+  const int cbase = n0 + wn * WN + (lane & 15);
   OutT* out = reinterpret_cast<OutT*>(e.out);
   if (EPI == EPI_SILU || EPI == EPI_SCALE_AUX) {
 #pragma unroll
-    for (int tj = 0; tj < 4; ++tj) {
+    for (int tj = 0; tj < TJ; ++tj) {
       const int col = cbase + tj * 16;
       const float bc = EPI == EPI_SILU ? e.bias[col] : 0.0f;
 #pragma unroll
-      for (int ti = 0; ti < 4; ++ti)
+      for (int ti = 0; ti < TI; ++ti)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const size_t row = (size_t)(rbase + ti * 16 + tile_row<F64>(lane, r));
@@ -106,9 +109,9 @@ __device__ void epilogue(const VecT (&acc)[4][4], int m0, int n0, int tn, int wm
   }
   if (EPI == EPI_STORE) {
 #pragma unroll
-    for (int ti = 0; ti < 4; ++ti)
+    for (int ti = 0; ti < TI; ++ti)
 #pragma unroll
-      for (int tj = 0; tj < 4; ++tj)
+      for (int tj = 0; tj < TJ; ++tj)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           out[(size_t)(rbase + ti * 16 + tile_row<F64>(lane, r)) * e.ldo + cbase + tj * 16] =
@@ -117,10 +120,10 @@ __device__ void epilogue(const VecT (&acc)[4][4], int m0, int n0, int tn, int wm
   }
   // one 16-row slab at a time keeps only four row partials live
 #pragma unroll
-  for (int ti = 0; ti < 4; ++ti) {
+  for (int ti = 0; ti < TI; ++ti) {
     double er[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int tj = 0; tj < 4; ++tj) {
+    for (int tj = 0; tj < TJ; ++tj) {
       const int col = cbase + tj * 16;
       const AccT al = e.alpha_f ? (AccT)e.alpha_f[col] : (AccT)e.alpha[col];
 #pragma unroll
@@ -139,13 +142,13 @@ __device__ void epilogue(const VecT (&acc)[4][4], int m0, int n0, int tn, int wm
       v += __shfl_xor(v, 2);
       v += __shfl_xor(v, 4);
       v += __shfl_xor(v, 8);
-      if ((lane & 15) == 0) sE[wn * BM + wm * 64 + ti * 16 + tile_row<F64>(lane, r)] = v;
+      if ((lane & 15) == 0) sE[wn * BMT + wm * WM + ti * 16 + tile_row<F64>(lane, r)] = v;
     }
   }
   __syncthreads();
-  if (threadIdx.x < BM) {
+  if (threadIdx.x < BMT) {
     const int row = m0 + threadIdx.x;
-    e.epart[(size_t)row * e.n_tiles + tn] = sE[threadIdx.x] + sE[BM + threadIdx.x];
+    e.epart[(size_t)row * e.n_tiles + tn] = sE[threadIdx.x] + sE[BMT + threadIdx.x];
   }
 }
 

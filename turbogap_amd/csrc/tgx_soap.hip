@@ -97,11 +97,11 @@ template <class T> struct PairScratch {
   }
 };
 
-// Step 1 of a pair, by `nthr` threads with local index t: Gaussians at the
-// quadrature nodes, harmonics, angular damping.
+// Gaussians at the quadrature nodes (threads t < nq of nthr) and the angular
+// damping A_l (thread nthr - 1) of one pair.
 template <class T>
-__device__ void pair_step1(const SoapParams& P, int s, T r, T ux, T uy, T uz, int t, int nthr, PairScratch<T>& ps,
-                           bool need_grad) {
+__device__ __forceinline__ void pair_gauss(const SoapParams& P, int s, T r, int t, int nthr, PairScratch<T>& ps,
+                                           bool need_grad) {
   const SpeciesParams& sp = P.sp[s];
   const T inv_rc = (T)sp.inv_rc;
   const T x = r * inv_rc;
@@ -113,14 +113,7 @@ __device__ void pair_step1(const SoapParams& P, int s, T r, T ux, T uy, T uz, in
     const T dx = (T)xq[q] - x;
     const T g = t_exp<T>(-(T)0.5 * dx * dx * inv_s2);
     ps.G[q] = g;
-    ps.dG[q] = g * (dx * inv_s2 * inv_rc + dx * dx * inv_s2 / sig * dsig);
-  }
-  for (int lm = t; lm < P.NLM; lm += nthr) {
-    const int l = lm_l(lm), m = lm - l * l - l;
-    T Y, gx, gy, gz;
-    ylm_one<T>(l, m, (T)P.ynorm[lm], ux, uy, uz, Y, gx, gy, gz);
-    ps.Y[lm] = Y;
-    if (need_grad) { ps.gY[3 * lm] = gx; ps.gY[3 * lm + 1] = gy; ps.gY[3 * lm + 2] = gz; }
+    if (need_grad) ps.dG[q] = g * (dx * inv_s2 * inv_rc + dx * dx * inv_s2 / sig * dsig);
   }
   if (t == nthr - 1) {
     // A_l = exp(-l(l+1) k), k = st^2/(2 r^2): one exponential, A_l = A_{l-1} e^(2l) with e = exp(-k)
@@ -135,6 +128,81 @@ __device__ void pair_step1(const SoapParams& P, int s, T r, T ux, T uy, T uz, in
       const T ll = (T)(l * (l + 1));
       ps.A[l] = Al;
       ps.dA[l] = Al * ll * dk;
+    }
+  }
+}
+
+// Step 1 of a pair, by `nthr` threads with local index t: Gaussians at the
+// quadrature nodes, harmonics, angular damping.
+template <class T>
+__device__ void pair_step1(const SoapParams& P, int s, T r, T ux, T uy, T uz, int t, int nthr, PairScratch<T>& ps,
+                           bool need_grad) {
+  pair_gauss<T>(P, s, r, t, nthr, ps, true);
+  for (int lm = t; lm < P.NLM; lm += nthr) {
+    const int l = lm_l(lm), m = lm - l * l - l;
+    T Y, gx, gy, gz;
+    ylm_one<T>(l, m, (T)P.ynorm[lm], ux, uy, uz, Y, gx, gy, gz);
+    ps.Y[lm] = Y;
+    if (need_grad) { ps.gY[3 * lm] = gx; ps.gY[3 * lm + 1] = gy; ps.gY[3 * lm + 2] = gz; }
+  }
+}
+
+// Real harmonics of one pair by lanes over m, for a compile-time l_max LL:
+// lane t = m + LL (t <= 2 LL) runs the l recurrence of its |m| and writes
+// Y_lm, and with GRAD the gradient of the solid harmonic r^l Y_lm at the unit
+// vector (xyz interleaved), for l = |m|..LL. Same recurrences and constants
+// as ylm_one; rinv[k] = 1/k replaces the division.
+template <int LL, bool GRAD>
+__device__ __forceinline__ void harmonics_m(int t, const double* ynorm, const double* rinv, double ux, double uy,
+                                            double uz, double* Y, double* gY) {
+  if (t > 2 * LL) return;
+  // opaque to the optimiser: the per-lane recurrence constants are rebuilt per call, not hoisted
+  // out of the caller's pair loop where they would stay live across the whole site
+  asm volatile("" : "+v"(t));
+  const int m = t - LL, am = m < 0 ? -m : m;
+  double c = 1, cx = 0, cy = 0, s = 0, sx = 0, sy = 0;
+#pragma unroll
+  for (int k = 0; k < LL; ++k) {
+    if (k < am) {
+      const double nc = c * ux - s * uy, ns = c * uy + s * ux;
+      if (GRAD) {
+        const double ncx = cx * ux + c - sx * uy, ncy = cy * ux - sy * uy - s;
+        const double nsx = cx * uy + sx * ux + s, nsy = cy * uy + c + sy * ux;
+        cx = ncx; cy = ncy; sx = nsx; sy = nsy;
+      }
+      c = nc;
+      s = ns;
+    }
+  }
+  const double f = m < 0 ? s : c, fx = m < 0 ? sx : cx, fy = m < 0 ? sy : cy;
+  double q = 1, qx = 0, qy = 0, qz = 0, p = 0, px = 0, py = 0, pz = 0;
+#pragma unroll
+  for (int l = 0; l <= LL; ++l) {
+    if (l < am) continue;
+    if (l == am + 1) {
+      p = q; px = qx; py = qy; pz = qz;
+      q = (double)(2 * am + 1) * uz;
+      qx = 0; qy = 0; qz = (double)(2 * am + 1);
+    } else if (l > am + 1) {
+      const double a = (double)(2 * l - 1), b = (double)(l + am - 1), inv = rinv[l - am];
+      const double nq = (a * uz * q - b * p) * inv;
+      if (GRAD) {
+        const double nqx = (a * uz * qx - b * (2.0 * ux * p + px)) * inv;
+        const double nqy = (a * uz * qy - b * (2.0 * uy * p + py)) * inv;
+        const double nqz = (a * (q + uz * qz) - b * (2.0 * uz * p + pz)) * inv;
+        px = qx; py = qy; pz = qz;
+        qx = nqx; qy = nqy; qz = nqz;
+      }
+      p = q;
+      q = nq;
+    }
+    const int lm = l * l + l + m;
+    const double Nn = ynorm[lm];
+    Y[lm] = Nn * q * f;
+    if (GRAD) {
+      gY[3 * lm] = Nn * (qx * f + q * fx);
+      gY[3 * lm + 1] = Nn * (qy * f + q * fy);
+      gY[3 * lm + 2] = Nn * qz * f;
     }
   }
 }
@@ -477,6 +545,599 @@ __global__ __launch_bounds__(WG) void k_soap_site(SoapParams P, SiteArgs a, int 
   }
 }
 
+// Workgroup barrier that orders LDS only: it waits for this wavefront's LDS
+// operations, not for its global stores, so streamed output stays in flight
+// across the barrier (no thread of a workgroup reads another's global output).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+__device__ __forceinline__ void der_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Per-pair channel tables of the plain dq/dr kernel, one slot per wavefront:
+//   s(n,l) = sum_m Y_lm c(n,lm)
+//   v_k(n,l) = (sum_m dY_lm/du_k c(n,lm) - l s u_k) / r     (k = x, y, z)
+//   E(n,l) = R_n A_l, Dd(n,l) = (R_n A_l)'   (zero outside the pair's block)
+// and misc = { q.dp_x, q.dp_y, q.dp_z, u_x, u_y, u_z, valid }.
+struct DerSlot {
+  static __host__ __device__ int size(int NL, int L, int NLM) {
+    return 6 * NL + 8 + PairScratch<double>::size(L, NLM);
+  }
+};
+
+// One wavefront reduces pair p of the site to its channel tables in `slot`.
+__device__ void der_pair_tables(const SoapParams& P, const SiteArgs& a, int64_t p, bool site_ok, const double* c,
+                                const double* H, double* slot) {
+  const int lane = threadIdx.x & 63;
+  const int NLM = P.NLM, L = P.L, L1 = L + 1, NL = P.N * L1;
+  double* s_t = slot;
+  double* v_t = s_t + NL;
+  double* E_t = v_t + 3 * NL;
+  double* Dd_t = E_t + NL;
+  double* misc = Dd_t + NL;
+  int s = a.pr_spec[p];
+  const double rr = a.pr_r[p];
+  if (!site_ok || s < 0 || !(rr < P.sp[s].rc)) {
+    if (lane == 0) misc[6] = 0.0;
+    return;
+  }
+  PairScratch<double> ps;
+  ps.bind(misc + 8, L, NLM);
+  const double r = rr, inv_r = 1.0 / r;
+  const double ux = a.pr_xyz[3 * p] * inv_r, uy = a.pr_xyz[3 * p + 1] * inv_r, uz = a.pr_xyz[3 * p + 2] * inv_r;
+  pair_step1<double>(P, s, r, ux, uy, uz, lane, 64, ps, true);
+  der_wave_sync();
+  pair_step2<double>(P, s, r, lane, 64, ps);
+  der_wave_sync();
+  const int nb = P.sp[s].nb, off = P.sp[s].off;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (int t = lane; t < NL; t += 64) {
+    const int n = t / L1, l = t - n * L1;
+    const int nl = n - off;
+    const bool inb = nl >= 0 && nl < nb;
+    const double* cr = c + n * NLM;
+    const double* hr = H + n * NLM;
+    double s0 = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    double h = 0.0, h0 = 0.0, h1 = 0.0, h2 = 0.0;
+    for (int lm = l * l; lm < (l + 1) * (l + 1); ++lm) {
+      const double Y = ps.Y[lm], gx = ps.gY[3 * lm], gy = ps.gY[3 * lm + 1], gz = ps.gY[3 * lm + 2];
+      const double v = cr[lm];
+      s0 += Y * v;
+      g0 += gx * v;
+      g1 += gy * v;
+      g2 += gz * v;
+      if (inb) {
+        const double hv = hr[lm];
+        h += Y * hv;
+        h0 += gx * hv;
+        h1 += gy * hv;
+        h2 += gz * hv;
+      }
+    }
+    const double lf = (double)l;
+    s_t[t] = s0;
+    v_t[t] = (g0 - lf * s0 * ux) * inv_r;
+    v_t[NL + t] = (g1 - lf * s0 * uy) * inv_r;
+    v_t[2 * NL + t] = (g2 - lf * s0 * uz) * inv_r;
+    double E = 0.0, Dd = 0.0;
+    if (inb) {
+      E = ps.R[nl] * ps.A[l];
+      Dd = ps.dR[nl] * ps.A[l] + ps.R[nl] * ps.dA[l];
+      a0 += Dd * h * ux + E * (h0 - lf * h * ux) * inv_r;
+      a1 += Dd * h * uy + E * (h1 - lf * h * uy) * inv_r;
+      a2 += Dd * h * uz + E * (h2 - lf * h * uz) * inv_r;
+    }
+    E_t[t] = E;
+    Dd_t[t] = Dd;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a0 += __shfl_xor(a0, o);
+    a1 += __shfl_xor(a1, o);
+    a2 += __shfl_xor(a2, o);
+  }
+  if (lane == 0) {
+    misc[0] = a0; misc[1] = a1; misc[2] = a2;
+    misc[3] = ux; misc[4] = uy; misc[5] = uz;
+    misc[6] = 1.0;
+  }
+}
+
+// Plain-path dq/dr in FP64, (3, D) per pair like soap_cart_der
+// (src/gap_interface.f90:312-317), the self pair holding -sum of the others
+// (src/gap.f90:275-277). One workgroup per site. The site's coefficients c,
+// its power spectrum and H = d(q . p)/dc stay in LDS; the pairs go four at a
+// time: each wavefront reduces one pair to its channel tables, then the 256
+// threads stream the four slabs. Thread t owns elements e = t + 256 j of a
+// slab (e = 3 d + k, k the Cartesian component), so each store instruction
+// writes 2 KB of consecutive doubles, and it keeps the self-pair sums of its
+// elements in registers. J * 256 >= 3 D.
+template <int J>
+__global__ __launch_bounds__(WG, 2) void k_soap_der(SoapParams P, SiteArgs a, int n_sites, double* __restrict__ q_out,
+                                                    double* __restrict__ pnorm_out, double* __restrict__ der_out) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* sm = reinterpret_cast<double*>(smem_raw);
+  const int site = blockIdx.x;
+  if (site >= n_sites) return;
+  const int tid = threadIdx.x, w = tid >> 6;
+  const int NLM = P.NLM, N = P.N, L1 = P.L + 1, D = P.D, D3 = 3 * D, NL = N * L1;
+  double* c = sm;
+  double* H = c + N * NLM;
+  double* X = H + N * NLM;  // p, then q: kept for the whole site
+  double* U = X + D;
+  double* red = U;
+  double* wscr = red + WG;
+  const int64_t b0 = a.site_beg[site];
+  const int nn = (int)(a.site_beg[site + 1] - b0);
+  site_coefficients<double>(P, a, b0, nn, c, wscr);
+  const double nrm = power_spectrum<double>(P, c, X, red);
+  double* qrow = q_out + (size_t)site * P.Dpad;
+  for (int d = tid; d < P.Dpad; d += WG) qrow[d] = d < D ? X[d] : 0.0;
+  if (tid == 0) pnorm_out[site] = nrm;
+  contract_channel<double>(P, X, c, H);
+  // element codes: i1 = n L1 + l | i2 = n' L1 + l << 8 | k << 16 | (n == n') << 18 | d << 19
+  int code[J];
+  double self[J];
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int e = tid + WG * j;
+    code[j] = -1;
+    self[j] = 0.0;
+    if (e < D3) {
+      const int d = e / 3, k = e - 3 * d;
+      const int cd = P.dtab[d];
+      const int n = cd & 0xff, np = (cd >> 8) & 0xff, l = cd >> 16;
+      code[j] = (n * L1 + l) | ((np * L1 + l) << 8) | (k << 16) | ((n == np ? 1 : 0) << 18) | (d << 19);
+    }
+  }
+  const double inv_n = nrm > 0.0 ? 1.0 / nrm : 0.0;
+  const bool site_ok = nrm > 0.0;
+  const int SL = DerSlot::size(NL, P.L, NLM);
+  __syncthreads();  // red and the coefficient scratch are dead: the slots overlay them
+  for (int g = 1; g < nn; g += 4) {
+    if (g + w < nn) der_pair_tables(P, a, b0 + g + w, site_ok, c, H, U + w * SL);
+    lds_barrier();
+    const int cnt = min(4, nn - g);
+    double* out = der_out + (size_t)(b0 + g) * D3 + tid;
+    // element-major: an element's decoded indices are live only while its four pairs are written
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      int cj = code[j];
+      if (cj < 0) continue;
+      // opaque to the optimiser: the decoded fields are not hoisted out of the pair loop, where
+      // they would stay live (16 elements' worth) across the table phase
+      asm volatile("" : "+v"(cj));
+      const int i1 = cj & 0xff, i2 = (cj >> 8) & 0xff, k = (cj >> 16) & 3;
+      const double wgt = ((cj >> 18) & 1) ? 1.0 : M_SQRT2;
+      const double qd = X[cj >> 19];
+      for (int ww = 0; ww < cnt; ++ww) {
+        const double* s_t = U + ww * SL;
+        const double* v_t = s_t + NL;
+        const double* E_t = v_t + 3 * NL;
+        const double* Dd_t = E_t + NL;
+        const double* misc = Dd_t + NL;
+        double o = 0.0;  // zero for a pair the descriptor does not see: q does not depend on it
+        if (misc[6] != 0.0) {
+          const double rad = Dd_t[i1] * s_t[i2] + Dd_t[i2] * s_t[i1];
+          const double dp = wgt * (rad * misc[3 + k] + E_t[i1] * v_t[k * NL + i2] + E_t[i2] * v_t[k * NL + i1]);
+          o = (dp - qd * misc[k]) * inv_n;
+        }
+        out[(size_t)ww * D3 + WG * j] = o;
+        self[j] -= o;
+      }
+    }
+    lds_barrier();
+  }
+  double* out = der_out + (size_t)b0 * D3;
+#pragma unroll
+  for (int j = 0; j < J; ++j)
+    if (code[j] >= 0) out[tid + WG * j] = self[j];
+}
+
+// ---- The same dq/dr for a compile-time l_max and at most two species of at
+// most eight radial functions each. Every wavefront works on its own pairs in
+// both pair phases; the site's pair data, the radial basis and the nodes sit in
+// registers, so a pair's set-up does not wait on memory.
+
+template <int LL> struct DerL {
+  static constexpr int L1 = LL + 1, NLM = L1 * L1, NL = 16 * L1;
+  static constexpr int PS = 2 * TGX_MAX_Q + 2 * TGX_MAX_NB + 2 * L1 + 4 * NLM;  // PairScratch
+  static constexpr int SL = 6 * NL + 8 + PS;                                    // one pair slot
+};
+
+__device__ __forceinline__ double lane_read(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Register copies, per lane: pair 1 + lane of the site; B[s][n][q] at
+// n = lane / 8, q = lane % 8 + 8 i; the node x_q of each species at q = lane.
+struct DerRegs {
+  int spec;
+  double r, x, y, z;
+  double B[2][8];
+  double xq[2];
+  __device__ void load(const SoapParams& P, const SiteArgs& a, int64_t b0, int nn, int lane) {
+    const int k = 1 + lane;
+    spec = -1;
+    r = 1.0; x = y = z = 0.0;
+    if (k < nn) {
+      spec = a.pr_spec[b0 + k];
+      r = a.pr_r[b0 + k];
+      x = a.pr_xyz[3 * (b0 + k)];
+      y = a.pr_xyz[3 * (b0 + k) + 1];
+      z = a.pr_xyz[3 * (b0 + k) + 2];
+    }
+    const int n = lane >> 3, part = lane & 7;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const bool have = s < P.n_species;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) B[s][i] = have ? P.B[((size_t)s * TGX_MAX_NB + n) * TGX_MAX_Q + part + 8 * i] : 0.0;
+      xq[s] = have ? P.xq[s * TGX_MAX_Q + lane] : 0.0;
+    }
+  }
+};
+
+struct DerPair {
+  int s;
+  bool ok;
+  double r, ux, uy, uz;
+};
+
+// Pair k >= 1 of the site (wavefront-uniform): from the lanes for k <= 64, else from memory.
+__device__ __forceinline__ DerPair der_pair(const SoapParams& P, const SiteArgs& a, const DerRegs& R, int64_t b0,
+                                            int k) {
+  DerPair q;
+  double x, y, z;
+  if (k <= 64) {
+    q.s = __builtin_amdgcn_readlane(R.spec, k - 1);
+    q.r = lane_read(R.r, k - 1);
+    x = lane_read(R.x, k - 1);
+    y = lane_read(R.y, k - 1);
+    z = lane_read(R.z, k - 1);
+  } else {
+    const int64_t p = b0 + k;
+    q.s = a.pr_spec[p];
+    q.r = a.pr_r[p];
+    x = a.pr_xyz[3 * p];
+    y = a.pr_xyz[3 * p + 1];
+    z = a.pr_xyz[3 * p + 2];
+  }
+  // wavefront-uniform: the species parameters then come from scalar loads
+  q.s = __builtin_amdgcn_readfirstlane(q.s);
+  q.r = lane_read(q.r, 0);
+  x = lane_read(x, 0);
+  y = lane_read(y, 0);
+  z = lane_read(z, 0);
+  q.ok = q.s >= 0 && q.r < P.sp[q.s].rc;
+  const double inv = 1.0 / q.r;
+  q.ux = x * inv;
+  q.uy = y * inv;
+  q.uz = z * inv;
+  return q;
+}
+
+// pair_gauss with the nodes from registers (lane q < nq) and the damping on lane 63.
+__device__ __forceinline__ void der_gauss(const SoapParams& P, const DerRegs& R, int s, double r, int lane,
+                                          PairScratch<double>& ps, bool need_grad) {
+  const SpeciesParams& sp = P.sp[s];
+  const double inv_rc = sp.inv_rc;
+  const double x = r * inv_rc;
+  const double sig = sp.sig_r * (1.0 + sp.sig_r_sc * r) * inv_rc;
+  const double dsig = (sp.sig_r * sp.sig_r_sc) * inv_rc;
+  const double inv_s2 = 1.0 / (sig * sig);
+  if (lane < sp.nq) {
+    const double dx = (s == 0 ? R.xq[0] : R.xq[1]) - x;
+    const double g = exp(-0.5 * dx * dx * inv_s2);
+    ps.G[lane] = g;
+    if (need_grad) ps.dG[lane] = g * (dx * inv_s2 * inv_rc + dx * dx * inv_s2 / sig * dsig);
+  }
+  if (lane == 63) {
+    const double st = sp.sig_t * (1.0 + sp.sig_t_sc * r);
+    const double dst = sp.sig_t * sp.sig_t_sc;
+    const double kk = st * st / (2.0 * r * r);
+    const double dk = st * st / (r * r * r) - st * dst / (r * r);  // -dk/dr
+    const double e1 = exp(-kk), e2 = e1 * e1;
+    double Al = 1, pw = 1;
+    for (int l = 0; l <= P.L; ++l) {
+      if (l > 0) { pw *= e2; Al *= pw; }
+      ps.A[l] = Al;
+      ps.dA[l] = Al * (double)(l * (l + 1)) * dk;
+    }
+  }
+}
+
+// pair_step2 for nb <= 8 (eight node slices per radial function) with B from
+// registers; the node values are read before they are summed, in the same order.
+__device__ __forceinline__ void der_radial(const SoapParams& P, const DerRegs& R, int s, double r, int lane,
+                                           PairScratch<double>& ps) {
+  const SpeciesParams& sp = P.sp[s];
+  const double inv_rc = sp.inv_rc;
+  const double x = r * inv_rc;
+  const double sig = sp.sig_r * (1.0 + sp.sig_r_sc * r) * inv_rc;
+  const double dsig = (sp.sig_r * sp.sig_r_sc) * inv_rc;
+  const double aa = sp.amp_a;
+  const double Pp = 1.0 + 2.0 * x * x * x - 3.0 * x * x;
+  const double dPp = (6.0 * x * x - 6.0 * x) * inv_rc;
+  double amp, damp;
+  if (aa == 0.0) {
+    amp = 1.0 / sig;
+    damp = -dsig / (sig * sig);
+  } else if (Pp <= 1.0e-10) {
+    amp = 0; damp = 0;
+  } else {
+    const double Pa = pow(Pp, aa);
+    amp = Pa / sig;
+    damp = aa * pow(Pp, aa - 1.0) * dPp / sig - Pa * dsig / (sig * sig);
+  }
+  amp *= sp.gs;
+  damp *= sp.gs;
+  const int n = lane >> 3, part = lane & 7, nq = sp.nq;
+  double g[8], dg[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int q = min(part + 8 * i, nq - 1);
+    g[i] = ps.G[q];
+    dg[i] = ps.dG[q];
+  }
+  double I = 0, dI = 0;
+  if (n < sp.nb) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (part + 8 * i < nq) {
+        const double b = s == 0 ? R.B[0][i] : R.B[1][i];
+        I += b * g[i];
+        dI += b * dg[i];
+      }
+  }
+  for (int o = 1; o < 8; o <<= 1) {
+    I += __shfl_xor(I, o);
+    dI += __shfl_xor(dI, o);
+  }
+  if (n < sp.nb && part == 0) {
+    ps.R[n] = amp * I;
+    ps.dR[n] = damp * I + amp * dI;
+  }
+}
+
+// One wavefront reduces a pair to its channel tables (layout of
+// der_pair_tables, 16 rows; all zero for a pair the descriptor does not see).
+// Lanes (n, k) = (lane & 15, lane >> 4), k < 3 the Cartesian component, sum
+// over m for their row.
+template <int LL>
+__device__ void der_tables_l(const SoapParams& P, const DerRegs& RG, const DerPair& pr, bool site_ok, const double* c,
+                             const double* H, const double* ynorm, const double* rinv, double* slot) {
+  using G = DerL<LL>;
+  constexpr int L1 = G::L1, NLM = G::NLM, NL = G::NL;
+  const int lane = threadIdx.x & 63;
+  double* s_t = slot;
+  double* v_t = s_t + NL;
+  double* E_t = v_t + 3 * NL;
+  double* Dd_t = E_t + NL;
+  double* misc = Dd_t + NL;
+  if (!site_ok || !pr.ok) {
+    for (int i = lane; i < 6 * NL + 6; i += 64) slot[i] = 0.0;
+    return;
+  }
+  PairScratch<double> ps;
+  ps.bind(misc + 8, LL, NLM);
+  const int s = pr.s;
+  const double r = pr.r, inv_r = 1.0 / r, ux = pr.ux, uy = pr.uy, uz = pr.uz;
+  der_gauss(P, RG, s, r, lane, ps, true);
+  harmonics_m<LL, true>(lane, ynorm, rinv, ux, uy, uz, ps.Y, ps.gY);
+  der_wave_sync();
+  der_radial(P, RG, s, r, lane, ps);
+  der_wave_sync();
+  const int nb = P.sp[s].nb, off = P.sp[s].off;
+  const int n = lane & 15, k = lane >> 4;
+  double qk = 0.0;
+  if (k < 3 && n < P.N) {
+    const double uk = k == 0 ? ux : (k == 1 ? uy : uz);
+    const double* gk = ps.gY + k;
+    const double* cr = c + n * NLM;
+    const int nl = n - off;
+    const bool inb = nl >= 0 && nl < nb;
+    const double Rn = inb ? ps.R[nl] : 0.0, dRn = inb ? ps.dR[nl] : 0.0;
+    // q . dp takes the block's rows of H: lane n reads row off + n (n < nb), others a
+    // valid row with weight 0, so one loop serves both sums and each harmonic is read once
+    const bool hb = n < nb;
+    const double* hr = H + (off + (hb ? n : 0)) * NLM;
+    const double hw = hb ? 1.0 : 0.0;
+    const double R0 = hb ? ps.R[n] : 0.0, dR0 = hb ? ps.dR[n] : 0.0;
+    // a fully unrolled (l, m) nest is one basic block whose hoisted loads exceed the register file
+#pragma unroll 1
+    for (int l = 0; l < L1; ++l) {
+      double s0 = 0.0, g0 = 0.0, h = 0.0, hk = 0.0;
+#pragma unroll 4
+      for (int lm = l * l; lm < (l + 1) * (l + 1); ++lm) {
+        const double y = ps.Y[lm], gy = gk[3 * lm];
+        const double v = cr[lm], hv = hr[lm];
+        s0 += y * v;
+        g0 += gy * v;
+        h += y * hv;
+        hk += gy * hv;
+      }
+      const double lf = (double)l;
+      v_t[k * NL + n * L1 + l] = (g0 - lf * s0 * uk) * inv_r;
+      if (k == 0) {
+        s_t[n * L1 + l] = s0;
+        E_t[n * L1 + l] = inb ? Rn * ps.A[l] : 0.0;
+        Dd_t[n * L1 + l] = inb ? dRn * ps.A[l] + Rn * ps.dA[l] : 0.0;
+      }
+      const double E = R0 * ps.A[l], Dd = dR0 * ps.A[l] + R0 * ps.dA[l];
+      qk += hw * (Dd * h * uk + E * (hk - lf * h * uk) * inv_r);
+    }
+  }
+  // sum over the 16 row lanes of each component
+  qk += __shfl_xor(qk, 1);
+  qk += __shfl_xor(qk, 2);
+  qk += __shfl_xor(qk, 4);
+  qk += __shfl_xor(qk, 8);
+  if ((lane & 15) == 0 && k < 3) misc[k] = qk;
+  if (lane == 0) {
+    misc[3] = ux; misc[4] = uy; misc[5] = uz;
+  }
+}
+
+// Plain-path dq/dr for l_max = LL (the output and element layout of
+// k_soap_der). Coefficients: wavefront w expands pairs 1 + w, 5 + w, ... with
+// lanes over (l, m) and the 16 rows in registers, and the four partial
+// expansions are added in wavefront order. Pairs: groups of four, one
+// wavefront per pair for the tables, then the 256 threads write the slabs;
+// table offsets are compile-time, so the write loop addresses LDS with
+// immediates and reads all four pairs of an element before it stores them.
+template <int LL, int J>
+__global__ __launch_bounds__(WG, 2) void k_soap_der_l(SoapParams P, SiteArgs a, int n_sites,
+                                                      double* __restrict__ q_out, double* __restrict__ pnorm_out,
+                                                      double* __restrict__ der_out) {
+  using G = DerL<LL>;
+  constexpr int L1 = G::L1, NLM = G::NLM, NL = G::NL, SL = G::SL;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* sm = reinterpret_cast<double*>(smem_raw);
+  const int site = blockIdx.x;
+  if (site >= n_sites) return;
+  const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int N = P.N, D = P.D, D3 = 3 * D, CN = N * NLM;
+  double* ynorm = sm;                          // NLM
+  double* rinv = ynorm + NLM;                  // L1
+  double* c = rinv + L1;                       // N NLM
+  double* H = c + CN;                          // N NLM
+  double* X = H + CN;                          // D: p, then q
+  double* U = X + D;                           // max(4 SL, 4 N NLM, WG)
+  const int64_t b0 = a.site_beg[site];
+  const int nn = (int)(a.site_beg[site + 1] - b0);
+  DerRegs RG;
+  RG.load(P, a, b0, nn, lane);
+  for (int i = tid; i < NLM; i += WG) ynorm[i] = P.ynorm[i];
+  if (tid < L1) rinv[tid] = tid > 0 ? 1.0 / (double)tid : 0.0;
+  lds_barrier();
+  // ---- expansion coefficients ----
+  {
+    double acc[2][16];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int n = 0; n < 16; ++n) acc[h][n] = 0.0;
+    PairScratch<double> ps;
+    ps.bind(U + w * G::PS, LL, NLM);
+    int lh[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) lh[h] = lm_l(lane + 64 * h);
+    for (int k = 1 + w; k < nn; k += 4) {
+      const DerPair pr = der_pair(P, a, RG, b0, k);
+      if (!pr.ok) continue;
+      der_gauss(P, RG, pr.s, pr.r, lane, ps, true);
+      harmonics_m<LL, false>(lane, ynorm, rinv, pr.ux, pr.uy, pr.uz, ps.Y, nullptr);
+      der_wave_sync();
+      der_radial(P, RG, pr.s, pr.r, lane, ps);
+      der_wave_sync();
+      const int nb = P.sp[pr.s].nb, off = P.sp[pr.s].off;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int lm = lane + 64 * h;
+        if (lm < NLM) {
+          const double al = ps.A[lh[h]], yv = ps.Y[lm];
+#pragma unroll
+          for (int n = 0; n < 16; ++n) {
+            const int nl = n - off;
+            if (nl >= 0 && nl < nb) acc[h][n] += ps.R[nl] * al * yv;
+          }
+        }
+      }
+      der_wave_sync();
+    }
+    lds_barrier();  // the pair scratch is dead: the partial sums overlay it
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int lm = lane + 64 * h;
+      if (lm < NLM)
+#pragma unroll
+        for (int n = 0; n < 16; ++n)
+          if (n < N) U[w * CN + n * NLM + lm] = acc[h][n];
+    }
+  }
+  lds_barrier();
+  for (int e = tid; e < CN; e += WG) c[e] = ((U[e] + U[CN + e]) + U[2 * CN + e]) + U[3 * CN + e];
+  lds_barrier();
+  {
+    // central atom (self pair): its own species block, l = 0
+    const int s = a.pr_spec[b0];
+    if (s >= 0)
+      for (int n = tid; n < P.sp[s].nb; n += WG) c[(P.sp[s].off + n) * NLM] += P.cen[s * TGX_MAX_NB + n];
+  }
+  lds_barrier();
+  const double nrm = power_spectrum<double>(P, c, X, U);
+  double* qrow = q_out + (size_t)site * P.Dpad;
+  for (int d = tid; d < P.Dpad; d += WG) qrow[d] = d < D ? X[d] : 0.0;
+  if (tid == 0) pnorm_out[site] = nrm;
+  contract_channel<double>(P, X, c, H);
+  int code[J];
+  double self[J];
+#pragma unroll
+  for (int j = 0; j < J; ++j) code[j] = P.dtab[min((tid + WG * j) / 3, D - 1)];  // all loads in flight at once
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int e = tid + WG * j;
+    const int d = e / 3, k = e - 3 * d;
+    const int cd = code[j];
+    const int n = cd & 0xff, np = (cd >> 8) & 0xff, l = cd >> 16;
+    code[j] = e < D3 ? (n * L1 + l) | ((np * L1 + l) << 8) | (k << 16) | ((n == np ? 1 : 0) << 18) | (d << 19) : -1;
+    self[j] = 0.0;
+  }
+  const double inv_n = nrm > 0.0 ? 1.0 / nrm : 0.0;
+  const bool site_ok = nrm > 0.0;
+  lds_barrier();
+  for (int g = 1; g < nn; g += 4) {
+    if (g + w < nn) der_tables_l<LL>(P, RG, der_pair(P, a, RG, b0, g + w), site_ok, c, H, ynorm, rinv, U + w * SL);
+    lds_barrier();
+    const int cnt = min(4, nn - g);
+    double* out = der_out + (size_t)(b0 + g) * D3 + tid;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      if (code[j] < 0) continue;
+      // g >> 30 is zero (g < 2^30) but changes with the loop as far as the optimiser can tell: the
+      // decoded fields are rebuilt per group instead of being hoisted and kept live across the tables
+      const int cj = code[j] + (g >> 30);
+      const int i1 = cj & 0xff, i2 = (cj >> 8) & 0xff, k = (cj >> 16) & 3;
+      const double wgt = ((cj >> 18) & 1) ? 1.0 : M_SQRT2;
+      const double qd = X[cj >> 19];
+      const double* t1 = U + i1;
+      const double* t2 = U + i2;
+      const double* v1 = U + NL + k * NL + i1;
+      const double* v2 = U + NL + k * NL + i2;
+      const double* mk = U + 6 * NL + k;
+      // slots past cnt hold stale tables: computed, never stored
+      double o[4];
+#pragma unroll
+      for (int ww = 0; ww < 4; ++ww) {
+        const int o0 = ww * SL;
+        const double rad = t1[o0 + 5 * NL] * t2[o0] + t2[o0 + 5 * NL] * t1[o0];
+        const double dp = wgt * (rad * mk[o0 + 3] + t1[o0 + 4 * NL] * v2[o0] + t2[o0 + 4 * NL] * v1[o0]);
+        o[ww] = (dp - qd * mk[o0]) * inv_n;
+      }
+#pragma unroll
+      for (int ww = 0; ww < 4; ++ww)
+        if (ww < cnt) {
+          out[(size_t)ww * D3 + WG * j] = o[ww];
+          self[j] -= o[ww];
+        }
+    }
+    lds_barrier();
+  }
+  double* out = der_out + (size_t)b0 * D3;
+#pragma unroll
+  for (int j = 0; j < J; ++j)
+    if (code[j] >= 0) out[tid + WG * j] = self[j];
+}
+
 // Plain-path pair force: f_p = Qss_i . soap_cart_der(:, :, p), one wavefront per
 // pair, streamed over the (3, D) slab. This is the scatter of src/gap.f90:206-224.
 __global__ __launch_bounds__(256) void k_plain_force(int n_sites, const int64_t* __restrict__ site_beg, int D,
@@ -560,18 +1221,55 @@ int tgx_soap_descriptors(tgx_ctx* ctx, tgx_model_t* m, int path, bool with_der) 
     TGX_TRY(m->q64.ensure(sizeof(double) * nsp * m->Dpad));
     if (with_der) {
       TGX_TRY(m->soap_der.ensure(sizeof(double) * (size_t)(m->n_pairs_b > 0 ? m->n_pairs_b : 1) * 3 * m->D));
-      const size_t lds = soap_lds_bytes(m->P, 1, sizeof(double));
-      TGX_TRY(check_lds(lds));
-      const int kmax = (m->P.D + WG - 1) / WG;
+      const SoapParams& P = m->P;
+      const int NL = P.N * (P.L + 1);
+      const int j_need = (3 * P.D + WG - 1) / WG;
+      bool small_blocks = P.n_species <= 2;
+      for (int sp = 0; sp < P.n_species; ++sp) small_blocks = small_blocks && P.sp[sp].nb <= 8 && P.sp[sp].nq <= 64;
+      if (P.L == 8 && P.N <= 16 && small_blocks && j_need <= 16 && P.D < 4096) {
+        using G = DerL<8>;
+        const size_t cn = (size_t)P.N * P.NLM;
+        size_t u = 4 * (size_t)G::SL;
+        if (4 * cn > u) u = 4 * cn;
+        if ((size_t)WG > u) u = WG;
+        const size_t lds = sizeof(double) * ((size_t)G::NLM + G::L1 + 2 * cn + P.D + u);
+        TGX_TRY(check_lds(lds));
+#define TGX_SOAP_DER(J)                                                                                     \
+  TGX_LAUNCH(ctx, "soap_site_der_f64", (k_soap_der_l<8, J>), dim3(ns), dim3(WG), lds, m->P, a, ns, \
+             m->q64.as<double>(), m->pnorm.as<double>(), m->soap_der.as<double>())
+        if (j_need <= 4) TGX_SOAP_DER(4);
+        else if (j_need <= 8) TGX_SOAP_DER(8);
+        else if (j_need <= 12) TGX_SOAP_DER(12);
+        else TGX_SOAP_DER(16);
+#undef TGX_SOAP_DER
+      } else if (j_need <= 16 && NL <= 255 && P.D < 4096) {
+        // coefficient, H and four pair slots; the slots overlay the spectrum and pair scratch
+        const size_t head = 2 * (size_t)P.N * P.NLM + P.D;
+        const size_t ps = (size_t)(2 * TGX_MAX_Q + 2 * TGX_MAX_NB + 2 * (P.L + 1) + 4 * P.NLM);
+        const size_t u1 = (size_t)WG + 4 * ps;
+        const size_t u2 = 4 * (size_t)(6 * NL + 8 + ps);
+        const size_t lds = sizeof(double) * (head + (u1 > u2 ? u1 : u2));
+        TGX_TRY(check_lds(lds));
+#define TGX_SOAP_DER(J)                                                                                           \
+  TGX_LAUNCH(ctx, "soap_site_der_f64", (k_soap_der<J>), dim3(ns), dim3(WG), lds, m->P, a, ns, m->q64.as<double>(), \
+             m->pnorm.as<double>(), m->soap_der.as<double>())
+        if (j_need <= 4) TGX_SOAP_DER(4);
+        else if (j_need <= 8) TGX_SOAP_DER(8);
+        else if (j_need <= 12) TGX_SOAP_DER(12);
+        else TGX_SOAP_DER(16);
+#undef TGX_SOAP_DER
+      } else {
+        const size_t lds = soap_lds_bytes(m->P, 1, sizeof(double));
+        TGX_TRY(check_lds(lds));
+        const int kmax = (m->P.D + WG - 1) / WG;
 #define TGX_SOAP_DER(K)                                                                                 \
   TGX_LAUNCH(ctx, "soap_site_der_f64", (k_soap_site<double, 1, K>), dim3(ns), dim3(WG), lds, m->P, a, ns, \
              m->q64.as<double>(), m->pnorm.as<double>(), m->soap_der.as<double>(), (const int*)nullptr)
-      if (kmax <= 2) TGX_SOAP_DER(2);
-      else if (kmax <= 5) TGX_SOAP_DER(5);
-      else if (kmax <= 8) TGX_SOAP_DER(8);
-      else if (kmax <= 12) TGX_SOAP_DER(12);
-      else TGX_SOAP_DER(24);
+        if (kmax <= 8) TGX_SOAP_DER(8);
+        else if (kmax <= 12) TGX_SOAP_DER(12);
+        else TGX_SOAP_DER(24);
 #undef TGX_SOAP_DER
+      }
     } else {
       TGX_TRY(tgx_desc_wave(ctx, m, true, nullptr, ns, m->q64.p, m->pnorm.p));
     }
