@@ -163,6 +163,35 @@ def test_surrogate_energies_match_mlp_oracle(ctx):
     assert np.abs(en - e_ref).max() <= 2e-2 * np.abs(e_ref + 0.5).max()
 
 
+def test_surrogate_65k_atoms_match_mlp_oracle(ctx):
+    """65,536 atoms, one network per block (the benchmark's GEMM row counts):
+    site energies of each block against the oracle network on the FP64
+    descriptors, as above."""
+    _, models = H.subset(500, 4096)
+    sysm = S.binary_system(65536, seed=2001)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    ctx.build_neighbors_list(3.7, fetch=False)
+    ids = H.load_models(ctx, models)
+    nets = []
+    for k, (mid, (h, _, _)) in enumerate(zip(ids, models)):
+        net = SG.MLP(ctx, h.dim, 256, 256, seed=31 + k)
+        net.set_normalization(-0.5, 0.03)
+        net.install(mid, 0)
+        nets.append(net)
+    SG.set_gate(ctx, 1, 1e9)
+    ctx.compute("surrogate")
+    en, F = ctx.results()
+    assert np.isfinite(F).all()
+    for k, (mid, (h, _, _)) in enumerate(zip(ids, models)):
+        ctx.soap_backend_begin(mid)
+        q, _ = ctx.get_soap(derivatives=False)
+        ctx.soap_backend_end()
+        e_ref = -0.5 + 0.03 * MO.forward(nets[k].get_weights(), q, emulate_bf16=True)["E"]
+        e_blk = en[sysm.species == h.central_species]
+        assert e_blk.shape == e_ref.shape
+        assert np.abs(e_blk - e_ref).max() <= 2e-2 * np.abs(e_ref + 0.5).max()
+
+
 def test_surrogate_ensemble_is_member_mean(ctx):
     """Two members with different normalisations: energies are the mean of the
     single-member runs, the spread their difference, and forces (linear in

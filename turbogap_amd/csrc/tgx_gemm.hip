@@ -80,7 +80,7 @@ template <bool F64> __device__ __forceinline__ int tile_row(int lane, int r) {
 // workgroup tile BMT rows tall with two wavefronts along N.
 template <int EPI, class OutT, class AccT, class VecT, bool F64, bool ZINT, int TI = 4, int TJ = 4, int WM = 64,
           int WN = 64, int BMT = BM>
-__device__ void epilogue(const VecT (&acc)[TI][TJ], int m0, int n0, int tn, int wm, int wn, const EpiArgs& e,
+__device__ __forceinline__ void epilogue(const VecT (&acc)[TI][TJ], int m0, int n0, int tn, int wm, int wn, const EpiArgs& e,
                          double* sE) {
   const int lane = threadIdx.x & 63;
   const int rbase = m0 + wm * WM;
@@ -326,6 +326,16 @@ static EpiArgs gap_epi(tgx_model_t* m) {
   return e;
 }
 
+// BF16 NT GEMM launch (128 x 128 tiles).
+template <int EPI, class OutT, bool ZINT>
+static int launch_bf16(tgx_ctx* ctx, const char* name, int M, int N, int K, int batch, const __bf16* A, int lda,
+                       const __bf16* B, int ldb, EpiArgs e) {
+  TGX_REQUIRE(M % BM == 0 && N % BN == 0 && K % 64 == 0 && lda % 8 == 0 && ldb % 8 == 0,
+              "gemm_bf16 %s: unpadded shape %d x %d x %d", name, M, N, K);
+  TGX_LAUNCH(ctx, name, (k_gemm_bf16<EPI, OutT, ZINT>), dim3(N / BN, M / BM, batch), dim3(256), 0, K, A, lda, B, ldb, e);
+  return TGX_OK;
+}
+
 static int launch_f64(tgx_ctx* ctx, int epi, int M, int N, int K, const double* A, int lda, const double* B, int ldb,
                       EpiArgs e, const char* name) {
   TGX_REQUIRE(M % BM == 0 && N % BN == 0 && K % 16 == 0, "gemm_f64: unpadded shape %d x %d x %d", M, N, K);
@@ -363,13 +373,12 @@ int tgx_gap_contract(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces) {
     e.alpha_f = m->alpha_f.as<float>();
     e.out = m->kdbf.p;
     e.ldo = Sp;
-    TGX_REQUIRE(Dp % 64 == 0 && Sp % 64 == 0, "gemm_bf16: unpadded shape");
     if (e.zeta_int >= 1)
-      TGX_LAUNCH(ctx, "gemm_kernel_bf16", (k_gemm_bf16<EPI_GAP, __bf16, true>), dim3(Sp / BN, Mp / BM), dim3(256), 0,
-                 Dp, m->qbf.as<__bf16>(), Dp, m->Qs_bf.as<__bf16>(), Dp, e);
+      TGX_TRY((launch_bf16<EPI_GAP, __bf16, true>(ctx, "gemm_kernel_bf16", Mp, Sp, Dp, 1, m->qbf.as<__bf16>(), Dp,
+                                                   m->Qs_bf.as<__bf16>(), Dp, e)));
     else
-      TGX_LAUNCH(ctx, "gemm_kernel_bf16", (k_gemm_bf16<EPI_GAP, __bf16, false>), dim3(Sp / BN, Mp / BM), dim3(256), 0,
-                 Dp, m->qbf.as<__bf16>(), Dp, m->Qs_bf.as<__bf16>(), Dp, e);
+      TGX_TRY((launch_bf16<EPI_GAP, __bf16, false>(ctx, "gemm_kernel_bf16", Mp, Sp, Dp, 1, m->qbf.as<__bf16>(), Dp,
+                                                    m->Qs_bf.as<__bf16>(), Dp, e)));
   }
   TGX_LAUNCH(ctx, "energy_finalize", k_energy_finalize, dim3((ns + 255) / 256), dim3(256), 0, ns,
              m->epart.as<double>(), n_tiles, m->delta * m->delta, m->e0, m->e_site.as<double>());
@@ -386,8 +395,8 @@ int tgx_gap_contract(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces) {
     TGX_TRY(m->qss32.ensure(sizeof(float) * (size_t)Mp * Dp));
     s.out = m->qss32.p;
     s.ldo = Dp;
-    TGX_LAUNCH(ctx, "gemm_qss_bf16", (k_gemm_bf16<EPI_STORE, float, true>), dim3(Dp / BN, Mp / BM), dim3(256), 0, Sp,
-               m->kdbf.as<__bf16>(), Sp, m->QsT_bf.as<__bf16>(), Sp, s);
+    TGX_TRY((launch_bf16<EPI_STORE, float, true>(ctx, "gemm_qss_bf16", Mp, Dp, Sp, 1, m->kdbf.as<__bf16>(), Sp,
+                                                  m->QsT_bf.as<__bf16>(), Sp, s)));
   }
   return TGX_OK;
 }
@@ -396,14 +405,11 @@ int tgx_gap_contract(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces) {
 // M, N multiples of 128, K a multiple of 64, leading dimensions in elements.
 int tgx_gemm_bf16_nt(tgx_ctx* ctx, const char* name, int M, int N, int K, const __bf16* A, int lda, const __bf16* B,
                      int ldb, float* C, int ldc, double scale) {
-  TGX_REQUIRE(M % BM == 0 && N % BN == 0 && K % 64 == 0 && lda % 8 == 0 && ldb % 8 == 0,
-              "gemm_bf16_nt: unpadded shape %d x %d x %d", M, N, K);
   EpiArgs s{};
   s.pref = scale;
   s.out = C;
   s.ldo = ldc;
-  TGX_LAUNCH(ctx, name, (k_gemm_bf16<EPI_STORE, float, true>), dim3(N / BN, M / BM), dim3(256), 0, K, A, lda, B, ldb, s);
-  return TGX_OK;
+  return launch_bf16<EPI_STORE, float, true>(ctx, name, M, N, K, 1, A, lda, B, ldb, s);
 }
 
 int tgx_gemm_bf16(tgx_ctx* ctx, const char* name, const tgx_gemm_desc& d) {
@@ -425,22 +431,16 @@ int tgx_gemm_bf16(tgx_ctx* ctx, const char* name, const tgx_gemm_desc& d) {
   e.sAux = d.sAux;
   e.sBias = d.sBias;
   e.zscale = d.zscale;
-  const dim3 grid(d.N / BN, d.M / BM, d.batch);
   switch (d.epi) {
     case TGX_EPI_STORE_F32:
-      TGX_LAUNCH(ctx, name, (k_gemm_bf16<EPI_STORE, float, true>), grid, dim3(256), 0, d.K, d.A, d.lda, d.B, d.ldb, e);
-      break;
+      return launch_bf16<EPI_STORE, float, true>(ctx, name, d.M, d.N, d.K, d.batch, d.A, d.lda, d.B, d.ldb, e);
     case TGX_EPI_SILU:
-      TGX_LAUNCH(ctx, name, (k_gemm_bf16<EPI_SILU, __bf16, true>), grid, dim3(256), 0, d.K, d.A, d.lda, d.B, d.ldb, e);
-      break;
+      return launch_bf16<EPI_SILU, __bf16, true>(ctx, name, d.M, d.N, d.K, d.batch, d.A, d.lda, d.B, d.ldb, e);
     case TGX_EPI_SCALE_AUX:
-      TGX_LAUNCH(ctx, name, (k_gemm_bf16<EPI_SCALE_AUX, __bf16, true>), grid, dim3(256), 0, d.K, d.A, d.lda, d.B, d.ldb,
-                 e);
-      break;
+      return launch_bf16<EPI_SCALE_AUX, __bf16, true>(ctx, name, d.M, d.N, d.K, d.batch, d.A, d.lda, d.B, d.ldb, e);
     default:
       TGX_REQUIRE(false, "gemm_bf16 %s: unknown epilogue %d", name, d.epi);
   }
-  return TGX_OK;
 }
 
 int tgx_f32_to_bf16(tgx_ctx* ctx, const float* x, __bf16* y, int64_t n) {
