@@ -153,55 +153,71 @@ __device__ __forceinline__ void epilogue(const VecT (&acc)[TI][TJ], int m0, int 
 }
 
 // FP64 NT GEMM. M, N multiples of 128; K multiple of 16.
+// LDS holds both operands k-major ([k][row], row stride 144 doubles: the four
+// k rows an MFMA fragment read touches fall on disjoint bank halves), in two
+// stages: the global loads for stage s+1 are in flight during the MFMAs of
+// stage s and land in the other buffer, one barrier per stage. Threads 0-127
+// stage one row of A each, 128-255 one row of B (16 doubles, 128 contiguous
+// bytes), so a wavefront's LDS writes cover 64 consecutive rows.
 template <int EPI, bool ZINT>
-__global__ __launch_bounds__(256) void k_gemm_f64(int K, const double* __restrict__ A, int lda,
-                                                  const double* __restrict__ B, int ldb, EpiArgs e) {
-  constexpr int BK = 16, LDS_STRIDE = BM + 4;
-  __shared__ double As[BK * LDS_STRIDE];
-  __shared__ double Bs[BK * LDS_STRIDE];
+__global__ __launch_bounds__(256, 2) void k_gemm_f64(int K, const double* __restrict__ A, int lda,
+                                                     const double* __restrict__ B, int ldb, EpiArgs e) {
+  constexpr int BK = 16, LS = BM + 16;
+  __shared__ __align__(16) double As[2][BK * LS];
+  __shared__ __align__(16) double Bs[2][BK * LS];
   __shared__ double sE[2 * BM];
   int tm, tn;
   tile_of(tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
-  const int lrow = tid >> 1, lk = (tid & 1) * 8;
-  const double* Ag = A + (size_t)(m0 + lrow) * lda + lk;
-  const double* Bg = B + (size_t)(n0 + lrow) * ldb + lk;
+  const int srow = tid & 127;
+  const bool isB = tid >= 128;
+  const double* G = isB ? B + (size_t)(n0 + srow) * ldb : A + (size_t)(m0 + srow) * lda;
   f64x4 acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = (f64x4){0, 0, 0, 0};
-  double ra[8], rb[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) { ra[j] = Ag[j]; rb[j] = Bg[j]; }
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      As[(lk + j) * LDS_STRIDE + lrow] = ra[j];
-      Bs[(lk + j) * LDS_STRIDE + lrow] = rb[j];
-    }
-    __syncthreads();
-    if (k0 + BK < K) {
-#pragma unroll
-      for (int j = 0; j < 8; ++j) { ra[j] = Ag[k0 + BK + j]; rb[j] = Bg[k0 + BK + j]; }
-    }
+  double2 r0, r1, r2, r3, r4, r5, r6, r7;
+  auto gload = [&](int k) {
+    const double2* g = reinterpret_cast<const double2*>(G + k);
+    r0 = g[0]; r1 = g[1]; r2 = g[2]; r3 = g[3]; r4 = g[4]; r5 = g[5]; r6 = g[6]; r7 = g[7];
+  };
+  auto lstore = [&](int buf) {
+    double* d = (isB ? Bs[buf] : As[buf]) + srow;
+    d[0 * LS] = r0.x; d[1 * LS] = r0.y; d[2 * LS] = r1.x; d[3 * LS] = r1.y;
+    d[4 * LS] = r2.x; d[5 * LS] = r2.y; d[6 * LS] = r3.x; d[7 * LS] = r3.y;
+    d[8 * LS] = r4.x; d[9 * LS] = r4.y; d[10 * LS] = r5.x; d[11 * LS] = r5.y;
+    d[12 * LS] = r6.x; d[13 * LS] = r6.y; d[14 * LS] = r7.x; d[15 * LS] = r7.y;
+  };
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  const int n_stages = K / BK;
+  for (int st = 0; st < n_stages; ++st) {
+    const int buf = st & 1;
+    gload(min(st + 1, n_stages - 1) * BK);  // the last stage reloads itself: always in bounds
+    __builtin_amdgcn_sched_barrier(0);
+    const double* as = As[buf] + wm * 64 + (lane & 15);
+    const double* bs = Bs[buf] + wn * 64 + (lane & 15);
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
-      const int kr = (ks * 4 + (lane >> 4)) * LDS_STRIDE;
+      const int kr = (ks * 4 + (lane >> 4)) * LS;
       double a[4], b[4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        a[t] = As[kr + wm * 64 + t * 16 + (lane & 15)];
-        b[t] = Bs[kr + wn * 64 + t * 16 + (lane & 15)];
+        a[t] = as[kr + t * 16];
+        b[t] = bs[kr + t * 16];
       }
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
+    __builtin_amdgcn_sched_barrier(0);
+    lstore(buf ^ 1);
+    __syncthreads();
   }
   epilogue<EPI, double, double, f64x4, true, ZINT>(acc, m0, n0, tn, wm, wn, e, sE);
 }
