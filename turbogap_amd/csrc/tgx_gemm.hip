@@ -314,6 +314,86 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int K, const __bf16* __restri
   epilogue<EPI, OutT, float, f32x4, false, ZINT>(acc, m0, n0, tn, wm, wn, e, sE);
 }
 
+// BF16 NT GEMM, workgroup tile 256 x 128: four wavefronts of 128 x 64 (8 x 4
+// MFMA tiles), BK = 32 in two LDS stages of (256 + 128) rows x 80 bytes
+// (61 KB: two workgroups per CU). A wavefront reads 12 fragments per 32 MFMAs
+// against 8 per 16 in the 128 x 128 kernel, a quarter less LDS traffic per
+// flop. M a multiple of 256, N of 128, K of 32.
+template <int EPI, class OutT, bool ZINT>
+__global__ __launch_bounds__(256, 2) void k_gemm_bf16_w(int K, const __bf16* __restrict__ A, int lda,
+                                                        const __bf16* __restrict__ B, int ldb, EpiArgs e) {
+  constexpr int TM = 256, BK = 32, LS = BK + 8;  // 80-byte rows: 16-byte fragment reads hit distinct banks
+  __shared__ __align__(16) __bf16 As[2][TM * LS];
+  __shared__ __align__(16) __bf16 Bs[2][BN * LS];
+  __shared__ double sE[2 * TM];
+  int tm, tn;
+  tile_of(tm, tn);
+  const int m0 = tm * TM, n0 = tn * BN;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const int lr = tid >> 2, lc = (tid & 3) * 8;  // rows lr + 64 j, 16-byte chunk lc
+  {
+    const int z = blockIdx.z;
+    A += z * e.sA;
+    B += z * e.sB;
+    e.out = reinterpret_cast<OutT*>(e.out) + z * e.sOut;
+    if (e.aux) e.aux += z * e.sAux;
+    if (e.bias) e.bias += z * e.sBias;
+    if (e.zscale) e.pref = e.zscale[z];
+  }
+  const __bf16* Ag = A + (size_t)(m0 + lr) * lda + lc;
+  const __bf16* Bg = B + (size_t)(n0 + lr) * ldb + lc;
+  const size_t a64 = (size_t)64 * lda, b64 = (size_t)64 * ldb;
+  f32x4 acc[8][4];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (f32x4){0, 0, 0, 0};
+  uint4 ra0, ra1, ra2, ra3, rb0, rb1;  // named: arrays of staging registers are demoted to private memory
+  auto gload = [&](int k) {
+    ra0 = *reinterpret_cast<const uint4*>(Ag + k);
+    ra1 = *reinterpret_cast<const uint4*>(Ag + a64 + k);
+    ra2 = *reinterpret_cast<const uint4*>(Ag + 2 * a64 + k);
+    ra3 = *reinterpret_cast<const uint4*>(Ag + 3 * a64 + k);
+    rb0 = *reinterpret_cast<const uint4*>(Bg + k);
+    rb1 = *reinterpret_cast<const uint4*>(Bg + b64 + k);
+  };
+  auto lstore = [&](int buf) {
+    __bf16* as = &As[buf][lr * LS + lc];
+    __bf16* bs = &Bs[buf][lr * LS + lc];
+    *reinterpret_cast<uint4*>(as) = ra0;
+    *reinterpret_cast<uint4*>(as + 64 * LS) = ra1;
+    *reinterpret_cast<uint4*>(as + 128 * LS) = ra2;
+    *reinterpret_cast<uint4*>(as + 192 * LS) = ra3;
+    *reinterpret_cast<uint4*>(bs) = rb0;
+    *reinterpret_cast<uint4*>(bs + 64 * LS) = rb1;
+  };
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  const int n_stages = K / BK;
+  for (int st = 0; st < n_stages; ++st) {
+    const int buf = st & 1;
+    gload(min(st + 1, n_stages - 1) * BK);
+    __builtin_amdgcn_sched_barrier(0);
+    const __bf16* as = &As[buf][(wm * 128 + (lane & 15)) * LS + 8 * (lane >> 4)];
+    const __bf16* bs = &Bs[buf][(wn * 64 + (lane & 15)) * LS + 8 * (lane >> 4)];
+    bf16x8 b[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) b[t] = *reinterpret_cast<const bf16x8*>(bs + t * 16 * LS);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(as + i * 16 * LS);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[i][j], 0, 0, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    lstore(buf ^ 1);
+    __syncthreads();
+  }
+  epilogue<EPI, OutT, float, f32x4, false, ZINT, 8, 4, 128, 64, TM>(acc, m0, n0, tn, wm, wn, e, sE);
+}
+
 // E_i = delta^2 sum_t epart[i][t] + e0, the partials summed in tile order.
 __global__ void k_energy_finalize(int n, const double* __restrict__ epart, int n_tiles, double d2, double e0,
                                   double* __restrict__ e_site) {
@@ -342,13 +422,22 @@ static EpiArgs gap_epi(tgx_model_t* m) {
   return e;
 }
 
-// BF16 NT GEMM launch (128 x 128 tiles).
+// BF16 NT GEMM launch: 256 x 128 tiles when M is a multiple of 256 and there
+// are at least 1,024 of them (two rounds of two workgroups per CU; with fewer,
+// the last round's imbalance costs more than the wider tile saves, measured on
+// the 32,768-row MLP shapes), else 128 x 128. The GAP epilogue writes one
+// energy partial per row and 128-column tile either way.
 template <int EPI, class OutT, bool ZINT>
 static int launch_bf16(tgx_ctx* ctx, const char* name, int M, int N, int K, int batch, const __bf16* A, int lda,
                        const __bf16* B, int ldb, EpiArgs e) {
   TGX_REQUIRE(M % BM == 0 && N % BN == 0 && K % 64 == 0 && lda % 8 == 0 && ldb % 8 == 0,
               "gemm_bf16 %s: unpadded shape %d x %d x %d", name, M, N, K);
-  TGX_LAUNCH(ctx, name, (k_gemm_bf16<EPI, OutT, ZINT>), dim3(N / BN, M / BM, batch), dim3(256), 0, K, A, lda, B, ldb, e);
+  if (M % 256 == 0 && (long)(M / 256) * (N / BN) * batch >= 1024)
+    TGX_LAUNCH(ctx, name, (k_gemm_bf16_w<EPI, OutT, ZINT>), dim3(N / BN, M / 256, batch), dim3(256), 0, K, A, lda, B,
+               ldb, e);
+  else
+    TGX_LAUNCH(ctx, name, (k_gemm_bf16<EPI, OutT, ZINT>), dim3(N / BN, M / BM, batch), dim3(256), 0, K, A, lda, B, ldb,
+               e);
   return TGX_OK;
 }
 
