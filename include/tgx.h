@@ -12,6 +12,8 @@
  *   tgx_set_system                      src/turbogap.f90:871-886    positions/species/box handed to the build
  *   tgx_build_neighbors_list            src/neighbors.f90:243-605   build_neighbors_list (count phase)
  *   tgx_get_neighbors_list              src/neighbors.f90:544-591   the allocatable outputs of build_neighbors_list
+ *   tgx_refresh_neighbors_list          src/neighbors.f90:554-585   build_neighbors_list, rebuild_neighbors_list = .false.
+ *   tgx_md_displacement                 src/turbogap_md.f90:451-471 the neighbors_buffer rebuild test
  *   tgx_soap_model_create               src/read_files.f90:660-724, 4641-5236  the soap_turbo block + sparse set
  *   tgx_soap_backend_begin / _end       src/gap.f90:49-64           soap_backend_begin / soap_backend_end
  *   tgx_get_gap_soap                    src/gap_interface.f90:41-480 get_gap_soap (marshal, get_soap, prediction, fold)
@@ -100,6 +102,11 @@ int tgx_set_positions_device(tgx_ctx* ctx, const double* d_positions);
  * do_list may be NULL (all sites). Writes the number of pairs. */
 int tgx_build_neighbors_list(tgx_ctx* ctx, double rcut_max, const int* indices, const int* do_list,
                              int64_t* n_atom_pairs);
+/* build_neighbors_list with rebuild_neighbors_list = .false.: the resident list
+ * keeps its pairs and order, and rjs/xyz are recomputed from the current
+ * positions with get_distance (thetas/phis on the next tgx_get_neighbors_list).
+ * A list built for a supercell (indices > 1) must be rebuilt instead. */
+int tgx_refresh_neighbors_list(tgx_ctx* ctx);
 /* Copy the list to host arrays (any may be NULL): n_neigh(n_sites),
  * neighbors_list(P) 1-based supercell indices, rjs(P), thetas(P), phis(P),
  * xyz(3,P), neighbor_species(P). */
@@ -211,6 +218,13 @@ int tgx_berendsen(tgx_ctx* ctx, double T0, double T, double tau, double dt);
 int tgx_remove_cm_vel(tgx_ctx* ctx);
 /* wrap positions back into the orthorhombic box. */
 int tgx_wrap_pbc(tgx_ctx* ctx);
+/* The neighbors_buffer test of turbogap_md.f90:451-471, after a velocity-Verlet
+ * step: adds this step's displacement (minimum image, so wrapping is not motion)
+ * to each atom's total since the last list build and sets *rebuild when an atom
+ * has moved more than neighbors_buffer / 2. (The reference compares the squared
+ * displacement with buffer / 2, which lets two atoms approach each other by more
+ * than the buffer before a rebuild.) reset = 1 after a rebuild zeroes the totals. */
+int tgx_md_displacement(tgx_ctx* ctx, double neighbors_buffer, int reset, int* rebuild);
 
 /* ---- profiling ----------------------------------------------------------- */
 /* Record HIP events around every kernel launch on the context's stream. */

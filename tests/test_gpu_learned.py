@@ -163,6 +163,27 @@ def test_surrogate_energies_match_mlp_oracle(ctx):
     assert np.abs(en - e_ref).max() <= 2e-2 * np.abs(e_ref + 0.5).max()
 
 
+def test_md_neighbour_buffer_matches_rebuild_every_step(ctx):
+    """neighbors_buffer = 0.5 A (src/turbogap_md.f90:451-471, turbogap_setup.f90:285): the list is
+    built at rcut + 0.5 and refreshed between builds; 30 steps at 1500 K give the energies of the
+    run that rebuilds every step to round-off, with fewer builds."""
+    sysm, models = H.subset(300, 4096)
+    mass = S.masses_for(sysm.species)
+    out = []
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    H.load_models(ctx, models)
+    for buf in (0.0, 0.5):
+        ctx.set_positions(sysm.positions)
+        md = MD(ctx, mass, maxwell_velocities(mass, 1500.0, seed=7), 3.7, "fp64_adjoint", neighbors_buffer=buf)
+        es = [md.step(1.0)[0] for _ in range(30)]
+        x, v = md.state()
+        out.append((np.array(es), x, md.n_builds))
+    (e0, x0, nb0), (e1, x1, nb1) = out
+    assert nb0 == 30 and 1 <= nb1 < 30
+    np.testing.assert_allclose(e1, e0, rtol=1e-11, atol=0)
+    assert np.abs(x1 - x0).max() <= 1e-9
+
+
 def test_surrogate_65k_atoms_match_mlp_oracle(ctx):
     """65,536 atoms, one network per block (the benchmark's GEMM row counts):
     site energies of each block against the oracle network on the FP64

@@ -61,6 +61,39 @@ def test_neighbors_do_list(ctx, orc):
     _compare_lists(g, o)
 
 
+def _pairs_within(nl, n, rc):
+    """{(i, j): (rjs, xyz)} of the pairs of a list with rjs < rc (self entries excluded)."""
+    out = {}
+    off = np.concatenate([[0], np.cumsum(nl["n_neigh"])])
+    for i in range(n):
+        for p in range(off[i] + 1, off[i + 1]):
+            if nl["rjs"][p] < rc:
+                key = (i, int(nl["neighbors_list"][p]))
+                assert key not in out
+                out[key] = (nl["rjs"][p], tuple(nl["xyz"][p]))
+    return out
+
+
+def test_neighbors_refresh_matches_rebuild(ctx, orc):
+    """rebuild_neighbors_list = .false. (src/neighbors.f90:554-585): after atoms move by less than
+    half the buffer, the resident list (built at rcut + buffer) holds every pair within rcut, with
+    rjs and xyz bit-exact to the oracle's fresh build at the new positions."""
+    sysm, _ = H.subset(300, 4096)
+    rc, buf = 3.7, 0.5
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    ctx.build_neighbors_list(rc + buf, fetch=False)
+    d = np.random.default_rng(3).normal(scale=0.08, size=sysm.positions.shape)
+    d *= np.minimum(1.0, 0.24 / np.linalg.norm(d, axis=1))[:, None]
+    x1 = sysm.positions + d
+    ctx.set_positions(x1)
+    g = ctx.build_neighbors_list(rc + buf, rebuild=False)
+    o = orc.build_neighbors_list(x1, sysm.species, sysm.box, rc)
+    gp, op = _pairs_within(g, sysm.n, rc), _pairs_within(o, sysm.n, rc)
+    assert gp.keys() == op.keys()
+    for k, v in op.items():
+        assert gp[k][0] == v[0] and gp[k][1] == v[1], k
+
+
 def test_soap_carbon(ctx, orc):
     sysm, h, Qs, al = H.carbon216(200)
     ctx.set_system(sysm.positions, sysm.species, sysm.box)

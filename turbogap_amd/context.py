@@ -95,19 +95,26 @@ class Context:
         pos = _f64(positions)
         L.check(self._lib.tgx_set_positions(self.h, L.dptr(pos)), "set_positions")
 
-    def build_neighbors_list(self, rcut_max: float, indices=(1, 1, 1), do_list=None, fetch: bool = True):
+    def build_neighbors_list(self, rcut_max: float, indices=(1, 1, 1), do_list=None, fetch: bool = True,
+                             rebuild: bool = True):
         """build_neighbors_list (src/neighbors.f90:243-605) on the device.
 
         Returns a dict with n_atom_pairs, n_neigh, neighbors_list (1-based),
         rjs, xyz (P, 3), thetas, phis, neighbor_species; with fetch=False only
         the pair count (the list stays resident for get_gap_soap/compute).
+        rebuild=False is rebuild_neighbors_list = .false.: the resident list keeps
+        its pairs and order and gets distances from the current positions.
         """
-        ind = _i32(indices)
-        dl = None if do_list is None else _i32(np.asarray(do_list).astype(np.int32))
-        npairs = C.c_int64()
-        L.check(self._lib.tgx_build_neighbors_list(self.h, float(rcut_max), L.iptr(ind), L.iptr(dl), C.byref(npairs)),
-                "build_neighbors_list")
-        P = npairs.value
+        if rebuild:
+            ind = _i32(indices)
+            dl = None if do_list is None else _i32(np.asarray(do_list).astype(np.int32))
+            npairs = C.c_int64()
+            L.check(self._lib.tgx_build_neighbors_list(self.h, float(rcut_max), L.iptr(ind), L.iptr(dl),
+                                                       C.byref(npairs)), "build_neighbors_list")
+            P = npairs.value
+        else:
+            L.check(self._lib.tgx_refresh_neighbors_list(self.h), "refresh_neighbors_list")
+            P = self.n_atom_pairs
         self.n_atom_pairs = P
         if not fetch:
             return {"n_atom_pairs": P}

@@ -154,9 +154,10 @@ extern "C" int tgx_finalize(tgx_ctx* ctx) {
                   &ctx->thetas, &ctx->phis, &ctx->nspec, &ctx->rev, &ctx->cell_of, &ctx->cell_cnt, &ctx->cell_start,
                   &ctx->cell_atoms, &ctx->scan_tmp, &ctx->do_list, &ctx->energies, &ctx->forces, &ctx->fglob,
                   &ctx->virial_part, &ctx->scratch, &ctx->red, &ctx->vel, &ctx->mass, &ctx->forces_prev,
-                  &ctx->pos_prev, &ctx->e_members, &ctx->gate_mask, &ctx->cell_pos};
+                  &ctx->pos_prev, &ctx->e_members, &ctx->gate_mask, &ctx->cell_pos, &ctx->md_disp, &ctx->md_flag};
   for (dbuf* b : bufs) b->release();
   if (ctx->gate_host) (void)hipHostFree(ctx->gate_host);
+  if (ctx->md_flag_host) (void)hipHostFree(ctx->md_flag_host);
   if (ctx->pin64) (void)hipHostFree(ctx->pin64);
   for (auto& r : ctx->prof) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -226,6 +227,11 @@ extern "C" int tgx_build_neighbors_list(tgx_ctx* ctx, double rcut_max, const int
   TGX_TRY(tgx_neighbors_run(ctx, rcut_max, indices, do_list));
   if (n_atom_pairs) *n_atom_pairs = ctx->n_pairs;
   return TGX_OK;
+}
+
+extern "C" int tgx_refresh_neighbors_list(tgx_ctx* ctx) {
+  TGX_CTX_CHECK(ctx);
+  return tgx_neighbors_refresh(ctx);
 }
 
 extern "C" int tgx_get_neighbors_list(tgx_ctx* ctx, int* n_neigh, int* neighbors_list, double* rjs, double* thetas,

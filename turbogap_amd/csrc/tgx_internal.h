@@ -169,6 +169,8 @@ struct tgx_ctx {
   dbuf energies, forces, fglob, virial_part, scratch, red;
   // md
   dbuf vel, mass, forces_prev, pos_prev;
+  dbuf md_disp, md_flag;      // displacement since the last list build, rebuild flag
+  int* md_flag_host = nullptr;  // pinned
   double dt_prev = 0.0;
   // surrogate gate
   int gate_members = 1;
@@ -210,6 +212,7 @@ void tgx_gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w);
 // Kernels' host entry points (one per translation unit).
 int tgx_neighbors_run(tgx_ctx* ctx, double rcut_max, const int* indices, const int* do_list);
 int tgx_neighbor_angles(tgx_ctx* ctx);
+int tgx_neighbors_refresh(tgx_ctx* ctx);
 int tgx_block_select(tgx_ctx* ctx, tgx_model_t* m);
 int tgx_soap_descriptors(tgx_ctx* ctx, tgx_model_t* m, int path, bool with_der);
 int tgx_gap_contract(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces);

@@ -65,6 +65,27 @@ __global__ void k_wrap(int n, double Lx, double Ly, double Lz, double* __restric
   }
 }
 
+// Displacement accumulated since the last list build (src/turbogap_md.f90:451-471),
+// taken by minimum image so that wrapping into the box is not motion; flag[0] is
+// raised when an atom has moved more than lim (half the neighbour buffer).
+__global__ void k_disp(int n, double Lx, double Ly, double Lz, const double* __restrict__ x,
+                       const double* __restrict__ xprev, double* __restrict__ disp, double lim2, int* __restrict__ flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double L[3] = {Lx, Ly, Lz};
+  double s2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const size_t j = 3 * (size_t)i + k;
+    double d = x[j] - xprev[j];
+    d = d - rint(d / L[k]) * L[k];
+    const double a = disp[j] + d;
+    disp[j] = a;
+    s2 = s2 + a * a;
+  }
+  if (s2 > lim2) atomicOr(flag, 1);
+}
+
 }  // namespace
 
 #define MD_CTX(ctx)                                                                \
@@ -169,5 +190,31 @@ extern "C" int tgx_wrap_pbc(tgx_ctx* ctx) {
               "tgx_wrap_pbc: orthorhombic boxes only");
   const int n = ctx->n_sites;
   TGX_LAUNCH(ctx, "md_wrap", k_wrap, dim3((n + 255) / 256), dim3(256), 0, n, b[0], b[4], b[8], ctx->pos.as<double>());
+  return TGX_OK;
+}
+
+extern "C" int tgx_md_displacement(tgx_ctx* ctx, double neighbors_buffer, int reset, int* rebuild) {
+  MD_CTX(ctx);
+  TGX_REQUIRE(rebuild && neighbors_buffer >= 0.0, "tgx_md_displacement: null output or negative buffer");
+  const double* b = ctx->box;
+  TGX_REQUIRE(b[1] == 0 && b[2] == 0 && b[3] == 0 && b[5] == 0 && b[6] == 0 && b[7] == 0,
+              "tgx_md_displacement: orthorhombic boxes only");
+  TGX_REQUIRE(ctx->pos_prev.p, "tgx_md_displacement: no velocity-Verlet step taken");
+  const int n = ctx->n_sites;
+  TGX_TRY(ctx->md_disp.ensure(sizeof(double) * 3 * (size_t)n));
+  TGX_TRY(ctx->md_flag.ensure(sizeof(int)));
+  if (!ctx->md_flag_host) TGX_CHECK_HIP(hipHostMalloc((void**)&ctx->md_flag_host, sizeof(int), hipHostMallocDefault));
+  if (reset) {
+    TGX_TRY(tgx_zero(ctx, ctx->md_disp.p, sizeof(double) * 3 * (size_t)n));
+    *rebuild = 0;
+    return TGX_OK;
+  }
+  TGX_TRY(tgx_zero(ctx, ctx->md_flag.p, sizeof(int)));
+  const double lim = 0.5 * neighbors_buffer;
+  TGX_LAUNCH(ctx, "md_disp", k_disp, dim3((n + 255) / 256), dim3(256), 0, n, b[0], b[4], b[8], ctx->pos.as<double>(),
+             ctx->pos_prev.as<double>(), ctx->md_disp.as<double>(), lim * lim, ctx->md_flag.as<int>());
+  TGX_CHECK_HIP(hipMemcpyAsync(ctx->md_flag_host, ctx->md_flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  *rebuild = *ctx->md_flag_host;
   return TGX_OK;
 }

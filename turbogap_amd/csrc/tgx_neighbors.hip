@@ -428,6 +428,32 @@ __global__ void k_reverse(int n, const int* __restrict__ n_neigh, const int64_t*
   }
 }
 
+// Distances of the resident list from the current positions, as
+// build_neighbors_list recomputes them for rebuild_neighbors_list = .false.
+// (src/neighbors.f90:554-585): the same pairs in the same order, the self
+// entry left at zero, get_distance for every other pair. One wavefront per atom.
+__global__ void k_refresh(int n, const int* __restrict__ n_neigh, const int64_t* __restrict__ off,
+                          const int* __restrict__ nlist, const double* __restrict__ pos, BoxInfo bx,
+                          double* __restrict__ rjs, double* __restrict__ xyz) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (i >= n) return;
+  const int64_t b = off[i];
+  const int nn = n_neigh[i];
+  const double pi[3] = {pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2]};
+  for (int t = 1 + lane; t < nn; t += 64) {
+    const int64_t p = b + t;
+    const int j = nlist[p] - 1;
+    const double pj[3] = {pos[3 * (size_t)j], pos[3 * (size_t)j + 1], pos[3 * (size_t)j + 2]};
+    double dist[3], d;
+    get_distance(pi, pj, bx, dist, d);
+    rjs[p] = d;
+    xyz[3 * p] = dist[0];
+    xyz[3 * p + 1] = dist[1];
+    xyz[3 * p + 2] = dist[2];
+  }
+}
+
 }  // namespace
 
 static void host_box_info(const double* box, BoxInfo& bx) {
@@ -544,6 +570,22 @@ int tgx_neighbors_run(tgx_ctx* ctx, double rcut_max, const int* indices, const i
              ctx->rev.as<int64_t>());
   ctx->rcut_max = rcut_max;
   ctx->have_list = true;
+  ctx->have_angles = false;
+  return TGX_OK;
+}
+
+// rebuild_neighbors_list = .false.: new distances for the resident list. Lists
+// of a supercell are rebuilt instead, as the reference's MD does
+// (src/turbogap_md.f90:461-463), because the replicas are not moved.
+int tgx_neighbors_refresh(tgx_ctx* ctx) {
+  TGX_REQUIRE(ctx->have_list, "refresh_neighbors_list: no list built");
+  TGX_REQUIRE(ctx->n_sites == ctx->n_sites_sc, "refresh_neighbors_list: a supercell list must be rebuilt");
+  const int n = ctx->n_sites;
+  BoxInfo bx;
+  host_box_info(ctx->box, bx);
+  TGX_LAUNCH(ctx, "nbr_refresh", k_refresh, dim3((n + NBW - 1) / NBW), dim3(64 * NBW), 0, n, ctx->n_neigh.as<int>(),
+             ctx->nl_off.as<int64_t>(), ctx->nlist.as<int>(), ctx->pos.as<double>(), bx, ctx->rjs.as<double>(),
+             ctx->xyz.as<double>());
   ctx->have_angles = false;
   return TGX_OK;
 }

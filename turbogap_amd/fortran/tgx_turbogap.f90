@@ -61,6 +61,15 @@ module tgx_turbogap
          integer(c_int) :: indices(*), do_list(*)
          integer(c_int64_t) :: npairs
       end function
+      integer(c_int) function c_tgx_set_positions(c, pos) bind(C, name="tgx_set_positions")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c
+         real(c_double) :: pos(*)
+      end function
+      integer(c_int) function c_tgx_refresh(c) bind(C, name="tgx_refresh_neighbors_list")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: c
+      end function
       integer(c_int) function c_tgx_get_nl(c, nn, nl, rjs, th, ph, xyz, ns) bind(C, name="tgx_get_neighbors_list")
          import :: c_int, c_ptr, c_double
          type(c_ptr), value :: c
@@ -138,9 +147,11 @@ contains
    ! build_neighbors_list with the reference's 20 arguments in its order
    ! (src/neighbors.f90:243-246) and the same outputs: callee-allocated arrays,
    ! 1-based supercell indices in neighbors_list, xyz(3, n_atom_pairs) = r_j - r_i.
-   ! The device always builds the list: with rebuild_neighbors_list = .false.
-   ! the pairs inside the cutoff are the same ones, in the device's order.
-   ! do_timing prints the build time; rank only selects who prints.
+   ! With rebuild_neighbors_list = .false. and a list already returned, the list
+   ! keeps its pairs and order and rjs, xyz, thetas, phis are recomputed from
+   ! positions into the same arrays (src/neighbors.f90:554-585); the box and
+   ! species are those of the last build. do_timing prints the build time; rank
+   ! only selects who prints.
    subroutine build_neighbors_list(positions, a_box, b_box, c_box, do_timing, species_supercell, rcut_max, &
                                    n_atom_pairs, rjs, thetas, phis, xyz, n_neigh, neighbors_list, neighbor_species, &
                                    n_sites, indices, rebuild_neighbors_list, do_list, rank)
@@ -157,6 +168,14 @@ contains
       integer(c_int64_t) :: np
       integer(8) :: t0, t1, rate
       if (do_timing) call system_clock(t0, rate)
+      if (.not. rebuild_neighbors_list .and. allocated(n_neigh) .and. allocated(rjs)) then
+         call tgx_check(c_tgx_set_positions(ctx, positions), "build_neighbors_list")
+         call tgx_check(c_tgx_refresh(ctx), "build_neighbors_list")
+         n_atom_pairs = size(rjs)
+         call tgx_check(c_tgx_get_nl(ctx, n_neigh, neighbors_list, rjs, thetas, phis, xyz, neighbor_species), &
+                        "build_neighbors_list")
+         return
+      end if
       box(1:3) = a_box; box(4:6) = b_box; box(7:9) = c_box
       ind = int(indices, c_int)
       allocate (spec(size(species_supercell)), dl(n_sites))

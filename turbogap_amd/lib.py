@@ -75,6 +75,7 @@ _SIGS = {
     "tgx_set_positions": [_P, _PD],
     "tgx_set_positions_device": [_P, _P],
     "tgx_build_neighbors_list": [_P, _D, _PI, _PI, C.POINTER(_I64)],
+    "tgx_refresh_neighbors_list": [_P],
     "tgx_get_neighbors_list": [_P, _PI, _PI, _PD, _PD, _PD, _PD, _PI],
     "tgx_soap_model_create": [_P, C.POINTER(SoapHypersC), _PI, _I, _D, _D, _D, _I, _I, _PD, _PD, _PI],
     "tgx_soap_backend_begin": [_P, _I],
@@ -109,6 +110,7 @@ _SIGS = {
     "tgx_berendsen": [_P, _D, _D, _D, _D],
     "tgx_remove_cm_vel": [_P],
     "tgx_wrap_pbc": [_P],
+    "tgx_md_displacement": [_P, _D, _I, _PI],
     "tgx_set_profiling": [_P, _I],
     "tgx_kernel_time": [_P, C.c_char_p, _PI, _PD],
     "tgx_reset_profile": [_P],

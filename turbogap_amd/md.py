@@ -1,8 +1,10 @@
 """Molecular dynamics on the device with the semantics of TurboGAP's md
 (src/md.f90:95-345) and its step order (src/turbogap_md.f90:160-470):
-forces at x(t) -> velocity_verlet -> E_kin, T -> Berendsen rescale, with the
-neighbour list rebuilt every step (neighbors_buffer = 0 rebuilds on any motion,
-src/turbogap_md.f90:452-470)."""
+forces at x(t) -> velocity_verlet -> E_kin, T -> Berendsen rescale. The
+neighbour list is built with rcut + neighbors_buffer (src/turbogap_setup.f90:285)
+and rebuilt when an atom has moved more than half the buffer since the last
+build (src/turbogap_md.f90:451-471); in between, its distances are recomputed
+(rebuild_neighbors_list = .false.). neighbors_buffer = 0 rebuilds every step."""
 from __future__ import annotations
 
 import ctypes as C
@@ -26,18 +28,34 @@ def maxwell_velocities(masses: np.ndarray, T: float, seed: int) -> np.ndarray:
 
 
 class MD:
-    def __init__(self, ctx, masses: np.ndarray, velocities: np.ndarray, rcut: float, path: str = "fp64_plain"):
+    def __init__(self, ctx, masses: np.ndarray, velocities: np.ndarray, rcut: float, path: str = "fp64_plain",
+                 neighbors_buffer: float = 0.0):
         self.ctx, self.rcut, self.path = ctx, rcut, path
+        self.buffer = float(neighbors_buffer)
         self.masses = np.ascontiguousarray(masses, dtype=np.float64)
         v = np.ascontiguousarray(velocities, dtype=np.float64)
         L.check(L.load().tgx_md_set_state(ctx.h, L.dptr(v), L.dptr(self.masses)), "md_set_state")
         self.step_count = 0
         self.energy = None
+        self.rebuild = True
+        self.n_builds = 0
 
     def forces(self, path=None):
-        self.ctx.build_neighbors_list(self.rcut, fetch=False)
+        if self.rebuild or self.buffer <= 0.0:
+            self.ctx.build_neighbors_list(self.rcut + self.buffer, fetch=False)
+            self.n_builds += 1
+            if self.buffer > 0.0:
+                self._displacement(reset=True)  # totals count from this build
+            self.rebuild = False
+        else:
+            self.ctx.build_neighbors_list(self.rcut + self.buffer, fetch=False, rebuild=False)
         self.energy, self.virial = self.ctx.compute(path or self.path)
         return self.energy
+
+    def _displacement(self, reset=False):
+        flag = C.c_int()
+        L.check(L.load().tgx_md_displacement(self.ctx.h, self.buffer, int(reset), C.byref(flag)), "md_displacement")
+        return bool(flag.value)
 
     def kinetic(self):
         ek, T = C.c_double(), C.c_double()
@@ -53,6 +71,8 @@ class MD:
         if T0 is not None:
             L.check(lib.tgx_berendsen(self.ctx.h, float(T0), float(T), float(tau), float(dt)), "berendsen")
         L.check(lib.tgx_wrap_pbc(self.ctx.h), "wrap_pbc")
+        if self.buffer > 0.0:
+            self.rebuild = self._displacement()
         self.step_count += 1
         return self.energy, ek, T
 
