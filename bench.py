@@ -201,7 +201,9 @@ def prepare_surrogate(ctx, ids, sysm, models, args):
 
 def production_md(ctx, sysm, args):
     """Surrogate MD on the benchmark box: args.nvt_steps of Berendsen NVT at
-    1500 K then NVE, dt = 1 fs, neighbour list rebuilt every step. Every
+    1500 K then NVE, dt = 1 fs, neighbour list built at rcut + args.neighbors_buffer
+    and rebuilt when an atom has moved more than half the buffer (every step
+    with a zero buffer), its distances refreshed in between. Every
     args.check_every-th step takes its forces from the FP64 GAP (the full-system
     check) after the surrogate has been evaluated on the same positions, and
     the surrogate's error there is recorded; gated sites fall back to the GAP on
@@ -212,7 +214,8 @@ def production_md(ctx, sysm, args):
     from turbogap_amd import structures as S
     mass = S.masses_for(sysm.species)
     ctx.set_positions(sysm.positions)
-    md = MD(ctx, mass, maxwell_velocities(mass, 1500.0, seed=4444), 3.7, "surrogate")
+    md = MD(ctx, mass, maxwell_velocities(mass, 1500.0, seed=4444), 3.7, "surrogate",
+            neighbors_buffer=args.neighbors_buffer)
     n = sysm.n
     e_nve, t_nve, gated, checks = [], [], [], []
     ctx.synchronize()
@@ -243,7 +246,8 @@ def production_md(ctx, sysm, args):
     out = {"steps": args.production_steps, "nvt_steps_1500K": args.nvt_steps, "dt_fs": 1.0,
            "atom_steps_per_s": n * args.production_steps / dt, "wall_s": dt,
            "energy_drift_meV_atom_ps": drift, "fallback_fraction": float(np.mean(gated)) if gated else None,
-           "check_every": args.check_every,
+           "check_every": args.check_every, "neighbors_buffer_A": args.neighbors_buffer,
+           "list_builds": md.n_builds,
            "checked_force_rmse_meV_A": float(np.mean([c["force_rmse_meV_A"] for c in checks])) if checks else None,
            "checked_energy_mae_meV_atom": float(np.mean([c["energy_mae_meV_atom"] for c in checks])) if checks else None,
            "checks": checks}
@@ -284,6 +288,8 @@ def main():
     ap.add_argument("--production-steps", type=int, default=2000)
     ap.add_argument("--nvt-steps", type=int, default=200)
     ap.add_argument("--check-every", type=int, default=100)
+    ap.add_argument("--neighbors-buffer", type=float, default=0.5,
+                    help="neighbour-list buffer (A) of the production MD; 0 rebuilds every step")
     ap.add_argument("--hbm-bytes", type=int, default=1 << 30, help="bytes of the streaming-copy reference (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
