@@ -76,6 +76,17 @@ class MD:
         self.step_count += 1
         return self.energy, ek, T
 
+    def write_frame(self, fh, species_names, cell, dt: float = 1.0):
+        """One extended-XYZ frame of the current state (write_extxyz, src/xyz.f90:169-487):
+        positions, velocities (A/fs), the forces and local energies of the last
+        force evaluation, lattice, temperature, time step, time and step."""
+        from . import xyz as X
+        x, v = self.state()
+        en, F = self.ctx.results()
+        _, T = self.kinetic()
+        X.write_extxyz(fh, species_names, x, velocities=v, forces=F, local_energies=en, cell=cell, temperature=T,
+                       time_step=dt, time=self.step_count * dt, step=self.step_count)
+
     def state(self):
         n = self.masses.shape[0]
         x = np.empty((n, 3))
