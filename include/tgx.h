@@ -21,6 +21,9 @@
  *   tgx_get_soap_energy_and_forces      src/gap.f90:66-259          get_soap_energy_and_forces
  *   tgx_compute                         src/turbogap.f90:1253-1380,1941-1954  the per-step SOAP loop and totals
  *   tgx_velocity_verlet etc.            src/md.f90:226-345          velocity_verlet / berendsen_thermostat / remove_cm_vel
+ *   tgx_distance_2b_create              src/read_files.f90:722-742,5238-5305  a distance_2b block and its sparse set
+ *   tgx_core_pot_create                 src/read_files.f90:5392-5510 a core_pot block, its table and spline
+ *   tgx_pair_energy_and_forces          src/gap.f90:400-563 / 565-721 get_2b_ / get_core_pot_energy_and_forces
  */
 #ifndef TGX_H
 #define TGX_H
@@ -203,6 +206,26 @@ int tgx_surrogate_install(tgx_ctx* ctx, int model_id, int member, tgx_mlp* mlp);
 int tgx_surrogate_gate(tgx_ctx* ctx, int n_members, double threshold);
 /* Of the last surrogate compute: fraction of centres gated, largest spread (eV). */
 int tgx_surrogate_stats(tgx_ctx* ctx, double* gated_fraction, double* max_spread);
+
+/* ---- pair-potential families on the resident list ------------------------ */
+/* distance_2b: centres of species1 or species2 with a neighbour of the other
+ * (species codes as in tgx_set_system), r < rcut, cosine cutoff over the last
+ * `buffer` Angstrom; E_i = e0 + delta^2 sum_s alpha_s cutoff_s fcut
+ * exp(-(r - Qs_s)^2 / (2 sigma^2)) over the centre's pairs. */
+int tgx_distance_2b_create(tgx_ctx* ctx, int species1, int species2, double rcut, double buffer, double delta,
+                           double sigma, double e0, int n_sparse, const double* Qs, const double* alphas,
+                           const double* cutoff, int* model_id);
+/* core_pot: the tabulated pair potential V(x) (n nodes, ascending; end slopes
+ * yp1, ypn, >= 0.99e30 for a natural end) as a cubic spline; nodes beyond
+ * `cutoff` are dropped and those in [cutoff - buffer, cutoff] damped
+ * (core_pot_cutoff / core_pot_buffer; 1e10 and 1 by default). E_i = 1/2 sum V(r)
+ * over the centre's pairs with r < max(x). */
+int tgx_core_pot_create(tgx_ctx* ctx, int species1, int species2, int n, const double* x, const double* V, double yp1,
+                        double ypn, double cutoff, double buffer, int* model_id);
+/* Energies (n_sites), forces (3, n_sites) on the centres and virial (3, 3) of
+ * one pair model on the resident list, as the reference's subroutines return
+ * them (any output may be NULL). */
+int tgx_pair_energy_and_forces(tgx_ctx* ctx, int model_id, double* energies, double* forces, double* virial);
 
 /* ---- MD (src/md.f90:226-345, src/turbogap_md.f90:160-470) --------------- */
 /* Velocities (3, n_sites) and masses (n_sites) in eV fs^2/A^2 live on the device. */

@@ -151,6 +151,40 @@ class Context:
         self.models[mid.value] = hypers
         return mid.value
 
+    def distance_2b_model(self, species1: int, species2: int, rcut: float, buffer: float, delta: float,
+                          sigma: float, Qs, alphas, cutoff, e0: float = 0.0) -> int:
+        """A distance_2b block (src/gap.f90:400-563): Qs (n_sparse,) distances,
+        alphas and cutoff (n_sparse,), species as the system's codes."""
+        q, a, c = _f64(Qs).reshape(-1), _f64(alphas).reshape(-1), _f64(cutoff).reshape(-1)
+        if not (q.shape == a.shape == c.shape):
+            raise ValueError("Qs, alphas and cutoff must have one entry per sparse point")
+        mid = C.c_int()
+        L.check(self._lib.tgx_distance_2b_create(self.h, int(species1), int(species2), float(rcut), float(buffer),
+                                                 float(delta), float(sigma), float(e0), q.shape[0], L.dptr(q),
+                                                 L.dptr(a), L.dptr(c), C.byref(mid)), "distance_2b_create")
+        return mid.value
+
+    def core_pot_model(self, species1: int, species2: int, x, V, yp1: float, ypn: float, cutoff: float = 1.0e10,
+                       buffer: float = 1.0) -> int:
+        """A core_pot block (src/gap.f90:565-721): the table V(x), end slopes
+        (>= 0.99e30 for a natural end), core_pot_cutoff / core_pot_buffer."""
+        xx, vv = _f64(x).reshape(-1), _f64(V).reshape(-1)
+        if xx.shape != vv.shape:
+            raise ValueError("x and V must have the same length")
+        mid = C.c_int()
+        L.check(self._lib.tgx_core_pot_create(self.h, int(species1), int(species2), xx.shape[0], L.dptr(xx),
+                                              L.dptr(vv), float(yp1), float(ypn), float(cutoff), float(buffer),
+                                              C.byref(mid)), "core_pot_create")
+        return mid.value
+
+    def pair_energy_and_forces(self, model_id: int):
+        """(energies (n,), forces (n, 3), virial (3, 3)) of a pair model on the resident list."""
+        n = self.n_sites
+        e, f, w = np.empty(n), np.empty((n, 3)), np.empty((3, 3))
+        L.check(self._lib.tgx_pair_energy_and_forces(self.h, int(model_id), L.dptr(e), L.dptr(f), L.dptr(w)),
+                "pair_energy_and_forces")
+        return e, f, w
+
     def soap_backend_begin(self, model_id: int):
         L.check(self._lib.tgx_soap_backend_begin(self.h, int(model_id)), "soap_backend_begin")
         self._current = model_id

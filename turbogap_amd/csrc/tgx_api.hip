@@ -12,14 +12,6 @@ int tgx_energy_scatter(tgx_ctx* ctx, tgx_model_t* m);
 int tgx_block_seen(tgx_ctx* ctx, tgx_model_t* m, int* d_seen);
 int tgx_radial_table_build(tgx_ctx* ctx, tgx_model_t* m);
 
-#define TGX_CTX_CHECK(ctx)                                      \
-  do {                                                          \
-    if (!(ctx)) {                                               \
-      tgx_set_error("%s: null context", __func__);              \
-      return TGX_ERR_ARG;                                       \
-    }                                                           \
-    TGX_CHECK_HIP(hipSetDevice((ctx)->device));                 \
-  } while (0)
 
 // ---- profiling ----------------------------------------------------------------
 int tgx_prof_begin(tgx_ctx* ctx, const char* name, int* slot) {
@@ -149,6 +141,7 @@ extern "C" int tgx_finalize(tgx_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   tgx_surrogate_release(ctx);
+  tgx_pair_models_release(ctx);
   for (auto* m : ctx->models) free_model(m);
   dbuf* bufs[] = {&ctx->pos, &ctx->species_sc, &ctx->n_neigh, &ctx->nl_off, &ctx->nlist, &ctx->rjs, &ctx->xyz,
                   &ctx->thetas, &ctx->phis, &ctx->nspec, &ctx->rev, &ctx->cell_of, &ctx->cell_cnt, &ctx->cell_start,

@@ -146,6 +146,19 @@ struct tgx_model_t {
 
 struct prof_rec { std::string name; hipEvent_t a, b; };
 
+// Entry-point prologue: a non-null context, its device current.
+#define TGX_CTX_CHECK(ctx)                                      \
+  do {                                                          \
+    if (!(ctx)) {                                               \
+      tgx_set_error("%s: null context", __func__);              \
+      return TGX_ERR_ARG;                                       \
+    }                                                           \
+    TGX_CHECK_HIP(hipSetDevice((ctx)->device));                 \
+  } while (0)
+
+struct tgx_pair_model;  // distance_2b / core_pot model (tgx_pair.hip)
+void tgx_pair_models_release(tgx_ctx* ctx);
+
 struct tgx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -165,6 +178,7 @@ struct tgx_ctx {
   // models
   std::vector<tgx_model_t*> models;
   int current = -1;
+  std::vector<tgx_pair_model*> pair_models;
   // results
   dbuf energies, forces, fglob, virial_part, scratch, red;
   // md

@@ -22,9 +22,12 @@ Species names map to the integer codes the caller uses for atoms through
 
 load_soap_blocks reads the sparse sets of the soap_turbo blocks
 (alphas_sparse, desc_sparse; sparse_io) into (hypers, Qs, alphas) triples for
-Context.soap_model. Blocks of other types (distance_2b, angle_3b, core_pot,
-...) are returned by read_gap_file but not evaluated by this library, and a
-compressed soap_turbo block (compress_soap = .true.) is rejected.
+Context.soap_model; a compressed soap_turbo block (compress_soap = .true.) is
+rejected. load_pair_blocks reads distance_2b blocks (alphas and cutoff per
+line, one distance per line; src/read_files.f90:722-742, 5238-5305) and
+core_pot blocks (a table headed by n, yp1, ypn; src/read_files.f90:5392-5427)
+into keyword arguments of Context.distance_2b_model / core_pot_model. angle_3b
+and the other block types are returned by read_gap_file but not evaluated.
 """
 from __future__ import annotations
 
@@ -188,4 +191,38 @@ def load_soap_blocks(path: str, species_map: Optional[Dict[str, int]] = None,
         if Qs.shape[1] != b.hypers.dim:
             raise ValueError(f"{b.file_desc}: descriptor size {Qs.shape[1]} does not match the hypers ({b.hypers.dim})")
         out.append((b.hypers, Qs, alphas))
+    return out
+
+
+def load_pair_blocks(path: str, species_map: Optional[Dict[str, int]] = None, base_dir: Optional[str] = None,
+                     core_pot_cutoff: float = 1.0e10, core_pot_buffer: float = 1.0) -> List[Tuple[str, dict]]:
+    """[(kind, kwargs)] of the distance_2b and core_pot blocks, in file order."""
+    base = os.getcwd() if base_dir is None else base_dir
+    out = []
+    for b in read_gap_file(path, species_map):
+        v = b.values
+        if b.kind == "distance_2b":
+            arec = [ln.split() for ln in open(os.path.join(base, v["alphas_sparse"][0])).read().splitlines()]
+            drec = [ln.split() for ln in open(os.path.join(base, v["desc_sparse"][0])).read().splitlines()]
+            if len(drec) != len(arec):
+                raise ValueError("distance_2b: descriptor and alphas files differ in length")
+            f = lambda t: float(t.replace("d", "e").replace("D", "e"))
+            out.append(("distance_2b", dict(
+                species1=_species_code(v["species1"][0], species_map),
+                species2=_species_code(v["species2"][0], species_map),
+                # the driver passes buffer 0.5 and e0 = 0 (src/gap_backend_cpu.f90:79-139); delta and sigma
+                # default to 1 (src/types.f90:137-151)
+                rcut=f(v["rcut"][0]), buffer=0.5, delta=f(v.get("delta", ["1.0"])[0]),
+                sigma=f(v.get("sigma", ["1.0"])[0]), e0=0.0,
+                Qs=np.array([f(r[0]) for r in drec]), alphas=np.array([f(r[0]) for r in arec]),
+                cutoff=np.array([f(r[1]) for r in arec]))))
+        elif b.kind == "core_pot":
+            lines = open(os.path.join(base, v["core_pot_file"][0])).read().split("\n")
+            head = lines[0].split()
+            n, yp1, ypn = int(head[0]), float(head[1]), float(head[2])
+            tab = np.array([[float(t) for t in ln.split()[:2]] for ln in lines[1:1 + n]])
+            out.append(("core_pot", dict(
+                species1=_species_code(v["species1"][0], species_map),
+                species2=_species_code(v["species2"][0], species_map),
+                x=tab[:, 0], V=tab[:, 1], yp1=yp1, ypn=ypn, cutoff=core_pot_cutoff, buffer=core_pot_buffer)))
     return out
