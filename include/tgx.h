@@ -20,6 +20,7 @@
  *   tgx_get_soap                        soap_turbo get_soap, call shape src/gap_interface.f90:320-325
  *   tgx_get_soap_energy_and_forces      src/gap.f90:66-259          get_soap_energy_and_forces
  *   tgx_compute                         src/turbogap.f90:1253-1380,1941-1954  the per-step SOAP loop and totals
+ *   tgx_local_property_predict          src/local_properties.f90:174-297 local_property_predict
  *   tgx_velocity_verlet etc.            src/md.f90:226-345          velocity_verlet / berendsen_thermostat / remove_cm_vel
  *   tgx_distance_2b_create              src/read_files.f90:722-742,5238-5305  a distance_2b block and its sparse set
  *   tgx_core_pot_create                 src/read_files.f90:5392-5510 a core_pot block, its table and spline
@@ -144,6 +145,13 @@ int tgx_get_gap_soap(tgx_ctx* ctx, int path, int do_forces, double* energies, do
  * come from a preceding tgx_get_gap_soap or tgx_soap_block_size call. */
 int tgx_soap_block_size(tgx_ctx* ctx, int* n_sites_block, int64_t* n_pairs_block);
 int tgx_get_soap(tgx_ctx* ctx, double* soap, double* soap_cart_der);
+/* local_property_predict (src/local_properties.f90:174-297) with the current
+ * block as the property model (its e0 is V0): V (n_sites_block) =
+ * max(0, delta^2 sum_s alpha_s K_is^zeta + V0) and, with do_derivatives,
+ * V_der (3, n_pairs_block) = dV_i/dq . soap_cart_der(:, :, p), zero where
+ * V_i = 0, on the FP64 plain path. Sizes as tgx_soap_block_size gives them. */
+int tgx_local_property_predict(tgx_ctx* ctx, int do_derivatives, double* V, double* V_der, int* n_sites_block,
+                               int64_t* n_pairs_block);
 /* get_soap_energy_and_forces for the current block, host inputs: soap(n_soap,
  * n_sites), soap_der(3, n_soap, n_pairs), neighbors_list 1-based local indices,
  * xyz(3, n_pairs); outputs energies(n_sites), forces(3, n_sites0), virial(3,3). */

@@ -219,6 +219,18 @@ class Context:
         L.check(self._lib.tgx_get_soap(self.h, L.dptr(soap), L.dptr(der)), "get_soap")
         return soap, der
 
+    def local_property_predict(self, derivatives: bool = True):
+        """local_property_predict (src/local_properties.f90:174-297) with the current
+        block as the property model: V (n_sites_block,) and V_der (n_pairs_block, 3)."""
+        ns = C.c_int()
+        npb = C.c_int64()
+        L.check(self._lib.tgx_soap_block_size(self.h, C.byref(ns), C.byref(npb)), "soap_block_size")
+        V = np.empty(ns.value)
+        Vd = np.empty((npb.value, 3)) if derivatives else None
+        L.check(self._lib.tgx_local_property_predict(self.h, int(derivatives), L.dptr(V), L.dptr(Vd), None, None),
+                "local_property_predict")
+        return V, Vd
+
     def get_soap_energy_and_forces(self, soap, soap_der, n_neigh, neighbors_list, xyz, n_sites0, e0=0.0,
                                    do_forces=True):
         """src/gap.f90:66-259 with host arrays: soap (n, D), soap_der (P, D, 3),

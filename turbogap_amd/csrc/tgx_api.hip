@@ -529,6 +529,28 @@ extern "C" int tgx_get_gap_soap(tgx_ctx* ctx, int path, int do_forces, double* e
   return TGX_OK;
 }
 
+extern "C" int tgx_local_property_predict(tgx_ctx* ctx, int do_derivatives, double* V, double* V_der,
+                                          int* n_sites_block, int64_t* n_pairs_block) {
+  TGX_CTX_CHECK(ctx);
+  tgx_model_t* m;
+  TGX_TRY(current_model(ctx, &m));
+  TGX_REQUIRE(ctx->have_list, "no neighbour list: call tgx_build_neighbors_list first");
+  TGX_TRY(tgx_block_select(ctx, m));
+  TGX_TRY(tgx_soap_descriptors(ctx, m, TGX_PATH_FP64_PLAIN, do_derivatives != 0));
+  TGX_TRY(tgx_gap_contract(ctx, m, TGX_PATH_FP64_PLAIN, do_derivatives != 0));
+  double *dV = nullptr, *dD = nullptr;
+  TGX_TRY(tgx_local_property_finish(ctx, m, do_derivatives != 0, &dV, &dD));
+  if (V && m->n_sites_b)
+    TGX_CHECK_HIP(hipMemcpyAsync(V, dV, sizeof(double) * m->n_sites_b, hipMemcpyDeviceToHost, ctx->stream));
+  if (V_der && do_derivatives && m->n_pairs_b)
+    TGX_CHECK_HIP(hipMemcpyAsync(V_der, dD, sizeof(double) * 3 * (size_t)m->n_pairs_b, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  if (n_sites_block) *n_sites_block = m->n_sites_b;
+  if (n_pairs_block) *n_pairs_block = m->n_pairs_b;
+  return TGX_OK;
+}
+
 extern "C" int tgx_soap_block_size(tgx_ctx* ctx, int* n_sites_block, int64_t* n_pairs_block) {
   TGX_CTX_CHECK(ctx);
   tgx_model_t* m;

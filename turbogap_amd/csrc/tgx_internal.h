@@ -227,6 +227,7 @@ void tgx_gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w);
 int tgx_neighbors_run(tgx_ctx* ctx, double rcut_max, const int* indices, const int* do_list);
 int tgx_neighbor_angles(tgx_ctx* ctx);
 int tgx_neighbors_refresh(tgx_ctx* ctx);
+int tgx_local_property_finish(tgx_ctx* ctx, tgx_model_t* m, bool with_der, double** d_V, double** d_vder);
 int tgx_block_select(tgx_ctx* ctx, tgx_model_t* m);
 int tgx_soap_descriptors(tgx_ctx* ctx, tgx_model_t* m, int path, bool with_der);
 int tgx_gap_contract(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces);

@@ -1171,6 +1171,42 @@ __global__ __launch_bounds__(256) void k_plain_force(int n_sites, const int64_t*
   }
 }
 
+// local_property_predict (src/local_properties.f90:174-297) from the block's
+// GAP contraction: V_s = max(0, E_s) (the model's e0 is V0), and per pair
+// V_der = dV_i/dq . dq/dr = -Qss_i . soap_cart_der(:, :, p) (Qss here is
+// -dE/dq), zero for a centre whose V is 0. One wavefront per pair.
+__global__ __launch_bounds__(256) void k_local_property(int n_sites, int D, int Dpad, const double* __restrict__ e_site,
+                                                         const double* __restrict__ qss, const double* __restrict__ der,
+                                                         int64_t n_pairs, const int* __restrict__ pair_site,
+                                                         double* __restrict__ V, double* __restrict__ vder) {
+  const int lane = threadIdx.x & 63;
+  const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (p < n_sites && lane == 0) V[p] = e_site[p] < 0.0 ? 0.0 : e_site[p];
+  if (p >= n_pairs || !vder) return;
+  const int i = pair_site[p];
+  double a0 = 0, a1 = 0, a2 = 0;
+  if (e_site[i] > 0.0) {
+    const double* s = qss + (size_t)i * Dpad;
+    const double* g = der + (size_t)p * 3 * D;
+    for (int d = lane; d < D; d += 64) {
+      const double sv = s[d];
+      a0 += sv * g[3 * d];
+      a1 += sv * g[3 * d + 1];
+      a2 += sv * g[3 * d + 2];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      a0 += __shfl_xor(a0, o);
+      a1 += __shfl_xor(a1, o);
+      a2 += __shfl_xor(a2, o);
+    }
+  }
+  if (lane == 0) {
+    vder[3 * p] = -a0;
+    vder[3 * p + 1] = -a1;
+    vder[3 * p + 2] = -a2;
+  }
+}
+
 __global__ void k_pair_site(int n_sites, const int64_t* __restrict__ site_beg, int* __restrict__ pair_site) {
   const int i = blockIdx.x;
   if (i >= n_sites) return;
@@ -1302,6 +1338,27 @@ int tgx_plain_pair_forces(tgx_ctx* ctx, tgx_model_t* m) {
   TGX_LAUNCH(ctx, "plain_force", k_plain_force, dim3((unsigned)((np + 3) / 4)), dim3(256), 0, ns,
              m->site_beg.as<int64_t>(), m->D, m->Dpad, m->qss64.as<double>(), m->soap_der.as<double>(),
              m->pr_gidx.as<int64_t>(), np, pair_site, ctx->fglob.as<double>());
+  return TGX_OK;
+}
+
+// V (n_sites_b) and, with derivatives, V_der (3, n_pairs_b) on the device
+// (ctx->scratch) after the FP64 plain descriptors and contraction of the block.
+int tgx_local_property_finish(tgx_ctx* ctx, tgx_model_t* m, bool with_der, double** d_V, double** d_vder) {
+  const int ns = m->n_sites_b;
+  const int64_t np = with_der ? m->n_pairs_b : 0;
+  TGX_TRY(ctx->scratch.ensure(sizeof(double) * ((size_t)ns + 3 * (size_t)np) + sizeof(int) * (size_t)np + 64));
+  double* V = ctx->scratch.as<double>();
+  double* vder = V + ns;
+  int* pair_site = reinterpret_cast<int*>(vder + 3 * np);
+  if (np > 0)
+    TGX_LAUNCH(ctx, "pair_site", k_pair_site, dim3(ns), dim3(64), 0, ns, m->site_beg.as<int64_t>(), pair_site);
+  const int64_t rows = np > ns ? np : ns;
+  if (rows > 0)
+    TGX_LAUNCH(ctx, "local_property", k_local_property, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, ns, m->D,
+               m->Dpad, m->e_site.as<double>(), np > 0 ? m->qss64.as<double>() : nullptr,
+               np > 0 ? m->soap_der.as<double>() : nullptr, np, pair_site, V, np > 0 ? vder : nullptr);
+  *d_V = V;
+  *d_vder = vder;
   return TGX_OK;
 }
 

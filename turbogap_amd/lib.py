@@ -83,6 +83,7 @@ _SIGS = {
     "tgx_soap_model_dim": [_P, _I, _PI],
     "tgx_get_gap_soap": [_P, _I, _I, _PD, _PD, _PD, _PI, C.POINTER(_I64)],
     "tgx_soap_block_size": [_P, _PI, C.POINTER(_I64)],
+    "tgx_local_property_predict": [_P, _I, _PD, _PD, _PI, C.POINTER(_I64)],
     "tgx_get_soap": [_P, _PD, _PD],
     "tgx_get_soap_energy_and_forces": [_P, _I, _I, _I64, _PD, _PD, _D, _PI, _PI, _PD, _I, _I, _PD, _PD, _PD],
     "tgx_compute": [_P, _I, _PD, _PD],
