@@ -24,7 +24,9 @@
  *   tgx_velocity_verlet etc.            src/md.f90:226-345          velocity_verlet / berendsen_thermostat / remove_cm_vel
  *   tgx_distance_2b_create              src/read_files.f90:722-742,5238-5305  a distance_2b block and its sparse set
  *   tgx_core_pot_create                 src/read_files.f90:5392-5510 a core_pot block, its table and spline
- *   tgx_pair_energy_and_forces          src/gap.f90:400-563 / 565-721 get_2b_ / get_core_pot_energy_and_forces
+ *   tgx_angle_3b_create                 src/read_files.f90:743-770,5305-5384  an angle_3b block and its sparse set
+ *   tgx_pair_energy_and_forces          src/gap.f90:400-563 / 565-721 / 724-1112 get_2b_ / get_core_pot_ /
+ *                                       get_3b_energy_and_forces
  */
 #ifndef TGX_H
 #define TGX_H
@@ -215,7 +217,7 @@ int tgx_surrogate_gate(tgx_ctx* ctx, int n_members, double threshold);
 /* Of the last surrogate compute: fraction of centres gated, largest spread (eV). */
 int tgx_surrogate_stats(tgx_ctx* ctx, double* gated_fraction, double* max_spread);
 
-/* ---- pair-potential families on the resident list ------------------------ */
+/* ---- pair and triplet GAP families on the resident list ------------------ */
 /* distance_2b: centres of species1 or species2 with a neighbour of the other
  * (species codes as in tgx_set_system), r < rcut, cosine cutoff over the last
  * `buffer` Angstrom; E_i = e0 + delta^2 sum_s alpha_s cutoff_s fcut
@@ -230,9 +232,21 @@ int tgx_distance_2b_create(tgx_ctx* ctx, int species1, int species2, double rcut
  * over the centre's pairs with r < max(x). */
 int tgx_core_pot_create(tgx_ctx* ctx, int species1, int species2, int n, const double* x, const double* V, double yp1,
                         double ypn, double cutoff, double buffer, int* model_id);
-/* Energies (n_sites), forces (3, n_sites) on the centres and virial (3, 3) of
- * one pair model on the resident list, as the reference's subroutines return
- * them (any output may be NULL). */
+/* angle_3b: centres of species_center; every unordered pair of neighbours with
+ * species (species1, species2) in either order and both distances < rcut (cosine
+ * cutoff over the last `buffer` Angstrom on each); the triplet descriptor
+ * q = (r_ij + r_ik, (r_ij - r_ik)^2, r_jk) against the sparse points Qs (row-major
+ * n_sparse x 3) scaled by sigma[3]; kernel_type "exp" (exp(-r^2/2)) or "pol"
+ * (cov_pp, d = 3, q = 1); E_i = e0 + fc_ij fc_ik sum_s 2 delta^2 alpha_s cutoff_s
+ * k(r_s). Forces act on the centre and on both neighbours (their home atoms).
+ * A centre may have at most 1024 such neighbours. */
+int tgx_angle_3b_create(tgx_ctx* ctx, int species_center, int species1, int species2, double rcut, double buffer,
+                        double delta, const double* sigma, double e0, const char* kernel_type, int n_sparse,
+                        const double* Qs, const double* alphas, const double* cutoff, int* model_id);
+/* Energies (n_sites), forces (3, n_sites) and virial (3, 3) of one pair or
+ * triplet model on the resident list, as the reference's subroutines return
+ * them (any output may be NULL). distance_2b and core_pot forces act on the
+ * centres only, as in the reference; angle_3b needs every atom's own list. */
 int tgx_pair_energy_and_forces(tgx_ctx* ctx, int model_id, double* energies, double* forces, double* virial);
 
 /* ---- MD (src/md.f90:226-345, src/turbogap_md.f90:160-470) --------------- */

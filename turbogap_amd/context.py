@@ -177,8 +177,24 @@ class Context:
                                               C.byref(mid)), "core_pot_create")
         return mid.value
 
+    def angle_3b_model(self, species_center: int, species1: int, species2: int, rcut: float, buffer: float,
+                       delta: float, sigma, Qs, alphas, cutoff, e0: float = 0.0, kernel_type: str = "exp") -> int:
+        """An angle_3b block (src/gap.f90:724-1112): Qs (n_sparse, 3) triplet
+        descriptors, alphas and cutoff (n_sparse,), sigma (3,), kernel_type
+        "exp" or "pol", species as the system's codes."""
+        q, a, c = _f64(Qs).reshape(-1, 3), _f64(alphas).reshape(-1), _f64(cutoff).reshape(-1)
+        sg = _f64(sigma).reshape(-1)
+        if not (q.shape[0] == a.shape[0] == c.shape[0]) or sg.shape != (3,):
+            raise ValueError("Qs (n, 3), alphas and cutoff (n,) per sparse point and three sigmas")
+        mid = C.c_int()
+        L.check(self._lib.tgx_angle_3b_create(self.h, int(species_center), int(species1), int(species2), float(rcut),
+                                              float(buffer), float(delta), L.dptr(sg), float(e0),
+                                              str(kernel_type).encode(), q.shape[0], L.dptr(q), L.dptr(a), L.dptr(c),
+                                              C.byref(mid)), "angle_3b_create")
+        return mid.value
+
     def pair_energy_and_forces(self, model_id: int):
-        """(energies (n,), forces (n, 3), virial (3, 3)) of a pair model on the resident list."""
+        """(energies (n,), forces (n, 3), virial (3, 3)) of a pair or triplet model on the resident list."""
         n = self.n_sites
         e, f, w = np.empty(n), np.empty((n, 3)), np.empty((3, 3))
         L.check(self._lib.tgx_pair_energy_and_forces(self.h, int(model_id), L.dptr(e), L.dptr(f), L.dptr(w)),

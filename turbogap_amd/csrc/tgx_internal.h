@@ -234,6 +234,9 @@ int tgx_gap_contract(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces);
 int tgx_plain_pair_forces(tgx_ctx* ctx, tgx_model_t* m);
 int tgx_adjoint_pair_forces(tgx_ctx* ctx, tgx_model_t* m, int path);
 int tgx_gather_forces(tgx_ctx* ctx, double* energy_out, double* virial_out);
+// F (device, n x 3) and the pair virial (host, 9, optional) from per-entry pair
+// forces fglob (device, n_pairs x 3) on the resident list.
+int tgx_gather_pair_forces(tgx_ctx* ctx, const double* fglob, double* F, double* virial_out);
 int tgx_zero(tgx_ctx* ctx, void* p, size_t bytes);
 int tgx_surrogate_energy_grad(tgx_ctx* ctx, tgx_model_t* m);
 int tgx_device_sum(tgx_ctx* ctx, const double* d_x, int64_t n, double* out_host);

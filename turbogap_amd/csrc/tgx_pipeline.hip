@@ -196,16 +196,22 @@ int tgx_block_select(tgx_ctx* ctx, tgx_model_t* m) {
 // Forces and virial from the accumulated pair forces; adds every model's site
 // energies was done by the caller. energy_out/virial_out may be NULL.
 int tgx_gather_forces(tgx_ctx* ctx, double* energy_out, double* virial_out) {
+  TGX_TRY(ctx->forces.ensure(sizeof(double) * 3 * (size_t)ctx->n_sites));
+  TGX_TRY(tgx_gather_pair_forces(ctx, ctx->fglob.as<double>(), ctx->forces.as<double>(), virial_out));
+  if (energy_out) TGX_TRY(tgx_device_sum(ctx, ctx->energies.as<double>(), ctx->n_sites, energy_out));
+  return TGX_OK;
+}
+
+int tgx_gather_pair_forces(tgx_ctx* ctx, const double* fglob, double* F, double* virial_out) {
   const int n = ctx->n_sites;
   const int64_t P = ctx->n_pairs;
-  TGX_TRY(ctx->forces.ensure(sizeof(double) * 3 * (size_t)n));
   TGX_TRY(ctx->red.ensure(sizeof(double) * 9 * 512 + 64));
   int* bad = reinterpret_cast<int*>(ctx->red.as<double>() + 9 * 512);
   TGX_TRY(tgx_zero(ctx, bad, sizeof(int)));
   TGX_LAUNCH(ctx, "force_gather", k_gather, dim3((n + 255) / 256), dim3(256), 0, n, ctx->n_neigh.as<int>(),
-             ctx->nl_off.as<int64_t>(), ctx->rev.as<int64_t>(), ctx->fglob.as<double>(), ctx->forces.as<double>(), bad);
+             ctx->nl_off.as<int64_t>(), ctx->rev.as<int64_t>(), fglob, F, bad);
   const int nbk = 512;
-  TGX_LAUNCH(ctx, "virial", k_virial, dim3(nbk), dim3(256), 0, P, ctx->fglob.as<double>(), ctx->xyz.as<double>(),
+  TGX_LAUNCH(ctx, "virial", k_virial, dim3(nbk), dim3(256), 0, P, fglob, ctx->xyz.as<double>(),
              ctx->red.as<double>());
   std::vector<double> part(9 * nbk);
   int badh = 0;
@@ -221,7 +227,6 @@ int tgx_gather_forces(tgx_ctx* ctx, double* energy_out, double* virial_out) {
     for (int b = 0; b < nbk; ++b)
       for (int k = 0; k < 9; ++k) virial_out[k] += part[9 * b + k];
   }
-  if (energy_out) TGX_TRY(tgx_device_sum(ctx, ctx->energies.as<double>(), n, energy_out));
   return TGX_OK;
 }
 

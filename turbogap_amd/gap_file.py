@@ -26,8 +26,12 @@ Context.soap_model; a compressed soap_turbo block (compress_soap = .true.) is
 rejected. load_pair_blocks reads distance_2b blocks (alphas and cutoff per
 line, one distance per line; src/read_files.f90:722-742, 5238-5305) and
 core_pot blocks (a table headed by n, yp1, ypn; src/read_files.f90:5392-5427)
-into keyword arguments of Context.distance_2b_model / core_pot_model. angle_3b
-and the other block types are returned by read_gap_file but not evaluated.
+and angle_3b blocks (alphas and cutoff per line, three descriptor components
+per sparse point, one per line; src/read_files.f90:743-770, 5305-5384;
+defaults delta 1, sigma 1 1 1, kernel_type "exp", src/types.f90:154-170)
+into keyword arguments of Context.distance_2b_model / core_pot_model /
+angle_3b_model. The other block types are returned by read_gap_file but not
+evaluated.
 """
 from __future__ import annotations
 
@@ -196,7 +200,7 @@ def load_soap_blocks(path: str, species_map: Optional[Dict[str, int]] = None,
 
 def load_pair_blocks(path: str, species_map: Optional[Dict[str, int]] = None, base_dir: Optional[str] = None,
                      core_pot_cutoff: float = 1.0e10, core_pot_buffer: float = 1.0) -> List[Tuple[str, dict]]:
-    """[(kind, kwargs)] of the distance_2b and core_pot blocks, in file order."""
+    """[(kind, kwargs)] of the distance_2b, angle_3b and core_pot blocks, in file order."""
     base = os.getcwd() if base_dir is None else base_dir
     out = []
     for b in read_gap_file(path, species_map):
@@ -215,6 +219,30 @@ def load_pair_blocks(path: str, species_map: Optional[Dict[str, int]] = None, ba
                 rcut=f(v["rcut"][0]), buffer=0.5, delta=f(v.get("delta", ["1.0"])[0]),
                 sigma=f(v.get("sigma", ["1.0"])[0]), e0=0.0,
                 Qs=np.array([f(r[0]) for r in drec]), alphas=np.array([f(r[0]) for r in arec]),
+                cutoff=np.array([f(r[1]) for r in arec]))))
+        elif b.kind == "angle_3b":
+            f = lambda t: float(t.replace("d", "e").replace("D", "e"))
+            arec = [ln.split() for ln in open(os.path.join(base, v["alphas_sparse"][0])).read().splitlines() if ln.strip()]
+            desc = [f(ln.split()[0]) for ln in open(os.path.join(base, v["desc_sparse"][0])).read().splitlines()
+                    if ln.strip()]
+            if len(desc) != 3 * len(arec):
+                raise ValueError("angle_3b: the descriptor file must hold three values per sparse point")
+            sigma = [f(t) for t in v["sigma"][:3]] if "sigma" in v else [1.0, 1.0, 1.0]
+            if len(sigma) != 3:
+                raise ValueError("angle_3b: sigma needs three values")
+
+            def code(*keys):
+                for key in keys:
+                    if key in v:
+                        return _species_code(v[key][0], species_map)
+                raise ValueError(f"angle_3b block without {keys[-1]}")
+            out.append(("angle_3b", dict(
+                species_center=code("Z_center", "z_center", "species_center"),
+                species1=code("Z1", "z1", "species1"), species2=code("Z2", "z2", "species2"),
+                # the driver passes buffer 0.5 and e0 = 0 (src/gap_backend_cpu.f90:250-258)
+                rcut=f(v["rcut"][0]), buffer=0.5, delta=f(v.get("delta", ["1.0"])[0]), sigma=sigma, e0=0.0,
+                kernel_type=v.get("kernel_type", ["exp"])[0],
+                Qs=np.array(desc).reshape(-1, 3), alphas=np.array([f(r[0]) for r in arec]),
                 cutoff=np.array([f(r[1]) for r in arec]))))
         elif b.kind == "core_pot":
             lines = open(os.path.join(base, v["core_pot_file"][0])).read().split("\n")

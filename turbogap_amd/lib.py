@@ -114,6 +114,7 @@ _SIGS = {
     "tgx_md_displacement": [_P, _D, _I, _PI],
     "tgx_distance_2b_create": [_P, _I, _I, _D, _D, _D, _D, _D, _I, _PD, _PD, _PD, _PI],
     "tgx_core_pot_create": [_P, _I, _I, _I, _PD, _PD, _D, _D, _D, _D, _PI],
+    "tgx_angle_3b_create": [_P, _I, _I, _I, _D, _D, _D, _PD, _D, C.c_char_p, _I, _PD, _PD, _PD, _PI],
     "tgx_pair_energy_and_forces": [_P, _I, _PD, _PD, _PD],
     "tgx_set_profiling": [_P, _I],
     "tgx_kernel_time": [_P, C.c_char_p, _PI, _PD],
