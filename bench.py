@@ -271,6 +271,121 @@ def cpu_baseline(models, n_sample, threads):
                       f"(same hypers and 2 x 4000 sparse points), FP64 oracle, OpenMP {threads} threads, {dt:.2f} s"}
 
 
+def silicon_crystal(cells, perturb=0.05, seed=4):
+    """Diamond silicon, cells^3 conventional cells (a = 5.43 A), seeded Gaussian displacements."""
+    from turbogap_amd import structures as S
+    a = 5.43
+    base = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0],
+                     [.25, .25, .25], [.25, .75, .75], [.75, .25, .75], [.75, .75, .25]])
+    g = np.arange(cells)
+    grid = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+    pos = ((grid[:, None, :] + base[None, :, :]).reshape(-1, 3)) * a
+    pos = pos + np.random.default_rng(seed).normal(scale=perturb, size=pos.shape)
+    return S.System(pos, np.full(pos.shape[0], 14, np.int32), S.cubic_box(cells * a))
+
+
+FAMILY_KERNELS = {"distance_2b": ["pair_model"], "core_pot": ["pair_model"],
+                  "angle_3b": ["angle_3b", "angle_3b_large", "force_gather", "virial"]}
+
+
+def family_bench(args):
+    """One evaluation of a distance_2b / angle_3b / core_pot block of the reference
+    suite's silicon potential (tests/golden/silicon.gap) on a resident list: the
+    kernels, the force gather and the copies of energies, forces and virial."""
+    from turbogap_amd import Context
+    from turbogap_amd import gap_file as G
+    gold = os.path.join(ROOT, "tests", "golden")
+    kw = dict(G.load_pair_blocks(os.path.join(gold, "silicon.gap"), base_dir=gold))[args.family]
+    sysm = silicon_crystal(args.family_cells)
+    rc = float(kw["x"].max()) if args.family == "core_pot" else float(kw["rcut"])
+    log(f"{args.family}: {sysm.n} silicon atoms, rcut {rc} A")
+    with Context(0) as ctx:
+        dev = ctx.device_name()
+        ctx.set_system(sysm.positions, sysm.species, sysm.box)
+        nl = ctx.build_neighbors_list(rc, fetch=True)
+        make = {"distance_2b": ctx.distance_2b_model, "core_pot": ctx.core_pot_model,
+                "angle_3b": ctx.angle_3b_model}[args.family]
+        mid = make(**kw)
+        for _ in range(args.warmup):
+            ctx.pair_energy_and_forces(mid)
+        ctx.synchronize()
+        ctx.reset_profile()
+        ctx.set_profiling(True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            e, f, w = ctx.pair_energy_and_forces(mid)
+        ctx.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        ctx.set_profiling(False)
+        kt = kernel_table(ctx, FAMILY_KERNELS[args.family])
+    n_pairs = int(nl["n_atom_pairs"])
+    roof = None
+    if args.family == "angle_3b":
+        # triplets: unordered pairs of selected neighbours per centre (one species here)
+        nn = np.asarray(nl["n_neigh"])
+        off = np.concatenate([[0], np.cumsum(nn)])
+        inside = np.asarray(nl["rjs"]) < kw["rcut"]
+        m = np.add.reduceat(inside.astype(np.int64), off[:-1]) - 1  # minus the self entry (r = 0)
+        triplets = int(np.sum(m * (m - 1) // 2))
+        flops = 30.0 * triplets * kw["Qs"].shape[0]
+        k = kt.get("angle_3b")
+        if k:
+            ach = flops / (k["avg_ms"] * 1e-3) / 1e12
+            roof = {"kernel": "angle_3b", "bound": "fp64 valu", "achieved": ach, "peak": PEAK["fp64_TF"],
+                    "unit": "TFLOP/s", "frac": ach / PEAK["fp64_TF"], "traffic": None,
+                    "work": f"{triplets} triplets x {kw['Qs'].shape[0]} sparse points x 30 FP64 flops"}
+    elif args.family == "distance_2b":
+        # every list entry is a selected pair here (one species); 20 FP64 operations per (pair, sparse point)
+        k = kt.get("pair_model")
+        inside = int(np.sum(np.asarray(nl["rjs"]) < kw["rcut"])) - sysm.n
+        if k:
+            ach = 20.0 * inside * kw["Qs"].shape[0] / (k["avg_ms"] * 1e-3) / 1e12
+            roof = {"kernel": "pair_model", "bound": "fp64 valu", "achieved": ach, "peak": PEAK["fp64_TF"],
+                    "unit": "TFLOP/s", "frac": ach / PEAK["fp64_TF"], "traffic": None,
+                    "work": f"{inside} pairs x {kw['Qs'].shape[0]} sparse points x 20 FP64 flops (one exp each)"}
+    else:
+        k = kt.get("pair_model")
+        if k:
+            byts = 44.0 * n_pairs + 80.0 * sysm.n
+            ach = byts / (k["avg_ms"] * 1e-3) / 1e9
+            roof = {"kernel": "pair_model", "bound": "hbm", "achieved": ach, "peak": PEAK["hbm_GBs"], "unit": "GB/s",
+                    "frac": ach / PEAK["hbm_GBs"], "traffic": None,
+                    "work": f"{n_pairs} pairs x 44 B read + {sysm.n} centres x 80 B written"}
+    cpu = None
+    if not args.no_cpu:
+        from oracle import angle_3b as A3
+        from oracle import oracle as O
+        from oracle import pair_oracle as PO
+        small = silicon_crystal(2)
+        t1 = time.perf_counter()
+        snl = O.build_neighbors_list(small.positions, small.species, small.box, rc)
+        if args.family == "angle_3b":
+            A3.angle_3b_energy_and_forces(snl, small.species, small.n, kw["species_center"], kw["species1"],
+                                          kw["species2"], kw["rcut"], kw["buffer"], kw["delta"], kw["sigma"],
+                                          kw["e0"], kw["Qs"], kw["alphas"], kw["cutoff"], kw["kernel_type"])
+        elif args.family == "distance_2b":
+            PO.distance_2b_energy_and_forces(snl, small.species, small.n, kw["species1"], kw["species2"], kw["rcut"],
+                                             kw["buffer"], kw["delta"], kw["sigma"], kw["e0"], kw["Qs"],
+                                             kw["alphas"], kw["cutoff"])
+        else:
+            model = PO.core_pot_model(kw["x"], kw["V"], kw["yp1"], kw["ypn"], kw["cutoff"], kw["buffer"])
+            PO.core_pot_energy_and_forces(snl, small.species, small.n, kw["species1"], kw["species2"], model)
+        ct = time.perf_counter() - t1
+        cpu = {"value": small.n / ct, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+               "sample": f"one evaluation on a {small.n}-atom silicon cell, numpy oracle (loops in the "
+                         f"reference's order), {ct:.2f} s"}
+    out = {"metric": f"{args.family} atom-steps/s", "value": sysm.n / dt, "unit": "atom-steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (diamond silicon with seeded 0.05 A displacements); the reference suite's silicon "
+                   "potential blocks",
+           "config": {"workload": f"{sysm.n}-atom silicon, {args.family} block of silicon.gap, E+F+virial on a "
+                                  f"resident list (rcut {rc} A, {n_pairs} list entries)", "device": dev},
+           "kernels": kt, "roofline": roof, "cpu_baseline": cpu,
+           "energy": float(np.sum(e)), "virial_trace": float(np.trace(w))}
+    print(json.dumps(out, default=float))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -291,10 +406,16 @@ def main():
     ap.add_argument("--neighbors-buffer", type=float, default=0.5,
                     help="neighbour-list buffer (A) of the production MD; 0 rebuilds every step")
     ap.add_argument("--hbm-bytes", type=int, default=1 << 30, help="bytes of the streaming-copy reference (0: skip)")
+    ap.add_argument("--family", choices=["distance_2b", "angle_3b", "core_pot"], default=None,
+                    help="time one pair / triplet GAP family instead of the SOAP step")
+    ap.add_argument("--family-cells", type=int, default=20, help="silicon cells per edge of the --family workload")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.family:
+        family_bench(args)
+        return
 
     from turbogap_amd import Context
     from turbogap_amd import structures as S
