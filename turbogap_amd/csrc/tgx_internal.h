@@ -49,13 +49,24 @@ static inline int64_t tgx_round_up(int64_t x, int64_t g) { return (x + g - 1) / 
 struct dbuf {
   void* p = nullptr;
   size_t bytes = 0;
+  // A buffer that has outgrown its allocation once (list and block sizes
+  // drift during MD) is regrown with 25% headroom, so that a slowly rising
+  // size does not free and reallocate it, synchronising the device, on every
+  // new maximum; exact size if the headroom does not fit.
   int ensure(size_t n) {
     if (n <= bytes && p) return TGX_OK;
+    const bool regrow = p != nullptr;
     if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
     size_t want = n < 256 ? 256 : n;
-    hipError_t e = hipMalloc(&p, want);
+    hipError_t e = hipErrorOutOfMemory;
+    if (regrow) {
+      e = hipMalloc(&p, want + want / 4);
+      if (e == hipSuccess) want += want / 4;
+      else (void)hipGetLastError();
+    }
+    if (e != hipSuccess) e = hipMalloc(&p, want);
     if (e != hipSuccess) {
       tgx_set_error("hipMalloc(%zu bytes) failed: %s", want, hipGetErrorString(e));
       p = nullptr;
