@@ -152,6 +152,7 @@ extern "C" int tgx_finalize(tgx_ctx* ctx) {
   if (ctx->gate_host) (void)hipHostFree(ctx->gate_host);
   if (ctx->md_flag_host) (void)hipHostFree(ctx->md_flag_host);
   if (ctx->pin64) (void)hipHostFree(ctx->pin64);
+  if (ctx->red_host) (void)hipHostFree(ctx->red_host);
   for (auto& r : ctx->prof) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);

@@ -167,7 +167,7 @@ struct prof_rec { std::string name; hipEvent_t a, b; };
     TGX_CHECK_HIP(hipSetDevice((ctx)->device));                 \
   } while (0)
 
-struct tgx_pair_model;  // distance_2b / core_pot model (tgx_pair.hip)
+struct tgx_pair_model;  // distance_2b / angle_3b / core_pot model (tgx_pair.hip)
 void tgx_pair_models_release(tgx_ctx* ctx);
 
 struct tgx_ctx {
@@ -204,6 +204,7 @@ struct tgx_ctx {
   int64_t gate_sites = 0, gate_gated = 0;
   int* gate_host = nullptr;  // pinned: {count, max spread bits} of every block
   int64_t* pin64 = nullptr;  // pinned: two sizes read back together
+  double* red_host = nullptr;  // pinned: virial and energy partials of the force gather
   int gate_host_n = 0;
   dbuf e_members, gate_mask;
   // profiling
@@ -247,7 +248,8 @@ int tgx_adjoint_pair_forces(tgx_ctx* ctx, tgx_model_t* m, int path);
 int tgx_gather_forces(tgx_ctx* ctx, double* energy_out, double* virial_out);
 // F (device, n x 3) and the pair virial (host, 9, optional) from per-entry pair
 // forces fglob (device, n_pairs x 3) on the resident list.
-int tgx_gather_pair_forces(tgx_ctx* ctx, const double* fglob, double* F, double* virial_out);
+int tgx_gather_pair_forces(tgx_ctx* ctx, const double* fglob, double* F, double* virial_out,
+                           const double* d_energies = nullptr, double* energy_out = nullptr);
 int tgx_zero(tgx_ctx* ctx, void* p, size_t bytes);
 int tgx_surrogate_energy_grad(tgx_ctx* ctx, tgx_model_t* m);
 int tgx_device_sum(tgx_ctx* ctx, const double* d_x, int64_t n, double* out_host);

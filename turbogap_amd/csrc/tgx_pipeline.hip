@@ -161,13 +161,12 @@ int tgx_block_select(tgx_ctx* ctx, tgx_model_t* m) {
   int* flag = ctx->scratch.as<int>();
   int* npairs = flag + n;
   int64_t* site_idx = reinterpret_cast<int64_t*>(ctx->scratch.as<char>() + tgx_round_up(sizeof(int) * 2 * (size_t)n, 16));
-  int64_t* pair_off = site_idx + (n + 1);
+  int64_t* pair_off = site_idx + tgx_round_up(n + 1, 2);  // 16-byte aligned for the scan
   SpeciesMap sm = species_map(m);
   TGX_LAUNCH(ctx, "select_count", k_select_count, dim3((n + 255) / 256), dim3(256), 0, n, sm, m->central_species,
              m->rcut_max, ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(), ctx->species_sc.as<int>(),
              ctx->rjs.as<double>(), ctx->nspec.as<int>(), flag, npairs);
-  TGX_TRY(tgx_scan_i32(ctx, flag, site_idx, n, nullptr));
-  TGX_TRY(tgx_scan_i32(ctx, npairs, pair_off, n, nullptr));
+  TGX_TRY(tgx_scan2_i32(ctx, flag, site_idx, npairs, pair_off, n));
   // both totals in one readback
   if (!ctx->pin64) TGX_CHECK_HIP(hipHostMalloc((void**)&ctx->pin64, 2 * sizeof(int64_t), hipHostMallocDefault));
   TGX_CHECK_HIP(hipMemcpyAsync(ctx->pin64, site_idx + n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -197,27 +196,35 @@ int tgx_block_select(tgx_ctx* ctx, tgx_model_t* m) {
 // energies was done by the caller. energy_out/virial_out may be NULL.
 int tgx_gather_forces(tgx_ctx* ctx, double* energy_out, double* virial_out) {
   TGX_TRY(ctx->forces.ensure(sizeof(double) * 3 * (size_t)ctx->n_sites));
-  TGX_TRY(tgx_gather_pair_forces(ctx, ctx->fglob.as<double>(), ctx->forces.as<double>(), virial_out));
-  if (energy_out) TGX_TRY(tgx_device_sum(ctx, ctx->energies.as<double>(), ctx->n_sites, energy_out));
-  return TGX_OK;
+  return tgx_gather_pair_forces(ctx, ctx->fglob.as<double>(), ctx->forces.as<double>(), virial_out,
+                                ctx->energies.as<double>(), energy_out);
 }
 
-int tgx_gather_pair_forces(tgx_ctx* ctx, const double* fglob, double* F, double* virial_out) {
+// Layout of ctx->red (and its pinned host copy): 512 blocks x 9 virial
+// partials, TGX_SUM_PARTS energy partials, the missing-reverse flag; one
+// readback and one synchronisation for all of them.
+int tgx_gather_pair_forces(tgx_ctx* ctx, const double* fglob, double* F, double* virial_out,
+                           const double* d_energies, double* energy_out) {
   const int n = ctx->n_sites;
   const int64_t P = ctx->n_pairs;
-  TGX_TRY(ctx->red.ensure(sizeof(double) * 9 * 512 + 64));
-  int* bad = reinterpret_cast<int*>(ctx->red.as<double>() + 9 * 512);
-  TGX_TRY(tgx_zero(ctx, bad, sizeof(int)));
+  const int nbk = 512;
+  const size_t nred = 9 * (size_t)nbk + TGX_SUM_PARTS + 1;
+  TGX_TRY(ctx->red.ensure(sizeof(double) * nred));
+  if (!ctx->red_host) TGX_CHECK_HIP(hipHostMalloc((void**)&ctx->red_host, sizeof(double) * nred, hipHostMallocDefault));
+  double* vpart = ctx->red.as<double>();
+  double* epart = vpart + 9 * nbk;
+  int* bad = reinterpret_cast<int*>(epart + TGX_SUM_PARTS);
+  TGX_TRY(tgx_zero(ctx, bad, sizeof(double)));
   TGX_LAUNCH(ctx, "force_gather", k_gather, dim3((n + 255) / 256), dim3(256), 0, n, ctx->n_neigh.as<int>(),
              ctx->nl_off.as<int64_t>(), ctx->rev.as<int64_t>(), fglob, F, bad);
-  const int nbk = 512;
-  TGX_LAUNCH(ctx, "virial", k_virial, dim3(nbk), dim3(256), 0, P, fglob, ctx->xyz.as<double>(),
-             ctx->red.as<double>());
-  std::vector<double> part(9 * nbk);
-  int badh = 0;
-  TGX_CHECK_HIP(hipMemcpyAsync(part.data(), ctx->red.p, sizeof(double) * 9 * nbk, hipMemcpyDeviceToHost, ctx->stream));
-  TGX_CHECK_HIP(hipMemcpyAsync(&badh, bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  TGX_LAUNCH(ctx, "virial", k_virial, dim3(nbk), dim3(256), 0, P, fglob, ctx->xyz.as<double>(), vpart);
+  const bool with_e = energy_out && d_energies;
+  if (with_e) TGX_TRY(tgx_device_sum_launch(ctx, d_energies, n, epart));
+  TGX_CHECK_HIP(hipMemcpyAsync(ctx->red_host, vpart, sizeof(double) * nred, hipMemcpyDeviceToHost, ctx->stream));
   TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  const double* hv = ctx->red_host;
+  int badh = 0;
+  std::memcpy(&badh, hv + 9 * nbk + TGX_SUM_PARTS, sizeof(int));
   if (badh) {
     tgx_set_error("force gather: a pair has no reverse entry (an atom without its own list); build with do_list = all");
     return TGX_ERR_UNSUPPORTED;
@@ -225,7 +232,12 @@ int tgx_gather_pair_forces(tgx_ctx* ctx, const double* fglob, double* F, double*
   if (virial_out) {
     for (int k = 0; k < 9; ++k) virial_out[k] = 0.0;
     for (int b = 0; b < nbk; ++b)
-      for (int k = 0; k < 9; ++k) virial_out[k] += part[9 * b + k];
+      for (int k = 0; k < 9; ++k) virial_out[k] += hv[9 * b + k];
+  }
+  if (with_e) {
+    double e = 0.0;
+    for (int i = 0; i < TGX_SUM_PARTS; ++i) e += hv[9 * nbk + i];
+    *energy_out = e;
   }
   return TGX_OK;
 }

@@ -6,6 +6,13 @@
 // entries (out[n] is the total). When total is not NULL the total is copied to
 // the host (this synchronises the stream).
 int tgx_scan_i32(tgx_ctx* ctx, const int* in, int64_t* out, int64_t n, int64_t* total);
+// The same for two arrays of n values (one launch when n <= 65,536).
+int tgx_scan2_i32(tgx_ctx* ctx, const int* in0, int64_t* out0, const int* in1, int64_t* out1, int64_t n);
+
+// Partial sums of n doubles into TGX_SUM_PARTS device values, each over a fixed
+// contiguous slice in a fixed tree (summed on the host in order).
+constexpr int TGX_SUM_PARTS = 256;
+int tgx_device_sum_launch(tgx_ctx* ctx, const double* d_x, int64_t n, double* d_part);
 
 // Streaming device-to-device copy of `bytes` (a multiple of 16) by k_hbm_copy.
 int tgx_hbm_copy(tgx_ctx* ctx, const void* src, void* dst, size_t bytes);

@@ -94,6 +94,83 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_single(const int* __restric
   if (threadIdx.x == 0) out[n] = tot;
 }
 
+// Exclusive scan of one value per thread over a 1024-thread workgroup: wave
+// scans by shuffles, the 16 wave totals scanned by the first wave, two barriers.
+__device__ int64_t block_exclusive_scan_waves(int64_t v, int64_t* sh, int64_t* total) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  constexpr int NW = SCAN_BLOCK / 64;
+  int64_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int64_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sh[w] = x;
+  __syncthreads();
+  if (w == 0) {
+    int64_t ws = lane < NW ? sh[lane] : 0;
+#pragma unroll
+    for (int o = 1; o < NW; o <<= 1) {
+      const int64_t y = __shfl_up(ws, o);
+      if (lane >= o) ws += y;
+    }
+    if (lane < NW) sh[NW + lane] = ws;
+  }
+  __syncthreads();
+  *total = sh[2 * NW - 1];
+  return (w > 0 ? sh[NW + w - 1] : 0) + x - v;
+}
+
+// Whole exclusive scan of up to SCAN_WG_MAX values in one workgroup, one
+// workgroup per array (blockIdx.x picks in0/out0 or in1/out1): thread t owns the
+// contiguous chunk [t c, (t+1) c), c <= 64, read as int4 (and written as pairs
+// of int64) when the chunk is whole and both arrays are 16-byte aligned there:
+// once for its sum, once more (from cache) to write its prefixes.
+// out[n] is the total.
+constexpr int SCAN_WG_C = 64;
+constexpr int64_t SCAN_WG_MAX = (int64_t)SCAN_WG_C * SCAN_BLOCK;
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_wg(const int* __restrict__ in0, int64_t* __restrict__ out0,
+                                                       const int* __restrict__ in1, int64_t* __restrict__ out1,
+                                                       int64_t n) {
+  __shared__ int64_t sh[2 * SCAN_BLOCK / 64];
+  const int* in = blockIdx.x ? in1 : in0;
+  int64_t* out = blockIdx.x ? out1 : out0;
+  const int c = (int)((n + SCAN_BLOCK - 1) / SCAN_BLOCK);
+  const int64_t b = (int64_t)threadIdx.x * c;
+  const bool whole = (c & 3) == 0 && b + c <= n && ((uintptr_t)(in + b) & 15) == 0 &&
+                     ((uintptr_t)(out + b) & 15) == 0;
+  const int4* in4 = reinterpret_cast<const int4*>(in + b);
+  int64_t s = 0;
+  if (whole) {
+    for (int k = 0; k < c / 4; ++k) {
+      const int4 q = in4[k];
+      s += (int64_t)q.x + q.y + q.z + q.w;
+    }
+  } else {
+    for (int k = 0; k < c; ++k)
+      if (b + k < n) s += in[b + k];
+  }
+  int64_t tot;
+  int64_t run = block_exclusive_scan_waves(s, sh, &tot);
+  if (whole) {
+    longlong2* o2 = reinterpret_cast<longlong2*>(out + b);
+    for (int k = 0; k < c / 4; ++k) {
+      const int4 q = in4[k];
+      const int64_t r1 = run + q.x, r2 = r1 + q.y, r3 = r2 + q.z;
+      o2[2 * k] = make_longlong2(run, r1);
+      o2[2 * k + 1] = make_longlong2(r2, r3);
+      run = r3 + q.w;
+    }
+  } else {
+    for (int k = 0; k < c; ++k)
+      if (b + k < n) {
+        out[b + k] = run;
+        run += in[b + k];
+      }
+  }
+  if (threadIdx.x == 0) out[n] = tot;
+}
+
 // Streaming copy, 16 bytes per thread and iteration (the HBM reference).
 __global__ __launch_bounds__(256) void k_hbm_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -139,6 +216,9 @@ int tgx_scan_i32(tgx_ctx* ctx, const int* in, int64_t* out, int64_t n, int64_t* 
   int64_t* sums = ctx->scan_tmp.as<int64_t>();
   if (n > 0 && n <= SCAN_SINGLE_MAX) {
     TGX_LAUNCH(ctx, "scan_single", k_scan_single, dim3(1), dim3(SCAN_BLOCK), 0, in, out, n);
+  } else if (n > 0 && n <= SCAN_WG_MAX) {
+    TGX_LAUNCH(ctx, "scan_wg", k_scan_wg, dim3(1), dim3(SCAN_BLOCK), 0, in, out, (const int*)nullptr,
+               (int64_t*)nullptr, n);
   } else if (n > 0) {
     TGX_LAUNCH(ctx, "scan_local", k_scan_local, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, in, out, n, sums);
     TGX_LAUNCH(ctx, "scan_sums", k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, sums, nb);
@@ -153,8 +233,22 @@ int tgx_scan_i32(tgx_ctx* ctx, const int* in, int64_t* out, int64_t n, int64_t* 
   return TGX_OK;
 }
 
+int tgx_scan2_i32(tgx_ctx* ctx, const int* in0, int64_t* out0, const int* in1, int64_t* out1, int64_t n) {
+  if (n > SCAN_SINGLE_MAX && n <= SCAN_WG_MAX) {
+    TGX_LAUNCH(ctx, "scan_wg", k_scan_wg, dim3(2), dim3(SCAN_BLOCK), 0, in0, out0, in1, out1, n);
+    return TGX_OK;
+  }
+  TGX_TRY(tgx_scan_i32(ctx, in0, out0, n, nullptr));
+  return tgx_scan_i32(ctx, in1, out1, n, nullptr);
+}
+
+int tgx_device_sum_launch(tgx_ctx* ctx, const double* d_x, int64_t n, double* d_part) {
+  TGX_LAUNCH(ctx, "sum_partial", k_sum_partial, dim3(TGX_SUM_PARTS), dim3(256), 0, d_x, n, d_part);
+  return TGX_OK;
+}
+
 int tgx_device_sum(tgx_ctx* ctx, const double* d_x, int64_t n, double* out_host) {
-  const int nbk = 256;
+  const int nbk = TGX_SUM_PARTS;
   TGX_TRY(ctx->red.ensure(sizeof(double) * nbk));
   TGX_LAUNCH(ctx, "sum_partial", k_sum_partial, dim3(nbk), dim3(256), 0, d_x, n, ctx->red.as<double>());
   double part[nbk];
