@@ -22,6 +22,9 @@
  *   tgx_compute                         src/turbogap.f90:1253-1380,1941-1954  the per-step SOAP loop and totals
  *   tgx_local_property_predict          src/local_properties.f90:174-297 local_property_predict
  *   tgx_velocity_verlet etc.            src/md.f90:226-345          velocity_verlet / berendsen_thermostat / remove_cm_vel
+ *   tgx_md_set_fixed                    src/md.f90:226-283          velocity_verlet's fix_atom argument
+ *   tgx_md_pressure                     src/turbogap_md.f90:287-295 instant_pressure, instant_pressure_tensor
+ *   tgx_berendsen_barostat              src/md.f90:384-417, src/turbogap_md.f90:413-426 berendsen_barostat on box and positions
  *   tgx_distance_2b_create              src/read_files.f90:722-742,5238-5305  a distance_2b block and its sparse set
  *   tgx_core_pot_create                 src/read_files.f90:5392-5510 a core_pot block, its table and spline
  *   tgx_angle_3b_create                 src/read_files.f90:743-770,5305-5384  an angle_3b block and its sparse set
@@ -255,6 +258,22 @@ int tgx_md_set_state(tgx_ctx* ctx, const double* velocities, const double* masse
 int tgx_md_get_state(tgx_ctx* ctx, double* positions, double* velocities);
 /* velocity_verlet (md.f90:226-286) with the resident forces and forces_prev. */
 int tgx_velocity_verlet(tgx_ctx* ctx, double dt, int first_step);
+/* fix_atom (3, n_sites), nonzero = that Cartesian component is fixed: velocity_verlet
+ * then zeroes its velocity (after the first step) and leaves its position. NULL
+ * (or all zero) frees every component. */
+int tgx_md_set_fixed(tgx_ctx* ctx, const int* fix_atom);
+/* Instantaneous pressure and pressure tensor in bar from the virial (3, 3) of the
+ * last force evaluation and the temperature: (kB (N - 1) T + tr(W)/3) / V and
+ * W / V + kB (N - 1) T / V on the diagonal, V the box volume. */
+int tgx_md_pressure(tgx_ctx* ctx, const double* virial, double temperature, double* pressure,
+                    double* pressure_tensor);
+/* berendsen_barostat: box vectors and positions scaled by
+ * (1 + dt/tau 4.5e-5/gamma (P - P0))^(1/3), P the mean of the pressure tensor's
+ * diagonal (sym "iso") or per Cartesian component from its diagonal ("diag");
+ * pressures in bar. The new box (a, b, c) goes to box_out (9, may be NULL). The
+ * resident list then needs a rebuild or refresh. */
+int tgx_berendsen_barostat(tgx_ctx* ctx, double P0, const double* P, const char* sym, double tau, double gamma,
+                           double dt, double* box_out);
 /* E_kin and instantaneous T = 2/(3 (N-1) kB) E_kin (turbogap_md.f90:281-295). */
 int tgx_kinetic(tgx_ctx* ctx, double* e_kin, double* temperature);
 /* berendsen_thermostat (md.f90:289-313). */

@@ -1,0 +1,59 @@
+"""CPU restatement of TurboGAP's MD integrator pieces (numpy, FP64).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the library's
+device MD; the product path never imports it.
+
+  * velocity_verlet follows src/md.f90:226-283: velocities t-dt -> t with the
+    mean of the two forces (not on the first step; a fixed component's
+    velocity is set to zero instead), positions_prev and forces_prev copied,
+    positions t -> t+dt except fixed components.
+  * berendsen_barostat follows src/md.f90:384-417: positions (or lattice
+    vectors, as columns) scaled by (1 + dt/tau 4.5e-5/gamma (P - P0))^(1/3),
+    P the mean diagonal pressure ("iso") or per component ("diag").
+  * pressure follows src/turbogap_md.f90:287-295 (bar; eVperA3tobar =
+    1602176.6208 as in src/turbogap.f90:95).
+Arrays are (n, 3) here; the reference's are (3, n).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+KB = 8.6173303e-5
+EV_PER_A3_TO_BAR = 1602176.6208
+
+
+def velocity_verlet(positions, velocities, forces, forces_prev, masses, dt, dt_prev, first_step, fix_atom=None):
+    """Returns (positions, positions_prev, velocities, forces_prev)."""
+    x = np.array(positions, dtype=np.float64)
+    v = np.array(velocities, dtype=np.float64)
+    F = np.asarray(forces, dtype=np.float64)
+    Fp = np.asarray(forces_prev, dtype=np.float64)
+    m = np.asarray(masses, dtype=np.float64)[:, None]
+    fix = np.zeros(x.shape, bool) if fix_atom is None else np.asarray(fix_atom, bool)
+    if not first_step:
+        v = np.where(fix, 0.0, v + 0.5 * (F + Fp) / m * dt_prev)
+    x_prev = x.copy()
+    x = np.where(fix, x, x + v * dt + 0.5 * F / m * dt ** 2)
+    return x, x_prev, v, F.copy()
+
+
+def berendsen_barostat(positions, P0, P, sym, tau, gamma, dt):
+    """positions (n, 3) (or the lattice vectors as rows); P (3, 3) in bar."""
+    x = np.array(positions, dtype=np.float64)
+    P = np.asarray(P, dtype=np.float64)
+    if sym[:3] == "iso":
+        P_iso = (P[0, 0] + P[1, 1] + P[2, 2]) / 3.0
+        return x * (1.0 + dt / tau * 4.5e-5 / gamma * (P_iso - P0)) ** (1.0 / 3.0)
+    if sym[:4] == "diag":
+        f = np.array([(1.0 + dt / tau * 4.5e-5 / gamma * (P[i, i] - P0)) ** (1.0 / 3.0) for i in range(3)])
+        return x * f[None, :]
+    raise ValueError(sym)
+
+
+def pressure(virial, temperature, n_sites, volume):
+    """(instant_pressure, instant_pressure_tensor) in bar."""
+    W = np.asarray(virial, dtype=np.float64).reshape(3, 3)
+    kin = KB * (n_sites - 1) * temperature
+    p = (kin + np.trace(W) / 3.0) / volume * EV_PER_A3_TO_BAR
+    pt = W / volume * EV_PER_A3_TO_BAR + np.eye(3) * kin / volume * EV_PER_A3_TO_BAR
+    return p, pt

@@ -1,0 +1,158 @@
+"""Fixed atoms, pressure and the Berendsen barostat of the device MD
+(src/md.f90:226-283, 384-417; src/turbogap_md.f90:287-295, 413-426).
+
+CPU (oracle pins):
+  * the velocity-Verlet restatement, driven the way the MD loop drives it,
+    follows the exact trajectory of a harmonic oscillator with second-order
+    error (halving dt divides it by four) and keeps fixed components where they
+    started, with zero velocity;
+  * the pressure's virial term tr(W)/3V is -dE/dV of the 2b oracle under
+    isotropic scaling (central differences), and the eV/A^3 -> bar constant is
+    e * 1e25 to 1e-8;
+  * the iso barostat scales the volume by exactly 1 + dt/tau 4.5e-5/gamma
+    (P - P0) and expands the box when P > P0; diag scales each axis by its own
+    component.
+GPU (parity): the library's velocity Verlet with fixed components against the
+oracle fed the library's own forces, four steps, to 1e-12; its pressure and
+barostat (iso and diag) against the oracle on the device state; a short NPT run.
+"""
+import numpy as np
+import pytest
+
+from oracle import md_oracle as MDO
+from oracle import oracle as O
+from oracle import pair_oracle as PO
+from tests import helpers as H
+from turbogap_amd import lib as L
+from turbogap_amd import structures as S
+from turbogap_amd.md import MD, maxwell_velocities
+
+from tests.test_pair_models import silicon_crystal, well_conditioned_2b
+
+
+def test_vv_follows_a_harmonic_oscillator():
+    """Driven as the MD loop drives it (forces at x(t), then the update), the
+    positions after t = 20 match the exact solution with the second-order error
+    of velocity Verlet: halving dt divides it by four."""
+    rng = np.random.default_rng(0)
+    n = 5
+    x0, v0 = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+    m, k, t_end = np.full(n, 2.0), 3.0, 20.0
+    w = np.sqrt(k / m[0])
+    fix = np.zeros((n, 3), bool)
+    fix[1, 2] = fix[3, 0] = True
+    free = ~fix
+    exact = x0 * np.cos(w * t_end) + v0 / w * np.sin(w * t_end)
+    err = []
+    for dt in (0.02, 0.01):
+        x, v, Fp = x0.copy(), v0.copy(), np.zeros((n, 3))
+        for s in range(int(round(t_end / dt))):
+            x, _, v, Fp = MDO.velocity_verlet(x, v, -k * x, Fp, m, dt, dt, s == 0, fix)
+        err.append(np.abs(x[free] - exact[free]).max())
+        assert np.array_equal(x[fix], x0[fix]) and not v[fix].any()
+    assert err[1] < 1e-3 and 3.6 < err[0] / err[1] < 4.4, err
+
+
+def test_pressure_virial_term_is_minus_dE_dV():
+    kw = well_conditioned_2b()
+    sysm = silicon_crystal()
+
+    def energy(s):
+        p, b = sysm.positions * s, sysm.box * s
+        nl = O.build_neighbors_list(p, sysm.species, b, kw["rcut"])
+        return PO.distance_2b_energy_and_forces(nl, sysm.species, sysm.n, kw["species1"], kw["species2"], kw["rcut"],
+                                                kw["buffer"], kw["delta"], kw["sigma"], kw["e0"], kw["Qs"],
+                                                kw["alphas"], kw["cutoff"])
+    E, _, W = energy(1.0)
+    V = abs(np.linalg.det(sysm.box))
+    h = 1e-5
+    dEdV = (energy(1 + h)[0].sum() - energy(1 - h)[0].sum()) / (V * ((1 + h) ** 3 - (1 - h) ** 3))
+    p, pt = MDO.pressure(W, 0.0, sysm.n, V)
+    assert abs(p / MDO.EV_PER_A3_TO_BAR - (-dEdV)) < 1e-6 * max(1.0, abs(dEdV))
+    assert abs(np.trace(pt) / 3 - p) < 1e-9 * max(1.0, abs(p))
+    assert abs(MDO.EV_PER_A3_TO_BAR / (1.602176634e-19 * 1e30 / 1e5) - 1) < 1e-8
+    # the kinetic term alone: P V = (N - 1) kB T
+    p_kin, _ = MDO.pressure(np.zeros((3, 3)), 500.0, 1001, 1.0e4)
+    assert abs(p_kin * 1.0e4 / MDO.EV_PER_A3_TO_BAR - 1000 * MDO.KB * 500.0) < 1e-12
+
+
+def test_barostat_scaling():
+    box = np.diag([10.0, 11.0, 12.0])
+    P = np.diag([2.0e4, 1.0e4, -5.0e3])
+    dt, tau, gamma, P0 = 1.0, 100.0, 2.0, 3.0e3
+    iso = MDO.berendsen_barostat(box, P0, P, "iso", tau, gamma, dt)
+    ratio = np.linalg.det(iso) / np.linalg.det(box)
+    assert abs(ratio - (1 + dt / tau * 4.5e-5 / gamma * (np.trace(P) / 3 - P0))) < 1e-14
+    assert ratio > 1.0  # mean pressure above P0: the box grows
+    diag = MDO.berendsen_barostat(box, P0, P, "diag", tau, gamma, dt)
+    for i in range(3):
+        assert abs((diag[i, i] / box[i, i]) ** 3 - (1 + dt / tau * 4.5e-5 / gamma * (P[i, i] - P0))) < 1e-14
+    assert diag[2, 2] < box[2, 2] and diag[0, 0] > box[0, 0]
+
+
+def _md_system(ctx, T=600.0, seed=3):
+    sysm, h, Qs, al = H.carbon216(300)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    ctx.soap_model(h, Qs, al)
+    mass = S.masses_for(sysm.species)
+    return sysm, mass, maxwell_velocities(mass, T, seed=seed)
+
+
+@pytest.mark.gpu
+def test_gpu_fixed_atoms_match_oracle(ctx):
+    sysm, mass, v0 = _md_system(ctx)
+    fix = np.random.default_rng(5).random((sysm.n, 3)) < 0.2
+    md = MD(ctx, mass, v0, 3.7, "fp64_adjoint", fix_atom=fix)
+    x, v, Fp = sysm.positions.copy(), v0.copy(), np.zeros((sysm.n, 3))
+    dt = 1.0
+    for s in range(4):
+        md.forces()
+        _, F = ctx.results()
+        L.check(L.load().tgx_velocity_verlet(ctx.h, dt, int(s == 0)), "velocity_verlet")
+        x, _, v, Fp = MDO.velocity_verlet(x, v, F, Fp, mass, dt, dt, s == 0, fix)
+        xd, vd = md.state()
+        np.testing.assert_allclose(xd, x, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(vd, v, rtol=0, atol=1e-13)
+    assert np.array_equal(xd[fix], sysm.positions[fix]) and not vd[fix].any()
+    assert np.abs(xd[~fix] - sysm.positions[~fix]).max() > 1e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sym", ["iso", "diag"])
+def test_gpu_pressure_and_barostat_match_oracle(ctx, sym):
+    sysm, mass, v0 = _md_system(ctx)
+    md = MD(ctx, mass, v0, 3.7, "fp64_adjoint", box=sysm.box)
+    md.forces()
+    L.check(L.load().tgx_velocity_verlet(ctx.h, 1.0, 1), "velocity_verlet")
+    _, T = md.kinetic()
+    p, pt = md.pressure_of(T)
+    V = abs(np.linalg.det(sysm.box))
+    po, pto = MDO.pressure(md.virial, T, sysm.n, V)
+    assert abs(p - po) <= 1e-12 * abs(po) and np.abs(pt - pto).max() <= 1e-12 * np.abs(pto).max()
+    x_before, _ = md.state()
+    P0 = float(np.trace(pt) / 3) - 2.0e4
+    nb = np.empty(9)
+    L.check(L.load().tgx_berendsen_barostat(ctx.h, P0, L.dptr(np.ascontiguousarray(pt).reshape(9)), sym.encode(),
+                                            500.0, 1.0, 1.0, L.dptr(nb)), "berendsen_barostat")
+    x_after, _ = md.state()
+    np.testing.assert_allclose(x_after, MDO.berendsen_barostat(x_before, P0, pt, sym, 500.0, 1.0, 1.0), rtol=1e-14,
+                               atol=1e-13)
+    np.testing.assert_allclose(nb.reshape(3, 3), MDO.berendsen_barostat(sysm.box, P0, pt, sym, 500.0, 1.0, 1.0),
+                               rtol=1e-14, atol=0)
+    if sym == "iso":
+        assert np.linalg.det(nb.reshape(3, 3)) > V  # P above P0: expansion
+
+
+@pytest.mark.gpu
+def test_gpu_npt_steps(ctx):
+    """Berendsen NPT on the device: the box follows the pressure's sign each
+    step, the list is refreshed against the current box, energies stay finite."""
+    sysm, mass, v0 = _md_system(ctx, T=300.0)
+    md = MD(ctx, mass, v0, 3.7, "fp64_adjoint", neighbors_buffer=0.3, box=sysm.box)
+    vol = abs(np.linalg.det(sysm.box))
+    for _ in range(6):
+        E, ek, T = md.step(1.0, T0=300.0, tau=100.0, P0=0.0, tau_p=200.0, gamma_p=1.0)
+        new = abs(np.linalg.det(md.box))
+        assert np.isfinite(E) and np.isfinite(md.pressure)
+        assert (new - vol) * md.pressure > 0  # grows under positive pressure, shrinks under negative
+        vol = new

@@ -195,6 +195,8 @@ struct tgx_ctx {
   // md
   dbuf vel, mass, forces_prev, pos_prev;
   dbuf md_disp, md_flag;      // displacement since the last list build, rebuild flag
+  dbuf md_fix;                 // unsigned char[3 n]: fixed Cartesian components
+  bool md_fixed = false;
   int* md_flag_host = nullptr;  // pinned
   double dt_prev = 0.0;
   // surrogate gate

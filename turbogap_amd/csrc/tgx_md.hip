@@ -1,29 +1,44 @@
 // Molecular-dynamics step on the device, with the semantics of
-// src/md.f90:226-345 (velocity_verlet, berendsen_thermostat, remove_cm_vel)
-// and the temperature of src/turbogap_md.f90:281-295.
+// src/md.f90:226-417 (velocity_verlet with fixed components,
+// berendsen_thermostat, remove_cm_vel, berendsen_barostat) and the temperature
+// and pressure of src/turbogap_md.f90:281-295.
 // Units: Å, fs, eV; masses in eV fs^2 / Å^2 (amu * 103.6426965268).
 #include "tgx_internal.h"
 #include "tgx_util.h"
 
+#include <cmath>
+
 namespace {
 
 // velocities at t-dt -> t (not on the first step), then positions t -> t+dt.
+// A fixed component (fix[j] != 0) gets velocity 0 after the first step and
+// keeps its position; positions_prev and forces_prev are copied for every atom.
 __global__ void k_vv(int n, double dt, double dt_prev, int first, const double* __restrict__ F,
                      double* __restrict__ Fprev, const double* __restrict__ mass, double* __restrict__ v,
-                     double* __restrict__ x, double* __restrict__ xprev) {
+                     double* __restrict__ x, double* __restrict__ xprev, const unsigned char* __restrict__ fix) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double m = mass[i];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const size_t j = 3 * (size_t)i + k;
+    const bool fx = fix && fix[j];
     double vj = v[j];
-    if (!first) vj = vj + 0.5 * (F[j] + Fprev[j]) / m * dt_prev;
+    if (!first) vj = fx ? 0.0 : vj + 0.5 * (F[j] + Fprev[j]) / m * dt_prev;
     v[j] = vj;
     xprev[j] = x[j];
     Fprev[j] = F[j];
-    x[j] = x[j] + vj * dt + 0.5 * F[j] / m * dt * dt;
+    if (!fx) x[j] = x[j] + vj * dt + 0.5 * F[j] / m * dt * dt;
   }
+}
+
+// x_c *= f_c for every atom (the barostat's scaling, per Cartesian component).
+__global__ void k_scale3(int n, double f0, double f1, double f2, double* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[3 * (size_t)i] *= f0;
+  x[3 * (size_t)i + 1] *= f1;
+  x[3 * (size_t)i + 2] *= f2;
 }
 
 __global__ void k_scale(int n3, double f, double* __restrict__ v) {
@@ -131,8 +146,74 @@ extern "C" int tgx_velocity_verlet(tgx_ctx* ctx, double dt, int first_step) {
   const int n = ctx->n_sites;
   TGX_LAUNCH(ctx, "md_verlet", k_vv, dim3((n + 255) / 256), dim3(256), 0, n, dt, ctx->dt_prev, first_step ? 1 : 0,
              ctx->forces.as<double>(), ctx->forces_prev.as<double>(), ctx->mass.as<double>(), ctx->vel.as<double>(),
-             ctx->pos.as<double>(), ctx->pos_prev.as<double>());
+             ctx->pos.as<double>(), ctx->pos_prev.as<double>(),
+             ctx->md_fixed ? ctx->md_fix.as<unsigned char>() : (const unsigned char*)nullptr);
   ctx->dt_prev = dt;
+  return TGX_OK;
+}
+
+extern "C" int tgx_md_set_fixed(tgx_ctx* ctx, const int* fix_atom) {
+  MD_CTX(ctx);
+  ctx->md_fixed = false;
+  if (!fix_atom) return TGX_OK;
+  const size_t n3 = 3 * (size_t)ctx->n_sites;
+  std::vector<unsigned char> f(n3);
+  bool any = false;
+  for (size_t j = 0; j < n3; ++j) {
+    f[j] = fix_atom[j] ? 1 : 0;
+    any = any || f[j];
+  }
+  if (!any) return TGX_OK;
+  TGX_TRY(ctx->md_fix.ensure(n3));
+  TGX_CHECK_HIP(hipMemcpyAsync(ctx->md_fix.p, f.data(), n3, hipMemcpyHostToDevice, ctx->stream));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->md_fixed = true;
+  return TGX_OK;
+}
+
+static double box_volume(const double* b) {
+  return std::fabs(b[0] * (b[4] * b[8] - b[5] * b[7]) - b[1] * (b[3] * b[8] - b[5] * b[6]) +
+                   b[2] * (b[3] * b[7] - b[4] * b[6]));
+}
+
+extern "C" int tgx_md_pressure(tgx_ctx* ctx, const double* virial, double temperature, double* pressure,
+                               double* pressure_tensor) {
+  MD_CTX(ctx);
+  TGX_REQUIRE(virial, "tgx_md_pressure: null virial");
+  const double kB = 8.6173303e-5, eVperA3tobar = 1602176.6208;
+  const double v = box_volume(ctx->box);
+  TGX_REQUIRE(v > 0.0, "tgx_md_pressure: degenerate box");
+  const double kin = kB * (double)(ctx->n_sites - 1) * temperature;
+  if (pressure) *pressure = (kin + (virial[0] + virial[4] + virial[8]) / 3.0) / v * eVperA3tobar;
+  if (pressure_tensor) {
+    for (int k = 0; k < 9; ++k) pressure_tensor[k] = virial[k] / v * eVperA3tobar;
+    for (int c = 0; c < 3; ++c) pressure_tensor[4 * c] += kin / v * eVperA3tobar;
+  }
+  return TGX_OK;
+}
+
+extern "C" int tgx_berendsen_barostat(tgx_ctx* ctx, double P0, const double* P, const char* sym, double tau,
+                                      double gamma, double dt, double* box_out) {
+  MD_CTX(ctx);
+  TGX_REQUIRE(P && sym && tau > 0.0 && gamma > 0.0, "tgx_berendsen_barostat: null input, tau <= 0 or gamma <= 0");
+  double f[3];
+  if (!strncmp(sym, "iso", 3)) {
+    const double Piso = (P[0] + P[4] + P[8]) / 3.0;
+    f[0] = f[1] = f[2] = std::pow(1.0 + dt / tau * 4.5e-5 / gamma * (Piso - P0), 1.0 / 3.0);
+  } else if (!strncmp(sym, "diag", 4)) {
+    for (int c = 0; c < 3; ++c) f[c] = std::pow(1.0 + dt / tau * 4.5e-5 / gamma * (P[4 * c] - P0), 1.0 / 3.0);
+  } else {
+    TGX_REQUIRE(false, "tgx_berendsen_barostat: barostat_sym must be \"iso\" or \"diag\"");
+  }
+  for (int c = 0; c < 3; ++c) TGX_REQUIRE(std::isfinite(f[c]) && f[c] > 0.0, "tgx_berendsen_barostat: scaling %g", f[c]);
+  // lattice vectors a, b, c (rows of box): component c of each scales with f[c]
+  for (int v = 0; v < 3; ++v)
+    for (int c = 0; c < 3; ++c) ctx->box[3 * v + c] *= f[c];
+  if (box_out)
+    for (int k = 0; k < 9; ++k) box_out[k] = ctx->box[k];
+  const int n = ctx->n_sites;
+  TGX_LAUNCH(ctx, "md_barostat", k_scale3, dim3((n + 255) / 256), dim3(256), 0, n, f[0], f[1], f[2],
+             ctx->pos.as<double>());
   return TGX_OK;
 }
 

@@ -28,15 +28,30 @@ def maxwell_velocities(masses: np.ndarray, T: float, seed: int) -> np.ndarray:
 
 
 class MD:
+    """fix_atom (n, 3) booleans hold Cartesian components in place
+    (velocity_verlet's fix_atom). box (3, 3) rows a, b, c: the system's box, kept
+    up to date when a barostat rescales it."""
+
     def __init__(self, ctx, masses: np.ndarray, velocities: np.ndarray, rcut: float, path: str = "fp64_plain",
-                 neighbors_buffer: float = 0.0):
+                 neighbors_buffer: float = 0.0, fix_atom=None, box=None):
         self.ctx, self.rcut, self.path = ctx, rcut, path
         self.buffer = float(neighbors_buffer)
         self.masses = np.ascontiguousarray(masses, dtype=np.float64)
         v = np.ascontiguousarray(velocities, dtype=np.float64)
         L.check(L.load().tgx_md_set_state(ctx.h, L.dptr(v), L.dptr(self.masses)), "md_set_state")
+        if fix_atom is not None:
+            fx = np.ascontiguousarray(np.asarray(fix_atom, dtype=bool).reshape(-1, 3), dtype=np.int32)
+            if fx.shape[0] != self.masses.shape[0]:
+                raise ValueError("fix_atom must be (n_sites, 3)")
+            L.check(L.load().tgx_md_set_fixed(ctx.h, L.iptr(fx)), "md_set_fixed")
+        else:
+            L.check(L.load().tgx_md_set_fixed(ctx.h, None), "md_set_fixed")
+        self.box = None if box is None else np.array(box, dtype=np.float64).reshape(3, 3)
         self.step_count = 0
         self.energy = None
+        self.virial = None
+        self.pressure = None
+        self.pressure_tensor = None
         self.rebuild = True
         self.n_builds = 0
 
@@ -62,12 +77,31 @@ class MD:
         L.check(L.load().tgx_kinetic(self.ctx.h, C.byref(ek), C.byref(T)), "kinetic")
         return ek.value, T.value
 
-    def step(self, dt: float, T0: float = None, tau: float = 100.0, path=None):
-        """One step: forces at x(t), velocity Verlet, optional Berendsen to T0, wrap."""
+    def pressure_of(self, T: float):
+        """(pressure, pressure tensor (3, 3)) in bar from the last forces' virial
+        and temperature T (src/turbogap_md.f90:287-295)."""
+        p = C.c_double()
+        pt = np.empty(9)
+        w = np.ascontiguousarray(self.virial, dtype=np.float64).reshape(9)
+        L.check(L.load().tgx_md_pressure(self.ctx.h, L.dptr(w), float(T), C.byref(p), L.dptr(pt)), "md_pressure")
+        return p.value, pt.reshape(3, 3)
+
+    def step(self, dt: float, T0: float = None, tau: float = 100.0, path=None, P0: float = None,
+             tau_p: float = 1000.0, gamma_p: float = 1.0, barostat_sym: str = "iso"):
+        """One step: forces at x(t), velocity Verlet, E_kin, T and pressure, then
+        (P0 given) the Berendsen barostat on box and positions, (T0 given) the
+        Berendsen thermostat, wrap (src/turbogap_md.f90:280-440)."""
         lib = L.load()
         self.forces(path)
         L.check(lib.tgx_velocity_verlet(self.ctx.h, float(dt), int(self.step_count == 0)), "velocity_verlet")
         ek, T = self.kinetic()
+        self.pressure, self.pressure_tensor = self.pressure_of(T)
+        if P0 is not None:
+            nb = np.empty(9)
+            pt = np.ascontiguousarray(self.pressure_tensor).reshape(9)
+            L.check(lib.tgx_berendsen_barostat(self.ctx.h, float(P0), L.dptr(pt), barostat_sym.encode(), float(tau_p),
+                                               float(gamma_p), float(dt), L.dptr(nb)), "berendsen_barostat")
+            self.box = nb.reshape(3, 3)
         if T0 is not None:
             L.check(lib.tgx_berendsen(self.ctx.h, float(T0), float(T), float(tau), float(dt)), "berendsen")
         L.check(lib.tgx_wrap_pbc(self.ctx.h), "wrap_pbc")
