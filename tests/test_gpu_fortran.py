@@ -3,7 +3,9 @@
 gpu_context_init -> build_neighbors_list -> soap_backend_begin ->
 get_gap_soap -> soap_backend_end on the 216-atom carbon box, with the sparse set
 read from the reference's alphas / desc_sparse file format. Results must equal
-the committed fixture (energy 1e-8 relative, forces 1e-6 eV/Å)."""
+the committed fixture (energy 1e-8 relative, forces 1e-6 eV/Å). A second host
+(tgx_pair_example) creates and evaluates the distance_2b, angle_3b and core_pot
+families through the same shim, checked against the oracle."""
 import os
 import pathlib
 import subprocess
@@ -46,3 +48,76 @@ def test_fortran_host_matches_fixture(tmp_path):
     assert abs(E - ref["energies"].sum()) <= 1e-8 * abs(ref["energies"].sum())
     assert np.abs(np.array(vals["force1"]) - ref["forces"][0]).max() <= 1e-6
     assert abs(vals["virial_trace"][0] - np.trace(ref["virial"])) <= 3e-6
+
+
+PAIR_EXE = ROOT / "turbogap_amd" / "fortran" / "_build" / "tgx_pair_example"
+
+
+def test_fortran_pair_families_match_oracle(tmp_path):
+    """distance_2b, angle_3b and core_pot blocks of the reference suite's silicon
+    potential created and evaluated from Fortran (tgx_distance_2b_model /
+    tgx_angle_3b_model / tgx_core_pot_model, add_pair_contribution) on a
+    216-atom silicon crystal, against the oracle at the tolerances of the
+    Python-side parity tests."""
+    from oracle import angle_3b as A3
+    from oracle import oracle as O
+    from oracle import pair_oracle as PO
+    from tests.test_angle_3b import silicon_3b
+    from tests.test_pair_models import cancel_scale, silicon_crystal, silicon_models
+
+    if not PAIR_EXE.exists():
+        pytest.fail(f"{PAIR_EXE} not built (make -C turbogap_amd/fortran)")
+    sysm = silicon_crystal()
+    m = silicon_models()
+    b2, cp, b3 = m["distance_2b"], m["core_pot"], silicon_3b()
+    with open(tmp_path / "positions.dat", "w") as f:
+        f.write(f"{sysm.n} {float(sysm.box[0, 0]):.17g}\n")
+        for p, s in zip(sysm.positions, sysm.species):
+            f.write(f"{p[0]:.17g} {p[1]:.17g} {p[2]:.17g} {int(s)}\n")
+    with open(tmp_path / "2b.dat", "w") as f:
+        f.write(f"{b2['species1']} {b2['species2']} {b2['rcut']:.17g} {b2['delta']:.17g} {b2['sigma']:.17g} "
+                f"{len(b2['Qs'])}\n")
+        for q, a, c in zip(b2["Qs"], b2["alphas"], b2["cutoff"]):
+            f.write(f"{q:.17g} {a:.17g} {c:.17g}\n")
+    with open(tmp_path / "3b.dat", "w") as f:
+        sg = " ".join(f"{x:.17g}" for x in b3["sigma"])
+        f.write(f"{b3['species_center']} {b3['species1']} {b3['species2']} {b3['rcut']:.17g} {b3['delta']:.17g} {sg} "
+                f"{b3['kernel_type']} {len(b3['alphas'])}\n")
+        for q, a, c in zip(b3["Qs"], b3["alphas"], b3["cutoff"]):
+            f.write(f"{q[0]:.17g} {q[1]:.17g} {q[2]:.17g} {a:.17g} {c:.17g}\n")
+    with open(tmp_path / "corepot.dat", "w") as f:
+        f.write(f"{cp['species1']} {cp['species2']} {len(cp['x'])} {cp['yp1']:.17g} {cp['ypn']:.17g}\n")
+        for x, v in zip(cp["x"], cp["V"]):
+            f.write(f"{x:.17g} {v:.17g}\n")
+    out = subprocess.run([str(PAIR_EXE)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+
+    def read(kind):
+        rows = (tmp_path / f"out_{kind}.dat").read_text().split("\n")
+        W = np.array([float(t) for t in rows[0].split()]).reshape(3, 3).T
+        body = np.array([[float(t) for t in r.split()] for r in rows[1:1 + sysm.n]])
+        return body[:, 0], body[:, 1:], W
+
+    def nl(rc):
+        return O.build_neighbors_list(sysm.positions, sysm.species, sysm.box, rc)
+
+    ref = {
+        "distance_2b": PO.distance_2b_energy_and_forces(nl(b2["rcut"]), sysm.species, sysm.n, b2["species1"],
+                                                        b2["species2"], b2["rcut"], 0.5, b2["delta"], b2["sigma"], 0.0,
+                                                        b2["Qs"], b2["alphas"], b2["cutoff"]),
+        "angle_3b": A3.angle_3b_energy_and_forces(nl(b3["rcut"]), sysm.species, sysm.n, b3["species_center"],
+                                                  b3["species1"], b3["species2"], b3["rcut"], 0.5, b3["delta"],
+                                                  b3["sigma"], 0.0, b3["Qs"], b3["alphas"], b3["cutoff"],
+                                                  b3["kernel_type"]),
+        "core_pot": PO.core_pot_energy_and_forces(nl(float(cp["x"].max())), sysm.species, sysm.n, cp["species1"],
+                                                  cp["species2"], PO.core_pot_model(cp["x"], cp["V"], cp["yp1"],
+                                                                                    cp["ypn"])),
+    }
+    scale = {"distance_2b": cancel_scale("distance_2b", b2), "core_pot": 1.0,
+             "angle_3b": max(1.0, 2 * b3["delta"] ** 2 * np.abs(b3["alphas"] * b3["cutoff"]).sum())}
+    for kind, (Eo, Fo, Wo) in ref.items():
+        e, F, W = read(kind)
+        s = scale[kind]
+        assert np.abs(e - Eo).max() <= 1e-12 * s, kind
+        assert np.abs(F - Fo).max() <= 1e-11 * max(s, np.abs(Fo).max()), kind
+        assert np.abs(W - Wo).max() <= 1e-11 * max(s, np.abs(Wo).max()), kind

@@ -11,6 +11,10 @@
 !   soap_backend_begin / soap_backend_end     src/gap.f90:49-64
 !   get_gap_soap                              src/gap_interface.f90:41-480 (results on the host)
 !   get_soap_energy_and_forces                src/gap.f90:66-259
+!   tgx_distance_2b_model / tgx_angle_3b_model / tgx_core_pot_model and
+!   add_pair_contribution                     the gap_backend seam's add_2b_ /
+!                                             add_3b_ / add_core_pot_contribution
+!                                             (src/gap_backend_cpu.f90:79-267)
 !
 ! Errors: a non-zero status prints tgx_last_error() and stops with code 1, the
 ! behaviour of turbogap_abort (src/error.f90:48-61).
@@ -32,7 +36,8 @@ module tgx_turbogap
    type(c_ptr), save :: ctx = c_null_ptr
 
    public :: gpu_context_init, gpu_context_finalize, build_neighbors_list, soap_model_from_files, &
-             soap_backend_begin, soap_backend_end, get_gap_soap, get_soap_energy_and_forces, tgx_check
+             soap_backend_begin, soap_backend_end, get_gap_soap, get_soap_energy_and_forces, tgx_check, &
+             tgx_distance_2b_model, tgx_angle_3b_model, tgx_core_pot_model, add_pair_contribution
 
    interface
       integer(c_int) function c_tgx_init(device, ctx_out) bind(C, name="tgx_init")
@@ -113,6 +118,39 @@ module tgx_turbogap
          real(c_double), value :: e0
          real(c_double) :: soap(*), sder(*), xyz(*), en(*), f(*), vir(*)
          integer(c_int) :: nn(*), nl(*)
+      end function
+      integer(c_int) function c_tgx_2b(c, s1, s2, rcut, buffer, delta, sigma, e0, ns, qs, al, cut, mid) &
+         bind(C, name="tgx_distance_2b_create")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c
+         integer(c_int), value :: s1, s2, ns
+         real(c_double), value :: rcut, buffer, delta, sigma, e0
+         real(c_double) :: qs(*), al(*), cut(*)
+         integer(c_int) :: mid
+      end function
+      integer(c_int) function c_tgx_3b(c, sc, s1, s2, rcut, buffer, delta, sigma, e0, kt, ns, qs, al, cut, mid) &
+         bind(C, name="tgx_angle_3b_create")
+         import :: c_int, c_ptr, c_double, c_char
+         type(c_ptr), value :: c
+         integer(c_int), value :: sc, s1, s2, ns
+         real(c_double), value :: rcut, buffer, delta, e0
+         real(c_double) :: sigma(*), qs(*), al(*), cut(*)
+         character(kind=c_char) :: kt(*)
+         integer(c_int) :: mid
+      end function
+      integer(c_int) function c_tgx_core_pot(c, s1, s2, n, x, v, yp1, ypn, cut, buf, mid) bind(C, name="tgx_core_pot_create")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c
+         integer(c_int), value :: s1, s2, n
+         real(c_double), value :: yp1, ypn, cut, buf
+         real(c_double) :: x(*), v(*)
+         integer(c_int) :: mid
+      end function
+      integer(c_int) function c_tgx_pair_eaf(c, mid, en, f, vir) bind(C, name="tgx_pair_energy_and_forces")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c
+         integer(c_int), value :: mid
+         real(c_double) :: en(*), f(*), vir(*)
       end function
       integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
          import :: c_size_t, c_ptr
@@ -312,6 +350,75 @@ contains
          call system_clock(t1)
          write (*, '(A, F9.3, A)') '  *) GAP prediction: ', real(t1 - t0, dp)/real(rate, dp), ' seconds |'
       end if
+   end subroutine
+
+   ! A distance_2b block (species as the codes given to build_neighbors_list),
+   ! with the buffer 0.5 and e0 = 0 the gap_backend passes
+   ! (src/gap_backend_cpu.f90:120-126). Qs(n_sparse) distances.
+   function tgx_distance_2b_model(species1, species2, rcut, delta, sigma, Qs, alphas, cutoff) result(id)
+      integer, intent(in) :: species1, species2
+      real(dp), intent(in) :: rcut, delta, sigma, Qs(:), alphas(:), cutoff(:)
+      integer :: id
+      integer(c_int) :: mid
+      real(c_double), allocatable :: q(:), a(:), c(:)
+      q = Qs
+      a = alphas
+      c = cutoff
+      call tgx_check(c_tgx_2b(ctx, int(species1, c_int), int(species2, c_int), rcut, 0.5_dp, delta, sigma, 0.0_dp, &
+                              int(size(q), c_int), q, a, c, mid), "distance_2b")
+      id = int(mid)
+   end function
+
+   ! An angle_3b block: Qs(n_sparse, 3) as the reference stores it
+   ! (src/read_files.f90:743-770), sigma(3), kernel_type "exp" or "pol"; buffer
+   ! 0.5 and e0 = 0 as the gap_backend passes (src/gap_backend_cpu.f90:250-258).
+   function tgx_angle_3b_model(species_center, species1, species2, rcut, delta, sigma, kernel_type, Qs, alphas, &
+                               cutoff) result(id)
+      integer, intent(in) :: species_center, species1, species2
+      real(dp), intent(in) :: rcut, delta, sigma(3), Qs(:, :), alphas(:), cutoff(:)
+      character(len=*), intent(in) :: kernel_type
+      integer :: id
+      integer(c_int) :: mid
+      real(c_double), allocatable :: qt(:, :), a(:), c(:)
+      real(c_double) :: sg(3)
+      qt = transpose(Qs)  ! (3, n_sparse): each sparse point's components contiguous
+      a = alphas
+      c = cutoff
+      sg = sigma
+      call tgx_check(c_tgx_3b(ctx, int(species_center, c_int), int(species1, c_int), int(species2, c_int), rcut, &
+                              0.5_dp, delta, sg, 0.0_dp, trim(kernel_type)//c_null_char, int(size(a), c_int), qt, a, &
+                              c, mid), "angle_3b")
+      id = int(mid)
+   end function
+
+   ! A core_pot block: the table V(x), end slopes (>= 0.99e30: natural end) and
+   ! core_pot_cutoff / core_pot_buffer (src/read_files.f90:5392-5510).
+   function tgx_core_pot_model(species1, species2, x, V, yp1, ypn, cutoff, buffer) result(id)
+      integer, intent(in) :: species1, species2
+      real(dp), intent(in) :: x(:), V(:), yp1, ypn, cutoff, buffer
+      integer :: id
+      integer(c_int) :: mid
+      real(c_double), allocatable :: xx(:), vv(:)
+      xx = x
+      vv = V
+      call tgx_check(c_tgx_core_pot(ctx, int(species1, c_int), int(species2, c_int), int(size(xx), c_int), xx, vv, &
+                                    yp1, ypn, cutoff, buffer, mid), "core_pot")
+      id = int(mid)
+   end function
+
+   ! One model's energies, forces and virial on the resident list, accumulated
+   ! into the driver's arrays as add_2b_contribution / add_3b_contribution /
+   ! add_core_pot_contribution accumulate into energies_2b etc.
+   subroutine add_pair_contribution(id, energies, forces, virial)
+      integer, intent(in) :: id
+      real(dp), intent(inout) :: energies(:), forces(:, :), virial(3, 3)
+      real(c_double), allocatable :: e(:), f(:, :)
+      real(c_double) :: w(3, 3)
+      allocate (e(size(energies)), f(3, size(forces, 2)))
+      call tgx_check(c_tgx_pair_eaf(ctx, int(id, c_int), e, f, w), "pair_energy_and_forces")
+      energies = energies + e
+      forces = forces + f
+      virial = virial + w
    end subroutine
 
 end module tgx_turbogap
