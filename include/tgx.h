@@ -291,8 +291,13 @@ int tgx_wrap_pbc(tgx_ctx* ctx);
 int tgx_md_displacement(tgx_ctx* ctx, double neighbors_buffer, int reset, int* rebuild);
 
 /* ---- profiling ----------------------------------------------------------- */
-/* Record HIP events around every kernel launch on the context's stream. */
+/* Record HIP events around every kernel launch on the context's stream. While
+ * on, the surrogate path's blocks run one after the other. */
 int tgx_set_profiling(tgx_ctx* ctx, int on);
+/* Execution option with no counterpart in the reference: the surrogate path runs
+ * blocks of distinct central species concurrently, each on a stream of its own
+ * (on by default); 0 runs them in turn on the context's stream. */
+int tgx_set_block_streams(tgx_ctx* ctx, int on);
 /* Launch count and total milliseconds of kernels whose name contains `name`. */
 int tgx_kernel_time(tgx_ctx* ctx, const char* name, int* count, double* total_ms);
 int tgx_reset_profile(tgx_ctx* ctx);

@@ -213,6 +213,38 @@ def test_surrogate_65k_atoms_match_mlp_oracle(ctx):
         assert np.abs(e_blk - e_ref).max() <= 2e-2 * np.abs(e_ref + 0.5).max()
 
 
+def test_surrogate_block_streams_match_serial(ctx):
+    """The two blocks (central species A and B) on streams of their own give
+    bit-identical energies, forces and virial to the same step with the blocks
+    in turn, also with the ensemble gate sending sites to the FP64 fallback."""
+    _, models = H.subset(500, 1024)
+    sysm = S.binary_system(8192, seed=2001)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    ctx.build_neighbors_list(3.7, fetch=False)
+    ids = H.load_models(ctx, models)
+    for k, (mid, (h, _, _)) in enumerate(zip(ids, models)):
+        for i in range(2):
+            net = SG.MLP(ctx, h.dim, 128, 256, seed=41 + 7 * k + i)
+            net.set_normalization(-0.5, 0.03 + 0.01 * i)
+            net.install(mid, i)
+    SG.set_gate(ctx, 2, 1e9)
+    ctx.compute("surrogate")
+    half = 0.5 * SG.gate_stats(ctx)[1]  # gates the sites whose members differ most
+    for threshold in (1e9, half):
+        SG.set_gate(ctx, 2, threshold)
+        out = []
+        for on in (True, False):
+            ctx.set_block_streams(on)
+            E, W = ctx.compute("surrogate")
+            en, F = ctx.results()
+            out.append((E, W, en, F, SG.gate_stats(ctx)))
+        ctx.set_block_streams(True)
+        (E1, W1, e1, F1, g1), (E2, W2, e2, F2, g2) = out
+        assert E1 == E2 and np.array_equal(W1, W2) and np.array_equal(e1, e2) and np.array_equal(F1, F2)
+        assert g1 == g2
+    assert 0.0 < g1[0] < 1.0  # the second threshold gates part of the sites
+
+
 def test_surrogate_ensemble_is_member_mean(ctx):
     """Two members with different normalisations: energies are the mean of the
     single-member runs, the spread their difference, and forces (linear in

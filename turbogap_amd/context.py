@@ -287,6 +287,10 @@ class Context:
     def set_profiling(self, on: bool):
         L.check(self._lib.tgx_set_profiling(self.h, int(on)))
 
+    def set_block_streams(self, on: bool):
+        """Run the surrogate path's blocks of distinct central species concurrently (default)."""
+        L.check(self._lib.tgx_set_block_streams(self.h, int(on)), "set_block_streams")
+
     def reset_profile(self):
         L.check(self._lib.tgx_reset_profile(self.h))
 

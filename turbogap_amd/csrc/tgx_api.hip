@@ -42,6 +42,12 @@ extern "C" int tgx_set_profiling(tgx_ctx* ctx, int on) {
   return TGX_OK;
 }
 
+extern "C" int tgx_set_block_streams(tgx_ctx* ctx, int on) {
+  TGX_CTX_CHECK(ctx);
+  ctx->block_streams = on != 0;
+  return TGX_OK;
+}
+
 extern "C" int tgx_reset_profile(tgx_ctx* ctx) {
   TGX_CTX_CHECK(ctx);
   TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
@@ -129,8 +135,10 @@ static void free_model(tgx_model_t* m) {
                   &m->site_atom, &m->site_beg, &m->pr_gidx, &m->pr_nbr, &m->pr_spec, &m->pr_r, &m->pr_xyz, &m->q64,
                   &m->q32, &m->qbf, &m->pnorm, &m->e_site, &m->qss64, &m->qss32, &m->kd64, &m->kdbf, &m->epart,
                   &m->soap_der, &m->fpair, &m->pbasis, &m->low_idx, &m->low_coef, &m->gate_pn, &m->gate_e,
-                  &m->gate_stat, &m->gate_list};
+                  &m->gate_stat, &m->gate_list, &m->sel_scratch, &m->mlp_ws};
   for (dbuf* b : bufs) b->release();
+  if (m->done) (void)hipEventDestroy(m->done);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
 }
 
@@ -153,6 +161,7 @@ extern "C" int tgx_finalize(tgx_ctx* ctx) {
   if (ctx->md_flag_host) (void)hipHostFree(ctx->md_flag_host);
   if (ctx->pin64) (void)hipHostFree(ctx->pin64);
   if (ctx->red_host) (void)hipHostFree(ctx->red_host);
+  if (ctx->ev_blocks) (void)hipEventDestroy(ctx->ev_blocks);
   for (auto& r : ctx->prof) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
@@ -660,6 +669,43 @@ extern "C" int tgx_compute(tgx_ctx* ctx, int path, double* energy, double* viria
   if (path == TGX_PATH_SURROGATE) {
     // every block's network first, then one readback of the gate counters
     const int nm = (int)ctx->models.size();
+    // Blocks of distinct central species touch disjoint centres, pair entries
+    // and atoms' energies: each then runs on a stream of its own, so that one
+    // block's kernels fill the gaps and tails of the other's. Blocks that share
+    // a central species (or take every site) run in turn on the context stream,
+    // and so do all blocks while kernels are being timed (each kernel's events
+    // then time it alone).
+    bool concurrent = nm > 1 && ctx->block_streams && !ctx->profiling;
+    for (int a = 0; a < nm && concurrent; ++a) {
+      const int ca = ctx->models[a]->central_species;
+      if (ca == 0) concurrent = false;
+      for (int b = 0; b < a && concurrent; ++b)
+        if (ctx->models[b]->central_species == ca) concurrent = false;
+    }
+    hipStream_t main_stream = ctx->stream;
+    // on every exit: the context stream back; on an error exit, no block work
+    // left running
+    struct StreamRestore {
+      tgx_ctx* c;
+      hipStream_t s;
+      bool blocks;
+      ~StreamRestore() {
+        c->stream = s;
+        if (blocks)
+          for (auto* m : c->models)
+            if (m->stream) (void)hipStreamSynchronize(m->stream);
+      }
+    } restore{ctx, main_stream, concurrent};
+    if (concurrent) {
+      if (!ctx->ev_blocks) TGX_CHECK_HIP(hipEventCreateWithFlags(&ctx->ev_blocks, hipEventDisableTiming));
+      TGX_CHECK_HIP(hipEventRecord(ctx->ev_blocks, main_stream));
+      for (auto* m : ctx->models) {
+        if (!m->stream) TGX_CHECK_HIP(hipStreamCreate(&m->stream));
+        if (!m->done) TGX_CHECK_HIP(hipEventCreateWithFlags(&m->done, hipEventDisableTiming));
+        TGX_CHECK_HIP(hipStreamWaitEvent(m->stream, ctx->ev_blocks, 0));
+      }
+    }
+    auto block_stream = [&](tgx_model_t* m) { return concurrent ? m->stream : main_stream; };
     if (ctx->gate_host_n < 2 * nm) {
       if (ctx->gate_host) (void)hipHostFree(ctx->gate_host);
       ctx->gate_host = nullptr;
@@ -669,15 +715,17 @@ extern "C" int tgx_compute(tgx_ctx* ctx, int path, double* energy, double* viria
     }
     for (int k = 0; k < nm; ++k) {
       tgx_model_t* m = ctx->models[k];
+      ctx->stream = block_stream(m);
       TGX_TRY(tgx_surrogate_eval(ctx, m));
       ctx->gate_host[2 * k] = ctx->gate_host[2 * k + 1] = 0;
       if (m->n_sites_b > 0 && ctx->gate_members > 1)
         TGX_CHECK_HIP(hipMemcpyAsync(ctx->gate_host + 2 * k, m->gate_stat.p, sizeof(int) * 2, hipMemcpyDeviceToHost,
                                      ctx->stream));
     }
-    TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto* m : ctx->models) TGX_CHECK_HIP(hipStreamSynchronize(block_stream(m)));
     for (int k = 0; k < nm; ++k) {
       tgx_model_t* m = ctx->models[k];
+      ctx->stream = block_stream(m);
       const int ng = ctx->gate_host[2 * k];
       float sp;
       std::memcpy(&sp, &ctx->gate_host[2 * k + 1], sizeof(float));
@@ -686,7 +734,12 @@ extern "C" int tgx_compute(tgx_ctx* ctx, int path, double* energy, double* viria
       ctx->max_spread = std::max(ctx->max_spread, (double)sp);
       TGX_TRY(tgx_surrogate_finish(ctx, m, ng));
       TGX_TRY(tgx_energy_scatter(ctx, m));
+      if (concurrent) TGX_CHECK_HIP(hipEventRecord(m->done, m->stream));
     }
+    ctx->stream = main_stream;
+    if (concurrent)
+      for (auto* m : ctx->models) TGX_CHECK_HIP(hipStreamWaitEvent(main_stream, m->done, 0));
+    restore.blocks = false;  // the context stream now orders everything after the blocks
     ctx->gated_fraction = ctx->gate_sites > 0 ? (double)ctx->gate_gated / (double)ctx->gate_sites : 0.0;
   } else {
     for (auto* m : ctx->models) {

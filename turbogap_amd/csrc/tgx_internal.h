@@ -153,6 +153,10 @@ struct tgx_model_t {
   dbuf gate_stat, gate_list;  // surrogate gate: {count, max spread bits}, gated sites
   dbuf pbasis;      // [n_pairs_b][ldb] FP32 pair basis: A_l Y_lm | R_n | dR_n (FP32 paths)
   dbuf fpair;       // [n_pairs_b][3] pair forces
+  dbuf sel_scratch; // block selection: flags, pair counts and their scans
+  dbuf mlp_ws;      // surrogate work space (activations, member energies)
+  hipStream_t stream = nullptr;  // the block's own stream when blocks run concurrently
+  hipEvent_t done = nullptr;     // recorded on that stream after the block's last launch
 };
 
 struct prof_rec { std::string name; hipEvent_t a, b; };
@@ -209,6 +213,8 @@ struct tgx_ctx {
   double* red_host = nullptr;  // pinned: virial and energy partials of the force gather
   int gate_host_n = 0;
   dbuf e_members, gate_mask;
+  hipEvent_t ev_blocks = nullptr;  // recorded on the context stream before concurrent blocks start
+  bool block_streams = true;       // surrogate blocks of distinct central species run concurrently
   // profiling
   bool profiling = false;
   std::vector<prof_rec> prof;

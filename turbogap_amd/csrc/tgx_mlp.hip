@@ -983,11 +983,11 @@ int tgx_surrogate_eval(tgx_ctx* ctx, tgx_model_t* m) {
   TGX_REQUIRE(m->qbf.bytes >= sizeof(__bf16) * (size_t)Mp * Dp, "surrogate: BF16 descriptors missing");
   TGX_TRY(m->qss32.ensure(sizeof(float) * (size_t)Mp * Dp));
   TGX_TRY(m->e_site.ensure(sizeof(double) * (size_t)Mp));
-  // work space in the context scratch: s1, z2 (FP32 Mp x K H), h1b, a2b, a1b (BF16 Mp x K H), Em (K x Mp)
+  // the block's work space: s1, z2 (FP32 Mp x K H), h1b, a2b, a1b (BF16 Mp x K H), Em (K x Mp)
   const size_t fh = (size_t)Mp * KH;
   const size_t bytes = sizeof(float) * (2 * fh + (size_t)K * Mp) + sizeof(__bf16) * 3 * fh + 8 * 256;
-  TGX_TRY(ctx->e_members.ensure(bytes));
-  char* p = ctx->e_members.as<char>();
+  TGX_TRY(m->mlp_ws.ensure(bytes));
+  char* p = m->mlp_ws.as<char>();
   auto take = [&](size_t n) { char* r = p; p += tgx_round_up((int64_t)n, 256); return r; };
   float* s1 = (float*)take(sizeof(float) * fh);
   float* z2 = (float*)take(sizeof(float) * fh);

@@ -157,10 +157,11 @@ static SpeciesMap species_map(const tgx_model_t* m) {
 int tgx_block_select(tgx_ctx* ctx, tgx_model_t* m) {
   TGX_REQUIRE(ctx->have_list, "no neighbour list: call tgx_build_neighbors_list first");
   const int n = ctx->n_sites;
-  TGX_TRY(ctx->scratch.ensure(sizeof(int) * 2 * (size_t)n + sizeof(int64_t) * 2 * (size_t)(n + 1) + 256));
-  int* flag = ctx->scratch.as<int>();
+  // the block's own scratch: blocks may be selected while another block's kernels still run
+  TGX_TRY(m->sel_scratch.ensure(sizeof(int) * 2 * (size_t)n + sizeof(int64_t) * 2 * (size_t)(n + 1) + 256));
+  int* flag = m->sel_scratch.as<int>();
   int* npairs = flag + n;
-  int64_t* site_idx = reinterpret_cast<int64_t*>(ctx->scratch.as<char>() + tgx_round_up(sizeof(int) * 2 * (size_t)n, 16));
+  int64_t* site_idx = reinterpret_cast<int64_t*>(m->sel_scratch.as<char>() + tgx_round_up(sizeof(int) * 2 * (size_t)n, 16));
   int64_t* pair_off = site_idx + tgx_round_up(n + 1, 2);  // 16-byte aligned for the scan
   SpeciesMap sm = species_map(m);
   TGX_LAUNCH(ctx, "select_count", k_select_count, dim3((n + 255) / 256), dim3(256), 0, n, sm, m->central_species,

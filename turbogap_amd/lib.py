@@ -120,6 +120,7 @@ _SIGS = {
     "tgx_angle_3b_create": [_P, _I, _I, _I, _D, _D, _D, _PD, _D, C.c_char_p, _I, _PD, _PD, _PD, _PI],
     "tgx_pair_energy_and_forces": [_P, _I, _PD, _PD, _PD],
     "tgx_set_profiling": [_P, _I],
+    "tgx_set_block_streams": [_P, _I],
     "tgx_kernel_time": [_P, C.c_char_p, _PI, _PD],
     "tgx_reset_profile": [_P],
     "tgx_synchronize": [_P],
