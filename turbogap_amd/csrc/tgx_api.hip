@@ -722,10 +722,12 @@ extern "C" int tgx_compute(tgx_ctx* ctx, int path, double* energy, double* viria
         TGX_CHECK_HIP(hipMemcpyAsync(ctx->gate_host + 2 * k, m->gate_stat.p, sizeof(int) * 2, hipMemcpyDeviceToHost,
                                      ctx->stream));
     }
-    for (auto* m : ctx->models) TGX_CHECK_HIP(hipStreamSynchronize(block_stream(m)));
+    // each block's second phase as soon as its own gate counters are back, so
+    // that it overlaps the other block's network
     for (int k = 0; k < nm; ++k) {
       tgx_model_t* m = ctx->models[k];
       ctx->stream = block_stream(m);
+      TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
       const int ng = ctx->gate_host[2 * k];
       float sp;
       std::memcpy(&sp, &ctx->gate_host[2 * k + 1], sizeof(float));
