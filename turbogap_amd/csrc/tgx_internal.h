@@ -154,6 +154,7 @@ struct tgx_model_t {
   dbuf pbasis;      // [n_pairs_b][ldb] FP32 pair basis: A_l Y_lm | R_n | dR_n (FP32 paths)
   dbuf fpair;       // [n_pairs_b][3] pair forces
   dbuf sel_scratch; // block selection: flags, pair counts and their scans
+  dbuf scan_tmp;    // block totals of the selection scans
   dbuf mlp_ws;      // surrogate work space (activations, member energies)
   hipStream_t stream = nullptr;  // the block's own stream when blocks run concurrently
   hipEvent_t done = nullptr;     // recorded on that stream after the block's last launch

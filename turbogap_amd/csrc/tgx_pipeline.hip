@@ -167,7 +167,7 @@ int tgx_block_select(tgx_ctx* ctx, tgx_model_t* m) {
   TGX_LAUNCH(ctx, "select_count", k_select_count, dim3((n + 255) / 256), dim3(256), 0, n, sm, m->central_species,
              m->rcut_max, ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(), ctx->species_sc.as<int>(),
              ctx->rjs.as<double>(), ctx->nspec.as<int>(), flag, npairs);
-  TGX_TRY(tgx_scan2_i32(ctx, flag, site_idx, npairs, pair_off, n));
+  TGX_TRY(tgx_scan2_i32(ctx, flag, site_idx, npairs, pair_off, n, m->scan_tmp));
   // both totals in one readback
   if (!ctx->pin64) TGX_CHECK_HIP(hipHostMalloc((void**)&ctx->pin64, 2 * sizeof(int64_t), hipHostMallocDefault));
   TGX_CHECK_HIP(hipMemcpyAsync(ctx->pin64, site_idx + n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));

@@ -6,8 +6,11 @@
 // entries (out[n] is the total). When total is not NULL the total is copied to
 // the host (this synchronises the stream).
 int tgx_scan_i32(tgx_ctx* ctx, const int* in, int64_t* out, int64_t n, int64_t* total);
-// The same for two arrays of n values (one launch when n <= 65,536).
-int tgx_scan2_i32(tgx_ctx* ctx, const int* in0, int64_t* out0, const int* in1, int64_t* out1, int64_t n);
+// The same for two arrays of n values (two launches for both when 16,384 < n <=
+// 65,536; block totals in tmp, which belongs to the caller so that scans on
+// different streams do not share it).
+int tgx_scan2_i32(tgx_ctx* ctx, const int* in0, int64_t* out0, const int* in1, int64_t* out1, int64_t n,
+                  dbuf& tmp);
 
 // Partial sums of n doubles into TGX_SUM_PARTS device values, each over a fixed
 // contiguous slice in a fixed tree (summed on the host in order).

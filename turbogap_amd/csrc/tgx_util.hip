@@ -121,54 +121,42 @@ __device__ int64_t block_exclusive_scan_waves(int64_t v, int64_t* sh, int64_t* t
   return (w > 0 ? sh[NW + w - 1] : 0) + x - v;
 }
 
-// Whole exclusive scan of up to SCAN_WG_MAX values in one workgroup, one
-// workgroup per array (blockIdx.x picks in0/out0 or in1/out1): thread t owns the
-// contiguous chunk [t c, (t+1) c), c <= 64, read as int4 (and written as pairs
-// of int64) when the chunk is whole and both arrays are 16-byte aligned there:
-// once for its sum, once more (from cache) to write its prefixes.
-// out[n] is the total.
-constexpr int SCAN_WG_C = 64;
-constexpr int64_t SCAN_WG_MAX = (int64_t)SCAN_WG_C * SCAN_BLOCK;
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_wg(const int* __restrict__ in0, int64_t* __restrict__ out0,
-                                                       const int* __restrict__ in1, int64_t* __restrict__ out1,
-                                                       int64_t n) {
+// Two-launch exclusive scan of up to SCAN_2L_MAX values, one or two arrays
+// (blockIdx.y picks in0/out0 or in1/out1): k_scan_local2 scans each block of
+// 1024 values and stores the block totals; k_scan_add2 adds to every block the
+// sum of the totals before it (at most 63 of them, summed by its first
+// wavefront), and the last block writes out[n].
+constexpr int64_t SCAN_2L_MAX = 64 * (int64_t)SCAN_BLOCK;
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_local2(const int* __restrict__ in0, int64_t* __restrict__ out0,
+                                                           const int* __restrict__ in1, int64_t* __restrict__ out1,
+                                                           int64_t n, int64_t* __restrict__ sums) {
   __shared__ int64_t sh[2 * SCAN_BLOCK / 64];
-  const int* in = blockIdx.x ? in1 : in0;
-  int64_t* out = blockIdx.x ? out1 : out0;
-  const int c = (int)((n + SCAN_BLOCK - 1) / SCAN_BLOCK);
-  const int64_t b = (int64_t)threadIdx.x * c;
-  const bool whole = (c & 3) == 0 && b + c <= n && ((uintptr_t)(in + b) & 15) == 0 &&
-                     ((uintptr_t)(out + b) & 15) == 0;
-  const int4* in4 = reinterpret_cast<const int4*>(in + b);
-  int64_t s = 0;
-  if (whole) {
-    for (int k = 0; k < c / 4; ++k) {
-      const int4 q = in4[k];
-      s += (int64_t)q.x + q.y + q.z + q.w;
-    }
-  } else {
-    for (int k = 0; k < c; ++k)
-      if (b + k < n) s += in[b + k];
-  }
+  const int* in = blockIdx.y ? in1 : in0;
+  int64_t* out = blockIdx.y ? out1 : out0;
+  const int64_t i = (int64_t)blockIdx.x * SCAN_BLOCK + threadIdx.x;
+  const int64_t v = i < n ? in[i] : 0;
   int64_t tot;
-  int64_t run = block_exclusive_scan_waves(s, sh, &tot);
-  if (whole) {
-    longlong2* o2 = reinterpret_cast<longlong2*>(out + b);
-    for (int k = 0; k < c / 4; ++k) {
-      const int4 q = in4[k];
-      const int64_t r1 = run + q.x, r2 = r1 + q.y, r3 = r2 + q.z;
-      o2[2 * k] = make_longlong2(run, r1);
-      o2[2 * k + 1] = make_longlong2(r2, r3);
-      run = r3 + q.w;
-    }
-  } else {
-    for (int k = 0; k < c; ++k)
-      if (b + k < n) {
-        out[b + k] = run;
-        run += in[b + k];
-      }
+  const int64_t ex = block_exclusive_scan_waves(v, sh, &tot);
+  if (i < n) out[i] = ex;
+  if (threadIdx.x == 0) sums[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_add2(int64_t* __restrict__ out0, int64_t* __restrict__ out1,
+                                                         int64_t n, const int64_t* __restrict__ sums) {
+  __shared__ int64_t off;
+  int64_t* out = blockIdx.y ? out1 : out0;
+  const int64_t* bs = sums + (size_t)blockIdx.y * gridDim.x;
+  if (threadIdx.x < 64) {
+    int64_t v = 0;
+    for (int k = threadIdx.x; k < (int)blockIdx.x; k += 64) v += bs[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (threadIdx.x == 0) off = v;
   }
-  if (threadIdx.x == 0) out[n] = tot;
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * SCAN_BLOCK + threadIdx.x;
+  if (i < n) out[i] += off;
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = off + bs[blockIdx.x];
 }
 
 // Streaming copy, 16 bytes per thread and iteration (the HBM reference).
@@ -216,9 +204,11 @@ int tgx_scan_i32(tgx_ctx* ctx, const int* in, int64_t* out, int64_t n, int64_t* 
   int64_t* sums = ctx->scan_tmp.as<int64_t>();
   if (n > 0 && n <= SCAN_SINGLE_MAX) {
     TGX_LAUNCH(ctx, "scan_single", k_scan_single, dim3(1), dim3(SCAN_BLOCK), 0, in, out, n);
-  } else if (n > 0 && n <= SCAN_WG_MAX) {
-    TGX_LAUNCH(ctx, "scan_wg", k_scan_wg, dim3(1), dim3(SCAN_BLOCK), 0, in, out, (const int*)nullptr,
-               (int64_t*)nullptr, n);
+  } else if (n > 0 && n <= SCAN_2L_MAX) {
+    TGX_LAUNCH(ctx, "scan_local", k_scan_local2, dim3((unsigned)nb, 1), dim3(SCAN_BLOCK), 0, in, out,
+               (const int*)nullptr, (int64_t*)nullptr, n, sums);
+    TGX_LAUNCH(ctx, "scan_add", k_scan_add2, dim3((unsigned)nb, 1), dim3(SCAN_BLOCK), 0, out, (int64_t*)nullptr, n,
+               (const int64_t*)sums);
   } else if (n > 0) {
     TGX_LAUNCH(ctx, "scan_local", k_scan_local, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, in, out, n, sums);
     TGX_LAUNCH(ctx, "scan_sums", k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, sums, nb);
@@ -233,9 +223,16 @@ int tgx_scan_i32(tgx_ctx* ctx, const int* in, int64_t* out, int64_t n, int64_t* 
   return TGX_OK;
 }
 
-int tgx_scan2_i32(tgx_ctx* ctx, const int* in0, int64_t* out0, const int* in1, int64_t* out1, int64_t n) {
-  if (n > SCAN_SINGLE_MAX && n <= SCAN_WG_MAX) {
-    TGX_LAUNCH(ctx, "scan_wg", k_scan_wg, dim3(2), dim3(SCAN_BLOCK), 0, in0, out0, in1, out1, n);
+int tgx_scan2_i32(tgx_ctx* ctx, const int* in0, int64_t* out0, const int* in1, int64_t* out1, int64_t n,
+                  dbuf& tmp) {
+  if (n > SCAN_SINGLE_MAX && n <= SCAN_2L_MAX) {
+    const int64_t nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    TGX_TRY(tmp.ensure(sizeof(int64_t) * 2 * (size_t)nb));
+    int64_t* sums = tmp.as<int64_t>();
+    TGX_LAUNCH(ctx, "scan_local", k_scan_local2, dim3((unsigned)nb, 2), dim3(SCAN_BLOCK), 0, in0, out0, in1, out1, n,
+               sums);
+    TGX_LAUNCH(ctx, "scan_add", k_scan_add2, dim3((unsigned)nb, 2), dim3(SCAN_BLOCK), 0, out0, out1, n,
+               (const int64_t*)sums);
     return TGX_OK;
   }
   TGX_TRY(tgx_scan_i32(ctx, in0, out0, n, nullptr));
