@@ -137,6 +137,7 @@ static void free_model(tgx_model_t* m) {
                   &m->soap_der, &m->fpair, &m->pbasis, &m->low_idx, &m->low_coef, &m->gate_pn, &m->gate_e,
                   &m->gate_stat, &m->gate_list, &m->sel_scratch, &m->mlp_ws, &m->scan_tmp};
   for (dbuf* b : bufs) b->release();
+  if (m->sel_host) (void)hipHostFree(m->sel_host);
   if (m->done) (void)hipEventDestroy(m->done);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
@@ -656,7 +657,7 @@ extern "C" int tgx_get_soap_energy_and_forces(tgx_ctx* ctx, int n_sites, int n_s
 }
 
 // ---- whole step ------------------------------------------------------------------------
-int tgx_surrogate_eval(tgx_ctx* ctx, tgx_model_t* m);
+int tgx_surrogate_eval(tgx_ctx* ctx, tgx_model_t* m, bool selected);
 int tgx_surrogate_finish(tgx_ctx* ctx, tgx_model_t* m, int n_gated);
 
 extern "C" int tgx_compute(tgx_ctx* ctx, int path, double* energy, double* virial) {
@@ -713,10 +714,17 @@ extern "C" int tgx_compute(tgx_ctx* ctx, int path, double* energy, double* viria
       TGX_CHECK_HIP(hipHostMalloc((void**)&ctx->gate_host, sizeof(int) * 2 * nm, hipHostMallocDefault));
       ctx->gate_host_n = 2 * nm;
     }
+    // every block's selection counts in flight before the first wait
+    for (auto* m : ctx->models) {
+      ctx->stream = block_stream(m);
+      TGX_TRY(tgx_block_select_count(ctx, m));
+    }
     for (int k = 0; k < nm; ++k) {
       tgx_model_t* m = ctx->models[k];
       ctx->stream = block_stream(m);
-      TGX_TRY(tgx_surrogate_eval(ctx, m));
+      TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+      TGX_TRY(tgx_block_select_fill(ctx, m));
+      TGX_TRY(tgx_surrogate_eval(ctx, m, true));
       ctx->gate_host[2 * k] = ctx->gate_host[2 * k + 1] = 0;
       if (m->n_sites_b > 0 && ctx->gate_members > 1)
         TGX_CHECK_HIP(hipMemcpyAsync(ctx->gate_host + 2 * k, m->gate_stat.p, sizeof(int) * 2, hipMemcpyDeviceToHost,

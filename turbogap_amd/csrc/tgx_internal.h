@@ -155,6 +155,7 @@ struct tgx_model_t {
   dbuf fpair;       // [n_pairs_b][3] pair forces
   dbuf sel_scratch; // block selection: flags, pair counts and their scans
   dbuf scan_tmp;    // block totals of the selection scans
+  int64_t* sel_host = nullptr;  // pinned: the selection's site and pair totals
   dbuf mlp_ws;      // surrogate work space (activations, member energies)
   hipStream_t stream = nullptr;  // the block's own stream when blocks run concurrently
   hipEvent_t done = nullptr;     // recorded on that stream after the block's last launch
@@ -250,6 +251,11 @@ int tgx_neighbor_angles(tgx_ctx* ctx);
 int tgx_neighbors_refresh(tgx_ctx* ctx);
 int tgx_local_property_finish(tgx_ctx* ctx, tgx_model_t* m, bool with_der, double** d_V, double** d_vder);
 int tgx_block_select(tgx_ctx* ctx, tgx_model_t* m);
+// The two halves of tgx_block_select: counts and scans with the totals copied to
+// pinned memory (no wait), then, after the caller has synchronised the stream,
+// allocation and fill.
+int tgx_block_select_count(tgx_ctx* ctx, tgx_model_t* m);
+int tgx_block_select_fill(tgx_ctx* ctx, tgx_model_t* m);
 int tgx_soap_descriptors(tgx_ctx* ctx, tgx_model_t* m, int path, bool with_der);
 int tgx_gap_contract(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces);
 int tgx_plain_pair_forces(tgx_ctx* ctx, tgx_model_t* m);

@@ -961,7 +961,7 @@ extern "C" int tgx_surrogate_stats(tgx_ctx* ctx, double* gated_fraction, double*
 //   act2  : E_k, a2_k = w3_k silu'(z2_k + b2_k)
 //   back2 : a1_k = -(scale_k/K) (a2_k W2_k) o s1_k           batched over k
 //   back1 : Qss = a1 W1Tcat^T = -(1/K) sum_k scale_k dE_k/dq  one GEMM, depth K H
-int tgx_surrogate_eval(tgx_ctx* ctx, tgx_model_t* m) {
+int tgx_surrogate_eval(tgx_ctx* ctx, tgx_model_t* m, bool selected) {
   int model_id = -1;
   for (int k = 0; k < (int)ctx->models.size(); ++k)
     if (ctx->models[k] == m) model_id = k;
@@ -972,7 +972,7 @@ int tgx_surrogate_eval(tgx_ctx* ctx, tgx_model_t* m) {
   for (int k = 0; k < K; ++k) TGX_REQUIRE(sl.members[k], "surrogate: member %d of model %d missing", k, model_id);
   const int H = sl.members[0]->H;
   for (int k = 1; k < K; ++k) TGX_REQUIRE(sl.members[k]->H == H, "surrogate members differ in width");
-  TGX_TRY(tgx_block_select(ctx, m));
+  if (!selected) TGX_TRY(tgx_block_select(ctx, m));
   const int ns = m->n_sites_b;
   if (ns == 0) return TGX_OK;
   TGX_TRY(tgx_soap_descriptors(ctx, m, TGX_PATH_MIXED, false));

@@ -154,26 +154,48 @@ static SpeciesMap species_map(const tgx_model_t* m) {
   return sm;
 }
 
-int tgx_block_select(tgx_ctx* ctx, tgx_model_t* m) {
+// Block selection in two halves: the count and scans with the two totals copied
+// to pinned host memory (no wait), then, once the caller has synchronised the
+// stream, the allocation and the fill. Several blocks can have their counts in
+// flight before one wait.
+struct SelectView {
+  int* flag;
+  int* npairs;
+  int64_t* site_idx;
+  int64_t* pair_off;
+};
+
+static SelectView select_view(tgx_model_t* m, int n) {
+  SelectView v;
+  v.flag = m->sel_scratch.as<int>();
+  v.npairs = v.flag + n;
+  v.site_idx = reinterpret_cast<int64_t*>(m->sel_scratch.as<char>() + tgx_round_up(sizeof(int) * 2 * (size_t)n, 16));
+  v.pair_off = v.site_idx + tgx_round_up(n + 1, 2);  // 16-byte aligned for the scan
+  return v;
+}
+
+int tgx_block_select_count(tgx_ctx* ctx, tgx_model_t* m) {
   TGX_REQUIRE(ctx->have_list, "no neighbour list: call tgx_build_neighbors_list first");
   const int n = ctx->n_sites;
   // the block's own scratch: blocks may be selected while another block's kernels still run
   TGX_TRY(m->sel_scratch.ensure(sizeof(int) * 2 * (size_t)n + sizeof(int64_t) * 2 * (size_t)(n + 1) + 256));
-  int* flag = m->sel_scratch.as<int>();
-  int* npairs = flag + n;
-  int64_t* site_idx = reinterpret_cast<int64_t*>(m->sel_scratch.as<char>() + tgx_round_up(sizeof(int) * 2 * (size_t)n, 16));
-  int64_t* pair_off = site_idx + tgx_round_up(n + 1, 2);  // 16-byte aligned for the scan
+  const SelectView v = select_view(m, n);
   SpeciesMap sm = species_map(m);
   TGX_LAUNCH(ctx, "select_count", k_select_count, dim3((n + 255) / 256), dim3(256), 0, n, sm, m->central_species,
              m->rcut_max, ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(), ctx->species_sc.as<int>(),
-             ctx->rjs.as<double>(), ctx->nspec.as<int>(), flag, npairs);
-  TGX_TRY(tgx_scan2_i32(ctx, flag, site_idx, npairs, pair_off, n, m->scan_tmp));
-  // both totals in one readback
-  if (!ctx->pin64) TGX_CHECK_HIP(hipHostMalloc((void**)&ctx->pin64, 2 * sizeof(int64_t), hipHostMallocDefault));
-  TGX_CHECK_HIP(hipMemcpyAsync(ctx->pin64, site_idx + n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-  TGX_CHECK_HIP(hipMemcpyAsync(ctx->pin64 + 1, pair_off + n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-  const int64_t ns = ctx->pin64[0], np = ctx->pin64[1];
+             ctx->rjs.as<double>(), ctx->nspec.as<int>(), v.flag, v.npairs);
+  TGX_TRY(tgx_scan2_i32(ctx, v.flag, v.site_idx, v.npairs, v.pair_off, n, m->scan_tmp));
+  if (!m->sel_host) TGX_CHECK_HIP(hipHostMalloc((void**)&m->sel_host, 2 * sizeof(int64_t), hipHostMallocDefault));
+  TGX_CHECK_HIP(hipMemcpyAsync(m->sel_host, v.site_idx + n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+  TGX_CHECK_HIP(hipMemcpyAsync(m->sel_host + 1, v.pair_off + n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+  return TGX_OK;
+}
+
+int tgx_block_select_fill(tgx_ctx* ctx, tgx_model_t* m) {
+  const int n = ctx->n_sites;
+  const SelectView v = select_view(m, n);
+  SpeciesMap sm = species_map(m);
+  const int64_t ns = m->sel_host[0], np = m->sel_host[1];
   m->n_sites_b = (int)ns;
   m->n_pairs_b = np;
   const size_t npz = (size_t)(np > 0 ? np : 1);
@@ -184,13 +206,19 @@ int tgx_block_select(tgx_ctx* ctx, tgx_model_t* m) {
   TGX_TRY(m->pr_spec.ensure(sizeof(int) * npz));
   TGX_TRY(m->pr_r.ensure(sizeof(double) * npz));
   TGX_TRY(m->pr_xyz.ensure(sizeof(double) * 3 * npz));
-  TGX_LAUNCH(ctx, "select_fill", k_select_fill, dim3((n + 3) / 4), dim3(256), 0, n, sm, m->rcut_max, flag, site_idx,
-             pair_off, ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(), ctx->nlist.as<int>(), ctx->rjs.as<double>(),
-             ctx->xyz.as<double>(), ctx->nspec.as<int>(), m->site_atom.as<int>(), m->site_beg.as<int64_t>(),
-             m->pr_gidx.as<int64_t>(), m->pr_nbr.as<int>(), m->pr_spec.as<int>(), m->pr_r.as<double>(),
-             m->pr_xyz.as<double>(), (int*)nullptr);
-  TGX_LAUNCH(ctx, "select_last", k_set_last, dim3(1), dim3(1), 0, m->site_beg.as<int64_t>(), (int)ns, pair_off, n);
+  TGX_LAUNCH(ctx, "select_fill", k_select_fill, dim3((n + 3) / 4), dim3(256), 0, n, sm, m->rcut_max, v.flag,
+             v.site_idx, v.pair_off, ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(), ctx->nlist.as<int>(),
+             ctx->rjs.as<double>(), ctx->xyz.as<double>(), ctx->nspec.as<int>(), m->site_atom.as<int>(),
+             m->site_beg.as<int64_t>(), m->pr_gidx.as<int64_t>(), m->pr_nbr.as<int>(), m->pr_spec.as<int>(),
+             m->pr_r.as<double>(), m->pr_xyz.as<double>(), (int*)nullptr);
+  TGX_LAUNCH(ctx, "select_last", k_set_last, dim3(1), dim3(1), 0, m->site_beg.as<int64_t>(), (int)ns, v.pair_off, n);
   return TGX_OK;
+}
+
+int tgx_block_select(tgx_ctx* ctx, tgx_model_t* m) {
+  TGX_TRY(tgx_block_select_count(ctx, m));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return tgx_block_select_fill(ctx, m);
 }
 
 // Forces and virial from the accumulated pair forces; adds every model's site
