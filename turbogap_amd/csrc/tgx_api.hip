@@ -470,8 +470,8 @@ static int current_model(tgx_ctx* ctx, tgx_model_t** m) {
 }
 
 // One block's energies and pair forces (pair forces accumulated into fglob).
-static int run_block(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces) {
-  TGX_TRY(tgx_block_select(ctx, m));
+// Descriptors, contraction and pair forces of a block whose selection is done.
+static int run_selected_block(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces) {
   const bool plain = path == TGX_PATH_FP64_PLAIN;
   TGX_TRY(tgx_soap_descriptors(ctx, m, path, plain && do_forces));
   TGX_TRY(tgx_gap_contract(ctx, m, path, do_forces));
@@ -480,6 +480,11 @@ static int run_block(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces) {
     else TGX_TRY(tgx_adjoint_pair_forces(ctx, m, path));
   }
   return TGX_OK;
+}
+
+static int run_block(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces) {
+  TGX_TRY(tgx_block_select(ctx, m));
+  return run_selected_block(ctx, m, path, do_forces);
 }
 
 static int begin_step(tgx_ctx* ctx) {
@@ -667,57 +672,59 @@ extern "C" int tgx_compute(tgx_ctx* ctx, int path, double* energy, double* viria
   TGX_TRY(begin_step(ctx));
   ctx->gate_sites = ctx->gate_gated = 0;
   ctx->gated_fraction = ctx->max_spread = 0.0;
+  const int nm = (int)ctx->models.size();
+  // Blocks of distinct central species touch disjoint centres, pair entries
+  // and atoms' energies: on the paths that keep no context-wide scratch (all
+  // but fp64_plain) each then runs on a stream of its own, so that one block's
+  // kernels fill the gaps and tails of the other's. Blocks that share a central
+  // species (or take every site) run in turn on the context stream, and so do
+  // all blocks while kernels are being timed (each kernel's events then time it
+  // alone).
+  bool concurrent = nm > 1 && path != TGX_PATH_FP64_PLAIN && ctx->block_streams && !ctx->profiling;
+  for (int a = 0; a < nm && concurrent; ++a) {
+    const int ca = ctx->models[a]->central_species;
+    if (ca == 0) concurrent = false;
+    for (int b = 0; b < a && concurrent; ++b)
+      if (ctx->models[b]->central_species == ca) concurrent = false;
+  }
+  hipStream_t main_stream = ctx->stream;
+  // on every exit: the context stream back; on an error exit, no block work
+  // left running
+  struct StreamRestore {
+    tgx_ctx* c;
+    hipStream_t s;
+    bool blocks;
+    ~StreamRestore() {
+      c->stream = s;
+      if (blocks)
+        for (auto* m : c->models)
+          if (m->stream) (void)hipStreamSynchronize(m->stream);
+    }
+  } restore{ctx, main_stream, concurrent};
+  if (concurrent) {
+    if (!ctx->ev_blocks) TGX_CHECK_HIP(hipEventCreateWithFlags(&ctx->ev_blocks, hipEventDisableTiming));
+    TGX_CHECK_HIP(hipEventRecord(ctx->ev_blocks, main_stream));
+    for (auto* m : ctx->models) {
+      if (!m->stream) TGX_CHECK_HIP(hipStreamCreate(&m->stream));
+      if (!m->done) TGX_CHECK_HIP(hipEventCreateWithFlags(&m->done, hipEventDisableTiming));
+      TGX_CHECK_HIP(hipStreamWaitEvent(m->stream, ctx->ev_blocks, 0));
+    }
+  }
+  auto block_stream = [&](tgx_model_t* m) { return concurrent ? m->stream : main_stream; };
+  // every block's selection counts in flight before the first wait
+  for (auto* m : ctx->models) {
+    ctx->stream = block_stream(m);
+    TGX_TRY(tgx_block_select_count(ctx, m));
+  }
   if (path == TGX_PATH_SURROGATE) {
-    // every block's network first, then one readback of the gate counters
-    const int nm = (int)ctx->models.size();
-    // Blocks of distinct central species touch disjoint centres, pair entries
-    // and atoms' energies: each then runs on a stream of its own, so that one
-    // block's kernels fill the gaps and tails of the other's. Blocks that share
-    // a central species (or take every site) run in turn on the context stream,
-    // and so do all blocks while kernels are being timed (each kernel's events
-    // then time it alone).
-    bool concurrent = nm > 1 && ctx->block_streams && !ctx->profiling;
-    for (int a = 0; a < nm && concurrent; ++a) {
-      const int ca = ctx->models[a]->central_species;
-      if (ca == 0) concurrent = false;
-      for (int b = 0; b < a && concurrent; ++b)
-        if (ctx->models[b]->central_species == ca) concurrent = false;
-    }
-    hipStream_t main_stream = ctx->stream;
-    // on every exit: the context stream back; on an error exit, no block work
-    // left running
-    struct StreamRestore {
-      tgx_ctx* c;
-      hipStream_t s;
-      bool blocks;
-      ~StreamRestore() {
-        c->stream = s;
-        if (blocks)
-          for (auto* m : c->models)
-            if (m->stream) (void)hipStreamSynchronize(m->stream);
-      }
-    } restore{ctx, main_stream, concurrent};
-    if (concurrent) {
-      if (!ctx->ev_blocks) TGX_CHECK_HIP(hipEventCreateWithFlags(&ctx->ev_blocks, hipEventDisableTiming));
-      TGX_CHECK_HIP(hipEventRecord(ctx->ev_blocks, main_stream));
-      for (auto* m : ctx->models) {
-        if (!m->stream) TGX_CHECK_HIP(hipStreamCreate(&m->stream));
-        if (!m->done) TGX_CHECK_HIP(hipEventCreateWithFlags(&m->done, hipEventDisableTiming));
-        TGX_CHECK_HIP(hipStreamWaitEvent(m->stream, ctx->ev_blocks, 0));
-      }
-    }
-    auto block_stream = [&](tgx_model_t* m) { return concurrent ? m->stream : main_stream; };
+    // every block's network first; each block's gate counters come back in a
+    // pinned copy
     if (ctx->gate_host_n < 2 * nm) {
       if (ctx->gate_host) (void)hipHostFree(ctx->gate_host);
       ctx->gate_host = nullptr;
       ctx->gate_host_n = 0;
       TGX_CHECK_HIP(hipHostMalloc((void**)&ctx->gate_host, sizeof(int) * 2 * nm, hipHostMallocDefault));
       ctx->gate_host_n = 2 * nm;
-    }
-    // every block's selection counts in flight before the first wait
-    for (auto* m : ctx->models) {
-      ctx->stream = block_stream(m);
-      TGX_TRY(tgx_block_select_count(ctx, m));
     }
     for (int k = 0; k < nm; ++k) {
       tgx_model_t* m = ctx->models[k];
@@ -746,17 +753,21 @@ extern "C" int tgx_compute(tgx_ctx* ctx, int path, double* energy, double* viria
       TGX_TRY(tgx_energy_scatter(ctx, m));
       if (concurrent) TGX_CHECK_HIP(hipEventRecord(m->done, m->stream));
     }
-    ctx->stream = main_stream;
-    if (concurrent)
-      for (auto* m : ctx->models) TGX_CHECK_HIP(hipStreamWaitEvent(main_stream, m->done, 0));
-    restore.blocks = false;  // the context stream now orders everything after the blocks
     ctx->gated_fraction = ctx->gate_sites > 0 ? (double)ctx->gate_gated / (double)ctx->gate_sites : 0.0;
   } else {
     for (auto* m : ctx->models) {
-      TGX_TRY(run_block(ctx, m, path, true));
+      ctx->stream = block_stream(m);
+      TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+      TGX_TRY(tgx_block_select_fill(ctx, m));
+      TGX_TRY(run_selected_block(ctx, m, path, true));
       TGX_TRY(tgx_energy_scatter(ctx, m));
+      if (concurrent) TGX_CHECK_HIP(hipEventRecord(m->done, m->stream));
     }
   }
+  ctx->stream = main_stream;
+  if (concurrent)
+    for (auto* m : ctx->models) TGX_CHECK_HIP(hipStreamWaitEvent(main_stream, m->done, 0));
+  restore.blocks = false;  // the context stream now orders everything after the blocks
   TGX_TRY(tgx_gather_forces(ctx, energy, virial));
   return TGX_OK;
 }
