@@ -210,7 +210,8 @@ def production_md(ctx, sysm, args):
     and rebuilt when an atom has moved more than half the buffer (every step
     with a zero buffer), its distances refreshed in between. Every
     args.check_every-th step takes its forces from the FP64 GAP (the full-system
-    check) after the surrogate has been evaluated on the same positions, and
+    check, on the FP64 adjoint path, equal to the plain path to round-off) after
+    the surrogate has been evaluated on the same positions, and
     the surrogate's error there is recorded; gated sites fall back to the GAP on
     every step. Reports the NVE energy drift, the fallback fraction, the force
     RMSE on the checked steps and the whole run's atom-steps/s."""
@@ -232,7 +233,7 @@ def production_md(ctx, sysm, args):
             md.forces("surrogate")
             gated.append(SG.gate_stats(ctx)[0])
             es, Fs = ctx.results()
-            E, ek, T = md.step(1.0, T0=T0, tau=100.0, path="fp64_plain")
+            E, ek, T = md.step(1.0, T0=T0, tau=100.0, path="fp64_adjoint")
             e64, F64 = ctx.results()
             checks.append({"step": k + 1, "force_rmse_meV_A": 1e3 * float(np.sqrt(np.mean((Fs - F64) ** 2))),
                            "energy_mae_meV_atom": 1e3 * float(np.mean(np.abs(es - e64)))})

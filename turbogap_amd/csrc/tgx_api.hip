@@ -156,11 +156,10 @@ extern "C" int tgx_finalize(tgx_ctx* ctx) {
                   &ctx->thetas, &ctx->phis, &ctx->nspec, &ctx->rev, &ctx->cell_of, &ctx->cell_cnt, &ctx->cell_start,
                   &ctx->cell_atoms, &ctx->scan_tmp, &ctx->do_list, &ctx->energies, &ctx->forces, &ctx->fglob,
                   &ctx->virial_part, &ctx->scratch, &ctx->red, &ctx->vel, &ctx->mass, &ctx->forces_prev,
-                  &ctx->pos_prev, &ctx->e_members, &ctx->gate_mask, &ctx->cell_pos, &ctx->md_disp, &ctx->md_flag};
+                  &ctx->pos_prev, &ctx->gate_mask, &ctx->cell_pos, &ctx->md_disp, &ctx->md_flag};
   for (dbuf* b : bufs) b->release();
   if (ctx->gate_host) (void)hipHostFree(ctx->gate_host);
   if (ctx->md_flag_host) (void)hipHostFree(ctx->md_flag_host);
-  if (ctx->pin64) (void)hipHostFree(ctx->pin64);
   if (ctx->red_host) (void)hipHostFree(ctx->red_host);
   if (ctx->ev_blocks) (void)hipEventDestroy(ctx->ev_blocks);
   for (auto& r : ctx->prof) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }

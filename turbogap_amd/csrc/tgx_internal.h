@@ -211,10 +211,9 @@ struct tgx_ctx {
   double gated_fraction = 0.0, max_spread = 0.0;  // over every block of the last tgx_compute
   int64_t gate_sites = 0, gate_gated = 0;
   int* gate_host = nullptr;  // pinned: {count, max spread bits} of every block
-  int64_t* pin64 = nullptr;  // pinned: two sizes read back together
   double* red_host = nullptr;  // pinned: virial and energy partials of the force gather
   int gate_host_n = 0;
-  dbuf e_members, gate_mask;
+  dbuf gate_mask;
   hipEvent_t ev_blocks = nullptr;  // recorded on the context stream before concurrent blocks start
   bool block_streams = true;       // surrogate blocks of distinct central species run concurrently
   // profiling
