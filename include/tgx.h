@@ -295,9 +295,9 @@ int tgx_md_displacement(tgx_ctx* ctx, double neighbors_buffer, int reset, int* r
  * on, the surrogate path's blocks run one after the other. */
 int tgx_set_profiling(tgx_ctx* ctx, int on);
 /* Execution option with no counterpart in the reference: tgx_compute runs blocks
- * of distinct central species concurrently, each on a stream of its own, on every
- * path but TGX_PATH_FP64_PLAIN (on by default); 0 runs them in turn on the
- * context's stream. Results are the same bits either way. */
+ * of distinct central species concurrently, each on a stream of its own (on by
+ * default); 0 runs them in turn on the context's stream. Results are the same
+ * bits either way. */
 int tgx_set_block_streams(tgx_ctx* ctx, int on);
 /* Launch count and total milliseconds of kernels whose name contains `name`. */
 int tgx_kernel_time(tgx_ctx* ctx, const char* name, int* count, double* total_ms);

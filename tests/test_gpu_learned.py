@@ -217,7 +217,7 @@ def test_surrogate_block_streams_match_serial(ctx):
     """The two blocks (central species A and B) on streams of their own give
     bit-identical energies, forces and virial to the same step with the blocks
     in turn, also with the ensemble gate sending sites to the FP64 fallback,
-    and on the mixed and FP64 adjoint paths."""
+    and on the mixed and both FP64 paths."""
     _, models = H.subset(500, 1024)
     sysm = S.binary_system(8192, seed=2001)
     ctx.set_system(sysm.positions, sysm.species, sysm.box)
@@ -244,7 +244,7 @@ def test_surrogate_block_streams_match_serial(ctx):
         assert E1 == E2 and np.array_equal(W1, W2) and np.array_equal(e1, e2) and np.array_equal(F1, F2)
         assert g1 == g2
     assert 0.0 < g1[0] < 1.0  # the second threshold gates part of the sites
-    for path in ("mixed", "fp64_adjoint"):  # the other paths that run blocks concurrently
+    for path in ("mixed", "fp64_adjoint", "fp64_plain"):  # the other paths
         out = []
         for on in (True, False):
             ctx.set_block_streams(on)

@@ -135,7 +135,7 @@ static void free_model(tgx_model_t* m) {
                   &m->site_atom, &m->site_beg, &m->pr_gidx, &m->pr_nbr, &m->pr_spec, &m->pr_r, &m->pr_xyz, &m->q64,
                   &m->q32, &m->qbf, &m->pnorm, &m->e_site, &m->qss64, &m->qss32, &m->kd64, &m->kdbf, &m->epart,
                   &m->soap_der, &m->fpair, &m->pbasis, &m->low_idx, &m->low_coef, &m->gate_pn, &m->gate_e,
-                  &m->gate_stat, &m->gate_list, &m->sel_scratch, &m->mlp_ws, &m->scan_tmp};
+                  &m->gate_stat, &m->gate_list, &m->sel_scratch, &m->mlp_ws, &m->scan_tmp, &m->pair_site};
   for (dbuf* b : bufs) b->release();
   if (m->sel_host) (void)hipHostFree(m->sel_host);
   if (m->done) (void)hipEventDestroy(m->done);
@@ -673,13 +673,12 @@ extern "C" int tgx_compute(tgx_ctx* ctx, int path, double* energy, double* viria
   ctx->gated_fraction = ctx->max_spread = 0.0;
   const int nm = (int)ctx->models.size();
   // Blocks of distinct central species touch disjoint centres, pair entries
-  // and atoms' energies: on the paths that keep no context-wide scratch (all
-  // but fp64_plain) each then runs on a stream of its own, so that one block's
-  // kernels fill the gaps and tails of the other's. Blocks that share a central
-  // species (or take every site) run in turn on the context stream, and so do
-  // all blocks while kernels are being timed (each kernel's events then time it
-  // alone).
-  bool concurrent = nm > 1 && path != TGX_PATH_FP64_PLAIN && ctx->block_streams && !ctx->profiling;
+  // and atoms' energies, and each keeps its scratch in its own buffers: each
+  // then runs on a stream of its own, so that one block's kernels fill the gaps
+  // and tails of the other's. Blocks that share a central species (or take
+  // every site) run in turn on the context stream, and so do all blocks while
+  // kernels are being timed (each kernel's events then time it alone).
+  bool concurrent = nm > 1 && ctx->block_streams && !ctx->profiling;
   for (int a = 0; a < nm && concurrent; ++a) {
     const int ca = ctx->models[a]->central_species;
     if (ca == 0) concurrent = false;

@@ -155,6 +155,7 @@ struct tgx_model_t {
   dbuf fpair;       // [n_pairs_b][3] pair forces
   dbuf sel_scratch; // block selection: flags, pair counts and their scans
   dbuf scan_tmp;    // block totals of the selection scans
+  dbuf pair_site;   // int[n_pairs_b]: centre of every kept pair (plain force pass)
   int64_t* sel_host = nullptr;  // pinned: the selection's site and pair totals
   dbuf mlp_ws;      // surrogate work space (activations, member energies)
   hipStream_t stream = nullptr;  // the block's own stream when blocks run concurrently

@@ -1332,8 +1332,8 @@ int tgx_plain_pair_forces(tgx_ctx* ctx, tgx_model_t* m) {
   const int ns = m->n_sites_b;
   const int64_t np = m->n_pairs_b;
   if (ns == 0 || np == 0) return TGX_OK;
-  TGX_TRY(ctx->scratch.ensure(sizeof(int) * (size_t)np));
-  int* pair_site = ctx->scratch.as<int>();
+  TGX_TRY(m->pair_site.ensure(sizeof(int) * (size_t)np));
+  int* pair_site = m->pair_site.as<int>();
   TGX_LAUNCH(ctx, "pair_site", k_pair_site, dim3(ns), dim3(64), 0, ns, m->site_beg.as<int64_t>(), pair_site);
   TGX_LAUNCH(ctx, "plain_force", k_plain_force, dim3((unsigned)((np + 3) / 4)), dim3(256), 0, ns,
              m->site_beg.as<int64_t>(), m->D, m->Dpad, m->qss64.as<double>(), m->soap_der.as<double>(),
