@@ -241,11 +241,15 @@ __device__ __forceinline__ void harmonics_fixed(const float* ynorm, float ux, fl
 
 constexpr int PB = 64;  // pairs (threads) per k_pair_basis workgroup
 
+// One pair per thread. Rows leave through LDS in two passes so that the
+// staging buffer holds only the wider part (harmonics and padding, roff
+// columns) at a time: first the damped harmonics, then R and dR. The smaller
+// buffer lets more workgroups share a CU.
 __global__ __launch_bounds__(PB) void k_pair_basis(SoapParams P, F32Args a, int64_t n_pairs) {
   extern __shared__ __align__(16) float smf[];
   Shared sh;
   load_shared(P, a, smf, sh);
-  const int LS = a.ldb + 1;  // odd row pitch: the per-thread row writes hit distinct banks
+  const int LS = a.roff + 1;  // odd row pitch: the per-thread row writes hit distinct banks
   float* st = smf + shared_floats(P.NLM, P.n_species);
   const int t = threadIdx.x;
   float* row = st + t * LS;
@@ -260,35 +264,43 @@ __global__ __launch_bounds__(PB) void k_pair_basis(SoapParams P, F32Args a, int6
     // the self pair (r = 0) and neighbours at or beyond their species' cutoff carry nothing
     if (s >= 0 && !(rd > 0.0 && rd < P.sp[s].rc)) s = -1;
   }
-  // zero what this row does not overwrite: the radial slots, and the harmonics of an empty row
-  for (int i = s >= 0 ? a.roff : 0; i < a.ldb; ++i) row[i] = 0.f;
-  if (p < n_pairs) {
-    if (s >= 0) {
-      const SpecLite sp = sh.sp[s];
-      const float r = (float)rd, inv = 1.f / r;
-      const float ux = (float)a.pr_xyz[3 * p] * inv, uy = (float)a.pr_xyz[3 * p + 1] * inv,
-                  uz = (float)a.pr_xyz[3 * p + 2] * inv;
-      float R[TGX_MAX_NB], dR[TGX_MAX_NB];
-      radial_tab<true>(a, sp, s, r, R, dR);
-#pragma unroll
-      for (int n = 0; n < TGX_MAX_NB; ++n)
-        if (n < sp.nb) {
-          row[a.roff + sp.off + n] = R[n];
-          row[a.roff + NMAXF + sp.off + n] = dR[n];
-        }
-      float A[TGX_MAX_L + 1], dA[TGX_MAX_L + 1];
-      damping(sp, r, A, dA);
-      if (P.L == 8) harmonics_fixed<8>(sh.ynorm, ux, uy, uz, A, row);
-      else harmonics_all(P.L, sh.ynorm, ux, uy, uz, A, row);
-      for (int i = P.NLM; i < a.roff; ++i) row[i] = 0.f;  // padding between W and R
-    }
-  }
-  __syncthreads();
   const int64_t left = n_pairs - p0;
   const int nrows = left < PB ? (int)left : PB;
   float* dst = a.basis + p0 * a.ldb;
+  float R[TGX_MAX_NB], dR[TGX_MAX_NB];
+  SpecLite sp{};
+  // pass 1: harmonics and the padding up to roff
+  if (s >= 0) {
+    sp = sh.sp[s];
+    const float r = (float)rd, inv = 1.f / r;
+    const float ux = (float)a.pr_xyz[3 * p] * inv, uy = (float)a.pr_xyz[3 * p + 1] * inv,
+                uz = (float)a.pr_xyz[3 * p + 2] * inv;
+    radial_tab<true>(a, sp, s, r, R, dR);
+    float A[TGX_MAX_L + 1], dA[TGX_MAX_L + 1];
+    damping(sp, r, A, dA);
+    if (P.L == 8) harmonics_fixed<8>(sh.ynorm, ux, uy, uz, A, row);
+    else harmonics_all(P.L, sh.ynorm, ux, uy, uz, A, row);
+    for (int i = P.NLM; i < a.roff; ++i) row[i] = 0.f;
+  } else {
+    for (int i = 0; i < a.roff; ++i) row[i] = 0.f;
+  }
+  __syncthreads();
   for (int rr = 0; rr < nrows; ++rr)
-    for (int cc = t; cc < a.ldb; cc += PB) dst[(size_t)rr * a.ldb + cc] = st[rr * LS + cc];
+    for (int cc = t; cc < a.roff; cc += PB) dst[(size_t)rr * a.ldb + cc] = st[rr * LS + cc];
+  __syncthreads();  // pass 1 read out: the rows are reused
+  // pass 2: R and dR (2 NMAXF columns from roff), the block's species at its offset
+  for (int i = 0; i < 2 * NMAXF; ++i) row[i] = 0.f;
+  if (s >= 0) {
+#pragma unroll
+    for (int n = 0; n < TGX_MAX_NB; ++n)
+      if (n < sp.nb) {
+        row[sp.off + n] = R[n];
+        row[NMAXF + sp.off + n] = dR[n];
+      }
+  }
+  __syncthreads();
+  for (int rr = 0; rr < nrows; ++rr)
+    for (int cc = t; cc < 2 * NMAXF; cc += PB) dst[(size_t)rr * a.ldb + a.roff + cc] = st[rr * LS + cc];
 }
 
 // c(n, lm) of a site in registers, lanes over lm (lm = lane and lane + 64).
@@ -772,7 +784,7 @@ int tgx_desc_f32(tgx_ctx* ctx, tgx_model_t* m) {
   TGX_TRY(m->pbasis.ensure(sizeof(float) * (size_t)(np > 0 ? np : 1) * a.ldb));
   a.basis = m->pbasis.as<float>();
   if (np > 0) {
-    const size_t lds = sizeof(float) * ((size_t)shared_floats(m->P.NLM, m->P.n_species) + (size_t)PB * (a.ldb + 1));
+    const size_t lds = sizeof(float) * ((size_t)shared_floats(m->P.NLM, m->P.n_species) + (size_t)PB * (a.roff + 1));
     TGX_REQUIRE(lds <= 64 * 1024, "pair basis needs %zu bytes of LDS", lds);
     TGX_LAUNCH(ctx, "soap_pair_basis", k_pair_basis, dim3((unsigned)((np + PB - 1) / PB)), dim3(PB), lds, m->P, a, np);
   }
