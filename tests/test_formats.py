@@ -59,6 +59,29 @@ def test_gap_file_defaults_and_fallbacks(tmp_path):
     assert h.nf == [4.0, 4.0] and h.central_weight == [1.0, 1.0] and h.zeta == 2.0
 
 
+def test_angle_3b_block_defaults(tmp_path):
+    """An angle_3b block without delta, sigma or kernel_type takes the defaults of
+    src/types.f90:154-170 (1, 1 1 1, "exp"); Z_center / Z1 / Z2 name the species;
+    a descriptor file without three values per sparse point is rejected."""
+    (tmp_path / "a.dat").write_text("0.5 1.0\n-0.25 0.5\n")
+    (tmp_path / "d.dat").write_text("\n".join(["4.0", "1.0", "3.0", "5.0", "0.5", "2.5"]) + "\n")
+    (tmp_path / "bad.dat").write_text("4.0\n1.0\n3.0\n5.0\n")
+    p = tmp_path / "m.gap"
+    p.write_text("gap_beg angle_3b\nZ_center = C\nZ1 = C\nZ2 = H\nrcut = 3.5\n"
+                 "desc_sparse = 'd.dat'\nalphas_sparse = 'a.dat'\ngap_end\n")
+    ((kind, kw),) = G.load_pair_blocks(str(p), base_dir=str(tmp_path))
+    assert kind == "angle_3b"
+    assert (kw["species_center"], kw["species1"], kw["species2"]) == (6, 6, 1)
+    assert kw["delta"] == 1.0 and kw["sigma"] == [1.0, 1.0, 1.0] and kw["kernel_type"] == "exp"
+    assert kw["buffer"] == 0.5 and kw["e0"] == 0.0 and kw["rcut"] == 3.5
+    np.testing.assert_array_equal(kw["Qs"], [[4.0, 1.0, 3.0], [5.0, 0.5, 2.5]])
+    np.testing.assert_array_equal(kw["alphas"], [0.5, -0.25])
+    np.testing.assert_array_equal(kw["cutoff"], [1.0, 0.5])
+    p.write_text(p.read_text().replace("'d.dat'", "'bad.dat'"))
+    with pytest.raises(ValueError):
+        G.load_pair_blocks(str(p), base_dir=str(tmp_path))
+
+
 def test_gap_file_round_trip(tmp_path):
     models = []
     lines = []
