@@ -22,7 +22,8 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int BM = 128, BN = 128;
 
-enum { EPI_STORE = TGX_EPI_STORE_F32, EPI_GAP = 1, EPI_SILU = TGX_EPI_SILU, EPI_SCALE_AUX = TGX_EPI_SCALE_AUX };
+enum { EPI_STORE = TGX_EPI_STORE_F32, EPI_GAP = 1, EPI_SILU = TGX_EPI_SILU, EPI_SCALE_AUX = TGX_EPI_SCALE_AUX,
+       EPI_ACT2 = TGX_EPI_ACT2 };
 
 struct EpiArgs {
   double pref;           // -zeta delta^2 (EPI_GAP) / output scale (EPI_STORE, EPI_SCALE_AUX)
@@ -39,6 +40,8 @@ struct EpiArgs {
   int ldaux;
   int64_t sA, sB, sOut, sAux, sBias;  // per-blockIdx.z element strides
   const float* zscale;   // per-z scale (EPI_SCALE_AUX), or null for pref
+  const float* w3;       // [N] output weights (EPI_ACT2)
+  int64_t sW3, sEpart;   // per-blockIdx.z strides of w3 and epart
 };
 
 __device__ __forceinline__ void tile_of(int& tm, int& tn) {
@@ -104,6 +107,41 @@ __device__ __forceinline__ void epilogue(const VecT (&acc)[TI][TJ], int m0, int 
             out[row * e.ldo + col] = (OutT)((float)acc[ti][tj][r] * e.aux[row * e.ldaux + col] * (float)e.pref);
           }
         }
+    }
+    return;
+  }
+  if (EPI == EPI_ACT2) {
+    // per 16-row slab: a2 = w3 silu'(v) out, and the row sums of w3 silu(v)
+    // over the tile's columns as partial energies (as the GAP epilogue's)
+#pragma unroll
+    for (int ti = 0; ti < TI; ++ti) {
+      double er[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int tj = 0; tj < TJ; ++tj) {
+        const int col = cbase + tj * 16;
+        const float bc = e.bias[col], wc = e.w3[col];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const size_t row = (size_t)(rbase + ti * 16 + tile_row<F64>(lane, r));
+          const float v = (float)acc[ti][tj][r] + bc;
+          out[row * e.ldo + col] = (OutT)(wc * tgx_silu1(v));
+          er[r] += (double)(wc * tgx_silu(v));
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double v = er[r];
+        v += __shfl_xor(v, 1);
+        v += __shfl_xor(v, 2);
+        v += __shfl_xor(v, 4);
+        v += __shfl_xor(v, 8);
+        if ((lane & 15) == 0) sE[wn * BMT + wm * WM + ti * 16 + tile_row<F64>(lane, r)] = v;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < BMT) {
+      const int row = m0 + threadIdx.x;
+      e.epart[(size_t)row * e.n_tiles + tn] = sE[threadIdx.x] + sE[BMT + threadIdx.x];
     }
     return;
   }
@@ -250,6 +288,8 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int K, const __bf16* __restri
     e.out = reinterpret_cast<OutT*>(e.out) + z * e.sOut;
     if (e.aux) e.aux += z * e.sAux;
     if (e.bias) e.bias += z * e.sBias;
+    if (e.w3) e.w3 += z * e.sW3;
+    if (e.epart) e.epart += z * e.sEpart;
     if (e.zscale) e.pref = e.zscale[z];
   }
   const __bf16* Ag = A + (size_t)(m0 + lr) * lda + lc;
@@ -339,6 +379,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16_w(int K, const __bf16* __r
     e.out = reinterpret_cast<OutT*>(e.out) + z * e.sOut;
     if (e.aux) e.aux += z * e.sAux;
     if (e.bias) e.bias += z * e.sBias;
+    if (e.w3) e.w3 += z * e.sW3;
+    if (e.epart) e.epart += z * e.sEpart;
     if (e.zscale) e.pref = e.zscale[z];
   }
   const __bf16* Ag = A + (size_t)(m0 + lr) * lda + lc;
@@ -523,6 +565,8 @@ int tgx_gemm_bf16(tgx_ctx* ctx, const char* name, const tgx_gemm_desc& d) {
               "gemm_bf16 %s: unpadded shape %d x %d x %d (batch %d)", name, d.M, d.N, d.K, d.batch);
   TGX_REQUIRE(d.epi != TGX_EPI_SILU || (d.bias && d.aux), "gemm_bf16 %s: SiLU epilogue needs bias and aux", name);
   TGX_REQUIRE(d.epi != TGX_EPI_SCALE_AUX || d.aux, "gemm_bf16 %s: scaled epilogue needs aux", name);
+  TGX_REQUIRE(d.epi != TGX_EPI_ACT2 || (d.bias && d.w3 && d.epart), "gemm_bf16 %s: ACT2 needs bias, w3, epart",
+              name);
   EpiArgs e{};
   e.pref = d.scale;
   e.out = d.C;
@@ -536,6 +580,11 @@ int tgx_gemm_bf16(tgx_ctx* ctx, const char* name, const tgx_gemm_desc& d) {
   e.sAux = d.sAux;
   e.sBias = d.sBias;
   e.zscale = d.zscale;
+  e.w3 = d.w3;
+  e.sW3 = d.sW3;
+  e.epart = d.epart;
+  e.sEpart = d.sEpart;
+  e.n_tiles = d.N / BN;
   switch (d.epi) {
     case TGX_EPI_STORE_F32:
       return launch_bf16<EPI_STORE, float, true>(ctx, name, d.M, d.N, d.K, d.batch, d.A, d.lda, d.B, d.ldb, e);
@@ -543,6 +592,8 @@ int tgx_gemm_bf16(tgx_ctx* ctx, const char* name, const tgx_gemm_desc& d) {
       return launch_bf16<EPI_SILU, __bf16, true>(ctx, name, d.M, d.N, d.K, d.batch, d.A, d.lda, d.B, d.ldb, e);
     case TGX_EPI_SCALE_AUX:
       return launch_bf16<EPI_SCALE_AUX, __bf16, true>(ctx, name, d.M, d.N, d.K, d.batch, d.A, d.lda, d.B, d.ldb, e);
+    case TGX_EPI_ACT2:
+      return launch_bf16<EPI_ACT2, __bf16, true>(ctx, name, d.M, d.N, d.K, d.batch, d.A, d.lda, d.B, d.ldb, e);
     default:
       TGX_REQUIRE(false, "gemm_bf16 %s: unknown epilogue %d", name, d.epi);
   }

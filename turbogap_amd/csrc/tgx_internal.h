@@ -290,9 +290,12 @@ __device__ __forceinline__ float tgx_silu2(float z) {
 //   TGX_EPI_STORE_F32 : C_z (FP32)  = scale * acc
 //   TGX_EPI_SILU      : v = acc + bias_z[col]; C_z (BF16) = silu(v); aux_z (FP32) = silu'(v)
 //   TGX_EPI_SCALE_AUX : C_z (BF16)  = acc * aux_z * (zscale ? zscale[z] : scale)
+//   TGX_EPI_ACT2      : v = acc + bias_z[col]; C_z (BF16) = w3_z[col] silu'(v);
+//                       epart_z[row][col / 128] (FP64) = sum over the 128-column
+//                       tile of w3_z[col] silu(v)
 // M, N multiples of 128, K of 64, leading dimensions and strides in elements,
 // multiples of 8 for A and B.
-enum { TGX_EPI_STORE_F32 = 0, TGX_EPI_SILU = 2, TGX_EPI_SCALE_AUX = 3 };
+enum { TGX_EPI_STORE_F32 = 0, TGX_EPI_SILU = 2, TGX_EPI_SCALE_AUX = 3, TGX_EPI_ACT2 = 4 };
 struct tgx_gemm_desc {
   int epi = TGX_EPI_STORE_F32;
   int M = 0, N = 0, K = 0, batch = 1;
@@ -312,5 +315,9 @@ struct tgx_gemm_desc {
   int ldaux = 0;
   int64_t sAux = 0;
   const float* zscale = nullptr;
+  const float* w3 = nullptr;  // TGX_EPI_ACT2
+  int64_t sW3 = 0;
+  double* epart = nullptr;    // TGX_EPI_ACT2: [batch][M][N / 128]
+  int64_t sEpart = 0;
 };
 int tgx_gemm_bf16(tgx_ctx* ctx, const char* name, const tgx_gemm_desc& d);

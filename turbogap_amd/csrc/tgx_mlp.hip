@@ -288,19 +288,25 @@ __global__ void k_append(int n_take, int stride, int offset, int Dp, const doubl
 }
 
 // Surrogate output: member energies e_k = shift_k + scale_k E_k (FP64),
-// e_site = mean_k e_k, spread = max_k e_k - min_k e_k. Em is [K][ldm]. With more
+// e_site = mean_k e_k, spread = max_k e_k - min_k e_k, E_k = b3_k plus the nt
+// partials of member k's row in epart ([K][ldm][nt]). With more
 // than one member, sites whose spread exceeds the threshold are appended to
 // gated[] (stat[0] counts them; the order of the list does not matter, every
 // listed site is re-evaluated on its own) and stat[1] keeps the largest spread
 // (as the bits of a non-negative float, which order like the values).
-__global__ void k_surrogate_out(int n, int K, const float* __restrict__ Em, int ldm, const double* __restrict__ shift,
+__global__ void k_surrogate_out(int n, int K, const double* __restrict__ epart, int ldm, int nt,
+                                const float* __restrict__ b3, const double* __restrict__ shift,
                                 const double* __restrict__ scale, double* __restrict__ e_site, float threshold,
                                 int* __restrict__ stat, int* __restrict__ gated) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0.0, lo = 1e300, hi = -1e300;
   for (int k = 0; k < K; ++k) {
-    const double v = shift[k] + scale[k] * (double)Em[(size_t)k * ldm + i];
+    // member k's network output: its second layer's tile partials plus b3
+    const double* pk = epart + ((size_t)k * ldm + i) * nt;
+    double ek = (double)b3[k];
+    for (int t = 0; t < nt; ++t) ek += pk[t];
+    const double v = shift[k] + scale[k] * ek;
     s += v;
     lo = v < lo ? v : lo;
     hi = v > hi ? v : hi;
@@ -311,30 +317,6 @@ __global__ void k_surrogate_out(int n, int K, const float* __restrict__ Em, int 
     atomicMax(&stat[1], __float_as_int(sp));
     if (sp > threshold) gated[atomicAdd(&stat[0], 1)] = i;
   }
-}
-
-// Second layer of every ensemble member, one wavefront per (row, member):
-// v = z2 + b2_k, E_k = w3_k . silu(v) + b3_k, a2 = w3_k silu'(v) (BF16). The
-// members' hidden units sit side by side in rows of K H.
-__global__ __launch_bounds__(256) void k_l2_ens(int rows, int H, int K, const float* __restrict__ z2,
-                                                const float* __restrict__ b2, const float* __restrict__ w3,
-                                                const float* __restrict__ b3, float* __restrict__ Em,
-                                                __bf16* __restrict__ a2b) {
-  const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int k = blockIdx.y;
-  if (row >= rows) return;
-  const size_t base = (size_t)row * K * H + (size_t)k * H;
-  const float* bk = b2 + (size_t)k * H;
-  const float* wk = w3 + (size_t)k * H;
-  float acc = 0.f;
-  for (int j = lane; j < H; j += 64) {
-    const float v = z2[base + j] + bk[j];
-    acc += wk[j] * silu(v);
-    a2b[base + j] = (__bf16)(wk[j] * silu1(v));
-  }
-  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-  if (lane == 0) Em[(size_t)k * rows + row] = acc + b3[k];
 }
 
 // Gated sites take the FP64 GAP energy and gradient.
@@ -957,8 +939,9 @@ extern "C" int tgx_surrogate_stats(tgx_ctx* ctx, double* gated_fraction, double*
 // gradients, and the device-side gate marks. tgx_surrogate_finish completes the
 // block once the caller has read the number of gated sites.
 //   L1    : [h1 | s1] = silu / silu' (q W1cat^T + b1)       one GEMM, N = K H
-//   L2    : z2_k = h1_k W2_k^T                              batched over k
-//   act2  : E_k, a2_k = w3_k silu'(z2_k + b2_k)
+//   L2    : z2_k = h1_k W2_k^T, fused with its activation:   batched over k
+//           a2_k = w3_k silu'(z2_k + b2_k) and the partial sums of
+//           w3_k silu(z2_k + b2_k), from which the output kernel forms E_k
 //   back2 : a1_k = -(scale_k/K) (a2_k W2_k) o s1_k           batched over k
 //   back1 : Qss = a1 W1Tcat^T = -(1/K) sum_k scale_k dE_k/dq  one GEMM, depth K H
 int tgx_surrogate_eval(tgx_ctx* ctx, tgx_model_t* m, bool selected) {
@@ -983,18 +966,19 @@ int tgx_surrogate_eval(tgx_ctx* ctx, tgx_model_t* m, bool selected) {
   TGX_REQUIRE(m->qbf.bytes >= sizeof(__bf16) * (size_t)Mp * Dp, "surrogate: BF16 descriptors missing");
   TGX_TRY(m->qss32.ensure(sizeof(float) * (size_t)Mp * Dp));
   TGX_TRY(m->e_site.ensure(sizeof(double) * (size_t)Mp));
-  // the block's work space: s1, z2 (FP32 Mp x K H), h1b, a2b, a1b (BF16 Mp x K H), Em (K x Mp)
+  // the block's work space: s1 (FP32 Mp x K H), h1b, a2b, a1b (BF16 Mp x K H), the second
+  // layer's energy partials (FP64 K x Mp x H/128)
   const size_t fh = (size_t)Mp * KH;
-  const size_t bytes = sizeof(float) * (2 * fh + (size_t)K * Mp) + sizeof(__bf16) * 3 * fh + 8 * 256;
+  const int nt2 = H / 128;
+  const size_t bytes = sizeof(float) * fh + sizeof(__bf16) * 3 * fh + sizeof(double) * (size_t)K * Mp * nt2 + 8 * 256;
   TGX_TRY(m->mlp_ws.ensure(bytes));
   char* p = m->mlp_ws.as<char>();
   auto take = [&](size_t n) { char* r = p; p += tgx_round_up((int64_t)n, 256); return r; };
   float* s1 = (float*)take(sizeof(float) * fh);
-  float* z2 = (float*)take(sizeof(float) * fh);
   __bf16* h1b = (__bf16*)take(sizeof(__bf16) * fh);
   __bf16* a2b = (__bf16*)take(sizeof(__bf16) * fh);
   __bf16* a1b = (__bf16*)take(sizeof(__bf16) * fh);
-  float* Em = (float*)take(sizeof(float) * K * Mp);
+  double* e2part = (double*)take(sizeof(double) * (size_t)K * Mp * nt2);
 
   tgx_gemm_desc l1;
   l1.epi = TGX_EPI_SILU;
@@ -1006,15 +990,18 @@ int tgx_surrogate_eval(tgx_ctx* ctx, tgx_model_t* m, bool selected) {
   l1.aux = s1; l1.ldaux = KH;
   TGX_TRY(tgx_gemm_bf16(ctx, "mlp_l1", l1));
 
+  // second layer with its activation fused: a2 = w3 silu'(z2 + b2) and the
+  // per-tile partial sums of w3 silu(z2 + b2)
   tgx_gemm_desc l2;
+  l2.epi = TGX_EPI_ACT2;
   l2.M = Mp; l2.N = H; l2.K = H; l2.batch = K;
   l2.A = h1b; l2.lda = KH; l2.sA = H;
   l2.B = pk.W2s.as<__bf16>(); l2.ldb = H; l2.sB = (int64_t)H * H;
-  l2.C = z2; l2.ldc = KH; l2.sC = H;
+  l2.C = a2b; l2.ldc = KH; l2.sC = H;
+  l2.bias = pk.b2.as<float>(); l2.sBias = H;
+  l2.w3 = pk.w3.as<float>(); l2.sW3 = H;
+  l2.epart = e2part; l2.sEpart = (int64_t)Mp * nt2;
   TGX_TRY(tgx_gemm_bf16(ctx, "mlp_l2", l2));
-
-  TGX_LAUNCH(ctx, "mlp_act2", k_l2_ens, dim3((Mp + 3) / 4, K), dim3(256), 0, Mp, H, K, z2, pk.b2.as<float>(),
-             pk.w3.as<float>(), pk.b3.as<float>(), Em, a2b);
 
   tgx_gemm_desc b2;
   b2.epi = TGX_EPI_SCALE_AUX;
@@ -1036,7 +1023,8 @@ int tgx_surrogate_eval(tgx_ctx* ctx, tgx_model_t* m, bool selected) {
   TGX_TRY(m->gate_stat.ensure(sizeof(int) * 2));
   TGX_TRY(m->gate_list.ensure(sizeof(int) * (size_t)Mp));
   TGX_TRY(tgx_zero(ctx, m->gate_stat.p, sizeof(int) * 2));
-  TGX_LAUNCH(ctx, "mlp_out", k_surrogate_out, g1(ns), dim3(256), 0, ns, K, Em, Mp, pk.shift.as<double>(),
+  TGX_LAUNCH(ctx, "mlp_out", k_surrogate_out, g1(ns), dim3(256), 0, ns, K, (const double*)e2part, Mp, nt2,
+             pk.b3.as<float>(), pk.shift.as<double>(),
              pk.scale.as<double>(), m->e_site.as<double>(), (float)ctx->gate_threshold, m->gate_stat.as<int>(),
              m->gate_list.as<int>());
   return TGX_OK;
