@@ -289,6 +289,13 @@ int tgx_wrap_pbc(tgx_ctx* ctx);
  * displacement with buffer / 2, which lets two atoms approach each other by more
  * than the buffer before a rebuild.) reset = 1 after a rebuild zeroes the totals. */
 int tgx_md_displacement(tgx_ctx* ctx, double neighbors_buffer, int reset, int* rebuild);
+/* No counterpart in the reference (a driver shortcut): after the step's forces,
+ * tgx_velocity_verlet, tgx_kinetic, tgx_berendsen to T0 (T0 <= 0: none),
+ * tgx_wrap_pbc and, with neighbors_buffer > 0, the tgx_md_displacement test in
+ * one call with one synchronisation; E_kin and T are those after the Verlet
+ * update, as tgx_kinetic returns them. */
+int tgx_md_advance(tgx_ctx* ctx, double dt, int first_step, double T0, double tau, double neighbors_buffer,
+                   double* e_kin, double* temperature, int* rebuild);
 
 /* ---- profiling ----------------------------------------------------------- */
 /* Record HIP events around every kernel launch on the context's stream. While

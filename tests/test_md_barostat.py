@@ -14,7 +14,8 @@ CPU (oracle pins):
     component.
 GPU (parity): the library's velocity Verlet with fixed components against the
 oracle fed the library's own forces, four steps, to 1e-12; its pressure and
-barostat (iso and diag) against the oracle on the device state; a short NPT run.
+barostat (iso and diag) against the oracle on the device state; the one-call
+MD advance against the separate calls, bit for bit; a short NPT run.
 """
 import numpy as np
 import pytest
@@ -141,6 +142,29 @@ def test_gpu_pressure_and_barostat_match_oracle(ctx, sym):
                                rtol=1e-14, atol=0)
     if sym == "iso":
         assert np.linalg.det(nb.reshape(3, 3)) > V  # P above P0: expansion
+
+
+@pytest.mark.gpu
+def test_gpu_md_advance_matches_separate_calls(ctx):
+    """tgx_md_advance (one synchronisation per step) against velocity Verlet,
+    kinetic energy, Berendsen, wrap and the displacement test called one by one:
+    the same positions, velocities, temperatures and rebuild decisions, bit for
+    bit, over NVT steps with a neighbour buffer."""
+    sysm, mass, v0 = _md_system(ctx, T=900.0, seed=8)
+    out = []
+    for fused in (True, False):
+        ctx.set_positions(sysm.positions)  # same start; the MD object resets velocities and forces_prev
+        md = MD(ctx, mass, v0, 3.7, "fp64_adjoint", neighbors_buffer=0.2, fused=fused)
+        Ts, builds = [], []
+        for _ in range(8):
+            E, ek, T = md.step(1.0, T0=600.0, tau=20.0)
+            Ts.append((E, ek, T))
+            builds.append(md.n_builds)
+        x, v = md.state()
+        out.append((Ts, builds, x, v))
+    (T1, b1, x1, v1), (T2, b2, x2, v2) = out
+    assert T1 == T2 and b1 == b2 and b1[-1] > 1
+    assert np.array_equal(x1, x2) and np.array_equal(v1, v2)
 
 
 @pytest.mark.gpu

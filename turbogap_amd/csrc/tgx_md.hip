@@ -299,3 +299,67 @@ extern "C" int tgx_md_displacement(tgx_ctx* ctx, double neighbors_buffer, int re
   *rebuild = *ctx->md_flag_host;
   return TGX_OK;
 }
+
+// One MD step after the forces, with one synchronisation: velocity Verlet (with
+// fixed components), the per-atom kinetic energies and their fixed-order
+// partial sums, and the neighbour-buffer displacement test, read back
+// together; then, on the host's temperature, the Berendsen rescale and the
+// wrap. The same kernels and sums as tgx_velocity_verlet, tgx_kinetic,
+// tgx_berendsen, tgx_wrap_pbc and tgx_md_displacement called in that order
+// (the displacement is taken before the wrap; by minimum image it is the
+// same).
+extern "C" int tgx_md_advance(tgx_ctx* ctx, double dt, int first_step, double T0, double tau,
+                              double neighbors_buffer, double* e_kin, double* temperature, int* rebuild) {
+  MD_CTX(ctx);
+  TGX_REQUIRE(ctx->vel.p && ctx->mass.p && ctx->forces.p, "tgx_md_advance: state or forces missing");
+  TGX_REQUIRE(!(T0 > 0.0) || tau > 0.0, "tgx_md_advance: tau <= 0");
+  const double* b = ctx->box;
+  TGX_REQUIRE(b[1] == 0 && b[2] == 0 && b[3] == 0 && b[5] == 0 && b[6] == 0 && b[7] == 0,
+              "tgx_md_advance: orthorhombic boxes only");
+  const int n = ctx->n_sites;
+  TGX_LAUNCH(ctx, "md_verlet", k_vv, dim3((n + 255) / 256), dim3(256), 0, n, dt, ctx->dt_prev, first_step ? 1 : 0,
+             ctx->forces.as<double>(), ctx->forces_prev.as<double>(), ctx->mass.as<double>(), ctx->vel.as<double>(),
+             ctx->pos.as<double>(), ctx->pos_prev.as<double>(),
+             ctx->md_fixed ? ctx->md_fix.as<unsigned char>() : (const unsigned char*)nullptr);
+  ctx->dt_prev = dt;
+  // kinetic energy partials and the displacement flag in one block of device memory
+  TGX_TRY(ctx->scratch.ensure(sizeof(double) * (4 * (size_t)n + TGX_SUM_PARTS + 1)));
+  double* ke = ctx->scratch.as<double>();
+  double* mv = ke + n;
+  double* part = ke + 4 * (size_t)n;
+  int* flag = reinterpret_cast<int*>(part + TGX_SUM_PARTS);
+  TGX_LAUNCH(ctx, "md_ke", k_ke_terms, dim3((n + 255) / 256), dim3(256), 0, n, ctx->mass.as<double>(),
+             ctx->vel.as<double>(), ke, mv);
+  TGX_TRY(tgx_device_sum_launch(ctx, ke, n, part));
+  TGX_TRY(tgx_zero(ctx, flag, sizeof(double)));
+  const bool buffered = neighbors_buffer > 0.0;
+  if (buffered) {
+    TGX_TRY(ctx->md_disp.ensure(sizeof(double) * 3 * (size_t)n));
+    const double lim = 0.5 * neighbors_buffer;
+    TGX_LAUNCH(ctx, "md_disp", k_disp, dim3((n + 255) / 256), dim3(256), 0, n, b[0], b[4], b[8], ctx->pos.as<double>(),
+               ctx->pos_prev.as<double>(), ctx->md_disp.as<double>(), lim * lim, flag);
+  }
+  if (!ctx->red_host) {
+    const size_t nred = 9 * 512 + (size_t)TGX_SUM_PARTS + 1;
+    TGX_CHECK_HIP(hipHostMalloc((void**)&ctx->red_host, sizeof(double) * nred, hipHostMallocDefault));
+  }
+  TGX_CHECK_HIP(hipMemcpyAsync(ctx->red_host, part, sizeof(double) * (TGX_SUM_PARTS + 1), hipMemcpyDeviceToHost,
+                               ctx->stream));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  double ek = 0.0;
+  for (int i = 0; i < TGX_SUM_PARTS; ++i) ek += ctx->red_host[i];
+  int fl = 0;
+  std::memcpy(&fl, ctx->red_host + TGX_SUM_PARTS, sizeof(int));
+  const double kB = 8.6173303e-5;
+  const double T = n > 1 ? 2.0 / 3.0 / (double)(n - 1) / kB * ek : 0.0;
+  if (T0 > 0.0 && T > 0.0) {
+    const double f = std::sqrt(1.0 + dt / tau * (T0 / T - 1.0));
+    const int n3 = 3 * n;
+    TGX_LAUNCH(ctx, "md_berendsen", k_scale, dim3((n3 + 255) / 256), dim3(256), 0, n3, f, ctx->vel.as<double>());
+  }
+  TGX_LAUNCH(ctx, "md_wrap", k_wrap, dim3((n + 255) / 256), dim3(256), 0, n, b[0], b[4], b[8], ctx->pos.as<double>());
+  if (e_kin) *e_kin = ek;
+  if (temperature) *temperature = T;
+  if (rebuild) *rebuild = buffered ? fl : 0;
+  return TGX_OK;
+}

@@ -33,8 +33,9 @@ class MD:
     up to date when a barostat rescales it."""
 
     def __init__(self, ctx, masses: np.ndarray, velocities: np.ndarray, rcut: float, path: str = "fp64_plain",
-                 neighbors_buffer: float = 0.0, fix_atom=None, box=None):
+                 neighbors_buffer: float = 0.0, fix_atom=None, box=None, fused: bool = True):
         self.ctx, self.rcut, self.path = ctx, rcut, path
+        self.fused = fused  # steps without a barostat through tgx_md_advance (one synchronisation)
         self.buffer = float(neighbors_buffer)
         self.masses = np.ascontiguousarray(masses, dtype=np.float64)
         v = np.ascontiguousarray(velocities, dtype=np.float64)
@@ -93,6 +94,17 @@ class MD:
         Berendsen thermostat, wrap (src/turbogap_md.f90:280-440)."""
         lib = L.load()
         self.forces(path)
+        if self.fused and P0 is None and (T0 is None or T0 > 0.0):
+            ekc, Tc, rb = C.c_double(), C.c_double(), C.c_int()
+            L.check(lib.tgx_md_advance(self.ctx.h, float(dt), int(self.step_count == 0),
+                                       float(T0) if T0 is not None else 0.0, float(tau), self.buffer, C.byref(ekc),
+                                       C.byref(Tc), C.byref(rb)), "md_advance")
+            ek, T = ekc.value, Tc.value
+            self.pressure, self.pressure_tensor = self.pressure_of(T)
+            if self.buffer > 0.0:
+                self.rebuild = bool(rb.value)
+            self.step_count += 1
+            return self.energy, ek, T
         L.check(lib.tgx_velocity_verlet(self.ctx.h, float(dt), int(self.step_count == 0)), "velocity_verlet")
         ek, T = self.kinetic()
         self.pressure, self.pressure_tensor = self.pressure_of(T)
