@@ -25,6 +25,7 @@
  *   tgx_md_set_fixed                    src/md.f90:226-283          velocity_verlet's fix_atom argument
  *   tgx_md_pressure                     src/turbogap_md.f90:287-295 instant_pressure, instant_pressure_tensor
  *   tgx_berendsen_barostat              src/md.f90:384-417, src/turbogap_md.f90:413-426 berendsen_barostat on box and positions
+ *   tgx_box_scaling                     src/md.f90:421-458          box_scaling (scale_box, box_scaling_factor)
  *   tgx_distance_2b_create              src/read_files.f90:722-742,5238-5305  a distance_2b block and its sparse set
  *   tgx_core_pot_create                 src/read_files.f90:5392-5510 a core_pot block, its table and spline
  *   tgx_angle_3b_create                 src/read_files.f90:743-770,5305-5384  an angle_3b block and its sparse set
@@ -289,6 +290,12 @@ int tgx_wrap_pbc(tgx_ctx* ctx);
  * displacement with buffer / 2, which lets two atoms approach each other by more
  * than the buffer before a rebuild.) reset = 1 after a rebuild zeroes the totals. */
 int tgx_md_displacement(tgx_ctx* ctx, double neighbors_buffer, int reset, int* rebuild);
+/* box_scaling: positions += (gamma - I) positions / n_steps and the box vectors set to
+ * f . (a0, b0, c0), f = I + (gamma - I)(i_step + 1)/n_steps, where (a0, b0, c0) is the
+ * box when i_step was 0. gamma is the reference's box_scaling_factor(3, 3) row-major
+ * (gamma[3 r + c] = box_scaling_factor(r + 1, c + 1)); the new box goes to box_out
+ * (9, may be NULL). */
+int tgx_box_scaling(tgx_ctx* ctx, const double* gamma, int i_step, int n_steps, double* box_out);
 /* No counterpart in the reference (a driver shortcut): after the step's forces,
  * tgx_velocity_verlet, tgx_kinetic, tgx_berendsen to T0 (T0 <= 0: none),
  * tgx_wrap_pbc and, with neighbors_buffer > 0, the tgx_md_displacement test in

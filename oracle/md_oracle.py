@@ -12,6 +12,9 @@ device MD; the product path never imports it.
     P the mean diagonal pressure ("iso") or per component ("diag").
   * pressure follows src/turbogap_md.f90:287-295 (bar; eVperA3tobar =
     1602176.6208 as in src/turbogap.f90:95).
+  * box_scaling follows src/md.f90:421-458 (indices fixed at 1): positions +=
+    (gamma - I) positions / n_steps, box vectors f . box0 vectors with
+    f = I + (gamma - I)(i_step + 1)/n_steps.
 Arrays are (n, 3) here; the reference's are (3, n).
 """
 from __future__ import annotations
@@ -57,3 +60,14 @@ def pressure(virial, temperature, n_sites, volume):
     p = (kin + np.trace(W) / 3.0) / volume * EV_PER_A3_TO_BAR
     pt = W / volume * EV_PER_A3_TO_BAR + np.eye(3) * kin / volume * EV_PER_A3_TO_BAR
     return p, pt
+
+
+def box_scaling(positions, box0, gamma, i_step, n_steps):
+    """(positions, box) after step i_step; box0 rows are the vectors a0, b0, c0."""
+    x = np.asarray(positions, dtype=np.float64)
+    G = np.asarray(gamma, dtype=np.float64).reshape(3, 3)
+    I = np.eye(3)
+    x = x + (x @ (G - I).T) / float(n_steps)
+    f = I + (G - I) * float(i_step + 1) / float(n_steps)
+    box = (np.asarray(box0, dtype=np.float64) @ f.T)
+    return x, box

@@ -180,3 +180,74 @@ def test_gpu_npt_steps(ctx):
         assert np.isfinite(E) and np.isfinite(md.pressure)
         assert (new - vol) * md.pressure > 0  # grows under positive pressure, shrinks under negative
         vol = new
+
+
+def test_box_scaling_oracle():
+    """box_scaling (src/md.f90:421-458): one step with a single off-diagonal
+    entry moves only the x component of positions, by gamma_xy y / n_steps, and
+    the box vectors by f . (a0, b0, c0); after n_steps the box is gamma . box0
+    exactly and the positions (1 + (g - 1)/n)^n x0 for gamma = g I; the input's
+    one, three or nine numbers fill the matrix as the reference reads them."""
+    from turbogap_amd.md import box_scaling_factor
+    rng = np.random.default_rng(7)
+    x0 = rng.uniform(0, 10, (6, 3))
+    box0 = np.array([[10.0, 0.5, 0.0], [0.2, 11.0, 0.3], [0.0, -0.4, 12.0]])
+    G = np.eye(3)
+    G[0, 1] = 0.06
+    n = 4
+    x1, b1 = MDO.box_scaling(x0, box0, G, 0, n)
+    assert np.array_equal(x1[:, 1:], x0[:, 1:])
+    np.testing.assert_allclose(x1[:, 0], x0[:, 0] + 0.06 * x0[:, 1] / n, rtol=1e-15, atol=1e-15)
+    np.testing.assert_allclose(b1[:, 0], box0[:, 0] + 0.06 * box0[:, 1] / n, rtol=1e-15)
+    assert np.array_equal(b1[:, 1:], box0[:, 1:])
+    Gm = np.array([[1.03, 0.01, -0.02], [0.005, 0.98, 0.0], [0.01, 0.02, 1.05]])
+    x = x0
+    for i in range(n):
+        x, b = MDO.box_scaling(x, box0, Gm, i, n)
+    np.testing.assert_allclose(b, box0 @ Gm.T, rtol=1e-14, atol=1e-14)
+    x = x0
+    for i in range(n):
+        x, b = MDO.box_scaling(x, box0, 1.02 * np.eye(3), i, n)
+    np.testing.assert_allclose(x, x0 * (1 + 0.02 / n) ** n, rtol=1e-14)
+    np.testing.assert_allclose(b, 1.02 * box0, rtol=1e-14)
+    assert np.array_equal(box_scaling_factor([1.1]), 1.1 * np.eye(3))
+    assert np.array_equal(box_scaling_factor([1.0, 2.0, 3.0]), np.diag([1.0, 2.0, 3.0]))
+    nine = box_scaling_factor(np.arange(9.0))
+    assert nine[0, 1] == 1.0 and nine[1, 0] == 3.0
+    with pytest.raises(ValueError):
+        box_scaling_factor([1.0, 2.0])
+
+
+@pytest.mark.gpu
+def test_gpu_box_scaling_matches_oracle(ctx):
+    """The library's box_scaling against the oracle on the device positions,
+    three steps of a non-symmetric gamma (no dynamics between them), to 1e-13;
+    a call at a later step before any step 0 is an error."""
+    sysm, mass, v0 = _md_system(ctx)
+    md = MD(ctx, mass, v0, 3.7, "fp64_adjoint")
+    lib = L.load()
+    G = np.array([[1.03, 0.01, -0.02], [0.005, 0.98, 0.0], [0.01, 0.02, 1.05]])
+    g = np.ascontiguousarray(G.reshape(9))
+    nb = np.empty(9)
+    assert lib.tgx_box_scaling(ctx.h, L.dptr(g), 1, 3, L.dptr(nb)) != 0
+    x = sysm.positions.copy()
+    for i in range(3):
+        L.check(lib.tgx_box_scaling(ctx.h, L.dptr(g), i, 3, L.dptr(nb)), "box_scaling")
+        x, bo = MDO.box_scaling(x, sysm.box, G, i, 3)
+        xd, _ = md.state()
+        np.testing.assert_allclose(xd, x, rtol=0, atol=1e-13)
+        np.testing.assert_allclose(nb.reshape(3, 3), bo, rtol=1e-15, atol=1e-15)
+    np.testing.assert_allclose(nb.reshape(3, 3), sysm.box @ G.T, rtol=1e-14, atol=1e-14)
+
+
+@pytest.mark.gpu
+def test_gpu_md_with_box_scaling(ctx):
+    """An MD run that strains the box by 3 % over its steps: the box ends at
+    gamma . box0, the energies stay finite and the neighbour list follows."""
+    sysm, mass, v0 = _md_system(ctx, T=300.0)
+    md = MD(ctx, mass, v0, 3.7, "fp64_adjoint", neighbors_buffer=0.3, box=sysm.box)
+    G = np.diag([1.03, 1.03, 1.03])
+    for _ in range(5):
+        E, ek, T = md.step(1.0, T0=300.0, tau=100.0, scale_box=(G, 5))
+        assert np.isfinite(E) and np.isfinite(T)
+    np.testing.assert_allclose(md.box, sysm.box @ G.T, rtol=1e-14)

@@ -204,6 +204,7 @@ extern "C" int tgx_set_system(tgx_ctx* ctx, int n_sites, int n_sites_sc, const d
                                ctx->stream));
   TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   ctx->have_list = false;
+  ctx->md_box0_set = false;
   return TGX_OK;
 }
 

@@ -204,6 +204,8 @@ struct tgx_ctx {
   dbuf md_disp, md_flag;      // displacement since the last list build, rebuild flag
   dbuf md_fix;                 // unsigned char[3 n]: fixed Cartesian components
   bool md_fixed = false;
+  double md_box0[9] = {0};     // box at step 0 of a box-scaling run
+  bool md_box0_set = false;
   int* md_flag_host = nullptr;  // pinned
   double dt_prev = 0.0;
   // surrogate gate

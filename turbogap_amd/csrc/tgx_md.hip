@@ -32,6 +32,17 @@ __global__ void k_vv(int n, double dt, double dt_prev, int first, const double* 
   }
 }
 
+// x += M x for every atom (box_scaling's linear map, M = (gamma - I) / n_steps).
+__global__ void k_linmap(int n, double m00, double m01, double m02, double m10, double m11, double m12, double m20,
+                         double m21, double m22, double* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = x[3 * (size_t)i], b = x[3 * (size_t)i + 1], c = x[3 * (size_t)i + 2];
+  x[3 * (size_t)i] = a + (m00 * a + m01 * b + m02 * c);
+  x[3 * (size_t)i + 1] = b + (m10 * a + m11 * b + m12 * c);
+  x[3 * (size_t)i + 2] = c + (m20 * a + m21 * b + m22 * c);
+}
+
 // x_c *= f_c for every atom (the barostat's scaling, per Cartesian component).
 __global__ void k_scale3(int n, double f0, double f1, double f2, double* __restrict__ x) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -361,5 +372,35 @@ extern "C" int tgx_md_advance(tgx_ctx* ctx, double dt, int first_step, double T0
   if (e_kin) *e_kin = ek;
   if (temperature) *temperature = T;
   if (rebuild) *rebuild = buffered ? fl : 0;
+  return TGX_OK;
+}
+
+extern "C" int tgx_box_scaling(tgx_ctx* ctx, const double* gamma, int i_step, int n_steps, double* box_out) {
+  MD_CTX(ctx);
+  TGX_REQUIRE(gamma && n_steps > 0 && i_step >= 0, "tgx_box_scaling: null gamma or bad step counts");
+  if (i_step == 0) {
+    for (int k = 0; k < 9; ++k) ctx->md_box0[k] = ctx->box[k];
+    ctx->md_box0_set = true;
+  }
+  TGX_REQUIRE(ctx->md_box0_set, "tgx_box_scaling: step 0 not taken (the starting box is saved there)");
+  double M[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) M[3 * r + c] = (gamma[3 * r + c] - (r == c ? 1.0 : 0.0)) / (double)n_steps;
+  const int n = ctx->n_sites;
+  TGX_LAUNCH(ctx, "md_box_scaling", k_linmap, dim3((n + 255) / 256), dim3(256), 0, n, M[0], M[1], M[2], M[3], M[4],
+             M[5], M[6], M[7], M[8], ctx->pos.as<double>());
+  // box vectors a, b, c (rows of box) = f . (a0, b0, c0), f = I + (gamma - I)(i_step + 1)/n_steps
+  const double t = (double)(i_step + 1) / (double)n_steps;
+  for (int v = 0; v < 3; ++v)
+    for (int r = 0; r < 3; ++r) {
+      double s = 0.0;
+      for (int c = 0; c < 3; ++c) {
+        const double f = (r == c ? 1.0 : 0.0) + (gamma[3 * r + c] - (r == c ? 1.0 : 0.0)) * t;
+        s += f * ctx->md_box0[3 * v + c];
+      }
+      ctx->box[3 * v + r] = s;
+    }
+  if (box_out)
+    for (int k = 0; k < 9; ++k) box_out[k] = ctx->box[k];
   return TGX_OK;
 }

@@ -27,6 +27,20 @@ def maxwell_velocities(masses: np.ndarray, T: float, seed: int) -> np.ndarray:
     return v - cm
 
 
+def box_scaling_factor(values) -> np.ndarray:
+    """The 3x3 box_scaling_factor from the input's one, three or nine numbers
+    (src/read_files.f90:1727-1760): one number on the diagonal, three on the
+    diagonal in order, nine row by row."""
+    v = np.asarray(values, dtype=np.float64).reshape(-1)
+    if v.size == 1:
+        return np.eye(3) * v[0]
+    if v.size == 3:
+        return np.diag(v)
+    if v.size == 9:
+        return v.reshape(3, 3).copy()
+    raise ValueError(f"box_scaling_factor takes 1, 3 or 9 numbers, got {v.size}")
+
+
 class MD:
     """fix_atom (n, 3) booleans hold Cartesian components in place
     (velocity_verlet's fix_atom). box (3, 3) rows a, b, c: the system's box, kept
@@ -88,13 +102,14 @@ class MD:
         return p.value, pt.reshape(3, 3)
 
     def step(self, dt: float, T0: float = None, tau: float = 100.0, path=None, P0: float = None,
-             tau_p: float = 1000.0, gamma_p: float = 1.0, barostat_sym: str = "iso"):
+             tau_p: float = 1000.0, gamma_p: float = 1.0, barostat_sym: str = "iso", scale_box=None):
         """One step: forces at x(t), velocity Verlet, E_kin, T and pressure, then
-        (P0 given) the Berendsen barostat on box and positions, (T0 given) the
+        (scale_box = (gamma (3, 3), n_steps) given) box_scaling at this step's index
+        or (P0 given) the Berendsen barostat on box and positions, (T0 given) the
         Berendsen thermostat, wrap (src/turbogap_md.f90:280-440)."""
         lib = L.load()
         self.forces(path)
-        if self.fused and P0 is None and (T0 is None or T0 > 0.0):
+        if self.fused and P0 is None and scale_box is None and (T0 is None or T0 > 0.0):
             ekc, Tc, rb = C.c_double(), C.c_double(), C.c_int()
             L.check(lib.tgx_md_advance(self.ctx.h, float(dt), int(self.step_count == 0),
                                        float(T0) if T0 is not None else 0.0, float(tau), self.buffer, C.byref(ekc),
@@ -108,7 +123,14 @@ class MD:
         L.check(lib.tgx_velocity_verlet(self.ctx.h, float(dt), int(self.step_count == 0)), "velocity_verlet")
         ek, T = self.kinetic()
         self.pressure, self.pressure_tensor = self.pressure_of(T)
-        if P0 is not None:
+        if scale_box is not None:
+            g, n_steps = scale_box
+            gm = np.ascontiguousarray(np.asarray(g, dtype=np.float64).reshape(9))
+            nb = np.empty(9)
+            L.check(lib.tgx_box_scaling(self.ctx.h, L.dptr(gm), int(self.step_count), int(n_steps), L.dptr(nb)),
+                    "box_scaling")
+            self.box = nb.reshape(3, 3)
+        elif P0 is not None:
             nb = np.empty(9)
             pt = np.ascontiguousarray(self.pressure_tensor).reshape(9)
             L.check(lib.tgx_berendsen_barostat(self.ctx.h, float(P0), L.dptr(pt), barostat_sym.encode(), float(tau_p),
