@@ -26,6 +26,8 @@
  *   tgx_md_pressure                     src/turbogap_md.f90:287-295 instant_pressure, instant_pressure_tensor
  *   tgx_berendsen_barostat              src/md.f90:384-417, src/turbogap_md.f90:413-426 berendsen_barostat on box and positions
  *   tgx_box_scaling                     src/md.f90:421-458          box_scaling (scale_box, box_scaling_factor)
+ *   tgx_bussi_thermostat                src/turbogap_md.f90:438-449, src/third_party/bussi_thermostat/resamplekin.f90:50-84
+ *                                       thermostat = "bussi" (resamplekin on the device E_kin)
  *   tgx_distance_2b_create              src/read_files.f90:722-742,5238-5305  a distance_2b block and its sparse set
  *   tgx_core_pot_create                 src/read_files.f90:5392-5510 a core_pot block, its table and spline
  *   tgx_angle_3b_create                 src/read_files.f90:743-770,5305-5384  an angle_3b block and its sparse set
@@ -296,6 +298,14 @@ int tgx_md_displacement(tgx_ctx* ctx, double neighbors_buffer, int reset, int* r
  * (gamma[3 r + c] = box_scaling_factor(r + 1, c + 1)); the new box goes to box_out
  * (9, may be NULL). */
 int tgx_box_scaling(tgx_ctx* ctx, const double* gamma, int i_step, int n_steps, double* box_out);
+/* thermostat = "bussi": velocities scaled by sqrt(K'/K), K the device E_kin and K'
+ * the Bussi-Donadio-Parrinello resample of it (resamplekin) towards T0 with
+ * ndeg = 3 N - 3, relaxation time tau and step dt. The caller supplies the random
+ * draws: gauss ~ N(0, 1) and chi2 ~ chi-squared with ndeg - 1 degrees of freedom
+ * (resamplekin's gasdev() and sumnoises(ndeg - 1)). K = 0 zeroes the velocities.
+ * K' goes to e_kin_out (may be NULL). */
+int tgx_bussi_thermostat(tgx_ctx* ctx, double T0, double tau, double dt, double gauss, double chi2,
+                         double* e_kin_out);
 /* No counterpart in the reference (a driver shortcut): after the step's forces,
  * tgx_velocity_verlet, tgx_kinetic, tgx_berendsen to T0 (T0 <= 0: none),
  * tgx_wrap_pbc and, with neighbors_buffer > 0, the tgx_md_displacement test in

@@ -15,6 +15,12 @@ device MD; the product path never imports it.
   * box_scaling follows src/md.f90:421-458 (indices fixed at 1): positions +=
     (gamma - I) positions / n_steps, box vectors f . box0 vectors with
     f = I + (gamma - I)(i_step + 1)/n_steps.
+  * bussi_resample follows resamplekin (src/third_party/bussi_thermostat/
+    resamplekin.f90:50-84; Bussi, Donadio and Parrinello, J. Chem. Phys. 126,
+    014101 (2007), eq. A7) with its two random draws as arguments: the Gaussian
+    of gasdev() and the chi-squared (ndeg - 1) of sumnoises(ndeg - 1).
+  * target_temperature follows get_target_temp (src/md.f90:1094-1128), the
+    t_beg -> t_end ramp with its t_hold plateaus.
 Arrays are (n, 3) here; the reference's are (3, n).
 """
 from __future__ import annotations
@@ -71,3 +77,27 @@ def box_scaling(positions, box0, gamma, i_step, n_steps):
     f = I + (G - I) * float(i_step + 1) / float(n_steps)
     box = (np.asarray(box0, dtype=np.float64) @ f.T)
     return x, box
+
+
+def bussi_resample(kk, T0, ndeg, tau, dt, gauss, chi2):
+    """The resampled kinetic energy; the operations in the reference's order."""
+    sigma = float(np.float32(ndeg)) * KB * T0 / 2.0
+    taut = tau / dt
+    c = np.exp(-1.0 / taut) if taut > 0.1 else 0.0
+    return kk + (1.0 - c) * (sigma * (chi2 + gauss ** 2) / ndeg - kk) + 2.0 * gauss * np.sqrt(
+        kk * sigma / ndeg * (1.0 - c) * c)
+
+
+def target_temperature(t_beg, t_end, md_istep, md_nsteps, t_hold=()):
+    """t_hold: flat (T_hold, n_start, n_end) triples; the temperature holds at the
+    ramp's value at n_start until n_end, then ramps to t_end by the last step."""
+    s = float(md_istep + 1)
+    T = t_beg + (t_end - t_beg) * s / float(md_nsteps)
+    h = np.asarray(t_hold, dtype=np.float64).reshape(-1, 3)
+    for _, n_start, n_end in h:
+        if n_start < s <= n_end:
+            T = t_beg + (t_end - t_beg) * (n_start / float(md_nsteps))
+        elif s >= n_end:
+            th = t_beg + (t_end - t_beg) * (n_start / float(md_nsteps))
+            T = th + (t_end - th) * (s - n_end) / (float(md_nsteps) - n_end)
+    return T

@@ -375,6 +375,41 @@ extern "C" int tgx_md_advance(tgx_ctx* ctx, double dt, int first_step, double T0
   return TGX_OK;
 }
 
+// Bussi-Donadio-Parrinello stochastic velocity rescaling (J. Chem. Phys. 126,
+// 014101 (2007), eq. A7) as the reference evaluates it: sigma = ndeg kB T0 / 2,
+// c = exp(-dt/tau) (0 when tau/dt <= 0.1), K' = K + (1 - c)(sigma (chi2 + g^2)/ndeg
+// - K) + 2 g sqrt(K sigma/ndeg (1 - c) c). The Gaussian g and the chi-squared draw
+// with ndeg - 1 degrees of freedom come from the caller.
+static double bussi_resample(double kk, double T0, int ndeg, double tau, double dt, double g, double chi2) {
+  const double kB = 8.6173303e-5;
+  const double sigma = (double)(float)ndeg * kB * T0 / 2.0;
+  const double taut = tau / dt;
+  const double c = taut > 0.1 ? std::exp(-1.0 / taut) : 0.0;
+  return kk + (1.0 - c) * (sigma * (chi2 + g * g) / (double)ndeg - kk) +
+         2.0 * g * std::sqrt(kk * sigma / (double)ndeg * (1.0 - c) * c);
+}
+
+extern "C" int tgx_bussi_thermostat(tgx_ctx* ctx, double T0, double tau, double dt, double gauss, double chi2,
+                                    double* e_kin_out) {
+  MD_CTX(ctx);
+  TGX_REQUIRE(ctx->vel.p && dt > 0.0 && ctx->n_sites > 1 && chi2 >= 0.0,
+              "tgx_bussi_thermostat: no velocities, dt <= 0, one atom or negative chi2");
+  double ek = 0.0;
+  TGX_TRY(ke_and_cm(ctx, &ek, nullptr, nullptr));
+  const int n3 = 3 * ctx->n_sites;
+  double knew = 0.0;
+  if (ek > 0.0) {
+    knew = bussi_resample(ek, T0, n3 - 3, tau, dt, gauss, chi2);
+    TGX_REQUIRE(std::isfinite(knew) && knew >= 0.0, "tgx_bussi_thermostat: resampled kinetic energy %g", knew);
+    TGX_LAUNCH(ctx, "md_bussi", k_scale, dim3((n3 + 255) / 256), dim3(256), 0, n3, std::sqrt(knew / ek),
+               ctx->vel.as<double>());
+  } else {
+    TGX_TRY(tgx_zero(ctx, ctx->vel.p, sizeof(double) * (size_t)n3));
+  }
+  if (e_kin_out) *e_kin_out = knew;
+  return TGX_OK;
+}
+
 extern "C" int tgx_box_scaling(tgx_ctx* ctx, const double* gamma, int i_step, int n_steps, double* box_out) {
   MD_CTX(ctx);
   TGX_REQUIRE(gamma && n_steps > 0 && i_step >= 0, "tgx_box_scaling: null gamma or bad step counts");

@@ -41,13 +41,36 @@ def box_scaling_factor(values) -> np.ndarray:
     raise ValueError(f"box_scaling_factor takes 1, 3 or 9 numbers, got {v.size}")
 
 
+def target_temperature(t_beg: float, t_end: float, md_istep: int, md_nsteps: int, t_hold=()) -> float:
+    """The thermostat's target at step md_istep (get_target_temp, src/md.f90:1094-1128):
+    the t_beg -> t_end ramp, and for each (T, n_start, n_end) triple of t_hold a
+    plateau at the ramp's value at n_start until n_end, after which it ramps to
+    t_end by the last step."""
+    s = float(md_istep + 1)
+    T = t_beg + (t_end - t_beg) * s / float(md_nsteps)
+    for _, n_start, n_end in np.asarray(t_hold, dtype=np.float64).reshape(-1, 3):
+        start = t_beg + (t_end - t_beg) * (n_start / float(md_nsteps))
+        if n_start < s <= n_end:
+            T = start
+        elif s >= n_end:
+            T = start + (t_end - start) * (s - n_end) / (float(md_nsteps) - n_end)
+    return T
+
+
 class MD:
     """fix_atom (n, 3) booleans hold Cartesian components in place
     (velocity_verlet's fix_atom). box (3, 3) rows a, b, c: the system's box, kept
-    up to date when a barostat rescales it."""
+    up to date when a barostat rescales it. thermostat "berendsen" or "bussi"
+    (src/turbogap_md.f90:430-449); the Bussi thermostat's random draws come from
+    a numpy generator seeded with thermostat_seed."""
 
     def __init__(self, ctx, masses: np.ndarray, velocities: np.ndarray, rcut: float, path: str = "fp64_plain",
-                 neighbors_buffer: float = 0.0, fix_atom=None, box=None, fused: bool = True):
+                 neighbors_buffer: float = 0.0, fix_atom=None, box=None, fused: bool = True,
+                 thermostat: str = "berendsen", thermostat_seed: int = 0):
+        if thermostat not in ("berendsen", "bussi"):
+            raise ValueError(f"thermostat must be 'berendsen' or 'bussi', got {thermostat!r}")
+        self.thermostat = thermostat
+        self.rng = np.random.default_rng(thermostat_seed)
         self.ctx, self.rcut, self.path = ctx, rcut, path
         self.fused = fused  # steps without a barostat through tgx_md_advance (one synchronisation)
         self.buffer = float(neighbors_buffer)
@@ -106,10 +129,11 @@ class MD:
         """One step: forces at x(t), velocity Verlet, E_kin, T and pressure, then
         (scale_box = (gamma (3, 3), n_steps) given) box_scaling at this step's index
         or (P0 given) the Berendsen barostat on box and positions, (T0 given) the
-        Berendsen thermostat, wrap (src/turbogap_md.f90:280-440)."""
+        Berendsen or Bussi thermostat, wrap (src/turbogap_md.f90:280-449)."""
         lib = L.load()
         self.forces(path)
-        if self.fused and P0 is None and scale_box is None and (T0 is None or T0 > 0.0):
+        bussi = self.thermostat == "bussi" and T0 is not None
+        if self.fused and P0 is None and scale_box is None and not bussi and (T0 is None or T0 > 0.0):
             ekc, Tc, rb = C.c_double(), C.c_double(), C.c_int()
             L.check(lib.tgx_md_advance(self.ctx.h, float(dt), int(self.step_count == 0),
                                        float(T0) if T0 is not None else 0.0, float(tau), self.buffer, C.byref(ekc),
@@ -136,7 +160,13 @@ class MD:
             L.check(lib.tgx_berendsen_barostat(self.ctx.h, float(P0), L.dptr(pt), barostat_sym.encode(), float(tau_p),
                                                float(gamma_p), float(dt), L.dptr(nb)), "berendsen_barostat")
             self.box = nb.reshape(3, 3)
-        if T0 is not None:
+        if bussi:
+            ndeg = 3 * self.masses.shape[0] - 3
+            g = float(self.rng.standard_normal())
+            chi2 = float(self.rng.chisquare(ndeg - 1)) if ndeg > 1 else 0.0
+            L.check(lib.tgx_bussi_thermostat(self.ctx.h, float(T0), float(tau), float(dt), g, chi2, None),
+                    "bussi_thermostat")
+        elif T0 is not None:
             L.check(lib.tgx_berendsen(self.ctx.h, float(T0), float(T), float(tau), float(dt)), "berendsen")
         L.check(lib.tgx_wrap_pbc(self.ctx.h), "wrap_pbc")
         if self.buffer > 0.0:
