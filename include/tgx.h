@@ -26,6 +26,7 @@
  *   tgx_md_pressure                     src/turbogap_md.f90:287-295 instant_pressure, instant_pressure_tensor
  *   tgx_berendsen_barostat              src/md.f90:384-417, src/turbogap_md.f90:413-426 berendsen_barostat on box and positions
  *   tgx_box_scaling                     src/md.f90:421-458          box_scaling (scale_box, box_scaling_factor)
+ *   tgx_gradient_descent                src/md.f90:540-664          gradient_descent (optimize = "gd")
  *   tgx_bussi_thermostat                src/turbogap_md.f90:438-449, src/third_party/bussi_thermostat/resamplekin.f90:50-84
  *                                       thermostat = "bussi" (resamplekin on the device E_kin)
  *   tgx_distance_2b_create              src/read_files.f90:722-742,5238-5305  a distance_2b block and its sparse set
@@ -298,6 +299,18 @@ int tgx_md_displacement(tgx_ctx* ctx, double neighbors_buffer, int reset, int* r
  * (gamma[3 r + c] = box_scaling_factor(r + 1, c + 1)); the new box goes to box_out
  * (9, may be NULL). */
 int tgx_box_scaling(tgx_ctx* ctx, const double* gamma, int i_step, int n_steps, double* box_out);
+/* optimize = "gd": one relaxation step in place of velocity Verlet, from the device
+ * forces and the total energy at the current positions. Velocities are zeroed. On
+ * first_step the positions and forces are kept and the step is
+ * max_opt_step / max_i |F_i| (or the step the last line search accepted, for the
+ * context's life); while that line search runs, a step that fails the Armijo test
+ * E <= E0 - gamma/2 sum F^2 goes back to the kept positions and forces (restored = 1;
+ * the device forces are replaced too) with half the step. After it, the
+ * Barzilai-Borwein step |s.y|/|y|^2 with s and y the changes of positions (minimum
+ * image) and forces. Fixed components stay put. Orthorhombic boxes; gamma_out and
+ * restored may be NULL. */
+int tgx_gradient_descent(tgx_ctx* ctx, double energy, double max_opt_step, int first_step, double* gamma_out,
+                         int* restored);
 /* thermostat = "bussi": velocities scaled by sqrt(K'/K), K the device E_kin and K'
  * the Bussi-Donadio-Parrinello resample of it (resamplekin) towards T0 with
  * ndeg = 3 N - 3, relaxation time tau and step dt. The caller supplies the random

@@ -19,6 +19,11 @@ device MD; the product path never imports it.
     resamplekin.f90:50-84; Bussi, Donadio and Parrinello, J. Chem. Phys. 126,
     014101 (2007), eq. A7) with its two random draws as arguments: the Gaussian
     of gasdev() and the chi-squared (ndeg - 1) of sumnoises(ndeg - 1).
+  * GradientDescent follows gradient_descent (src/md.f90:540-664): the first
+    step max_opt_step / max |F| (or the accepted step of an earlier line
+    search), backtracking by halves until the Armijo test E <= E0 - gamma/2
+    sum F^2 holds, then Barzilai-Borwein steps |s.y|/|y|^2 with positions_prev
+    taken to the minimum image (orthorhombic box).
   * target_temperature follows get_target_temp (src/md.f90:1094-1128), the
     t_beg -> t_end ramp with its t_hold plateaus.
 Arrays are (n, 3) here; the reference's are (3, n).
@@ -101,3 +106,54 @@ def target_temperature(t_beg, t_end, md_istep, md_nsteps, t_hold=()):
             th = t_beg + (t_end - t_beg) * (n_start / float(md_nsteps))
             T = th + (t_end - th) * (s - n_end) / (float(md_nsteps) - n_end)
     return T
+
+
+class GradientDescent:
+    """State of the reference's save variables; step() mutates the arrays given."""
+
+    def __init__(self):
+        self.initialized = False
+        self.backtracking = False
+        self.energy0 = self.gamma_prev = self.m_prev = self.gamma_back0 = 0.0
+        self.x0 = self.f0 = None
+
+    def step(self, x, xprev, F, Fprev, energy, max_opt_step, first_step, box_diag, fix=None):
+        """x, xprev, F, Fprev (n, 3) updated in place; returns (gamma, restored)."""
+        restored = False
+        gamma = 0.0
+        if first_step:
+            self.backtracking = True
+            self.x0, self.f0 = x.copy(), F.copy()
+            self.energy0 = energy
+            fmax = float(np.sqrt(np.sum(F * F, axis=1)).max())
+            if fmax == 0.0:
+                gamma = 0.0
+            elif self.initialized:
+                gamma = self.gamma_back0
+            else:
+                gamma = max_opt_step / fmax
+        elif self.backtracking:
+            if energy <= self.energy0 - self.gamma_prev * 0.5 * self.m_prev:
+                self.backtracking = False
+                self.initialized = True
+                self.gamma_back0 = self.gamma_prev
+            else:
+                gamma = self.gamma_prev * 0.5
+                x[:] = self.x0
+                F[:] = self.f0
+                restored = True
+        if not first_step and not self.backtracking:
+            L = np.asarray(box_diag, dtype=np.float64)
+            d = x - xprev
+            d = d - np.rint(d / L) * L
+            xprev[:] = x - d
+            y = F - Fprev
+            syy = float(np.sum(y * y))
+            gamma = 0.0 if syy == 0.0 else abs(float(np.sum((x - xprev) * y)) / syy)
+        xprev[:] = x
+        Fprev[:] = F
+        move = np.ones_like(x, dtype=bool) if fix is None else ~np.asarray(fix, dtype=bool)
+        x[move] = xprev[move] + gamma * Fprev[move]
+        self.gamma_prev = gamma
+        self.m_prev = float(np.sum(F * F))
+        return gamma, restored

@@ -1,6 +1,7 @@
-"""The Bussi thermostat and the target-temperature ramp of the device MD
-(src/turbogap_md.f90:430-449, src/third_party/bussi_thermostat/resamplekin.f90:50-84,
-src/md.f90:1094-1128).
+"""The Bussi thermostat, the target-temperature ramp and gradient-descent
+relaxation of the device MD (src/turbogap_md.f90:224-228, 430-449,
+src/third_party/bussi_thermostat/resamplekin.f90:50-84, src/md.f90:540-664,
+1094-1128).
 
 CPU (oracle pins):
   * resample limits: tau/dt huge keeps K (the memory factor is 1); tau/dt <= 0.1
@@ -9,12 +10,19 @@ CPU (oracle pins):
     and resampled keeps its mean sigma and variance 2 sigma^2/ndeg (a dropped or
     mis-scaled cross term changes the variance), and from a K far from sigma the
     mean relaxes by exactly (1 - c)(sigma - K), c = exp(-dt/tau);
-  * target_temperature: the linear ramp and a t_hold plateau, by hand.
+  * target_temperature: the linear ramp and a t_hold plateau, by hand;
+  * gradient descent on a quadratic well: the line search's halvings and
+    restarts, the Barzilai-Borwein step equal to 1/k that lands on the
+    minimum, the accepted step reused by a later relaxation, the minimum-image
+    convention across a wrap, fixed components.
 GPU (parity): the library's Bussi step against the oracle's resample of the
 device kinetic energy with the same draws: velocities to 1e-14 relative, the
 new kinetic energy equal to K'; zero kinetic energy zeroes the velocities;
-a short Bussi NVT run.
+a short Bussi NVT run; six gradient-descent steps of the carbon box against the
+oracle fed the library's energies and forces.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -110,3 +118,92 @@ def test_gpu_bussi_nvt_run(ctx):
         assert np.isfinite(E) and np.isfinite(T)
     _, Tend = md.kinetic()
     assert Tend < 700.0
+
+
+def _quadratic(x, c, k, L):
+    d = x - c
+    d = d - np.rint(d / L) * L
+    return 0.5 * k * float(np.sum(d * d)), -k * d
+
+
+def test_gradient_descent_line_search_and_bb_on_a_quadratic():
+    """E = k/2 sum |x - c|^2 (minimum image): a first step of 3.2/k fails the
+    Armijo test twice (back to the start with 1.6/k, then 0.8/k), the third
+    passes, and the Barzilai-Borwein step is then exactly 1/k, which lands on
+    c; a later relaxation starts from the accepted 0.8/k; fixed components do
+    not move; an atom moved by a box length between calls (a wrap) changes
+    nothing."""
+    rng = np.random.default_rng(3)
+    n, k, L = 7, 2.5, np.array([20.0, 21.0, 22.0])
+    c = rng.uniform(5, 15, (n, 3))
+    x0 = c + rng.normal(scale=0.3, size=(n, 3))
+    out = []
+    for wrap in (False, True):
+        gd = MDO.GradientDescent()
+        x, xp, Fp = x0.copy(), np.zeros((n, 3)), np.zeros((n, 3))
+        E0, F0 = _quadratic(x, c, k, L)
+        max_opt = 3.2 / k * np.sqrt((F0 ** 2).sum(1)).max()
+        gam, res = [], []
+        for s in range(4):
+            if wrap and s == 3:
+                x[2, 1] += L[1]
+            E, F = _quadratic(x, c, k, L)
+            g, r = gd.step(x, xp, F, Fp, E, max_opt, s == 0, L)
+            gam.append(g)
+            res.append(r)
+        d = x - c
+        d = d - np.rint(d / L) * L
+        assert np.abs(d).max() < 1e-12
+        out.append(gam)
+        np.testing.assert_allclose(np.array(gam) * k, [3.2, 1.6, 0.8, 1.0], rtol=1e-12)
+        assert res == [False, True, True, False]
+        # a new relaxation starts from the accepted step
+        x2, xp2, Fp2 = x0.copy(), np.zeros((n, 3)), np.zeros((n, 3))
+        E, F = _quadratic(x2, c, k, L)
+        g, _ = gd.step(x2, xp2, F, Fp2, E, max_opt, True, L)
+        assert abs(g * k - 0.8) < 1e-12
+    assert out[0][3] == out[1][3] or abs(out[0][3] / out[1][3] - 1) < 1e-12
+    gd = MDO.GradientDescent()
+    x, xp, Fp = x0.copy(), np.zeros((n, 3)), np.zeros((n, 3))
+    fix = np.zeros((n, 3), bool)
+    fix[0, :] = fix[4, 2] = True
+    E, F = _quadratic(x, c, k, L)
+    gd.step(x, xp, F, Fp, E, 0.1, True, L, fix)
+    assert np.array_equal(x[fix], x0[fix]) and np.abs(x[~fix] - x0[~fix]).max() > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("forced", [False, True])
+def test_gpu_gradient_descent_matches_oracle(ctx, forced):
+    """Six relaxation steps of the carbon box on the device against the oracle
+    fed the library's own energies and forces: the same step sizes and
+    line-search decisions, positions to 1e-11 A; the energy goes down. Forced:
+    the second call is given an energy 10 eV above the true one, which fails the
+    line search's test, so both go back to the first positions and forces with
+    half the step."""
+    max_opt = 0.2
+    sysm, mass, v0 = _md_system(ctx)
+    md = MD(ctx, mass, v0, 3.7, "fp64_adjoint")
+    lib = L.load()
+    gd = MDO.GradientDescent()
+    n = sysm.n
+    x, xp, Fp = sysm.positions.copy(), np.zeros((n, 3)), np.zeros((n, 3))
+    L_box = np.diag(sysm.box)
+    energies, restores = [], []
+    for s in range(6):
+        E = md.forces()
+        _, F = ctx.results()
+        if forced and s == 1:
+            E = E + 10.0
+        g, r = C.c_double(), C.c_int()
+        L.check(lib.tgx_gradient_descent(ctx.h, float(E), max_opt, int(s == 0), C.byref(g), C.byref(r)),
+                "gradient_descent")
+        go, ro = gd.step(x, xp, F.copy(), Fp, E, max_opt, s == 0, L_box)
+        restores.append(ro)
+        assert bool(r.value) == ro and abs(g.value - go) <= 1e-10 * max(go, 1e-300)
+        xd, vd = md.state()
+        np.testing.assert_allclose(xd, x, rtol=0, atol=1e-11)
+        assert not vd.any()
+        energies.append(E)
+    assert energies[-1] < energies[0]
+    assert restores[1] == forced and not any(restores[2:])

@@ -206,6 +206,11 @@ struct tgx_ctx {
   bool md_fixed = false;
   double md_box0[9] = {0};     // box at step 0 of a box-scaling run
   bool md_box0_set = false;
+  // gradient descent: positions and forces of its first step, then the line
+  // search and Barzilai-Borwein state (gd_initialized outlives a restart)
+  dbuf md_gd0;
+  bool gd_started = false, gd_backtracking = false, gd_initialized = false;
+  double gd_energy0 = 0.0, gd_gamma_prev = 0.0, gd_m_prev = 0.0, gd_gamma_back0 = 0.0;
   int* md_flag_host = nullptr;  // pinned
   double dt_prev = 0.0;
   // surrogate gate

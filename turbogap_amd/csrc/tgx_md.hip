@@ -7,6 +7,8 @@
 #include "tgx_util.h"
 
 #include <cmath>
+#include <algorithm>
+#include <vector>
 
 namespace {
 
@@ -41,6 +43,52 @@ __global__ void k_linmap(int n, double m00, double m01, double m02, double m10, 
   x[3 * (size_t)i] = a + (m00 * a + m01 * b + m02 * c);
   x[3 * (size_t)i + 1] = b + (m10 * a + m11 * b + m12 * c);
   x[3 * (size_t)i + 2] = c + (m20 * a + m21 * b + m22 * c);
+}
+
+// Gradient descent, Barzilai-Borwein terms: positions_prev taken to the image of
+// positions' convention (minimum image, orthorhombic box), then per atom
+// (x - x_prev) . (F - F_prev) and |F - F_prev|^2.
+__global__ void k_gd_terms(int n, double Lx, double Ly, double Lz, const double* __restrict__ x,
+                           double* __restrict__ xprev, const double* __restrict__ F, const double* __restrict__ Fprev,
+                           double* __restrict__ sxy, double* __restrict__ syy) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double L[3] = {Lx, Ly, Lz};
+  double a = 0.0, b = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const size_t j = 3 * (size_t)i + k;
+    double d = x[j] - xprev[j];
+    d = d - rint(d / L[k]) * L[k];
+    const double xp = x[j] - d;
+    xprev[j] = xp;
+    const double y = F[j] - Fprev[j];
+    a = a + (x[j] - xp) * y;
+    b = b + y * y;
+  }
+  sxy[i] = a;
+  syy[i] = b;
+}
+
+// |F_i|^2 per atom
+__global__ void k_fsq(int n, const double* __restrict__ F, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = F[3 * (size_t)i], b = F[3 * (size_t)i + 1], c = F[3 * (size_t)i + 2];
+  out[i] = a * a + b * b + c * c;
+}
+
+// positions_prev = x, forces_prev = F, then x = x_prev + gamma F_prev except
+// fixed components.
+__global__ void k_gd_update(int n3, double gamma, double* __restrict__ x, double* __restrict__ xprev,
+                            const double* __restrict__ F, double* __restrict__ Fprev,
+                            const unsigned char* __restrict__ fix) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n3) return;
+  const double xj = x[j], fj = F[j];
+  xprev[j] = xj;
+  Fprev[j] = fj;
+  if (!(fix && fix[j])) x[j] = xj + gamma * fj;
 }
 
 // x_c *= f_c for every atom (the barostat's scaling, per Cartesian component).
@@ -407,6 +455,79 @@ extern "C" int tgx_bussi_thermostat(tgx_ctx* ctx, double T0, double tau, double 
     TGX_TRY(tgx_zero(ctx, ctx->vel.p, sizeof(double) * (size_t)n3));
   }
   if (e_kin_out) *e_kin_out = knew;
+  return TGX_OK;
+}
+
+extern "C" int tgx_gradient_descent(tgx_ctx* ctx, double energy, double max_opt_step, int first_step,
+                                    double* gamma_out, int* restored) {
+  MD_CTX(ctx);
+  TGX_REQUIRE(ctx->vel.p && ctx->pos_prev.p && ctx->forces_prev.p && ctx->forces.p && max_opt_step >= 0.0,
+              "tgx_gradient_descent: no MD state, no forces or negative max_opt_step");
+  const double* b = ctx->box;
+  TGX_REQUIRE(b[1] == 0 && b[2] == 0 && b[3] == 0 && b[5] == 0 && b[6] == 0 && b[7] == 0,
+              "tgx_gradient_descent: orthorhombic boxes only");
+  TGX_REQUIRE(first_step || ctx->gd_started, "tgx_gradient_descent: the first call must have first_step = 1");
+  const int n = ctx->n_sites;
+  const size_t n3 = 3 * (size_t)n;
+  const dim3 g1((n + 255) / 256), g3((unsigned)((n3 + 255) / 256));
+  double* x = ctx->pos.as<double>();
+  double* F = ctx->forces.as<double>();
+  TGX_TRY(ctx->scratch.ensure(sizeof(double) * 2 * (size_t)n));
+  double* t0 = ctx->scratch.as<double>();
+  double* t1 = t0 + n;
+  TGX_TRY(ctx->md_gd0.ensure(sizeof(double) * 2 * n3));
+  double* x0 = ctx->md_gd0.as<double>();
+  double* f0 = x0 + n3;
+  TGX_TRY(tgx_zero(ctx, ctx->vel.p, sizeof(double) * n3));
+  double gamma = 0.0;
+  bool backtrack_now = false;
+  if (first_step) {
+    ctx->gd_started = true;
+    ctx->gd_backtracking = true;
+    TGX_CHECK_HIP(hipMemcpyAsync(x0, x, sizeof(double) * n3, hipMemcpyDeviceToDevice, ctx->stream));
+    TGX_CHECK_HIP(hipMemcpyAsync(f0, F, sizeof(double) * n3, hipMemcpyDeviceToDevice, ctx->stream));
+    ctx->gd_energy0 = energy;
+    TGX_LAUNCH(ctx, "md_gd_fsq", k_fsq, g1, dim3(256), 0, n, (const double*)F, t0);
+    std::vector<double> fsq(n);
+    TGX_CHECK_HIP(hipMemcpyAsync(fsq.data(), t0, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    double fmax = 0.0;
+    for (int i = 0; i < n; ++i) fmax = std::max(fmax, std::sqrt(fsq[i]));
+    if (fmax == 0.0) gamma = 0.0;
+    else if (ctx->gd_initialized) gamma = ctx->gd_gamma_back0;
+    else gamma = max_opt_step / fmax;
+  } else if (ctx->gd_backtracking) {
+    // Armijo-Goldstein: accept the step, else go back to the start with half of it
+    if (energy <= ctx->gd_energy0 - ctx->gd_gamma_prev * 0.5 * ctx->gd_m_prev) {
+      ctx->gd_backtracking = false;
+      ctx->gd_initialized = true;
+      ctx->gd_gamma_back0 = ctx->gd_gamma_prev;
+    } else {
+      gamma = ctx->gd_gamma_prev * 0.5;
+      TGX_CHECK_HIP(hipMemcpyAsync(x, x0, sizeof(double) * n3, hipMemcpyDeviceToDevice, ctx->stream));
+      TGX_CHECK_HIP(hipMemcpyAsync(F, f0, sizeof(double) * n3, hipMemcpyDeviceToDevice, ctx->stream));
+      backtrack_now = true;
+    }
+  }
+  if (!first_step && !ctx->gd_backtracking) {
+    // Barzilai-Borwein step |s.y| / |y|^2, s = x - x_prev, y = F - F_prev
+    TGX_LAUNCH(ctx, "md_gd_terms", k_gd_terms, g1, dim3(256), 0, n, b[0], b[4], b[8], (const double*)x,
+               ctx->pos_prev.as<double>(), (const double*)F, (const double*)ctx->forces_prev.as<double>(), t0, t1);
+    double sxy = 0.0, syy = 0.0;
+    TGX_TRY(tgx_device_sum(ctx, t0, n, &sxy));
+    TGX_TRY(tgx_device_sum(ctx, t1, n, &syy));
+    gamma = syy == 0.0 ? 0.0 : std::fabs(sxy / syy);
+  }
+  TGX_LAUNCH(ctx, "md_gd_update", k_gd_update, g3, dim3(256), 0, (int)n3, gamma, x, ctx->pos_prev.as<double>(),
+             (const double*)F, ctx->forces_prev.as<double>(),
+             ctx->md_fixed ? ctx->md_fix.as<unsigned char>() : (const unsigned char*)nullptr);
+  TGX_LAUNCH(ctx, "md_gd_fsq", k_fsq, g1, dim3(256), 0, n, (const double*)F, t0);
+  double m = 0.0;
+  TGX_TRY(tgx_device_sum(ctx, t0, n, &m));
+  ctx->gd_gamma_prev = gamma;
+  ctx->gd_m_prev = m;
+  if (gamma_out) *gamma_out = gamma;
+  if (restored) *restored = backtrack_now ? 1 : 0;
   return TGX_OK;
 }
 

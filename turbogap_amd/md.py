@@ -174,6 +174,23 @@ class MD:
         self.step_count += 1
         return self.energy, ek, T
 
+    def gd_step(self, max_opt_step: float = 0.1, path=None):
+        """One gradient-descent relaxation step (optimize = "gd",
+        src/turbogap_md.f90:224-228, src/md.f90:540-664): forces and energy at
+        the current positions, then the step on the device. Returns (energy,
+        gamma, restored); restored means the line search went back to the first
+        positions, and the neighbour list is rebuilt before the next forces."""
+        lib = L.load()
+        self.forces(path)
+        g, r = C.c_double(), C.c_int()
+        L.check(lib.tgx_gradient_descent(self.ctx.h, float(self.energy), float(max_opt_step),
+                                         int(self.step_count == 0), C.byref(g), C.byref(r)), "gradient_descent")
+        L.check(lib.tgx_wrap_pbc(self.ctx.h), "wrap_pbc")
+        if self.buffer > 0.0:
+            self.rebuild = self._displacement() or bool(r.value)
+        self.step_count += 1
+        return self.energy, g.value, bool(r.value)
+
     def write_frame(self, fh, species_names, cell, dt: float = 1.0):
         """One extended-XYZ frame of the current state (write_extxyz, src/xyz.f90:169-487):
         positions, velocities (A/fs), the forces and local energies of the last
