@@ -27,6 +27,7 @@
  *   tgx_berendsen_barostat              src/md.f90:384-417, src/turbogap_md.f90:413-426 berendsen_barostat on box and positions
  *   tgx_box_scaling                     src/md.f90:421-458          box_scaling (scale_box, box_scaling_factor)
  *   tgx_gradient_descent                src/md.f90:540-664          gradient_descent (optimize = "gd")
+ *   tgx_variable_time_step              src/md.f90:464-535, src/turbogap_md.f90:178-181 variable_time_step
  *   tgx_bussi_thermostat                src/turbogap_md.f90:438-449, src/third_party/bussi_thermostat/resamplekin.f90:50-84
  *                                       thermostat = "bussi" (resamplekin on the device E_kin)
  *   tgx_distance_2b_create              src/read_files.f90:722-742,5238-5305  a distance_2b block and its sparse set
@@ -299,6 +300,12 @@ int tgx_md_displacement(tgx_ctx* ctx, double neighbors_buffer, int reset, int* r
  * (gamma[3 r + c] = box_scaling_factor(r + 1, c + 1)); the new box goes to box_out
  * (9, may be NULL). */
 int tgx_box_scaling(tgx_ctx* ctx, const double* gamma, int i_step, int n_steps, double* box_out);
+/* variable_time_step, from the device velocities and forces: the step new_dt at
+ * which the largest |v dt + F/(2m) dt^2| over atoms first crosses target_pos_step,
+ * searched from *dt in steps of 1% (down while too large, up while too small);
+ * then *dt = min(new_dt, dt0) when init, else min(dt tau_dt/(tau_dt + dt - new_dt), dt0).
+ * An error when 16384 steps of 1% do not reach the target (no motion at all). */
+int tgx_variable_time_step(tgx_ctx* ctx, int init, double target_pos_step, double tau_dt, double dt0, double* dt);
 /* optimize = "gd": one relaxation step in place of velocity Verlet, from the device
  * forces and the total energy at the current positions. Velocities are zeroed. On
  * first_step the positions and forces are kept and the step is

@@ -24,6 +24,10 @@ device MD; the product path never imports it.
     search), backtracking by halves until the Armijo test E <= E0 - gamma/2
     sum F^2 holds, then Barzilai-Borwein steps |s.y|/|y|^2 with positions_prev
     taken to the minimum image (orthorhombic box).
+  * variable_time_step follows src/md.f90:464-535: from dt, steps of 0.99
+    while the largest |v dt + 0.5 F/m dt^2| exceeds target_pos_step (or 1.01
+    while below it, whichever held at dt), then min(new_dt, dt0) at init, else
+    the relaxed min(dt tau_dt/(tau_dt + dt - new_dt), dt0).
   * target_temperature follows get_target_temp (src/md.f90:1094-1128), the
     t_beg -> t_end ramp with its t_hold plateaus.
 Arrays are (n, 3) here; the reference's are (3, n).
@@ -157,3 +161,26 @@ class GradientDescent:
         self.gamma_prev = gamma
         self.m_prev = float(np.sum(F * F))
         return gamma, restored
+
+
+def variable_time_step(init, velocities, forces, masses, target_pos_step, tau_dt, dt0, dt):
+    v = np.asarray(velocities, dtype=np.float64)
+    a = 0.5 * np.asarray(forces, dtype=np.float64) / np.asarray(masses, dtype=np.float64)[:, None]
+
+    def dmax(c):
+        x = v * c + a * (c * c)
+        return float(np.sqrt((x[:, 0] * x[:, 0] + x[:, 1] * x[:, 1]) + x[:, 2] * x[:, 2]).max())
+
+    too_large = dmax(dt) > target_pos_step
+    new_dt = dt
+    while True:
+        d = dmax(new_dt)
+        if d > target_pos_step and too_large:
+            new_dt = new_dt * 0.99
+        elif d < target_pos_step and not too_large:
+            new_dt = new_dt * 1.01
+        else:
+            break
+    if init:
+        return min(new_dt, dt0)
+    return min(dt * tau_dt / (tau_dt + dt - new_dt), dt0)

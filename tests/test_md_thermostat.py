@@ -1,6 +1,7 @@
-"""The Bussi thermostat, the target-temperature ramp and gradient-descent
-relaxation of the device MD (src/turbogap_md.f90:224-228, 430-449,
-src/third_party/bussi_thermostat/resamplekin.f90:50-84, src/md.f90:540-664,
+"""The Bussi thermostat, the target-temperature ramp, gradient-descent
+relaxation and the variable time step of the device MD
+(src/turbogap_md.f90:178-181, 224-228, 430-449,
+src/third_party/bussi_thermostat/resamplekin.f90:50-84, src/md.f90:464-664,
 1094-1128).
 
 CPU (oracle pins):
@@ -14,12 +15,15 @@ CPU (oracle pins):
   * gradient descent on a quadratic well: the line search's halvings and
     restarts, the Barzilai-Borwein step equal to 1/k that lands on the
     minimum, the accepted step reused by a later relaxation, the minimum-image
-    convention across a wrap, fixed components.
+    convention across a wrap, fixed components;
+  * the variable time step against closed forms (velocities only, forces only)
+    and the relaxed update by hand.
 GPU (parity): the library's Bussi step against the oracle's resample of the
 device kinetic energy with the same draws: velocities to 1e-14 relative, the
 new kinetic energy equal to K'; zero kinetic energy zeroes the velocities;
 a short Bussi NVT run; six gradient-descent steps of the carbon box against the
-oracle fed the library's energies and forces.
+oracle fed the library's energies and forces; the variable time step equal to
+the oracle's, bit for bit.
 """
 import ctypes as C
 
@@ -207,3 +211,58 @@ def test_gpu_gradient_descent_matches_oracle(ctx, forced):
         energies.append(E)
     assert energies[-1] < energies[0]
     assert restores[1] == forced and not any(restores[2:])
+
+
+def test_variable_time_step_oracle():
+    """Velocities only: the step found is within 1% above target/|v|max when dt
+    is too small and within 1% below it when too large; forces only: within 1%
+    of sqrt(2 m target/|F|); the relaxed update dt tau/(tau + dt - new) and the
+    dt0 ceiling by hand."""
+    rng = np.random.default_rng(9)
+    n = 40
+    v = rng.normal(size=(n, 3)) * 0.01
+    m = rng.uniform(10.0, 20.0, n)
+    F0 = np.zeros((n, 3))
+    vmax = np.sqrt((v ** 2).sum(1)).max()
+    target = 0.1
+    exact = target / vmax
+    up = MDO.variable_time_step(True, v, F0, m, target, 100.0, 1e9, exact / 7.0)
+    assert exact <= up * (1 + 1e-12) and up / 1.01 < exact
+    down = MDO.variable_time_step(True, v, F0, m, target, 100.0, 1e9, exact * 3.0)
+    assert down <= exact * (1 + 1e-12) and down / 0.99 > exact
+    F = rng.normal(size=(n, 3))
+    k = np.argmax(np.sqrt((F ** 2).sum(1)) / m)
+    exact_f = np.sqrt(2 * m[k] * target / np.sqrt((F[k] ** 2).sum()))
+    t = MDO.variable_time_step(True, np.zeros((n, 3)), F, m, target, 100.0, 1e9, exact_f / 2)
+    assert exact_f <= t * (1 + 1e-12) and t / 1.01 < exact_f
+    assert MDO.variable_time_step(True, v, F0, m, target, 100.0, 0.5 * exact, exact / 7.0) == 0.5 * exact
+    new = MDO.variable_time_step(True, v, F0, m, target, 10.0, 1e9, 1.0 * exact)  # already at the target
+    relaxed = MDO.variable_time_step(False, v, F0, m, target, 10.0, 1e9, 0.5 * exact)
+    new_half = MDO.variable_time_step(True, v, F0, m, target, 10.0, 1e9, 0.5 * exact)
+    assert abs(relaxed - 0.5 * exact * 10.0 / (10.0 + 0.5 * exact - new_half)) < 1e-15 * relaxed
+    assert new >= exact * (1 - 1e-12)
+
+
+@pytest.mark.gpu
+def test_gpu_variable_time_step_matches_oracle(ctx):
+    """The library's search on the device velocities and forces of the carbon
+    box against the oracle, from a step that is too large and from one that is
+    too small, at init and relaxed: identical steps (the comparisons with the
+    target are exact on both sides). An MD run with a variable step keeps it
+    under its ceiling."""
+    sysm, mass, v0 = _md_system(ctx, T=1500.0)
+    md = MD(ctx, mass, v0, 3.7, "fp64_adjoint")
+    md.forces()
+    _, F = ctx.results()
+    lib = L.load()
+    for init in (1, 0):
+        for dt_start in (0.05, 3.0):
+            t = C.c_double(dt_start)
+            L.check(lib.tgx_variable_time_step(ctx.h, init, 0.02, 20.0, 2.5, C.byref(t)), "variable_time_step")
+            to = MDO.variable_time_step(bool(init), v0, F, mass, 0.02, 20.0, 2.5, dt_start)
+            assert t.value == to, (init, dt_start, t.value, to)
+    ctx.set_positions(sysm.positions)
+    md = MD(ctx, mass, v0, 3.7, "fp64_adjoint", neighbors_buffer=0.3)
+    for _ in range(4):
+        E, ek, T = md.step(1.0, variable_dt=(0.02, 20.0))
+        assert np.isfinite(E) and 0.0 < md.time_step <= 1.0

@@ -91,6 +91,37 @@ __global__ void k_gd_update(int n3, double gamma, double* __restrict__ x, double
   if (!(fix && fix[j])) x[j] = xj + gamma * fj;
 }
 
+// variable_time_step: for candidate step c_k (block k), the largest squared
+// displacement over atoms of v c + (0.5 F / m) c^2, evaluated as the reference
+// writes it (no fused multiply-adds, so the comparisons with the target match).
+__global__ __launch_bounds__(256) void k_dt_candidates(int n, const double* __restrict__ c,
+                                                       const double* __restrict__ v, const double* __restrict__ F,
+                                                       const double* __restrict__ mass, double* __restrict__ d2max) {
+#pragma clang fp contract(off)
+  __shared__ double red[256];
+  const double ck = c[blockIdx.x];
+  const double c2 = ck * ck;
+  double best = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double m = mass[i];
+    double s2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const size_t j = 3 * (size_t)i + k;
+      const double x = v[j] * ck + 0.5 * F[j] / m * c2;
+      s2 = s2 + x * x;
+    }
+    best = fmax(best, s2);
+  }
+  red[threadIdx.x] = best;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) d2max[blockIdx.x] = red[0];
+}
+
 // x_c *= f_c for every atom (the barostat's scaling, per Cartesian component).
 __global__ void k_scale3(int n, double f0, double f1, double f2, double* __restrict__ x) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -528,6 +559,62 @@ extern "C" int tgx_gradient_descent(tgx_ctx* ctx, double energy, double max_opt_
   ctx->gd_m_prev = m;
   if (gamma_out) *gamma_out = gamma;
   if (restored) *restored = backtrack_now ? 1 : 0;
+  return TGX_OK;
+}
+
+extern "C" int tgx_variable_time_step(tgx_ctx* ctx, int init, double target_pos_step, double tau_dt, double dt0,
+                                      double* dt) {
+  MD_CTX(ctx);
+  TGX_REQUIRE(dt && *dt > 0.0 && target_pos_step > 0.0 && ctx->vel.p && ctx->mass.p && ctx->forces.p,
+              "tgx_variable_time_step: no MD state or forces, dt <= 0 or target_pos_step <= 0");
+  constexpr int KC = 256;       // candidates per launch
+  constexpr int MAX_ROUNDS = 64;  // 16384 steps of 1% from dt
+  const int n = ctx->n_sites;
+  TGX_TRY(ctx->scratch.ensure(sizeof(double) * 2 * KC));
+  double* dc = ctx->scratch.as<double>();
+  double* dd = dc + KC;
+  std::vector<double> c(KC), d2(KC);
+  auto evaluate = [&](int nc) -> int {
+    TGX_CHECK_HIP(hipMemcpyAsync(dc, c.data(), sizeof(double) * nc, hipMemcpyHostToDevice, ctx->stream));
+    TGX_LAUNCH(ctx, "md_variable_dt", k_dt_candidates, dim3(nc), dim3(256), 0, n, (const double*)dc,
+               ctx->vel.as<double>(), ctx->forces.as<double>(), ctx->mass.as<double>(), dd);
+    TGX_CHECK_HIP(hipMemcpyAsync(d2.data(), dd, sizeof(double) * nc, hipMemcpyDeviceToHost, ctx->stream));
+    TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < nc; ++k)
+      TGX_REQUIRE(std::isfinite(d2[k]), "tgx_variable_time_step: non-finite displacement at dt %g", c[k]);
+    return TGX_OK;
+  };
+  // the displacement at dt sets the direction: shrink by 0.99 while too large,
+  // else grow by 1.01 while too small (the reference's loop, candidates in batches)
+  c[0] = *dt;
+  TGX_TRY(evaluate(1));
+  const bool too_large = std::sqrt(d2[0]) > target_pos_step;
+  const double factor = too_large ? 0.99 : 1.01;
+  double cur = *dt, new_dt = *dt;
+  bool done = false;
+  for (int round = 0; round < MAX_ROUNDS && !done; ++round) {
+    for (int k = 0; k < KC; ++k) {
+      c[k] = cur;
+      cur = cur * factor;
+    }
+    TGX_TRY(evaluate(KC));
+    for (int k = 0; k < KC && !done; ++k) {
+      const double dmax = std::sqrt(d2[k]);
+      const bool go_on = too_large ? dmax > target_pos_step : dmax < target_pos_step;
+      if (!go_on) {
+        new_dt = c[k];
+        done = true;
+      }
+    }
+  }
+  TGX_REQUIRE(done, "tgx_variable_time_step: no step within 16384 steps of 1%% reaches target_pos_step %g",
+              target_pos_step);
+  if (init) {
+    *dt = std::min(new_dt, dt0);
+  } else {
+    const double relaxed = *dt * tau_dt / (tau_dt + *dt - new_dt);
+    *dt = std::min(relaxed, dt0);
+  }
   return TGX_OK;
 }
 

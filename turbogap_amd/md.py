@@ -86,6 +86,7 @@ class MD:
             L.check(L.load().tgx_md_set_fixed(ctx.h, None), "md_set_fixed")
         self.box = None if box is None else np.array(box, dtype=np.float64).reshape(3, 3)
         self.step_count = 0
+        self.time_step = None
         self.energy = None
         self.virial = None
         self.pressure = None
@@ -125,13 +126,24 @@ class MD:
         return p.value, pt.reshape(3, 3)
 
     def step(self, dt: float, T0: float = None, tau: float = 100.0, path=None, P0: float = None,
-             tau_p: float = 1000.0, gamma_p: float = 1.0, barostat_sym: str = "iso", scale_box=None):
+             tau_p: float = 1000.0, gamma_p: float = 1.0, barostat_sym: str = "iso", scale_box=None,
+             variable_dt=None):
         """One step: forces at x(t), velocity Verlet, E_kin, T and pressure, then
         (scale_box = (gamma (3, 3), n_steps) given) box_scaling at this step's index
         or (P0 given) the Berendsen barostat on box and positions, (T0 given) the
-        Berendsen or Bussi thermostat, wrap (src/turbogap_md.f90:280-449)."""
+        Berendsen or Bussi thermostat, wrap (src/turbogap_md.f90:280-449).
+        variable_dt = (target_pos_step, tau_dt): the step is chosen by
+        variable_time_step with dt as its ceiling (src/turbogap_md.f90:178-181)
+        and kept in self.time_step for the next one."""
         lib = L.load()
         self.forces(path)
+        if variable_dt is not None:
+            target, tau_dt = variable_dt
+            t = C.c_double(float(dt) if self.step_count == 0 or self.time_step is None else self.time_step)
+            L.check(lib.tgx_variable_time_step(self.ctx.h, int(self.step_count == 0), float(target), float(tau_dt),
+                                               float(dt), C.byref(t)), "variable_time_step")
+            self.time_step = t.value
+            dt = self.time_step
         bussi = self.thermostat == "bussi" and T0 is not None
         if self.fused and P0 is None and scale_box is None and not bussi and (T0 is None or T0 > 0.0):
             ekc, Tc, rb = C.c_double(), C.c_double(), C.c_int()
