@@ -283,7 +283,8 @@ int tgx_berendsen_barostat(tgx_ctx* ctx, double P0, const double* P, const char*
 int tgx_kinetic(tgx_ctx* ctx, double* e_kin, double* temperature);
 /* berendsen_thermostat (md.f90:289-313). */
 int tgx_berendsen(tgx_ctx* ctx, double T0, double T, double tau, double dt);
-/* remove_cm_vel (md.f90:316-345). */
+/* remove_cm_vel (md.f90:316-345), on the device: the centre-of-mass velocity from
+ * fixed-order partial sums, subtracted without a host round trip. */
 int tgx_remove_cm_vel(tgx_ctx* ctx);
 /* wrap positions back into the orthorhombic box. */
 int tgx_wrap_pbc(tgx_ctx* ctx);
@@ -327,12 +328,13 @@ int tgx_gradient_descent(tgx_ctx* ctx, double energy, double max_opt_step, int f
 int tgx_bussi_thermostat(tgx_ctx* ctx, double T0, double tau, double dt, double gauss, double chi2,
                          double* e_kin_out);
 /* No counterpart in the reference (a driver shortcut): after the step's forces,
- * tgx_velocity_verlet, tgx_kinetic, tgx_berendsen to T0 (T0 <= 0: none),
- * tgx_wrap_pbc and, with neighbors_buffer > 0, the tgx_md_displacement test in
- * one call with one synchronisation; E_kin and T are those after the Verlet
- * update, as tgx_kinetic returns them. */
-int tgx_md_advance(tgx_ctx* ctx, double dt, int first_step, double T0, double tau, double neighbors_buffer,
-                   double* e_kin, double* temperature, int* rebuild);
+ * tgx_remove_cm_vel (remove_cm != 0), tgx_velocity_verlet, tgx_kinetic,
+ * tgx_berendsen to T0 (T0 <= 0: none), tgx_wrap_pbc and, with
+ * neighbors_buffer > 0, the tgx_md_displacement test in one call with one
+ * synchronisation; E_kin and T are those after the Verlet update, as
+ * tgx_kinetic returns them. */
+int tgx_md_advance(tgx_ctx* ctx, double dt, int first_step, int remove_cm, double T0, double tau,
+                   double neighbors_buffer, double* e_kin, double* temperature, int* rebuild);
 
 /* ---- profiling ----------------------------------------------------------- */
 /* Record HIP events around every kernel launch on the context's stream. While

@@ -28,6 +28,7 @@ device MD; the product path never imports it.
     while the largest |v dt + 0.5 F/m dt^2| exceeds target_pos_step (or 1.01
     while below it, whichever held at dt), then min(new_dt, dt0) at init, else
     the relaxed min(dt tau_dt/(tau_dt + dt - new_dt), dt0).
+  * remove_cm_vel follows src/md.f90:316-345.
   * target_temperature follows get_target_temp (src/md.f90:1094-1128), the
     t_beg -> t_end ramp with its t_hold plateaus.
 Arrays are (n, 3) here; the reference's are (3, n).
@@ -184,3 +185,9 @@ def variable_time_step(init, velocities, forces, masses, target_pos_step, tau_dt
     if init:
         return min(new_dt, dt0)
     return min(dt * tau_dt / (tau_dt + dt - new_dt), dt0)
+
+
+def remove_cm_vel(velocities, masses):
+    v = np.array(velocities, dtype=np.float64)
+    m = np.asarray(masses, dtype=np.float64)
+    return v - (m[:, None] * v).sum(0) / m.sum()

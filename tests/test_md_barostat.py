@@ -251,3 +251,26 @@ def test_gpu_md_with_box_scaling(ctx):
         E, ek, T = md.step(1.0, T0=300.0, tau=100.0, scale_box=(G, 5))
         assert np.isfinite(E) and np.isfinite(T)
     np.testing.assert_allclose(md.box, sysm.box @ G.T, rtol=1e-14)
+
+
+def test_remove_cm_vel_oracle():
+    """The centre-of-mass momentum after remove_cm_vel is zero, relative motion
+    is kept, and a uniform drift is removed exactly."""
+    rng = np.random.default_rng(1)
+    v, m = rng.normal(size=(30, 3)), rng.uniform(1, 50, 30)
+    w = MDO.remove_cm_vel(v, m)
+    assert np.abs((m[:, None] * w).sum(0)).max() < 1e-12
+    np.testing.assert_allclose(w[1:] - w[0], v[1:] - v[0], rtol=0, atol=1e-13)
+    u = MDO.remove_cm_vel(np.tile([0.3, -0.2, 0.1], (30, 1)), m)
+    assert np.abs(u).max() < 1e-15
+
+
+@pytest.mark.gpu
+def test_gpu_remove_cm_vel_matches_oracle(ctx):
+    sysm, mass, v0 = _md_system(ctx)
+    drift = v0 + np.array([0.01, -0.02, 0.005])
+    md = MD(ctx, mass, drift, 3.7, "fp64_adjoint")
+    md.remove_cm()
+    _, v = md.state()
+    np.testing.assert_allclose(v, MDO.remove_cm_vel(drift, mass), rtol=0, atol=1e-15)
+    assert np.abs((mass[:, None] * v).sum(0)).max() < 1e-12

@@ -157,7 +157,7 @@ extern "C" int tgx_finalize(tgx_ctx* ctx) {
                   &ctx->cell_atoms, &ctx->scan_tmp, &ctx->do_list, &ctx->energies, &ctx->forces, &ctx->fglob,
                   &ctx->virial_part, &ctx->scratch, &ctx->red, &ctx->vel, &ctx->mass, &ctx->forces_prev,
                   &ctx->pos_prev, &ctx->gate_mask, &ctx->cell_pos, &ctx->md_disp, &ctx->md_flag,
-                  &ctx->md_fix, &ctx->md_gd0};
+                  &ctx->md_fix, &ctx->md_gd0, &ctx->md_cm};
   for (dbuf* b : bufs) b->release();
   if (ctx->gate_host) (void)hipHostFree(ctx->gate_host);
   if (ctx->md_flag_host) (void)hipHostFree(ctx->md_flag_host);

@@ -209,6 +209,7 @@ struct tgx_ctx {
   // gradient descent: positions and forces of its first step, then the line
   // search and Barzilai-Borwein state (gd_initialized outlives a restart)
   dbuf md_gd0;
+  dbuf md_cm;                  // centre-of-mass partial sums
   bool gd_started = false, gd_backtracking = false, gd_initialized = false;
   double gd_energy0 = 0.0, gd_gamma_prev = 0.0, gd_m_prev = 0.0, gd_gamma_back0 = 0.0;
   int* md_flag_host = nullptr;  // pinned

@@ -149,13 +149,48 @@ __global__ void k_ke_terms(int n, const double* __restrict__ mass, const double*
   mv[2 * (size_t)n + i] = m * c;
 }
 
-__global__ void k_sub_cm(int n, double cx, double cy, double cz, double* __restrict__ v) {
+// Centre-of-mass velocity without a host round trip: block b of CM_PARTS sums
+// m v and m over atoms b*256 + t + k*CM_PARTS*256; k_cm_sub's blocks each add the
+// partials in the same order and subtract sum(m v)/sum(m).
+constexpr int CM_PARTS = 64;
+__global__ __launch_bounds__(256) void k_cm_part(int n, const double* __restrict__ mass, const double* __restrict__ v,
+                                                 double* __restrict__ part) {
+  __shared__ double red[4][256];
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += CM_PARTS * 256) {
+    const double m = mass[i];
+    a[0] += m * v[3 * (size_t)i];
+    a[1] += m * v[3 * (size_t)i + 1];
+    a[2] += m * v[3 * (size_t)i + 2];
+    a[3] += m;
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) red[c][threadIdx.x] = a[c];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) part[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void k_cm_sub(int n, const double* __restrict__ part, double* __restrict__ v) {
+  __shared__ double cm[4];
+  if (threadIdx.x < 4) {
+    double s = 0.0;
+    for (int b = 0; b < CM_PARTS; ++b) s += part[4 * b + threadIdx.x];
+    cm[threadIdx.x] = s;
+  }
+  __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  v[3 * (size_t)i] -= cx;
-  v[3 * (size_t)i + 1] -= cy;
-  v[3 * (size_t)i + 2] -= cz;
+  const double M = cm[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) v[3 * (size_t)i + c] -= cm[c] / M;
 }
+
 
 __global__ void k_wrap(int n, double Lx, double Ly, double Lz, double* __restrict__ x) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -343,15 +378,21 @@ extern "C" int tgx_berendsen(tgx_ctx* ctx, double T0, double T, double tau, doub
   return TGX_OK;
 }
 
-extern "C" int tgx_remove_cm_vel(tgx_ctx* ctx) {
-  MD_CTX(ctx);
-  TGX_REQUIRE(ctx->vel.p, "tgx_remove_cm_vel: no velocities");
-  double ek, cm[3], mt;
-  TGX_TRY(ke_and_cm(ctx, &ek, cm, &mt));
+// remove_cm_vel on the device stream, no synchronisation
+static int remove_cm_device(tgx_ctx* ctx) {
   const int n = ctx->n_sites;
-  TGX_LAUNCH(ctx, "md_cm", k_sub_cm, dim3((n + 255) / 256), dim3(256), 0, n, cm[0] / mt, cm[1] / mt, cm[2] / mt,
+  TGX_TRY(ctx->md_cm.ensure(sizeof(double) * 4 * CM_PARTS));
+  TGX_LAUNCH(ctx, "md_cm_part", k_cm_part, dim3(CM_PARTS), dim3(256), 0, n, ctx->mass.as<double>(),
+             ctx->vel.as<double>(), ctx->md_cm.as<double>());
+  TGX_LAUNCH(ctx, "md_cm_sub", k_cm_sub, dim3((n + 255) / 256), dim3(256), 0, n, ctx->md_cm.as<double>(),
              ctx->vel.as<double>());
   return TGX_OK;
+}
+
+extern "C" int tgx_remove_cm_vel(tgx_ctx* ctx) {
+  MD_CTX(ctx);
+  TGX_REQUIRE(ctx->vel.p && ctx->mass.p, "tgx_remove_cm_vel: no velocities");
+  return remove_cm_device(ctx);
 }
 
 extern "C" int tgx_wrap_pbc(tgx_ctx* ctx) {
@@ -398,7 +439,7 @@ extern "C" int tgx_md_displacement(tgx_ctx* ctx, double neighbors_buffer, int re
 // tgx_berendsen, tgx_wrap_pbc and tgx_md_displacement called in that order
 // (the displacement is taken before the wrap; by minimum image it is the
 // same).
-extern "C" int tgx_md_advance(tgx_ctx* ctx, double dt, int first_step, double T0, double tau,
+extern "C" int tgx_md_advance(tgx_ctx* ctx, double dt, int first_step, int remove_cm, double T0, double tau,
                               double neighbors_buffer, double* e_kin, double* temperature, int* rebuild) {
   MD_CTX(ctx);
   TGX_REQUIRE(ctx->vel.p && ctx->mass.p && ctx->forces.p, "tgx_md_advance: state or forces missing");
@@ -407,6 +448,7 @@ extern "C" int tgx_md_advance(tgx_ctx* ctx, double dt, int first_step, double T0
   TGX_REQUIRE(b[1] == 0 && b[2] == 0 && b[3] == 0 && b[5] == 0 && b[6] == 0 && b[7] == 0,
               "tgx_md_advance: orthorhombic boxes only");
   const int n = ctx->n_sites;
+  if (remove_cm) TGX_TRY(remove_cm_device(ctx));
   TGX_LAUNCH(ctx, "md_verlet", k_vv, dim3((n + 255) / 256), dim3(256), 0, n, dt, ctx->dt_prev, first_step ? 1 : 0,
              ctx->forces.as<double>(), ctx->forces_prev.as<double>(), ctx->mass.as<double>(), ctx->vel.as<double>(),
              ctx->pos.as<double>(), ctx->pos_prev.as<double>(),

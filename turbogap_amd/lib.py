@@ -113,7 +113,7 @@ _SIGS = {
     "tgx_wrap_pbc": [_P],
     "tgx_md_displacement": [_P, _D, _I, _PI],
     "tgx_md_set_fixed": [_P, _PI],
-    "tgx_md_advance": [_P, _D, _I, _D, _D, _D, _PD, _PD, _PI],
+    "tgx_md_advance": [_P, _D, _I, _I, _D, _D, _D, _PD, _PD, _PI],
     "tgx_box_scaling": [_P, _PD, _I, _I, _PD],
     "tgx_bussi_thermostat": [_P, _D, _D, _D, _D, _D, _PD],
     "tgx_gradient_descent": [_P, _D, _D, _I, _PD, _PI],

@@ -66,10 +66,11 @@ class MD:
 
     def __init__(self, ctx, masses: np.ndarray, velocities: np.ndarray, rcut: float, path: str = "fp64_plain",
                  neighbors_buffer: float = 0.0, fix_atom=None, box=None, fused: bool = True,
-                 thermostat: str = "berendsen", thermostat_seed: int = 0):
+                 thermostat: str = "berendsen", thermostat_seed: int = 0, remove_cm: bool = True):
         if thermostat not in ("berendsen", "bussi"):
             raise ValueError(f"thermostat must be 'berendsen' or 'bussi', got {thermostat!r}")
         self.thermostat = thermostat
+        self.remove_cm_each_step = bool(remove_cm)  # remove_cm_vel before each step (src/turbogap_md.f90:172-173)
         self.rng = np.random.default_rng(thermostat_seed)
         self.ctx, self.rcut, self.path = ctx, rcut, path
         self.fused = fused  # steps without a barostat through tgx_md_advance (one synchronisation)
@@ -137,7 +138,11 @@ class MD:
         and kept in self.time_step for the next one."""
         lib = L.load()
         self.forces(path)
+        cm = self.remove_cm_each_step  # still to do
         if variable_dt is not None:
+            if cm:  # the step is chosen from the velocities without the CM drift
+                L.check(lib.tgx_remove_cm_vel(self.ctx.h), "remove_cm_vel")
+                cm = False
             target, tau_dt = variable_dt
             t = C.c_double(float(dt) if self.step_count == 0 or self.time_step is None else self.time_step)
             L.check(lib.tgx_variable_time_step(self.ctx.h, int(self.step_count == 0), float(target), float(tau_dt),
@@ -147,7 +152,7 @@ class MD:
         bussi = self.thermostat == "bussi" and T0 is not None
         if self.fused and P0 is None and scale_box is None and not bussi and (T0 is None or T0 > 0.0):
             ekc, Tc, rb = C.c_double(), C.c_double(), C.c_int()
-            L.check(lib.tgx_md_advance(self.ctx.h, float(dt), int(self.step_count == 0),
+            L.check(lib.tgx_md_advance(self.ctx.h, float(dt), int(self.step_count == 0), int(cm),
                                        float(T0) if T0 is not None else 0.0, float(tau), self.buffer, C.byref(ekc),
                                        C.byref(Tc), C.byref(rb)), "md_advance")
             ek, T = ekc.value, Tc.value
@@ -156,6 +161,8 @@ class MD:
                 self.rebuild = bool(rb.value)
             self.step_count += 1
             return self.energy, ek, T
+        if cm:
+            L.check(lib.tgx_remove_cm_vel(self.ctx.h), "remove_cm_vel")
         L.check(lib.tgx_velocity_verlet(self.ctx.h, float(dt), int(self.step_count == 0)), "velocity_verlet")
         ek, T = self.kinetic()
         self.pressure, self.pressure_tensor = self.pressure_of(T)
