@@ -5,7 +5,9 @@ get_gap_soap -> soap_backend_end on the 216-atom carbon box, with the sparse set
 read from the reference's alphas / desc_sparse file format. Results must equal
 the committed fixture (energy 1e-8 relative, forces 1e-6 eV/Å). A second host
 (tgx_pair_example) creates and evaluates the distance_2b, angle_3b and core_pot
-families through the same shim, checked against the oracle."""
+families through the same shim, checked against the oracle. A third
+(tgx_md_example) runs the device MD loop through the shim's MD bindings and
+must match the Python driver making the same calls."""
 import os
 import pathlib
 import subprocess
@@ -121,3 +123,54 @@ def test_fortran_pair_families_match_oracle(tmp_path):
         assert np.abs(e - Eo).max() <= 1e-12 * s, kind
         assert np.abs(F - Fo).max() <= 1e-11 * max(s, np.abs(Fo).max()), kind
         assert np.abs(W - Wo).max() <= 1e-11 * max(s, np.abs(Wo).max()), kind
+
+
+MD_EXE = ROOT / "turbogap_amd" / "fortran" / "_build" / "tgx_md_example"
+
+
+def test_fortran_md_loop_matches_python_driver(tmp_path):
+    """Five NVT steps of the carbon box driven from Fortran through the device
+    MD bindings (md_build_list, md_forces, remove_cm_vel_device,
+    velocity_verlet_device, md_kinetic, md_pressure, berendsen_thermostat_device,
+    wrap_pbc_device) against the Python driver making the same library calls in
+    the same order: identical energies, temperatures, pressures, positions and
+    velocities."""
+    from turbogap_amd import Context
+    from turbogap_amd import structures as S
+    from turbogap_amd.md import MD, maxwell_velocities
+
+    if not MD_EXE.exists():
+        pytest.fail(f"{MD_EXE} not built (make -C turbogap_amd/fortran)")
+    _write_inputs(tmp_path)
+    sysm, h, Qs, al = H.carbon216(200)
+    mass = S.masses_for(sysm.species)
+    v0 = maxwell_velocities(mass, 900.0, seed=12)
+    with open(tmp_path / "md_state.dat", "w") as f:
+        for v, m in zip(v0, mass):
+            f.write(f"{v[0]:.17g} {v[1]:.17g} {v[2]:.17g} {m:.17g}\n")
+    out = subprocess.run([str(MD_EXE)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    steps, atoms = [], []
+    for line in out.stdout.splitlines():
+        k, *v = line.split()
+        if k == "step":
+            steps.append([float(x) for x in v[1:]])
+        elif k == "atom":
+            atoms.append([float(x) for x in v])
+    steps, atoms = np.array(steps), np.array(atoms)
+    assert steps.shape == (5, 3) and atoms.shape == (sysm.n, 6)
+    ctx = Context(0)
+    try:
+        ctx.set_system(sysm.positions, sysm.species, sysm.box)
+        ctx.soap_model(h, Qs, al)
+        md = MD(ctx, mass, v0, 3.7, "fp64_adjoint", fused=False)
+        ref = []
+        for _ in range(5):
+            E, ek, T = md.step(1.0, T0=600.0, tau=50.0)
+            ref.append([E, T, md.pressure])
+        x, v = md.state()
+    finally:
+        ctx.close()
+    np.testing.assert_allclose(steps, np.array(ref), rtol=1e-15, atol=0)
+    np.testing.assert_allclose(atoms[:, :3], x, rtol=1e-15, atol=1e-15)
+    np.testing.assert_allclose(atoms[:, 3:], v, rtol=1e-15, atol=1e-18)

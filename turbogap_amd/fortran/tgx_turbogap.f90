@@ -16,6 +16,14 @@
 !                                             add_3b_ / add_core_pot_contribution
 !                                             (src/gap_backend_cpu.f90:79-267)
 !
+!   The device MD loop of src/turbogap_md.f90:160-470 on resident state
+!   (positions, velocities and forces stay on the device between steps):
+!   md_state_init / md_state_get, md_build_list / md_refresh_list, md_forces,
+!   wrap_pbc_device, remove_cm_vel_device, variable_time_step_device,
+!   velocity_verlet_device, gradient_descent_device, md_kinetic, md_pressure,
+!   box_scaling_device, berendsen_barostat_device, berendsen_thermostat_device,
+!   bussi_thermostat_device, md_displacement       src/md.f90:95-1128
+!
 ! Errors: a non-zero status prints tgx_last_error() and stops with code 1, the
 ! behaviour of turbogap_abort (src/error.f90:48-61).
 module tgx_turbogap
@@ -37,7 +45,11 @@ module tgx_turbogap
 
    public :: gpu_context_init, gpu_context_finalize, build_neighbors_list, soap_model_from_files, &
              soap_backend_begin, soap_backend_end, get_gap_soap, get_soap_energy_and_forces, tgx_check, &
-             tgx_distance_2b_model, tgx_angle_3b_model, tgx_core_pot_model, add_pair_contribution
+             tgx_distance_2b_model, tgx_angle_3b_model, tgx_core_pot_model, add_pair_contribution, &
+             md_state_init, md_state_get, md_build_list, md_refresh_list, md_forces, wrap_pbc_device, &
+             remove_cm_vel_device, variable_time_step_device, velocity_verlet_device, gradient_descent_device, &
+             md_kinetic, md_pressure, box_scaling_device, berendsen_barostat_device, berendsen_thermostat_device, &
+             bussi_thermostat_device, md_displacement
 
    interface
       integer(c_int) function c_tgx_init(device, ctx_out) bind(C, name="tgx_init")
@@ -61,9 +73,9 @@ module tgx_turbogap
       end function
       integer(c_int) function c_tgx_build(c, rcut, indices, do_list, npairs) bind(C, name="tgx_build_neighbors_list")
          import :: c_int, c_ptr, c_double, c_int64_t
-         type(c_ptr), value :: c
+         type(c_ptr), value :: c, do_list  ! do_list: int[n_sites], or null for every site
          real(c_double), value :: rcut
-         integer(c_int) :: indices(*), do_list(*)
+         integer(c_int) :: indices(*)
          integer(c_int64_t) :: npairs
       end function
       integer(c_int) function c_tgx_set_positions(c, pos) bind(C, name="tgx_set_positions")
@@ -152,6 +164,101 @@ module tgx_turbogap
          integer(c_int), value :: mid
          real(c_double) :: en(*), f(*), vir(*)
       end function
+      integer(c_int) function c_md_set_state(c, v, m) bind(C, name="tgx_md_set_state")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c
+         real(c_double), intent(in) :: v(*), m(*)
+      end function
+      integer(c_int) function c_md_set_fixed(c, f) bind(C, name="tgx_md_set_fixed")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: c
+         integer(c_int), intent(in) :: f(*)
+      end function
+      integer(c_int) function c_md_get_state(c, x, v) bind(C, name="tgx_md_get_state")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c
+         real(c_double), intent(out) :: x(*), v(*)
+      end function
+      integer(c_int) function c_compute(c, path, e, vir) bind(C, name="tgx_compute")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c
+         integer(c_int), value :: path
+         real(c_double), intent(out) :: e, vir(*)
+      end function
+      integer(c_int) function c_wrap(c) bind(C, name="tgx_wrap_pbc")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: c
+      end function
+      integer(c_int) function c_remove_cm(c) bind(C, name="tgx_remove_cm_vel")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: c
+      end function
+      integer(c_int) function c_vdt(c, init, target, tau, dt0, dt) bind(C, name="tgx_variable_time_step")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c
+         integer(c_int), value :: init
+         real(c_double), value :: target, tau, dt0
+         real(c_double), intent(inout) :: dt
+      end function
+      integer(c_int) function c_vv(c, dt, first) bind(C, name="tgx_velocity_verlet")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c
+         real(c_double), value :: dt
+         integer(c_int), value :: first
+      end function
+      integer(c_int) function c_gd(c, e, step, first, gamma, restored) bind(C, name="tgx_gradient_descent")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c
+         real(c_double), value :: e, step
+         integer(c_int), value :: first
+         real(c_double), intent(out) :: gamma
+         integer(c_int), intent(out) :: restored
+      end function
+      integer(c_int) function c_kinetic(c, ek, t) bind(C, name="tgx_kinetic")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c
+         real(c_double), intent(out) :: ek, t
+      end function
+      integer(c_int) function c_pressure(c, vir, t, p, pt) bind(C, name="tgx_md_pressure")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c
+         real(c_double), intent(in) :: vir(*)
+         real(c_double), value :: t
+         real(c_double), intent(out) :: p, pt(*)
+      end function
+      integer(c_int) function c_box_scaling(c, g, istep, nsteps, box) bind(C, name="tgx_box_scaling")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c
+         real(c_double), intent(in) :: g(*)
+         integer(c_int), value :: istep, nsteps
+         real(c_double), intent(out) :: box(*)
+      end function
+      integer(c_int) function c_barostat(c, p0, p, sym, tau, gamma, dt, box) bind(C, name="tgx_berendsen_barostat")
+         import :: c_int, c_ptr, c_double, c_char
+         type(c_ptr), value :: c
+         real(c_double), value :: p0, tau, gamma, dt
+         real(c_double), intent(in) :: p(*)
+         character(kind=c_char), intent(in) :: sym(*)
+         real(c_double), intent(out) :: box(*)
+      end function
+      integer(c_int) function c_berendsen(c, t0, t, tau, dt) bind(C, name="tgx_berendsen")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c
+         real(c_double), value :: t0, t, tau, dt
+      end function
+      integer(c_int) function c_bussi(c, t0, tau, dt, g, chi2, ek) bind(C, name="tgx_bussi_thermostat")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c
+         real(c_double), value :: t0, tau, dt, g, chi2
+         real(c_double), intent(out) :: ek
+      end function
+      integer(c_int) function c_displacement(c, buffer, reset, rebuild) bind(C, name="tgx_md_displacement")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c
+         real(c_double), value :: buffer
+         integer(c_int), value :: reset
+         integer(c_int), intent(out) :: rebuild
+      end function
       integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
          import :: c_size_t, c_ptr
          type(c_ptr), value :: s
@@ -202,7 +309,8 @@ contains
       integer, allocatable, intent(inout) :: n_neigh(:), neighbors_list(:), neighbor_species(:)
       real(dp) :: box(9)
       integer(c_int) :: ind(3)
-      integer(c_int), allocatable :: spec(:), dl(:)
+      integer(c_int), allocatable :: spec(:)
+      integer(c_int), allocatable, target :: dl(:)
       integer(c_int64_t) :: np
       integer(8) :: t0, t1, rate
       if (do_timing) call system_clock(t0, rate)
@@ -221,7 +329,7 @@ contains
       dl = merge(1_c_int, 0_c_int, do_list(1:n_sites))
       call tgx_check(c_tgx_set_system(ctx, int(n_sites, c_int), int(size(positions, 2), c_int), positions, spec, box), &
                      "build_neighbors_list")
-      call tgx_check(c_tgx_build(ctx, rcut_max, ind, dl, np), "build_neighbors_list")
+      call tgx_check(c_tgx_build(ctx, rcut_max, ind, c_loc(dl), np), "build_neighbors_list")
       n_atom_pairs = int(np)
       if (allocated(rjs)) deallocate (rjs)
       if (allocated(thetas)) deallocate (thetas)
@@ -419,6 +527,135 @@ contains
       energies = energies + e
       forces = forces + f
       virial = virial + w
+   end subroutine
+
+   ! ---- device MD (src/turbogap_md.f90:160-470) ----------------------------
+   ! The system is the one of the last build_neighbors_list; velocities(3, n) in
+   ! A/fs, masses(n) in eV fs^2/A^2, fix_atom(3, n) as velocity_verlet takes it.
+   subroutine md_state_init(velocities, masses, fix_atom)
+      real(dp), intent(in) :: velocities(:, :), masses(:)
+      logical, intent(in), optional :: fix_atom(:, :)
+      integer(c_int), allocatable :: f(:, :)
+      call tgx_check(c_md_set_state(ctx, velocities, masses), "md_state_init")
+      if (present(fix_atom)) then
+         allocate (f(size(fix_atom, 1), size(fix_atom, 2)))
+         f = merge(1_c_int, 0_c_int, fix_atom)
+         call tgx_check(c_md_set_fixed(ctx, f), "md_state_init")
+      end if
+   end subroutine
+
+   subroutine md_state_get(positions, velocities)
+      real(dp), intent(out) :: positions(:, :), velocities(:, :)
+      call tgx_check(c_md_get_state(ctx, positions, velocities), "md_state_get")
+   end subroutine
+
+   ! A resident list of every site within rcut_max, nothing copied to the host.
+   subroutine md_build_list(rcut_max)
+      real(dp), intent(in) :: rcut_max
+      integer(c_int) :: ind(3)
+      integer(c_int64_t) :: np
+      ind = 1_c_int
+      call tgx_check(c_tgx_build(ctx, rcut_max, ind, c_null_ptr, np), "md_build_list")
+   end subroutine
+
+   subroutine md_refresh_list()
+      call tgx_check(c_tgx_refresh(ctx), "md_refresh_list")
+   end subroutine
+
+   ! Energies, forces and virial of every block on the resident list; the
+   ! forces stay on the device for the integrator.
+   subroutine md_forces(path, energy, virial)
+      integer, intent(in) :: path
+      real(dp), intent(out) :: energy, virial(3, 3)
+      call tgx_check(c_compute(ctx, int(path, c_int), energy, virial), "md_forces")
+   end subroutine
+
+   subroutine wrap_pbc_device()
+      call tgx_check(c_wrap(ctx), "wrap_pbc")
+   end subroutine
+
+   subroutine remove_cm_vel_device()
+      call tgx_check(c_remove_cm(ctx), "remove_cm_vel")
+   end subroutine
+
+   subroutine variable_time_step_device(init, target_pos_step, tau_dt, dt0, dt)
+      logical, intent(in) :: init
+      real(dp), intent(in) :: target_pos_step, tau_dt, dt0
+      real(dp), intent(inout) :: dt
+      call tgx_check(c_vdt(ctx, merge(1_c_int, 0_c_int, init), target_pos_step, tau_dt, dt0, dt), &
+                     "variable_time_step")
+   end subroutine
+
+   subroutine velocity_verlet_device(dt, first_step)
+      real(dp), intent(in) :: dt
+      logical, intent(in) :: first_step
+      call tgx_check(c_vv(ctx, dt, merge(1_c_int, 0_c_int, first_step)), "velocity_verlet")
+   end subroutine
+
+   subroutine gradient_descent_device(energy, max_opt_step, first_step, gamma, restored)
+      real(dp), intent(in) :: energy, max_opt_step
+      logical, intent(in) :: first_step
+      real(dp), intent(out) :: gamma
+      logical, intent(out) :: restored
+      integer(c_int) :: r
+      call tgx_check(c_gd(ctx, energy, max_opt_step, merge(1_c_int, 0_c_int, first_step), gamma, r), &
+                     "gradient_descent")
+      restored = r /= 0
+   end subroutine
+
+   subroutine md_kinetic(e_kinetic, instant_temp)
+      real(dp), intent(out) :: e_kinetic, instant_temp
+      call tgx_check(c_kinetic(ctx, e_kinetic, instant_temp), "md_kinetic")
+   end subroutine
+
+   subroutine md_pressure(virial, instant_temp, instant_pressure, instant_pressure_tensor)
+      real(dp), intent(in) :: virial(3, 3), instant_temp
+      real(dp), intent(out) :: instant_pressure, instant_pressure_tensor(3, 3)
+      call tgx_check(c_pressure(ctx, virial, instant_temp, instant_pressure, instant_pressure_tensor), "md_pressure")
+   end subroutine
+
+   ! box_scaling_factor(3, 3) as the reference holds it; the box vectors are updated.
+   subroutine box_scaling_device(box_scaling_factor, md_istep, md_nsteps, a_box, b_box, c_box)
+      real(dp), intent(in) :: box_scaling_factor(3, 3)
+      integer, intent(in) :: md_istep, md_nsteps
+      real(dp), intent(inout) :: a_box(3), b_box(3), c_box(3)
+      real(dp) :: g(3, 3), box(3, 3)
+      g = transpose(box_scaling_factor)  ! the C ABI reads it row by row
+      call tgx_check(c_box_scaling(ctx, g, int(md_istep, c_int), int(md_nsteps, c_int), box), "box_scaling")
+      a_box = box(:, 1); b_box = box(:, 2); c_box = box(:, 3)
+   end subroutine
+
+   subroutine berendsen_barostat_device(p0, instant_pressure_tensor, barostat_sym, tau_p, gamma_p, dt, a_box, &
+                                        b_box, c_box)
+      real(dp), intent(in) :: p0, instant_pressure_tensor(3, 3), tau_p, gamma_p, dt
+      character(len=*), intent(in) :: barostat_sym
+      real(dp), intent(inout) :: a_box(3), b_box(3), c_box(3)
+      real(dp) :: box(3, 3)
+      call tgx_check(c_barostat(ctx, p0, instant_pressure_tensor, trim(barostat_sym)//c_null_char, tau_p, gamma_p, &
+                                dt, box), "berendsen_barostat")
+      a_box = box(:, 1); b_box = box(:, 2); c_box = box(:, 3)
+   end subroutine
+
+   subroutine berendsen_thermostat_device(target_temp, instant_temp, tau_t, dt)
+      real(dp), intent(in) :: target_temp, instant_temp, tau_t, dt
+      call tgx_check(c_berendsen(ctx, target_temp, instant_temp, tau_t, dt), "berendsen_thermostat")
+   end subroutine
+
+   ! gauss and chi2 are resamplekin's gasdev() and sumnoises(3 n_sites - 4).
+   subroutine bussi_thermostat_device(target_temp, tau_t, dt, gauss, chi2)
+      real(dp), intent(in) :: target_temp, tau_t, dt, gauss, chi2
+      real(dp) :: ek
+      call tgx_check(c_bussi(ctx, target_temp, tau_t, dt, gauss, chi2, ek), "bussi_thermostat")
+   end subroutine
+
+   ! The neighbors_buffer test (src/turbogap_md.f90:451-471): reset after a build.
+   subroutine md_displacement(neighbors_buffer, reset, rebuild_neighbors_list)
+      real(dp), intent(in) :: neighbors_buffer
+      logical, intent(in) :: reset
+      logical, intent(out) :: rebuild_neighbors_list
+      integer(c_int) :: r
+      call tgx_check(c_displacement(ctx, neighbors_buffer, merge(1_c_int, 0_c_int, reset), r), "md_displacement")
+      rebuild_neighbors_list = r /= 0
    end subroutine
 
 end module tgx_turbogap
