@@ -247,7 +247,8 @@ def test_variable_time_step_oracle():
 def test_gpu_variable_time_step_matches_oracle(ctx):
     """The library's search on the device velocities and forces of the carbon
     box against the oracle, from a step that is too large and from one that is
-    too small, at init and relaxed: identical steps (the comparisons with the
+    too small, at init and relaxed, and from one so small that the search takes
+    several launches of candidates: identical steps (the comparisons with the
     target are exact on both sides). An MD run with a variable step keeps it
     under its ceiling."""
     sysm, mass, v0 = _md_system(ctx, T=1500.0)
@@ -256,7 +257,7 @@ def test_gpu_variable_time_step_matches_oracle(ctx):
     _, F = ctx.results()
     lib = L.load()
     for init in (1, 0):
-        for dt_start in (0.05, 3.0):
+        for dt_start in (0.05, 3.0, 1e-3):  # 1e-3 fs: about 600 steps of 1%, three launches of 256
             t = C.c_double(dt_start)
             L.check(lib.tgx_variable_time_step(ctx.h, init, 0.02, 20.0, 2.5, C.byref(t)), "variable_time_step")
             to = MDO.variable_time_step(bool(init), v0, F, mass, 0.02, 20.0, 2.5, dt_start)
