@@ -206,6 +206,9 @@ extern "C" int tgx_set_system(tgx_ctx* ctx, int n_sites, int n_sites_sc, const d
   TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   ctx->have_list = false;
   ctx->md_box0_set = false;
+  // a new system starts a new descent (the accepted step of the last one is kept)
+  ctx->gd_started = false;
+  ctx->gd_backtracking = false;
   return TGX_OK;
 }
 
