@@ -211,6 +211,10 @@ def test_gpu_gradient_descent_matches_oracle(ctx, forced):
         energies.append(E)
     assert energies[-1] < energies[0]
     assert restores[1] == forced and not any(restores[2:])
+    # a new system needs a first step again
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    g, r = C.c_double(), C.c_int()
+    assert lib.tgx_gradient_descent(ctx.h, 0.0, max_opt, 0, C.byref(g), C.byref(r)) != 0
 
 
 def test_variable_time_step_oracle():
