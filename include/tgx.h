@@ -28,6 +28,8 @@
  *   tgx_box_scaling                     src/md.f90:421-458          box_scaling (scale_box, box_scaling_factor)
  *   tgx_gradient_descent                src/md.f90:540-664          gradient_descent (optimize = "gd")
  *   tgx_variable_time_step              src/md.f90:464-535, src/turbogap_md.f90:178-181 variable_time_step
+ *   tgx_adaptive_time_step              src/stopping/adaptive_time.f90:46-164, src/turbogap_md.f90:204-210
+ *                                       variable_time_step_adaptive (adaptive_time)
  *   tgx_bussi_thermostat                src/turbogap_md.f90:438-449, src/third_party/bussi_thermostat/resamplekin.f90:50-84
  *                                       thermostat = "bussi" (resamplekin on the device E_kin)
  *   tgx_distance_2b_create              src/read_files.f90:722-742,5238-5305  a distance_2b block and its sparse set
@@ -307,6 +309,16 @@ int tgx_box_scaling(tgx_ctx* ctx, const double* gamma, int i_step, int n_steps, 
  * then *dt = min(new_dt, dt0) when init, else min(dt tau_dt/(tau_dt + dt - new_dt), dt0).
  * An error when 16384 steps of 1% do not reach the target (no motion at all). */
 int tgx_variable_time_step(tgx_ctx* ctx, int init, double target_pos_step, double tau_dt, double dt0, double* dt);
+/* variable_time_step_adaptive from the device velocities and forces: per atom the
+ * smallest of xmax/|v|, sqrt(2 xmax m/|F|) and (emax > 0) emax/(|v||F|), an atom
+ * without velocity or force taking those of the last atom before it that has one
+ * (else the starting step: dt0 when init, else *dt), scaled down by xmax/d when the
+ * displacement d = |v dt + F/(2m) dt^2| exceeds xmax; then *dt = min(smallest, tmax)
+ * (the reference's tmin bound is overwritten, so tmin only has to be valid), and
+ * min(that, dt0) when init. Errors as the reference's checks: xmax, tmin, tmax > 0,
+ * tmax > tmin. */
+int tgx_adaptive_time_step(tgx_ctx* ctx, int init, double tmin, double tmax, double xmax, double emax, double dt0,
+                           double* dt);
 /* optimize = "gd": one relaxation step in place of velocity Verlet, from the device
  * forces and the total energy at the current positions. Velocities are zeroed. On
  * first_step the positions and forces are kept and the step is

@@ -29,6 +29,10 @@ device MD; the product path never imports it.
     while below it, whichever held at dt), then min(new_dt, dt0) at init, else
     the relaxed min(dt tau_dt/(tau_dt + dt - new_dt), dt0).
   * remove_cm_vel follows src/md.f90:316-345.
+  * adaptive_time_step follows variable_time_step_adaptive
+    (src/stopping/adaptive_time.f90:46-164), its loop over atoms as written:
+    the velocity and force times of the last atom that had them carry over,
+    and min(dtmin, tmax) overwrites max(dtmin, tmin).
   * target_temperature follows get_target_temp (src/md.f90:1094-1128), the
     t_beg -> t_end ramp with its t_hold plateaus.
 Arrays are (n, 3) here; the reference's are (3, n).
@@ -191,3 +195,30 @@ def remove_cm_vel(velocities, masses):
     v = np.array(velocities, dtype=np.float64)
     m = np.asarray(masses, dtype=np.float64)
     return v - (m[:, None] * v).sum(0) / m.sum()
+
+
+def adaptive_time_step(init, velocities, forces, masses, tmin, tmax, xmax, emax, dt0, dt):
+    v = np.asarray(velocities, dtype=np.float64)
+    F = np.asarray(forces, dtype=np.float64)
+    m = np.asarray(masses, dtype=np.float64)
+    dtmin = float(np.float32(1.0e20))
+    dtv = dtf = dt0 if init else dt
+    for i in range(v.shape[0]):
+        vsq = (v[i, 0] * v[i, 0] + v[i, 1] * v[i, 1]) + v[i, 2] * v[i, 2]
+        fsq = (F[i, 0] * F[i, 0] + F[i, 1] * F[i, 1]) + F[i, 2] * F[i, 2]
+        if vsq > 0.0:
+            dtv = xmax / np.sqrt(vsq)
+        if fsq > 0.0:
+            dtf = np.sqrt(2.0 * xmax * m[i] / np.sqrt(fsq))
+        t = min(dtv, dtf)
+        if emax > 0.0 and vsq * fsq > 0.0:
+            t = min(t, emax / np.sqrt(vsq * fsq))
+        x = v[i] * t + 0.5 * F[i] / m[i] * (t * t)
+        d = np.sqrt((x[0] * x[0] + x[1] * x[1]) + x[2] * x[2])
+        if d > xmax:
+            t = t * xmax / d
+        dtmin = min(dtmin, t)
+    out = min(dtmin, tmax)
+    if init:
+        out = min(out, dt0)
+    return float(out)

@@ -1,8 +1,8 @@
 """The Bussi thermostat, the target-temperature ramp, gradient-descent
-relaxation and the variable time step of the device MD
-(src/turbogap_md.f90:178-181, 224-228, 430-449,
+relaxation and the variable and adaptive time steps of the device MD
+(src/turbogap_md.f90:178-181, 204-210, 224-228, 430-449,
 src/third_party/bussi_thermostat/resamplekin.f90:50-84, src/md.f90:464-664,
-1094-1128).
+1094-1128, src/stopping/adaptive_time.f90:46-164).
 
 CPU (oracle pins):
   * resample limits: tau/dt huge keeps K (the memory factor is 1); tau/dt <= 0.1
@@ -17,13 +17,16 @@ CPU (oracle pins):
     minimum, the accepted step reused by a later relaxation, the minimum-image
     convention across a wrap, fixed components;
   * the variable time step against closed forms (velocities only, forces only)
-    and the relaxed update by hand.
+    and the relaxed update by hand;
+  * the adaptive time step against closed forms and a hand case of the carry
+    over atoms at rest.
 GPU (parity): the library's Bussi step against the oracle's resample of the
 device kinetic energy with the same draws: velocities to 1e-14 relative, the
 new kinetic energy equal to K'; zero kinetic energy zeroes the velocities;
 a short Bussi NVT run; six gradient-descent steps of the carbon box against the
 oracle fed the library's energies and forces; the variable time step equal to
-the oracle's, bit for bit.
+the oracle's, bit for bit; the adaptive step over seven blocks of atoms equal
+to the oracle's loop.
 """
 import ctypes as C
 
@@ -270,4 +273,72 @@ def test_gpu_variable_time_step_matches_oracle(ctx):
     md = MD(ctx, mass, v0, 3.7, "fp64_adjoint", neighbors_buffer=0.3)
     for _ in range(4):
         E, ek, T = md.step(1.0, variable_dt=(0.02, 20.0))
+        assert np.isfinite(E) and 0.0 < md.time_step <= 1.0
+
+
+def test_adaptive_time_step_oracle():
+    """variable_time_step_adaptive against closed forms: velocities only gives
+    xmax/|v|max, forces only gives min sqrt(2 xmax m/|F|), the emax criterion
+    emax/(|v||F|); an atom without velocity or force takes the last moving
+    atom's times (not the starting step); tmin does not bound the step from
+    below and tmax does from above; init caps at dt0."""
+    rng = np.random.default_rng(5)
+    n, xmax = 50, 0.01
+    v = rng.normal(size=(n, 3)) * 0.02
+    m = rng.uniform(10.0, 30.0, n)
+    F = rng.normal(size=(n, 3)) * 0.5
+    Z = np.zeros((n, 3))
+    vmax = np.sqrt((v ** 2).sum(1)).max()
+    t = MDO.adaptive_time_step(True, v, Z, m, 1e-6, 10.0, xmax, 0.0, 5.0, 5.0)
+    assert abs(t - xmax / vmax) <= 1e-14 * t
+    t = MDO.adaptive_time_step(True, Z, F, m, 1e-6, 10.0, xmax, 0.0, 5.0, 5.0)
+    expect = np.sqrt(2 * xmax * m / np.sqrt((F ** 2).sum(1))).min()
+    assert abs(t - expect) <= 1e-14 * t
+    v1, F1 = v[:1] * 0 + [[0.01, 0.0, 0.0]], np.array([[0.0, 2.0, 0.0]])
+    t = MDO.adaptive_time_step(True, v1, F1, m[:1], 1e-6, 10.0, 1.0, 1e-3, 5.0, 5.0)
+    assert abs(t - 1e-3 / (0.01 * 2.0)) <= 1e-14 * t  # emax/(|v||F|) = 0.05 fs, displacement far below xmax
+    # the second atom neither moves nor feels a force: it takes the first one's times
+    v2 = np.array([[0.02, 0.0, 0.0], [0.0, 0.0, 0.0]])
+    F2 = np.array([[0.0, 1e-9, 0.0], [0.0, 0.0, 0.0]])
+    alone = MDO.adaptive_time_step(False, v2[:1], F2[:1], m[:1], 1e-6, 10.0, xmax, 0.0, 5.0, 1e-4)
+    both = MDO.adaptive_time_step(False, v2, F2, m[:2], 1e-6, 10.0, xmax, 0.0, 5.0, 1e-4)
+    assert both == alone and both > 0.1  # not the 1e-4 starting step
+    # tmin is overwritten, tmax caps, init caps at dt0
+    assert MDO.adaptive_time_step(True, v, Z, m, 10.0, 20.0, xmax, 0.0, 50.0, 5.0) < 10.0
+    assert MDO.adaptive_time_step(True, v * 1e-6, Z, m, 1e-3, 2.0, xmax, 0.0, 50.0, 5.0) == 2.0
+    assert MDO.adaptive_time_step(True, v * 1e-6, Z, m, 1e-3, 2.0, xmax, 0.0, 1.5, 5.0) == 1.5
+
+
+@pytest.mark.gpu
+def test_gpu_adaptive_time_step_matches_oracle(ctx):
+    """The carbon box repeated 2 x 2 x 2 (1,728 atoms, seven blocks of 256)
+    with a run of 300 atoms at rest across a block boundary and every seventh
+    atom at rest: the library's step equals the oracle's loop over atoms to
+    1e-14, at init and relaxed, with and without the emax criterion. A short MD
+    run with the adaptive step stays within tmax."""
+    sysm, h, Qs, al = H.carbon216(300)
+    L1 = sysm.box[0, 0]
+    shifts = np.array([[a, b, c] for a in (0, 1) for b in (0, 1) for c in (0, 1)]) * L1
+    pos = (sysm.positions[None, :, :] + shifts[:, None, :]).reshape(-1, 3)
+    spec = np.tile(sysm.species, 8)
+    ctx.set_system(pos, spec, sysm.box * 2)
+    ctx.soap_model(h, Qs, al)
+    mass = S.masses_for(spec)
+    v0 = maxwell_velocities(mass, 1200.0, seed=2)
+    v0[200:500] = 0.0
+    v0[::7] = 0.0
+    md = MD(ctx, mass, v0, 3.7, "fp64_adjoint")
+    md.forces()
+    _, F = ctx.results()
+    lib = L.load()
+    for init in (1, 0):
+        for emax in (0.0, 0.05):
+            t = C.c_double(0.7)
+            L.check(lib.tgx_adaptive_time_step(ctx.h, init, 1e-3, 1.0, 0.01, emax, 2.0, C.byref(t)),
+                    "adaptive_time_step")
+            to = MDO.adaptive_time_step(bool(init), v0, F, mass, 1e-3, 1.0, 0.01, emax, 2.0, 0.7)
+            assert abs(t.value - to) <= 1e-14 * to, (init, emax, t.value, to)
+    md = MD(ctx, mass, maxwell_velocities(mass, 600.0, seed=3), 3.7, "fp64_adjoint", neighbors_buffer=0.3)
+    for _ in range(3):
+        E, ek, T = md.step(1.0, adaptive_dt=(1e-3, 1.0, 0.01, 10.0, 1))
         assert np.isfinite(E) and 0.0 < md.time_step <= 1.0

@@ -122,6 +122,106 @@ __global__ __launch_bounds__(256) void k_dt_candidates(int n, const double* __re
   if (threadIdx.x == 0) d2max[blockIdx.x] = red[0];
 }
 
+// variable_time_step_adaptive. Its loop over atoms keeps the last velocity and
+// force time (x/v, sqrt(2 x m/|F|)) when an atom has no velocity or no force, so
+// atom i uses those of the last atom j <= i that had one. Pass 1: per block of
+// 256 atoms, the last such index (or -1). Pass 2: each block takes the carry
+// from the blocks before it, scans its own atoms (prefix max of the index), and
+// writes the block's smallest step. Arithmetic as the reference writes it (no
+// fused multiply-adds).
+__device__ __forceinline__ double adt_vsq(const double* v, int i) {
+#pragma clang fp contract(off)
+  const double a = v[3 * (size_t)i], b = v[3 * (size_t)i + 1], c = v[3 * (size_t)i + 2];
+  return a * a + b * b + c * c;
+}
+
+__global__ __launch_bounds__(256) void k_adt_last(int n, const double* __restrict__ v, const double* __restrict__ F,
+                                                  int* __restrict__ last) {
+  __shared__ int sv[256], sf[256];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  int lv = -1, lf = -1;
+  if (i < n) {
+    if (adt_vsq(v, i) > 0.0) lv = i;
+    if (adt_vsq(F, i) > 0.0) lf = i;
+  }
+  sv[threadIdx.x] = lv;
+  sf[threadIdx.x] = lf;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sv[threadIdx.x] = max(sv[threadIdx.x], sv[threadIdx.x + w]);
+      sf[threadIdx.x] = max(sf[threadIdx.x], sf[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    last[2 * blockIdx.x] = sv[0];
+    last[2 * blockIdx.x + 1] = sf[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_adt_min(int n, const double* __restrict__ v, const double* __restrict__ F,
+                                                 const double* __restrict__ mass, const int* __restrict__ last,
+                                                 double xmax, double emax, double dt_init,
+                                                 double* __restrict__ part) {
+#pragma clang fp contract(off)
+  __shared__ int sv[256], sf[256];
+  __shared__ double sm[256];
+  __shared__ int cv, cf;
+  if (threadIdx.x == 0) {
+    int a = -1, b = -1;
+    for (int k = 0; k < (int)blockIdx.x; ++k) {
+      a = max(a, last[2 * k]);
+      b = max(b, last[2 * k + 1]);
+    }
+    cv = a;
+    cf = b;
+  }
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double vsq = 0.0, fsq = 0.0;
+  if (i < n) {
+    vsq = adt_vsq(v, i);
+    fsq = adt_vsq(F, i);
+  }
+  sv[threadIdx.x] = (i < n && vsq > 0.0) ? i : -1;
+  sf[threadIdx.x] = (i < n && fsq > 0.0) ? i : -1;
+  __syncthreads();
+  // inclusive prefix max over the block (Hillis-Steele)
+  for (int o = 1; o < 256; o <<= 1) {
+    const int a = (int)threadIdx.x >= o ? sv[threadIdx.x - o] : -1;
+    const int b = (int)threadIdx.x >= o ? sf[threadIdx.x - o] : -1;
+    __syncthreads();
+    sv[threadIdx.x] = max(sv[threadIdx.x], a);
+    sf[threadIdx.x] = max(sf[threadIdx.x], b);
+    __syncthreads();
+  }
+  double dt = (double)1.0e20f;
+  if (i < n) {
+    const int jv = max(sv[threadIdx.x], cv), jf = max(sf[threadIdx.x], cf);
+    const double dtv = jv >= 0 ? xmax / sqrt(adt_vsq(v, jv)) : dt_init;
+    const double dtf = jf >= 0 ? sqrt(2.0 * xmax * mass[jf] / sqrt(adt_vsq(F, jf))) : dt_init;
+    dt = fmin(dtv, dtf);
+    if (emax > 0.0 && vsq * fsq > 0.0) dt = fmin(dt, emax / sqrt(vsq * fsq));
+    const double m = mass[i];
+    double s2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const size_t j = 3 * (size_t)i + k;
+      const double x = v[j] * dt + 0.5 * F[j] / m * (dt * dt);
+      s2 = s2 + x * x;
+    }
+    const double d = sqrt(s2);
+    if (d > xmax) dt = dt * xmax / d;
+  }
+  sm[threadIdx.x] = dt;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sm[threadIdx.x] = fmin(sm[threadIdx.x], sm[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
 // x_c *= f_c for every atom (the barostat's scaling, per Cartesian component).
 __global__ void k_scale3(int n, double f0, double f1, double f2, double* __restrict__ x) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -657,6 +757,34 @@ extern "C" int tgx_variable_time_step(tgx_ctx* ctx, int init, double target_pos_
     const double relaxed = *dt * tau_dt / (tau_dt + *dt - new_dt);
     *dt = std::min(relaxed, dt0);
   }
+  return TGX_OK;
+}
+
+extern "C" int tgx_adaptive_time_step(tgx_ctx* ctx, int init, double tmin, double tmax, double xmax, double emax,
+                                      double dt0, double* dt) {
+  MD_CTX(ctx);
+  TGX_REQUIRE(dt && ctx->vel.p && ctx->mass.p && ctx->forces.p, "tgx_adaptive_time_step: no MD state or forces");
+  TGX_REQUIRE(xmax > 0.0 && tmax > 0.0 && tmin > 0.0 && tmax > tmin,
+              "tgx_adaptive_time_step: xmax, tmin and tmax must be positive and tmax > tmin");
+  const int n = ctx->n_sites;
+  const int nb = (n + 255) / 256;
+  TGX_TRY(ctx->scratch.ensure(sizeof(double) * (size_t)nb + sizeof(int) * 2 * (size_t)nb + 256));
+  double* part = ctx->scratch.as<double>();
+  int* last = reinterpret_cast<int*>(part + nb);
+  TGX_LAUNCH(ctx, "md_adaptive_dt_last", k_adt_last, dim3(nb), dim3(256), 0, n, ctx->vel.as<double>(),
+             ctx->forces.as<double>(), last);
+  TGX_LAUNCH(ctx, "md_adaptive_dt", k_adt_min, dim3(nb), dim3(256), 0, n, ctx->vel.as<double>(),
+             ctx->forces.as<double>(), ctx->mass.as<double>(), (const int*)last, xmax, emax, init ? dt0 : *dt, part);
+  std::vector<double> h(nb);
+  TGX_CHECK_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, ctx->stream));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  double dtmin = (double)1.0e20f;
+  for (double x : h) dtmin = std::min(dtmin, x);
+  // as the reference: its max(dtmin, tmin) is overwritten by min(dtmin, tmax),
+  // so tmin does not bound the step from below
+  double out = std::min(dtmin, tmax);
+  if (init) out = std::min(out, dt0);
+  *dt = out;
   return TGX_OK;
 }
 

@@ -20,6 +20,7 @@
 !   (positions, velocities and forces stay on the device between steps):
 !   md_state_init / md_state_get, md_build_list / md_refresh_list, md_forces,
 !   wrap_pbc_device, remove_cm_vel_device, variable_time_step_device,
+!   variable_time_step_adaptive_device (src/stopping/adaptive_time.f90:46-164),
 !   velocity_verlet_device, gradient_descent_device, md_kinetic, md_pressure,
 !   box_scaling_device, berendsen_barostat_device, berendsen_thermostat_device,
 !   bussi_thermostat_device, md_displacement       src/md.f90:95-1128
@@ -47,7 +48,8 @@ module tgx_turbogap
              soap_backend_begin, soap_backend_end, get_gap_soap, get_soap_energy_and_forces, tgx_check, &
              tgx_distance_2b_model, tgx_angle_3b_model, tgx_core_pot_model, add_pair_contribution, &
              md_state_init, md_state_get, md_build_list, md_refresh_list, md_forces, wrap_pbc_device, &
-             remove_cm_vel_device, variable_time_step_device, velocity_verlet_device, gradient_descent_device, &
+             remove_cm_vel_device, variable_time_step_device, variable_time_step_adaptive_device, &
+             velocity_verlet_device, gradient_descent_device, &
              md_kinetic, md_pressure, box_scaling_device, berendsen_barostat_device, berendsen_thermostat_device, &
              bussi_thermostat_device, md_displacement
 
@@ -198,6 +200,13 @@ module tgx_turbogap
          type(c_ptr), value :: c
          integer(c_int), value :: init
          real(c_double), value :: target, tau, dt0
+         real(c_double), intent(inout) :: dt
+      end function
+      integer(c_int) function c_adt(c, init, tmin, tmax, xmax, emax, dt0, dt) bind(C, name="tgx_adaptive_time_step")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c
+         integer(c_int), value :: init
+         real(c_double), value :: tmin, tmax, xmax, emax, dt0
          real(c_double), intent(inout) :: dt
       end function
       integer(c_int) function c_vv(c, dt, first) bind(C, name="tgx_velocity_verlet")
@@ -584,6 +593,14 @@ contains
       real(dp), intent(inout) :: dt
       call tgx_check(c_vdt(ctx, merge(1_c_int, 0_c_int, init), target_pos_step, tau_dt, dt0, dt), &
                      "variable_time_step")
+   end subroutine
+
+   subroutine variable_time_step_adaptive_device(init, tmin, tmax, xmax, emax, dt0, dt)
+      logical, intent(in) :: init
+      real(dp), intent(in) :: tmin, tmax, xmax, emax, dt0
+      real(dp), intent(inout) :: dt
+      call tgx_check(c_adt(ctx, merge(1_c_int, 0_c_int, init), tmin, tmax, xmax, emax, dt0, dt), &
+                     "variable_time_step_adaptive")
    end subroutine
 
    subroutine velocity_verlet_device(dt, first_step)

@@ -118,6 +118,7 @@ _SIGS = {
     "tgx_bussi_thermostat": [_P, _D, _D, _D, _D, _D, _PD],
     "tgx_gradient_descent": [_P, _D, _D, _I, _PD, _PI],
     "tgx_variable_time_step": [_P, _I, _D, _D, _D, _PD],
+    "tgx_adaptive_time_step": [_P, _I, _D, _D, _D, _D, _D, _PD],
     "tgx_md_pressure": [_P, _PD, _D, _PD, _PD],
     "tgx_berendsen_barostat": [_P, _D, _PD, C.c_char_p, _D, _D, _D, _PD],
     "tgx_distance_2b_create": [_P, _I, _I, _D, _D, _D, _D, _D, _I, _PD, _PD, _PD, _PI],

@@ -128,14 +128,16 @@ class MD:
 
     def step(self, dt: float, T0: float = None, tau: float = 100.0, path=None, P0: float = None,
              tau_p: float = 1000.0, gamma_p: float = 1.0, barostat_sym: str = "iso", scale_box=None,
-             variable_dt=None):
+             variable_dt=None, adaptive_dt=None):
         """One step: forces at x(t), velocity Verlet, E_kin, T and pressure, then
         (scale_box = (gamma (3, 3), n_steps) given) box_scaling at this step's index
         or (P0 given) the Berendsen barostat on box and positions, (T0 given) the
         Berendsen or Bussi thermostat, wrap (src/turbogap_md.f90:280-449).
         variable_dt = (target_pos_step, tau_dt): the step is chosen by
         variable_time_step with dt as its ceiling (src/turbogap_md.f90:178-181)
-        and kept in self.time_step for the next one."""
+        and kept in self.time_step for the next one. adaptive_dt = (tmin, tmax,
+        xmax, emax, interval): variable_time_step_adaptive every interval steps
+        (src/turbogap_md.f90:204-210), likewise kept."""
         lib = L.load()
         self.forces(path)
         cm = self.remove_cm_each_step  # still to do
@@ -148,6 +150,18 @@ class MD:
             L.check(lib.tgx_variable_time_step(self.ctx.h, int(self.step_count == 0), float(target), float(tau_dt),
                                                float(dt), C.byref(t)), "variable_time_step")
             self.time_step = t.value
+            dt = self.time_step
+        if adaptive_dt is not None:
+            tmin, tmax, xmax, emax, interval = adaptive_dt
+            if cm:
+                L.check(lib.tgx_remove_cm_vel(self.ctx.h), "remove_cm_vel")
+                cm = False
+            if self.step_count % int(interval) == 0 or self.time_step is None:
+                t = C.c_double(float(dt) if self.step_count == 0 or self.time_step is None else self.time_step)
+                L.check(lib.tgx_adaptive_time_step(self.ctx.h, int(self.step_count == 0), float(tmin), float(tmax),
+                                                   float(xmax), float(emax), float(dt), C.byref(t)),
+                        "adaptive_time_step")
+                self.time_step = t.value
             dt = self.time_step
         bussi = self.thermostat == "bussi" and T0 is not None
         if self.fused and P0 is None and scale_box is None and not bussi and (T0 is None or T0 > 0.0):
