@@ -10,6 +10,8 @@
  *   tgx_init / tgx_finalize             src/gpu_context.f90:41-58   gpu_context_init / _finalize
  *   tgx_memory_budget                   src/gpu_context.f90:100-162 gpu_memory_budget_init
  *   tgx_set_system                      src/turbogap.f90:871-886    positions/species/box handed to the build
+ *   tgx_number_of_unit_cells            src/neighbors.f90:175-236   number_of_unit_cells_for_given_cutoff
+ *   tgx_build_supercell                 src/read_files.f90:320-367  the supercell replication of read_xyz
  *   tgx_build_neighbors_list            src/neighbors.f90:243-605   build_neighbors_list (count phase)
  *   tgx_get_neighbors_list              src/neighbors.f90:544-591   the allocatable outputs of build_neighbors_list
  *   tgx_refresh_neighbors_list          src/neighbors.f90:554-585   build_neighbors_list, rebuild_neighbors_list = .false.
@@ -109,6 +111,18 @@ int tgx_device_name(tgx_ctx* ctx, char* buf, int buflen);
  * first n_sites atoms are the unit cell (src/read_files.f90:320-367 order). */
 int tgx_set_system(tgx_ctx* ctx, int n_sites, int n_sites_sc, const double* positions, const int* species_sc,
                    const double* box);
+/* number_of_unit_cells_for_given_cutoff (host only, no context): the repetitions
+ * along a, b, c (box = a, b, c as three 3-vectors) after which the supercell's
+ * opposite faces are at least 2 rcut apart; (1, 1, 1) when the cell already holds
+ * a cutoff sphere. */
+int tgx_number_of_unit_cells(const double* box, double rcut, int* indices);
+/* The supercell of read_xyz (host only): positions (3, n_sites) and species
+ * repeated indices[0] x indices[1] x indices[2] times, copy (i, j, k), i slowest,
+ * shifted by i a + j b + k c, into positions_sc (3, n_sites * prod(indices)) and
+ * species_sc; box_sc = (indices[0] a, indices[1] b, indices[2] c). The first
+ * n_sites atoms are the cell itself, as tgx_set_system expects. */
+int tgx_build_supercell(int n_sites, const double* positions, const int* species, const double* box,
+                        const int* indices, double* positions_sc, int* species_sc, double* box_sc);
 /* Replace positions (host pointer, same shape as in tgx_set_system). */
 int tgx_set_positions(tgx_ctx* ctx, const double* positions);
 /* Replace positions from a device pointer holding (3, n_sites_sc) doubles. */
@@ -166,11 +180,12 @@ int tgx_local_property_predict(tgx_ctx* ctx, int do_derivatives, double* V, doub
                                int64_t* n_pairs_block);
 /* get_soap_energy_and_forces for the current block, host inputs: soap(n_soap,
  * n_sites), soap_der(3, n_soap, n_pairs), neighbors_list 1-based local indices,
- * xyz(3, n_pairs); outputs energies(n_sites), forces(3, n_sites0), virial(3,3). */
+ * xyz(3, n_pairs); outputs energies(n_sites), forces(3, n_sites0), virial(3,3).
+ * delta and zeta0 must equal the block's (TGX_ERR_ARG otherwise). */
 int tgx_get_soap_energy_and_forces(tgx_ctx* ctx, int n_sites, int n_soap, int64_t n_pairs, const double* soap,
-                                   const double* soap_der, double e0, const int* n_neigh,
-                                   const int* neighbors_list, const double* xyz, int n_sites0, int do_forces,
-                                   double* energies, double* forces, double* virial);
+                                   const double* soap_der, double delta, double zeta0, double e0,
+                                   const int* n_neigh, const int* neighbors_list, const double* xyz, int n_sites0,
+                                   int do_forces, double* energies, double* forces, double* virial);
 
 /* ---- whole-step evaluation ------------------------------------------------ */
 /* E, F and virial of every model on the resident positions (neighbour list must

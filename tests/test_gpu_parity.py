@@ -10,6 +10,7 @@ import pytest
 
 from tests import helpers as H
 from turbogap_amd import structures as S
+from turbogap_amd.context import build_supercell, number_of_unit_cells
 
 pytestmark = pytest.mark.gpu
 
@@ -42,13 +43,43 @@ def test_neighbors_subset_bitexact(ctx, orc):
 
 
 def test_neighbors_small_box_supercell(ctx, orc):
+    """A box shorter than 2 rcut: the library's own supercell (indices and
+    replicas) feeds the brute-force build; the list is the oracle's bit for bit."""
     sysm = H.small_box()
-    ind = orc.number_of_unit_cells(sysm.box, 3.7)
-    assert ind.max() > 1
-    psc, ssc, bsc = orc.build_supercell(sysm.positions, sysm.species, sysm.box, ind)
+    ind = number_of_unit_cells(sysm.box, 3.7)
+    assert ind.tolist() == orc.number_of_unit_cells(sysm.box, 3.7).tolist() and ind.max() > 1
+    psc, ssc, bsc = build_supercell(sysm.positions, sysm.species, sysm.box, ind)
     ctx.set_system(psc, ssc, bsc, n_sites=sysm.n)
     g = ctx.build_neighbors_list(3.7, indices=ind)
     o = orc.build_neighbors_list(psc, ssc, bsc, 3.7, n_sites=sysm.n, indices=ind)
+    _compare_lists(g, o)
+
+
+def _triclinic(n, box, seed):
+    frac = np.random.default_rng(seed).random((n, 3))
+    return frac @ box, np.full(n, S.CARBON, np.int32)
+
+
+def test_neighbors_triclinic_bitexact(ctx, orc):
+    """Triclinic cells (get_distance's 27-image branch, src/neighbors.f90:102-156):
+    a cell that holds the cutoff sphere and a small one that needs a supercell;
+    indices, species, rjs and xyz bit-exact against the oracle."""
+    box = np.array([[9.0, 0.0, 0.0], [2.1, 8.5, 0.0], [1.3, -1.7, 9.4]])
+    pos, sp = _triclinic(60, box, 5)
+    assert number_of_unit_cells(box, 3.0).tolist() == [1, 1, 1]
+    ctx.set_system(pos, sp, box)
+    g = ctx.build_neighbors_list(3.0)
+    o = orc.build_neighbors_list(pos, sp, box, 3.0)
+    _compare_lists(g, o)
+    box = np.array([[5.2, 0.0, 0.0], [1.4, 4.9, 0.0], [-0.9, 1.1, 5.6]])
+    pos, sp = _triclinic(14, box, 6)
+    ind = number_of_unit_cells(box, 3.2)
+    assert ind.max() > 1 and ind.tolist() == orc.number_of_unit_cells(box, 3.2).tolist()
+    psc, ssc, bsc = build_supercell(pos, sp, box, ind)
+    ctx.set_system(psc, ssc, bsc, n_sites=14)
+    g = ctx.build_neighbors_list(3.2, indices=ind)
+    o = orc.build_neighbors_list(psc, ssc, bsc, 3.2, n_sites=14, indices=ind)
+    assert g["n_atom_pairs"] > 14
     _compare_lists(g, o)
 
 

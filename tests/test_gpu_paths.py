@@ -109,3 +109,27 @@ def test_fp64_paths_agree_binary_density(ctx):
     assert abs(Ep - Ea) <= 1e-12 * max(1.0, abs(Ep))
     assert np.abs(Fp - Fa).max() <= 1e-11
     assert np.abs(Wp - Wa).max() <= 1e-10
+
+
+def test_block_streams_above_two_launch_scan_limit(ctx):
+    """81,920 atoms (two central species): the block selection scans more values
+    than the two-launch scan takes, so each block falls back to the three-launch
+    scan with its own block totals. Running the blocks on streams of their own
+    gives bit-identical energies, forces and virial to the blocks in turn."""
+    _, models = H.subset(500, 1024)
+    sysm = S.binary_system(81920, seed=2003)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    ctx.build_neighbors_list(3.7, fetch=False)
+    H.load_models(ctx, models)
+    for path in ("mixed", "fp64_adjoint"):
+        out = []
+        for on in (True, False, True):
+            ctx.set_block_streams(on)
+            E, W = ctx.compute(path)
+            out.append((E, W) + ctx.results())
+        ctx.set_block_streams(True)
+        ref = out[1]
+        for o in (out[0], out[2]):
+            assert o[0] == ref[0] and np.array_equal(o[1], ref[1]), path
+            assert np.array_equal(o[2], ref[2]) and np.array_equal(o[3], ref[3]), path
+        assert np.isfinite(ref[3]).all() and ref[3].shape == (sysm.n, 3)

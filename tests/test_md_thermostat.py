@@ -311,25 +311,37 @@ def test_adaptive_time_step_oracle():
 
 @pytest.mark.gpu
 def test_gpu_adaptive_time_step_matches_oracle(ctx):
-    """The carbon box repeated 2 x 2 x 2 (1,728 atoms, seven blocks of 256)
-    with a run of 300 atoms at rest across a block boundary and every seventh
-    atom at rest: the library's step equals the oracle's loop over atoms to
-    1e-14, at init and relaxed, with and without the emax criterion. A short MD
-    run with the adaptive step stays within tmax."""
+    """The carbon box repeated 2 x 2 x 2 (1,728 atoms) under a vacuum slab
+    holding 180 isolated atoms (more than rcut apart: zero force), 1,908 atoms
+    in eight blocks of 256. Atoms at rest run across block boundaries, both in
+    the crystal (200-499) and among the isolated atoms (1750-1849, neither
+    velocity nor force), and every seventh atom is at rest, so the device takes
+    both the velocity-time and the force-time carry across blocks. The
+    library's step equals the oracle's loop over atoms to 1e-14, at init and
+    relaxed, with and without the emax criterion. A short MD run with the
+    adaptive step stays within tmax."""
     sysm, h, Qs, al = H.carbon216(300)
     L1 = sysm.box[0, 0]
     shifts = np.array([[a, b, c] for a in (0, 1) for b in (0, 1) for c in (0, 1)]) * L1
-    pos = (sysm.positions[None, :, :] + shifts[:, None, :]).reshape(-1, 3)
-    spec = np.tile(sysm.species, 8)
-    ctx.set_system(pos, spec, sysm.box * 2)
+    crystal = (sysm.positions[None, :, :] + shifts[:, None, :]).reshape(-1, 3)
+    g = np.arange(6) * (2 * L1 / 6)  # 4.30 A apart in x and y
+    gz = 2 * L1 + 4.0 + np.arange(5) * 4.3  # 4.0 A above the crystal, 4.6 A below its periodic image
+    iso = np.array([[x, y, z] for z in gz for y in g for x in g])
+    pos = np.concatenate([crystal, iso])
+    spec = np.full(pos.shape[0], sysm.species[0], np.int32)
+    box = np.diag([2 * L1, 2 * L1, 4 * L1])
+    ctx.set_system(pos, spec, box)
     ctx.soap_model(h, Qs, al)
     mass = S.masses_for(spec)
     v0 = maxwell_velocities(mass, 1200.0, seed=2)
     v0[200:500] = 0.0
+    v0[1750:1850] = 0.0
     v0[::7] = 0.0
     md = MD(ctx, mass, v0, 3.7, "fp64_adjoint")
     md.forces()
     _, F = ctx.results()
+    n_cr = crystal.shape[0]
+    assert (F[n_cr:] == 0.0).all() and (np.abs(F[:n_cr]).sum(1) > 0.0).all()  # isolated atoms feel no force
     lib = L.load()
     for init in (1, 0):
         for emax in (0.0, 0.05):

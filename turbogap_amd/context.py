@@ -5,6 +5,8 @@ src/gpu_context.f90:41-58). Its methods keep the reference's names, argument
 meaning and array conventions:
 
   build_neighbors_list       src/neighbors.f90:243-605
+  number_of_unit_cells       src/neighbors.f90:175-236 (module function, host only)
+  build_supercell            src/read_files.f90:320-367 (module function, host only)
   soap_model / backend_begin src/gap.f90:49-64 (one resident sparse set per block)
   get_gap_soap               src/gap_interface.f90:41-480
   get_soap                   soap_turbo get_soap (call shape src/gap_interface.f90:320-325)
@@ -35,6 +37,32 @@ def _f64(a, shape=None):
 
 def _i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def number_of_unit_cells(box, rcut: float) -> np.ndarray:
+    """number_of_unit_cells_for_given_cutoff (src/neighbors.f90:175-236): the
+    repetitions along a, b, c (rows of box) that put the supercell's opposite
+    faces at least 2 rcut apart. Host only."""
+    b = _f64(box, (3, 3))
+    ind = np.empty(3, np.int32)
+    L.check(L.load().tgx_number_of_unit_cells(L.dptr(b), float(rcut), L.iptr(ind)), "number_of_unit_cells")
+    return ind
+
+
+def build_supercell(positions, species, box, indices):
+    """The supercell of read_xyz (src/read_files.f90:320-367): positions (N, 3)
+    and species repeated indices times (copies in a-slowest order, the cell
+    first); returns (positions_sc, species_sc, box_sc). Host only."""
+    pos = _f64(positions)
+    n = pos.shape[0]
+    ind = _i32(indices).reshape(3)
+    m = int(np.prod(ind))
+    psc = np.empty((n * m, 3))
+    ssc = np.empty(n * m, np.int32)
+    bsc = np.empty((3, 3))
+    L.check(L.load().tgx_build_supercell(n, L.dptr(pos), L.iptr(_i32(species)), L.dptr(_f64(box, (3, 3))),
+                                         L.iptr(ind), L.dptr(psc), L.iptr(ssc), L.dptr(bsc)), "build_supercell")
+    return psc, ssc, bsc
 
 
 class Context:
@@ -248,9 +276,16 @@ class Context:
         return V, Vd
 
     def get_soap_energy_and_forces(self, soap, soap_der, n_neigh, neighbors_list, xyz, n_sites0, e0=0.0,
-                                   do_forces=True):
+                                   do_forces=True, delta=None, zeta0=None):
         """src/gap.f90:66-259 with host arrays: soap (n, D), soap_der (P, D, 3),
-        neighbors_list 1-based local indices, xyz (P, 3)."""
+        neighbors_list 1-based local indices, xyz (P, 3). delta and zeta0 default
+        to the current block's and must equal them (TgxError otherwise)."""
+        if delta is None or zeta0 is None:
+            if self._current is None:
+                raise L.TgxError("get_soap_energy_and_forces: no block selected (soap_backend_begin)")
+            h = self.models[self._current]
+            delta = h.delta if delta is None else delta
+            zeta0 = h.zeta if zeta0 is None else zeta0
         soap = _f64(soap)
         n, D = soap.shape
         energies = np.zeros(n)
@@ -263,7 +298,8 @@ class Context:
             P = sd.shape[0]
             args = [sd, _i32(n_neigh), _i32(neighbors_list), _f64(xyz)]
         L.check(self._lib.tgx_get_soap_energy_and_forces(
-            self.h, n, D, P, L.dptr(soap), L.dptr(args[0]), float(e0), L.iptr(args[1]), L.iptr(args[2]),
+            self.h, n, D, P, L.dptr(soap), L.dptr(args[0]), float(delta), float(zeta0), float(e0),
+            L.iptr(args[1]), L.iptr(args[2]),
             L.dptr(args[3]), int(n_sites0), int(do_forces), L.dptr(energies), L.dptr(forces), L.dptr(virial)),
             "get_soap_energy_and_forces")
         return energies, forces, virial.T.copy()

@@ -193,6 +193,13 @@ extern "C" int tgx_set_system(tgx_ctx* ctx, int n_sites, int n_sites_sc, const d
   TGX_CTX_CHECK(ctx);
   TGX_REQUIRE(n_sites > 0 && n_sites_sc >= n_sites && positions && species_sc && box,
               "tgx_set_system: bad arguments (n_sites=%d, n_sites_sc=%d)", n_sites, n_sites_sc);
+  if (n_sites != ctx->n_sites) {
+    // buffers sized for the previous system's atoms: no MD state, fixed-atom
+    // mask or results carry over to a system of another size
+    ctx->md_n = 0;
+    ctx->md_fixed = false;
+    ctx->results_n = 0;
+  }
   ctx->n_sites = n_sites;
   ctx->n_sites_sc = n_sites_sc;
   for (int k = 0; k < 9; ++k) ctx->box[k] = box[k];
@@ -500,6 +507,7 @@ static int begin_step(tgx_ctx* ctx) {
   TGX_TRY(ctx->fglob.ensure(sizeof(double) * 3 * P));
   TGX_TRY(tgx_zero(ctx, ctx->energies.p, sizeof(double) * (size_t)n));
   TGX_TRY(tgx_zero(ctx, ctx->fglob.p, sizeof(double) * 3 * P));
+  ctx->results_n = n;
   return TGX_OK;
 }
 
@@ -599,14 +607,20 @@ extern "C" int tgx_get_soap(tgx_ctx* ctx, double* soap, double* soap_cart_der) {
 }
 
 extern "C" int tgx_get_soap_energy_and_forces(tgx_ctx* ctx, int n_sites, int n_soap, int64_t n_pairs,
-                                              const double* soap, const double* soap_der, double e0,
-                                              const int* n_neigh, const int* neighbors_list, const double* xyz,
-                                              int n_sites0, int do_forces, double* energies, double* forces,
-                                              double* virial) {
+                                              const double* soap, const double* soap_der, double delta,
+                                              double zeta0, double e0, const int* n_neigh,
+                                              const int* neighbors_list, const double* xyz, int n_sites0,
+                                              int do_forces, double* energies, double* forces, double* virial) {
   TGX_CTX_CHECK(ctx);
   tgx_model_t* m;
   TGX_TRY(current_model(ctx, &m));
   TGX_REQUIRE(n_soap == m->D, "n_soap=%d does not match the model's descriptor size %d", n_soap, m->D);
+  // delta and zeta are fixed when the block is created (its sparse set and
+  // kernel are prepared with them): the caller's values must be the block's
+  TGX_REQUIRE(std::fabs(delta - m->delta) <= 1e-12 * std::fabs(m->delta) &&
+                  std::fabs(zeta0 - m->zeta) <= 1e-12 * std::fabs(m->zeta),
+              "tgx_get_soap_energy_and_forces: delta=%g, zeta0=%g differ from the block's delta=%g, zeta=%g", delta,
+              zeta0, m->delta, m->zeta);
   TGX_REQUIRE(n_sites >= 0 && soap && energies, "tgx_get_soap_energy_and_forces: bad arguments");
   if (do_forces)
     TGX_REQUIRE(soap_der && n_neigh && neighbors_list && xyz && forces && virial && n_sites0 > 0,
@@ -777,6 +791,7 @@ extern "C" int tgx_compute(tgx_ctx* ctx, int path, double* energy, double* viria
 
 extern "C" int tgx_get_results(tgx_ctx* ctx, double* energies, double* forces) {
   TGX_CTX_CHECK(ctx);
+  TGX_REQUIRE(ctx->results_n > 0 && ctx->results_n == ctx->n_sites, "tgx_get_results: no results for this system");
   const size_t n = (size_t)ctx->n_sites;
   if (energies) TGX_CHECK_HIP(hipMemcpyAsync(energies, ctx->energies.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
   if (forces) TGX_CHECK_HIP(hipMemcpyAsync(forces, ctx->forces.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
@@ -786,6 +801,7 @@ extern "C" int tgx_get_results(tgx_ctx* ctx, double* energies, double* forces) {
 
 extern "C" int tgx_results_device(tgx_ctx* ctx, double** d_forces, double** d_energies) {
   TGX_CTX_CHECK(ctx);
+  TGX_REQUIRE(ctx->results_n > 0 && ctx->results_n == ctx->n_sites, "tgx_results_device: no results for this system");
   if (d_forces) *d_forces = ctx->forces.as<double>();
   if (d_energies) *d_energies = ctx->energies.as<double>();
   return TGX_OK;

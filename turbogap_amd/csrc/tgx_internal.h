@@ -199,7 +199,9 @@ struct tgx_ctx {
   std::vector<tgx_pair_model*> pair_models;
   // results
   dbuf energies, forces, fglob, virial_part, scratch, red;
+  int results_n = 0;  // atoms the energies / forces buffers hold results for (0: none)
   // md
+  int md_n = 0;       // atoms of the MD state (velocities, masses, previous step); 0: none
   dbuf vel, mass, forces_prev, pos_prev;
   dbuf md_disp, md_flag;      // displacement since the last list build, rebuild flag
   dbuf md_fix;                 // unsigned char[3 n]: fixed Cartesian components

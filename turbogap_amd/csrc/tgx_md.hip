@@ -336,6 +336,11 @@ __global__ void k_disp(int n, double Lx, double Ly, double Lz, const double* __r
                 "MD needs a system without supercell replicas");                   \
   } while (0)
 
+// The MD state (velocities, masses, previous positions and forces) was set for
+// the current system's atoms; the forces, when needed, are this system's.
+#define MD_HAVE_STATE(ctx) ((ctx)->md_n == (ctx)->n_sites)
+#define MD_HAVE_FORCES(ctx) ((ctx)->forces.p && (ctx)->results_n == (ctx)->n_sites)
+
 extern "C" int tgx_md_set_state(tgx_ctx* ctx, const double* velocities, const double* masses) {
   MD_CTX(ctx);
   const size_t n = (size_t)ctx->n_sites;
@@ -349,6 +354,7 @@ extern "C" int tgx_md_set_state(tgx_ctx* ctx, const double* velocities, const do
   TGX_TRY(tgx_zero(ctx, ctx->forces_prev.p, sizeof(double) * 3 * n));
   TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   ctx->dt_prev = 0.0;
+  ctx->md_n = ctx->n_sites;
   return TGX_OK;
 }
 
@@ -358,7 +364,7 @@ extern "C" int tgx_md_get_state(tgx_ctx* ctx, double* positions, double* velocit
   if (positions)
     TGX_CHECK_HIP(hipMemcpyAsync(positions, ctx->pos.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
   if (velocities) {
-    TGX_REQUIRE(ctx->vel.p, "tgx_md_get_state: no velocities set");
+    TGX_REQUIRE(MD_HAVE_STATE(ctx), "tgx_md_get_state: no velocities set for this system");
     TGX_CHECK_HIP(hipMemcpyAsync(velocities, ctx->vel.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
   }
   TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
@@ -367,7 +373,7 @@ extern "C" int tgx_md_get_state(tgx_ctx* ctx, double* positions, double* velocit
 
 extern "C" int tgx_velocity_verlet(tgx_ctx* ctx, double dt, int first_step) {
   MD_CTX(ctx);
-  TGX_REQUIRE(ctx->vel.p && ctx->mass.p && ctx->forces.p, "tgx_velocity_verlet: state or forces missing");
+  TGX_REQUIRE(MD_HAVE_STATE(ctx) && MD_HAVE_FORCES(ctx), "tgx_velocity_verlet: state or forces missing for this system");
   const int n = ctx->n_sites;
   TGX_LAUNCH(ctx, "md_verlet", k_vv, dim3((n + 255) / 256), dim3(256), 0, n, dt, ctx->dt_prev, first_step ? 1 : 0,
              ctx->forces.as<double>(), ctx->forces_prev.as<double>(), ctx->mass.as<double>(), ctx->vel.as<double>(),
@@ -459,7 +465,7 @@ static int ke_and_cm(tgx_ctx* ctx, double* ekin, double cm[3], double* mtot) {
 
 extern "C" int tgx_kinetic(tgx_ctx* ctx, double* e_kin, double* temperature) {
   MD_CTX(ctx);
-  TGX_REQUIRE(ctx->vel.p, "tgx_kinetic: no velocities");
+  TGX_REQUIRE(MD_HAVE_STATE(ctx), "tgx_kinetic: no velocities for this system");
   double ek = 0.0;
   TGX_TRY(ke_and_cm(ctx, &ek, nullptr, nullptr));
   const double kB = 8.6173303e-5;
@@ -470,7 +476,7 @@ extern "C" int tgx_kinetic(tgx_ctx* ctx, double* e_kin, double* temperature) {
 
 extern "C" int tgx_berendsen(tgx_ctx* ctx, double T0, double T, double tau, double dt) {
   MD_CTX(ctx);
-  TGX_REQUIRE(ctx->vel.p && tau > 0.0, "tgx_berendsen: no velocities or tau <= 0");
+  TGX_REQUIRE(MD_HAVE_STATE(ctx) && tau > 0.0, "tgx_berendsen: no velocities for this system or tau <= 0");
   if (!(T > 0.0)) return TGX_OK;
   const double f = std::sqrt(1.0 + dt / tau * (T0 / T - 1.0));
   const int n3 = 3 * ctx->n_sites;
@@ -491,7 +497,7 @@ static int remove_cm_device(tgx_ctx* ctx) {
 
 extern "C" int tgx_remove_cm_vel(tgx_ctx* ctx) {
   MD_CTX(ctx);
-  TGX_REQUIRE(ctx->vel.p && ctx->mass.p, "tgx_remove_cm_vel: no velocities");
+  TGX_REQUIRE(MD_HAVE_STATE(ctx), "tgx_remove_cm_vel: no velocities for this system");
   return remove_cm_device(ctx);
 }
 
@@ -511,7 +517,7 @@ extern "C" int tgx_md_displacement(tgx_ctx* ctx, double neighbors_buffer, int re
   const double* b = ctx->box;
   TGX_REQUIRE(b[1] == 0 && b[2] == 0 && b[3] == 0 && b[5] == 0 && b[6] == 0 && b[7] == 0,
               "tgx_md_displacement: orthorhombic boxes only");
-  TGX_REQUIRE(ctx->pos_prev.p, "tgx_md_displacement: no velocity-Verlet step taken");
+  TGX_REQUIRE(MD_HAVE_STATE(ctx), "tgx_md_displacement: no MD state for this system");
   const int n = ctx->n_sites;
   TGX_TRY(ctx->md_disp.ensure(sizeof(double) * 3 * (size_t)n));
   TGX_TRY(ctx->md_flag.ensure(sizeof(int)));
@@ -542,7 +548,7 @@ extern "C" int tgx_md_displacement(tgx_ctx* ctx, double neighbors_buffer, int re
 extern "C" int tgx_md_advance(tgx_ctx* ctx, double dt, int first_step, int remove_cm, double T0, double tau,
                               double neighbors_buffer, double* e_kin, double* temperature, int* rebuild) {
   MD_CTX(ctx);
-  TGX_REQUIRE(ctx->vel.p && ctx->mass.p && ctx->forces.p, "tgx_md_advance: state or forces missing");
+  TGX_REQUIRE(MD_HAVE_STATE(ctx) && MD_HAVE_FORCES(ctx), "tgx_md_advance: state or forces missing for this system");
   TGX_REQUIRE(!(T0 > 0.0) || tau > 0.0, "tgx_md_advance: tau <= 0");
   const double* b = ctx->box;
   TGX_REQUIRE(b[1] == 0 && b[2] == 0 && b[3] == 0 && b[5] == 0 && b[6] == 0 && b[7] == 0,
@@ -613,8 +619,8 @@ static double bussi_resample(double kk, double T0, int ndeg, double tau, double 
 extern "C" int tgx_bussi_thermostat(tgx_ctx* ctx, double T0, double tau, double dt, double gauss, double chi2,
                                     double* e_kin_out) {
   MD_CTX(ctx);
-  TGX_REQUIRE(ctx->vel.p && dt > 0.0 && ctx->n_sites > 1 && chi2 >= 0.0,
-              "tgx_bussi_thermostat: no velocities, dt <= 0, one atom or negative chi2");
+  TGX_REQUIRE(MD_HAVE_STATE(ctx) && dt > 0.0 && ctx->n_sites > 1 && chi2 >= 0.0,
+              "tgx_bussi_thermostat: no velocities for this system, dt <= 0, one atom or negative chi2");
   double ek = 0.0;
   TGX_TRY(ke_and_cm(ctx, &ek, nullptr, nullptr));
   const int n3 = 3 * ctx->n_sites;
@@ -634,8 +640,8 @@ extern "C" int tgx_bussi_thermostat(tgx_ctx* ctx, double T0, double tau, double 
 extern "C" int tgx_gradient_descent(tgx_ctx* ctx, double energy, double max_opt_step, int first_step,
                                     double* gamma_out, int* restored) {
   MD_CTX(ctx);
-  TGX_REQUIRE(ctx->vel.p && ctx->pos_prev.p && ctx->forces_prev.p && ctx->forces.p && max_opt_step >= 0.0,
-              "tgx_gradient_descent: no MD state, no forces or negative max_opt_step");
+  TGX_REQUIRE(MD_HAVE_STATE(ctx) && MD_HAVE_FORCES(ctx) && max_opt_step >= 0.0,
+              "tgx_gradient_descent: no MD state or forces for this system, or negative max_opt_step");
   const double* b = ctx->box;
   TGX_REQUIRE(b[1] == 0 && b[2] == 0 && b[3] == 0 && b[5] == 0 && b[6] == 0 && b[7] == 0,
               "tgx_gradient_descent: orthorhombic boxes only");
@@ -707,8 +713,8 @@ extern "C" int tgx_gradient_descent(tgx_ctx* ctx, double energy, double max_opt_
 extern "C" int tgx_variable_time_step(tgx_ctx* ctx, int init, double target_pos_step, double tau_dt, double dt0,
                                       double* dt) {
   MD_CTX(ctx);
-  TGX_REQUIRE(dt && *dt > 0.0 && target_pos_step > 0.0 && ctx->vel.p && ctx->mass.p && ctx->forces.p,
-              "tgx_variable_time_step: no MD state or forces, dt <= 0 or target_pos_step <= 0");
+  TGX_REQUIRE(dt && *dt > 0.0 && target_pos_step > 0.0 && MD_HAVE_STATE(ctx) && MD_HAVE_FORCES(ctx),
+              "tgx_variable_time_step: no MD state or forces for this system, dt <= 0 or target_pos_step <= 0");
   constexpr int KC = 256;       // candidates per launch
   constexpr int MAX_ROUNDS = 64;  // 16384 steps of 1% from dt
   const int n = ctx->n_sites;
@@ -763,7 +769,8 @@ extern "C" int tgx_variable_time_step(tgx_ctx* ctx, int init, double target_pos_
 extern "C" int tgx_adaptive_time_step(tgx_ctx* ctx, int init, double tmin, double tmax, double xmax, double emax,
                                       double dt0, double* dt) {
   MD_CTX(ctx);
-  TGX_REQUIRE(dt && ctx->vel.p && ctx->mass.p && ctx->forces.p, "tgx_adaptive_time_step: no MD state or forces");
+  TGX_REQUIRE(dt && MD_HAVE_STATE(ctx) && MD_HAVE_FORCES(ctx),
+              "tgx_adaptive_time_step: no MD state or forces for this system");
   TGX_REQUIRE(xmax > 0.0 && tmax > 0.0 && tmin > 0.0 && tmax > tmin,
               "tgx_adaptive_time_step: xmax, tmin and tmax must be positive and tmax > tmin");
   const int n = ctx->n_sites;

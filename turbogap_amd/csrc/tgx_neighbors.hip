@@ -599,3 +599,101 @@ int tgx_neighbor_angles(tgx_ctx* ctx) {
   ctx->have_angles = true;
   return TGX_OK;
 }
+
+// ---- supercell (host only) ------------------------------------------------------
+// number_of_unit_cells_for_given_cutoff (src/neighbors.f90:175-236). The
+// repetitions follow from the heights of the cell over its three face pairs:
+// with unit normals n_ab, n_ca, n_bc, the matrix of their projections on
+// (a, b, c) is inverted in closed form and applied to (2 rc, 2 rc, 2 rc); each
+// count is that value rounded up. The operations are the reference's, in its
+// order (the unit compiles without fused multiply-adds), so the counts agree
+// with it also where 2 rc lands on a multiple of a height.
+static void cross3(const double* u, const double* v, double* w) {
+  w[0] = u[1] * v[2] - u[2] * v[1];
+  w[1] = u[2] * v[0] - u[0] * v[2];
+  w[2] = u[0] * v[1] - u[1] * v[0];
+}
+
+static double dot3h(const double* u, const double* v) {
+  double s = u[0] * v[0];
+  s = s + u[1] * v[1];
+  return s + u[2] * v[2];
+}
+
+static void unit_normal(const double* u, const double* v, double* n) {
+  cross3(u, v, n);
+  const double r = std::sqrt(dot3h(n, n));
+  for (int k = 0; k < 3; ++k) n[k] = n[k] / r;
+}
+
+extern "C" int tgx_number_of_unit_cells(const double* box, double rcut, int* indices) {
+  TGX_REQUIRE(box && indices && rcut > 0.0, "tgx_number_of_unit_cells: null box/indices or rcut <= 0");
+  const double *a = box, *b = box + 3, *c = box + 6;
+  double nab[3], nca[3], nbc[3];
+  unit_normal(a, b, nab);
+  unit_normal(c, a, nca);
+  unit_normal(b, c, nbc);
+  const double* nr[3] = {nab, nca, nbc};
+  const double* ax[3] = {a, b, c};
+  double M[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int s = 0; s < 3; ++s) M[r][s] = dot3h(nr[r], ax[s]);
+  TGX_REQUIRE(std::isfinite(M[0][2]) && std::isfinite(M[1][1]) && std::isfinite(M[2][0]),
+              "tgx_number_of_unit_cells: degenerate box");
+  const double det = -M[0][2] * M[1][1] * M[2][0] + M[0][1] * M[1][2] * M[2][0] + M[0][2] * M[1][0] * M[2][1] -
+                     M[0][0] * M[1][2] * M[2][1] - M[0][1] * M[1][0] * M[2][2] + M[0][0] * M[1][1] * M[2][2];
+  TGX_REQUIRE(det != 0.0 && std::isfinite(det), "tgx_number_of_unit_cells: degenerate box");
+  // adjugate (transposed cofactors), then divided by the determinant
+  double Mi[3][3];
+  Mi[0][0] = M[1][1] * M[2][2] - M[1][2] * M[2][1];
+  Mi[0][1] = M[0][2] * M[2][1] - M[0][1] * M[2][2];
+  Mi[0][2] = M[0][1] * M[1][2] - M[0][2] * M[1][1];
+  Mi[1][0] = M[1][2] * M[2][0] - M[1][0] * M[2][2];
+  Mi[1][1] = M[0][0] * M[2][2] - M[0][2] * M[2][0];
+  Mi[1][2] = M[0][2] * M[1][0] - M[0][0] * M[1][2];
+  Mi[2][0] = M[1][0] * M[2][1] - M[1][1] * M[2][0];
+  Mi[2][1] = M[0][1] * M[2][0] - M[0][0] * M[2][1];
+  Mi[2][2] = M[0][0] * M[1][1] - M[0][1] * M[1][0];
+  const double d = 2.0 * rcut;
+  for (int r = 0; r < 3; ++r) {
+    double v = (Mi[r][0] / det) * d;
+    v = v + (Mi[r][1] / det) * d;
+    v = std::fabs(v + (Mi[r][2] / det) * d);
+    TGX_REQUIRE(v < 1.0e6, "tgx_number_of_unit_cells: %g repetitions along axis %d", v, r + 1);
+    int k = (int)v;
+    if (v > (double)k) ++k;
+    indices[r] = k;
+  }
+  return TGX_OK;
+}
+
+// The supercell of src/read_files.f90:320-367: the cell repeated indices[0] x
+// indices[1] x indices[2] times, copy (i, j, k) (i slowest) holding every atom
+// shifted by i a + j b + k c, in atom order; the box vectors scaled by the
+// counts. positions (3, n_sites) and positions_sc (3, n_sites * prod(indices))
+// column-major; box and box_sc hold a, b, c.
+extern "C" int tgx_build_supercell(int n_sites, const double* positions, const int* species, const double* box,
+                                   const int* indices, double* positions_sc, int* species_sc, double* box_sc) {
+  TGX_REQUIRE(n_sites > 0 && positions && species && box && indices && positions_sc && species_sc && box_sc,
+              "tgx_build_supercell: bad arguments");
+  TGX_REQUIRE(indices[0] >= 1 && indices[1] >= 1 && indices[2] >= 1, "tgx_build_supercell: indices must be >= 1");
+  const double *a = box, *b = box + 3, *c = box + 6;
+  size_t at = 0;
+  for (int i = 0; i < indices[0]; ++i)
+    for (int j = 0; j < indices[1]; ++j)
+      for (int k = 0; k < indices[2]; ++k)
+        for (int s = 0; s < n_sites; ++s, ++at) {
+          for (int x = 0; x < 3; ++x) {
+            double p = positions[3 * (size_t)s + x] + (double)i * a[x];
+            p = p + (double)j * b[x];
+            positions_sc[3 * at + x] = p + (double)k * c[x];
+          }
+          species_sc[at] = species[s];
+        }
+  for (int x = 0; x < 3; ++x) {
+    box_sc[x] = (double)indices[0] * a[x];
+    box_sc[3 + x] = (double)indices[1] * b[x];
+    box_sc[6 + x] = (double)indices[2] * c[x];
+  }
+  return TGX_OK;
+}

@@ -198,10 +198,11 @@ int tgx_zero(tgx_ctx* ctx, void* p, size_t bytes) {
   return TGX_OK;
 }
 
-int tgx_scan_i32(tgx_ctx* ctx, const int* in, int64_t* out, int64_t n, int64_t* total) {
+int tgx_scan_i32(tgx_ctx* ctx, const int* in, int64_t* out, int64_t n, int64_t* total, dbuf* tmp) {
   const int64_t nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
-  TGX_TRY(ctx->scan_tmp.ensure(sizeof(int64_t) * (size_t)(nb + 2)));
-  int64_t* sums = ctx->scan_tmp.as<int64_t>();
+  dbuf& sb = tmp ? *tmp : ctx->scan_tmp;
+  TGX_TRY(sb.ensure(sizeof(int64_t) * (size_t)(nb + 2)));
+  int64_t* sums = sb.as<int64_t>();
   if (n > 0 && n <= SCAN_SINGLE_MAX) {
     TGX_LAUNCH(ctx, "scan_single", k_scan_single, dim3(1), dim3(SCAN_BLOCK), 0, in, out, n);
   } else if (n > 0 && n <= SCAN_2L_MAX) {
@@ -235,8 +236,9 @@ int tgx_scan2_i32(tgx_ctx* ctx, const int* in0, int64_t* out0, const int* in1, i
                (const int64_t*)sums);
     return TGX_OK;
   }
-  TGX_TRY(tgx_scan_i32(ctx, in0, out0, n, nullptr));
-  return tgx_scan_i32(ctx, in1, out1, n, nullptr);
+  // one after the other on the caller's stream, both through the caller's tmp
+  TGX_TRY(tgx_scan_i32(ctx, in0, out0, n, nullptr, &tmp));
+  return tgx_scan_i32(ctx, in1, out1, n, nullptr, &tmp);
 }
 
 int tgx_device_sum_launch(tgx_ctx* ctx, const double* d_x, int64_t n, double* d_part) {

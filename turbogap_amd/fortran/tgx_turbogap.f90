@@ -7,6 +7,8 @@
 !
 !   gpu_context_init / gpu_context_finalize   src/gpu_context.f90:41-58
 !   build_neighbors_list                      src/neighbors.f90:243-605
+!   number_of_unit_cells_for_given_cutoff     src/neighbors.f90:175-236
+!   build_supercell                           src/read_files.f90:320-367 (the replication in read_xyz)
 !   soap_model_from_files                     src/read_files.f90:660-724 (alphas / desc_sparse)
 !   soap_backend_begin / soap_backend_end     src/gap.f90:49-64
 !   get_gap_soap                              src/gap_interface.f90:41-480 (results on the host)
@@ -45,6 +47,7 @@ module tgx_turbogap
    type(c_ptr), save :: ctx = c_null_ptr
 
    public :: gpu_context_init, gpu_context_finalize, build_neighbors_list, soap_model_from_files, &
+             number_of_unit_cells_for_given_cutoff, build_supercell, &
              soap_backend_begin, soap_backend_end, get_gap_soap, get_soap_energy_and_forces, tgx_check, &
              tgx_distance_2b_model, tgx_angle_3b_model, tgx_core_pot_model, add_pair_contribution, &
              md_state_init, md_state_get, md_build_list, md_refresh_list, md_forces, wrap_pbc_device, &
@@ -123,13 +126,13 @@ module tgx_turbogap
          integer(c_int) :: nsb
          integer(c_int64_t) :: npb
       end function
-      integer(c_int) function c_tgx_eaf(c, n, nsoap, np, soap, sder, e0, nn, nl, xyz, n0, dof, en, f, vir) &
-         bind(C, name="tgx_get_soap_energy_and_forces")
+      integer(c_int) function c_tgx_eaf(c, n, nsoap, np, soap, sder, delta, zeta0, e0, nn, nl, xyz, n0, dof, en, &
+                                        f, vir) bind(C, name="tgx_get_soap_energy_and_forces")
          import :: c_int, c_ptr, c_double, c_int64_t
          type(c_ptr), value :: c
          integer(c_int), value :: n, nsoap, n0, dof
          integer(c_int64_t), value :: np
-         real(c_double), value :: e0
+         real(c_double), value :: delta, zeta0, e0
          real(c_double) :: soap(*), sder(*), xyz(*), en(*), f(*), vir(*)
          integer(c_int) :: nn(*), nl(*)
       end function
@@ -268,6 +271,18 @@ module tgx_turbogap
          integer(c_int), value :: reset
          integer(c_int), intent(out) :: rebuild
       end function
+      integer(c_int) function c_unit_cells(box, rcut, ind) bind(C, name="tgx_number_of_unit_cells")
+         import :: c_int, c_double
+         real(c_double) :: box(*)
+         real(c_double), value :: rcut
+         integer(c_int) :: ind(*)
+      end function
+      integer(c_int) function c_supercell(n, pos, spec, box, ind, psc, ssc, bsc) bind(C, name="tgx_build_supercell")
+         import :: c_int, c_double
+         integer(c_int), value :: n
+         real(c_double) :: pos(*), box(*), psc(*), bsc(*)
+         integer(c_int) :: spec(*), ind(*), ssc(*)
+      end function
       integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
          import :: c_size_t, c_ptr
          type(c_ptr), value :: s
@@ -296,6 +311,48 @@ contains
    subroutine gpu_context_finalize()
       if (c_associated(ctx)) call tgx_check(c_tgx_finalize(ctx), "gpu_context_finalize")
       ctx = c_null_ptr
+   end subroutine
+
+   ! number_of_unit_cells_for_given_cutoff with the reference's arguments
+   ! (src/neighbors.f90:175-236); PBC is accepted and, as there, not used.
+   subroutine number_of_unit_cells_for_given_cutoff(a, b, c, rcut, PBC, indices)
+      real(dp), intent(in) :: a(1:3), b(1:3), c(1:3), rcut
+      logical, intent(in) :: PBC(1:3)
+      integer, intent(out) :: indices(1:3)
+      real(dp) :: box(9)
+      integer(c_int) :: ind(3)
+      box(1:3) = a; box(4:6) = b; box(7:9) = c
+      call tgx_check(c_unit_cells(box, rcut, ind), "number_of_unit_cells_for_given_cutoff")
+      indices = int(ind)
+   end subroutine
+
+   ! The supercell replication of read_xyz (src/read_files.f90:320-367):
+   ! positions(3, n_sites) and species(n_sites) become positions(3, n_sites_supercell)
+   ! and species_supercell, copies in the reference's order (a slowest, the cell
+   ! first), and a_box, b_box, c_box are scaled by indices. Nothing changes when
+   ! indices is (1, 1, 1) beyond species_supercell = species.
+   subroutine build_supercell(positions, species, a_box, b_box, c_box, indices, species_supercell, n_sites_supercell)
+      real(dp), allocatable, intent(inout) :: positions(:, :)
+      integer, intent(in) :: species(:), indices(1:3)
+      real(dp), intent(inout) :: a_box(1:3), b_box(1:3), c_box(1:3)
+      integer, allocatable, intent(inout) :: species_supercell(:)
+      integer, intent(out) :: n_sites_supercell
+      real(dp), allocatable :: psc(:, :)
+      integer(c_int), allocatable :: ssc(:), spec(:)
+      integer(c_int) :: ind(3)
+      real(dp) :: box(9), bsc(9)
+      integer :: n
+      n = size(positions, 2)
+      n_sites_supercell = n*indices(1)*indices(2)*indices(3)
+      allocate (psc(3, n_sites_supercell), ssc(n_sites_supercell), spec(n))
+      spec = int(species(1:n), c_int); ind = int(indices, c_int)
+      box(1:3) = a_box; box(4:6) = b_box; box(7:9) = c_box
+      call tgx_check(c_supercell(int(n, c_int), positions, spec, box, ind, psc, ssc, bsc), "build_supercell")
+      call move_alloc(psc, positions)
+      if (allocated(species_supercell)) deallocate (species_supercell)
+      allocate (species_supercell(n_sites_supercell))
+      species_supercell = int(ssc)
+      a_box = bsc(1:3); b_box = bsc(4:6); c_box = bsc(7:9)
    end subroutine
 
    ! build_neighbors_list with the reference's 20 arguments in its order
@@ -412,7 +469,16 @@ contains
       r_h = rcut_hard; r_s = rcut_soft; nf = 4.0_dp; gs = 1.0_dp; sr = atom_sigma_r; srs = 0.0_dp
       stt = atom_sigma_t; sts = 0.0_dp; amp = 1.0_dp; cw = central_weight
       h%n_species = ns; h%l_max = l_max; h%radial_enhancement = 0; h%n_quad_soft = 32; h%n_quad_buffer = 16
-      h%basis = merge(1, 0, trim(basis) == "poly3gauss")
+      select case (trim(basis))
+      case ("poly3")
+         h%basis = 0
+      case ("poly3gauss")
+         h%basis = 1
+      case default
+         write (*, '(A,A,A)') 'ERROR in soap_model_from_files: basis "', trim(basis), &
+            '" is not implemented (poly3, poly3gauss)'
+         error stop 1
+      end select
       h%alpha_max = c_loc(am); h%rcut_hard = c_loc(r_h); h%rcut_soft = c_loc(r_s); h%nf = c_loc(nf)
       h%global_scaling = c_loc(gs); h%atom_sigma_r = c_loc(sr); h%atom_sigma_r_scaling = c_loc(srs)
       h%atom_sigma_t = c_loc(stt); h%atom_sigma_t_scaling = c_loc(sts); h%amplitude_scaling = c_loc(amp)
@@ -447,7 +513,8 @@ contains
 
    ! get_soap_energy_and_forces with the reference's arguments in its order
    ! (src/gap.f90:66-68) on host arrays, for the current block. delta and zeta0
-   ! belong to the block (given when it was created) and must equal its values.
+   ! belong to the block (given when it was created): other values stop with an
+   ! error.
    subroutine get_soap_energy_and_forces(soap, soap_der, delta, zeta0, e0, n_neigh, neighbors_list, xyz, do_forces, &
                                          do_timing, energies, forces, virial)
       real(dp), intent(in) :: soap(:, :), soap_der(:, :, :), delta, zeta0, e0, xyz(:, :)
@@ -460,7 +527,7 @@ contains
       allocate (nn(size(n_neigh)), nl(size(neighbors_list)))
       nn = int(n_neigh, c_int); nl = int(neighbors_list, c_int)
       call tgx_check(c_tgx_eaf(ctx, int(size(soap, 2), c_int), int(size(soap, 1), c_int), &
-                               int(size(soap_der, 3), c_int64_t), soap, soap_der, e0, nn, nl, xyz, &
+                               int(size(soap_der, 3), c_int64_t), soap, soap_der, delta, zeta0, e0, nn, nl, xyz, &
                                int(size(forces, 2), c_int), merge(1_c_int, 0_c_int, do_forces), energies, forces, &
                                virial), "get_soap_energy_and_forces")
       if (do_timing) then
@@ -549,8 +616,12 @@ contains
       if (present(fix_atom)) then
          allocate (f(size(fix_atom, 1), size(fix_atom, 2)))
          f = merge(1_c_int, 0_c_int, fix_atom)
-         call tgx_check(c_md_set_fixed(ctx, f), "md_state_init")
+      else
+         allocate (f(3, size(masses)))
+         f = 0_c_int
       end if
+      ! without fix_atom every component is free (a mask of an earlier run is dropped)
+      call tgx_check(c_md_set_fixed(ctx, f), "md_state_init")
    end subroutine
 
    subroutine md_state_get(positions, velocities)

@@ -85,7 +85,7 @@ _SIGS = {
     "tgx_soap_block_size": [_P, _PI, C.POINTER(_I64)],
     "tgx_local_property_predict": [_P, _I, _PD, _PD, _PI, C.POINTER(_I64)],
     "tgx_get_soap": [_P, _PD, _PD],
-    "tgx_get_soap_energy_and_forces": [_P, _I, _I, _I64, _PD, _PD, _D, _PI, _PI, _PD, _I, _I, _PD, _PD, _PD],
+    "tgx_get_soap_energy_and_forces": [_P, _I, _I, _I64, _PD, _PD, _D, _D, _D, _PI, _PI, _PD, _I, _I, _PD, _PD, _PD],
     "tgx_compute": [_P, _I, _PD, _PD],
     "tgx_get_results": [_P, _PD, _PD],
     "tgx_results_device": [_P, C.POINTER(_P), C.POINTER(_P)],
@@ -131,6 +131,8 @@ _SIGS = {
     "tgx_reset_profile": [_P],
     "tgx_synchronize": [_P],
     "tgx_hbm_probe": [_P, C.c_size_t, _I, _PD],
+    "tgx_number_of_unit_cells": [_PD, _D, _PI],
+    "tgx_build_supercell": [_I, _PD, _PI, _PD, _PI, _PD, _PI, _PD],
     "tgx_generate_rsa": [_I, _PD, _D, _U64, _PD],
     "tgx_generate_permutation": [_I, _U64, _PI],
     "tgx_generate_normal": [_I, _D, _U64, _PD],
