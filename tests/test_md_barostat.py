@@ -274,3 +274,37 @@ def test_gpu_remove_cm_vel_matches_oracle(ctx):
     _, v = md.state()
     np.testing.assert_allclose(v, MDO.remove_cm_vel(drift, mass), rtol=0, atol=1e-15)
     assert np.abs((mass[:, None] * v).sum(0)).max() < 1e-12
+
+
+@pytest.mark.gpu
+def test_gpu_md_state_belongs_to_its_system(ctx):
+    """After tgx_set_system to a system of another size, the MD entry points and
+    the result copies fail with an error instead of using buffers sized for the
+    previous system; a new tgx_md_set_state and force evaluation restore them, and
+    a fixed-atom mask of the previous system is dropped."""
+    sysm, mass, v0 = _md_system(ctx)
+    fix = np.zeros((sysm.n, 3), bool)
+    fix[:5] = True
+    md = MD(ctx, mass, v0, 3.7, "fp64_adjoint", fix_atom=fix)
+    md.forces()
+    lib = L.load()
+    L.check(lib.tgx_velocity_verlet(ctx.h, 1.0, 1), "velocity_verlet")
+    big = S.binary_system(1024, seed=9)
+    ctx.set_system(big.positions, np.full(big.n, S.CARBON, np.int32), big.box)
+    ek, T = L.C.c_double(), L.C.c_double()
+    flag = L.C.c_int()
+    for call in (lambda: lib.tgx_velocity_verlet(ctx.h, 1.0, 0),
+                 lambda: lib.tgx_kinetic(ctx.h, L.C.byref(ek), L.C.byref(T)),
+                 lambda: lib.tgx_remove_cm_vel(ctx.h),
+                 lambda: lib.tgx_md_displacement(ctx.h, 0.5, 0, L.C.byref(flag)),
+                 lambda: lib.tgx_md_advance(ctx.h, 1.0, 0, 1, 0.0, 100.0, 0.5, None, None, L.C.byref(flag))):
+        assert call() == 1  # TGX_ERR_ARG
+    with pytest.raises(L.TgxError):
+        ctx.results()
+    m2 = S.masses_for(np.full(big.n, S.CARBON))
+    md2 = MD(ctx, m2, maxwell_velocities(m2, 300.0, seed=4), 3.7, "fp64_adjoint")
+    md2.forces()
+    x0, _ = md2.state()
+    L.check(lib.tgx_velocity_verlet(ctx.h, 1.0, 1), "velocity_verlet")
+    x1, v1 = md2.state()
+    assert np.abs(x1[:5] - x0[:5]).max() > 0.0 and v1[:5].any()  # the old mask no longer holds atoms 0-4
