@@ -187,6 +187,24 @@ int tgx_get_soap_energy_and_forces(tgx_ctx* ctx, int n_sites, int n_soap, int64_
                                    const int* n_neigh, const int* neighbors_list, const double* xyz, int n_sites0,
                                    int do_forces, double* energies, double* forces, double* virial);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): tgx_get_soap_energy_and_forces on DEVICE arrays (same
+ * shapes and meaning) that calls no hipMalloc, hipFree or allocating routine.
+ * Its work buffers live in a 64 MiB device array that every call on a GPU shares
+ * (held until the process ends); inputs whose work space exceeds it, or more
+ * than 16,384 sites (n_sites or n_sites0), are rejected with TGX_ERR_ARG: split
+ * the sites over several calls. The work is enqueued on the context's stream
+ * and the call returns without waiting (tgx_synchronize waits). Calls that
+ * overlap on one GPU (several contexts or host threads) race on the shared
+ * array. Energies and forces equal tgx_get_soap_energy_and_forces's bit for bit;
+ * the virial is a fixed-order device sum. */
+int tgx_get_soap_energy_and_forces_noalloc(tgx_ctx* ctx, int n_sites, int n_soap, int64_t n_pairs,
+                                           const double* d_soap, const double* d_soap_der, double delta,
+                                           double zeta0, double e0, const int* d_n_neigh,
+                                           const int* d_neighbors_list, const double* d_xyz, int n_sites0,
+                                           int do_forces, double* d_energies, double* d_forces, double* d_virial);
+
 /* ---- whole-step evaluation ------------------------------------------------ */
 /* E, F and virial of every model on the resident positions (neighbour list must
  * have been built). Per-atom energies include each model's e0 for its central

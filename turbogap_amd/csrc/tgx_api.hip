@@ -118,6 +118,8 @@ extern "C" int tgx_init(int device, tgx_ctx** out) {
   TGX_CHECK_HIP(hipGetDeviceCount(&n));
   TGX_REQUIRE(device >= 0 && device < n, "tgx_init: device %d not present (%d visible)", device, n);
   TGX_CHECK_HIP(hipSetDevice(device));
+  // the device address of the allocation-free entry point's work array, once per device
+  TGX_TRY(tgx_scratch_setup(device));
   tgx_ctx* ctx = new tgx_ctx();
   ctx->device = device;
   hipError_t e = hipStreamCreate(&ctx->stream);

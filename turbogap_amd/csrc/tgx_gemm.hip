@@ -454,7 +454,7 @@ __global__ void k_f32_to_bf16(int64_t n, const float* __restrict__ x, __bf16* __
 
 }  // namespace
 
-static EpiArgs gap_epi(tgx_model_t* m) {
+static EpiArgs gap_epi(const tgx_model_t* m) {
   EpiArgs e{};
   const double z = m->zeta;
   const bool zint = std::fabs(z - (double)(int)z) < 1.0e-5;
@@ -496,6 +496,29 @@ static int launch_f64(tgx_ctx* ctx, int epi, int M, int N, int K, const double* 
   return TGX_OK;
 }
 
+int tgx_gap_contract_f64(tgx_ctx* ctx, const tgx_model_t* m, int ns, const double* q, double* kd, double* epart,
+                         double* e_site, double* qss) {
+  if (ns == 0) return TGX_OK;
+  const int Mp = (int)tgx_round_up(ns, BM);
+  const int Sp = m->Spad, Dp = m->Dpad;
+  const int n_tiles = Sp / BN;
+  EpiArgs e = gap_epi(m);
+  e.epart = epart;
+  e.n_tiles = n_tiles;
+  e.alpha = m->alpha.as<double>();
+  e.out = kd;
+  e.ldo = Sp;
+  TGX_TRY(launch_f64(ctx, EPI_GAP, Mp, Sp, Dp, q, Dp, m->Qs.as<double>(), Dp, e, "gemm_kernel_f64"));
+  TGX_LAUNCH(ctx, "energy_finalize", k_energy_finalize, dim3((ns + 255) / 256), dim3(256), 0, ns, epart, n_tiles,
+             m->delta * m->delta, m->e0, e_site);
+  if (!qss) return TGX_OK;
+  EpiArgs s{};
+  s.pref = 1.0;
+  s.out = qss;
+  s.ldo = Dp;
+  return launch_f64(ctx, EPI_STORE, Mp, Dp, Sp, kd, Sp, m->QsT.as<double>(), Sp, s, "gemm_qss_f64");
+}
+
 int tgx_gap_contract(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces) {
   const int ns = m->n_sites_b;
   if (ns == 0) return TGX_OK;
@@ -504,48 +527,37 @@ int tgx_gap_contract(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces) {
   const int n_tiles = Sp / BN;
   TGX_TRY(m->epart.ensure(sizeof(double) * (size_t)Mp * n_tiles));
   TGX_TRY(m->e_site.ensure(sizeof(double) * (size_t)Mp));
+  if (path == TGX_PATH_FP64_PLAIN || path == TGX_PATH_FP64_ADJOINT) {
+    TGX_TRY(m->kd64.ensure(sizeof(double) * (size_t)Mp * Sp));
+    if (do_forces) TGX_TRY(m->qss64.ensure(sizeof(double) * (size_t)Mp * Dp));
+    return tgx_gap_contract_f64(ctx, m, ns, m->q64.as<double>(), m->kd64.as<double>(), m->epart.as<double>(),
+                                m->e_site.as<double>(), do_forces ? m->qss64.as<double>() : nullptr);
+  }
+  // mixed path: qbf (BF16 descriptors) comes from the FP32 descriptor pass
   EpiArgs e = gap_epi(m);
   e.epart = m->epart.as<double>();
   e.n_tiles = n_tiles;
-  if (path == TGX_PATH_FP64_PLAIN || path == TGX_PATH_FP64_ADJOINT) {
-    TGX_TRY(m->kd64.ensure(sizeof(double) * (size_t)Mp * Sp));
-    e.alpha = m->alpha.as<double>();
-    e.out = m->kd64.p;
-    e.ldo = Sp;
-    TGX_TRY(launch_f64(ctx, EPI_GAP, Mp, Sp, Dp, m->q64.as<double>(), Dp, m->Qs.as<double>(), Dp, e, "gemm_kernel_f64"));
-  } else {
-    // qbf (BF16 descriptors) comes from the FP32 descriptor pass
-    TGX_REQUIRE(m->qbf.bytes >= sizeof(__bf16) * (size_t)Mp * Dp, "gemm_bf16: BF16 descriptors missing");
-    TGX_TRY(m->kdbf.ensure(sizeof(__bf16) * (size_t)Mp * Sp));
-    e.alpha_f = m->alpha_f.as<float>();
-    e.out = m->kdbf.p;
-    e.ldo = Sp;
-    if (e.zeta_int >= 1)
-      TGX_TRY((launch_bf16<EPI_GAP, __bf16, true>(ctx, "gemm_kernel_bf16", Mp, Sp, Dp, 1, m->qbf.as<__bf16>(), Dp,
-                                                   m->Qs_bf.as<__bf16>(), Dp, e)));
-    else
-      TGX_TRY((launch_bf16<EPI_GAP, __bf16, false>(ctx, "gemm_kernel_bf16", Mp, Sp, Dp, 1, m->qbf.as<__bf16>(), Dp,
-                                                    m->Qs_bf.as<__bf16>(), Dp, e)));
-  }
+  TGX_REQUIRE(m->qbf.bytes >= sizeof(__bf16) * (size_t)Mp * Dp, "gemm_bf16: BF16 descriptors missing");
+  TGX_TRY(m->kdbf.ensure(sizeof(__bf16) * (size_t)Mp * Sp));
+  e.alpha_f = m->alpha_f.as<float>();
+  e.out = m->kdbf.p;
+  e.ldo = Sp;
+  if (e.zeta_int >= 1)
+    TGX_TRY((launch_bf16<EPI_GAP, __bf16, true>(ctx, "gemm_kernel_bf16", Mp, Sp, Dp, 1, m->qbf.as<__bf16>(), Dp,
+                                                 m->Qs_bf.as<__bf16>(), Dp, e)));
+  else
+    TGX_TRY((launch_bf16<EPI_GAP, __bf16, false>(ctx, "gemm_kernel_bf16", Mp, Sp, Dp, 1, m->qbf.as<__bf16>(), Dp,
+                                                  m->Qs_bf.as<__bf16>(), Dp, e)));
   TGX_LAUNCH(ctx, "energy_finalize", k_energy_finalize, dim3((ns + 255) / 256), dim3(256), 0, ns,
              m->epart.as<double>(), n_tiles, m->delta * m->delta, m->e0, m->e_site.as<double>());
   if (!do_forces) return TGX_OK;
   EpiArgs s{};
   s.pref = 1.0;
-  if (path == TGX_PATH_FP64_PLAIN || path == TGX_PATH_FP64_ADJOINT) {
-    TGX_TRY(m->qss64.ensure(sizeof(double) * (size_t)Mp * Dp));
-    s.out = m->qss64.p;
-    s.ldo = Dp;
-    TGX_TRY(launch_f64(ctx, EPI_STORE, Mp, Dp, Sp, m->kd64.as<double>(), Sp, m->QsT.as<double>(), Sp, s,
-                       "gemm_qss_f64"));
-  } else {
-    TGX_TRY(m->qss32.ensure(sizeof(float) * (size_t)Mp * Dp));
-    s.out = m->qss32.p;
-    s.ldo = Dp;
-    TGX_TRY((launch_bf16<EPI_STORE, float, true>(ctx, "gemm_qss_bf16", Mp, Dp, Sp, 1, m->kdbf.as<__bf16>(), Sp,
-                                                  m->QsT_bf.as<__bf16>(), Sp, s)));
-  }
-  return TGX_OK;
+  TGX_TRY(m->qss32.ensure(sizeof(float) * (size_t)Mp * Dp));
+  s.out = m->qss32.p;
+  s.ldo = Dp;
+  return launch_bf16<EPI_STORE, float, true>(ctx, "gemm_qss_bf16", Mp, Dp, Sp, 1, m->kdbf.as<__bf16>(), Sp,
+                                             m->QsT_bf.as<__bf16>(), Sp, s);
 }
 
 // C[M][N] (FP32) = scale * A[M][K] . B[N][K]^T in BF16 with FP32 accumulation.

@@ -269,6 +269,18 @@ int tgx_block_select_fill(tgx_ctx* ctx, tgx_model_t* m);
 int tgx_soap_descriptors(tgx_ctx* ctx, tgx_model_t* m, int path, bool with_der);
 int tgx_gap_contract(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces);
 int tgx_plain_pair_forces(tgx_ctx* ctx, tgx_model_t* m);
+// The FP64 contraction of ns sites on explicit buffers (no allocation): q, kd,
+// epart as the model's padded layouts, energies to e_site, and Qss when qss is
+// not NULL.
+int tgx_gap_contract_f64(tgx_ctx* ctx, const tgx_model_t* m, int ns, const double* q, double* kd, double* epart,
+                         double* e_site, double* qss);
+int tgx_plain_pair_forces_raw(tgx_ctx* ctx, int ns, const int64_t* site_beg, int64_t np, int D, int Dpad,
+                              const double* qss, const double* der, int* pair_site, double* fpair);
+// Device address of this device's allocation-free work array (set up by tgx_init)
+// and its size in bytes.
+int tgx_scratch_setup(int device);
+void* tgx_scratch_ptr(int device);
+size_t tgx_scratch_bytes();
 int tgx_adjoint_pair_forces(tgx_ctx* ctx, tgx_model_t* m, int path);
 int tgx_gather_forces(tgx_ctx* ctx, double* energy_out, double* virial_out);
 // F (device, n x 3) and the pair virial (host, 9, optional) from per-entry pair

@@ -1149,6 +1149,7 @@ __global__ __launch_bounds__(256) void k_plain_force(int n_sites, const int64_t*
   const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (p >= n_pairs) return;
   const int i = pair_site[p];
+  if (i < 0) return;  // a pair no site claims (caller-supplied offsets)
   const double* s = qss + (size_t)i * Dpad;
   const double* g = der + (size_t)p * 3 * D;
   double a0 = 0, a1 = 0, a2 = 0;
@@ -1164,7 +1165,7 @@ __global__ __launch_bounds__(256) void k_plain_force(int n_sites, const int64_t*
     a2 += __shfl_xor(a2, o);
   }
   if (lane == 0) {
-    const int64_t gi = pr_gidx[p];
+    const int64_t gi = pr_gidx ? pr_gidx[p] : p;
     fglob[3 * gi] += a0;
     fglob[3 * gi + 1] += a1;
     fglob[3 * gi + 2] += a2;
@@ -1211,6 +1212,16 @@ __global__ void k_pair_site(int n_sites, const int64_t* __restrict__ site_beg, i
   const int i = blockIdx.x;
   if (i >= n_sites) return;
   for (int64_t p = site_beg[i] + threadIdx.x; p < site_beg[i + 1]; p += blockDim.x) pair_site[p] = i;
+}
+
+// k_pair_site for offsets the caller supplied: entries at or beyond n_pairs
+// (offsets summing past it) are not written.
+__global__ void k_pair_site_clip(int n_sites, const int64_t* __restrict__ site_beg, int64_t n_pairs,
+                                 int* __restrict__ pair_site) {
+  const int i = blockIdx.x;
+  if (i >= n_sites) return;
+  const int64_t e = site_beg[i + 1] < n_pairs ? site_beg[i + 1] : n_pairs;
+  for (int64_t p = site_beg[i] + threadIdx.x; p < e; p += blockDim.x) pair_site[p] = i;
 }
 
 }  // namespace
@@ -1338,6 +1349,19 @@ int tgx_plain_pair_forces(tgx_ctx* ctx, tgx_model_t* m) {
   TGX_LAUNCH(ctx, "plain_force", k_plain_force, dim3((unsigned)((np + 3) / 4)), dim3(256), 0, ns,
              m->site_beg.as<int64_t>(), m->D, m->Dpad, m->qss64.as<double>(), m->soap_der.as<double>(),
              m->pr_gidx.as<int64_t>(), np, pair_site, ctx->fglob.as<double>());
+  return TGX_OK;
+}
+
+// The plain pair forces on caller-supplied buffers: fpair (3, n_pairs) +=
+// Qss_i . der(:, :, p) for the pairs p of every site i (offsets site_beg, which
+// may sum past n_pairs: those entries are skipped), pair_site a work array of
+// n_pairs ints. No allocation.
+int tgx_plain_pair_forces_raw(tgx_ctx* ctx, int ns, const int64_t* site_beg, int64_t np, int D, int Dpad,
+                              const double* qss, const double* der, int* pair_site, double* fpair) {
+  if (ns == 0 || np == 0) return TGX_OK;
+  TGX_LAUNCH(ctx, "pair_site", k_pair_site_clip, dim3(ns), dim3(64), 0, ns, site_beg, np, pair_site);
+  TGX_LAUNCH(ctx, "plain_force", k_plain_force, dim3((unsigned)((np + 3) / 4)), dim3(256), 0, ns, site_beg, D, Dpad,
+             qss, der, (const int64_t*)nullptr, np, (const int*)pair_site, fpair);
   return TGX_OK;
 }
 

@@ -86,6 +86,8 @@ _SIGS = {
     "tgx_local_property_predict": [_P, _I, _PD, _PD, _PI, C.POINTER(_I64)],
     "tgx_get_soap": [_P, _PD, _PD],
     "tgx_get_soap_energy_and_forces": [_P, _I, _I, _I64, _PD, _PD, _D, _D, _D, _PI, _PI, _PD, _I, _I, _PD, _PD, _PD],
+    "tgx_get_soap_energy_and_forces_noalloc": [_P, _I, _I, _I64, _P, _P, _D, _D, _D, _P, _P, _P, _I, _I, _P, _P,
+                                               _P],
     "tgx_compute": [_P, _I, _PD, _PD],
     "tgx_get_results": [_P, _PD, _PD],
     "tgx_results_device": [_P, C.POINTER(_P), C.POINTER(_P)],
