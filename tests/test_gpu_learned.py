@@ -281,3 +281,86 @@ def test_surrogate_ensemble_is_member_mean(ctx):
     assert frac == 0.0
     np.testing.assert_allclose(spread, np.abs(ea - eb).max(), rtol=1e-5)
     assert np.abs(F2 - 0.5 * (Fa + Fb)).max() <= 1e-4 * np.abs(F2).max()
+
+
+def _gate_threshold(ctx, lo_frac, hi_frac):
+    """A threshold at which the gate sends between lo_frac and hi_frac of the
+    sites at the current positions to the FP64 GAP (bisection on the spread)."""
+    SG.set_gate(ctx, 3, 1e9)
+    ctx.compute("surrogate")
+    lo, hi = 0.0, SG.gate_stats(ctx)[1]
+    for _ in range(40):
+        t = 0.5 * (lo + hi)
+        SG.set_gate(ctx, 3, t)
+        ctx.compute("surrogate")
+        f = SG.gate_stats(ctx)[0]
+        if f > hi_frac:
+            lo = t
+        elif f < lo_frac:
+            hi = t
+        else:
+            return t, f
+    raise AssertionError("no threshold gates the requested share of sites")
+
+
+def test_surrogate_production_md_with_gate(ctx):
+    """Production MD with the surrogate (the 65,536-atom run's loop on the
+    4,096-atom two-species box): a 3-member ensemble per block trained on FP64
+    GAP MD environments, the gate threshold set so that 10-25 % of the sites go
+    to the FP64 GAP at the start, 30 steps of Berendsen NVT at 1500 K then 30
+    of NVE (dt = 1 fs, neighbour list at rcut + 0.5 A), an FP64 GAP check every
+    20 steps on the same positions as the surrogate evaluation.
+      * Gated sites carry the FP64 GAP energies: at every check, the sites whose
+        surrogate energy equals the GAP's to 1e-10 eV are exactly the gated
+        share the library reports, and that share is not zero.
+      * Energies and forces stay finite on every step.
+      * Check-step force RMSE against the FP64 GAP below 50 meV/A (the
+        benchmark's bound); the surrogate's own error here is a few meV/A.
+      * NVE: the total energy's linear drift below 20 meV/atom/ps and its range
+        below 2 meV/atom (the check steps, on the GAP's surface, excluded)."""
+    sysm, models, ids = _small_models(ctx)
+    D = models[0][0].dim
+    nets = [[SG.MLP(ctx, D, 256, 1024, seed=51 + 3 * k + i) for i in range(3)] for k in range(2)]
+    SG.md_dataset(ctx, ids, sysm, nets, temperatures=(500.0, 1500.0, 2500.0), steps=10)
+    n_train = [nets[k][0].add_samples(ids[k], 1 << 30, 1 << 30) for k in range(2)]
+    SG.train_ensemble(nets, n_train, epochs=60, batch=1024)
+    for k in range(2):
+        for i in range(3):
+            nets[k][i].install(ids[k], i)
+    ctx.set_positions(sysm.positions)
+    ctx.build_neighbors_list(3.7 + 0.5, fetch=False)
+    thr, f0 = _gate_threshold(ctx, 0.10, 0.25)
+    SG.set_gate(ctx, 3, thr)
+    mass = S.masses_for(sysm.species)
+    md = MD(ctx, mass, maxwell_velocities(mass, 1500.0, seed=8), 3.7, "surrogate", neighbors_buffer=0.5)
+    n = sysm.n
+    etot, fracs, checks = [], [], []
+    for k in range(60):
+        T0 = 1500.0 if k < 30 else None
+        if (k + 1) % 20 == 0:
+            md.forces("surrogate")
+            frac = SG.gate_stats(ctx)[0]
+            es, Fs = ctx.results()
+            E, ek, T = md.step(1.0, T0=T0, tau=100.0, path="fp64_adjoint")
+            e64, F64 = ctx.results()
+            exact = int(np.sum(np.abs(es - e64) <= 1e-10))
+            checks.append((frac, exact, float(np.sqrt(np.mean((Fs - F64) ** 2)))))
+            assert np.isfinite(Fs).all() and np.isfinite(es).all()
+        else:
+            E, ek, T = md.step(1.0, T0=T0, tau=100.0)
+            fracs.append(SG.gate_stats(ctx)[0])
+            en, F = ctx.results()
+            assert np.isfinite(F).all() and np.isfinite(en).all() and np.isfinite(E)
+            if k >= 30:
+                etot.append((k, (E + ek) / n))
+    print(f"gate threshold {1e3 * thr:.3f} meV (start share {f0:.3f}); checks (share, exact sites, "
+          f"force RMSE eV/A) {checks}; mean share {np.mean(fracs):.3f}", flush=True)
+    for frac, exact, rmse in checks:
+        assert exact == int(round(frac * n)) and exact > 0, (frac, exact)
+        assert rmse < 0.05, rmse
+    assert 0.0 < np.mean(fracs) < 0.6
+    t_ps = np.array([k for k, _ in etot]) * 1e-3
+    e = np.array([v for _, v in etot])
+    drift = np.polyfit(t_ps, e, 1)[0] * 1e3
+    print(f"NVE: drift {drift:.4f} meV/atom/ps, range {1e3 * np.ptp(e):.4f} meV/atom", flush=True)
+    assert abs(drift) < 20.0 and 1e3 * np.ptp(e) < 2.0
