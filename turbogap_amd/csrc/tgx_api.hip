@@ -136,7 +136,8 @@ static void free_model(tgx_model_t* m) {
   dbuf* bufs[] = {&m->B, &m->xq, &m->cen, &m->dtab, &m->ynorm, &m->rtab, &m->Qs, &m->QsT, &m->alpha, &m->Qs_bf, &m->QsT_bf, &m->alpha_f,
                   &m->site_atom, &m->site_beg, &m->pr_gidx, &m->pr_nbr, &m->pr_spec, &m->pr_r, &m->pr_xyz, &m->q64,
                   &m->q32, &m->qbf, &m->pnorm, &m->e_site, &m->qss64, &m->qss32, &m->kd64, &m->kdbf, &m->epart,
-                  &m->soap_der, &m->fpair, &m->pbasis, &m->low_idx, &m->low_coef, &m->gate_pn, &m->gate_e,
+                  &m->soap_der, &m->fpair, &m->pbasis, &m->low_idx, &m->low_coef, &m->raise_idx, &m->raise_coef,
+                  &m->gate_pn, &m->gate_e,
                   &m->gate_stat, &m->gate_list, &m->sel_scratch, &m->mlp_ws, &m->scan_tmp, &m->pair_site};
   for (dbuf* b : bufs) b->release();
   if (m->sel_host) (void)hipHostFree(m->sel_host);

@@ -129,6 +129,8 @@ struct tgx_model_t {
   // device copies of the tables
   dbuf B, xq, cen, dtab, ynorm, rtab;  // rtab: FP32 Hermite table of R_n, dR_n/dr
   dbuf low_idx, low_coef;              // lowering table of the solid harmonics [NLM][3][2]
+  dbuf raise_idx, raise_coef;          // its transpose onto degree l_max - 1 and below [L^2][3][4]
+  bool have_raise = false;
   // sparse set: Qs[Spad][Dpad], QsT[Dpad][Spad], alphas[Spad] (FP64) and BF16 copies
   dbuf Qs, QsT, alpha, Qs_bf, QsT_bf, alpha_f;
   // per-step block state (device)

@@ -682,6 +682,492 @@ __global__ __launch_bounds__(64 * WAVES) void k_site_adjoint(SoapParams P, F32Ar
   }
 }
 
+// ---- MFMA adjoint for l_max = 8 -----------------------------------------------
+// One workgroup (four wavefronts) per site; every per-site product runs on the
+// matrix cores (v_mfma_f32_16x16x4f32, lanes t supply A[t % 16][t / 16] and
+// B[t / 16][t % 16] and hold D[4 (t / 16) + r][t % 16]):
+//   c(n, lm)  = cen + sum_j R_n(j) W_lm(j)             (K over the site's pairs)
+//   p(n<=n',l) = w sum_m c(n, lm) c(n', lm)            (Gram per l, as k_site_desc)
+//   g          = (Qss - (Qss.p) p / |p|^2) / |p|       (q is not read: p is recomputed)
+//   H_l        = S_l C_l, S_l(n, n') = w(n, n') g(n, n', l)
+//   x^T = H^T R^T, y^T = H^T dR^T per tile of 16 pairs, and for the gradients
+//   of the harmonics x~_c^T = H~_c^T R^T with the raised H~_c(n, lm') =
+//   sum over (lm, t) lowering onto lm' of coef_t H(n, lm): the lowering of
+//   W_(l-1) in V_c = sum_lm x_lm rho_l sum_t coef_t W_(l-1, m_t) moves onto H,
+//   V_c = sum_lm' rho_(l'+1) W_lm' x~_c(lm'), rho_l = A_l / A_(l-1) = exp(-2 l k).
+// Then per pair T = sum_lm ((l(l+1) k' - l / r) x + y) W_lm and
+// f = u T + V / r, as k_site_adjoint. The site's pair rows stay in LDS from the
+// c stage to the forces when there are at most AM_NRES of them (one read of the
+// rows); longer sites stream their rows twice. Pairs of a 16-pair tile share a
+// species when every species has at most 8 radial functions (stacked at 0 and
+// 8), so each tile needs half the k-steps. Every global read of the site is
+// issued at the start, and pair forces are added with f64 atomics (one per
+// entry and launch, so the sum equals a plain +=) so that no read of the
+// force array sits on the site's critical path.
+constexpr int AM_NLM = 81;   // (l_max + 1)^2
+constexpr int AM_NT = 6;     // lm tiles of 16 (the last holds lm 80 only)
+constexpr int AM_HP = 20;    // LDS pitch of the [lm][n] rows of C and H
+constexpr int AM_NRES = 32;  // pair rows kept in LDS through the whole site
+constexpr int AM_WIN = 64;   // neighbour entries per list window
+constexpr int AM_KR = 4;     // raising terms per (lm', component)
+constexpr int AM_MF = 8;     // LDS floats per window entry: 1/r, k, k', u (3), pair index (int64)
+constexpr int AM_QV = 5;     // Qss values per thread (D <= 1280)
+constexpr float AM_LOG2E = 1.4426950408889634f;
+
+constexpr int AM_NRT = 64 * 3 * AM_KR;  // raising-table entries (lm' < 64)
+
+__host__ __device__ inline int am_lds_floats(const SoapParams& P, int ldb) {
+  return AM_NRES * ldb + AM_NLM * AM_HP + ((P.D + 3) & ~3) + 8 + AM_WIN * AM_MF + 2 * 4 * 16 * 4 + 2 * AM_WIN + 4 +
+         2 * AM_NRT;
+}
+
+__device__ __forceinline__ int am_degree(int lm) {
+  int l = (int)sqrtf((float)lm + 0.5f);
+  if (l * l > lm) --l;
+  return l;
+}
+
+// One window of neighbour entries [win0, win0 + wcnt) (wave 0): the per-pair
+// data of the contributing pairs (the block's species, 0 < r < rcut of that
+// species) into meta, their window indices into one list per group (the
+// species when tiles share one, else a single group), the counts into gcnt.
+// The loads are independent of each other, so a window costs one round trip.
+struct AmEntry {
+  int grp;
+  float ir, k, kp, ux, uy, uz;
+  int64_t gi;
+};
+
+// The raw loads of one window entry (independent of each other).
+struct AmRaw {
+  int s;
+  double r, x, y, z;
+  int64_t gi;
+};
+
+__device__ __forceinline__ AmRaw am_load_raw(const F32Args& a, int64_t p, bool valid) {
+  AmRaw e{-1, 0.0, 0.0, 0.0, 0.0, 0};
+  if (!valid) return e;
+  e.s = a.pr_spec[p];
+  e.r = a.pr_r[p];
+  e.x = a.pr_xyz[3 * p];
+  e.y = a.pr_xyz[3 * p + 1];
+  e.z = a.pr_xyz[3 * p + 2];
+  e.gi = a.pr_gidx[p];
+  return e;
+}
+
+__device__ __forceinline__ AmEntry am_entry(const SoapParams& P, const F32Args& a, const AmRaw& q) {
+  AmEntry e{};
+  e.grp = -1;
+  e.gi = q.gi;
+  if (!(q.s >= 0 && q.r > 0.0 && q.r < P.sp[q.s].rc)) return e;
+  e.grp = a.split8 ? q.s : 0;
+  const float r = (float)q.r, ir = __builtin_amdgcn_rcpf(r);
+  const float sgt = (float)P.sp[q.s].sig_t, sgs = (float)P.sp[q.s].sig_t_sc;
+  const float stv = sgt * (1.f + sgs * r), dst = sgt * sgs;
+  e.ir = ir;
+  e.k = stv * stv * 0.5f * ir * ir;
+  e.kp = stv * stv * ir * ir * ir - stv * dst * ir * ir;  // -dk/dr
+  e.ux = (float)q.x * ir;
+  e.uy = (float)q.y * ir;
+  e.uz = (float)q.z * ir;
+  return e;
+}
+
+// Everything a site needs from global memory before its first stage: the
+// first AM_NRES pair rows (this thread's share), this thread's Qss values and,
+// in wave 0, the first window's entries, the self pair's index and species.
+struct AmPrefetch {
+  int64_t b0;
+  int np1;
+  float4 rv[4];
+  float qv[AM_QV];
+  AmRaw raw;
+  int64_t gself;
+  int sself;
+};
+
+__device__ __forceinline__ void am_prefetch(const SoapParams& P, const F32Args& a, const float* __restrict__ qss,
+                                            int site, int w, int lane, AmPrefetch& f) {
+  f.b0 = a.site_beg[site];
+  f.np1 = (int)(a.site_beg[site + 1] - f.b0) - 1;
+  const int n4 = min(AM_NRES, max(f.np1, 0)) * a.ldb / 4;
+  const float4* src = reinterpret_cast<const float4*>(a.basis + (f.b0 + 1) * a.ldb);
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int t = threadIdx.x + 256 * u;
+    f.rv[u] = t < n4 ? src[t] : float4{0.f, 0.f, 0.f, 0.f};
+  }
+  const float* srow = qss + (size_t)site * P.Dpad;
+#pragma unroll
+  for (int j = 0; j < AM_QV; ++j) {
+    const int d = threadIdx.x + 256 * j;
+    f.qv[j] = d < P.D ? srow[d] : 0.f;
+  }
+  f.raw = AmRaw{-1, 0.0, 0.0, 0.0, 0.0, 0};
+  f.gself = 0;
+  f.sself = -1;
+  if (w == 0) {
+    f.raw = am_load_raw(a, f.b0 + 1 + lane, lane < min(AM_WIN, max(f.np1, 0)));
+    f.gself = a.pr_gidx[f.b0];
+    f.sself = a.pr_spec[f.b0];
+  }
+}
+
+__device__ __forceinline__ void am_store_entry(const AmEntry& e, int lane, float* meta, int* lists, int* gcnt) {
+  if (e.grp >= 0) {
+    float* mt = meta + lane * AM_MF;
+    mt[0] = e.ir;
+    mt[1] = e.k;
+    mt[2] = e.kp;
+    mt[3] = e.ux;
+    mt[4] = e.uy;
+    mt[5] = e.uz;
+    *reinterpret_cast<int64_t*>(mt + 6) = e.gi;
+  }
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+    const uint64_t bal = __ballot(e.grp == g);
+    if (e.grp == g) lists[g * AM_WIN + __popcll(bal & ((1ull << lane) - 1ull))] = lane;
+    if (lane == 0) gcnt[g] = __popcll(bal);
+  }
+}
+
+__global__ __launch_bounds__(256, 4) void k_site_adjoint_mfma(SoapParams P, F32Args a, int n_sites,
+                                                              const int* __restrict__ raise_idx,
+                                                              const float* __restrict__ raise_coef,
+                                                              const float* __restrict__ qss,
+                                                              double* __restrict__ fglob) {
+  extern __shared__ __align__(16) float smf[];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int li = lane & 15, lg = lane >> 4;
+  const int ldb = a.ldb, roff = a.roff, N = P.N, D = P.D, L1 = P.L + 1;
+  float* rows = smf;                        // [AM_NRES][ldb]
+  float* CH = rows + AM_NRES * ldb;         // [AM_NLM][AM_HP]: C, then H in place
+  float* X = CH + AM_NLM * AM_HP;           // [D]: p, then g
+  float* red = X + ((D + 3) & ~3);          // [8]
+  float* meta = red + 8;                    // [AM_WIN][AM_MF]
+  float* part = meta + AM_WIN * AM_MF;      // [2 buffers][4 waves][16][4]
+  int* lists = reinterpret_cast<int*>(part + 2 * 4 * 16 * 4);  // [2][AM_WIN]
+  int* gcnt = lists + 2 * AM_WIN;           // [4]
+  int* rt_idx = gcnt + 4;                   // [AM_NRT] the raising table, loaded once
+  float* rt_coef = reinterpret_cast<float*>(rt_idx + AM_NRT);
+  const int tx = w, te = 4 + (w >> 1), tpart = w & 1, tu = w;
+  for (int t = threadIdx.x; t < AM_NRT; t += 256) {
+    rt_idx[t] = raise_idx[t];
+    rt_coef[t] = raise_coef[t];
+  }
+  const int site = blockIdx.x;
+  AmPrefetch pf;
+  am_prefetch(P, a, qss, site, w, lane, pf);
+  const int64_t b0 = pf.b0;
+  const int np1 = pf.np1;  // neighbour entries (entry 0 is the self pair)
+  const int64_t p1 = b0 + 1;
+  const bool resident = np1 <= AM_NRES;
+  const int cnt0 = min(AM_NRES, max(np1, 0));
+  float qv[AM_QV];
+#pragma unroll
+  for (int j = 0; j < AM_QV; ++j) qv[j] = pf.qv[j];
+  const int64_t gself = pf.gself;
+  const int sself = pf.sself;
+  {
+    float4* dst = reinterpret_cast<float4*>(rows);
+    const int n4 = cnt0 * ldb / 4;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int t = threadIdx.x + 256 * u;
+      if (t < n4) dst[t] = pf.rv[u];
+    }
+  }
+  if (w == 0) am_store_entry(am_entry(P, a, pf.raw), lane, meta, lists, gcnt);
+  __syncthreads();
+  // ---- c(n, lm): this wave's lm tiles w and w + 4
+  f32x4_t cacc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+  for (int c0 = 0; c0 < np1; c0 += AM_NRES) {
+    const int cnt = min(AM_NRES, np1 - c0);
+    if (c0 > 0) {  // a site longer than AM_NRES: the next chunk of rows
+      __syncthreads();
+      const float4* src = reinterpret_cast<const float4*>(a.basis + (p1 + c0) * ldb);
+      float4* dst = reinterpret_cast<float4*>(rows);
+      const int n4 = cnt * ldb / 4;
+      for (int t = threadIdx.x; t < n4; t += 256) dst[t] = src[t];
+      __syncthreads();
+    }
+#pragma unroll 2
+    for (int k0 = 0; k0 < cnt; k0 += 4) {
+      const int j = k0 + lg;
+      const bool ok = j < cnt;
+      const float av = ok ? rows[j * ldb + roff + li] : 0.f;  // R_n(j), n = li
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int lm = 16 * (w + 4 * t) + li;
+        if (w + 4 * t < AM_NT) {
+          const float bv = (ok && lm < AM_NLM) ? rows[j * ldb + lm] : 0.f;
+          cacc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, cacc[t], 0, 0, 0);
+        }
+      }
+    }
+  }
+  if (w == 0 && li == 0) {  // the central atom's term, lm = 0
+    const int s = sself;
+    if (s >= 0) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int nl = 4 * lg + r - P.sp[s].off;
+        if (nl >= 0 && nl < P.sp[s].nb) cacc[0][r] += (float)P.cen[s * TGX_MAX_NB + nl];
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int lm = 16 * (w + 4 * t) + li;
+    if (w + 4 * t < AM_NT && lm < AM_NLM) *reinterpret_cast<f32x4_t*>(CH + lm * AM_HP + 4 * lg) = cacc[t];
+  }
+  __syncthreads();
+
+  // ---- power spectrum into X, degrees l = w, w + 4, w + 8
+  for (int l = w; l < L1; l += 4) {
+    const int K = 2 * l + 1;
+    f32x4_t g = {0.f, 0.f, 0.f, 0.f};
+    float v[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      const int k = 4 * q + lg;
+      v[q] = (li < N && k < K) ? CH[(l * l + min(k, 2 * l)) * AM_HP + li] : 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+      if (4 * q < K) g = __builtin_amdgcn_mfma_f32_16x16x4f32(v[q], v[q], g, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n = 4 * lg + r, np = li;
+      if (n <= np && np < N) {
+        const int pidx = n * N - n * (n - 1) / 2 + (np - n);
+        X[pidx * L1 + l] = n == np ? g[r] : (float)M_SQRT2 * g[r];
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- g = (Qss - (Qss.p) p / |p|^2) / |p|
+  {
+    float s1 = 0.f, s2 = 0.f;
+#pragma unroll
+    for (int j = 0; j < AM_QV; ++j) {
+      const int d = threadIdx.x + 256 * j;
+      if (d < D) {
+        const float x = X[d];
+        s1 += qv[j] * x;
+        s2 += x * x;
+      }
+    }
+    s1 = wave_total(s1);
+    s2 = wave_total(s2);
+    if (lane == 0) {
+      red[2 * w] = s1;
+      red[2 * w + 1] = s2;
+    }
+  }
+  __syncthreads();
+  {
+    const float qp = (red[0] + red[2]) + (red[4] + red[6]);
+    const float pp = (red[1] + red[3]) + (red[5] + red[7]);
+    const float inv_n = pp > 0.f ? 1.f / sqrtf(pp) : 0.f;
+    const float cq = pp > 0.f ? qp / pp : 0.f;
+#pragma unroll
+    for (int j = 0; j < AM_QV; ++j) {
+      const int d = threadIdx.x + 256 * j;
+      if (d < D) X[d] = (qv[j] - cq * X[d]) * inv_n;
+    }
+  }
+  __syncthreads();
+
+  // ---- H_l = S_l C_l in place: tiles l = 0..7 (2l + 1 columns), then lm 64..79 and lm 80 of l = 8
+  {
+    int sidx[4];
+    float swt[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int n = li, np = 4 * s + lg;
+      if (n < N && np < N) {
+        const int u = min(n, np), v = max(n, np);
+        sidx[s] = (u * N - u * (u - 1) / 2 + (v - u)) * L1;
+        swt[s] = n == np ? 2.f : (float)M_SQRT2;
+      } else {
+        sidx[s] = 0;
+        swt[s] = 0.f;
+      }
+    }
+    for (int ht = w; ht < 10; ht += 4) {
+      const int l = ht < 8 ? ht : 8;
+      const int base = ht < 8 ? l * l : (ht == 8 ? 64 : 80);
+      const int width = ht < 8 ? 2 * l + 1 : (ht == 8 ? 16 : 1);
+      f32x4_t h = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const float av = swt[s] * X[sidx[s] + l];
+        const float bv = li < width ? CH[(base + li) * AM_HP + 4 * s + lg] : 0.f;
+        h = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, h, 0, 0, 0);
+      }
+      if (li < width) *reinterpret_cast<f32x4_t*>(CH + (base + li) * AM_HP + 4 * lg) = h;
+    }
+  }
+  __syncthreads();
+
+  // ---- this wave's operands for the pair tiles: H for lm tile w and half of
+  // tile 4 + w / 2 (its x part for even w, y part for odd w), raised H~_c for
+  // lm' tile w; k-step kk covers n = 4 kk + lane / 16
+  float hx[4], he[4], hr[3][4];
+  {
+    const int lmx = 16 * tx + li, lme = 16 * te + li, lmr = 16 * tu + li;
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const int n = 4 * kk + lg;
+      hx[kk] = CH[lmx * AM_HP + n];
+      he[kk] = lme < AM_NLM ? CH[lme * AM_HP + n] : 0.f;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        float v = 0.f;
+#pragma unroll
+        for (int t = 0; t < AM_KR; ++t) {
+          const int e = (lmr * 3 + c) * AM_KR + t;
+          v += rt_coef[e] * CH[rt_idx[e] * AM_HP + n];
+        }
+        hr[c][kk] = v;
+      }
+    }
+  }
+  // degrees of this lane's output rows: lm = 16 tile + 4 (lane / 16) + r
+  float lx1[4], lx2[4], le1[4], le2[4], lr1[4];
+  bool ve[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int l = am_degree(16 * tx + 4 * lg + r);
+    lx1[r] = (float)(l * (l + 1));
+    lx2[r] = (float)l;
+    const int lme = 16 * te + 4 * lg + r;
+    ve[r] = lme < AM_NLM;
+    const int le = am_degree(min(lme, AM_NLM - 1));
+    le1[r] = (float)(le * (le + 1));
+    le2[r] = (float)le;
+    lr1[r] = -2.f * AM_LOG2E * (float)(am_degree(16 * tu + 4 * lg + r) + 1);
+  }
+
+  // ---- pair forces, windows of AM_WIN neighbour entries, tiles of 16 pairs
+  float fs0 = 0.f, fs1 = 0.f, fs2 = 0.f;  // wave 0, lanes 0..15: the self pair's sum
+  const int ng = a.split8 ? 2 : 1;
+  int tcount = 0;
+  for (int win0 = 0; win0 < np1; win0 += AM_WIN) {
+    if (win0 > 0) {  // a site with more than AM_WIN entries: the next window's lists
+      if (w == 0) {
+        const AmRaw q = am_load_raw(a, p1 + win0 + lane, lane < min(AM_WIN, np1 - win0));
+        am_store_entry(am_entry(P, a, q), lane, meta, lists, gcnt);
+      }
+      __syncthreads();
+    }
+    for (int g = 0; g < ng; ++g) {
+      const int gc = gcnt[g];
+      for (int mt0 = 0; mt0 < gc; mt0 += 16, ++tcount) {
+        const int mrow = mt0 + li;
+        const bool pv = mrow < gc;
+        const int wl = pv ? lists[g * AM_WIN + mrow] : 0;
+        const float* rw;
+        if (resident) {
+          rw = rows + (win0 + wl) * ldb;
+        } else {
+          // stage the tile's rows in slots 0..15 (the previous tile's reads ended at its barrier)
+          const int q = threadIdx.x >> 4, c4 = threadIdx.x & 15;
+          __syncthreads();
+          if (mt0 + q < gc) {
+            const float4* src =
+                reinterpret_cast<const float4*>(a.basis + (p1 + win0 + lists[g * AM_WIN + mt0 + q]) * ldb);
+            float4* dst = reinterpret_cast<float4*>(rows + q * ldb);
+            for (int t = c4; t < ldb / 4; t += 16) dst[t] = src[t];
+          }
+          __syncthreads();
+          rw = rows + li * ldb;
+        }
+        f32x4_t xa = {0.f, 0.f, 0.f, 0.f}, ya = xa, xe = xa, x0 = xa, x1 = xa, x2 = xa;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+          if (!a.split8 || (kk >> 1) == g) {
+            const int n = 4 * kk + lg;
+            const float bR = pv ? rw[roff + n] : 0.f;
+            const float bD = pv ? rw[roff + NMAXF + n] : 0.f;
+            xa = __builtin_amdgcn_mfma_f32_16x16x4f32(hx[kk], bR, xa, 0, 0, 0);
+            ya = __builtin_amdgcn_mfma_f32_16x16x4f32(hx[kk], bD, ya, 0, 0, 0);
+            xe = __builtin_amdgcn_mfma_f32_16x16x4f32(he[kk], tpart ? bD : bR, xe, 0, 0, 0);
+            x0 = __builtin_amdgcn_mfma_f32_16x16x4f32(hr[0][kk], bR, x0, 0, 0, 0);
+            x1 = __builtin_amdgcn_mfma_f32_16x16x4f32(hr[1][kk], bR, x1, 0, 0, 0);
+            x2 = __builtin_amdgcn_mfma_f32_16x16x4f32(hr[2][kk], bR, x2, 0, 0, 0);
+          }
+        }
+        float T = 0.f, V0 = 0.f, V1 = 0.f, V2 = 0.f;
+        if (pv) {
+          const float* mt = meta + wl * AM_MF;
+          const float ir = mt[0], k = mt[1], kp = mt[2];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const float W = rw[16 * tx + 4 * lg + r];
+            T += ((lx1[r] * kp - lx2[r] * ir) * xa[r] + ya[r]) * W;
+            if (ve[r]) {
+              const float We = rw[16 * te + 4 * lg + r];
+              T += (tpart ? xe[r] : (le1[r] * kp - le2[r] * ir) * xe[r]) * We;
+            }
+            const float Wr = rw[16 * tu + 4 * lg + r] * __builtin_amdgcn_exp2f(lr1[r] * k);
+            V0 += Wr * x0[r];
+            V1 += Wr * x1[r];
+            V2 += Wr * x2[r];
+          }
+        }
+        // the four lane rows hold the same pair
+        T += __shfl_xor(T, 16);
+        V0 += __shfl_xor(V0, 16);
+        V1 += __shfl_xor(V1, 16);
+        V2 += __shfl_xor(V2, 16);
+        T += __shfl_xor(T, 32);
+        V0 += __shfl_xor(V0, 32);
+        V1 += __shfl_xor(V1, 32);
+        V2 += __shfl_xor(V2, 32);
+        float* pb = part + (tcount & 1) * (4 * 16 * 4);
+        if (lg == 0) *reinterpret_cast<f32x4_t*>(pb + (w * 16 + li) * 4) = f32x4_t{T, V0, V1, V2};
+        __syncthreads();
+        if (w == 0 && lane < 16 && pv) {
+          const f32x4_t q0 = *reinterpret_cast<const f32x4_t*>(pb + lane * 4);
+          const f32x4_t q1 = *reinterpret_cast<const f32x4_t*>(pb + (16 + lane) * 4);
+          const f32x4_t q2 = *reinterpret_cast<const f32x4_t*>(pb + (32 + lane) * 4);
+          const f32x4_t q3 = *reinterpret_cast<const f32x4_t*>(pb + (48 + lane) * 4);
+          const f32x4_t t4 = (q0 + q1) + (q2 + q3);
+          const float* mt = meta + wl * AM_MF;
+          const float ir = mt[0];
+          const float f0 = mt[3] * t4[0] + t4[1] * ir;
+          const float f1 = mt[4] * t4[0] + t4[2] * ir;
+          const float f2 = mt[5] * t4[0] + t4[3] * ir;
+          // each entry gets one addition per launch: the sum is the same as a plain += (no read back here)
+          const int64_t gi = *reinterpret_cast<const int64_t*>(mt + 6);
+          unsafeAtomicAdd(fglob + 3 * gi, (double)f0);
+          unsafeAtomicAdd(fglob + 3 * gi + 1, (double)f1);
+          unsafeAtomicAdd(fglob + 3 * gi + 2, (double)f2);
+          fs0 += f0;
+          fs1 += f1;
+          fs2 += f2;
+        }
+      }
+    }
+  }
+  if (w == 0) {
+    fs0 = wave_total(fs0);
+    fs1 = wave_total(fs1);
+    fs2 = wave_total(fs2);
+    if (lane == 0) {
+      unsafeAtomicAdd(fglob + 3 * gself, -(double)fs0);
+      unsafeAtomicAdd(fglob + 3 * gself + 1, -(double)fs1);
+      unsafeAtomicAdd(fglob + 3 * gself + 2, -(double)fs2);
+    }
+  }
+}
+
 // Radial table: R_n(r_k), dR_n/dr(r_k) in FP64 from the quadrature, stored FP32.
 // One thread per (species, node).
 __global__ void k_radial_table(SoapParams P, int K, float* __restrict__ tab) {
@@ -751,6 +1237,37 @@ int tgx_radial_table_build(tgx_ctx* ctx, tgx_model_t* m) {
   TGX_TRY(m->low_coef.ensure(sizeof(float) * lc.size()));
   TGX_CHECK_HIP(hipMemcpyAsync(m->low_idx.p, li.data(), sizeof(int) * li.size(), hipMemcpyHostToDevice, ctx->stream));
   TGX_CHECK_HIP(hipMemcpyAsync(m->low_coef.p, lc.data(), sizeof(float) * lc.size(), hipMemcpyHostToDevice, ctx->stream));
+  // the raising table of the MFMA adjoint (l_max = 8): for every lm' below the
+  // top degree and component c, the (lm, coef) pairs whose lowering lands on lm'
+  m->have_raise = false;
+  if (m->l_max == 8) {
+    const int NLM = (m->l_max + 1) * (m->l_max + 1), NR = m->l_max * m->l_max;
+    std::vector<int> ri((size_t)NR * 3 * AM_KR, 0);
+    std::vector<float> rc((size_t)NR * 3 * AM_KR, 0.f);
+    std::vector<int> cnt((size_t)NR * 3, 0);
+    bool fits = true;
+    for (int lm = 0; lm < NLM && fits; ++lm)
+      for (int c = 0; c < 3; ++c)
+        for (int t = 0; t < KLOW; ++t) {
+          const int ix = li[((size_t)lm * 3 + c) * KLOW + t];
+          if (ix < 0) continue;
+          if (ix >= NR || cnt[(size_t)ix * 3 + c] == AM_KR) {
+            fits = false;
+            break;
+          }
+          const size_t k = ((size_t)ix * 3 + c) * AM_KR + cnt[(size_t)ix * 3 + c]++;
+          ri[k] = lm;
+          rc[k] = lc[((size_t)lm * 3 + c) * KLOW + t];
+        }
+    if (fits) {
+      TGX_TRY(m->raise_idx.ensure(sizeof(int) * ri.size()));
+      TGX_TRY(m->raise_coef.ensure(sizeof(float) * rc.size()));
+      TGX_CHECK_HIP(hipMemcpyAsync(m->raise_idx.p, ri.data(), sizeof(int) * ri.size(), hipMemcpyHostToDevice, ctx->stream));
+      TGX_CHECK_HIP(hipMemcpyAsync(m->raise_coef.p, rc.data(), sizeof(float) * rc.size(), hipMemcpyHostToDevice,
+                                   ctx->stream));
+      m->have_raise = true;
+    }
+  }
   TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   return TGX_OK;
 }
@@ -805,6 +1322,14 @@ int tgx_adjoint_f32(tgx_ctx* ctx, tgx_model_t* m) {
   F32Args a = f32_args(m);
   TGX_REQUIRE(m->pbasis.p && m->pbasis.bytes >= sizeof(float) * (size_t)m->n_pairs_b * a.ldb,
               "FP32 adjoint: the pair basis of the block was not built");
+  if (m->have_raise && m->P.L == 8 && m->P.NLM == AM_NLM && a.roff >= AM_NLM) {
+    // l_max = 8: the matrix-core kernel, one workgroup per site
+    const size_t lds = sizeof(float) * (size_t)am_lds_floats(m->P, a.ldb);
+    TGX_REQUIRE(lds <= 64 * 1024, "FP32 adjoint kernel needs %zu bytes of LDS", lds);
+    TGX_LAUNCH(ctx, "soap_adjoint_f32", k_site_adjoint_mfma, dim3(ns), dim3(256), lds, m->P, a, ns,
+               m->raise_idx.as<int>(), m->raise_coef.as<float>(), m->qss32.as<float>(), ctx->fglob.as<double>());
+    return TGX_OK;
+  }
   const size_t lds = sizeof(float) * (size_t)WAVES * adj_wave_floats(m->P, a.ldb);
   TGX_REQUIRE(lds <= 160 * 1024, "FP32 adjoint kernel needs %zu bytes of LDS", lds);
   TGX_LAUNCH(ctx, "soap_adjoint_f32", k_site_adjoint, dim3((ns + WAVES - 1) / WAVES), dim3(64 * WAVES), lds, m->P, a,
