@@ -78,10 +78,11 @@ def loss_and_grads(w, q, e_ref, g_ref, w_e=1.0, w_g=1.0, emulate_bf16=False):
     wz = r(zet2) @ r(W2)
     zet1 = f["v1"] * t0 * s1pp + wz * s1
     a1, a2, h1, h2 = f["a1"], f["a2"], f["h1"], f["h2"]
+    # the weight gradients are products too: BF16 operands under emulation
     g = {
-        "W1": a1.T @ (dE[:, None] * q + cG * u) + cG * zet1.T @ q,
+        "W1": r(a1).T @ r(dE[:, None] * q + cG * u) + r(cG * zet1).T @ r(q),
         "b1": (dE[:, None] * a1 + cG * zet1).sum(0),
-        "W2": a2.T @ (dE[:, None] * h1 + cG * t1) + cG * zet2.T @ h1,
+        "W2": r(a2).T @ r(dE[:, None] * h1 + cG * t1) + r(cG * zet2).T @ r(h1),
         "b2": (dE[:, None] * a2 + cG * zet2).sum(0),
         "w3": (dE[:, None] * h2 + cG * gam2).sum(0),
         "b3": np.array([dE.sum()]),

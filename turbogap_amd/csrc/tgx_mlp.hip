@@ -149,81 +149,93 @@ __global__ void k_z1(int64_t n, const float* __restrict__ v1, const float* __res
   zet1[i] = v1[i] * t0[i] * s1pp[i] + wz[i] * s1[i];
 }
 
-// Y = dE_row * X + c * U (row-scaled combination for the weight gradients)
-__global__ void k_rowcomb(int64_t n, int cols, const float* __restrict__ dE, const float* __restrict__ X, float c,
-                          const float* __restrict__ U, float* __restrict__ Y) {
+// Transposed BF16 operands of the weight-gradient GEMMs, with the batch of the
+// gradient's two terms concatenated along K (2B columns):
+//   out[c][b]     = bf16(dE[b] P[b][c] + beta U[b][c])   (dE null: P[b][c]; U null: no second term)
+//   out[c][B + b] = bf16(gamma V[b][c])
+// for b < B, c < cols. 64 x 64 tiles go through LDS so that the reads run along
+// c and the writes along b.
+__global__ __launch_bounds__(256) void k_pack_t(int B, int cols, const float* __restrict__ dE,
+                                                const float* __restrict__ P, float beta, const float* __restrict__ U,
+                                                float gamma, const float* __restrict__ V, __bf16* __restrict__ out) {
+  __shared__ float tile[64][65];
+  const int c0 = blockIdx.x * 64, b0 = blockIdx.y * 64, half = blockIdx.z;
+  const int tc = threadIdx.x & 63, tr = threadIdx.x >> 6;
+#pragma unroll 4
+  for (int i = 0; i < 16; ++i) {
+    const int rr = tr + 4 * i, b = b0 + rr, c = c0 + tc;
+    float v = 0.f;
+    if (b < B && c < cols) {
+      const size_t k = (size_t)b * cols + c;
+      if (half == 0) {
+        v = dE ? dE[b] * P[k] : P[k];
+        if (U) v += beta * U[k];
+      } else {
+        v = gamma * V[k];
+      }
+    }
+    tile[rr][tc] = v;
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (int i = 0; i < 16; ++i) {
+    const int cc = tr + 4 * i, c = c0 + cc, b = b0 + tc;
+    if (c < cols && b < B) out[(size_t)c * (2 * (size_t)B) + (size_t)half * B + b] = (__bf16)tile[tc][cc];
+  }
+}
+
+// out[i] = sum_z part[z][i] over the split-K partials, in z order.
+__global__ void k_splitk_sum(int64_t n, int nz, const float* __restrict__ part, float* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  Y[i] = dE[i / cols] * X[i] + c * U[i];
+  float s = 0.f;
+  for (int z = 0; z < nz; ++z) s += part[(size_t)z * n + i];
+  out[i] = s;
 }
 
-// out[j] = sum_b (dE_b X[b][j] + c Y[b][j]); one thread per column, rows in order.
-// out[j] = sum_b (dE[b] X[b][j] + c Y[b][j]) (X null: X = 1; Y null: no second
-// term). A 64 x 16 thread block per 64 columns: each of the 16 row groups sums
-// every 16th row in FP64 (coalesced along the columns), then the 16 partials are
-// added in a fixed order.
-__global__ __launch_bounds__(1024) void k_colsum(int rows, int cols, const float* __restrict__ dE,
-                                                 const float* __restrict__ X, float c, const float* __restrict__ Y,
-                                                 float* __restrict__ out) {
-  __shared__ double part[16][64];
+// Column sums out[j] = sum_b (dE[b] X[b][j] + c Y[b][j]) (X null: X = 1; Y null:
+// no second term) of up to four arrays at once (blockIdx.z), in two fixed-order
+// passes: every 64-row chunk of 64 columns summed by 4 row groups in FP64, then
+// the chunks added in order.
+struct ColTask {
+  const float* X;
+  const float* Y;
+  float c;
+  int cols;
+  float* out;
+};
+struct ColTasks {
+  ColTask t[4];
+  const float* dE;
+  int rows;
+};
+
+__global__ __launch_bounds__(256) void k_colsum_part(ColTasks ts, double* __restrict__ part) {
+  __shared__ double sh[4][64];
+  const ColTask& t = ts.t[blockIdx.z];
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const int j = blockIdx.x * 64 + tx;
+  const int r0 = blockIdx.y * 64, r1 = min(ts.rows, r0 + 64);
   double acc = 0.0;
-  if (j < cols) {
-    for (int b = ty; b < rows; b += 16) {
-      double v = dE ? (double)dE[b] * (X ? (double)X[(size_t)b * cols + j] : 1.0) : 0.0;
-      if (Y) v += (double)c * (double)Y[(size_t)b * cols + j];
+  if (j < t.cols)
+    for (int b = r0 + ty; b < r1; b += 4) {
+      double v = (double)ts.dE[b] * (t.X ? (double)t.X[(size_t)b * t.cols + j] : 1.0);
+      if (t.Y) v += (double)t.c * (double)t.Y[(size_t)b * t.cols + j];
       acc += v;
     }
-  }
-  part[ty][tx] = acc;
+  sh[ty][tx] = acc;
   __syncthreads();
-  if (ty == 0 && j < cols) {
-    double t = 0.0;
-    for (int g = 0; g < 16; ++g) t += part[g][tx];
-    out[j] = (float)t;
-  }
+  if (ty == 0 && j < t.cols)
+    part[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x * 64 + j] = (sh[0][tx] + sh[1][tx]) + (sh[2][tx] + sh[3][tx]);
 }
 
-// C[m][n] = sum_b X1[b][m] Y1[b][n] + c2 sum_b X2[b][m] Y2[b][n] (FP32), a 64x64 tile
-// per 256-thread block, 4x4 outputs per thread, the batch staged 16 rows at a time.
-__global__ __launch_bounds__(256) void k_tn_gemm(int rows, int Mc, int Nc, const float* __restrict__ X1, const float* __restrict__ Y1,
-                                                 const float* __restrict__ X2, const float* __restrict__ Y2, float c2,
-                                                 float* __restrict__ C) {
-  __shared__ float sx[16][64 + 4];
-  __shared__ float sy[16][64 + 4];
-  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  float acc[4][4] = {};
-  for (int pass = 0; pass < (X2 ? 2 : 1); ++pass) {
-    const float* X = pass ? X2 : X1;
-    const float* Y = pass ? Y2 : Y1;
-    const float cc = pass ? c2 : 1.0f;
-    for (int b0 = 0; b0 < rows; b0 += 16) {
-      __syncthreads();
-      for (int e = threadIdx.x; e < 16 * 64; e += 256) {
-        const int rb = e >> 6, cidx = e & 63;
-        const int b = b0 + rb;
-        sx[rb][cidx] = b < rows ? cc * X[(size_t)b * Mc + m0 + cidx] : 0.f;
-        sy[rb][cidx] = b < rows ? Y[(size_t)b * Nc + n0 + cidx] : 0.f;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        float xv[4], yv[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) { xv[i] = sx[k][ty * 4 + i]; yv[i] = sy[k][tx * 4 + i]; }
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) acc[i][j] += xv[i] * yv[j];
-      }
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) C[(size_t)(m0 + ty * 4 + i) * Nc + n0 + tx * 4 + j] = acc[i][j];
+__global__ void k_colsum_fin(ColTasks ts, int nch, int ldp, const double* __restrict__ part) {
+  const ColTask& t = ts.t[blockIdx.z];
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= t.cols) return;
+  double s = 0.0;
+  for (int k = 0; k < nch; ++k) s += part[((size_t)blockIdx.z * nch + k) * ldp + j];
+  t.out[j] = (float)s;
 }
 
 __global__ void k_adam(int64_t n, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
@@ -348,18 +360,18 @@ struct tgx_mlp {
   dbuf dq, dg, de;
   // batch work space
   dbuf idx, bq, bqb, bg, be, z1, h1, h1b, s1, s1pp, z2, h2, a2, a2b, s2p, s2pp, E, v1, a1, a1b, G, u, ub, t0, t1, t1b,
-      t2, gam2, zet2, zet2b, wz, zet1, dE, Y1, Y2, lossb;
+      t2, gam2, zet2, zet2b, wz, zet1, dE, packA, packB, cpart, kpart, lossb;
   uint64_t seed = 0;
 };
 
 static int mlp_workspace(tgx_mlp* m, int B) {
   const size_t H = m->H, Dp = m->Dp, b = (size_t)B;
   dbuf* f_bh[] = {&m->z1, &m->h1, &m->s1, &m->s1pp, &m->z2, &m->h2, &m->a2, &m->s2p, &m->s2pp, &m->v1, &m->a1,
-                  &m->t0, &m->t1, &m->t2, &m->gam2, &m->zet2, &m->wz, &m->zet1, &m->Y2};
+                  &m->t0, &m->t1, &m->t2, &m->gam2, &m->zet2, &m->wz, &m->zet1};
   for (dbuf* x : f_bh) TGX_TRY(x->ensure(sizeof(float) * b * H));
   dbuf* h_bh[] = {&m->h1b, &m->a2b, &m->a1b, &m->t1b, &m->zet2b};
   for (dbuf* x : h_bh) TGX_TRY(x->ensure(sizeof(__bf16) * b * H));
-  dbuf* f_bd[] = {&m->bq, &m->bg, &m->G, &m->u, &m->Y1};
+  dbuf* f_bd[] = {&m->bq, &m->bg, &m->G, &m->u};
   for (dbuf* x : f_bd) TGX_TRY(x->ensure(sizeof(float) * b * Dp));
   TGX_TRY(m->bqb.ensure(sizeof(__bf16) * b * Dp));
   TGX_TRY(m->ub.ensure(sizeof(__bf16) * b * Dp));
@@ -368,6 +380,10 @@ static int mlp_workspace(tgx_mlp* m, int B) {
   TGX_TRY(m->dE.ensure(sizeof(float) * b));
   TGX_TRY(m->lossb.ensure(sizeof(double) * 2 * b));
   TGX_TRY(m->idx.ensure(sizeof(int) * b));
+  // weight-gradient operands over the concatenated batch, column-sum partials
+  TGX_TRY(m->packA.ensure(sizeof(__bf16) * H * 2 * b));
+  TGX_TRY(m->packB.ensure(sizeof(__bf16) * (Dp > H ? Dp : H) * 2 * b));
+  TGX_TRY(m->cpart.ensure(sizeof(double) * 4 * ((b + 63) / 64) * (size_t)tgx_round_up((int64_t)H, 64)));
   return TGX_OK;
 }
 
@@ -436,8 +452,8 @@ extern "C" int tgx_mlp_destroy(tgx_mlp* m) {
   dbuf* all[] = {&m->theta, &m->grad, &m->mom, &m->vel, &m->W1b, &m->W1Tb, &m->W2b, &m->W2Tb, &m->dq, &m->dg, &m->de,
                  &m->idx, &m->bq, &m->bqb, &m->bg, &m->be, &m->z1, &m->h1, &m->h1b, &m->s1, &m->s1pp, &m->z2, &m->h2,
                  &m->a2, &m->a2b, &m->s2p, &m->s2pp, &m->E, &m->v1, &m->a1, &m->a1b, &m->G, &m->u, &m->ub, &m->t0,
-                 &m->t1, &m->t1b, &m->t2, &m->gam2, &m->zet2, &m->zet2b, &m->wz, &m->zet1, &m->dE, &m->Y1, &m->Y2,
-                 &m->lossb};
+                 &m->t1, &m->t1b, &m->t2, &m->gam2, &m->zet2, &m->zet2b, &m->wz, &m->zet1, &m->dE, &m->packA,
+                 &m->packB, &m->cpart, &m->kpart, &m->lossb};
   for (dbuf* b : all) b->release();
   delete m;
   return TGX_OK;
@@ -468,6 +484,39 @@ static int mlp_forward(tgx_mlp* m, int B, const __bf16* xb, bool train, float* E
   return TGX_OK;
 }
 
+// C[M][N] = A[M][K] B[N][K]^T (BF16 operands, FP32) for the weight gradients, whose
+// M x N grid has few tiles against a long K: K is split into nz chunks run as a
+// batch of GEMMs into FP32 partials, which are then added in chunk order.
+static int wgrad_gemm(tgx_mlp* m, const char* name, int M, int N, int K, const __bf16* A, const __bf16* B, float* C) {
+  tgx_ctx* ctx = m->ctx;
+  const int tiles = (M / 128) * (N / 128);
+  int nz = 1;
+  while (nz < 32 && tiles * nz * 2 <= 1024 && (K / (nz * 2)) % 64 == 0 && K / (nz * 2) >= 256) nz *= 2;
+  if (nz == 1) return tgx_gemm_bf16_nt(ctx, name, M, N, K, A, K, B, K, C, N, 1.0);
+  const int64_t mn = (int64_t)M * N;
+  TGX_TRY(m->kpart.ensure(sizeof(float) * (size_t)(nz * mn)));
+  tgx_gemm_desc d;
+  d.epi = TGX_EPI_STORE_F32;
+  d.M = M;
+  d.N = N;
+  d.K = K / nz;
+  d.batch = nz;
+  d.A = A;
+  d.lda = K;
+  d.sA = K / nz;
+  d.B = B;
+  d.ldb = K;
+  d.sB = K / nz;
+  d.C = m->kpart.p;
+  d.ldc = N;
+  d.sC = mn;
+  d.scale = 1.0;
+  TGX_TRY(tgx_gemm_bf16(ctx, name, d));
+  TGX_LAUNCH(ctx, "mlp_splitk_sum", k_splitk_sum, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, mn, nz,
+             (const float*)m->kpart.as<float>(), C);
+  return TGX_OK;
+}
+
 // Loss and weight gradients of the batch in the batch buffers (bq, bqb, bg, be).
 static int mlp_backward(tgx_mlp* m, int B, float w_e, float w_g, double* loss_e, double* loss_g) {
   tgx_ctx* ctx = m->ctx;
@@ -495,25 +544,34 @@ static int mlp_backward(tgx_mlp* m, int B, float w_e, float w_g, double* loss_e,
                            m->wz.as<float>(), H, 1.0));
   TGX_LAUNCH(ctx, "mlp_z1", k_z1, g1(nbh), dim3(256), 0, nbh, m->v1.as<float>(), m->t0.as<float>(), m->s1pp.as<float>(),
              m->wz.as<float>(), m->s1.as<float>(), m->zet1.as<float>());
-  // dW1 = a1^T (dE q + cG u) + cG zeta1^T q
-  TGX_LAUNCH(ctx, "mlp_y1", k_rowcomb, g1(nbd), dim3(256), 0, nbd, Dp, m->dE.as<float>(), m->bq.as<float>(), cG,
-             m->u.as<float>(), m->Y1.as<float>());
-  TGX_LAUNCH(ctx, "mlp_dW1", k_tn_gemm, dim3(Dp / 64, H / 64), dim3(256), 0, B, H, Dp, m->a1.as<float>(),
-             m->Y1.as<float>(), m->zet1.as<float>(), m->bq.as<float>(), cG, gr + m->oW1);
-  // dW2 = a2^T (dE h1 + cG tau1) + cG zeta2^T h1
-  TGX_LAUNCH(ctx, "mlp_y2", k_rowcomb, g1(nbh), dim3(256), 0, nbh, H, m->dE.as<float>(), m->h1.as<float>(), cG,
-             m->t1.as<float>(), m->Y2.as<float>());
-  TGX_LAUNCH(ctx, "mlp_dW2", k_tn_gemm, dim3(H / 64, H / 64), dim3(256), 0, B, H, H, m->a2.as<float>(),
-             m->Y2.as<float>(), m->zet2.as<float>(), m->h1.as<float>(), cG, gr + m->oW2);
-  // biases and output weights
-  TGX_LAUNCH(ctx, "mlp_db1", k_colsum, dim3((H + 63) / 64), dim3(1024), 0, B, H, m->dE.as<float>(), m->a1.as<float>(), cG,
-             m->zet1.as<float>(), gr + m->ob1);
-  TGX_LAUNCH(ctx, "mlp_db2", k_colsum, dim3((H + 63) / 64), dim3(1024), 0, B, H, m->dE.as<float>(), m->a2.as<float>(), cG,
-             m->zet2.as<float>(), gr + m->ob2);
-  TGX_LAUNCH(ctx, "mlp_dw3", k_colsum, dim3((H + 63) / 64), dim3(1024), 0, B, H, m->dE.as<float>(), m->h2.as<float>(), cG,
-             m->gam2.as<float>(), gr + m->ow3);
-  TGX_LAUNCH(ctx, "mlp_db3", k_colsum, dim3(1), dim3(1024), 0, B, 1, m->dE.as<float>(), (const float*)nullptr, 0.0f,
-             (const float*)nullptr, gr + m->ob3);
+  // dW1 = a1^T (dE q + cG u) + cG zeta1^T q = [a1 | cG zeta1]^T [dE q + cG u ; q], one
+  // BF16 GEMM with FP32 accumulation over the concatenated batch (K = 2B)
+  const dim3 gH(H / 64, B / 64, 2), gD(Dp / 64, B / 64, 2);
+  __bf16* pA = m->packA.as<__bf16>();
+  __bf16* pB = m->packB.as<__bf16>();
+  TGX_LAUNCH(ctx, "mlp_packA1", k_pack_t, gH, dim3(256), 0, B, H, (const float*)nullptr, m->a1.as<float>(), 0.0f,
+             (const float*)nullptr, cG, m->zet1.as<float>(), pA);
+  TGX_LAUNCH(ctx, "mlp_packB1", k_pack_t, gD, dim3(256), 0, B, Dp, m->dE.as<float>(), m->bq.as<float>(), cG,
+             m->u.as<float>(), 1.0f, m->bq.as<float>(), pB);
+  TGX_TRY(wgrad_gemm(m, "mlp_dW1", H, Dp, 2 * B, pA, pB, gr + m->oW1));
+  // dW2 = a2^T (dE h1 + cG tau1) + cG zeta2^T h1 = [a2 | cG zeta2]^T [dE h1 + cG tau1 ; h1]
+  TGX_LAUNCH(ctx, "mlp_packA2", k_pack_t, gH, dim3(256), 0, B, H, (const float*)nullptr, m->a2.as<float>(), 0.0f,
+             (const float*)nullptr, cG, m->zet2.as<float>(), pA);
+  TGX_LAUNCH(ctx, "mlp_packB2", k_pack_t, gH, dim3(256), 0, B, H, m->dE.as<float>(), m->h1.as<float>(), cG,
+             m->t1.as<float>(), 1.0f, m->h1.as<float>(), pB);
+  TGX_TRY(wgrad_gemm(m, "mlp_dW2", H, H, 2 * B, pA, pB, gr + m->oW2));
+  // biases and output weights: fixed-order column sums over 64-row chunks
+  ColTasks ct{};
+  ct.dE = m->dE.as<float>();
+  ct.rows = B;
+  ct.t[0] = ColTask{m->a1.as<float>(), m->zet1.as<float>(), cG, H, gr + m->ob1};
+  ct.t[1] = ColTask{m->a2.as<float>(), m->zet2.as<float>(), cG, H, gr + m->ob2};
+  ct.t[2] = ColTask{m->h2.as<float>(), m->gam2.as<float>(), cG, H, gr + m->ow3};
+  ct.t[3] = ColTask{nullptr, nullptr, 0.0f, 1, gr + m->ob3};
+  const int nch = (B + 63) / 64, ldp = (int)tgx_round_up(H, 64);
+  TGX_LAUNCH(ctx, "mlp_colsum", k_colsum_part, dim3(ldp / 64, nch, 4), dim3(256), 0, ct, m->cpart.as<double>());
+  TGX_LAUNCH(ctx, "mlp_colsum_fin", k_colsum_fin, dim3((ldp + 63) / 64, 1, 4), dim3(64), 0, ct, nch, ldp,
+             (const double*)m->cpart.as<double>());
   if (loss_e || loss_g) {
     double se = 0.0, sg = 0.0;
     TGX_TRY(tgx_device_sum(ctx, le, B, &se));
