@@ -9,6 +9,7 @@
  * Reference interfaces replaced (paths relative to the TurboGAP tree):
  *   tgx_init / tgx_finalize             src/gpu_context.f90:41-58   gpu_context_init / _finalize
  *   tgx_memory_budget                   src/gpu_context.f90:100-162 gpu_memory_budget_init
+ *   tgx_set_memory_budget               src/neighbors.f90:637-778   the batch split of get_number_of_atom_pairs
  *   tgx_set_system                      src/turbogap.f90:871-886    positions/species/box handed to the build
  *   tgx_number_of_unit_cells            src/neighbors.f90:175-236   number_of_unit_cells_for_given_cutoff
  *   tgx_build_supercell                 src/read_files.f90:320-367  the supercell replication of read_xyz
@@ -19,7 +20,10 @@
  *   tgx_soap_model_create               src/read_files.f90:660-724, 4641-5236  the soap_turbo block + sparse set
  *   tgx_soap_backend_begin / _end       src/gap.f90:49-64           soap_backend_begin / soap_backend_end
  *   tgx_get_gap_soap                    src/gap_interface.f90:41-480 get_gap_soap (marshal, get_soap, prediction, fold)
+ *   tgx_get_gap_soap_sites              src/gap_interface.f90:41-480 one this_i_beg:this_i_end batch of it
  *   tgx_get_soap                        soap_turbo get_soap, call shape src/gap_interface.f90:320-325
+ *   tgx_get_soap_list                   soap_turbo get_soap on a caller-marshalled list (the same call shape)
+ *   tgx_device_count                    src/gpu_context.f90:41-48   the device a rank binds
  *   tgx_get_soap_energy_and_forces      src/gap.f90:66-259          get_soap_energy_and_forces
  *   tgx_compute                         src/turbogap.f90:1253-1380,1941-1954  the per-step SOAP loop and totals
  *   tgx_local_property_predict          src/local_properties.f90:174-297 local_property_predict
@@ -102,6 +106,13 @@ int tgx_finalize(tgx_ctx* ctx);
 const char* tgx_last_error(void);
 /* gpu_memory_budget_init: free and total device memory in GiB. */
 int tgx_memory_budget(tgx_ctx* ctx, double* free_gib, double* total_gib);
+/* The device memory the FP64 plain path may hold for one block (the role of
+ * max_Gbytes_per_process, src/neighbors.f90:637-778): a block whose
+ * materialised dq/dr and per-centre rows exceed it runs in batches of
+ * consecutive centres that fit. 0 (the default) means 80% of the device. */
+int tgx_set_memory_budget(tgx_ctx* ctx, double gib);
+/* Number of HIP devices visible (the rank -> device map of gpu_context_init). */
+int tgx_device_count(int* n);
 /* Name of the device the context is bound to. */
 int tgx_device_name(tgx_ctx* ctx, char* buf, int buflen);
 
@@ -165,12 +176,25 @@ int tgx_soap_model_dim(tgx_ctx* ctx, int model_id, int* dim);
  * TGX_PATH_MIXED. n_sites_block / n_pairs_block may be NULL. */
 int tgx_get_gap_soap(tgx_ctx* ctx, int path, int do_forces, double* energies, double* forces, double* virial,
                      int* n_sites_block, int64_t* n_pairs_block);
+/* The same for the centres among sites i_beg..i_end (1-based, inclusive): one
+ * batch of the reference's loop over this_i_beg:this_i_end
+ * (src/turbogap.f90:1281-1344). */
+int tgx_get_gap_soap_sites(tgx_ctx* ctx, int path, int do_forces, int i_beg, int i_end, double* energies,
+                           double* forces, double* virial, int* n_sites_block, int64_t* n_pairs_block);
 /* get_soap for the current block: soap(dim, n_sites_block) in the block's
  * central-site order and, when soap_cart_der is not NULL, soap_cart_der(3, dim,
  * n_pairs_block) in the block's pair order (self pair first per site). Sizes
  * come from a preceding tgx_get_gap_soap or tgx_soap_block_size call. */
 int tgx_soap_block_size(tgx_ctx* ctx, int* n_sites_block, int64_t* n_pairs_block);
 int tgx_get_soap(tgx_ctx* ctx, double* soap, double* soap_cart_der);
+/* get_soap on a list the caller marshalled, soap_turbo's call shape
+ * (src/gap_interface.f90:320-325), with the current block's hypers: n_sites
+ * centres, n_neigh(n_sites) entries each (self pair first), per pair the
+ * 0-based index of its species in the block's species list (-1: a species the
+ * block does not see), rjs(P) and xyz(3, P). Writes soap(dim, n_sites) and, with
+ * do_derivatives, soap_cart_der(3, dim, P). */
+int tgx_get_soap_list(tgx_ctx* ctx, int n_sites, const int* n_neigh, int64_t n_pairs, const int* pair_species,
+                      const double* rjs, const double* xyz, int do_derivatives, double* soap, double* soap_cart_der);
 /* local_property_predict (src/local_properties.f90:174-297) with the current
  * block as the property model (its e0 is V0): V (n_sites_block) =
  * max(0, delta^2 sum_s alpha_s K_is^zeta + V0) and, with do_derivatives,

@@ -7,7 +7,9 @@ the committed fixture (energy 1e-8 relative, forces 1e-6 eV/Å). A second host
 (tgx_pair_example) creates and evaluates the distance_2b, angle_3b and core_pot
 families through the same shim, checked against the oracle. A third
 (tgx_md_example) runs the device MD loop through the shim's MD bindings and
-must match the Python driver making the same calls."""
+must match the Python driver making the same calls. A fourth
+(tgx_ref_example) makes the reference's own calls with its argument lists
+(get_gap_soap with 66 arguments over site batches, get_soap)."""
 import os
 import pathlib
 import subprocess
@@ -174,3 +176,62 @@ def test_fortran_md_loop_matches_python_driver(tmp_path):
     np.testing.assert_allclose(steps, np.array(ref), rtol=1e-15, atol=0)
     np.testing.assert_allclose(atoms[:, :3], x, rtol=1e-15, atol=1e-15)
     np.testing.assert_allclose(atoms[:, 3:], v, rtol=1e-15, atol=1e-18)
+
+
+REF_EXE = ROOT / "turbogap_amd" / "fortran" / "_build" / "tgx_ref_example"
+
+
+def _run_ref_example(d, sysm, models, names, n_batches):
+    with open(d / "positions.dat", "w") as f:
+        f.write(f"{sysm.n} {float(sysm.box[0, 0]):.17g}\n")
+        for p, s in zip(sysm.positions, sysm.species):
+            f.write(f"{p[0]:.17g} {p[1]:.17g} {p[2]:.17g} {int(s)} {names[int(s)]}\n")
+    with open(d / "blocks.dat", "w") as f:
+        f.write(f"{len(models)} {n_batches}\n")
+        for k, (h, Qs, al) in enumerate(models):
+            np.savetxt(d / f"alphas_{k}.dat", al, fmt="%.17g")
+            np.savetxt(d / f"desc_{k}.dat", Qs.reshape(-1), fmt="%.17g")
+            f.write(f"alphas_{k}.dat desc_{k}.dat {h.central_species} {len(h.species_types)}\n")
+            f.write(" ".join(names[int(c)] for c in h.species_types) + "\n")
+    out = subprocess.run([str(REF_EXE)], cwd=d, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    vals, F = {}, np.zeros((sysm.n, 3))
+    for line in (d / "out.dat").read_text().splitlines():
+        k, *v = line.split()
+        if k == "force":
+            F[int(v[0]) - 1] = [float(x) for x in v[1:]]
+        else:
+            vals[k] = np.array([float(x) for x in v])
+    return vals, F
+
+
+@pytest.mark.parametrize("system,n_batches", [("carbon216", 1), ("carbon216", 3), ("binary4096", 3)])
+def test_fortran_reference_call_surface(tmp_path, orc, system, n_batches):
+    """tgx_ref_example makes TurboGAP's own calls with the reference's argument
+    lists (gpu_context_init(params, rank, n_omp), build_neighbors_list with 20
+    arguments, soap_backend_begin(hypers), get_gap_soap with 66 arguments over
+    this_i_beg:this_i_end batches, soap_backend_end, per-batch energies and
+    forces zeroed and added as src/turbogap.f90:1281-1373 does): the totals
+    equal the oracle's (energy 1e-8 relative, forces and virial 1e-6). On the
+    one-block carbon box it also calls get_soap on the whole list, whose
+    descriptor and derivative sums equal the oracle's to 1e-10 relative."""
+    if not REF_EXE.exists():
+        pytest.fail(f"{REF_EXE} not built (make -C turbogap_amd/fortran)")
+    if system == "carbon216":
+        sysm, h, Qs, al = H.carbon216(200)
+        models = [(h, Qs, al)]
+        names = {6: "C"}
+    else:
+        sysm, models = H.subset(300, 4096)
+        names = {1: "A", 2: "B"}
+    vals, F = _run_ref_example(tmp_path, sysm, models, names, n_batches)
+    Eo, eno, Fo, Wo, nl = orc.compute(sysm, models)
+    assert abs(vals["energy"][0] - Eo) <= 1e-8 * abs(Eo)
+    assert np.abs(F - Fo).max() <= 1e-6
+    assert np.abs(vals["virial"].reshape(3, 3).T - Wo).max() <= 1e-6
+    if len(models) == 1:
+        h = models[0][0]
+        ps = np.zeros(nl["n_atom_pairs"], np.int32)
+        soap, der = orc.get_soap(h, nl["n_neigh"], ps, nl["rjs"], nl["xyz"])
+        assert abs(vals["soap_sum"][0] - soap.sum()) <= 1e-10 * np.abs(soap).sum()
+        assert abs(vals["soap_der_abs_sum"][0] - np.abs(der).sum()) <= 1e-10 * np.abs(der).sum()

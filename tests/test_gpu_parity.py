@@ -252,3 +252,62 @@ def test_perfect_crystal(ctx, orc, path):
     assert np.abs(F).max() <= 1e-9
     assert np.abs(W - Wo).max() <= 1e-6
     assert np.abs(W - np.diag(np.diag(W))).max() <= 1e-9 * np.abs(W).max()
+
+
+def test_plain_path_memory_budget_batches(ctx):
+    """get_number_of_atom_pairs's role (src/neighbors.f90:637-778): with a device
+    budget far below one block's materialised dq/dr, the FP64 plain path runs
+    each block's centres in several batches (more dq/dr launches than blocks);
+    E matches the unbatched run to 1e-12 relative, F and the virial to 1e-10
+    (the batches only change which launch adds each pair force)."""
+    sysm, models = H.subset(300, 4096)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    ctx.build_neighbors_list(3.7, fetch=False)
+    H.load_models(ctx, models)
+    E0, W0 = ctx.compute("fp64_plain")
+    e0, F0 = ctx.results()
+    ctx.set_memory_budget(0.05)
+    ctx.set_profiling(True)
+    ctx.reset_profile()
+    E1, W1 = ctx.compute("fp64_plain")
+    e1, F1 = ctx.results()
+    n_launch, _ = ctx.kernel_time("soap_site_der")
+    ctx.set_profiling(False)
+    ctx.set_memory_budget(0.0)
+    assert n_launch >= 2 * len(models) + 2, n_launch
+    assert abs(E1 - E0) <= 1e-12 * abs(E0)
+    np.testing.assert_allclose(e1, e0, rtol=0, atol=1e-13)
+    assert np.abs(F1 - F0).max() <= 1e-10 and np.abs(W1 - W0).max() <= 1e-10
+
+
+def test_get_gap_soap_site_batches(ctx, orc):
+    """get_gap_soap over this_i_beg:this_i_end batches, folded and summed as the
+    reference driver does (src/turbogap.f90:1314-1373: this_energies and
+    this_forces zeroed, assigned by each call, added to the totals): the totals of
+    three batches equal the one-call result, and the one-call result the oracle."""
+    sysm, models = H.subset(300, 4096)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    ctx.build_neighbors_list(3.7, fetch=False)
+    ids = H.load_models(ctx, models)
+    n = sysm.n
+    full_e, full_F, full_W = np.zeros(n), np.zeros((n, 3)), np.zeros((3, 3))
+    bat_e, bat_F, bat_W = np.zeros(n), np.zeros((n, 3)), np.zeros((3, 3))
+    cuts = [1, 1400, 2900, n + 1]
+    for mid in ids:
+        ctx.soap_backend_begin(mid)
+        e, F, W, _, _ = ctx.get_gap_soap("fp64_plain")
+        full_e += e
+        full_F += F
+        full_W += W
+        for b, c in zip(cuts[:-1], cuts[1:]):
+            e, F, W, ns, _ = ctx.get_gap_soap("fp64_plain", sites=(b, c - 1))
+            assert ns > 0
+            bat_e += e
+            bat_F += F
+            bat_W += W
+        ctx.soap_backend_end()
+    Eo, eno, Fo, Wo, _ = orc.compute(sysm, models)
+    np.testing.assert_allclose(full_e, eno, rtol=1e-8, atol=1e-12)
+    assert np.abs(full_F - Fo).max() <= 1e-6 and np.abs(full_W - Wo).max() <= 1e-6
+    np.testing.assert_allclose(bat_e, full_e, rtol=0, atol=1e-13)
+    assert np.abs(bat_F - full_F).max() <= 1e-10 and np.abs(bat_W - full_W).max() <= 1e-10

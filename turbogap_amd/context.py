@@ -238,18 +238,32 @@ class Context:
         self._current = None
 
     # -- one block -------------------------------------------------------------------
-    def get_gap_soap(self, path: str = "fp64_plain", do_forces: bool = True, energies=None, forces=None):
+    def get_gap_soap(self, path: str = "fp64_plain", do_forces: bool = True, energies=None, forces=None,
+                     sites=None):
         """get_gap_soap for the current block: energies/forces are ASSIGNED for the
-        atoms the block touches (src/gap_interface.f90:422-431)."""
+        atoms the block touches (src/gap_interface.f90:422-431). sites = (i_beg, i_end),
+        1-based inclusive, restricts the centres to one batch of the reference's
+        this_i_beg:this_i_end loop."""
         n = self.n_sites
         energies = np.zeros(n) if energies is None else energies
         forces = np.zeros((n, 3)) if forces is None else forces
         virial = np.zeros((3, 3))
         ns = C.c_int()
         npb = C.c_int64()
-        L.check(self._lib.tgx_get_gap_soap(self.h, L.PATHS[path], int(do_forces), L.dptr(energies), L.dptr(forces),
-                                           L.dptr(virial), C.byref(ns), C.byref(npb)), "get_gap_soap")
+        if sites is None:
+            L.check(self._lib.tgx_get_gap_soap(self.h, L.PATHS[path], int(do_forces), L.dptr(energies),
+                                               L.dptr(forces), L.dptr(virial), C.byref(ns), C.byref(npb)),
+                    "get_gap_soap")
+        else:
+            L.check(self._lib.tgx_get_gap_soap_sites(self.h, L.PATHS[path], int(do_forces), int(sites[0]),
+                                                     int(sites[1]), L.dptr(energies), L.dptr(forces),
+                                                     L.dptr(virial), C.byref(ns), C.byref(npb)), "get_gap_soap")
         return energies, forces, virial.T.copy(), ns.value, npb.value
+
+    def set_memory_budget(self, gib: float):
+        """Device memory (GiB) the FP64 plain path may hold for one block before it
+        splits the block's centres into batches (0: 80% of the device)."""
+        L.check(self._lib.tgx_set_memory_budget(self.h, float(gib)), "set_memory_budget")
 
     def get_soap(self, derivatives: bool = True):
         """Descriptors of the current block: soap (n_sites_block, D) and

@@ -9,7 +9,7 @@
 #include "tgx_util.h"
 
 int tgx_energy_scatter(tgx_ctx* ctx, tgx_model_t* m);
-int tgx_block_seen(tgx_ctx* ctx, tgx_model_t* m, int* d_seen);
+int tgx_block_seen(tgx_ctx* ctx, tgx_model_t* m, int* d_seen, bool clear);
 int tgx_radial_table_build(tgx_ctx* ctx, tgx_model_t* m);
 
 
@@ -496,11 +496,6 @@ static int run_selected_block(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_fo
   return TGX_OK;
 }
 
-static int run_block(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces) {
-  TGX_TRY(tgx_block_select(ctx, m));
-  return run_selected_block(ctx, m, path, do_forces);
-}
-
 static int begin_step(tgx_ctx* ctx) {
   TGX_REQUIRE(ctx->have_list, "no neighbour list: call tgx_build_neighbors_list first");
   const int n = ctx->n_sites;
@@ -514,50 +509,190 @@ static int begin_step(tgx_ctx* ctx) {
   return TGX_OK;
 }
 
-extern "C" int tgx_get_gap_soap(tgx_ctx* ctx, int path, int do_forces, double* energies, double* forces,
-                                double* virial, int* n_sites_block, int64_t* n_pairs_block) {
+// ---- memory-bounded batches of a block (src/neighbors.f90:637-778) -------------
+// Device bytes the FP64 plain path holds for ns centres and np pairs: the
+// materialised dq/dr (3 x D FP64 per pair) and the pair data, and per centre
+// q, Qss and K^(zeta-1) alpha rows.
+static double plain_block_bytes(const tgx_model_t* m, int64_t ns, int64_t np) {
+  return (24.0 * m->D + 64.0) * (double)np + (8.0 * (2.0 * m->Dpad + m->Spad) + 64.0) * (double)ns;
+}
+
+static int memory_budget_bytes(tgx_ctx* ctx, double* bytes) {
+  if (ctx->mem_budget > 0.0) {
+    *bytes = ctx->mem_budget;
+    return TGX_OK;
+  }
+  size_t f = 0, t = 0;
+  TGX_CHECK_HIP(hipMemGetInfo(&f, &t));
+  *bytes = 0.8 * (double)t;
+  return TGX_OK;
+}
+
+// After the count half of a block whose centres come from atoms [lo, hi): the
+// atom ranges, in order, whose centres' plain-path work space fits the budget
+// (get_number_of_atom_pairs's split of the centres into batches). One range
+// when the whole block fits. A single centre larger than the budget is a
+// range of its own.
+static int plain_batches(tgx_ctx* ctx, tgx_model_t* m, int lo, int hi, std::vector<std::pair<int, int>>& out) {
+  out.clear();
+  double budget = 0.0;
+  TGX_TRY(memory_budget_bytes(ctx, &budget));
+  if (plain_block_bytes(m, m->sel_host[0], m->sel_host[1]) <= budget) {
+    out.emplace_back(lo, hi);
+    return TGX_OK;
+  }
+  std::vector<int> flag;
+  std::vector<int64_t> off;
+  TGX_TRY(tgx_block_select_counts_host(ctx, m, flag, off));
+  int b = lo;
+  int64_t ns = 0, np = 0;
+  for (int i = lo; i < hi; ++i) {
+    if (!flag[(size_t)i]) continue;
+    const int64_t pi = off[(size_t)i + 1] - off[(size_t)i];
+    if (ns > 0 && plain_block_bytes(m, ns + 1, np + pi) > budget) {
+      out.emplace_back(b, i);
+      b = i;
+      ns = 0;
+      np = 0;
+    }
+    ++ns;
+    np += pi;
+  }
+  out.emplace_back(b, hi);
+  return TGX_OK;
+}
+
+// A block's energies and pair forces over the centres of atoms [lo, hi),
+// batched by the memory budget on the FP64 plain path. Energies are added into
+// the context's per-atom energies and pair forces into fglob. When `sites` is
+// given, the centres' atoms and energies are appended to it and `seen` marks
+// (device) the atoms the pairs reach. Runs on ctx->stream.
+struct BlockOut {
+  std::vector<int>* atoms = nullptr;
+  std::vector<double>* e = nullptr;
+  int* d_seen = nullptr;
+};
+
+static int run_block_range(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces, int lo, int hi, BlockOut* out,
+                           int* n_sites_tot, int64_t* n_pairs_tot) {
+  const int save_lo = m->sel_lo, save_hi = m->sel_hi;
+  struct Restore {
+    tgx_model_t* m;
+    int lo, hi;
+    ~Restore() { m->sel_lo = lo; m->sel_hi = hi; }
+  } restore{m, save_lo, save_hi};
+  m->sel_lo = lo;
+  m->sel_hi = hi;
+  TGX_TRY(tgx_block_select_count(ctx, m));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<std::pair<int, int>> batches;
+  if (path == TGX_PATH_FP64_PLAIN && do_forces) TGX_TRY(plain_batches(ctx, m, lo, hi, batches));
+  else batches.emplace_back(lo, hi);
+  int ns_tot = 0;
+  int64_t np_tot = 0;
+  for (size_t k = 0; k < batches.size(); ++k) {
+    if (batches.size() > 1) {  // this batch's selection
+      m->sel_lo = batches[k].first;
+      m->sel_hi = batches[k].second;
+      TGX_TRY(tgx_block_select_count(ctx, m));
+      TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    TGX_TRY(tgx_block_select_fill(ctx, m));
+    TGX_TRY(run_selected_block(ctx, m, path, do_forces));
+    TGX_TRY(tgx_energy_scatter(ctx, m));
+    ns_tot += m->n_sites_b;
+    np_tot += m->n_pairs_b;
+    if (out && out->atoms && m->n_sites_b > 0) {
+      const size_t b0 = out->atoms->size();
+      out->atoms->resize(b0 + (size_t)m->n_sites_b);
+      out->e->resize(b0 + (size_t)m->n_sites_b);
+      TGX_CHECK_HIP(hipMemcpyAsync(out->atoms->data() + b0, m->site_atom.p, sizeof(int) * (size_t)m->n_sites_b,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+      TGX_CHECK_HIP(hipMemcpyAsync(out->e->data() + b0, m->e_site.p, sizeof(double) * (size_t)m->n_sites_b,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+      TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    if (out && out->d_seen) TGX_TRY(tgx_block_seen(ctx, m, out->d_seen, false));
+  }
+  if (n_sites_tot) *n_sites_tot = ns_tot;
+  if (n_pairs_tot) *n_pairs_tot = np_tot;
+  return TGX_OK;
+}
+
+extern "C" int tgx_set_memory_budget(tgx_ctx* ctx, double gib) {
   TGX_CTX_CHECK(ctx);
+  TGX_REQUIRE(gib >= 0.0, "tgx_set_memory_budget: negative budget");
+  ctx->mem_budget = gib * 1024.0 * 1024.0 * 1024.0;
+  return TGX_OK;
+}
+
+// get_gap_soap over the centres of atoms [lo, hi) (0-based, hi exclusive).
+static int get_gap_soap_impl(tgx_ctx* ctx, int path, int do_forces, int lo, int hi, double* energies,
+                             double* forces, double* virial, int* n_sites_block, int64_t* n_pairs_block) {
   TGX_REQUIRE(path == TGX_PATH_FP64_PLAIN || path == TGX_PATH_FP64_ADJOINT || path == TGX_PATH_MIXED,
               "tgx_get_gap_soap: path %d not available here", path);
   tgx_model_t* m;
   TGX_TRY(current_model(ctx, &m));
   TGX_TRY(begin_step(ctx));
-  TGX_TRY(run_block(ctx, m, path, do_forces != 0));
-  TGX_TRY(tgx_energy_scatter(ctx, m));
   const int n = ctx->n_sites;
-  double vir[9] = {0};
-  if (do_forces) TGX_TRY(tgx_gather_forces(ctx, nullptr, vir));
-  // fold back like src/gap_interface.f90:422-431: assign, for touched atoms only
-  std::vector<int> sa(m->n_sites_b), seen(n, 0);
-  std::vector<double> es(m->n_sites_b), F;
-  if (m->n_sites_b) {
-    TGX_CHECK_HIP(hipMemcpyAsync(sa.data(), m->site_atom.p, sizeof(int) * sa.size(), hipMemcpyDeviceToHost, ctx->stream));
-    TGX_CHECK_HIP(hipMemcpyAsync(es.data(), m->e_site.p, sizeof(double) * es.size(), hipMemcpyDeviceToHost, ctx->stream));
-  }
+  TGX_REQUIRE(lo >= 0 && lo <= hi && hi <= n, "tgx_get_gap_soap: sites [%d, %d) outside [0, %d)", lo, hi, n);
+  std::vector<int> sa;
+  std::vector<double> es;
+  BlockOut out;
+  out.atoms = &sa;
+  out.e = &es;
   if (do_forces) {
     TGX_TRY(ctx->gate_mask.ensure(sizeof(int) * (size_t)n));
-    TGX_TRY(tgx_block_seen(ctx, m, ctx->gate_mask.as<int>()));
+    TGX_TRY(tgx_zero(ctx, ctx->gate_mask.p, sizeof(int) * (size_t)n));
+    out.d_seen = ctx->gate_mask.as<int>();
+  }
+  int ns_b = 0;
+  int64_t np_b = 0;
+  TGX_TRY(run_block_range(ctx, m, path, do_forces != 0, lo, hi, &out, &ns_b, &np_b));
+  double vir[9] = {0};
+  std::vector<int> seen;
+  std::vector<double> F;
+  if (do_forces) {
+    TGX_TRY(tgx_gather_forces(ctx, nullptr, vir));
+    seen.resize((size_t)n);
     F.resize(3 * (size_t)n);
     TGX_CHECK_HIP(hipMemcpyAsync(seen.data(), ctx->gate_mask.p, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost,
                                  ctx->stream));
     TGX_CHECK_HIP(hipMemcpyAsync(F.data(), ctx->forces.p, sizeof(double) * F.size(), hipMemcpyDeviceToHost, ctx->stream));
+    TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   }
-  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  // fold back like src/gap_interface.f90:422-431: assign, for touched atoms only
   if (energies)
-    for (int s = 0; s < m->n_sites_b; ++s) energies[sa[s]] = es[s];
+    for (size_t s = 0; s < sa.size(); ++s) energies[sa[s]] = es[s];
   if (do_forces) {
     if (forces)
       for (int i = 0; i < n; ++i)
-        if (seen[i])
+        if (seen[(size_t)i])
           for (int k = 0; k < 3; ++k) forces[3 * (size_t)i + k] = F[3 * (size_t)i + k];
     if (virial)
       for (int k = 0; k < 9; ++k) virial[k] = vir[k];
   } else if (virial) {
     for (int k = 0; k < 9; ++k) virial[k] = 0.0;
   }
-  if (n_sites_block) *n_sites_block = m->n_sites_b;
-  if (n_pairs_block) *n_pairs_block = m->n_pairs_b;
+  if (n_sites_block) *n_sites_block = ns_b;
+  if (n_pairs_block) *n_pairs_block = np_b;
   return TGX_OK;
+}
+
+extern "C" int tgx_get_gap_soap(tgx_ctx* ctx, int path, int do_forces, double* energies, double* forces,
+                                double* virial, int* n_sites_block, int64_t* n_pairs_block) {
+  TGX_CTX_CHECK(ctx);
+  return get_gap_soap_impl(ctx, path, do_forces, 0, ctx->n_sites, energies, forces, virial, n_sites_block,
+                           n_pairs_block);
+}
+
+extern "C" int tgx_get_gap_soap_sites(tgx_ctx* ctx, int path, int do_forces, int i_beg, int i_end, double* energies,
+                                      double* forces, double* virial, int* n_sites_block, int64_t* n_pairs_block) {
+  TGX_CTX_CHECK(ctx);
+  TGX_REQUIRE(i_beg >= 1 && i_end >= i_beg - 1 && i_end <= ctx->n_sites,
+              "tgx_get_gap_soap_sites: sites %d..%d outside 1..%d", i_beg, i_end, ctx->n_sites);
+  return get_gap_soap_impl(ctx, path, do_forces, i_beg - 1, i_end, energies, forces, virial, n_sites_block,
+                           n_pairs_block);
 }
 
 extern "C" int tgx_local_property_predict(tgx_ctx* ctx, int do_derivatives, double* V, double* V_der,
@@ -606,6 +741,73 @@ extern "C" int tgx_get_soap(tgx_ctx* ctx, double* soap, double* soap_cart_der) {
     TGX_CHECK_HIP(hipMemcpyAsync(soap_cart_der, m->soap_der.p, sizeof(double) * 3 * m->D * (size_t)m->n_pairs_b,
                                  hipMemcpyDeviceToHost, ctx->stream));
   TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return TGX_OK;
+}
+
+// get_soap on a list the caller marshalled (soap_turbo's call shape,
+// src/gap_interface.f90:320-325): the current block's hypers, centres with
+// their pairs in order (self pair first), each pair's species as a 0-based
+// entry of the block's species list (-1: not seen).
+extern "C" int tgx_get_soap_list(tgx_ctx* ctx, int n_sites, const int* n_neigh, int64_t n_pairs,
+                                 const int* pair_species, const double* rjs, const double* xyz, int do_derivatives,
+                                 double* soap, double* soap_cart_der) {
+  TGX_CTX_CHECK(ctx);
+  tgx_model_t* m;
+  TGX_TRY(current_model(ctx, &m));
+  TGX_REQUIRE(n_sites >= 0 && n_pairs >= 0 && (n_sites == 0 || (n_neigh && soap)) &&
+                  (n_pairs == 0 || (pair_species && rjs && xyz)) && (!do_derivatives || n_pairs == 0 || soap_cart_der),
+              "tgx_get_soap_list: bad arguments");
+  if (n_sites == 0) return TGX_OK;
+  std::vector<int64_t> beg((size_t)n_sites + 1, 0), gidx((size_t)(n_pairs > 0 ? n_pairs : 1));
+  std::vector<int> atom((size_t)n_sites), zero((size_t)(n_pairs > 0 ? n_pairs : 1), 0);
+  for (int i = 0; i < n_sites; ++i) {
+    TGX_REQUIRE(n_neigh[i] >= 1, "tgx_get_soap_list: site %d has no self entry", i + 1);
+    beg[(size_t)i + 1] = beg[(size_t)i] + n_neigh[i];
+    atom[(size_t)i] = i;
+  }
+  TGX_REQUIRE(beg[(size_t)n_sites] == n_pairs, "tgx_get_soap_list: sum of n_neigh (%lld) differs from n_pairs (%lld)",
+              (long long)beg[(size_t)n_sites], (long long)n_pairs);
+  for (int64_t p = 0; p < n_pairs; ++p) {
+    TGX_REQUIRE(pair_species[p] >= -1 && pair_species[p] < m->n_species,
+                "tgx_get_soap_list: pair %lld has species index %d", (long long)p + 1, pair_species[p]);
+    gidx[(size_t)p] = p;
+  }
+  const size_t np = (size_t)(n_pairs > 0 ? n_pairs : 1);
+  TGX_TRY(m->site_atom.ensure(sizeof(int) * (size_t)n_sites));
+  TGX_TRY(m->site_beg.ensure(sizeof(int64_t) * beg.size()));
+  TGX_TRY(m->pr_gidx.ensure(sizeof(int64_t) * np));
+  TGX_TRY(m->pr_nbr.ensure(sizeof(int) * np));
+  TGX_TRY(m->pr_spec.ensure(sizeof(int) * np));
+  TGX_TRY(m->pr_r.ensure(sizeof(double) * np));
+  TGX_TRY(m->pr_xyz.ensure(sizeof(double) * 3 * np));
+  TGX_CHECK_HIP(hipMemcpyAsync(m->site_atom.p, atom.data(), sizeof(int) * atom.size(), hipMemcpyHostToDevice, ctx->stream));
+  TGX_CHECK_HIP(hipMemcpyAsync(m->site_beg.p, beg.data(), sizeof(int64_t) * beg.size(), hipMemcpyHostToDevice, ctx->stream));
+  if (n_pairs > 0) {
+    TGX_CHECK_HIP(hipMemcpyAsync(m->pr_gidx.p, gidx.data(), sizeof(int64_t) * (size_t)n_pairs, hipMemcpyHostToDevice,
+                                 ctx->stream));
+    TGX_CHECK_HIP(hipMemcpyAsync(m->pr_nbr.p, zero.data(), sizeof(int) * (size_t)n_pairs, hipMemcpyHostToDevice,
+                                 ctx->stream));
+    TGX_CHECK_HIP(hipMemcpyAsync(m->pr_spec.p, pair_species, sizeof(int) * (size_t)n_pairs, hipMemcpyHostToDevice,
+                                 ctx->stream));
+    TGX_CHECK_HIP(hipMemcpyAsync(m->pr_r.p, rjs, sizeof(double) * (size_t)n_pairs, hipMemcpyHostToDevice, ctx->stream));
+    TGX_CHECK_HIP(hipMemcpyAsync(m->pr_xyz.p, xyz, sizeof(double) * 3 * (size_t)n_pairs, hipMemcpyHostToDevice,
+                                 ctx->stream));
+  }
+  m->n_sites_b = n_sites;
+  m->n_pairs_b = n_pairs;
+  TGX_TRY(tgx_soap_descriptors(ctx, m, TGX_PATH_FP64_PLAIN, do_derivatives != 0 && n_pairs > 0));
+  TGX_CHECK_HIP(hipMemcpy2DAsync(soap, sizeof(double) * m->D, m->q64.p, sizeof(double) * m->Dpad, sizeof(double) * m->D,
+                                 (size_t)n_sites, hipMemcpyDeviceToHost, ctx->stream));
+  if (do_derivatives && n_pairs > 0)
+    TGX_CHECK_HIP(hipMemcpyAsync(soap_cart_der, m->soap_der.p, sizeof(double) * 3 * m->D * (size_t)n_pairs,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return TGX_OK;
+}
+
+extern "C" int tgx_device_count(int* n) {
+  TGX_REQUIRE(n, "tgx_device_count: null output");
+  TGX_CHECK_HIP(hipGetDeviceCount(n));
   return TGX_OK;
 }
 
@@ -775,12 +977,19 @@ extern "C" int tgx_compute(tgx_ctx* ctx, int path, double* energy, double* viria
     }
     ctx->gated_fraction = ctx->gate_sites > 0 ? (double)ctx->gate_gated / (double)ctx->gate_sites : 0.0;
   } else {
+    double budget = 0.0;
+    TGX_TRY(memory_budget_bytes(ctx, &budget));
     for (auto* m : ctx->models) {
       ctx->stream = block_stream(m);
       TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-      TGX_TRY(tgx_block_select_fill(ctx, m));
-      TGX_TRY(run_selected_block(ctx, m, path, true));
-      TGX_TRY(tgx_energy_scatter(ctx, m));
+      if (path == TGX_PATH_FP64_PLAIN && plain_block_bytes(m, m->sel_host[0], m->sel_host[1]) > budget) {
+        // dq/dr of the whole block does not fit: its centres in memory-bounded batches
+        TGX_TRY(run_block_range(ctx, m, path, true, 0, ctx->n_sites, nullptr, nullptr, nullptr));
+      } else {
+        TGX_TRY(tgx_block_select_fill(ctx, m));
+        TGX_TRY(run_selected_block(ctx, m, path, true));
+        TGX_TRY(tgx_energy_scatter(ctx, m));
+      }
       if (concurrent) TGX_CHECK_HIP(hipEventRecord(m->done, m->stream));
     }
   }

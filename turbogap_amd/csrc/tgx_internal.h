@@ -159,6 +159,7 @@ struct tgx_model_t {
   dbuf scan_tmp;    // block totals of the selection scans
   dbuf pair_site;   // int[n_pairs_b]: centre of every kept pair (plain force pass)
   int64_t* sel_host = nullptr;  // pinned: the selection's site and pair totals
+  int sel_lo = 0, sel_hi = -1;  // centres are taken from atoms [sel_lo, sel_hi) only (sel_hi < 0: all)
   dbuf mlp_ws;      // surrogate work space (activations, member energies)
   hipStream_t stream = nullptr;  // the block's own stream when blocks run concurrently
   hipEvent_t done = nullptr;     // recorded on that stream after the block's last launch
@@ -202,6 +203,7 @@ struct tgx_ctx {
   // results
   dbuf energies, forces, fglob, virial_part, scratch, red;
   int results_n = 0;  // atoms the energies / forces buffers hold results for (0: none)
+  double mem_budget = 0.0;  // device bytes the FP64 plain path may hold for one block (0: 80% of the device)
   // md
   int md_n = 0;       // atoms of the MD state (velocities, masses, previous step); 0: none
   dbuf vel, mass, forces_prev, pos_prev;
@@ -267,6 +269,9 @@ int tgx_block_select(tgx_ctx* ctx, tgx_model_t* m);
 // pinned memory (no wait), then, after the caller has synchronised the stream,
 // allocation and fill.
 int tgx_block_select_count(tgx_ctx* ctx, tgx_model_t* m);
+// Host copies of the count half's per-atom centre flags and pair offsets
+// (n and n + 1 values), after the stream has been synchronised.
+int tgx_block_select_counts_host(tgx_ctx* ctx, tgx_model_t* m, std::vector<int>& flag, std::vector<int64_t>& pair_off);
 int tgx_block_select_fill(tgx_ctx* ctx, tgx_model_t* m);
 int tgx_soap_descriptors(tgx_ctx* ctx, tgx_model_t* m, int path, bool with_der);
 int tgx_gap_contract(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces);

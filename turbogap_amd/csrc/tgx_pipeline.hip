@@ -24,14 +24,15 @@ struct SpeciesMap {
 // A site is a centre of the block when its species is the block's central
 // species (or the block takes every atom); a pair is kept when it lies inside
 // the block's largest cutoff and the block sees the neighbour's species.
-__global__ void k_select_count(int n, SpeciesMap sm, int central, double rcut, const int* __restrict__ n_neigh,
-                               const int64_t* __restrict__ off, const int* __restrict__ species,
-                               const double* __restrict__ rjs, const int* __restrict__ nspec, int* __restrict__ flag,
-                               int* __restrict__ npairs) {
+// Only atoms in [lo, hi) can be centres (a batch of the block's sites).
+__global__ void k_select_count(int n, int lo, int hi, SpeciesMap sm, int central, double rcut,
+                               const int* __restrict__ n_neigh, const int64_t* __restrict__ off,
+                               const int* __restrict__ species, const double* __restrict__ rjs,
+                               const int* __restrict__ nspec, int* __restrict__ flag, int* __restrict__ npairs) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int sc = sm.index(species[i]);
-  const bool sel = n_neigh[i] > 0 && sc >= 0 && (central == 0 || sc == central - 1);
+  const bool sel = i >= lo && i < hi && n_neigh[i] > 0 && sc >= 0 && (central == 0 || sc == central - 1);
   int cnt = 0;
   if (sel) {
     const int64_t b = off[i];
@@ -181,13 +182,26 @@ int tgx_block_select_count(tgx_ctx* ctx, tgx_model_t* m) {
   TGX_TRY(m->sel_scratch.ensure(sizeof(int) * 2 * (size_t)n + sizeof(int64_t) * 2 * (size_t)(n + 1) + 256));
   const SelectView v = select_view(m, n);
   SpeciesMap sm = species_map(m);
-  TGX_LAUNCH(ctx, "select_count", k_select_count, dim3((n + 255) / 256), dim3(256), 0, n, sm, m->central_species,
-             m->rcut_max, ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(), ctx->species_sc.as<int>(),
-             ctx->rjs.as<double>(), ctx->nspec.as<int>(), v.flag, v.npairs);
+  const int lo = m->sel_lo > 0 ? m->sel_lo : 0, hi = (m->sel_hi >= 0 && m->sel_hi < n) ? m->sel_hi : n;
+  TGX_LAUNCH(ctx, "select_count", k_select_count, dim3((n + 255) / 256), dim3(256), 0, n, lo, hi, sm,
+             m->central_species, m->rcut_max, ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(),
+             ctx->species_sc.as<int>(), ctx->rjs.as<double>(), ctx->nspec.as<int>(), v.flag, v.npairs);
   TGX_TRY(tgx_scan2_i32(ctx, v.flag, v.site_idx, v.npairs, v.pair_off, n, m->scan_tmp));
   if (!m->sel_host) TGX_CHECK_HIP(hipHostMalloc((void**)&m->sel_host, 2 * sizeof(int64_t), hipHostMallocDefault));
   TGX_CHECK_HIP(hipMemcpyAsync(m->sel_host, v.site_idx + n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
   TGX_CHECK_HIP(hipMemcpyAsync(m->sel_host + 1, v.pair_off + n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+  return TGX_OK;
+}
+
+int tgx_block_select_counts_host(tgx_ctx* ctx, tgx_model_t* m, std::vector<int>& flag, std::vector<int64_t>& pair_off) {
+  const int n = ctx->n_sites;
+  const SelectView v = select_view(m, n);
+  flag.resize((size_t)n);
+  pair_off.resize((size_t)n + 1);
+  TGX_CHECK_HIP(hipMemcpyAsync(flag.data(), v.flag, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+  TGX_CHECK_HIP(hipMemcpyAsync(pair_off.data(), v.pair_off, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyDeviceToHost,
+                               ctx->stream));
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   return TGX_OK;
 }
 
@@ -285,8 +299,8 @@ __global__ void k_mark_seen(int64_t np, const int* __restrict__ pr_nbr, int* __r
   if (p < np) seen[pr_nbr[p]] = 1;
 }
 
-int tgx_block_seen(tgx_ctx* ctx, tgx_model_t* m, int* d_seen) {
-  TGX_TRY(tgx_zero(ctx, d_seen, sizeof(int) * (size_t)ctx->n_sites));
+int tgx_block_seen(tgx_ctx* ctx, tgx_model_t* m, int* d_seen, bool clear) {
+  if (clear) TGX_TRY(tgx_zero(ctx, d_seen, sizeof(int) * (size_t)ctx->n_sites));
   const int64_t np = m->n_pairs_b;
   if (np == 0) return TGX_OK;
   TGX_LAUNCH(ctx, "mark_seen", k_mark_seen, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, np, m->pr_nbr.as<int>(),

@@ -5,13 +5,17 @@
 ! iso_c_binding. Device memory, streams and kernels stay behind the C ABI: the
 ! Fortran side never holds a device pointer.
 !
-!   gpu_context_init / gpu_context_finalize   src/gpu_context.f90:41-58
+!   gpu_context_init(params, rank, n_omp) /
+!   gpu_context_finalize(params, n_omp)       src/gpu_context.f90:41-58 (also (device) / ())
 !   build_neighbors_list                      src/neighbors.f90:243-605
 !   number_of_unit_cells_for_given_cutoff     src/neighbors.f90:175-236
 !   build_supercell                           src/read_files.f90:320-367 (the replication in read_xyz)
 !   soap_model_from_files                     src/read_files.f90:660-724 (alphas / desc_sparse)
-!   soap_backend_begin / soap_backend_end     src/gap.f90:49-64
-!   get_gap_soap                              src/gap_interface.f90:41-480 (results on the host)
+!   soap_backend_begin(hypers) / _end         src/gap.f90:49-64 (also (model_id))
+!   get_gap_soap                              src/gap_interface.f90:41-480, the reference's 66
+!                                             arguments (also (path, do_forces, energies0,
+!                                             forces0, virial) on the resident list and block)
+!   get_soap                                  soap_turbo get_soap, call shape src/gap_interface.f90:320-325
 !   get_soap_energy_and_forces                src/gap.f90:66-259
 !   tgx_distance_2b_model / tgx_angle_3b_model / tgx_core_pot_model and
 !   add_pair_contribution                     the gap_backend seam's add_2b_ /
@@ -31,6 +35,7 @@
 ! behaviour of turbogap_abort (src/error.f90:48-61).
 module tgx_turbogap
    use iso_c_binding
+   use tgx_types
    implicit none
    private
 
@@ -46,7 +51,34 @@ module tgx_turbogap
 
    type(c_ptr), save :: ctx = c_null_ptr
 
+   ! The reference surface's state: the list build_neighbors_list last made
+   ! resident (checked against the lists get_gap_soap is handed), species names
+   ! to the integer codes of that build, the hypers of soap_backend_begin, and
+   ! the blocks created for them (keyed by the address of their sparse set).
+   integer, allocatable, save :: res_species(:), res_n_neigh(:), res_nlist(:)
+   integer, save :: res_n_sites = 0
+   character(len=8), save :: sp_name(64)
+   integer, save :: sp_code(64), n_sp = 0
+   type(soap_turbo), pointer, save :: cur_hypers => null()
+   type(c_ptr), save :: reg_key(128), reg_desc_key(128)
+   integer, save :: reg_id(128), reg_desc_id(128), n_reg = 0, n_reg_desc = 0
+   integer(c_int), save :: gap_path = 0
+
+   interface gpu_context_init
+      module procedure gpu_context_init_device, gpu_context_init_params
+   end interface
+   interface gpu_context_finalize
+      module procedure gpu_context_finalize_default, gpu_context_finalize_params
+   end interface
+   interface soap_backend_begin
+      module procedure soap_backend_begin_id, soap_backend_begin_hypers
+   end interface
+   interface get_gap_soap
+      module procedure get_gap_soap_resident, get_gap_soap_ref
+   end interface
+
    public :: gpu_context_init, gpu_context_finalize, build_neighbors_list, soap_model_from_files, &
+             get_soap, tgx_set_gap_path, tgx_set_memory_budget, &
              number_of_unit_cells_for_given_cutoff, build_supercell, &
              soap_backend_begin, soap_backend_end, get_gap_soap, get_soap_energy_and_forces, tgx_check, &
              tgx_distance_2b_model, tgx_angle_3b_model, tgx_core_pot_model, add_pair_contribution, &
@@ -283,6 +315,33 @@ module tgx_turbogap
          real(c_double) :: pos(*), box(*), psc(*), bsc(*)
          integer(c_int) :: spec(*), ind(*), ssc(*)
       end function
+      integer(c_int) function c_tgx_gap_soap_sites(c, path, do_forces, ib, ie, en, f, vir, nsb, npb) &
+         bind(C, name="tgx_get_gap_soap_sites")
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value :: c
+         integer(c_int), value :: path, do_forces, ib, ie
+         real(c_double) :: en(*), f(*), vir(*)
+         integer(c_int) :: nsb
+         integer(c_int64_t) :: npb
+      end function
+      integer(c_int) function c_tgx_soap_list(c, n, nn, np, ps, rjs, xyz, der, soap, sder) &
+         bind(C, name="tgx_get_soap_list")
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value :: c
+         integer(c_int), value :: n, der
+         integer(c_int64_t), value :: np
+         integer(c_int) :: nn(*), ps(*)
+         real(c_double) :: rjs(*), xyz(*), soap(*), sder(*)
+      end function
+      integer(c_int) function c_tgx_device_count(n) bind(C, name="tgx_device_count")
+         import :: c_int
+         integer(c_int) :: n
+      end function
+      integer(c_int) function c_tgx_set_budget(c, gib) bind(C, name="tgx_set_memory_budget")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c
+         real(c_double), value :: gib
+      end function
       integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
          import :: c_size_t, c_ptr
          type(c_ptr), value :: s
@@ -303,14 +362,53 @@ contains
       error stop 1
    end subroutine
 
-   subroutine gpu_context_init(device)
+   subroutine gpu_context_init_device(device)
       integer, intent(in) :: device
       call tgx_check(c_tgx_init(int(device, c_int), ctx), "gpu_context_init")
    end subroutine
 
-   subroutine gpu_context_finalize()
+   ! gpu_context_init with the reference's arguments (src/gpu_context.f90:41-48):
+   ! rank r binds device mod(r, number of devices); one host thread per rank.
+   subroutine gpu_context_init_params(params, rank, n_omp)
+      type(input_parameters), intent(in) :: params
+      integer, intent(in) :: rank
+      integer, intent(out) :: n_omp
+      integer(c_int) :: nd
+      call tgx_check(c_tgx_device_count(nd), "gpu_context_init")
+      if (nd < 1) then
+         write (*, '(A)') 'ERROR in gpu_context_init: no HIP device'
+         error stop 1
+      end if
+      call tgx_check(c_tgx_init(int(mod(rank, nd), c_int), ctx), "gpu_context_init")
+      n_omp = 1
+   end subroutine
+
+   subroutine gpu_context_finalize_default()
       if (c_associated(ctx)) call tgx_check(c_tgx_finalize(ctx), "gpu_context_finalize")
       ctx = c_null_ptr
+      n_reg = 0
+      n_reg_desc = 0
+      nullify (cur_hypers)
+   end subroutine
+
+   ! gpu_context_finalize with the reference's arguments (src/gpu_context.f90:52-58).
+   subroutine gpu_context_finalize_params(params, n_omp)
+      type(input_parameters), intent(in) :: params
+      integer, intent(in) :: n_omp
+      call gpu_context_finalize_default()
+   end subroutine
+
+   ! The device path's choice for get_gap_soap (TGX_PATH_*; FP64 plain by default).
+   subroutine tgx_set_gap_path(path)
+      integer, intent(in) :: path
+      gap_path = int(path, c_int)
+   end subroutine
+
+   ! Device memory (GiB) a block of the FP64 plain path may hold before its centres
+   ! run in batches (0: 80% of the device).
+   subroutine tgx_set_memory_budget(gib)
+      real(dp), intent(in) :: gib
+      call tgx_check(c_tgx_set_budget(ctx, gib), "tgx_set_memory_budget")
    end subroutine
 
    ! number_of_unit_cells_for_given_cutoff with the reference's arguments
@@ -408,6 +506,10 @@ contains
       allocate (n_neigh(n_sites), neighbors_list(n_atom_pairs), neighbor_species(n_atom_pairs))
       call tgx_check(c_tgx_get_nl(ctx, n_neigh, neighbors_list, rjs, thetas, phis, xyz, neighbor_species), &
                      "build_neighbors_list")
+      res_n_sites = n_sites
+      res_species = species_supercell
+      res_n_neigh = n_neigh
+      res_nlist = neighbors_list
       if (do_timing .and. rank == 0) then
          call system_clock(t1)
          write (*, '(A, F9.3, A)') '  *) Neighbors build: ', real(t1 - t0, dp)/real(rate, dp), ' seconds |'
@@ -488,19 +590,27 @@ contains
       model_id = int(mid)
    end subroutine
 
-   subroutine soap_backend_begin(model_id)
+   subroutine soap_backend_begin_id(model_id)
       integer, intent(in) :: model_id
       call tgx_check(c_tgx_begin(ctx, int(model_id, c_int)), "soap_backend_begin")
    end subroutine
 
+   ! soap_backend_begin(hypers) (src/gap.f90:49-58): the block get_gap_soap and
+   ! get_soap use next; its sparse set goes to the device on first use and stays.
+   subroutine soap_backend_begin_hypers(hypers)
+      type(soap_turbo), intent(in), target :: hypers
+      cur_hypers => hypers
+   end subroutine
+
    subroutine soap_backend_end()
+      nullify (cur_hypers)
       call tgx_check(c_tgx_end(ctx), "soap_backend_end")
    end subroutine
 
    ! get_gap_soap for the current block on the list last built: energies0 and
    ! forces0(3, n_sites) are ASSIGNED for the atoms the block touches
    ! (src/gap_interface.f90:422-431); virial(3,3) is written.
-   subroutine get_gap_soap(path, do_forces, energies0, forces0, virial)
+   subroutine get_gap_soap_resident(path, do_forces, energies0, forces0, virial)
       integer, intent(in) :: path
       logical, intent(in) :: do_forces
       real(dp), intent(inout) :: energies0(:), forces0(:, :)
@@ -509,6 +619,295 @@ contains
       integer(c_int64_t) :: npb
       call tgx_check(c_tgx_gap_soap(ctx, int(path, c_int), merge(1_c_int, 0_c_int, do_forces), energies0, forces0, &
                                     virial, nsb, npb), "get_gap_soap")
+   end subroutine
+
+   ! ---- the reference call surface (src/gap_interface.f90:41-480) --------------
+
+   ! Species names of the system to the integer codes of the last build, from
+   ! the names the caller passes per supercell atom.
+   subroutine learn_species(names)
+      character*8, intent(in) :: names(:)
+      integer :: k, t
+      logical :: known
+      do k = 1, min(size(names), size(res_species))
+         known = .false.
+         do t = 1, n_sp
+            if (sp_name(t) == names(k)) then
+               known = .true.
+               exit
+            end if
+         end do
+         if (.not. known .and. n_sp < size(sp_name)) then
+            n_sp = n_sp + 1
+            sp_name(n_sp) = names(k)
+            sp_code(n_sp) = res_species(k)
+         end if
+      end do
+   end subroutine
+
+   integer function species_code(name, slot)
+      character*8, intent(in) :: name
+      integer, intent(in) :: slot
+      integer :: t
+      species_code = 1000000 + slot  ! a species absent from the system
+      do t = 1, n_sp
+         if (sp_name(t) == name) then
+            species_code = sp_code(t)
+            return
+         end if
+      end do
+   end function
+
+   subroutine require(ok, what)
+      logical, intent(in) :: ok
+      character(len=*), intent(in) :: what
+      if (ok) return
+      write (*, '(A,A)') 'ERROR in get_gap_soap: ', what
+      error stop 1
+   end subroutine
+
+   ! The block of the current hypers' sparse set with the given SOAP hypers,
+   ! created on first use (the sparse set stays on the device). `for_energy`
+   ! blocks carry the system's species codes; descriptor-only blocks
+   ! (get_soap) any distinct codes.
+   integer function block_for(for_energy, species_types, alpha_max, l_max, n_soap, rcut_hard, rcut_soft, nf, &
+                              global_scaling, atom_sigma_r, atom_sigma_r_scaling, atom_sigma_t, &
+                              atom_sigma_t_scaling, amplitude_scaling, radial_enhancement, central_weight, basis, &
+                              scaling_mode, delta, zeta, central_species) result(mid_out)
+      logical, intent(in) :: for_energy
+      character*8, intent(in) :: species_types(:)
+      integer, intent(in) :: alpha_max(:), l_max, n_soap, radial_enhancement, central_species
+      real(dp), intent(in) :: rcut_hard(:), rcut_soft(:), nf(:), global_scaling(:), atom_sigma_r(:), &
+                              atom_sigma_r_scaling(:), atom_sigma_t(:), atom_sigma_t_scaling(:), &
+                              amplitude_scaling(:), central_weight(:), delta, zeta
+      character(len=*), intent(in) :: basis, scaling_mode
+      type(c_ptr) :: key
+      integer :: k, ns, nsp
+      integer(c_int), allocatable, target :: am(:)
+      integer(c_int), allocatable :: st(:)
+      real(dp), allocatable, target :: r_h(:), r_s(:), nf_(:), gs(:), sr(:), srs(:), stt(:), sts(:), amp(:), cw(:)
+      real(dp), allocatable :: qz(:, :), az(:)
+      type(tgx_soap_hypers) :: h
+      integer(c_int) :: mid
+      call require(associated(cur_hypers), "no soap_backend_begin(hypers) before this call")
+      if (allocated(cur_hypers%Qs)) then
+         key = c_loc(cur_hypers%Qs)
+      else
+         key = c_loc(cur_hypers)
+      end if
+      if (for_energy) then
+         do k = 1, n_reg
+            if (c_associated(reg_key(k), key)) then
+               mid_out = reg_id(k)
+               return
+            end if
+         end do
+      else
+         do k = 1, n_reg_desc
+            if (c_associated(reg_desc_key(k), key)) then
+               mid_out = reg_desc_id(k)
+               return
+            end if
+         end do
+      end if
+      call require(trim(scaling_mode) == "polynomial", 'scaling_mode "'//trim(scaling_mode)//'" is not implemented')
+      nsp = size(species_types)
+      allocate (am(nsp), st(nsp), r_h(nsp), r_s(nsp), nf_(nsp), gs(nsp), sr(nsp), srs(nsp), stt(nsp), sts(nsp), &
+                amp(nsp), cw(nsp))
+      do k = 1, nsp
+         st(k) = int(merge(species_code(species_types(k), k), 1000000 + k, for_energy), c_int)
+      end do
+      am = int(alpha_max, c_int)
+      r_h = rcut_hard; r_s = rcut_soft; nf_ = nf; gs = global_scaling; sr = atom_sigma_r
+      srs = atom_sigma_r_scaling; stt = atom_sigma_t; sts = atom_sigma_t_scaling; amp = amplitude_scaling
+      cw = central_weight
+      h%n_species = nsp; h%l_max = l_max; h%radial_enhancement = radial_enhancement
+      h%n_quad_soft = 32; h%n_quad_buffer = 16
+      select case (trim(basis))
+      case ("poly3")
+         h%basis = 0
+      case ("poly3gauss")
+         h%basis = 1
+      case default
+         call require(.false., 'basis "'//trim(basis)//'" is not implemented (poly3, poly3gauss)')
+      end select
+      h%alpha_max = c_loc(am); h%rcut_hard = c_loc(r_h); h%rcut_soft = c_loc(r_s); h%nf = c_loc(nf_)
+      h%global_scaling = c_loc(gs); h%atom_sigma_r = c_loc(sr); h%atom_sigma_r_scaling = c_loc(srs)
+      h%atom_sigma_t = c_loc(stt); h%atom_sigma_t_scaling = c_loc(sts); h%amplitude_scaling = c_loc(amp)
+      h%central_weight = c_loc(cw)
+      if (allocated(cur_hypers%Qs) .and. allocated(cur_hypers%alphas)) then
+         ns = size(cur_hypers%alphas)
+         call require(size(cur_hypers%Qs, 1) == n_soap, "the sparse set's descriptor size differs from n_soap")
+         call tgx_check(c_tgx_model(ctx, h, st, int(merge(central_species, 0, for_energy), c_int), delta, zeta, &
+                                    0.0_dp, int(ns, c_int), int(n_soap, c_int), cur_hypers%Qs, cur_hypers%alphas, &
+                                    mid), "soap_backend_begin")
+      else  ! a descriptor-only use needs no sparse set
+         allocate (qz(n_soap, 1), az(1))
+         qz = 0.0_dp; az = 0.0_dp
+         call tgx_check(c_tgx_model(ctx, h, st, 0_c_int, delta, zeta, 0.0_dp, 1_c_int, int(n_soap, c_int), qz, az, &
+                                    mid), "soap_backend_begin")
+      end if
+      mid_out = int(mid)
+      if (for_energy) then
+         call require(n_reg < size(reg_key), "too many soap_turbo blocks")
+         n_reg = n_reg + 1
+         reg_key(n_reg) = key
+         reg_id(n_reg) = mid_out
+      else
+         call require(n_reg_desc < size(reg_desc_key), "too many soap_turbo blocks")
+         n_reg_desc = n_reg_desc + 1
+         reg_desc_key(n_reg_desc) = key
+         reg_desc_id(n_reg_desc) = mid_out
+      end if
+   end function
+
+   ! get_gap_soap with the reference's arguments in its order
+   ! (src/gap_interface.f90:41-57) for the block of soap_backend_begin(hypers):
+   ! the centres among sites this_i_beg..this_i_end of the list that
+   ! build_neighbors_list made resident (n_neigh0 and neighbors_list0 must be
+   ! that list's slices), energies0 and forces0 ASSIGNED for the atoms the block
+   ! touches (src/gap_interface.f90:422-431), virial written. The descriptors
+   ! stay on the device: soap and soap_cart_der are returned unallocated, and
+   ! write_soap / write_derivatives, local properties, dipole models, compressed
+   ! descriptors, species multiplicity and the single-atom mode (all_atoms
+   ! false) stop with an error.
+   subroutine get_gap_soap_ref(n_total_sites, n_sites0, n_neigh0, neighbors_list0, n_species, species_types, &
+                               rjs0, thetas0, phis0, xyz0, alpha_max, l_max, n_soap, &
+                               rcut_hard, rcut_soft, nf, global_scaling, atom_sigma_r, &
+                               atom_sigma_r_scaling, atom_sigma_t, atom_sigma_t_scaling, &
+                               amplitude_scaling, radial_enhancement, central_weight, basis, &
+                               scaling_mode, do_timing, do_derivatives, do_forces, do_prediction, &
+                               write_soap, write_derivatives, compress_soap, &
+                               compress_P_nonzero, compress_P_i, compress_P_j, compress_P_el, &
+                               delta, zeta, central_species, &
+                               xyz_species, xyz_species_supercell, all_atoms, &
+                               which_atom, indices, soap, soap_cart_der, der_neighbors, der_neighbors_list, &
+                               has_local_properties, n_local_properties, local_property_models, &
+                               energies0, forces0, local_properties0, local_properties_cart_der0, &
+                               local_property_indexes, this_i_beg, this_i_end, this_j_beg, &
+                               this_j_end, virial, lp_index, is_dipole_model, local_dipoles0, energies_dipole0)
+      type(local_property_soap_turbo), allocatable, intent(in) :: local_property_models(:)
+      real(dp), intent(in) :: rjs0(:), thetas0(:), phis0(:), xyz0(:, :), rcut_hard(:), rcut_soft(:), nf(:), &
+                              global_scaling(:), atom_sigma_r(:), atom_sigma_r_scaling(:), atom_sigma_t(:), &
+                              atom_sigma_t_scaling(:), amplitude_scaling(:), central_weight(:), delta, zeta, &
+                              compress_P_el(:)
+      integer, intent(in) :: n_sites0, n_neigh0(:), neighbors_list0(:), n_species, central_species, &
+                             radial_enhancement, which_atom, indices(1:3), alpha_max(:), l_max, n_total_sites, &
+                             compress_P_nonzero, compress_P_i(:), compress_P_j(:), n_local_properties, &
+                             local_property_indexes(:), this_i_beg, this_i_end, this_j_beg, this_j_end, lp_index, &
+                             n_soap
+      logical, intent(in) :: is_dipole_model, do_timing, do_derivatives, compress_soap, do_forces, do_prediction, &
+                             all_atoms, write_soap, write_derivatives, has_local_properties
+      character*64, intent(in) :: basis
+      character*32, intent(in) :: scaling_mode
+      character*8, intent(in) :: xyz_species(:), xyz_species_supercell(:), species_types(:)
+      real(dp), allocatable, intent(out) :: soap(:, :), soap_cart_der(:, :, :)
+      real(dp), intent(out) :: virial(1:3, 1:3)
+      real(dp), intent(inout) :: energies0(:), forces0(:, :), local_properties0(:, :), &
+                                 local_properties_cart_der0(:, :, :), local_dipoles0(:, :), energies_dipole0(:)
+      integer, allocatable :: der_neighbors(:), der_neighbors_list(:)
+      integer :: mid, i, j
+      integer(c_int) :: nsb
+      integer(c_int64_t) :: npb
+      integer(8) :: t0, t1, rate
+      if (do_timing) call system_clock(t0, rate)
+      call require(.not. is_dipole_model, "dipole models are not implemented")
+      call require(.not. compress_soap, "compressed SOAP descriptors are not implemented")
+      call require(.not. (write_soap .or. write_derivatives), "write_soap / write_derivatives are not implemented")
+      call require(.not. has_local_properties, "local properties through get_gap_soap are not implemented")
+      call require(all_atoms, "all_atoms = .false. (which_atom) is not implemented")
+      do i = 1, n_species
+         do j = i + 1, n_species
+            call require(species_types(i) /= species_types(j), "species multiplicity above 1 is not implemented")
+         end do
+      end do
+      ! the caller's list must be the resident one
+      call require(allocated(res_n_neigh) .and. n_total_sites == res_n_sites, &
+                   "no resident neighbour list of this system (build_neighbors_list first)")
+      call require(this_i_beg >= 1 .and. this_i_end <= res_n_sites .and. &
+                   n_sites0 == this_i_end - this_i_beg + 1 .and. size(n_neigh0) == n_sites0, "bad site range")
+      call require(this_j_beg >= 1 .and. this_j_end <= size(res_nlist) .and. &
+                   size(neighbors_list0) == this_j_end - this_j_beg + 1, "bad pair range")
+      call require(all(n_neigh0 == res_n_neigh(this_i_beg:this_i_end)) .and. &
+                   all(neighbors_list0 == res_nlist(this_j_beg:this_j_end)), &
+                   "the neighbour list differs from the one build_neighbors_list made resident")
+      call require(size(energies0) >= res_n_sites .and. size(forces0, 2) >= res_n_sites, &
+                   "energies0 / forces0 smaller than the system")
+      call learn_species(xyz_species_supercell)
+      mid = block_for(.true., species_types, alpha_max, l_max, n_soap, rcut_hard, rcut_soft, nf, global_scaling, &
+                      atom_sigma_r, atom_sigma_r_scaling, atom_sigma_t, atom_sigma_t_scaling, amplitude_scaling, &
+                      radial_enhancement, central_weight, basis, scaling_mode, delta, zeta, central_species)
+      virial = 0.0_dp
+      if (do_prediction) then
+         call tgx_check(c_tgx_begin(ctx, int(mid, c_int)), "get_gap_soap")
+         call tgx_check(c_tgx_gap_soap_sites(ctx, gap_path, merge(1_c_int, 0_c_int, do_forces), &
+                                             int(this_i_beg, c_int), int(this_i_end, c_int), energies0, forces0, &
+                                             virial, nsb, npb), "get_gap_soap")
+      end if
+      if (do_timing) then
+         call system_clock(t1)
+         write (*, '(A, F9.3, A)') '  *) soap_turbo (device): ', real(t1 - t0, dp)/real(rate, dp), ' seconds |'
+      end if
+   end subroutine
+
+   ! soap_turbo's get_soap (call shape src/gap_interface.f90:320-325) on a list
+   ! the caller marshalled: n_sites centres with n_neigh entries each (self
+   ! first), the neighbours' species in mask(pair, species), rjs, thetas and
+   ! phis; soap(n_soap, n_sites) and, with do_derivatives,
+   ! soap_cart_der(3, n_soap, n_atom_pairs) are written. The hypers must be
+   ! those of soap_backend_begin(hypers)'s block.
+   subroutine get_soap(n_sites, n_neigh, n_species, species, species_multiplicity, n_atom_pairs, mask, rjs, &
+                       thetas, phis, alpha_max, l_max, rcut_hard, rcut_soft, nf, global_scaling, &
+                       atom_sigma_r, atom_sigma_r_scaling, atom_sigma_t, atom_sigma_t_scaling, &
+                       amplitude_scaling, radial_enhancement, central_weight, basis, scaling_mode, do_timing, &
+                       do_derivatives, compress_soap, compress_P_nonzero, compress_P_i, compress_P_j, &
+                       compress_P_el, soap, soap_cart_der)
+      integer, intent(in) :: n_sites, n_neigh(:), n_species, species(:, :), species_multiplicity(:), n_atom_pairs, &
+                             alpha_max(:), l_max, radial_enhancement, compress_P_nonzero, compress_P_i(:), &
+                             compress_P_j(:)
+      logical, intent(in) :: mask(:, :), do_timing, do_derivatives, compress_soap
+      real(dp), intent(in) :: rjs(:), thetas(:), phis(:), rcut_hard(:), rcut_soft(:), nf(:), global_scaling(:), &
+                              atom_sigma_r(:), atom_sigma_r_scaling(:), atom_sigma_t(:), atom_sigma_t_scaling(:), &
+                              amplitude_scaling(:), central_weight(:), compress_P_el(:)
+      character(len=*), intent(in) :: basis, scaling_mode
+      real(dp), intent(inout) :: soap(:, :), soap_cart_der(:, :, :)
+      integer :: mid, k, s
+      integer(c_int), allocatable :: nn(:), ps(:)
+      real(dp), allocatable :: xyz(:, :), der(:, :, :)
+      character*8, allocatable :: names(:)
+      call require(.not. compress_soap, "compressed SOAP descriptors are not implemented")
+      call require(all(species_multiplicity(1:n_sites) == 1), "species multiplicity above 1 is not implemented")
+      call require(size(soap, 2) >= n_sites, "soap smaller than n_sites")
+      allocate (names(n_species))
+      names = ""
+      mid = block_for(.false., names, alpha_max, l_max, size(soap, 1), rcut_hard, rcut_soft, nf, global_scaling, &
+                      atom_sigma_r, atom_sigma_r_scaling, atom_sigma_t, atom_sigma_t_scaling, amplitude_scaling, &
+                      radial_enhancement, central_weight, basis, scaling_mode, 1.0_dp, 2.0_dp, 0)
+      allocate (nn(n_sites), ps(max(n_atom_pairs, 1)), xyz(3, max(n_atom_pairs, 1)))
+      nn = int(n_neigh(1:n_sites), c_int)
+      do k = 1, n_atom_pairs
+         ps(k) = -1
+         do s = 1, n_species
+            if (mask(k, s)) then
+               ps(k) = int(s - 1, c_int)
+               exit
+            end if
+         end do
+         xyz(1, k) = rjs(k)*sin(thetas(k))*cos(phis(k))
+         xyz(2, k) = rjs(k)*sin(thetas(k))*sin(phis(k))
+         xyz(3, k) = rjs(k)*cos(thetas(k))
+      end do
+      call tgx_check(c_tgx_begin(ctx, int(mid, c_int)), "get_soap")
+      if (do_derivatives .and. n_atom_pairs > 0) then
+         call require(size(soap_cart_der, 3) >= n_atom_pairs, "soap_cart_der smaller than n_atom_pairs")
+         call tgx_check(c_tgx_soap_list(ctx, int(n_sites, c_int), nn, int(n_atom_pairs, c_int64_t), ps, rjs, xyz, &
+                                        1_c_int, soap, soap_cart_der), "get_soap")
+      else
+         allocate (der(3, 1, 1))
+         call tgx_check(c_tgx_soap_list(ctx, int(n_sites, c_int), nn, int(n_atom_pairs, c_int64_t), ps, rjs, xyz, &
+                                        0_c_int, soap, der), "get_soap")
+      end if
+      call tgx_check(c_tgx_end(ctx), "get_soap")
    end subroutine
 
    ! get_soap_energy_and_forces with the reference's arguments in its order

@@ -85,6 +85,10 @@ _SIGS = {
     "tgx_soap_block_size": [_P, _PI, C.POINTER(_I64)],
     "tgx_local_property_predict": [_P, _I, _PD, _PD, _PI, C.POINTER(_I64)],
     "tgx_get_soap": [_P, _PD, _PD],
+    "tgx_set_memory_budget": [_P, _D],
+    "tgx_get_soap_list": [_P, _I, _PI, _I64, _PI, _PD, _PD, _I, _PD, _PD],
+    "tgx_device_count": [_PI],
+    "tgx_get_gap_soap_sites": [_P, _I, _I, _I, _I, _PD, _PD, _PD, _PI, C.POINTER(_I64)],
     "tgx_get_soap_energy_and_forces": [_P, _I, _I, _I64, _PD, _PD, _D, _D, _D, _PI, _PI, _PD, _I, _I, _PD, _PD, _PD],
     "tgx_get_soap_energy_and_forces_noalloc": [_P, _I, _I, _I64, _P, _P, _D, _D, _D, _P, _P, _P, _I, _I, _P, _P,
                                                _P],
