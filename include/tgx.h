@@ -18,6 +18,7 @@
  *   tgx_refresh_neighbors_list          src/neighbors.f90:554-585   build_neighbors_list, rebuild_neighbors_list = .false.
  *   tgx_md_displacement                 src/turbogap_md.f90:451-471 the neighbors_buffer rebuild test
  *   tgx_soap_model_create               src/read_files.f90:660-724, 4641-5236  the soap_turbo block + sparse set
+ *   tgx_soap_model_create_compressed    src/read_files.f90:5162-5231            the same, compress_soap
  *   tgx_soap_backend_begin / _end       src/gap.f90:49-64           soap_backend_begin / soap_backend_end
  *   tgx_get_gap_soap                    src/gap_interface.f90:41-480 get_gap_soap (marshal, get_soap, prediction, fold)
  *   tgx_get_gap_soap_sites              src/gap_interface.f90:41-480 one this_i_beg:this_i_end batch of it
@@ -162,6 +163,17 @@ int tgx_get_neighbors_list(tgx_ctx* ctx, int* n_neigh, int* neighbors_list, doub
 int tgx_soap_model_create(tgx_ctx* ctx, const tgx_soap_hypers* hypers, const int* species_types,
                           int central_species, double delta, double zeta, double e0, int n_sparse, int dim,
                           const double* Qs, const double* alphas, int* model_id);
+/* The same for a compressed block (compress_soap; soap_turbo_hypers%compress_P_*
+ * as src/read_files.f90:5162-5211 fills them): dim is the compressed size, and
+ * each of the compress_P_nonzero entries adds compress_P_el times component
+ * compress_P_j of the raw power spectrum to component compress_P_i (1-based)
+ * before normalisation. Descriptors, derivatives, energies and forces of the
+ * block are those of the compressed descriptor; it runs on the FP64 paths. */
+int tgx_soap_model_create_compressed(tgx_ctx* ctx, const tgx_soap_hypers* hypers, const int* species_types,
+                                     int central_species, double delta, double zeta, double e0, int n_sparse,
+                                     int dim, const double* Qs, const double* alphas, int compress_P_nonzero,
+                                     const int* compress_P_i, const int* compress_P_j,
+                                     const double* compress_P_el, int* model_id);
 /* soap_backend_begin / soap_backend_end: select the block the next
  * tgx_get_gap_soap / tgx_get_soap / tgx_get_soap_energy_and_forces call uses. */
 int tgx_soap_backend_begin(tgx_ctx* ctx, int model_id);

@@ -68,6 +68,13 @@ def load():
     lib.orc_get_number_of_atom_pairs.argtypes = [C.c_int, _PI, C.c_long, _PD, C.c_double, C.c_int, C.c_int,
                                                  C.c_int, C.c_int, C.c_double, C.c_int, _PI, _PI, _PL, _PL]
     lib.orc_soap_batch_memory_model.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _PD, _PD]
+    lib.orc_get_soap_compressed.restype = C.c_int
+    lib.orc_get_soap_compressed.argtypes = [C.POINTER(OrcHypers), C.POINTER(OrcCompress), C.c_int, _PI, _PI, _PD,
+                                            _PD, _PD, _PD]
+    lib.orc_gap_soap_compressed.restype = C.c_long
+    lib.orc_gap_soap_compressed.argtypes = [C.POINTER(OrcHypers), C.POINTER(OrcCompress), C.c_int, C.c_int, _PI, _PI,
+                                            _PD, _PD, _PI, C.c_int, _PD, _PD, C.c_int, C.c_double, C.c_double,
+                                            C.c_int, _PD, _PD, _PD, _PD, _PI]
     lib.orc_num_threads.restype = C.c_int
     lib.orc_set_num_threads.argtypes = [C.c_int]
     _lib = lib
@@ -80,6 +87,20 @@ def _d(a):
 
 def _i(a):
     return a.ctypes.data_as(_PI) if a is not None else None
+
+
+class OrcCompress(C.Structure):
+    _fields_ = [("dim", C.c_int), ("nnz", C.c_int), ("P_i", _PI), ("P_j", _PI), ("P_el", _PD)]
+
+
+def compress_struct(h):
+    """OrcCompress of a compressed block (arrays kept alive), or (None, None)."""
+    if not getattr(h, "compressed", False):
+        return None, None
+    keep = (np.ascontiguousarray(h.compress_P_i, dtype=np.int32), np.ascontiguousarray(h.compress_P_j, dtype=np.int32),
+            np.ascontiguousarray(h.compress_P_el, dtype=np.float64))
+    c = OrcCompress(int(h.compress_dim), keep[0].shape[0], _i(keep[0]), _i(keep[1]), _d(keep[2]))
+    return c, keep
 
 
 def hypers_struct(h):
@@ -193,8 +214,9 @@ def get_soap(h, n_neigh, pair_species, rjs, xyz, derivatives=True):
     """soap_turbo get_soap restated (conventions in tgx_oracle.cpp). Returns
     soap (n, D) and soap_cart_der (P, D, 3)."""
     s, keep = hypers_struct(h)
+    comp, ckeep = compress_struct(h)
     lib = load()
-    D = lib.orc_soap_dim(C.byref(s))
+    D = comp.dim if comp is not None else lib.orc_soap_dim(C.byref(s))
     nn = np.ascontiguousarray(n_neigh, dtype=np.int32)
     ps = np.ascontiguousarray(pair_species, dtype=np.int32)
     r = np.ascontiguousarray(rjs, dtype=np.float64)
@@ -203,7 +225,11 @@ def get_soap(h, n_neigh, pair_species, rjs, xyz, derivatives=True):
     P = int(nn.sum())
     soap = np.empty((n, D))
     der = np.empty((P, D, 3)) if derivatives else None
-    lib.orc_get_soap(C.byref(s), n, _i(nn), _i(ps), _d(r), _d(x), _d(soap), _d(der))
+    if comp is None:
+        lib.orc_get_soap(C.byref(s), n, _i(nn), _i(ps), _d(r), _d(x), _d(soap), _d(der))
+    elif lib.orc_get_soap_compressed(C.byref(s), C.byref(comp), n, _i(nn), _i(ps), _d(r), _d(x), _d(soap),
+                                     _d(der)) < 0:
+        raise ValueError("compression entries outside the descriptor")
     return soap, der
 
 
@@ -234,6 +260,7 @@ def gap_soap(h, nl, species_sc, Qs, alphas, n_total_sites, do_forces=True):
     """src/gap_interface.f90:180-431 for one block (marshal, get_soap,
     prediction, fold). energies0/forces0 are zero where the block does not touch."""
     s, keep = hypers_struct(h)
+    comp, ckeep = compress_struct(h)
     lib = load()
     n0 = nl["n_neigh"].shape[0]
     spi = species_index(species_sc, h.species_types)
@@ -242,13 +269,18 @@ def gap_soap(h, nl, species_sc, Qs, alphas, n_total_sites, do_forces=True):
     e0 = np.zeros(n_total_sites)
     f0 = np.zeros((n_total_sites, 3))
     v = np.zeros(9)
-    D = lib.orc_soap_dim(C.byref(s))
+    D = comp.dim if comp is not None else lib.orc_soap_dim(C.byref(s))
     soap = np.empty((n0, D))
     nsb = C.c_int()
-    P = lib.orc_gap_soap(C.byref(s), int(n_total_sites), n0, _i(nl["n_neigh"]), _i(nl["neighbors_list"]),
-                         _d(nl["rjs"]), _d(np.ascontiguousarray(nl["xyz"])), _i(spi), int(h.central_species), _d(Qs),
-                         _d(al), Qs.shape[0], float(h.delta), float(h.zeta), int(do_forces), _d(e0), _d(f0), _d(v),
-                         _d(soap), C.byref(nsb))
+    args = (int(n_total_sites), n0, _i(nl["n_neigh"]), _i(nl["neighbors_list"]), _d(nl["rjs"]),
+            _d(np.ascontiguousarray(nl["xyz"])), _i(spi), int(h.central_species), _d(Qs), _d(al), Qs.shape[0],
+            float(h.delta), float(h.zeta), int(do_forces), _d(e0), _d(f0), _d(v), _d(soap), C.byref(nsb))
+    if comp is None:
+        P = lib.orc_gap_soap(C.byref(s), *args)
+    else:
+        P = lib.orc_gap_soap_compressed(C.byref(s), C.byref(comp), *args)
+        if P < 0:
+            raise ValueError("compression entries outside the descriptor")
     return e0, f0, v.reshape(3, 3).T.copy(), soap[:nsb.value].copy(), int(P)
 
 

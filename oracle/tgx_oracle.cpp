@@ -22,6 +22,8 @@
 //                                    with the conventions listed at soap_conventions below.
 //   orc_get_soap_energy_and_forces   src/gap.f90:66-259
 //   orc_gap_soap                     src/gap_interface.f90:180-431 (marshal + fold)
+//   orc_get_soap_compressed,         the same with compress_soap: the matrix read as
+//   orc_gap_soap_compressed          src/read_files.f90:5162-5211 reads it (C10)
 //
 // Parity status per function is recorded in the header of tests/test_oracle_*.py
 // and in the design notes: neighbour list, contraction and scatter are pinned by
@@ -388,6 +390,9 @@ int orc_get_number_of_atom_pairs(int n_sites, const int* n_neigh, long n_atom_pa
 //     (tools/compress_indices/compress_indices.py:39-57), D = N(N+1)/2 (l_max+1).
 //  C8 q = p / |p|. dq_i/dr_j per pair; the self pair is -sum of the others.
 //  C9 Neighbours with r >= rcut_hard(s) of their species contribute nothing.
+//  C10 Compression (compress_soap) acts on the raw spectrum p before C8:
+//     p_c(P_i) += P_el p(P_j) over the matrix entries, q = p_c / |p_c|, and the
+//     derivative is that of q through the same linear map.
 // ===========================================================================
 
 struct orc_soap_hypers {
@@ -653,14 +658,32 @@ static void pair_radial(const orc_soap_hypers& h, int s, const radial_table& T, 
 // descriptor's species list, or -1 for a neighbour the descriptor does not see.
 // xyz is (3, n_pairs) with xyz fastest. soap is (n_soap, n_sites) column-major
 // (component fastest); soap_cart_der is (3, n_soap, n_pairs), or null.
-int orc_get_soap(const orc_soap_hypers* hp, int n_sites, const int* n_neigh, const int* pair_species,
-                 const double* rjs, const double* xyz, double* soap, double* soap_cart_der) {
+// Compression of the power spectrum (compress_soap): dim components, and nnz
+// entries (1-based P_i, P_j, weight P_el) each adding P_el p(P_j) to component
+// P_i of the raw (unnormalised) spectrum (C10).
+struct orc_compress {
+  int dim;
+  int nnz;
+  const int* P_i;
+  const int* P_j;
+  const double* P_el;
+};
+
+// get_soap, compressed when comp is not null: soap is (comp->dim, n_sites) and
+// soap_cart_der (3, comp->dim, n_pairs).
+int orc_get_soap_compressed(const orc_soap_hypers* hp, const orc_compress* comp, int n_sites, const int* n_neigh,
+                            const int* pair_species, const double* rjs, const double* xyz, double* soap,
+                            double* soap_cart_der) {
   const orc_soap_hypers& h = *hp;
   const int L = h.l_max, NLM = (L + 1) * (L + 1);
   std::vector<int> off(h.n_species + 1, 0);
   for (int s = 0; s < h.n_species; ++s) off[s + 1] = off[s] + h.alpha_max[s];
   const int N = off[h.n_species];
   const int D = N * (N + 1) / 2 * (L + 1);
+  const int DO = comp ? comp->dim : D;  // output size
+  if (comp)
+    for (int k = 0; k < comp->nnz; ++k)
+      if (comp->P_i[k] < 1 || comp->P_i[k] > DO || comp->P_j[k] < 1 || comp->P_j[k] > D) return -1;
   std::vector<radial_table> T(h.n_species);
   for (int s = 0; s < h.n_species; ++s) build_radial_table(h, s, T[s]);
   std::vector<long> beg(n_sites + 1, 0);
@@ -733,21 +756,26 @@ int orc_get_soap(const orc_soap_hypers* hp, int n_sites, const int* n_neigh, con
         p[cnt++] = pw[pi] * s_;
       }
     }
+    if (comp) {
+      std::vector<double> pc((size_t)DO, 0.0);
+      for (int k = 0; k < comp->nnz; ++k) pc[(size_t)comp->P_i[k] - 1] += comp->P_el[k] * p[(size_t)comp->P_j[k] - 1];
+      p.swap(pc);
+    }
     double nrm2 = 0.0;
-    for (int d = 0; d < D; ++d) nrm2 += p[d] * p[d];
+    for (int d = 0; d < DO; ++d) nrm2 += p[d] * p[d];
     const double nrm = std::sqrt(nrm2);
-    double* qi = soap + (size_t)i * D;
-    for (int d = 0; d < D; ++d) qi[d] = nrm > 0.0 ? p[d] / nrm : 0.0;
+    double* qi = soap + (size_t)i * DO;
+    for (int d = 0; d < DO; ++d) qi[d] = nrm > 0.0 ? p[d] / nrm : 0.0;
     if (!soap_cart_der) continue;
     // derivatives
-    double* self = soap_cart_der + (size_t)b0 * 3 * D;
-    for (int e = 0; e < 3 * D; ++e) self[e] = 0.0;
-    std::vector<double> dc((size_t)N * NLM * 3), dp((size_t)3 * D);
+    double* self = soap_cart_der + (size_t)b0 * 3 * DO;
+    for (int e = 0; e < 3 * DO; ++e) self[e] = 0.0;
+    std::vector<double> dc((size_t)N * NLM * 3), dp((size_t)3 * D), dpc((size_t)3 * DO);
     for (int k = 1; k < nn; ++k) {
-      double* out = soap_cart_der + (size_t)(b0 + k) * 3 * D;
+      double* out = soap_cart_der + (size_t)(b0 + k) * 3 * DO;
       const int s = blk[k];
       if (s < 0 || nrm == 0.0) {
-        for (int e = 0; e < 3 * D; ++e) out[e] = 0.0;
+        for (int e = 0; e < 3 * DO; ++e) out[e] = 0.0;
         continue;
       }
       std::fill(dc.begin(), dc.end(), 0.0);
@@ -790,18 +818,30 @@ int orc_get_soap(const orc_soap_hypers* hp, int n_sites, const int* n_neigh, con
           ++cnt;
         }
       }
+      if (comp) {
+        std::fill(dpc.begin(), dpc.end(), 0.0);
+        for (int e = 0; e < comp->nnz; ++e)
+          for (int q = 0; q < 3; ++q)
+            dpc[3 * ((size_t)comp->P_i[e] - 1) + q] += comp->P_el[e] * dp[3 * ((size_t)comp->P_j[e] - 1) + q];
+      }
+      const double* dpo = comp ? dpc.data() : dp.data();
       double qd[3] = {0, 0, 0};
-      for (int d = 0; d < D; ++d)
-        for (int q = 0; q < 3; ++q) qd[q] += qi[d] * dp[3 * d + q];
-      for (int d = 0; d < D; ++d)
+      for (int d = 0; d < DO; ++d)
+        for (int q = 0; q < 3; ++q) qd[q] += qi[d] * dpo[3 * d + q];
+      for (int d = 0; d < DO; ++d)
         for (int q = 0; q < 3; ++q) {
-          double v = (dp[3 * d + q] - qi[d] * qd[q]) / nrm;
+          double v = (dpo[3 * d + q] - qi[d] * qd[q]) / nrm;
           out[3 * d + q] = v;
           self[3 * d + q] -= v;
         }
     }
   }
-  return D;
+  return DO;
+}
+
+int orc_get_soap(const orc_soap_hypers* hp, int n_sites, const int* n_neigh, const int* pair_species,
+                 const double* rjs, const double* xyz, double* soap, double* soap_cart_der) {
+  return orc_get_soap_compressed(hp, nullptr, n_sites, n_neigh, pair_species, rjs, xyz, soap, soap_cart_der);
 }
 
 // src/gap.f90:66-259. soap (n_soap, n_sites) and soap_der (3, n_soap, n_pairs)
@@ -886,14 +926,14 @@ void orc_get_soap_energy_and_forces(int n_sites, int n_soap, int n_sparse, const
 // energies0, forces0 (3, n_total_sites): assigned (not accumulated) for the
 // atoms this block touches, like the reference.
 // Returns the number of kept pairs.
-long orc_gap_soap(const orc_soap_hypers* h, int n_total_sites, int n_sites0, const int* n_neigh0,
-                  const int* neighbors_list0, const double* rjs0, const double* xyz0, const int* species_index_sc,
-                  int central_species, const double* Qs, const double* alphas, int n_sparse, double delta, double zeta,
-                  int do_forces, double* energies0, double* forces0, double* virial, double* soap_out,
-                  int* n_sites_out) {
+long orc_gap_soap_compressed(const orc_soap_hypers* h, const orc_compress* comp, int n_total_sites, int n_sites0,
+                             const int* n_neigh0, const int* neighbors_list0, const double* rjs0, const double* xyz0,
+                             const int* species_index_sc, int central_species, const double* Qs, const double* alphas,
+                             int n_sparse, double delta, double zeta, int do_forces, double* energies0,
+                             double* forces0, double* virial, double* soap_out, int* n_sites_out) {
   double rcut_max = 0.0;
   for (int s = 0; s < h->n_species; ++s) rcut_max = std::max(rcut_max, h->rcut_hard[s]);
-  const int D = soap_dim(*h);
+  const int D = comp ? comp->dim : soap_dim(*h);
   // species0(1,i) of the atom at the head of every site list
   std::vector<long> beg(n_sites0 + 1, 0);
   for (int i = 0; i < n_sites0; ++i) beg[i + 1] = beg[i] + n_neigh0[i];
@@ -958,8 +998,9 @@ long orc_gap_soap(const orc_soap_hypers* h, int n_total_sites, int n_sites0, con
   std::vector<double> soap((size_t)n_sites * D), soap_der;
   if (do_forces) soap_der.assign((size_t)n_pairs * 3 * D, 0.0);
   if (n_sites > 0)
-    orc_get_soap(h, n_sites, n_neigh.data(), spec.data(), rjs.data(), xyz.data(), soap.data(),
-                 do_forces ? soap_der.data() : nullptr);
+    if (orc_get_soap_compressed(h, comp, n_sites, n_neigh.data(), spec.data(), rjs.data(), xyz.data(), soap.data(),
+                                do_forces ? soap_der.data() : nullptr) < 0)
+      return -1;
   std::vector<double> energies(n_sites, 0.0), forces((size_t)3 * n_all, 0.0);
   double vir[9] = {0};
   if (n_sites > 0)
@@ -974,6 +1015,16 @@ long orc_gap_soap(const orc_soap_hypers* h, int n_total_sites, int n_sites0, con
   if (soap_out) std::memcpy(soap_out, soap.data(), sizeof(double) * soap.size());
   if (n_sites_out) *n_sites_out = n_sites;
   return n_pairs;
+}
+
+long orc_gap_soap(const orc_soap_hypers* h, int n_total_sites, int n_sites0, const int* n_neigh0,
+                  const int* neighbors_list0, const double* rjs0, const double* xyz0, const int* species_index_sc,
+                  int central_species, const double* Qs, const double* alphas, int n_sparse, double delta, double zeta,
+                  int do_forces, double* energies0, double* forces0, double* virial, double* soap_out,
+                  int* n_sites_out) {
+  return orc_gap_soap_compressed(h, nullptr, n_total_sites, n_sites0, n_neigh0, neighbors_list0, rjs0, xyz0,
+                                 species_index_sc, central_species, Qs, alphas, n_sparse, delta, zeta, do_forces,
+                                 energies0, forces0, virial, soap_out, n_sites_out);
 }
 
 int orc_num_threads(void) {

@@ -12,6 +12,7 @@ must match the Python driver making the same calls. A fourth
 (get_gap_soap with 66 arguments over site batches, get_soap)."""
 import os
 import pathlib
+import shutil
 import subprocess
 
 import numpy as np
@@ -181,17 +182,19 @@ def test_fortran_md_loop_matches_python_driver(tmp_path):
 REF_EXE = ROOT / "turbogap_amd" / "fortran" / "_build" / "tgx_ref_example"
 
 
-def _run_ref_example(d, sysm, models, names, n_batches):
+def _run_ref_example(d, sysm, models, names, n_batches, compress_file="none"):
     with open(d / "positions.dat", "w") as f:
         f.write(f"{sysm.n} {float(sysm.box[0, 0]):.17g}\n")
         for p, s in zip(sysm.positions, sysm.species):
             f.write(f"{p[0]:.17g} {p[1]:.17g} {p[2]:.17g} {int(s)} {names[int(s)]}\n")
+    h0 = models[0][0]
     with open(d / "blocks.dat", "w") as f:
-        f.write(f"{len(models)} {n_batches}\n")
+        f.write(f"{len(models)} {n_batches} {h0.rcut_hard[0]:.17g} {h0.rcut_soft[0]:.17g} {h0.zeta:.17g} "
+                f"{h0.radial_enhancement}\n")
         for k, (h, Qs, al) in enumerate(models):
             np.savetxt(d / f"alphas_{k}.dat", al, fmt="%.17g")
             np.savetxt(d / f"desc_{k}.dat", Qs.reshape(-1), fmt="%.17g")
-            f.write(f"alphas_{k}.dat desc_{k}.dat {h.central_species} {len(h.species_types)}\n")
+            f.write(f"alphas_{k}.dat desc_{k}.dat {h.central_species} {len(h.species_types)} {compress_file}\n")
             f.write(" ".join(names[int(c)] for c in h.species_types) + "\n")
     out = subprocess.run([str(REF_EXE)], cwd=d, capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout + out.stderr
@@ -205,30 +208,42 @@ def _run_ref_example(d, sysm, models, names, n_batches):
     return vals, F
 
 
-@pytest.mark.parametrize("system,n_batches", [("carbon216", 1), ("carbon216", 3), ("binary4096", 3)])
+@pytest.mark.parametrize("system,n_batches", [("carbon216", 1), ("carbon216", 3), ("binary4096", 3),
+                                              ("silicon_compressed", 2)])
 def test_fortran_reference_call_surface(tmp_path, orc, system, n_batches):
     """tgx_ref_example makes TurboGAP's own calls with the reference's argument
     lists (gpu_context_init(params, rank, n_omp), build_neighbors_list with 20
     arguments, soap_backend_begin(hypers), get_gap_soap with 66 arguments over
     this_i_beg:this_i_end batches, soap_backend_end, per-batch energies and
     forces zeroed and added as src/turbogap.f90:1281-1373 does): the totals
-    equal the oracle's (energy 1e-8 relative, forces and virial 1e-6). On the
-    one-block carbon box it also calls get_soap on the whole list, whose
-    descriptor and derivative sums equal the oracle's to 1e-10 relative."""
+    equal the oracle's (energy 1e-8 relative, forces and virial 1e-6 in the
+    potential's force scale), also for the compressed silicon block with the
+    reference suite's compress.dat. On one-block systems it also calls get_soap
+    on the whole list, whose descriptor and derivative sums equal the oracle's
+    to 1e-10 relative."""
     if not REF_EXE.exists():
         pytest.fail(f"{REF_EXE} not built (make -C turbogap_amd/fortran)")
+    cfile = "none"
     if system == "carbon216":
         sysm, h, Qs, al = H.carbon216(200)
         models = [(h, Qs, al)]
         names = {6: "C"}
+    elif system == "silicon_compressed":
+        sysm, h, Qs, al = H.silicon_compressed()
+        models = [(h, Qs, al)]
+        names = {14: "Si"}
+        # list-directed reads end at a slash: the file goes beside the inputs
+        shutil.copy(pathlib.Path(H.GOLD) / "gap_files" / "compress.dat", tmp_path / "compress.dat")
+        cfile = "compress.dat"
     else:
         sysm, models = H.subset(300, 4096)
         names = {1: "A", 2: "B"}
-    vals, F = _run_ref_example(tmp_path, sysm, models, names, n_batches)
+    vals, F = _run_ref_example(tmp_path, sysm, models, names, n_batches, cfile)
     Eo, eno, Fo, Wo, nl = orc.compute(sysm, models)
+    fs, ws = max(1.0, np.abs(Fo).max()), max(1.0, np.abs(Wo).max())
     assert abs(vals["energy"][0] - Eo) <= 1e-8 * abs(Eo)
-    assert np.abs(F - Fo).max() <= 1e-6
-    assert np.abs(vals["virial"].reshape(3, 3).T - Wo).max() <= 1e-6
+    assert np.abs(F - Fo).max() <= 1e-6 * fs
+    assert np.abs(vals["virial"].reshape(3, 3).T - Wo).max() <= 1e-6 * ws
     if len(models) == 1:
         h = models[0][0]
         ps = np.zeros(nl["n_atom_pairs"], np.int32)

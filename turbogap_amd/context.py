@@ -172,10 +172,17 @@ class Context:
         s, keep = hypers.c_struct()
         st = _i32(hypers.species_types)
         mid = C.c_int()
-        L.check(self._lib.tgx_soap_model_create(self.h, C.byref(s), L.iptr(st), int(hypers.central_species),
-                                                float(hypers.delta), float(hypers.zeta), float(e0), Qs.shape[0],
-                                                Qs.shape[1], L.dptr(Qs), L.dptr(al), C.byref(mid)),
-                "soap_model_create")
+        if hypers.compressed:
+            dim, P_i, P_j, P_el = hypers.compression_arrays()
+            L.check(self._lib.tgx_soap_model_create_compressed(
+                self.h, C.byref(s), L.iptr(st), int(hypers.central_species), float(hypers.delta), float(hypers.zeta),
+                float(e0), Qs.shape[0], Qs.shape[1], L.dptr(Qs), L.dptr(al), P_i.shape[0], L.iptr(P_i), L.iptr(P_j),
+                L.dptr(P_el), C.byref(mid)), "soap_model_create_compressed")
+        else:
+            L.check(self._lib.tgx_soap_model_create(self.h, C.byref(s), L.iptr(st), int(hypers.central_species),
+                                                    float(hypers.delta), float(hypers.zeta), float(e0), Qs.shape[0],
+                                                    Qs.shape[1], L.dptr(Qs), L.dptr(al), C.byref(mid)),
+                    "soap_model_create")
         self.models[mid.value] = hypers
         return mid.value
 

@@ -138,7 +138,9 @@ static void free_model(tgx_model_t* m) {
                   &m->q32, &m->qbf, &m->pnorm, &m->e_site, &m->qss64, &m->qss32, &m->kd64, &m->kdbf, &m->epart,
                   &m->soap_der, &m->fpair, &m->pbasis, &m->low_idx, &m->low_coef, &m->raise_idx, &m->raise_coef,
                   &m->gate_pn, &m->gate_e,
-                  &m->gate_stat, &m->gate_list, &m->sel_scratch, &m->mlp_ws, &m->scan_tmp, &m->pair_site};
+                  &m->gate_stat, &m->gate_list, &m->sel_scratch, &m->mlp_ws, &m->scan_tmp, &m->pair_site,
+                  &m->cp_row, &m->cp_col, &m->cp_el, &m->cpt_col, &m->cpt_row, &m->cpt_el, &m->qf64, &m->qssf64,
+                  &m->derf, &m->cnorm};
   for (dbuf* b : bufs) b->release();
   if (m->sel_host) (void)hipHostFree(m->sel_host);
   if (m->done) (void)hipEventDestroy(m->done);
@@ -280,9 +282,13 @@ static int copy_h(const double* src, int n, std::vector<double>& dst, const char
   return TGX_OK;
 }
 
-extern "C" int tgx_soap_model_create(tgx_ctx* ctx, const tgx_soap_hypers* h, const int* species_types,
-                                     int central_species, double delta, double zeta, double e0, int n_sparse,
-                                     int dim, const double* Qs, const double* alphas, int* model_id) {
+// A soap_turbo block. comp_nnz < 0: no compression, dim is the full power
+// spectrum size; otherwise dim is the compressed size and (P_i, P_j, P_el) the
+// comp_nnz entries (1-based) of the compression matrix.
+static int create_model(tgx_ctx* ctx, const tgx_soap_hypers* h, const int* species_types, int central_species,
+                        double delta, double zeta, double e0, int n_sparse, int dim, const double* Qs,
+                        const double* alphas, int comp_nnz, const int* P_i, const int* P_j, const double* P_el,
+                        int* model_id) {
   TGX_CTX_CHECK(ctx);
   TGX_REQUIRE(h && species_types && model_id, "tgx_soap_model_create: null argument");
   TGX_REQUIRE(h->n_species >= 1 && h->n_species <= TGX_MAX_SPECIES, "n_species=%d outside 1..%d", h->n_species,
@@ -338,10 +344,48 @@ extern "C" int tgx_soap_model_create(tgx_ctx* ctx, const tgx_soap_hypers* h, con
     return fail(TGX_ERR_ARG);
   }
   m->N = N;
-  m->D = N * (N + 1) / 2 * (m->l_max + 1);
+  const int d_full = N * (N + 1) / 2 * (m->l_max + 1);
+  m->D = comp_nnz < 0 ? d_full : dim;
   if (dim != m->D) {
     tgx_set_error("sparse set dimension %d does not match the descriptor size %d", dim, m->D);
     return fail(TGX_ERR_ARG);
+  }
+  std::vector<int> c_row, c_col, ct_col, ct_row;
+  std::vector<double> c_el, ct_el;
+  if (comp_nnz >= 0) {
+    // q_c(P_i) += P_el q(P_j) for every entry, as CSR over the compressed rows
+    // (forward map) and CSR over the full components (its transpose); entries
+    // keep their file order inside a row, so every sum has a fixed order
+    if (dim < 1 || comp_nnz < 1 || !P_i || !P_j || !P_el) {
+      tgx_set_error("compression: need dim >= 1 and at least one (P_i, P_j, P_el) entry");
+      return fail(TGX_ERR_ARG);
+    }
+    c_row.assign((size_t)dim + 1, 0);
+    ct_col.assign((size_t)d_full + 1, 0);
+    for (int k = 0; k < comp_nnz; ++k) {
+      if (P_i[k] < 1 || P_i[k] > dim || P_j[k] < 1 || P_j[k] > d_full) {
+        tgx_set_error("compression entry %d: (P_i, P_j) = (%d, %d) outside 1..%d x 1..%d", k + 1, P_i[k], P_j[k], dim,
+                      d_full);
+        return fail(TGX_ERR_ARG);
+      }
+      ++c_row[(size_t)P_i[k]];
+      ++ct_col[(size_t)P_j[k]];
+    }
+    for (int i = 0; i < dim; ++i) c_row[(size_t)i + 1] += c_row[(size_t)i];
+    for (int j = 0; j < d_full; ++j) ct_col[(size_t)j + 1] += ct_col[(size_t)j];
+    c_col.resize((size_t)comp_nnz);
+    c_el.resize((size_t)comp_nnz);
+    ct_row.resize((size_t)comp_nnz);
+    ct_el.resize((size_t)comp_nnz);
+    std::vector<int> fr(c_row.begin(), c_row.end() - 1), fc(ct_col.begin(), ct_col.end() - 1);
+    for (int k = 0; k < comp_nnz; ++k) {
+      const int i = P_i[k] - 1, j = P_j[k] - 1;
+      c_col[(size_t)fr[(size_t)i]] = j;
+      c_el[(size_t)fr[(size_t)i]++] = P_el[k];
+      ct_row[(size_t)fc[(size_t)j]] = i;
+      ct_el[(size_t)fc[(size_t)j]++] = P_el[k];
+    }
+    m->compress = true;
   }
   if (n_sparse < 1 || !Qs || !alphas) {
     tgx_set_error("empty sparse set");
@@ -361,8 +405,8 @@ extern "C" int tgx_soap_model_create(tgx_ctx* ctx, const tgx_soap_hypers* h, con
   P.L = m->l_max;
   P.NLM = (m->l_max + 1) * (m->l_max + 1);
   P.N = N;
-  P.D = m->D;
-  P.Dpad = m->Dpad;
+  P.D = d_full;
+  P.Dpad = (int)tgx_round_up(d_full, 128);
   int off = 0;
   for (int s = 0; s < S; ++s) {
     SpeciesParams& sp = P.sp[s];
@@ -430,6 +474,14 @@ extern "C" int tgx_soap_model_create(tgx_ctx* ctx, const tgx_soap_hypers* h, con
       (st = up(m->QsT_bf, qstb.data(), sizeof(__bf16) * qstb.size())) ||
       (st = up(m->alpha_f, alf.data(), sizeof(float) * alf.size())))
     return fail(st);
+  if (m->compress &&
+      ((st = up(m->cp_row, c_row.data(), sizeof(int) * c_row.size())) ||
+       (st = up(m->cp_col, c_col.data(), sizeof(int) * c_col.size())) ||
+       (st = up(m->cp_el, c_el.data(), sizeof(double) * c_el.size())) ||
+       (st = up(m->cpt_col, ct_col.data(), sizeof(int) * ct_col.size())) ||
+       (st = up(m->cpt_row, ct_row.data(), sizeof(int) * ct_row.size())) ||
+       (st = up(m->cpt_el, ct_el.data(), sizeof(double) * ct_el.size()))))
+    return fail(st);
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
     tgx_set_error("upload of the sparse set failed");
     return fail(TGX_ERR_HIP);
@@ -447,6 +499,24 @@ extern "C" int tgx_soap_model_create(tgx_ctx* ctx, const tgx_soap_hypers* h, con
   ctx->models.push_back(m);
   *model_id = (int)ctx->models.size() - 1;
   return TGX_OK;
+}
+
+extern "C" int tgx_soap_model_create(tgx_ctx* ctx, const tgx_soap_hypers* h, const int* species_types,
+                                     int central_species, double delta, double zeta, double e0, int n_sparse,
+                                     int dim, const double* Qs, const double* alphas, int* model_id) {
+  return create_model(ctx, h, species_types, central_species, delta, zeta, e0, n_sparse, dim, Qs, alphas, -1, nullptr,
+                      nullptr, nullptr, model_id);
+}
+
+extern "C" int tgx_soap_model_create_compressed(tgx_ctx* ctx, const tgx_soap_hypers* h, const int* species_types,
+                                                int central_species, double delta, double zeta, double e0,
+                                                int n_sparse, int dim, const double* Qs, const double* alphas,
+                                                int compress_P_nonzero, const int* compress_P_i,
+                                                const int* compress_P_j, const double* compress_P_el,
+                                                int* model_id) {
+  TGX_REQUIRE(compress_P_nonzero >= 1, "tgx_soap_model_create_compressed: compress_P_nonzero=%d", compress_P_nonzero);
+  return create_model(ctx, h, species_types, central_species, delta, zeta, e0, n_sparse, dim, Qs, alphas,
+                      compress_P_nonzero, compress_P_i, compress_P_j, compress_P_el, model_id);
 }
 
 static int get_model(tgx_ctx* ctx, int id, tgx_model_t** m) {
@@ -512,9 +582,11 @@ static int begin_step(tgx_ctx* ctx) {
 // ---- memory-bounded batches of a block (src/neighbors.f90:637-778) -------------
 // Device bytes the FP64 plain path holds for ns centres and np pairs: the
 // materialised dq/dr (3 x D FP64 per pair) and the pair data, and per centre
-// q, Qss and K^(zeta-1) alpha rows.
+// q, Qss and K^(zeta-1) alpha rows; a compressed block holds the full-size
+// dq/dr and q besides the compressed ones.
 static double plain_block_bytes(const tgx_model_t* m, int64_t ns, int64_t np) {
-  return (24.0 * m->D + 64.0) * (double)np + (8.0 * (2.0 * m->Dpad + m->Spad) + 64.0) * (double)ns;
+  const double full_np = m->compress ? 24.0 * m->P.D : 0.0, full_ns = m->compress ? 8.0 * m->P.Dpad + 8.0 : 0.0;
+  return (24.0 * m->D + full_np + 64.0) * (double)np + (8.0 * (2.0 * m->Dpad + m->Spad) + full_ns + 64.0) * (double)ns;
 }
 
 static int memory_budget_bytes(tgx_ctx* ctx, double* bytes) {

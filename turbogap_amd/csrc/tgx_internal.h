@@ -161,6 +161,16 @@ struct tgx_model_t {
   int64_t* sel_host = nullptr;  // pinned: the selection's site and pair totals
   int sel_lo = 0, sel_hi = -1;  // centres are taken from atoms [sel_lo, sel_hi) only (sel_hi < 0: all)
   dbuf mlp_ws;      // surrogate work space (activations, member energies)
+  // SOAP compression (compress_soap): q_c = P p / |P p| from the full power
+  // spectrum p. D and Dpad above are then the compressed sizes, P.D and P.Dpad
+  // the full ones. P as CSR over compressed rows (cp_*) and over full
+  // components (cpt_*), 0-based.
+  bool compress = false;
+  dbuf cp_row, cp_col, cp_el, cpt_col, cpt_row, cpt_el;
+  dbuf qf64;        // [n_sites_pad][P.Dpad] full normalised descriptors of a compressed block
+  dbuf qssf64;      // [n_sites_pad][P.Dpad] -dE/dq of the full descriptors (FP64 adjoint)
+  dbuf derf;        // [n_pairs_b][P.D][3] full dq/dr (FP64 plain)
+  dbuf cnorm;       // double[n_sites] |P q| of the normalised full q
   hipStream_t stream = nullptr;  // the block's own stream when blocks run concurrently
   hipEvent_t done = nullptr;     // recorded on that stream after the block's last launch
 };
@@ -289,6 +299,9 @@ int tgx_scratch_setup(int device);
 void* tgx_scratch_ptr(int device);
 size_t tgx_scratch_bytes();
 int tgx_adjoint_pair_forces(tgx_ctx* ctx, tgx_model_t* m, int path);
+// compressed blocks: q64 / soap_der from qf64 / derf, and qssf64 from qss64
+int tgx_compress_descriptors(tgx_ctx* ctx, tgx_model_t* m, bool with_der);
+int tgx_compress_gradient(tgx_ctx* ctx, tgx_model_t* m);
 int tgx_gather_forces(tgx_ctx* ctx, double* energy_out, double* virial_out);
 // F (device, n x 3) and the pair virial (host, 9, optional) from per-entry pair
 // forces fglob (device, n_pairs x 3) on the resident list.

@@ -1264,10 +1264,15 @@ int tgx_soap_descriptors(tgx_ctx* ctx, tgx_model_t* m, int path, bool with_der) 
   const size_t nsp = (size_t)tgx_round_up(ns, 128);
   TGX_TRY(m->pnorm.ensure(sizeof(double) * nsp));
   SiteArgs a = site_args(m);
+  TGX_REQUIRE(!m->compress || path == TGX_PATH_FP64_PLAIN || path == TGX_PATH_FP64_ADJOINT,
+              "compressed SOAP blocks run on the FP64 paths only");
   if (path == TGX_PATH_FP64_PLAIN || path == TGX_PATH_FP64_ADJOINT) {
-    TGX_TRY(m->q64.ensure(sizeof(double) * nsp * m->Dpad));
+    // a compressed block's kernels write the full descriptors, compressed below
+    dbuf& qout = m->compress ? m->qf64 : m->q64;
+    dbuf& dout = m->compress ? m->derf : m->soap_der;
+    TGX_TRY(qout.ensure(sizeof(double) * nsp * m->P.Dpad));
     if (with_der) {
-      TGX_TRY(m->soap_der.ensure(sizeof(double) * (size_t)(m->n_pairs_b > 0 ? m->n_pairs_b : 1) * 3 * m->D));
+      TGX_TRY(dout.ensure(sizeof(double) * (size_t)(m->n_pairs_b > 0 ? m->n_pairs_b : 1) * 3 * m->P.D));
       const SoapParams& P = m->P;
       const int NL = P.N * (P.L + 1);
       const int j_need = (3 * P.D + WG - 1) / WG;
@@ -1283,7 +1288,7 @@ int tgx_soap_descriptors(tgx_ctx* ctx, tgx_model_t* m, int path, bool with_der) 
         TGX_TRY(check_lds(lds));
 #define TGX_SOAP_DER(J)                                                                                     \
   TGX_LAUNCH(ctx, "soap_site_der_f64", (k_soap_der_l<8, J>), dim3(ns), dim3(WG), lds, m->P, a, ns, \
-             m->q64.as<double>(), m->pnorm.as<double>(), m->soap_der.as<double>())
+             qout.as<double>(), m->pnorm.as<double>(), dout.as<double>())
         if (j_need <= 4) TGX_SOAP_DER(4);
         else if (j_need <= 8) TGX_SOAP_DER(8);
         else if (j_need <= 12) TGX_SOAP_DER(12);
@@ -1298,8 +1303,8 @@ int tgx_soap_descriptors(tgx_ctx* ctx, tgx_model_t* m, int path, bool with_der) 
         const size_t lds = sizeof(double) * (head + (u1 > u2 ? u1 : u2));
         TGX_TRY(check_lds(lds));
 #define TGX_SOAP_DER(J)                                                                                           \
-  TGX_LAUNCH(ctx, "soap_site_der_f64", (k_soap_der<J>), dim3(ns), dim3(WG), lds, m->P, a, ns, m->q64.as<double>(), \
-             m->pnorm.as<double>(), m->soap_der.as<double>())
+  TGX_LAUNCH(ctx, "soap_site_der_f64", (k_soap_der<J>), dim3(ns), dim3(WG), lds, m->P, a, ns, qout.as<double>(), \
+             m->pnorm.as<double>(), dout.as<double>())
         if (j_need <= 4) TGX_SOAP_DER(4);
         else if (j_need <= 8) TGX_SOAP_DER(8);
         else if (j_need <= 12) TGX_SOAP_DER(12);
@@ -1311,15 +1316,16 @@ int tgx_soap_descriptors(tgx_ctx* ctx, tgx_model_t* m, int path, bool with_der) 
         const int kmax = (m->P.D + WG - 1) / WG;
 #define TGX_SOAP_DER(K)                                                                                 \
   TGX_LAUNCH(ctx, "soap_site_der_f64", (k_soap_site<double, 1, K>), dim3(ns), dim3(WG), lds, m->P, a, ns, \
-             m->q64.as<double>(), m->pnorm.as<double>(), m->soap_der.as<double>(), (const int*)nullptr)
+             qout.as<double>(), m->pnorm.as<double>(), dout.as<double>(), (const int*)nullptr)
         if (kmax <= 8) TGX_SOAP_DER(8);
         else if (kmax <= 12) TGX_SOAP_DER(12);
         else TGX_SOAP_DER(24);
 #undef TGX_SOAP_DER
       }
     } else {
-      TGX_TRY(tgx_desc_wave(ctx, m, true, nullptr, ns, m->q64.p, m->pnorm.p));
+      TGX_TRY(tgx_desc_wave(ctx, m, true, nullptr, ns, qout.p, m->pnorm.p));
     }
+    if (m->compress) TGX_TRY(tgx_compress_descriptors(ctx, m, with_der));
   } else {
     TGX_TRY(m->q32.ensure(sizeof(float) * nsp * m->Dpad));
     if (m->P.N <= 16) {
@@ -1335,6 +1341,8 @@ int tgx_soap_descriptors(tgx_ctx* ctx, tgx_model_t* m, int path, bool with_der) 
 
 int tgx_adjoint_pair_forces(tgx_ctx* ctx, tgx_model_t* m, int path) {
   const bool fp64 = path == TGX_PATH_FP64_ADJOINT || path == TGX_PATH_FP64_PLAIN;
+  TGX_REQUIRE(!m->compress || fp64, "compressed SOAP blocks run on the FP64 paths only");
+  if (m->compress) TGX_TRY(tgx_compress_gradient(ctx, m));
   if (!fp64 && m->P.N <= 16) return tgx_adjoint_f32(ctx, m);
   return tgx_adjoint_wave(ctx, m, fp64);
 }

@@ -567,7 +567,10 @@ int tgx_adjoint_wave(tgx_ctx* ctx, tgx_model_t* m, bool fp64) {
   if (m->n_sites_b == 0) return TGX_OK;
   const int N = m->P.N;
   if (fp64) {
-    const double *q = m->q64.as<double>(), *pn = m->pnorm.as<double>(), *g = m->qss64.as<double>();
+    // a compressed block contracts the full-size gradient with the full descriptors
+    const double* q = (m->compress ? m->qf64 : m->q64).as<double>();
+    const double* g = (m->compress ? m->qssf64 : m->qss64).as<double>();
+    const double* pn = m->pnorm.as<double>();
     if (N <= 8) return launch_adj<double, 8>(ctx, m, q, pn, g, "soap_adjoint_f64");
     if (N <= 16) return launch_adj<double, 16>(ctx, m, q, pn, g, "soap_adjoint_f64");
     return launch_adj<double, 32>(ctx, m, q, pn, g, "soap_adjoint_f64");

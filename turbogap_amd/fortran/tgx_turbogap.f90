@@ -141,6 +141,17 @@ module tgx_turbogap
          real(c_double) :: qs(*), al(*)
          integer(c_int) :: mid
       end function
+      integer(c_int) function c_tgx_model_c(c, h, st, cs, delta, zeta, e0, ns, dim, qs, al, nnz, pi, pj, pel, mid) &
+         bind(C, name="tgx_soap_model_create_compressed")
+         import :: c_int, c_ptr, c_double, tgx_soap_hypers
+         type(c_ptr), value :: c
+         type(tgx_soap_hypers) :: h
+         integer(c_int) :: st(*), pi(*), pj(*)
+         integer(c_int), value :: cs, ns, dim, nnz
+         real(c_double), value :: delta, zeta, e0
+         real(c_double) :: qs(*), al(*), pel(*)
+         integer(c_int) :: mid
+      end function
       integer(c_int) function c_tgx_begin(c, mid) bind(C, name="tgx_soap_backend_begin")
          import :: c_int, c_ptr
          type(c_ptr), value :: c
@@ -673,10 +684,13 @@ contains
    integer function block_for(for_energy, species_types, alpha_max, l_max, n_soap, rcut_hard, rcut_soft, nf, &
                               global_scaling, atom_sigma_r, atom_sigma_r_scaling, atom_sigma_t, &
                               atom_sigma_t_scaling, amplitude_scaling, radial_enhancement, central_weight, basis, &
-                              scaling_mode, delta, zeta, central_species) result(mid_out)
-      logical, intent(in) :: for_energy
+                              scaling_mode, delta, zeta, central_species, compress_soap, compress_P_nonzero, &
+                              compress_P_i, compress_P_j, compress_P_el) result(mid_out)
+      logical, intent(in) :: for_energy, compress_soap
       character*8, intent(in) :: species_types(:)
-      integer, intent(in) :: alpha_max(:), l_max, n_soap, radial_enhancement, central_species
+      integer, intent(in) :: alpha_max(:), l_max, n_soap, radial_enhancement, central_species, compress_P_nonzero, &
+                             compress_P_i(:), compress_P_j(:)
+      real(dp), intent(in) :: compress_P_el(:)
       real(dp), intent(in) :: rcut_hard(:), rcut_soft(:), nf(:), global_scaling(:), atom_sigma_r(:), &
                               atom_sigma_r_scaling(:), atom_sigma_t(:), atom_sigma_t_scaling(:), &
                               amplitude_scaling(:), central_weight(:), delta, zeta
@@ -687,8 +701,9 @@ contains
       integer(c_int), allocatable :: st(:)
       real(dp), allocatable, target :: r_h(:), r_s(:), nf_(:), gs(:), sr(:), srs(:), stt(:), sts(:), amp(:), cw(:)
       real(dp), allocatable :: qz(:, :), az(:)
+      integer(c_int), allocatable :: cpi(:), cpj(:)
       type(tgx_soap_hypers) :: h
-      integer(c_int) :: mid
+      integer(c_int) :: mid, cs
       call require(associated(cur_hypers), "no soap_backend_begin(hypers) before this call")
       if (allocated(cur_hypers%Qs)) then
          key = c_loc(cur_hypers%Qs)
@@ -735,16 +750,33 @@ contains
       h%global_scaling = c_loc(gs); h%atom_sigma_r = c_loc(sr); h%atom_sigma_r_scaling = c_loc(srs)
       h%atom_sigma_t = c_loc(stt); h%atom_sigma_t_scaling = c_loc(sts); h%amplitude_scaling = c_loc(amp)
       h%central_weight = c_loc(cw)
+      if (compress_soap) then
+         call require(compress_P_nonzero >= 1 .and. size(compress_P_i) >= compress_P_nonzero .and. &
+                      size(compress_P_j) >= compress_P_nonzero .and. size(compress_P_el) >= compress_P_nonzero, &
+                      "compress_soap without compress_P_i / compress_P_j / compress_P_el")
+         allocate (cpi(compress_P_nonzero), cpj(compress_P_nonzero))
+         cpi = int(compress_P_i(1:compress_P_nonzero), c_int)
+         cpj = int(compress_P_j(1:compress_P_nonzero), c_int)
+      end if
+      cs = int(merge(central_species, 0, for_energy), c_int)
       if (allocated(cur_hypers%Qs) .and. allocated(cur_hypers%alphas)) then
          ns = size(cur_hypers%alphas)
          call require(size(cur_hypers%Qs, 1) == n_soap, "the sparse set's descriptor size differs from n_soap")
-         call tgx_check(c_tgx_model(ctx, h, st, int(merge(central_species, 0, for_energy), c_int), delta, zeta, &
-                                    0.0_dp, int(ns, c_int), int(n_soap, c_int), cur_hypers%Qs, cur_hypers%alphas, &
-                                    mid), "soap_backend_begin")
+         allocate (qz(n_soap, ns), az(ns))
+         qz = cur_hypers%Qs(1:n_soap, 1:ns)
+         az = cur_hypers%alphas(1:ns)
       else  ! a descriptor-only use needs no sparse set
+         ns = 1
+         cs = 0_c_int
          allocate (qz(n_soap, 1), az(1))
          qz = 0.0_dp; az = 0.0_dp
-         call tgx_check(c_tgx_model(ctx, h, st, 0_c_int, delta, zeta, 0.0_dp, 1_c_int, int(n_soap, c_int), qz, az, &
+      end if
+      if (compress_soap) then
+         call tgx_check(c_tgx_model_c(ctx, h, st, cs, delta, zeta, 0.0_dp, int(ns, c_int), int(n_soap, c_int), qz, &
+                                      az, int(compress_P_nonzero, c_int), cpi, cpj, compress_P_el, mid), &
+                        "soap_backend_begin")
+      else
+         call tgx_check(c_tgx_model(ctx, h, st, cs, delta, zeta, 0.0_dp, int(ns, c_int), int(n_soap, c_int), qz, az, &
                                     mid), "soap_backend_begin")
       end if
       mid_out = int(mid)
@@ -768,9 +800,9 @@ contains
    ! that list's slices), energies0 and forces0 ASSIGNED for the atoms the block
    ! touches (src/gap_interface.f90:422-431), virial written. The descriptors
    ! stay on the device: soap and soap_cart_der are returned unallocated, and
-   ! write_soap / write_derivatives, local properties, dipole models, compressed
-   ! descriptors, species multiplicity and the single-atom mode (all_atoms
-   ! false) stop with an error.
+   ! write_soap / write_derivatives, local properties, dipole models, species
+   ! multiplicity and the single-atom mode (all_atoms false) stop with an
+   ! error. Compressed blocks (compress_soap, compress_P_*) are evaluated.
    subroutine get_gap_soap_ref(n_total_sites, n_sites0, n_neigh0, neighbors_list0, n_species, species_types, &
                                rjs0, thetas0, phis0, xyz0, alpha_max, l_max, n_soap, &
                                rcut_hard, rcut_soft, nf, global_scaling, atom_sigma_r, &
@@ -812,7 +844,6 @@ contains
       integer(8) :: t0, t1, rate
       if (do_timing) call system_clock(t0, rate)
       call require(.not. is_dipole_model, "dipole models are not implemented")
-      call require(.not. compress_soap, "compressed SOAP descriptors are not implemented")
       call require(.not. (write_soap .or. write_derivatives), "write_soap / write_derivatives are not implemented")
       call require(.not. has_local_properties, "local properties through get_gap_soap are not implemented")
       call require(all_atoms, "all_atoms = .false. (which_atom) is not implemented")
@@ -836,7 +867,8 @@ contains
       call learn_species(xyz_species_supercell)
       mid = block_for(.true., species_types, alpha_max, l_max, n_soap, rcut_hard, rcut_soft, nf, global_scaling, &
                       atom_sigma_r, atom_sigma_r_scaling, atom_sigma_t, atom_sigma_t_scaling, amplitude_scaling, &
-                      radial_enhancement, central_weight, basis, scaling_mode, delta, zeta, central_species)
+                      radial_enhancement, central_weight, basis, scaling_mode, delta, zeta, central_species, &
+                      compress_soap, compress_P_nonzero, compress_P_i, compress_P_j, compress_P_el)
       virial = 0.0_dp
       if (do_prediction) then
          call tgx_check(c_tgx_begin(ctx, int(mid, c_int)), "get_gap_soap")
@@ -875,14 +907,14 @@ contains
       integer(c_int), allocatable :: nn(:), ps(:)
       real(dp), allocatable :: xyz(:, :), der(:, :, :)
       character*8, allocatable :: names(:)
-      call require(.not. compress_soap, "compressed SOAP descriptors are not implemented")
       call require(all(species_multiplicity(1:n_sites) == 1), "species multiplicity above 1 is not implemented")
       call require(size(soap, 2) >= n_sites, "soap smaller than n_sites")
       allocate (names(n_species))
       names = ""
       mid = block_for(.false., names, alpha_max, l_max, size(soap, 1), rcut_hard, rcut_soft, nf, global_scaling, &
                       atom_sigma_r, atom_sigma_r_scaling, atom_sigma_t, atom_sigma_t_scaling, amplitude_scaling, &
-                      radial_enhancement, central_weight, basis, scaling_mode, 1.0_dp, 2.0_dp, 0)
+                      radial_enhancement, central_weight, basis, scaling_mode, 1.0_dp, 2.0_dp, 0, compress_soap, &
+                      compress_P_nonzero, compress_P_i, compress_P_j, compress_P_el)
       allocate (nn(n_sites), ps(max(n_atom_pairs, 1)), xyz(3, max(n_atom_pairs, 1)))
       nn = int(n_neigh(1:n_sites), c_int)
       do k = 1, n_atom_pairs

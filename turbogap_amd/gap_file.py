@@ -22,8 +22,16 @@ Species names map to the integer codes the caller uses for atoms through
 
 load_soap_blocks reads the sparse sets of the soap_turbo blocks
 (alphas_sparse, desc_sparse; sparse_io) into (hypers, Qs, alphas) triples for
-Context.soap_model; a compressed soap_turbo block (compress_soap = .true.) is
-rejected. load_pair_blocks reads distance_2b blocks (alphas and cutoff per
+Context.soap_model. A compressed block (compress_soap = .true.) takes its
+matrix from file_compress_soap (src/read_files.f90:5166-5211; the file takes
+precedence) or from compress_mode "trivial" (the recipe of the reference's
+tools/compress_indices/compress_indices.py); other modes are rejected.
+
+soap_turbo blocks describe atoms with this library's SOAP conventions (radial
+basis, quadrature, normalisation). A sparse set written by soap_turbo itself
+was made with soap_turbo's, which this library does not reproduce bit for bit,
+so load_soap_blocks warns (SparseSetWarning) that the energies of such a
+potential are not those soap_turbo would give. load_pair_blocks reads distance_2b blocks (alphas and cutoff per
 line, one distance per line; src/read_files.f90:722-742, 5238-5305) and
 core_pot blocks (a table headed by n, yp1, ypn; src/read_files.f90:5392-5427)
 and angle_3b blocks (alphas and cutoff per line, three descriptor components
@@ -38,12 +46,17 @@ from __future__ import annotations
 import dataclasses
 import os
 import shlex
+import warnings
 from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
-from .hypers import SoapTurbo
+from .hypers import SoapTurbo, compress_trivial, read_compress_file
 from .sparse_io import read_alphas_and_descriptors
+
+
+class SparseSetWarning(UserWarning):
+    """A sparse set made by another SOAP implementation is being evaluated."""
 
 _ELEMENTS = ("H He Li Be B C N O F Ne Na Mg Al Si P S Cl Ar K Ca Sc Ti V Cr Mn Fe Co Ni Cu Zn Ga Ge As Se Br Kr "
              "Rb Sr Y Zr Nb Mo Tc Ru Rh Pd Ag Cd In Sn Sb Te I Xe Cs Ba La Ce Pr Nd Pm Sm Eu Gd Tb Dy Ho Er Tm Yb "
@@ -186,16 +199,35 @@ def load_soap_blocks(path: str, species_map: Optional[Dict[str, int]] = None,
     for b in read_gap_file(path, species_map):
         if b.kind != "soap_turbo":
             continue
-        if b.compress:
-            raise NotImplementedError("compressed soap_turbo descriptors (compress_soap = .true.) are not "
-                                      "implemented")
+        hyp = compressed_hypers(b, base) if b.compress else b.hypers
         if not b.file_desc or not b.file_alphas:
             raise ValueError("soap_turbo block without desc_sparse / alphas_sparse")
         alphas, Qs = read_alphas_and_descriptors(os.path.join(base, b.file_desc), os.path.join(base, b.file_alphas))
-        if Qs.shape[1] != b.hypers.dim:
-            raise ValueError(f"{b.file_desc}: descriptor size {Qs.shape[1]} does not match the hypers ({b.hypers.dim})")
-        out.append((b.hypers, Qs, alphas))
+        if Qs.shape[1] != hyp.dim:
+            raise ValueError(f"{b.file_desc}: descriptor size {Qs.shape[1]} does not match the hypers ({hyp.dim})")
+        warnings.warn(f"{b.file_desc}: soap_turbo sparse sets are evaluated with this library's SOAP descriptor, "
+                      "which does not reproduce soap_turbo's bit for bit; energies differ from soap_turbo's unless the "
+                      "sparse set was made with this library", SparseSetWarning, stacklevel=2)
+        out.append((hyp, Qs, alphas))
     return out
+
+
+def compressed_hypers(b: GapBlock, base_dir: str) -> SoapTurbo:
+    """The hypers of a compress_soap block with its matrix: from
+    file_compress_soap when given (it takes precedence), else from
+    compress_mode (src/read_files.f90:5166-5226)."""
+    fname = b.values.get("file_compress_soap", ["none"])[0]
+    mode = b.values.get("compress_mode", ["none"])[0]
+    h = b.hypers
+    if fname != "none":
+        dim, P_i, P_j, P_el, _, _ = read_compress_file(os.path.join(base_dir, fname), h.n_species)
+    elif mode == "trivial":
+        dim, P_i, P_j, P_el = compress_trivial(h.alpha_max, h.l_max)
+    elif mode != "none":
+        raise NotImplementedError(f'compress_mode "{mode}" is not implemented (file_compress_soap, or "trivial")')
+    else:
+        raise ValueError("compress_soap = .true. needs file_compress_soap or compress_mode")
+    return h.with_compression(dim, P_i, P_j, P_el)
 
 
 def load_pair_blocks(path: str, species_map: Optional[Dict[str, int]] = None, base_dir: Optional[str] = None,

@@ -47,6 +47,12 @@ class SoapTurbo:
     delta: float = 1.0
     n_quad_soft: int = 32
     n_quad_buffer: int = 16
+    # compress_soap: the compressed size and the matrix entries (1-based), as
+    # src/read_files.f90:5162-5211 stores them; None for an uncompressed block
+    compress_dim: Optional[int] = None
+    compress_P_i: Optional[List[int]] = None
+    compress_P_j: Optional[List[int]] = None
+    compress_P_el: Optional[List[float]] = None
 
     def __post_init__(self):
         n = len(self.species_types)
@@ -78,9 +84,36 @@ class SoapTurbo:
         return int(sum(self.alpha_max))
 
     @property
-    def dim(self) -> int:
+    def full_dim(self) -> int:
+        """Size of the uncompressed power spectrum."""
         N = self.n_max
         return N * (N + 1) // 2 * (self.l_max + 1)
+
+    @property
+    def compressed(self) -> bool:
+        return self.compress_P_i is not None
+
+    @property
+    def dim(self) -> int:
+        """Descriptor size the sparse set has: the compressed size when compressed."""
+        return int(self.compress_dim) if self.compressed else self.full_dim
+
+    def with_compression(self, dim: int, P_i, P_j, P_el) -> "SoapTurbo":
+        """A copy carrying the compression matrix (entries 1-based)."""
+        P_i, P_j, P_el = (np.asarray(P_i, dtype=np.int64).reshape(-1), np.asarray(P_j, dtype=np.int64).reshape(-1),
+                          np.asarray(P_el, dtype=np.float64).reshape(-1))
+        if not (P_i.shape == P_j.shape == P_el.shape) or P_i.size == 0:
+            raise ValueError("compression needs matching, non-empty P_i, P_j, P_el")
+        if P_i.min() < 1 or P_i.max() > dim or P_j.min() < 1 or P_j.max() > self.full_dim:
+            raise ValueError(f"compression entries outside 1..{dim} x 1..{self.full_dim}")
+        return dataclasses.replace(self, compress_dim=int(dim), compress_P_i=P_i.tolist(), compress_P_j=P_j.tolist(),
+                                   compress_P_el=P_el.tolist())
+
+    def compression_arrays(self):
+        """(dim, P_i int32, P_j int32, P_el float64) of a compressed block."""
+        return (int(self.compress_dim), np.ascontiguousarray(self.compress_P_i, dtype=np.int32),
+                np.ascontiguousarray(self.compress_P_j, dtype=np.int32),
+                np.ascontiguousarray(self.compress_P_el, dtype=np.float64))
 
     @property
     def rcut_max(self) -> float:
@@ -119,3 +152,60 @@ class SoapTurbo:
             if k != "alpha_max":
                 setattr(s, k, L.dptr(keep[k]))
         return s, keep
+
+
+def compress_trivial(alpha_max: List[int], l_max: int):
+    """compress_mode "trivial": keep the components p(n, n', l) whose n or n' is
+    the first radial function of a species block, in descriptor order, as the
+    reference's tools/compress_indices/compress_indices.py:39-57 lists them.
+    Returns (dim, P_i, P_j, P_el) with 1-based entries and unit weights."""
+    N = int(sum(alpha_max))
+    pivots = set(int(x) for x in np.cumsum([1] + list(alpha_max[:-1])))
+    keep = []
+    k = 1
+    for n in range(1, N + 1):
+        for m in range(n, N + 1):
+            for _ in range(l_max + 1):
+                if n in pivots or m in pivots:
+                    keep.append(k)
+                k += 1
+    dim = len(keep)
+    return dim, list(range(1, dim + 1)), keep, [1.0] * dim
+
+
+def read_compress_file(path: str, n_species: int):
+    """A file_compress_soap file (src/read_files.f90:5166-5211): a header of
+    n_species alpha_max values, l_max and the compressed size, then either a
+    P_transformation ... end_transformation block of "i j el" lines or (the
+    older form) one kept component index per line. Returns (dim, P_i, P_j, P_el,
+    alpha_max, l_max) with 1-based entries."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    toks = []
+    k = 0
+    while len(toks) < n_species + 2:
+        if k >= len(lines):
+            raise ValueError(f"{path}: header needs {n_species} alpha_max values, l_max and the compressed size")
+        toks += lines[k].replace(",", " ").split()
+        k += 1
+    head = [int(t) for t in toks[:n_species + 2]]
+    alpha_max, l_max, dim = head[:n_species], head[n_species], head[n_species + 1]
+    body = lines[k:]
+    if body and body[0].strip() == "P_transformation":
+        P_i, P_j, P_el = [], [], []
+        for ln in body[1:]:
+            if ln.strip() == "end_transformation":
+                break
+            t = ln.replace(",", " ").split()
+            P_i.append(int(t[0]))
+            P_j.append(int(t[1]))
+            P_el.append(float(t[2].replace("d", "e").replace("D", "e")))
+        else:
+            raise ValueError(f"{path}: P_transformation without end_transformation")
+    else:
+        if len(body) < dim:
+            raise ValueError(f"{path}: {dim} component indices expected, {len(body)} found")
+        P_j = [int(ln.split()[0]) for ln in body[:dim]]
+        P_i = list(range(1, dim + 1))
+        P_el = [1.0] * dim
+    return dim, P_i, P_j, P_el, alpha_max, l_max

@@ -78,6 +78,8 @@ _SIGS = {
     "tgx_refresh_neighbors_list": [_P],
     "tgx_get_neighbors_list": [_P, _PI, _PI, _PD, _PD, _PD, _PD, _PI],
     "tgx_soap_model_create": [_P, C.POINTER(SoapHypersC), _PI, _I, _D, _D, _D, _I, _I, _PD, _PD, _PI],
+    "tgx_soap_model_create_compressed": [_P, C.POINTER(SoapHypersC), _PI, _I, _D, _D, _D, _I, _I, _PD, _PD, _I, _PI,
+                                         _PI, _PD, _PI],
     "tgx_soap_backend_begin": [_P, _I],
     "tgx_soap_backend_end": [_P],
     "tgx_soap_model_dim": [_P, _I, _PI],
