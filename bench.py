@@ -80,7 +80,8 @@ def roofline_entries(stats, n_atoms, blocks, n_pairs, members=1, hidden=256):
       plain_force                3 D 8 B/pair + D 8 B/site + 28 B/pair + 24 B/site
       soap_pair_basis            36 B/pair read + 4 ldb B/pair written
       soap_desc_f32              4 ldb B/pair + 4 Dpad B/site
-      soap_adjoint_f32           4 ldb + 36 + 48 B/pair + 8 Dpad B/site
+      soap_adjoint_f32           4 ldb + 36 + 48 B/pair + 4 Dpad B/site (rows read once, q not
+                                 read, FP64 force entry read and written, FP32 Qss)
       gemm_*                     2 n S D flops per GEMM
       mlp_l1 / mlp_back1         2 n (K H) Dpad flops
       mlp_l2 / mlp_back2         2 n K H^2 flops
@@ -113,7 +114,7 @@ def roofline_entries(stats, n_atoms, blocks, n_pairs, members=1, hidden=256):
     hbm("plain_force", np.mean([p * 3 * D * 8 + n * D * 8 + 28 * p + 24 * n for n, p in zip(Nb, Pb)]))
     hbm("soap_pair_basis", np.mean([p * (36 + 4 * ldb) for p in Pb]))
     hbm("soap_desc_f32", np.mean([p * 4 * ldb + n * 4 * Dpad for n, p in zip(Nb, Pb)]))
-    hbm("soap_adjoint_f32", np.mean([p * (4 * ldb + 36 + 48) + n * 8 * Dpad for n, p in zip(Nb, Pb)]))
+    hbm("soap_adjoint_f32", np.mean([p * (4 * ldb + 36 + 48) + n * 4 * Dpad for n, p in zip(Nb, Pb)]))
     for name, peak in (("gemm_kernel_f64", "fp64_TF"), ("gemm_qss_f64", "fp64_TF"), ("gemm_kernel_bf16", "bf16_TF"),
                        ("gemm_qss_bf16", "bf16_TF")):
         mfma(name, np.mean([2.0 * n * S * D for n in Nb]), peak)
@@ -125,7 +126,7 @@ def roofline_entries(stats, n_atoms, blocks, n_pairs, members=1, hidden=256):
 
 
 # bench kernel label -> device kernel name in the rocprofv3 counter summary
-KERNEL_SYMBOL = {"soap_adjoint_f32": "k_site_adjoint", "soap_desc_f32": "k_site_desc", "soap_pair_basis": "k_pair_basis",
+KERNEL_SYMBOL = {"soap_adjoint_f32": "k_site_adjoint_mfma", "soap_desc_f32": "k_site_desc", "soap_pair_basis": "k_pair_basis",
                  "nbr_fill": "k_cell_neigh<1>", "soap_site_der_f64": "k_soap_der_l<8, 16>",
                  "plain_force": "k_plain_force", "gemm_kernel_f64": "k_gemm_f64<1, true>",
                  "gemm_qss_f64": "k_gemm_f64<0, true>"}
@@ -248,6 +249,7 @@ def production_md(ctx, sysm, args):
     ctx.synchronize()
     dt = time.perf_counter() - t0
     drift = float(np.polyfit(t_nve, e_nve, 1)[0]) * 1e3 if len(e_nve) > 2 else None
+    gated_seg = gated_segment(ctx, md, n, args) if args.gated_steps > 0 else None
     ctx.set_positions(sysm.positions)
     out = {"steps": args.production_steps, "nvt_steps_1500K": args.nvt_steps, "dt_fs": 1.0,
            "atom_steps_per_s": n * args.production_steps / dt, "wall_s": dt,
@@ -256,25 +258,104 @@ def production_md(ctx, sysm, args):
            "list_builds": md.n_builds,
            "checked_force_rmse_meV_A": float(np.mean([c["force_rmse_meV_A"] for c in checks])) if checks else None,
            "checked_energy_mae_meV_atom": float(np.mean([c["energy_mae_meV_atom"] for c in checks])) if checks else None,
-           "checks": checks}
+           "checks": checks, "gated_segment": gated_seg}
     log(f"production md: {out['atom_steps_per_s']:.4g} atom-steps/s, drift {drift} meV/atom/ps, "
         f"fallback {out['fallback_fraction']}, checked force RMSE {out['checked_force_rmse_meV_A']} meV/A")
     return out
 
 
-def cpu_baseline(models, n_sample, threads):
-    """The FP64 oracle on a bounded sample of the workload: a box of n_sample
-    atoms at the density of the 65,536-atom box with the same blocks and sparse sets."""
+def gated_segment(ctx, md, n, args):
+    """NVE steps continuing the production run with the gate threshold lowered
+    so that a share of the sites (between 5% and 30%, found by bisection on the
+    current positions) falls back to the FP64 GAP on every step: the cost of the
+    fallback inside MD. The threshold goes back to args.gate afterwards."""
+    from turbogap_amd import surrogate as SG
+    SG.set_gate(ctx, args.members, 1e9)
+    ctx.compute("surrogate")
+    lo, hi = 0.0, SG.gate_stats(ctx)[1]
+    thr = None
+    for _ in range(40):
+        t = 0.5 * (lo + hi)
+        SG.set_gate(ctx, args.members, t)
+        ctx.compute("surrogate")
+        f = SG.gate_stats(ctx)[0]
+        if f > 0.30:
+            lo = t
+        elif f < 0.05:
+            hi = t
+        else:
+            thr = t
+            break
+    if thr is None:
+        SG.set_gate(ctx, args.members, args.gate)
+        return {"threshold_meV": None, "note": "no threshold gated 5-30% of the sites"}
+    gated = []
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.gated_steps):
+        md.step(1.0, T0=None, tau=100.0)
+        gated.append(SG.gate_stats(ctx)[0])
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    SG.set_gate(ctx, args.members, args.gate)
+    out = {"steps": args.gated_steps, "threshold_meV": 1e3 * thr, "fallback_fraction": float(np.mean(gated)),
+           "ms_per_step": 1e3 * dt / args.gated_steps, "atom_steps_per_s": n * args.gated_steps / dt}
+    log(f"gated segment: {out}")
+    return out
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(models, n_sample, threads, c0_sparse=2000):
+    """The FP64 oracle timed on the host cores, as planned for the baseline:
+    config 1 (the two-species box at the density of the 65,536-atom box, same
+    blocks and sparse sets) on a bounded sample of n_sample atoms with every
+    thread given, its per-atom time at n_sample / 4 (the single point is linear
+    in atoms at fixed density), a 1,024-atom sample on one core; and config 0
+    (216 carbon atoms, 2,000 sparse points) on every thread and on one core."""
     from oracle import oracle as O
     from turbogap_amd import structures as S
-    O.set_threads(threads)
-    sysm = S.binary_system(n_sample, seed=4242)
-    t0 = time.perf_counter()
-    O.compute(sysm, models)
-    dt = time.perf_counter() - t0
-    return {"value": n_sample / dt, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+
+    def timed(sysm, mods, nthreads, reps=1):
+        O.set_threads(nthreads)
+        best = None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            O.compute(sysm, mods)
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        return best
+
+    t_all = timed(S.binary_system(n_sample, seed=4242), models, threads)
+    n_q = max(1024, n_sample // 4)
+    t_q = timed(S.binary_system(n_q, seed=4243), models, threads)
+    n_1 = 1024
+    t_1 = timed(S.binary_system(n_1, seed=4244), models, 1)
+    h0 = S.soap_carbon()
+    Qs0, al0 = S.sparse_set(h0, c0_sparse, O.block_descriptors, S.carbon216_system, seed=5001)
+    c0 = S.carbon216_system()
+    t0_all = timed(c0, [(h0, Qs0, al0)], threads, reps=3)
+    t0_1 = timed(c0, [(h0, Qs0, al0)], 1, reps=3)
+    S_c1 = models[0][1].shape[0]
+    return {"value": n_sample / t_all, "unit": "atom-steps/s", "cores": threads, "kind": "port",
             "sample": f"one E+F+virial single point of a {n_sample}-atom box at the density of the 65,536-atom box "
-                      f"(same hypers and 2 x 4000 sparse points), FP64 oracle, OpenMP {threads} threads, {dt:.2f} s"}
+                      f"(same hypers and 2 x {S_c1} sparse points), FP64 oracle, OpenMP {threads} threads, "
+                      f"{t_all:.2f} s",
+            "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)), "cpu_model": _cpu_model(),
+            "config1_scaling": {"atoms": [n_q, n_sample], "s": [t_q, t_all],
+                                "us_per_atom": [1e6 * t_q / n_q, 1e6 * t_all / n_sample],
+                                "projected_65536_atom_s": t_all * 65536 / n_sample},
+            "config1_one_core": {"atoms": n_1, "s": t_1, "atom_steps_per_s": n_1 / t_1},
+            "config0": {"atoms": 216, "sparse": c0_sparse, "all_threads_s": t0_all, "one_core_s": t0_1,
+                        "atom_steps_per_s_all": 216 / t0_all, "atom_steps_per_s_one_core": 216 / t0_1}}
 
 
 def silicon_crystal(cells, perturb=0.05, seed=4):
@@ -404,7 +485,9 @@ def main():
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--md-steps", type=int, default=50)
     ap.add_argument("--gate", type=float, default=0.020)
-    ap.add_argument("--cpu-sample", type=int, default=32768)
+    ap.add_argument("--cpu-sample", type=int, default=16384)
+    ap.add_argument("--gated-steps", type=int, default=100,
+                    help="NVE steps after the production run with the gate firing on 5-30%% of the sites")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--production-steps", type=int, default=2000)
     ap.add_argument("--nvt-steps", type=int, default=200)
@@ -495,7 +578,8 @@ def main():
         kernels_roofline[p] = {n: roofline_object(n, e) for n, e in ent.items()}
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        # the host's share for one GPU is 16 threads, whatever nproc reports
+        threads = min(16, len(os.sched_getaffinity(0)))
         log(f"cpu baseline: oracle on {args.cpu_sample} atoms, {threads} threads")
         cpu = cpu_baseline(models, args.cpu_sample, threads)
     out = {
