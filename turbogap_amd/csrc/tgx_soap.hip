@@ -151,10 +151,12 @@ __device__ void pair_step1(const SoapParams& P, int s, T r, T ux, T uy, T uz, in
 // lane t = m + LL (t <= 2 LL) runs the l recurrence of its |m| and writes
 // Y_lm, and with GRAD the gradient of the solid harmonic r^l Y_lm at the unit
 // vector (xyz interleaved), for l = |m|..LL. Same recurrences and constants
-// as ylm_one; rinv[k] = 1/k replaces the division.
-template <int LL, bool GRAD>
+// as ylm_one; rinv[k] = 1/k replaces the division. With PACK (and GRAD) the
+// four values of lm go to Y[4 lm .. 4 lm + 3] = (Y, gx, gy, gz) instead.
+template <int LL, bool GRAD, bool PACK = false>
 __device__ __forceinline__ void harmonics_m(int t, const double* ynorm, const double* rinv, double ux, double uy,
                                             double uz, double* Y, double* gY) {
+  static_assert(!PACK || GRAD, "the packed layout carries gradients");
   if (t > 2 * LL) return;
   // opaque to the optimiser: the per-lane recurrence constants are rebuilt per call, not hoisted
   // out of the caller's pair loop where they would stay live across the whole site
@@ -198,11 +200,17 @@ __device__ __forceinline__ void harmonics_m(int t, const double* ynorm, const do
     }
     const int lm = l * l + l + m;
     const double Nn = ynorm[lm];
-    Y[lm] = Nn * q * f;
-    if (GRAD) {
-      gY[3 * lm] = Nn * (qx * f + q * fx);
-      gY[3 * lm + 1] = Nn * (qy * f + q * fy);
-      gY[3 * lm + 2] = Nn * qz * f;
+    if (PACK) {
+      double2* y4 = reinterpret_cast<double2*>(Y + 4 * lm);
+      y4[0] = make_double2(Nn * q * f, Nn * (qx * f + q * fx));
+      y4[1] = make_double2(Nn * (qy * f + q * fy), Nn * qz * f);
+    } else {
+      Y[lm] = Nn * q * f;
+      if (GRAD) {
+        gY[3 * lm] = Nn * (qx * f + q * fx);
+        gY[3 * lm + 1] = Nn * (qy * f + q * fy);
+        gY[3 * lm + 2] = Nn * qz * f;
+      }
     }
   }
 }
@@ -757,109 +765,17 @@ __device__ __forceinline__ double lane_read(double v, int l) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Register copies, per lane: pair 1 + lane of the site; B[s][n][q] at
-// n = lane / 8, q = lane % 8 + 8 i; the node x_q of each species at q = lane.
-struct DerRegs {
-  int spec;
-  double r, x, y, z;
-  double B[2][8];
-  double xq[2];
-  __device__ void load(const SoapParams& P, const SiteArgs& a, int64_t b0, int nn, int lane) {
-    const int k = 1 + lane;
-    spec = -1;
-    r = 1.0; x = y = z = 0.0;
-    if (k < nn) {
-      spec = a.pr_spec[b0 + k];
-      r = a.pr_r[b0 + k];
-      x = a.pr_xyz[3 * (b0 + k)];
-      y = a.pr_xyz[3 * (b0 + k) + 1];
-      z = a.pr_xyz[3 * (b0 + k) + 2];
-    }
-    const int n = lane >> 3, part = lane & 7;
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const bool have = s < P.n_species;
-#pragma unroll
-      for (int i = 0; i < 8; ++i) B[s][i] = have ? P.B[((size_t)s * TGX_MAX_NB + n) * TGX_MAX_Q + part + 8 * i] : 0.0;
-      xq[s] = have ? P.xq[s * TGX_MAX_Q + lane] : 0.0;
-    }
-  }
+// The pair-only factors of a pair's radial values and angular damping (what
+// der_radial and der_gauss would otherwise form in every lane for every pair):
+// the amplitude A(r) = global_scaling / s (1 + 2x^3 - 3x^2)^amplitude_scaling
+// and dA/dr, and e = exp(-k) and -dk/dr of the damping exponent
+// k = st^2 / (2 r^2). Zero for a pair outside the cutoff or unseen.
+struct PairAmp {
+  double amp, damp, ea, dk;
 };
 
-struct DerPair {
-  int s;
-  bool ok;
-  double r, ux, uy, uz;
-};
-
-// Pair k >= 1 of the site (wavefront-uniform): from the lanes for k <= 64, else from memory.
-__device__ __forceinline__ DerPair der_pair(const SoapParams& P, const SiteArgs& a, const DerRegs& R, int64_t b0,
-                                            int k) {
-  DerPair q;
-  double x, y, z;
-  if (k <= 64) {
-    q.s = __builtin_amdgcn_readlane(R.spec, k - 1);
-    q.r = lane_read(R.r, k - 1);
-    x = lane_read(R.x, k - 1);
-    y = lane_read(R.y, k - 1);
-    z = lane_read(R.z, k - 1);
-  } else {
-    const int64_t p = b0 + k;
-    q.s = a.pr_spec[p];
-    q.r = a.pr_r[p];
-    x = a.pr_xyz[3 * p];
-    y = a.pr_xyz[3 * p + 1];
-    z = a.pr_xyz[3 * p + 2];
-  }
-  // wavefront-uniform: the species parameters then come from scalar loads
-  q.s = __builtin_amdgcn_readfirstlane(q.s);
-  q.r = lane_read(q.r, 0);
-  x = lane_read(x, 0);
-  y = lane_read(y, 0);
-  z = lane_read(z, 0);
-  q.ok = q.s >= 0 && q.r < P.sp[q.s].rc;
-  const double inv = 1.0 / q.r;
-  q.ux = x * inv;
-  q.uy = y * inv;
-  q.uz = z * inv;
-  return q;
-}
-
-// pair_gauss with the nodes from registers (lane q < nq) and the damping on lane 63.
-__device__ __forceinline__ void der_gauss(const SoapParams& P, const DerRegs& R, int s, double r, int lane,
-                                          PairScratch<double>& ps, bool need_grad) {
-  const SpeciesParams& sp = P.sp[s];
-  const double inv_rc = sp.inv_rc;
-  const double x = r * inv_rc;
-  const double sig = sp.sig_r * (1.0 + sp.sig_r_sc * r) * inv_rc;
-  const double dsig = (sp.sig_r * sp.sig_r_sc) * inv_rc;
-  const double inv_s2 = 1.0 / (sig * sig);
-  if (lane < sp.nq) {
-    const double dx = (s == 0 ? R.xq[0] : R.xq[1]) - x;
-    const double g = exp(-0.5 * dx * dx * inv_s2);
-    ps.G[lane] = g;
-    if (need_grad) ps.dG[lane] = g * (dx * inv_s2 * inv_rc + dx * dx * inv_s2 / sig * dsig);
-  }
-  if (lane == 63) {
-    const double st = sp.sig_t * (1.0 + sp.sig_t_sc * r);
-    const double dst = sp.sig_t * sp.sig_t_sc;
-    const double kk = st * st / (2.0 * r * r);
-    const double dk = st * st / (r * r * r) - st * dst / (r * r);  // -dk/dr
-    const double e1 = exp(-kk), e2 = e1 * e1;
-    double Al = 1, pw = 1;
-    for (int l = 0; l <= P.L; ++l) {
-      if (l > 0) { pw *= e2; Al *= pw; }
-      ps.A[l] = Al;
-      ps.dA[l] = Al * (double)(l * (l + 1)) * dk;
-    }
-  }
-}
-
-// pair_step2 for nb <= 8 (eight node slices per radial function) with B from
-// registers; the node values are read before they are summed, in the same order.
-__device__ __forceinline__ void der_radial(const SoapParams& P, const DerRegs& R, int s, double r, int lane,
-                                           PairScratch<double>& ps) {
-  const SpeciesParams& sp = P.sp[s];
+__device__ __forceinline__ PairAmp pair_amp(const SpeciesParams& sp, double r) {
+  PairAmp o;
   const double inv_rc = sp.inv_rc;
   const double x = r * inv_rc;
   const double sig = sp.sig_r * (1.0 + sp.sig_r_sc * r) * inv_rc;
@@ -878,128 +794,256 @@ __device__ __forceinline__ void der_radial(const SoapParams& P, const DerRegs& R
     amp = Pa / sig;
     damp = aa * pow(Pp, aa - 1.0) * dPp / sig - Pa * dsig / (sig * sig);
   }
-  amp *= sp.gs;
-  damp *= sp.gs;
+  o.amp = amp * sp.gs;
+  o.damp = damp * sp.gs;
+  const double st = sp.sig_t * (1.0 + sp.sig_t_sc * r);
+  const double dst = sp.sig_t * sp.sig_t_sc;
+  const double kk = st * st / (2.0 * r * r);
+  o.dk = st * st / (r * r * r) - st * dst / (r * r);  // -dk/dr
+  o.ea = exp(-kk);
+  return o;
+}
+
+// Register copies, per lane: pair 1 + lane of the site with its PairAmp;
+// B[s][n][q] at n = lane / 8, q = lane % 8 + 8 i; the node x_q of each species
+// at q = lane. At most two species (the blocks this kernel serves).
+struct DerRegs {
+  int spec;
+  double r, x, y, z;
+  PairAmp pa;
+  double B[2][8];
+  double xq[2];
+  __device__ void load(const SoapParams& P, const SiteArgs& a, int64_t b0, int nn, int lane) {
+    const int k = 1 + lane;
+    spec = -1;
+    r = 1.0; x = y = z = 0.0;
+    if (k < nn) {
+      spec = a.pr_spec[b0 + k];
+      r = a.pr_r[b0 + k];
+      x = a.pr_xyz[3 * (b0 + k)];
+      y = a.pr_xyz[3 * (b0 + k) + 1];
+      z = a.pr_xyz[3 * (b0 + k) + 2];
+    }
+    pa = PairAmp{0.0, 0.0, 0.0, 0.0};
+    if (spec >= 0 && r < (spec == 0 ? P.sp[0].rc : P.sp[1].rc)) pa = pair_amp(spec == 0 ? P.sp[0] : P.sp[1], r);
+    const int n = lane >> 3, part = lane & 7;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const bool have = s < P.n_species;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) B[s][i] = have ? P.B[((size_t)s * TGX_MAX_NB + n) * TGX_MAX_Q + part + 8 * i] : 0.0;
+      xq[s] = have ? P.xq[s * TGX_MAX_Q + lane] : 0.0;
+    }
+  }
+};
+
+struct DerPair {
+  int s;
+  bool ok;
+  double r, ux, uy, uz;
+  PairAmp pa;
+};
+
+// Pair k >= 1 of the site (wavefront-uniform): from the lanes for k <= 64, else from memory.
+__device__ __forceinline__ DerPair der_pair(const SoapParams& P, const SiteArgs& a, const DerRegs& R, int64_t b0,
+                                            int k) {
+  DerPair q;
+  double x, y, z;
+  const bool reg = k <= 64;
+  if (reg) {
+    q.s = __builtin_amdgcn_readlane(R.spec, k - 1);
+    q.r = lane_read(R.r, k - 1);
+    x = lane_read(R.x, k - 1);
+    y = lane_read(R.y, k - 1);
+    z = lane_read(R.z, k - 1);
+    q.pa.amp = lane_read(R.pa.amp, k - 1);
+    q.pa.damp = lane_read(R.pa.damp, k - 1);
+    q.pa.ea = lane_read(R.pa.ea, k - 1);
+    q.pa.dk = lane_read(R.pa.dk, k - 1);
+  } else {
+    const int64_t p = b0 + k;
+    q.s = a.pr_spec[p];
+    q.r = a.pr_r[p];
+    x = a.pr_xyz[3 * p];
+    y = a.pr_xyz[3 * p + 1];
+    z = a.pr_xyz[3 * p + 2];
+  }
+  // wavefront-uniform: the species parameters then come from scalar loads
+  q.s = __builtin_amdgcn_readfirstlane(q.s);
+  q.r = lane_read(q.r, 0);
+  x = lane_read(x, 0);
+  y = lane_read(y, 0);
+  z = lane_read(z, 0);
+  q.ok = q.s >= 0 && q.r < P.sp[q.s].rc;
+  if (!reg && q.ok) q.pa = pair_amp(P.sp[q.s], q.r);
+  const double inv = 1.0 / q.r;
+  q.ux = x * inv;
+  q.uy = y * inv;
+  q.uz = z * inv;
+  return q;
+}
+
+// pair_gauss with the nodes from registers (lane q < nq) and the damping on
+// lane 63 from the pair's e = exp(-k).
+__device__ __forceinline__ void der_gauss(const SoapParams& P, const DerRegs& R, const DerPair& pr, int lane,
+                                          PairScratch<double>& ps, bool need_grad) {
+  const int s = pr.s;
+  const double r = pr.r;
+  const SpeciesParams& sp = P.sp[s];
+  const double inv_rc = sp.inv_rc;
+  const double x = r * inv_rc;
+  const double sig = sp.sig_r * (1.0 + sp.sig_r_sc * r) * inv_rc;
+  const double dsig = (sp.sig_r * sp.sig_r_sc) * inv_rc;
+  const double inv_s2 = 1.0 / (sig * sig);
+  if (lane < sp.nq) {
+    const double dx = (s == 0 ? R.xq[0] : R.xq[1]) - x;
+    const double g = exp(-0.5 * dx * dx * inv_s2);
+    ps.G[lane] = g;
+    if (need_grad) ps.dG[lane] = g * (dx * inv_s2 * inv_rc + dx * dx * inv_s2 / sig * dsig);
+  }
+  if (lane == 63) {
+    const double dk = pr.pa.dk;
+    const double e1 = pr.pa.ea, e2 = e1 * e1;
+    double Al = 1, pw = 1;
+    for (int l = 0; l <= P.L; ++l) {
+      if (l > 0) { pw *= e2; Al *= pw; }
+      ps.A[l] = Al;
+      ps.dA[l] = Al * (double)(l * (l + 1)) * dk;
+    }
+  }
+}
+
+// pair_step2 for nb <= 8 (eight node slices per radial function) with B from
+// registers and the pair's amplitude; the node values are read before they
+// are summed, in the same order.
+__device__ __forceinline__ void der_radial(const SoapParams& P, const DerRegs& R, const DerPair& pr, int lane,
+                                           PairScratch<double>& ps, bool need_grad) {
+  const SpeciesParams& sp = P.sp[pr.s];
+  const double amp = pr.pa.amp, damp = pr.pa.damp;
   const int n = lane >> 3, part = lane & 7, nq = sp.nq;
   double g[8], dg[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     const int q = min(part + 8 * i, nq - 1);
     g[i] = ps.G[q];
-    dg[i] = ps.dG[q];
+    dg[i] = need_grad ? ps.dG[q] : 0.0;
   }
   double I = 0, dI = 0;
   if (n < sp.nb) {
 #pragma unroll
     for (int i = 0; i < 8; ++i)
       if (part + 8 * i < nq) {
-        const double b = s == 0 ? R.B[0][i] : R.B[1][i];
+        const double b = pr.s == 0 ? R.B[0][i] : R.B[1][i];
         I += b * g[i];
         dI += b * dg[i];
       }
   }
   for (int o = 1; o < 8; o <<= 1) {
     I += __shfl_xor(I, o);
-    dI += __shfl_xor(dI, o);
+    if (need_grad) dI += __shfl_xor(dI, o);
   }
   if (n < sp.nb && part == 0) {
     ps.R[n] = amp * I;
-    ps.dR[n] = damp * I + amp * dI;
+    if (need_grad) ps.dR[n] = damp * I + amp * dI;
   }
 }
 
-// One wavefront reduces a pair to its channel tables (layout of
-// der_pair_tables, 16 rows; all zero for a pair the descriptor does not see).
-// Lanes (n, k) = (lane & 15, lane >> 4), k < 3 the Cartesian component, sum
-// over m for their row.
+// Channel tables of four pairs at once (slots 0..3 of U, pair data and
+// scratch filled by their wavefronts): lanes (n, p) = (lane & 15, lane >> 4)
+// take radial row n of pair p, and wavefront w the degrees of its set, so
+// every (n, l) entry of every pair is formed once and each c / H read serves
+// eight products. The pair's harmonics are packed (Y, gx, gy, gz) per lm.
+// Entry (n, l) of a slot is six doubles (s, E, v_x, v_y, v_z, Dd) at
+// 6 (n L1 + l), read back as three 16-byte loads:
+//   s(n,l) = sum_m Y c(n,lm),  v_k(n,l) = (sum_m dY_k c(n,lm) - l s u_k) / r,
+//   E(n,l) = R_n A_l,  Dd(n,l) = dR_n A_l + R_n dA_l  (rows of the pair's block),
+// and the pair's q . dp_k partial over the set, sum_(n,l) Dd h u_k +
+// E (h_k - l h u_k) / r with h = sum_m Y H(n,lm), h_k = sum_m dY_k H(n,lm),
+// reduced over the 16 row lanes into qpart[(w * 4 + p) * 3 + k].
 template <int LL>
-__device__ void der_tables_l(const SoapParams& P, const DerRegs& RG, const DerPair& pr, bool site_ok, const double* c,
-                             const double* H, const double* ynorm, const double* rinv, double* slot) {
+__device__ __forceinline__ void der_tables_w(const SoapParams& P, int w, int lane, int N, double* U, int SL,
+                                             const double* c, const double* H, double* qpart) {
+  static_assert(LL == 8, "degree sets are laid out for l_max = 8");
   using G = DerL<LL>;
   constexpr int L1 = G::L1, NLM = G::NLM, NL = G::NL;
-  const int lane = threadIdx.x & 63;
-  double* s_t = slot;
-  double* v_t = s_t + NL;
-  double* E_t = v_t + 3 * NL;
-  double* Dd_t = E_t + NL;
-  double* misc = Dd_t + NL;
-  if (!site_ok || !pr.ok) {
-    for (int i = lane; i < 6 * NL + 6; i += 64) slot[i] = 0.0;
-    return;
-  }
+  const int n = lane & 15, p = lane >> 4;
+  double* slot = U + p * SL;
+  const double* misc = slot + 6 * NL;
   PairScratch<double> ps;
-  ps.bind(misc + 8, LL, NLM);
-  const int s = pr.s;
-  const double r = pr.r, inv_r = 1.0 / r, ux = pr.ux, uy = pr.uy, uz = pr.uz;
-  der_gauss(P, RG, s, r, lane, ps, true);
-  harmonics_m<LL, true>(lane, ynorm, rinv, ux, uy, uz, ps.Y, ps.gY);
-  der_wave_sync();
-  der_radial(P, RG, s, r, lane, ps);
-  der_wave_sync();
-  const int nb = P.sp[s].nb, off = P.sp[s].off;
-  const int n = lane & 15, k = lane >> 4;
-  double qk = 0.0;
-  if (k < 3 && n < P.N) {
-    const double uk = k == 0 ? ux : (k == 1 ? uy : uz);
-    const double* gk = ps.gY + k;
-    const double* cr = c + n * NLM;
-    const int nl = n - off;
-    const bool inb = nl >= 0 && nl < nb;
-    const double Rn = inb ? ps.R[nl] : 0.0, dRn = inb ? ps.dR[nl] : 0.0;
-    // q . dp takes the block's rows of H: lane n reads row off + n (n < nb), others a
-    // valid row with weight 0, so one loop serves both sums and each harmonic is read once
-    const bool hb = n < nb;
-    const double* hr = H + (off + (hb ? n : 0)) * NLM;
-    const double hw = hb ? 1.0 : 0.0;
-    const double R0 = hb ? ps.R[n] : 0.0, dR0 = hb ? ps.dR[n] : 0.0;
-    // a fully unrolled (l, m) nest is one basic block whose hoisted loads exceed the register file
-#pragma unroll 1
-    for (int l = 0; l < L1; ++l) {
-      double s0 = 0.0, g0 = 0.0, h = 0.0, hk = 0.0;
-#pragma unroll 4
+  ps.bind(const_cast<double*>(misc) + 8, LL, NLM);
+  const bool ok = misc[0] != 0.0;
+  const int off = (int)misc[1], nb = (int)misc[2];
+  const double inv_r = 1.0 / misc[3], ux = misc[4], uy = misc[5], uz = misc[6];
+  const int nl = n - off;
+  const bool inb = ok && nl >= 0 && nl < nb;
+  const bool row = n < N;
+  const double* cr = c + n * NLM;
+  const double* hr = H + n * NLM;
+  double qx = 0.0, qy = 0.0, qz = 0.0;
+  const int nsets = w == 0 ? 3 : 2;
+  for (int li = 0; li < nsets; ++li) {
+    const int l = li == 0 ? LL - w : (li == 1 ? (w == 0 ? 0 : w + 1) : 1);
+    double s0 = 0.0, gx = 0.0, gy = 0.0, gz = 0.0, h0 = 0.0, hx = 0.0, hy = 0.0, hz = 0.0;
+    if (ok && row) {
+#pragma unroll 3
       for (int lm = l * l; lm < (l + 1) * (l + 1); ++lm) {
-        const double y = ps.Y[lm], gy = gk[3 * lm];
-        const double v = cr[lm], hv = hr[lm];
-        s0 += y * v;
-        g0 += gy * v;
-        h += y * hv;
-        hk += gy * hv;
+        const double2* y4 = reinterpret_cast<const double2*>(ps.Y + 4 * lm);  // (Y, gx), (gy, gz)
+        const double2 ya = y4[0], yb = y4[1];
+        const double y = ya.x, dx = ya.y, dy = yb.x, dz = yb.y;
+        const double cv = cr[lm], hv = hr[lm];
+        s0 += y * cv;
+        gx += dx * cv;
+        gy += dy * cv;
+        gz += dz * cv;
+        h0 += y * hv;
+        hx += dx * hv;
+        hy += dy * hv;
+        hz += dz * hv;
       }
-      const double lf = (double)l;
-      v_t[k * NL + n * L1 + l] = (g0 - lf * s0 * uk) * inv_r;
-      if (k == 0) {
-        s_t[n * L1 + l] = s0;
-        E_t[n * L1 + l] = inb ? Rn * ps.A[l] : 0.0;
-        Dd_t[n * L1 + l] = inb ? dRn * ps.A[l] + Rn * ps.dA[l] : 0.0;
-      }
-      const double E = R0 * ps.A[l], Dd = dR0 * ps.A[l] + R0 * ps.dA[l];
-      qk += hw * (Dd * h * uk + E * (hk - lf * h * uk) * inv_r);
     }
+    const double lf = (double)l;
+    const double E = inb ? ps.R[nl] * ps.A[l] : 0.0;
+    const double Dd = inb ? ps.dR[nl] * ps.A[l] + ps.R[nl] * ps.dA[l] : 0.0;
+    if (row) {
+      double2* t = reinterpret_cast<double2*>(slot + 6 * (n * L1 + l));
+      t[0] = make_double2(s0, E);
+      t[1] = make_double2((gx - lf * s0 * ux) * inv_r, (gy - lf * s0 * uy) * inv_r);
+      t[2] = make_double2((gz - lf * s0 * uz) * inv_r, Dd);
+    }
+    qx += Dd * h0 * ux + E * (hx - lf * h0 * ux) * inv_r;
+    qy += Dd * h0 * uy + E * (hy - lf * h0 * uy) * inv_r;
+    qz += Dd * h0 * uz + E * (hz - lf * h0 * uz) * inv_r;
   }
-  // sum over the 16 row lanes of each component
-  qk += __shfl_xor(qk, 1);
-  qk += __shfl_xor(qk, 2);
-  qk += __shfl_xor(qk, 4);
-  qk += __shfl_xor(qk, 8);
-  if ((lane & 15) == 0 && k < 3) misc[k] = qk;
-  if (lane == 0) {
-    misc[3] = ux; misc[4] = uy; misc[5] = uz;
+#pragma unroll
+  for (int o = 1; o < 16; o <<= 1) {
+    qx += __shfl_xor(qx, o);
+    qy += __shfl_xor(qy, o);
+    qz += __shfl_xor(qz, o);
+  }
+  if (n == 0) {
+    double* q = qpart + (w * 4 + p) * 3;
+    q[0] = qx;
+    q[1] = qy;
+    q[2] = qz;
   }
 }
 
 // Plain-path dq/dr for l_max = LL (the output and element layout of
 // k_soap_der). Coefficients: wavefront w expands pairs 1 + w, 5 + w, ... with
 // lanes over (l, m) and the 16 rows in registers, and the four partial
-// expansions are added in wavefront order. Pairs: groups of four, one
-// wavefront per pair for the tables, then the 256 threads write the slabs;
-// table offsets are compile-time, so the write loop addresses LDS with
-// immediates and reads all four pairs of an element before it stores them.
+// expansions are added in wavefront order. Pairs go in groups of four: each
+// wavefront prepares one pair's Gaussians, harmonics and radial values, the
+// four wavefronts then form the four pairs' channel tables together
+// (der_tables_w), and each thread writes whole components d = tid + WG j (its
+// three Cartesian entries, from 13 LDS reads) of every pair of the group,
+// keeping the self-pair sums of its components in registers. J >= D / WG.
 template <int LL, int J>
 __global__ __launch_bounds__(WG, 2) void k_soap_der_l(SoapParams P, SiteArgs a, int n_sites,
                                                       double* __restrict__ q_out, double* __restrict__ pnorm_out,
                                                       double* __restrict__ der_out) {
   using G = DerL<LL>;
   constexpr int L1 = G::L1, NLM = G::NLM, NL = G::NL, SL = G::SL;
+  static_assert(SL % 2 == 0 && (NLM + L1) % 2 == 0, "pair slots and tables need 16-byte alignment");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* sm = reinterpret_cast<double*>(smem_raw);
   const int site = blockIdx.x;
@@ -1011,7 +1055,7 @@ __global__ __launch_bounds__(WG, 2) void k_soap_der_l(SoapParams P, SiteArgs a, 
   double* c = rinv + L1;                       // N NLM
   double* H = c + CN;                          // N NLM
   double* X = H + CN;                          // D: p, then q
-  double* U = X + D;                           // max(4 SL, 4 N NLM, WG)
+  double* U = X + D + (D & 1);                 // max(4 SL + 48, 4 N NLM, WG), 16-byte aligned
   const int64_t b0 = a.site_beg[site];
   const int nn = (int)(a.site_beg[site + 1] - b0);
   DerRegs RG;
@@ -1034,10 +1078,10 @@ __global__ __launch_bounds__(WG, 2) void k_soap_der_l(SoapParams P, SiteArgs a, 
     for (int k = 1 + w; k < nn; k += 4) {
       const DerPair pr = der_pair(P, a, RG, b0, k);
       if (!pr.ok) continue;
-      der_gauss(P, RG, pr.s, pr.r, lane, ps, true);
+      der_gauss(P, RG, pr, lane, ps, false);
       harmonics_m<LL, false>(lane, ynorm, rinv, pr.ux, pr.uy, pr.uz, ps.Y, nullptr);
       der_wave_sync();
-      der_radial(P, RG, pr.s, pr.r, lane, ps);
+      der_radial(P, RG, pr, lane, ps, false);
       der_wave_sync();
       const int nb = P.sp[pr.s].nb, off = P.sp[pr.s].off;
 #pragma unroll
@@ -1079,63 +1123,103 @@ __global__ __launch_bounds__(WG, 2) void k_soap_der_l(SoapParams P, SiteArgs a, 
   for (int d = tid; d < P.Dpad; d += WG) qrow[d] = d < D ? X[d] : 0.0;
   if (tid == 0) pnorm_out[site] = nrm;
   contract_channel<double>(P, X, c, H);
+  // components d = tid + WG j of this thread: row offsets of (n, l) and (n', l)
+  // in the tables, n == n', d
   int code[J];
-  double self[J];
+  double self[J][3];
 #pragma unroll
-  for (int j = 0; j < J; ++j) code[j] = P.dtab[min((tid + WG * j) / 3, D - 1)];  // all loads in flight at once
+  for (int j = 0; j < J; ++j) code[j] = P.dtab[min(tid + WG * j, D - 1)];  // all loads in flight at once
 #pragma unroll
   for (int j = 0; j < J; ++j) {
-    const int e = tid + WG * j;
-    const int d = e / 3, k = e - 3 * d;
+    const int d = tid + WG * j;
     const int cd = code[j];
     const int n = cd & 0xff, np = (cd >> 8) & 0xff, l = cd >> 16;
-    code[j] = e < D3 ? (n * L1 + l) | ((np * L1 + l) << 8) | (k << 16) | ((n == np ? 1 : 0) << 18) | (d << 19) : -1;
-    self[j] = 0.0;
+    code[j] = d < D ? (n * L1 + l) | ((np * L1 + l) << 8) | ((n == np ? 1 : 0) << 16) | (d << 17) : -1;
+    self[j][0] = self[j][1] = self[j][2] = 0.0;
   }
   const double inv_n = nrm > 0.0 ? 1.0 / nrm : 0.0;
   const bool site_ok = nrm > 0.0;
+  double* qpart = U + 4 * SL;  // [wave][pair][k] partial q . dp
   lds_barrier();
   for (int g = 1; g < nn; g += 4) {
-    if (g + w < nn) der_tables_l<LL>(P, RG, der_pair(P, a, RG, b0, g + w), site_ok, c, H, ynorm, rinv, U + w * SL);
-    lds_barrier();
-    const int cnt = min(4, nn - g);
-    double* out = der_out + (size_t)(b0 + g) * D3 + tid;
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-      if (code[j] < 0) continue;
-      // g >> 30 is zero (g < 2^30) but changes with the loop as far as the optimiser can tell: the
-      // decoded fields are rebuilt per group instead of being hoisted and kept live across the tables
-      const int cj = code[j] + (g >> 30);
-      const int i1 = cj & 0xff, i2 = (cj >> 8) & 0xff, k = (cj >> 16) & 3;
-      const double wgt = ((cj >> 18) & 1) ? 1.0 : M_SQRT2;
-      const double qd = X[cj >> 19];
-      const double* t1 = U + i1;
-      const double* t2 = U + i2;
-      const double* v1 = U + NL + k * NL + i1;
-      const double* v2 = U + NL + k * NL + i2;
-      const double* mk = U + 6 * NL + k;
-      // slots past cnt hold stale tables: computed, never stored
-      double o[4];
-#pragma unroll
-      for (int ww = 0; ww < 4; ++ww) {
-        const int o0 = ww * SL;
-        const double rad = t1[o0 + 5 * NL] * t2[o0] + t2[o0 + 5 * NL] * t1[o0];
-        const double dp = wgt * (rad * mk[o0 + 3] + t1[o0 + 4 * NL] * v2[o0] + t2[o0 + 4 * NL] * v1[o0]);
-        o[ww] = (dp - qd * mk[o0]) * inv_n;
+    // (a) wavefront w: pair g + w's Gaussians, harmonics with gradients and radial values
+    {
+      double* slot = U + w * SL;
+      double* misc = slot + 6 * NL;
+      PairScratch<double> ps;
+      ps.bind(misc + 8, LL, NLM);
+      bool ok = false;
+      DerPair pr{};
+      if (g + w < nn) {
+        pr = der_pair(P, a, RG, b0, g + w);
+        ok = pr.ok && site_ok;
       }
+      if (ok) {
+        der_gauss(P, RG, pr, lane, ps, true);
+        harmonics_m<LL, true, true>(lane, ynorm, rinv, pr.ux, pr.uy, pr.uz, ps.Y, nullptr);
+        der_wave_sync();
+        der_radial(P, RG, pr, lane, ps, true);
+      }
+      if (lane == 0) {
+        misc[0] = ok ? 1.0 : 0.0;
+        misc[1] = ok ? (double)P.sp[pr.s].off : 0.0;
+        misc[2] = ok ? (double)P.sp[pr.s].nb : 0.0;
+        misc[3] = ok ? pr.r : 1.0;
+        misc[4] = ok ? pr.ux : 0.0;
+        misc[5] = ok ? pr.uy : 0.0;
+        misc[6] = ok ? pr.uz : 0.0;
+      }
+    }
+    lds_barrier();
+    // (b) channel tables of the four pairs: lanes (n, pair), wavefront w takes
+    // the degrees {8, 0, 1}, {7, 2}, {6, 3} or {5, 4} (20 or 21 (l, m) each)
+    der_tables_w<LL>(P, w, lane, N, U, SL, c, H, qpart);
+    lds_barrier();
+    // (c) the slabs of the (up to) four pairs: thread owns components d, all three k
+    const int cnt = min(4, nn - g);
+    for (int pp = 0; pp < cnt; ++pp) {
+      const double* slot = U + pp * SL;
+      const double* misc = slot + 6 * NL;
+      const double* qp = qpart + 3 * pp;
+      const double q0 = ((qp[0] + qp[12]) + qp[24]) + qp[36];
+      const double q1 = ((qp[1] + qp[13]) + qp[25]) + qp[37];
+      const double q2 = ((qp[2] + qp[14]) + qp[26]) + qp[38];
+      const double ux = misc[4], uy = misc[5], uz = misc[6];
+      double* out = der_out + (size_t)(b0 + g + pp) * D3;
 #pragma unroll
-      for (int ww = 0; ww < 4; ++ww)
-        if (ww < cnt) {
-          out[(size_t)ww * D3 + WG * j] = o[ww];
-          self[j] -= o[ww];
-        }
+      for (int j = 0; j < J; ++j) {
+        const int cj = code[j];
+        if (cj < 0) continue;
+        const int i1 = cj & 0xff, i2 = (cj >> 8) & 0xff, d = cj >> 17;
+        const double wgt = ((cj >> 16) & 1) ? 1.0 : M_SQRT2;
+        const double qd = X[d];
+        const double2* t1 = reinterpret_cast<const double2*>(slot + 6 * i1);
+        const double2* t2 = reinterpret_cast<const double2*>(slot + 6 * i2);
+        const double2 a1 = t1[0], b1 = t1[1], c1 = t1[2];  // (s, E), (v_x, v_y), (v_z, Dd)
+        const double2 a2 = t2[0], b2 = t2[1], c2 = t2[2];
+        const double rad = c1.y * a2.x + c2.y * a1.x;
+        const double ox = (wgt * (rad * ux + a1.y * b2.x + a2.y * b1.x) - qd * q0) * inv_n;
+        const double oy = (wgt * (rad * uy + a1.y * b2.y + a2.y * b1.y) - qd * q1) * inv_n;
+        const double oz = (wgt * (rad * uz + a1.y * c2.x + a2.y * c1.x) - qd * q2) * inv_n;
+        out[3 * d] = ox;
+        out[3 * d + 1] = oy;
+        out[3 * d + 2] = oz;
+        self[j][0] -= ox;
+        self[j][1] -= oy;
+        self[j][2] -= oz;
+      }
     }
     lds_barrier();
   }
   double* out = der_out + (size_t)b0 * D3;
 #pragma unroll
   for (int j = 0; j < J; ++j)
-    if (code[j] >= 0) out[tid + WG * j] = self[j];
+    if (code[j] >= 0) {
+      const int d = code[j] >> 17;
+      out[3 * d] = self[j][0];
+      out[3 * d + 1] = self[j][1];
+      out[3 * d + 2] = self[j][2];
+    }
 }
 
 // Plain-path pair force: f_p = Qss_i . soap_cart_der(:, :, p), one wavefront per
@@ -1278,20 +1362,22 @@ int tgx_soap_descriptors(tgx_ctx* ctx, tgx_model_t* m, int path, bool with_der) 
       const int j_need = (3 * P.D + WG - 1) / WG;
       bool small_blocks = P.n_species <= 2;
       for (int sp = 0; sp < P.n_species; ++sp) small_blocks = small_blocks && P.sp[sp].nb <= 8 && P.sp[sp].nq <= 64;
-      if (P.L == 8 && P.N <= 16 && small_blocks && j_need <= 16 && P.D < 4096) {
+      const int jd = (P.D + WG - 1) / WG;  // components per thread of k_soap_der_l
+      if (P.L == 8 && P.N <= 16 && small_blocks && P.D < 4096) {
         using G = DerL<8>;
         const size_t cn = (size_t)P.N * P.NLM;
-        size_t u = 4 * (size_t)G::SL;
+        size_t u = 4 * (size_t)G::SL + 48;  // four pair slots and the q . dp partials
         if (4 * cn > u) u = 4 * cn;
         if ((size_t)WG > u) u = WG;
-        const size_t lds = sizeof(double) * ((size_t)G::NLM + G::L1 + 2 * cn + P.D + u);
+        const size_t lds = sizeof(double) * ((size_t)G::NLM + G::L1 + 2 * cn + P.D + (P.D & 1) + u);
         TGX_TRY(check_lds(lds));
 #define TGX_SOAP_DER(J)                                                                                     \
   TGX_LAUNCH(ctx, "soap_site_der_f64", (k_soap_der_l<8, J>), dim3(ns), dim3(WG), lds, m->P, a, ns, \
              qout.as<double>(), m->pnorm.as<double>(), dout.as<double>())
-        if (j_need <= 4) TGX_SOAP_DER(4);
-        else if (j_need <= 8) TGX_SOAP_DER(8);
-        else if (j_need <= 12) TGX_SOAP_DER(12);
+        if (jd <= 2) TGX_SOAP_DER(2);
+        else if (jd <= 4) TGX_SOAP_DER(4);
+        else if (jd <= 5) TGX_SOAP_DER(5);
+        else if (jd <= 8) TGX_SOAP_DER(8);
         else TGX_SOAP_DER(16);
 #undef TGX_SOAP_DER
       } else if (j_need <= 16 && NL <= 255 && P.D < 4096) {
