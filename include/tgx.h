@@ -283,6 +283,23 @@ int tgx_mlp_forward_host(tgx_mlp* mlp, int batch, const float* q, float* e, floa
 int tgx_mlp_gradients_host(tgx_mlp* mlp, int batch, const float* q, const float* e_ref, const float* g_ref,
                            float w_energy, float w_grad, float* dW1, float* db1, float* dW2, float* db2,
                            float* dw3, float* db3, double* loss_e, double* loss_g);
+/* Energy plus force loss on whole structures (the configuration's "energy
+ * plus force loss"): the centres of blocks model_ids[0..n_blocks) of the
+ * context's current system, each block one batch of its network nets[b],
+ * after a whole-block FP64 plain evaluation with forces (tgx_compute with
+ * TGX_PATH_FP64_PLAIN). Loss = w_energy sum_blocks mean_i ((E_i -
+ * e_shift)/e_scale - E_hat_i)^2 + w_force / n_atoms sum_j |F_hat_j - F_j|^2,
+ * with the force error
+ * of every block together (eV/A) formed from the networks' dE/dq and the
+ * block's dq/dr. Leaves each network's weight gradients for
+ * tgx_mlp_get_gradients / tgx_mlp_adam_step; overwrites the context's pair
+ * forces (the computed results stay). */
+int tgx_mlp_force_gradients(tgx_ctx* ctx, int n_blocks, tgx_mlp* const* nets, const int* model_ids, double w_energy,
+                            double w_force, double* loss_energy, double* loss_force);
+/* The weight gradients of the network's last backward pass (layout of
+ * tgx_mlp_get_weights), and one Adam step (beta 0.9/0.999) with them. */
+int tgx_mlp_get_gradients(tgx_mlp* mlp, float* dW1, float* db1, float* dW2, float* db2, float* dw3, float* db3);
+int tgx_mlp_adam_step(tgx_mlp* mlp, float lr);
 /* FP32 master weights, row-major: W1 (hidden, dim), b1 (hidden), W2 (hidden,
  * hidden), b2 (hidden), w3 (hidden), b3 (1). */
 int tgx_mlp_get_weights(tgx_mlp* mlp, float* W1, float* b1, float* W2, float* b2, float* w3, float* b3);

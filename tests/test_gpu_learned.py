@@ -364,3 +364,73 @@ def test_surrogate_production_md_with_gate(ctx):
     drift = np.polyfit(t_ps, e, 1)[0] * 1e3
     print(f"NVE: drift {drift:.4f} meV/atom/ps, range {1e3 * np.ptp(e):.4f} meV/atom", flush=True)
     assert abs(drift) < 20.0 and 1e3 * np.ptp(e) < 2.0
+
+
+def _block_entries(nl, species, n, h):
+    """The list entries of one block in the library's order: centres are the
+    atoms of the block's central species in atom order, each with its list
+    entries inside the block's cutoff (self first); returns site_beg and the
+    0-based atom of every entry."""
+    c = h.species_types[h.central_species - 1] if h.central_species > 0 else None
+    beg, nbr = [0], []
+    off = np.concatenate([[0], np.cumsum(nl["n_neigh"])])
+    for i in range(n):
+        if c is not None and species[i] != c:
+            continue
+        for p in range(off[i], off[i + 1]):
+            if nl["rjs"][p] < h.rcut_max and int(nl["neighbor_species"][p]) in h.species_types:
+                nbr.append((nl["neighbors_list"][p] - 1) % n)
+        beg.append(len(nbr))
+    return np.array(beg), np.array(nbr)
+
+
+def test_force_loss_gradients_match_oracle(ctx, orc):
+    """Energy + force loss on a whole two-species structure (512 atoms, both
+    blocks, D = 324): the library's losses and weight gradients (its FP64 dq/dr,
+    pair-force kernel, fixed-order gather and the force-direction kernel, then
+    the networks' forward-then-reverse pass) against oracle/mlp_oracle.py's
+    force_loss_and_grads fed the same descriptors and dq/dr (BF16 operands
+    emulated; the tolerances of the environment-loss test)."""
+    rng = np.random.default_rng(11)
+    sysm = S.binary_system(512, seed=2024)
+    models = []
+    for k, h in enumerate(S.soap_hypers([S.SPECIES_A, S.SPECIES_B], central_species=c, n_max=4) for c in (1, 2)):
+        Qs, al = S.sparse_set(h, 96, orc.block_descriptors, lambda s: S.binary_system(256, seed=s), seed=501 + k)
+        models.append((h, Qs, al))
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    nl = ctx.build_neighbors_list(3.7)
+    ids = H.load_models(ctx, models)
+    blocks = []
+    for mid, (h, Qs, al) in zip(ids, models):
+        ctx.soap_backend_begin(mid)
+        q, J = ctx.get_soap(derivatives=True)
+        ctx.soap_backend_end()
+        K = q @ Qs.T
+        e_ref = h.delta ** 2 * (K ** h.zeta) @ al
+        g_ref = h.delta ** 2 * h.zeta * ((K ** (h.zeta - 1)) * al) @ Qs
+        beg, nbr = _block_entries(nl, sysm.species, sysm.n, h)
+        assert beg[-1] == J.shape[0] and len(beg) - 1 == q.shape[0]
+        blocks.append({"q": q.astype(np.float32).astype(np.float64), "e_ref": e_ref, "g_ref": g_ref, "J": J,
+                       "site_beg": beg, "nbr": nbr})
+    ctx.compute("fp64_plain")  # the block state the force loss reads (q, E, -Qss, dq/dr)
+    nets = []
+    D, Hd = models[0][0].dim, 128
+    for k, b in enumerate(blocks):
+        net = SG.MLP(ctx, D, Hd, 512, seed=5 + k)
+        w = {"W1": (rng.normal(size=(Hd, D)) * 0.5).astype(np.float32),
+             "b1": (0.1 * rng.normal(size=Hd)).astype(np.float32),
+             "W2": (rng.normal(size=(Hd, Hd)) / np.sqrt(Hd)).astype(np.float32),
+             "b2": (0.1 * rng.normal(size=Hd)).astype(np.float32),
+             "w3": (rng.normal(size=Hd) / np.sqrt(Hd)).astype(np.float32), "b3": np.array([0.02], np.float32)}
+        net.set_weights(w)
+        shift, scale = float(np.mean(b["e_ref"])), 0.05 + 0.02 * k
+        net.set_normalization(shift, scale)
+        b.update(w={kk: v.astype(np.float64) for kk, v in w.items()}, shift=shift, scale=scale)
+        nets.append(net)
+    le, lf = SG.force_gradients(ctx, nets, ids, 0.8, 1.4)
+    grads = [net.get_gradients() for net in nets]
+    le_o, lf_o, go = MO.force_loss_and_grads(blocks, sysm.n, 0.8, 1.4, emulate_bf16=True)
+    assert abs(le - le_o) <= 1e-2 * le_o and abs(lf - lf_o) <= 1e-2 * lf_o, (le, le_o, lf, lf_o)
+    for k in range(2):
+        for name in ("W1", "b1", "W2", "b2", "w3", "b3"):
+            assert _rel(grads[k][name], go[k][name]) < 2e-2, (k, name, _rel(grads[k][name], go[k][name]))

@@ -159,7 +159,7 @@ extern "C" int tgx_finalize(tgx_ctx* ctx) {
   for (auto* m : ctx->models) free_model(m);
   dbuf* bufs[] = {&ctx->pos, &ctx->species_sc, &ctx->n_neigh, &ctx->nl_off, &ctx->nlist, &ctx->rjs, &ctx->xyz,
                   &ctx->thetas, &ctx->phis, &ctx->nspec, &ctx->rev, &ctx->cell_of, &ctx->cell_cnt, &ctx->cell_start,
-                  &ctx->cell_atoms, &ctx->scan_tmp, &ctx->do_list, &ctx->energies, &ctx->forces, &ctx->fglob,
+                  &ctx->cell_atoms, &ctx->scan_tmp, &ctx->do_list, &ctx->energies, &ctx->forces, &ctx->fglob, &ctx->fres,
                   &ctx->virial_part, &ctx->scratch, &ctx->red, &ctx->vel, &ctx->mass, &ctx->forces_prev,
                   &ctx->pos_prev, &ctx->gate_mask, &ctx->cell_pos, &ctx->md_disp, &ctx->md_flag,
                   &ctx->md_fix, &ctx->md_gd0, &ctx->md_cm};
@@ -558,6 +558,7 @@ static int current_model(tgx_ctx* ctx, tgx_model_t** m) {
 static int run_selected_block(tgx_ctx* ctx, tgx_model_t* m, int path, bool do_forces) {
   const bool plain = path == TGX_PATH_FP64_PLAIN;
   TGX_TRY(tgx_soap_descriptors(ctx, m, path, plain && do_forces));
+  m->der_whole = plain && do_forces && m->sel_lo <= 0 && (m->sel_hi < 0 || m->sel_hi >= ctx->n_sites);
   TGX_TRY(tgx_gap_contract(ctx, m, path, do_forces));
   if (do_forces) {
     if (plain) TGX_TRY(tgx_plain_pair_forces(ctx, m));

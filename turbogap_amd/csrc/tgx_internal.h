@@ -160,6 +160,7 @@ struct tgx_model_t {
   dbuf pair_site;   // int[n_pairs_b]: centre of every kept pair (plain force pass)
   int64_t* sel_host = nullptr;  // pinned: the selection's site and pair totals
   int sel_lo = 0, sel_hi = -1;  // centres are taken from atoms [sel_lo, sel_hi) only (sel_hi < 0: all)
+  bool der_whole = false;       // soap_der holds dq/dr of every centre of the block (last FP64 plain evaluation)
   dbuf mlp_ws;      // surrogate work space (activations, member energies)
   // SOAP compression (compress_soap): q_c = P p / |P p| from the full power
   // spectrum p. D and Dpad above are then the compressed sizes, P.D and P.Dpad
@@ -212,6 +213,7 @@ struct tgx_ctx {
   std::vector<tgx_pair_model*> pair_models;
   // results
   dbuf energies, forces, fglob, virial_part, scratch, red;
+  dbuf fres;  // per-atom force residual of a force-loss training step
   int results_n = 0;  // atoms the energies / forces buffers hold results for (0: none)
   double mem_budget = 0.0;  // device bytes the FP64 plain path may hold for one block (0: 80% of the device)
   // md
@@ -299,6 +301,8 @@ int tgx_scratch_setup(int device);
 void* tgx_scratch_ptr(int device);
 size_t tgx_scratch_bytes();
 int tgx_adjoint_pair_forces(tgx_ctx* ctx, tgx_model_t* m, int path);
+// plain pair forces of the block with the given -dE/dq rows ([n_sites_b][Dpad]) into ctx->fglob
+int tgx_plain_pair_forces_q(tgx_ctx* ctx, tgx_model_t* m, const double* qss);
 // compressed blocks: q64 / soap_der from qf64 / derf, and qssf64 from qss64
 int tgx_compress_descriptors(tgx_ctx* ctx, tgx_model_t* m, bool with_der);
 int tgx_compress_gradient(tgx_ctx* ctx, tgx_model_t* m);

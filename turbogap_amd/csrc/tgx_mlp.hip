@@ -344,6 +344,67 @@ __global__ void k_gate_replace(int ng, const int* __restrict__ gated, int Dp, co
 
 inline dim3 g1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
+// ---- force-loss training (whole structures) -------------------------------------
+// Batch rows of a block's centres: q (FP32 and BF16) and the FP64 site
+// energies; rows past ns are zero.
+__global__ void k_force_batch(int ns, int Dp, const double* __restrict__ q64, const double* __restrict__ e_site,
+                              float* __restrict__ bq, __bf16* __restrict__ bqb, double* __restrict__ be) {
+  const int r = blockIdx.x;
+  const bool in = r < ns;
+  for (int j = threadIdx.x; j < Dp; j += blockDim.x) {
+    const float v = in ? (float)q64[(size_t)r * Dp + j] : 0.f;
+    bq[(size_t)r * Dp + j] = v;
+    bqb[(size_t)r * Dp + j] = (__bf16)v;
+  }
+  if (threadIdx.x == 0) be[r] = in ? e_site[r] : 0.0;
+}
+
+// -(g_hat - g) per centre, the -dE/dq rows the pair-force kernel takes: g_hat =
+// scale G is the network's gradient, g = -Qss the FP64 GAP's.
+__global__ void k_force_dg(int64_t n, double scale, const float* __restrict__ G, const double* __restrict__ qss,
+                           double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = -(scale * (double)G[i] + qss[i]);
+}
+
+// Energy residual weights dE_b = 2 w_e r_b / ns, r_b = E_b - (e_b - shift) / scale,
+// and r_b^2, for the block's centres; zero on the padding rows.
+__global__ void k_force_eres(int ns, int B, const float* __restrict__ E, const double* __restrict__ e_ref,
+                             double shift, double inv_scale, float w_e, float* __restrict__ dE,
+                             double* __restrict__ le) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= B) return;
+  const double x = r < ns ? (double)E[r] - (e_ref[r] - shift) * inv_scale : 0.0;
+  dE[r] = (float)(2.0 * (double)w_e * x / ns);
+  le[r] = x * x;
+}
+
+// u_i = coef sum_p der_p^T F_nbr(p) over the list entries p of centre i (the
+// self entry included): the force residual carried back to the centre's
+// descriptor. One workgroup per row, threads over components; padding rows and
+// columns are zero.
+__global__ __launch_bounds__(256) void k_force_jvp(int ns, int D, int Dp, const int64_t* __restrict__ site_beg,
+                                                   const int* __restrict__ pr_nbr, const double* __restrict__ der,
+                                                   const double* __restrict__ F, double coef, float* __restrict__ u,
+                                                   __bf16* __restrict__ ub) {
+  const int i = blockIdx.x;
+  float* ur = u + (size_t)i * Dp;
+  __bf16* ubr = ub + (size_t)i * Dp;
+  const int64_t p0 = i < ns ? site_beg[i] : 0, p1 = i < ns ? site_beg[i + 1] : 0;
+  for (int d = threadIdx.x; d < Dp; d += blockDim.x) {
+    double acc = 0.0;
+    if (d < D)
+      for (int64_t p = p0; p < p1; ++p) {
+        const double* f = F + 3 * (size_t)pr_nbr[p];
+        const double* j3 = der + ((size_t)p * D + d) * 3;
+        acc += j3[0] * f[0] + j3[1] * f[1] + j3[2] * f[2];
+      }
+    const float v = (float)(coef * acc);
+    ur[d] = v;
+    ubr[d] = (__bf16)v;
+  }
+}
+
 }  // namespace
 
 struct tgx_mlp {
@@ -361,6 +422,7 @@ struct tgx_mlp {
   // batch work space
   dbuf idx, bq, bqb, bg, be, z1, h1, h1b, s1, s1pp, z2, h2, a2, a2b, s2p, s2pp, E, v1, a1, a1b, G, u, ub, t0, t1, t1b,
       t2, gam2, zet2, zet2b, wz, zet1, dE, packA, packB, cpart, kpart, lossb;
+  dbuf fdg;  // force-loss step: -(g_hat - g) rows in FP64
   uint64_t seed = 0;
 };
 
@@ -453,7 +515,7 @@ extern "C" int tgx_mlp_destroy(tgx_mlp* m) {
                  &m->idx, &m->bq, &m->bqb, &m->bg, &m->be, &m->z1, &m->h1, &m->h1b, &m->s1, &m->s1pp, &m->z2, &m->h2,
                  &m->a2, &m->a2b, &m->s2p, &m->s2pp, &m->E, &m->v1, &m->a1, &m->a1b, &m->G, &m->u, &m->ub, &m->t0,
                  &m->t1, &m->t1b, &m->t2, &m->gam2, &m->zet2, &m->zet2b, &m->wz, &m->zet1, &m->dE, &m->packA,
-                 &m->packB, &m->cpart, &m->kpart, &m->lossb};
+                 &m->packB, &m->cpart, &m->kpart, &m->lossb, &m->fdg};
   for (dbuf* b : all) b->release();
   delete m;
   return TGX_OK;
@@ -517,20 +579,39 @@ static int wgrad_gemm(tgx_mlp* m, const char* name, int M, int N, int K, const _
   return TGX_OK;
 }
 
+static int mlp_backward_dirs(tgx_mlp* m, int B, float cG);
+
 // Loss and weight gradients of the batch in the batch buffers (bq, bqb, bg, be).
 static int mlp_backward(tgx_mlp* m, int B, float w_e, float w_g, double* loss_e, double* loss_g) {
   tgx_ctx* ctx = m->ctx;
-  const int H = m->H, Dp = m->Dp;
-  const int64_t nbh = (int64_t)B * H, nbd = (int64_t)B * Dp;
-  float* th = m->theta.as<float>();
-  float* gr = m->grad.as<float>();
+  const int Dp = m->Dp;
   TGX_TRY(mlp_forward(m, B, m->bqb.as<__bf16>(), true, m->E.as<float>(), m->G.as<float>(), 1.0));
   double* le = m->lossb.as<double>();
   double* lg = le + B;
   TGX_LAUNCH(ctx, "mlp_loss", k_loss, dim3((B + 3) / 4), dim3(256), 0, B, Dp, m->E.as<float>(), m->be.as<double>(),
              m->G.as<float>(), m->bg.as<float>(), m->bq.as<float>(), m->shift, 1.0 / m->scale, w_e, m->u.as<float>(),
              m->ub.as<__bf16>(), m->dE.as<float>(), le, lg);
-  const float cG = 2.0f * w_g / (float)B;
+  TGX_TRY(mlp_backward_dirs(m, B, 2.0f * w_g / (float)B));
+  if (loss_e || loss_g) {
+    double se = 0.0, sg = 0.0;
+    TGX_TRY(tgx_device_sum(ctx, le, B, &se));
+    TGX_TRY(tgx_device_sum(ctx, lg, B, &sg));
+    if (loss_e) *loss_e = se / B;
+    if (loss_g) *loss_g = sg / B;
+  }
+  return TGX_OK;
+}
+
+// Weight gradients of a batch whose forward pass (train = true) has run, from
+// the energy residual weights dE (per row) and the gradient-loss direction u
+// (FP32 and BF16 rows) with its weight cG: the reverse-mode derivative of
+// sum_b dE_b E_b + cG u_b . g_b with dE and u fixed.
+static int mlp_backward_dirs(tgx_mlp* m, int B, float cG) {
+  tgx_ctx* ctx = m->ctx;
+  const int H = m->H, Dp = m->Dp;
+  const int64_t nbh = (int64_t)B * H;
+  float* th = m->theta.as<float>();
+  float* gr = m->grad.as<float>();
   // forward mode along u, then reverse
   TGX_TRY(tgx_gemm_bf16_nt(ctx, "mlp_jvp1", B, H, Dp, m->ub.as<__bf16>(), Dp, m->W1b.as<__bf16>(), Dp, m->t0.as<float>(),
                            H, 1.0));
@@ -572,13 +653,6 @@ static int mlp_backward(tgx_mlp* m, int B, float w_e, float w_g, double* loss_e,
   TGX_LAUNCH(ctx, "mlp_colsum", k_colsum_part, dim3(ldp / 64, nch, 4), dim3(256), 0, ct, m->cpart.as<double>());
   TGX_LAUNCH(ctx, "mlp_colsum_fin", k_colsum_fin, dim3((ldp + 63) / 64, 1, 4), dim3(64), 0, ct, nch, ldp,
              (const double*)m->cpart.as<double>());
-  if (loss_e || loss_g) {
-    double se = 0.0, sg = 0.0;
-    TGX_TRY(tgx_device_sum(ctx, le, B, &se));
-    TGX_TRY(tgx_device_sum(ctx, lg, B, &sg));
-    if (loss_e) *loss_e = se / B;
-    if (loss_g) *loss_g = sg / B;
-  }
   return TGX_OK;
 }
 
@@ -786,6 +860,18 @@ extern "C" int tgx_mlp_gradients_host(tgx_mlp* m, int batch, const float* q, con
   TGX_CHECK_HIP(hipMemcpyAsync(m->be.p, ep.data(), sizeof(double) * ep.size(), hipMemcpyHostToDevice, ctx->stream));
   TGX_TRY(tgx_f32_to_bf16(ctx, m->bq.as<float>(), m->bqb.as<__bf16>(), (int64_t)B * Dp));
   TGX_TRY(mlp_backward(m, B, w_e, w_g, loss_e, loss_g));
+  (void)H;
+  (void)D;
+  return tgx_mlp_get_gradients(m, dW1, db1, dW2, db2, dw3, db3);
+}
+
+// The gradient of the last backward pass, in the layout of tgx_mlp_get_weights.
+extern "C" int tgx_mlp_get_gradients(tgx_mlp* m, float* dW1, float* db1, float* dW2, float* db2, float* dw3,
+                                     float* db3) {
+  TGX_REQUIRE(m, "null network");
+  tgx_ctx* ctx = m->ctx;
+  TGX_CHECK_HIP(hipSetDevice(ctx->device));
+  const int H = m->H, D = m->D, Dp = m->Dp;
   std::vector<float> gr((size_t)m->nparam);
   TGX_CHECK_HIP(hipMemcpyAsync(gr.data(), m->grad.p, sizeof(float) * gr.size(), hipMemcpyDeviceToHost, ctx->stream));
   TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
@@ -799,6 +885,109 @@ extern "C" int tgx_mlp_gradients_host(tgx_mlp* m, int batch, const float* q, con
   }
   if (dW2) std::memcpy(dW2, gr.data() + m->oW2, sizeof(float) * (size_t)H * H);
   if (db3) db3[0] = gr[m->ob3];
+  return TGX_OK;
+}
+
+extern "C" int tgx_mlp_adam_step(tgx_mlp* m, float lr) {
+  TGX_REQUIRE(m && lr >= 0.0f, "tgx_mlp_adam_step: bad arguments");
+  TGX_CHECK_HIP(hipSetDevice(m->ctx->device));
+  TGX_TRY(adam_step(m, lr));
+  TGX_CHECK_HIP(hipStreamSynchronize(m->ctx->stream));
+  return TGX_OK;
+}
+
+// Energy and force loss of whole structures: the centres of every listed
+// block (the last FP64 plain evaluation of the current system, which holds
+// q, E, -Qss and dq/dr of every centre) form one batch per network. With the
+// network gradients g_hat_i = scale G_i and the GAP's g_i = -Qss_i, the force
+// residual dF_j = -sum_i (g_hat_i - g_i) . dq_i/dr_j is gathered over all
+// blocks (the pair-force kernel and the fixed-order gather), and
+//   L = w_e sum_blocks mean_i r_i^2 + w_f / n_atoms sum_j |dF_j|^2,
+// r_i the normalised energy residual. dL/dG_i = cG u_i with cG = 2 w_f / n_atoms
+// and u_i = -scale sum_p (dq_i/dr_nbr(p))^T dF_nbr(p), which the network's
+// forward-then-reverse pass takes like the gradient-loss direction. The weight
+// gradients are left in each network (tgx_mlp_get_gradients, tgx_mlp_adam_step);
+// the context's pair-force scratch is overwritten.
+extern "C" int tgx_mlp_force_gradients(tgx_ctx* ctx, int n_blocks, tgx_mlp* const* nets, const int* model_ids,
+                                       double w_e, double w_f, double* loss_e, double* loss_f) {
+  TGX_CTX_CHECK(ctx);
+  TGX_REQUIRE(n_blocks >= 1 && nets && model_ids, "tgx_mlp_force_gradients: bad arguments");
+  TGX_REQUIRE(ctx->have_list && ctx->n_sites > 0, "tgx_mlp_force_gradients: no neighbour list");
+  std::vector<tgx_model_t*> md((size_t)n_blocks);
+  for (int b = 0; b < n_blocks; ++b) {
+    tgx_mlp* m = nets[b];
+    TGX_REQUIRE(m && m->ctx == ctx, "tgx_mlp_force_gradients: network %d belongs to another context", b);
+    TGX_REQUIRE(model_ids[b] >= 0 && model_ids[b] < (int)ctx->models.size(), "model %d not defined", model_ids[b]);
+    for (int c = 0; c < b; ++c)
+      TGX_REQUIRE(model_ids[c] != model_ids[b], "tgx_mlp_force_gradients: model %d listed twice", model_ids[b]);
+    md[b] = ctx->models[model_ids[b]];
+    TGX_REQUIRE(md[b]->D == m->D && md[b]->Dpad == m->Dp, "descriptor size of model %d does not match network %d",
+                model_ids[b], b);
+    TGX_REQUIRE(m->norm_set, "tgx_mlp_force_gradients: set the energy normalisation of network %d first", b);
+    TGX_REQUIRE(md[b]->der_whole && md[b]->soap_der.p && md[b]->qss64.p,
+                "tgx_mlp_force_gradients: model %d needs a whole-block FP64 plain evaluation with forces first",
+                model_ids[b]);
+    TGX_REQUIRE(md[b]->n_sites_b <= m->maxB, "model %d has %d centres, more than network %d's max_batch %d",
+                model_ids[b], md[b]->n_sites_b, b, m->maxB);
+  }
+  const int n_at = ctx->n_sites;
+  std::vector<int> Bs((size_t)n_blocks);
+  // forward pass of every network on its block's centres
+  for (int b = 0; b < n_blocks; ++b) {
+    tgx_mlp* m = nets[b];
+    const int ns = md[b]->n_sites_b;
+    const int B = (int)tgx_round_up(ns > 0 ? ns : 1, 128);
+    Bs[(size_t)b] = B;
+    TGX_LAUNCH(ctx, "mlp_force_batch", k_force_batch, dim3(B), dim3(256), 0, ns, m->Dp, md[b]->q64.as<double>(),
+               md[b]->e_site.as<double>(), m->bq.as<float>(), m->bqb.as<__bf16>(), m->be.as<double>());
+    TGX_TRY(mlp_forward(m, B, m->bqb.as<__bf16>(), true, m->E.as<float>(), m->G.as<float>(), 1.0));
+  }
+  // force residual of all blocks together
+  const size_t P3 = 3 * (size_t)(ctx->n_pairs > 0 ? ctx->n_pairs : 1);
+  TGX_TRY(ctx->fglob.ensure(sizeof(double) * P3));
+  TGX_TRY(tgx_zero(ctx, ctx->fglob.p, sizeof(double) * P3));
+  for (int b = 0; b < n_blocks; ++b) {
+    tgx_mlp* m = nets[b];
+    const int ns = md[b]->n_sites_b;
+    if (ns == 0) continue;
+    const int64_t n = (int64_t)ns * m->Dp;
+    TGX_TRY(m->fdg.ensure(sizeof(double) * (size_t)n));
+    TGX_LAUNCH(ctx, "mlp_force_dg", k_force_dg, g1(n), dim3(256), 0, n, m->scale, (const float*)m->G.as<float>(),
+               (const double*)md[b]->qss64.as<double>(), m->fdg.as<double>());
+    TGX_TRY(tgx_plain_pair_forces_q(ctx, md[b], m->fdg.as<double>()));
+  }
+  TGX_TRY(ctx->fres.ensure(sizeof(double) * 3 * (size_t)n_at));
+  TGX_TRY(tgx_gather_pair_forces(ctx, ctx->fglob.as<double>(), ctx->fres.as<double>(), nullptr, nullptr, nullptr));
+  // per network: energy residuals, the force direction, the weight gradients
+  const float cF = (float)(2.0 * w_f / n_at);
+  double se = 0.0;
+  for (int b = 0; b < n_blocks; ++b) {
+    tgx_mlp* m = nets[b];
+    const int ns = md[b]->n_sites_b, B = Bs[(size_t)b];
+    double* le = m->lossb.as<double>();
+    TGX_LAUNCH(ctx, "mlp_force_eres", k_force_eres, dim3((B + 255) / 256), dim3(256), 0, ns > 0 ? ns : 1, B,
+               (const float*)m->E.as<float>(), (const double*)m->be.as<double>(), m->shift, 1.0 / m->scale,
+               (float)w_e, m->dE.as<float>(), le);
+    TGX_LAUNCH(ctx, "mlp_force_jvp", k_force_jvp, dim3(B), dim3(256), 0, ns, m->D, m->Dp,
+               md[b]->site_beg.as<int64_t>(), md[b]->pr_nbr.as<int>(), md[b]->soap_der.as<double>(),
+               (const double*)ctx->fres.as<double>(), -m->scale, m->u.as<float>(), m->ub.as<__bf16>());
+    TGX_TRY(mlp_backward_dirs(m, B, cF));
+    if (loss_e && ns > 0) {
+      double e = 0.0;
+      TGX_TRY(tgx_device_sum(ctx, le, B, &e));
+      se += e / ns;
+    }
+  }
+  if (loss_e) *loss_e = se;
+  if (loss_f) {
+    std::vector<double> F(3 * (size_t)n_at);
+    TGX_CHECK_HIP(hipMemcpyAsync(F.data(), ctx->fres.p, sizeof(double) * F.size(), hipMemcpyDeviceToHost, ctx->stream));
+    TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    double sf = 0.0;
+    for (double v : F) sf += v * v;
+    *loss_f = sf / n_at;
+  }
+  TGX_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   return TGX_OK;
 }
 

@@ -1342,6 +1342,7 @@ static int check_lds(size_t bytes) {
 }
 
 int tgx_soap_descriptors(tgx_ctx* ctx, tgx_model_t* m, int path, bool with_der) {
+  m->der_whole = false;  // set again by a whole-block FP64 plain evaluation
   const int ns = m->n_sites_b;
   if (ns == 0) return TGX_OK;
   TGX_REQUIRE(m->P.D <= 24 * WG, "descriptor size %d exceeds this build's limit %d", m->P.D, 24 * WG);
@@ -1433,7 +1434,9 @@ int tgx_adjoint_pair_forces(tgx_ctx* ctx, tgx_model_t* m, int path) {
   return tgx_adjoint_wave(ctx, m, fp64);
 }
 
-int tgx_plain_pair_forces(tgx_ctx* ctx, tgx_model_t* m) {
+int tgx_plain_pair_forces(tgx_ctx* ctx, tgx_model_t* m) { return tgx_plain_pair_forces_q(ctx, m, m->qss64.as<double>()); }
+
+int tgx_plain_pair_forces_q(tgx_ctx* ctx, tgx_model_t* m, const double* qss) {
   const int ns = m->n_sites_b;
   const int64_t np = m->n_pairs_b;
   if (ns == 0 || np == 0) return TGX_OK;
@@ -1441,8 +1444,8 @@ int tgx_plain_pair_forces(tgx_ctx* ctx, tgx_model_t* m) {
   int* pair_site = m->pair_site.as<int>();
   TGX_LAUNCH(ctx, "pair_site", k_pair_site, dim3(ns), dim3(64), 0, ns, m->site_beg.as<int64_t>(), pair_site);
   TGX_LAUNCH(ctx, "plain_force", k_plain_force, dim3((unsigned)((np + 3) / 4)), dim3(256), 0, ns,
-             m->site_beg.as<int64_t>(), m->D, m->Dpad, m->qss64.as<double>(), m->soap_der.as<double>(),
-             m->pr_gidx.as<int64_t>(), np, pair_site, ctx->fglob.as<double>());
+             m->site_beg.as<int64_t>(), m->D, m->Dpad, qss, m->soap_der.as<double>(), m->pr_gidx.as<int64_t>(), np,
+             pair_site, ctx->fglob.as<double>());
   return TGX_OK;
 }
 

@@ -107,6 +107,9 @@ _SIGS = {
     "tgx_mlp_evaluate": [_P, _I, _I, _PD, _PD, _PD],
     "tgx_mlp_forward_host": [_P, _I, _PF, _PF, _PF],
     "tgx_mlp_gradients_host": [_P, _I, _PF, _PF, _PF, C.c_float, C.c_float, _PF, _PF, _PF, _PF, _PF, _PF, _PD, _PD],
+    "tgx_mlp_force_gradients": [_P, _I, C.POINTER(_P), _PI, _D, _D, _PD, _PD],
+    "tgx_mlp_get_gradients": [_P, _PF, _PF, _PF, _PF, _PF, _PF],
+    "tgx_mlp_adam_step": [_P, C.c_float],
     "tgx_mlp_get_weights": [_P, _PF, _PF, _PF, _PF, _PF, _PF],
     "tgx_mlp_set_weights": [_P, _PF, _PF, _PF, _PF, _PF, _PF],
     "tgx_surrogate_install": [_P, _I, _I, _P],

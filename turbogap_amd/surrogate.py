@@ -7,10 +7,13 @@ the gradient targets the descriptor-space gradients dE_i/dq_i (= -Qss_i of
 src/gap.f90:184), both taken from FP64 tgx_compute runs; the network is then
 installed as an ensemble member of a block's surrogate (TGX_PATH_SURROGATE).
 
-The gradient term stands in for the force loss: F_j = -sum_i (dE_i/dq_i) .
-(dq_i/dr_j), so matching dE_i/dq_i in the directions a force can see (the
-tangent space of the normalised descriptor) matches every force built from it,
-while batches remain single environments.
+Two losses train a network. On environments taken one by one (MLP.train):
+energy plus the descriptor-gradient term |P (g - g_ref)|^2, which matches
+dE_i/dq_i in the directions a force can see (the tangent space of the
+normalised descriptor). On whole structures (force_gradients, train_forces):
+energy plus the force error itself, F_j = -sum_i (dE_i/dq_i) . (dq_i/dr_j)
+summed over every block, formed from the FP64 plain path's dq/dr of the
+structure.
 """
 from __future__ import annotations
 
@@ -89,6 +92,18 @@ class MLP:
                 "mlp_gradients_host")
         return out, le.value, lg.value
 
+    def get_gradients(self):
+        """The weight gradients of the last backward pass (layout of get_weights)."""
+        H, D = self.hidden, self.dim
+        g = {"W1": np.empty((H, D), np.float32), "b1": np.empty(H, np.float32), "W2": np.empty((H, H), np.float32),
+             "b2": np.empty(H, np.float32), "w3": np.empty(H, np.float32), "b3": np.empty(1, np.float32)}
+        L.check(L.load().tgx_mlp_get_gradients(self.h, *[L.fptr(g[k]) for k in ("W1", "b1", "W2", "b2", "w3", "b3")]),
+                "mlp_get_gradients")
+        return g
+
+    def adam_step(self, lr: float = 1e-3):
+        L.check(L.load().tgx_mlp_adam_step(self.h, C.c_float(lr)), "mlp_adam_step")
+
     def get_weights(self):
         H, D = self.hidden, self.dim
         w = {"W1": np.empty((H, D), np.float32), "b1": np.empty(H, np.float32), "W2": np.empty((H, H), np.float32),
@@ -152,3 +167,67 @@ def train_ensemble(mlps, n_samples, epochs=30, batch=4096, lr=1e-3, w_energy=1.0
             if log:
                 log(f"train block {k} member {i}: {n_samples[k]} samples, {epochs} epochs, "
                     f"loss_E={le:.4g} loss_g={lg:.4g} ({time.perf_counter() - t0:.1f} s)")
+
+
+def force_gradients(ctx, nets, model_ids, w_energy=1.0, w_force=1.0):
+    """Energy plus force loss of the context's current structure for one network
+    per block (nets[b] for model_ids[b]) after ctx.compute("fp64_plain"): the
+    force error of all blocks together. Leaves each network's weight gradients;
+    returns (loss_energy, loss_force) with the force loss in (eV/A)^2 per atom."""
+    n = len(nets)
+    arr = (C.c_void_p * n)(*[net.h.value for net in nets])
+    ids = np.ascontiguousarray(model_ids, dtype=np.int32)
+    le, lf = C.c_double(), C.c_double()
+    L.check(L.load().tgx_mlp_force_gradients(ctx.h, n, arr, L.iptr(ids), float(w_energy), float(w_force), C.byref(le),
+                                             C.byref(lf)), "mlp_force_gradients")
+    return le.value, lf.value
+
+
+def md_structures(ctx, system, temperatures=(300.0, 1000.0, 2000.0, 3000.0), steps=200, every=50, dt=1.0,
+                  tau=100.0, seed=23, rcut=3.7, path="fp64_plain"):
+    """Whole-structure snapshots for the force loss: for each temperature, NVT
+    Berendsen MD of `system` from its initial structure with seeded Maxwell
+    velocities, a snapshot (positions) every `every` steps up to `steps`."""
+    from .md import MD, maxwell_velocities
+    from . import structures as S
+    mass = S.masses_for(system.species)
+    snaps = []
+    for it, T in enumerate(temperatures):
+        ctx.set_positions(system.positions)
+        md = MD(ctx, mass, maxwell_velocities(mass, T, seed=seed + it), rcut, path)
+        for k in range(steps):
+            md.step(dt, T0=T, tau=tau)
+            if (k + 1) % every == 0:
+                snaps.append(md.state()[0].copy())
+    ctx.set_positions(system.positions)
+    return snaps
+
+
+def train_forces(ctx, mlps, model_ids, structures, epochs=5, lr=1e-3, w_energy=1.0, w_force=1.0, rcut=3.7,
+                 log=None):
+    """Energy plus force loss over whole structures (positions of the context's
+    system): each epoch visits every structure once, evaluates it with the FP64
+    plain path and takes one Adam step per network, member i of every block
+    trained together (mlps[b][i]). Returns the last epoch's mean losses."""
+    import time
+    t0 = time.perf_counter()
+    n_mem = len(mlps[0])
+    le_m = lf_m = 0.0
+    for ep in range(epochs):
+        le_s = lf_s = 0.0
+        for pos in structures:
+            ctx.set_positions(pos)
+            ctx.build_neighbors_list(rcut, fetch=False)
+            ctx.compute("fp64_plain")
+            for i in range(n_mem):
+                nets = [mlps[b][i] for b in range(len(mlps))]
+                le, lf = force_gradients(ctx, nets, model_ids, w_energy, w_force)
+                for net in nets:
+                    net.adam_step(lr)
+                le_s += le / n_mem
+                lf_s += lf / n_mem
+        le_m, lf_m = le_s / len(structures), lf_s / len(structures)
+        if log:
+            log(f"force training epoch {ep + 1}: {len(structures)} structures, loss_E={le_m:.4g} "
+                f"loss_F={lf_m:.4g} (eV/A)^2 ({time.perf_counter() - t0:.1f} s)")
+    return le_m, lf_m
