@@ -127,7 +127,7 @@ def roofline_entries(stats, n_atoms, blocks, n_pairs, members=1, hidden=256):
 
 # bench kernel label -> device kernel name in the rocprofv3 counter summary
 KERNEL_SYMBOL = {"soap_adjoint_f32": "k_site_adjoint_mfma", "soap_desc_f32": "k_site_desc", "soap_pair_basis": "k_pair_basis",
-                 "nbr_fill": "k_cell_neigh<1>", "soap_site_der_f64": "k_soap_der_l<8, 16>",
+                 "nbr_fill": "k_cell_neigh<1>", "soap_site_der_f64": "k_soap_der_l<8, 5>",
                  "plain_force": "k_plain_force", "gemm_kernel_f64": "k_gemm_f64<1, true>",
                  "gemm_qss_f64": "k_gemm_f64<0, true>"}
 
