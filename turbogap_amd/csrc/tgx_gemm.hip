@@ -371,7 +371,10 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16_w(int K, const __bf16* __r
   const int m0 = tm * TM, n0 = tn * BN;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
-  const int lr = tid >> 2, lc = (tid & 3) * 8;  // rows lr + 64 j, 16-byte chunk lc
+  // rows lr + 64 j, 16-byte chunk lc; the eight lanes of a store group take rows r and r + 4 of an
+  // 8-row run (at the 80-byte pitch rows r and r + 1 would share a bank at byte 128)
+  const int jr = tid >> 2;
+  const int lr = (jr & ~7) | ((jr & 1) << 2) | ((jr >> 1) & 3), lc = (tid & 3) * 8;
   {
     const int z = blockIdx.z;
     A += z * e.sA;
