@@ -303,6 +303,79 @@ __global__ __launch_bounds__(PB) void k_pair_basis(SoapParams P, F32Args a, int6
     for (int cc = t; cc < 2 * NMAXF; cc += PB) dst[(size_t)rr * a.ldb + a.roff + cc] = st[rr * LS + cc];
 }
 
+// k_pair_basis for l_max = 8 (roff = 84): the pair's 81 damped harmonics stay
+// in registers and leave in three passes of 28 columns, then R and dR in one
+// pass of 32, through a 64 x 33 staging buffer (8.4 KB instead of 64 x 85
+// floats), so that LDS no longer bounds the number of resident workgroups.
+// Each pass stores its 64 rows x W columns as one flat run over the threads.
+constexpr int PB8_W = 28;  // harmonic columns per pass
+constexpr int PB8_LS = 33; // staging pitch (odd)
+
+__global__ __launch_bounds__(PB) void k_pair_basis8(SoapParams P, F32Args a, int64_t n_pairs) {
+  extern __shared__ __align__(16) float smf[];
+  Shared sh;
+  load_shared(P, a, smf, sh);
+  float* st = smf + shared_floats(P.NLM, P.n_species);
+  const int t = threadIdx.x;
+  float* row = st + t * PB8_LS;
+  __syncthreads();  // the shared tables are loaded
+  const int64_t p0 = (int64_t)blockIdx.x * PB;
+  const int64_t p = p0 + t;
+  int s = -1;
+  double rd = 0.0;
+  if (p < n_pairs) {
+    s = a.pr_spec[p];
+    rd = a.pr_r[p];
+    // the self pair (r = 0) and neighbours at or beyond their species' cutoff carry nothing
+    if (s >= 0 && !(rd > 0.0 && rd < P.sp[s].rc)) s = -1;
+  }
+  const int64_t left = n_pairs - p0;
+  const int nrows = left < PB ? (int)left : PB;
+  float* dst = a.basis + p0 * a.ldb;
+  float W[3 * PB8_W];
+  float R[TGX_MAX_NB], dR[TGX_MAX_NB];
+  SpecLite sp{};
+#pragma unroll
+  for (int i = 0; i < 3 * PB8_W; ++i) W[i] = 0.f;
+  if (s >= 0) {
+    sp = sh.sp[s];
+    const float r = (float)rd, inv = 1.f / r;
+    const float ux = (float)a.pr_xyz[3 * p] * inv, uy = (float)a.pr_xyz[3 * p + 1] * inv,
+                uz = (float)a.pr_xyz[3 * p + 2] * inv;
+    radial_tab<true>(a, sp, s, r, R, dR);
+    float A[TGX_MAX_L + 1], dA[TGX_MAX_L + 1];
+    damping(sp, r, A, dA);
+    harmonics_fixed<8>(sh.ynorm, ux, uy, uz, A, W);
+  }
+#pragma unroll
+  for (int pass = 0; pass < 3; ++pass) {
+#pragma unroll
+    for (int k = 0; k < PB8_W; ++k) row[k] = W[pass * PB8_W + k];
+    __syncthreads();
+    for (int e = t; e < nrows * PB8_W; e += PB) {
+      const int rr = e / PB8_W, cc = e - rr * PB8_W;
+      dst[(size_t)rr * a.ldb + pass * PB8_W + cc] = st[rr * PB8_LS + cc];
+    }
+    __syncthreads();  // the staging rows are rewritten by the next pass
+  }
+  // R and dR (2 NMAXF columns from roff), the block's species at its offset
+#pragma unroll
+  for (int i = 0; i < 2 * NMAXF; ++i) row[i] = 0.f;
+  if (s >= 0) {
+#pragma unroll
+    for (int n = 0; n < TGX_MAX_NB; ++n)
+      if (n < sp.nb) {
+        row[sp.off + n] = R[n];
+        row[NMAXF + sp.off + n] = dR[n];
+      }
+  }
+  __syncthreads();
+  for (int e = t; e < nrows * 2 * NMAXF; e += PB) {
+    const int rr = e / (2 * NMAXF), cc = e - rr * (2 * NMAXF);
+    dst[(size_t)rr * a.ldb + a.roff + cc] = st[rr * PB8_LS + cc];
+  }
+}
+
 // c(n, lm) of a site in registers, lanes over lm (lm = lane and lane + 64).
 // SPLIT allows the species-split accumulation (fewer FMAs, more registers).
 template <bool SPLIT>
@@ -1300,7 +1373,11 @@ int tgx_desc_f32(tgx_ctx* ctx, tgx_model_t* m) {
   F32Args a = f32_args(m);
   TGX_TRY(m->pbasis.ensure(sizeof(float) * (size_t)(np > 0 ? np : 1) * a.ldb));
   a.basis = m->pbasis.as<float>();
-  if (np > 0) {
+  if (np > 0 && m->P.L == 8 && a.roff == 3 * PB8_W && 2 * NMAXF < PB8_LS) {
+    const size_t lds = sizeof(float) * ((size_t)shared_floats(m->P.NLM, m->P.n_species) + (size_t)PB * PB8_LS);
+    TGX_LAUNCH(ctx, "soap_pair_basis", k_pair_basis8, dim3((unsigned)((np + PB - 1) / PB)), dim3(PB), lds, m->P, a,
+               np);
+  } else if (np > 0) {
     const size_t lds = sizeof(float) * ((size_t)shared_floats(m->P.NLM, m->P.n_species) + (size_t)PB * (a.roff + 1));
     TGX_REQUIRE(lds <= 64 * 1024, "pair basis needs %zu bytes of LDS", lds);
     TGX_LAUNCH(ctx, "soap_pair_basis", k_pair_basis, dim3((unsigned)((np + PB - 1) / PB)), dim3(PB), lds, m->P, a, np);
