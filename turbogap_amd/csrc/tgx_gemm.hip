@@ -439,6 +439,118 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16_w(int K, const __bf16* __r
   epilogue<EPI, OutT, float, f32x4, false, ZINT, 8, 4, 128, 64, TM>(acc, m0, n0, tn, wm, wn, e, sE);
 }
 
+// BF16 NT GEMM fed by direct global-to-LDS loads (global_load_lds_dwordx4):
+// workgroup tile TM x 128, four wavefronts of (TM / 2) x 64, BK = 32, an
+// NS-stage ring of LDS buffers. A stage is TM + 128 rows of 64 bytes (the A
+// rows, then the B rows); the four 16-byte chunks of row r sit at chunk
+// c ^ ((r >> 1) & 3), so that the eight rows a 16-byte fragment read serves
+// per cycle cover all 32 banks. One load instruction fills 16 rows (lane l
+// writes chunk l & 3 of row l >> 2 and fetches the chunk that belongs there).
+// Stage s + NS - 1 is issued right after the barrier that opens stage s, into
+// the buffer stage s - 1 used, so NS - 1 stages are in flight while the MFMAs
+// of stage s run; nothing is staged through registers. Each wavefront waits
+// for its own loads of stage s (vmcnt), the barrier then covers every
+// wavefront's. M a multiple of TM, N of 128, K of 32.
+constexpr int DK = 32;                      // k per stage
+constexpr int DROW = DK * 2;                // bytes per staged row
+__host__ __device__ constexpr int dma_stage_bytes(int tm) { return (tm + BN) * DROW; }
+__host__ __device__ constexpr int dma_lds_bytes(int tm, int ns) { return ns * dma_stage_bytes(tm) + 2 * tm * 8; }
+
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 16, "vmcnt immediate");
+  __builtin_amdgcn_s_waitcnt(0x3F70 | N);  // vmcnt(N); expcnt and lgkmcnt not waited on
+}
+
+// One stage's loads of a wavefront: its P blocks of 16 rows, block b of the
+// stage into bytes [1024 b, 1024 b + 1024) of the stage buffer.
+template <int P>
+__device__ __forceinline__ void dma_issue(const __bf16* const (&src)[P], int k0, unsigned char* stage, int w) {
+#pragma unroll
+  for (int j = 0; j < P; ++j)
+    __builtin_amdgcn_global_load_lds(const_cast<__bf16*>(src[j] + k0), stage + (w + 4 * j) * 1024, 16, 0, 0);
+}
+
+template <int EPI, class OutT, bool ZINT, int TM, int NS>
+__global__ __launch_bounds__(256, 2) void k_gemm_bf16_dma(int K, const __bf16* __restrict__ A, int lda,
+                                                          const __bf16* __restrict__ B, int ldb, EpiArgs e) {
+  constexpr int TI = TM / 32;                 // 16-row MFMA tiles per wavefront along M
+  constexpr int NBA = TM / 16, NBB = BN / 16; // 16-row load blocks per stage
+  constexpr int P = (NBA + NBB) / 4;          // load instructions per wavefront and stage
+  static_assert((NBA + NBB) % 4 == 0, "load blocks per wavefront");
+  static_assert((NS - 2) * P < 16, "vmcnt range");
+  constexpr int SB = dma_stage_bytes(TM);
+  extern __shared__ __align__(16) unsigned char dsm[];
+  double* sE = reinterpret_cast<double*>(dsm + NS * SB);
+  int tm, tn;
+  tile_of(tm, tn);
+  const int m0 = tm * TM, n0 = tn * BN;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  {
+    const int z = blockIdx.z;
+    A += z * e.sA;
+    B += z * e.sB;
+    e.out = reinterpret_cast<OutT*>(e.out) + z * e.sOut;
+    if (e.aux) e.aux += z * e.sAux;
+    if (e.bias) e.bias += z * e.sBias;
+    if (e.w3) e.w3 += z * e.sW3;
+    if (e.epart) e.epart += z * e.sEpart;
+    if (e.zscale) e.pref = e.zscale[z];
+  }
+  // this lane's source in each of its P load blocks: row 16 b + lane / 4, the
+  // logical chunk stored at physical chunk lane & 3
+  const int lrow = lane >> 2, lchunk = (lane & 3) ^ ((lane >> 3) & 3);
+  const __bf16* src[P];
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    const int b = w + 4 * j;
+    src[j] = b < NBA ? A + (size_t)(m0 + 16 * b + lrow) * lda + 8 * lchunk
+                     : B + (size_t)(n0 + 16 * (b - NBA) + lrow) * ldb + 8 * lchunk;
+  }
+  // fragment reads: row li of a 16-row tile, logical chunk lg (k 8 lg .. 8 lg + 7)
+  const int li = lane & 15, lg = lane >> 4;
+  const int foff = li * DROW + ((lg ^ ((li >> 1) & 3)) << 4);
+  const int aoff = wm * (TM / 2) * DROW + foff, boff = TM * DROW + wn * 64 * DROW + foff;
+  f32x4 acc[TI][4];
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (f32x4){0, 0, 0, 0};
+  const int n_stages = K / DK;
+#pragma unroll
+  for (int s = 0; s < NS - 1; ++s) dma_issue<P>(src, min(s, n_stages - 1) * DK, dsm + s * SB, w);
+  int rs = 0, ws = NS - 1;  // ring slots read and written this stage
+  for (int st = 0; st < n_stages; ++st) {
+    __builtin_amdgcn_sched_barrier(0);
+    if (st + NS - 2 < n_stages)
+      wait_vmcnt<(NS - 2) * P>();
+    else
+      wait_vmcnt<0>();
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_sched_barrier(0);
+    // the slot written here was read in stage st - 1, which every wavefront has finished
+    dma_issue<P>(src, min(st + NS - 1, n_stages - 1) * DK, dsm + ws * SB, w);
+    __builtin_amdgcn_sched_barrier(0);
+    const unsigned char* sb = dsm + rs * SB;
+    bf16x8 b[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) b[t] = *reinterpret_cast<const bf16x8*>(sb + boff + t * 16 * DROW);
+#pragma unroll
+    for (int i = 0; i < TI; ++i) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(sb + aoff + i * 16 * DROW);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[i][j], 0, 0, 0);
+    }
+    rs = rs + 1 == NS ? 0 : rs + 1;
+    ws = ws + 1 == NS ? 0 : ws + 1;
+  }
+  // the loads issued past the last stage (in-bounds repeats) land before the workgroup's LDS is released
+  wait_vmcnt<0>();
+  __builtin_amdgcn_sched_barrier(0);
+  epilogue<EPI, OutT, float, f32x4, false, ZINT, TI, 4, TM / 2, 64, TM>(acc, m0, n0, tn, wm, wn, e, sE);
+}
+
 // E_i = delta^2 sum_t epart[i][t] + e0, the partials summed in tile order.
 __global__ void k_energy_finalize(int n, const double* __restrict__ epart, int n_tiles, double d2, double e0,
                                   double* __restrict__ e_site) {
@@ -467,16 +579,27 @@ static EpiArgs gap_epi(const tgx_model_t* m) {
   return e;
 }
 
-// BF16 NT GEMM launch: 256 x 128 tiles when M is a multiple of 256 and there
-// are at least 1,024 of them (two rounds of two workgroups per CU; with fewer,
-// the last round's imbalance costs more than the wider tile saves, measured on
-// the 32,768-row MLP shapes), else 128 x 128. The GAP epilogue writes one
-// energy partial per row and 128-column tile either way.
+// BF16 NT GEMM launch. Plain FP32 stores with at least 512 tiles of 256 x 128
+// (one round of two workgroups per CU) go to the kernel fed by direct LDS
+// loads (0.387 -> 0.359 ms on the 32,768 x 1,280 x 4,096 Qss product); with
+// the fused epilogues it measured slower than the register-staged kernels.
+// Otherwise 256 x 128 tiles when M is a multiple of 256 and there are at least
+// 1,024 of them (two rounds of two workgroups per CU; with fewer, the last
+// round's imbalance costs more than the wider tile saves, measured on the
+// 32,768-row MLP shapes), else 128 x 128. The GAP epilogue writes one energy
+// partial per row and 128-column tile either way.
 template <int EPI, class OutT, bool ZINT>
 static int launch_bf16(tgx_ctx* ctx, const char* name, int M, int N, int K, int batch, const __bf16* A, int lda,
                        const __bf16* B, int ldb, EpiArgs e) {
   TGX_REQUIRE(M % BM == 0 && N % BN == 0 && K % 64 == 0 && lda % 8 == 0 && ldb % 8 == 0,
               "gemm_bf16 %s: unpadded shape %d x %d x %d", name, M, N, K);
+  if constexpr (EPI == EPI_STORE) {
+    if (M % 256 == 0 && (long)(M / 256) * (N / BN) * batch >= 512) {
+      TGX_LAUNCH(ctx, name, (k_gemm_bf16_dma<EPI, OutT, ZINT, 256, 3>), dim3(N / BN, M / 256, batch), dim3(256),
+                 dma_lds_bytes(256, 3), K, A, lda, B, ldb, e);
+      return TGX_OK;
+    }
+  }
   if (M % 256 == 0 && (long)(M / 256) * (N / BN) * batch >= 1024)
     TGX_LAUNCH(ctx, name, (k_gemm_bf16_w<EPI, OutT, ZINT>), dim3(N / BN, M / 256, batch), dim3(256), 0, K, A, lda, B,
                ldb, e);

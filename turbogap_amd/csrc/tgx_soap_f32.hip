@@ -476,14 +476,18 @@ __global__ __launch_bounds__(64 * WAVES) void k_site_desc(SoapParams P, F32Args 
   }
   wave_sync();
   const float nrm = power(P, C, X);
-  float* qrow = q_out + (size_t)site * P.Dpad;
   __bf16* qbrow = qb_out + (size_t)site * P.Dpad;
-  for (int d = lane; d < P.Dpad; d += 64) {
-    const float v = d < P.D ? X[d] : 0.f;
-    qrow[d] = v;
-    qbrow[d] = (__bf16)v;  // the BF16 operand of the contraction / network input
+  if (q_out) {  // the FP32 copy and norm, for the adjoint kernel that reads them
+    float* qrow = q_out + (size_t)site * P.Dpad;
+    for (int d = lane; d < P.Dpad; d += 64) {
+      const float v = d < P.D ? X[d] : 0.f;
+      qrow[d] = v;
+      qbrow[d] = (__bf16)v;
+    }
+    if (lane == 0) pnorm_out[site] = nrm;
+  } else {
+    for (int d = lane; d < P.Dpad; d += 64) qbrow[d] = (__bf16)(d < P.D ? X[d] : 0.f);
   }
-  if (lane == 0) pnorm_out[site] = nrm;
 }
 
 // Sum over each 16-lane row, left in the row's last lane (lanes 15, 31, 47, 63):
@@ -924,11 +928,11 @@ __global__ __launch_bounds__(256, 4) void k_site_adjoint_mfma(SoapParams P, F32A
   float* part = meta + AM_WIN * AM_MF;      // [2 buffers][4 waves][16][4]
   int* lists = reinterpret_cast<int*>(part + 2 * 4 * 16 * 4);  // [2][AM_WIN]
   int* gcnt = lists + 2 * AM_WIN;           // [4]
-  int* rt_idx = gcnt + 4;                   // [AM_NRT] the raising table, loaded once
+  int* rt_idx = gcnt + 4;                   // [AM_NRT] the raising table (row offsets in CH), loaded once
   float* rt_coef = reinterpret_cast<float*>(rt_idx + AM_NRT);
   const int tx = w, te = 4 + (w >> 1), tpart = w & 1, tu = w;
   for (int t = threadIdx.x; t < AM_NRT; t += 256) {
-    rt_idx[t] = raise_idx[t];
+    rt_idx[t] = raise_idx[t] * AM_HP;
     rt_coef[t] = raise_coef[t];
   }
   const int site = blockIdx.x;
@@ -1105,7 +1109,7 @@ __global__ __launch_bounds__(256, 4) void k_site_adjoint_mfma(SoapParams P, F32A
 #pragma unroll
         for (int t = 0; t < AM_KR; ++t) {
           const int e = (lmr * 3 + c) * AM_KR + t;
-          v += rt_coef[e] * CH[rt_idx[e] * AM_HP + n];
+          v += rt_coef[e] * CH[rt_idx[e] + n];
         }
         hr[c][kk] = v;
       }
@@ -1345,6 +1349,12 @@ int tgx_radial_table_build(tgx_ctx* ctx, tgx_model_t* m) {
   return TGX_OK;
 }
 
+// The block's FP32 adjoint runs on the matrix-core kernel (l_max = 8 with the
+// raising table built).
+static bool adjoint_on_mfma(const tgx_model_t* m, const F32Args& a) {
+  return m->have_raise && m->P.L == 8 && m->P.NLM == AM_NLM && a.roff >= AM_NLM;
+}
+
 static F32Args f32_args(tgx_model_t* m) {
   F32Args a;
   a.site_beg = m->site_beg.as<int64_t>();
@@ -1385,8 +1395,12 @@ int tgx_desc_f32(tgx_ctx* ctx, tgx_model_t* m) {
   const size_t lds = sizeof(float) * (size_t)WAVES * desc_wave_floats(m->P);
   TGX_REQUIRE(lds <= 160 * 1024, "FP32 descriptor kernel needs %zu bytes of LDS", lds);
   TGX_TRY(m->qbf.ensure(sizeof(__bf16) * (size_t)tgx_round_up(ns, 128) * m->Dpad));
+  // the matrix-core adjoint forms the power spectrum from the coefficients
+  // again: the FP32 descriptors and norms are written only for the other one
+  const bool fp32_copy = !adjoint_on_mfma(m, a);
   TGX_LAUNCH(ctx, "soap_desc_f32", k_site_desc, dim3((ns + WAVES - 1) / WAVES), dim3(64 * WAVES), lds, m->P, a, ns,
-             m->q32.as<float>(), m->qbf.as<__bf16>(), m->pnorm.as<float>());
+             fp32_copy ? m->q32.as<float>() : nullptr, m->qbf.as<__bf16>(),
+             fp32_copy ? m->pnorm.as<float>() : nullptr);
   return TGX_OK;
 }
 
@@ -1399,7 +1413,7 @@ int tgx_adjoint_f32(tgx_ctx* ctx, tgx_model_t* m) {
   F32Args a = f32_args(m);
   TGX_REQUIRE(m->pbasis.p && m->pbasis.bytes >= sizeof(float) * (size_t)m->n_pairs_b * a.ldb,
               "FP32 adjoint: the pair basis of the block was not built");
-  if (m->have_raise && m->P.L == 8 && m->P.NLM == AM_NLM && a.roff >= AM_NLM) {
+  if (adjoint_on_mfma(m, a)) {
     // l_max = 8: the matrix-core kernel, one workgroup per site
     const size_t lds = sizeof(float) * (size_t)am_lds_floats(m->P, a.ldb);
     TGX_REQUIRE(lds <= 64 * 1024, "FP32 adjoint kernel needs %zu bytes of LDS", lds);
