@@ -10,8 +10,8 @@
 // accumulation; the epilogue keeps the energy row sums in FP64.
 // Workgroup tile 128x128, four wavefronts of 64x64 (4x4 MFMA tiles each).
 // Tiles are handed out XCD-aware: the eight XCDs each walk a contiguous run of
-// tiles, N fastest, so the A row-panel a wavefront group reuses stays in the
-// XCD's L2.
+// tiles in groups of eight tile rows, column by column inside a group, so that
+// the tiles one XCD has in flight share their A and B panels in its L2.
 #include "tgx_internal.h"
 
 namespace {
@@ -21,6 +21,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int BM = 128, BN = 128;
+constexpr int TILE_GM = 8;  // tile rows per group of the GEMM tile order (tile_of)
 
 enum { EPI_STORE = TGX_EPI_STORE_F32, EPI_GAP = 1, EPI_SILU = TGX_EPI_SILU, EPI_SCALE_AUX = TGX_EPI_SCALE_AUX,
        EPI_ACT2 = TGX_EPI_ACT2 };
@@ -44,6 +45,10 @@ struct EpiArgs {
   int64_t sW3, sEpart;   // per-blockIdx.z strides of w3 and epart
 };
 
+// GM > 1: inside its range an XCD walks groups of GM tile rows column by
+// column, so that the tiles it has in flight at once share their A and B
+// panels GM ways each (instead of one row of tiles sharing A only).
+template <int GM = 1>
 __device__ __forceinline__ void tile_of(int& tm, int& tn) {
   const int ntn = gridDim.x, ntm = gridDim.y;
   const int total = ntn * ntm;
@@ -53,8 +58,15 @@ __device__ __forceinline__ void tile_of(int& tm, int& tn) {
   const int base = total / 8, rem = total % 8;
   const int start = xcd * base + (xcd < rem ? xcd : rem);
   const int tile = start + local;
-  tm = tile / ntn;
-  tn = tile - tm * ntn;
+  if (GM == 1) {
+    tm = tile / ntn;
+    tn = tile - tm * ntn;
+  } else {
+    const int group = tile / (GM * ntn), first = group * GM;
+    const int gm = min(ntm - first, GM), in = tile - first * ntn;
+    tm = first + in % gm;
+    tn = in / gm;
+  }
 }
 
 template <class T, bool ZINT>
@@ -205,7 +217,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(int K, const double* __rest
   __shared__ __align__(16) double Bs[2][BK * LS];
   __shared__ double sE[2 * BM];
   int tm, tn;
-  tile_of(tm, tn);
+  tile_of<TILE_GM>(tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
@@ -276,7 +288,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int K, const __bf16* __restri
   __shared__ __align__(16) __bf16 Bs[2][BN * LDS_STRIDE];
   __shared__ double sE[2 * BM];
   int tm, tn;
-  tile_of(tm, tn);
+  tile_of<TILE_GM>(tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
@@ -367,7 +379,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16_w(int K, const __bf16* __r
   __shared__ __align__(16) __bf16 Bs[2][BN * LS];
   __shared__ double sE[2 * TM];
   int tm, tn;
-  tile_of(tm, tn);
+  tile_of<TILE_GM>(tm, tn);
   const int m0 = tm * TM, n0 = tn * BN;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
@@ -441,16 +453,20 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16_w(int K, const __bf16* __r
 
 // BF16 NT GEMM fed by direct global-to-LDS loads (global_load_lds_dwordx4):
 // workgroup tile TM x 128, four wavefronts of (TM / 2) x 64, BK = 32, an
-// NS-stage ring of LDS buffers. A stage is TM + 128 rows of 64 bytes (the A
+// NS-slot ring of LDS stages. A stage is TM + 128 rows of 64 bytes (the A
 // rows, then the B rows); the four 16-byte chunks of row r sit at chunk
 // c ^ ((r >> 1) & 3), so that the eight rows a 16-byte fragment read serves
 // per cycle cover all 32 banks. One load instruction fills 16 rows (lane l
-// writes chunk l & 3 of row l >> 2 and fetches the chunk that belongs there).
-// Stage s + NS - 1 is issued right after the barrier that opens stage s, into
-// the buffer stage s - 1 used, so NS - 1 stages are in flight while the MFMAs
-// of stage s run; nothing is staged through registers. Each wavefront waits
-// for its own loads of stage s (vmcnt), the barrier then covers every
-// wavefront's. M a multiple of TM, N of 128, K of 32.
+// writes chunk l & 3 of row l >> 2 and fetches the chunk that belongs there);
+// nothing is staged through registers. The fragments of each stage are held
+// in registers, double-buffered: stage s + 1's fragments are read while stage
+// s's MFMAs run, so a ring slot is free as soon as every wavefront has read it
+// and stage s + NS is loaded into stage s's slot right after the barrier of
+// iteration s (NS stages in flight). Iteration s: the MFMAs of the first half
+// of the tile rows, then (vmcnt: this wavefront's part of stage s + 1 landed)
+// the barrier, the loads of stage s + NS and the fragment reads of stage
+// s + 1, then the second half of the MFMAs, which cover the reads' latency.
+// M a multiple of TM, N of 128, K of 32.
 constexpr int DK = 32;                      // k per stage
 constexpr int DROW = DK * 2;                // bytes per staged row
 __host__ __device__ constexpr int dma_stage_bytes(int tm) { return (tm + BN) * DROW; }
@@ -471,19 +487,53 @@ __device__ __forceinline__ void dma_issue(const __bf16* const (&src)[P], int k0,
     __builtin_amdgcn_global_load_lds(const_cast<__bf16*>(src[j] + k0), stage + (w + 4 * j) * 1024, 16, 0, 0);
 }
 
-template <int EPI, class OutT, bool ZINT, int TM, int NS>
-__global__ __launch_bounds__(256, 2) void k_gemm_bf16_dma(int K, const __bf16* __restrict__ A, int lda,
+template <int TI>
+__device__ __forceinline__ void pf_read(const unsigned char* sb, int aoff, int boff, bf16x8 (&a)[TI], bf16x8 (&b)[4]) {
+#pragma unroll
+  for (int t = 0; t < 4; ++t) b[t] = *reinterpret_cast<const bf16x8*>(sb + boff + t * 16 * DROW);
+#pragma unroll
+  for (int i = 0; i < TI; ++i) a[i] = *reinterpret_cast<const bf16x8*>(sb + aoff + i * 16 * DROW);
+}
+
+template <int TI, int I0, int I1>
+__device__ __forceinline__ void pf_mfma(f32x4 (&acc)[TI][4], const bf16x8 (&a)[TI], const bf16x8 (&b)[4]) {
+#pragma unroll
+  for (int i = I0; i < I1; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+}
+
+template <int TI, int P, int NS, int SB>
+__device__ __forceinline__ void pf_stage(f32x4 (&acc)[TI][4], const bf16x8 (&a)[TI], const bf16x8 (&b)[4],
+                                         bf16x8 (&an)[TI], bf16x8 (&bn)[4], const __bf16* const (&src)[P],
+                                         unsigned char* dsm, int st, int n_stages, int& slot, int aoff, int boff,
+                                         int w) {
+  pf_mfma<TI, 0, TI / 2>(acc, a, b);
+  __builtin_amdgcn_sched_barrier(0);
+  wait_vmcnt<(NS - 2) * P>();
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_sched_barrier(0);
+  // every wavefront has read this stage's slot: it takes stage st + NS (an in-bounds repeat past the end)
+  dma_issue<P>(src, min(st + NS, n_stages - 1) * DK, dsm + slot * SB, w);
+  slot = slot + 1 == NS ? 0 : slot + 1;
+  pf_read<TI>(dsm + slot * SB, aoff, boff, an, bn);
+  __builtin_amdgcn_sched_barrier(0);
+  pf_mfma<TI, TI / 2, TI>(acc, a, b);
+}
+
+template <int EPI, class OutT, bool ZINT, int TM, int NS, int OCC>
+__global__ __launch_bounds__(256, OCC) void k_gemm_bf16_pf(int K, const __bf16* __restrict__ A, int lda,
                                                           const __bf16* __restrict__ B, int ldb, EpiArgs e) {
-  constexpr int TI = TM / 32;                 // 16-row MFMA tiles per wavefront along M
-  constexpr int NBA = TM / 16, NBB = BN / 16; // 16-row load blocks per stage
-  constexpr int P = (NBA + NBB) / 4;          // load instructions per wavefront and stage
+  constexpr int TI = TM / 32;
+  constexpr int NBA = TM / 16, NBB = BN / 16;
+  constexpr int P = (NBA + NBB) / 4;
   static_assert((NBA + NBB) % 4 == 0, "load blocks per wavefront");
-  static_assert((NS - 2) * P < 16, "vmcnt range");
+  static_assert(NS >= 3 && (NS - 1) * P < 16, "vmcnt range");
   constexpr int SB = dma_stage_bytes(TM);
   extern __shared__ __align__(16) unsigned char dsm[];
   double* sE = reinterpret_cast<double*>(dsm + NS * SB);
   int tm, tn;
-  tile_of(tm, tn);
+  tile_of<TILE_GM>(tm, tn);
   const int m0 = tm * TM, n0 = tn * BN;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
@@ -498,8 +548,6 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16_dma(int K, const __bf16* _
     if (e.epart) e.epart += z * e.sEpart;
     if (e.zscale) e.pref = e.zscale[z];
   }
-  // this lane's source in each of its P load blocks: row 16 b + lane / 4, the
-  // logical chunk stored at physical chunk lane & 3
   const int lrow = lane >> 2, lchunk = (lane & 3) ^ ((lane >> 3) & 3);
   const __bf16* src[P];
 #pragma unroll
@@ -508,7 +556,6 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16_dma(int K, const __bf16* _
     src[j] = b < NBA ? A + (size_t)(m0 + 16 * b + lrow) * lda + 8 * lchunk
                      : B + (size_t)(n0 + 16 * (b - NBA) + lrow) * ldb + 8 * lchunk;
   }
-  // fragment reads: row li of a 16-row tile, logical chunk lg (k 8 lg .. 8 lg + 7)
   const int li = lane & 15, lg = lane >> 4;
   const int foff = li * DROW + ((lg ^ ((li >> 1) & 3)) << 4);
   const int aoff = wm * (TM / 2) * DROW + foff, boff = TM * DROW + wn * 64 * DROW + foff;
@@ -519,33 +566,19 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16_dma(int K, const __bf16* _
     for (int j = 0; j < 4; ++j) acc[i][j] = (f32x4){0, 0, 0, 0};
   const int n_stages = K / DK;
 #pragma unroll
-  for (int s = 0; s < NS - 1; ++s) dma_issue<P>(src, min(s, n_stages - 1) * DK, dsm + s * SB, w);
-  int rs = 0, ws = NS - 1;  // ring slots read and written this stage
-  for (int st = 0; st < n_stages; ++st) {
-    __builtin_amdgcn_sched_barrier(0);
-    if (st + NS - 2 < n_stages)
-      wait_vmcnt<(NS - 2) * P>();
-    else
-      wait_vmcnt<0>();
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_sched_barrier(0);
-    // the slot written here was read in stage st - 1, which every wavefront has finished
-    dma_issue<P>(src, min(st + NS - 1, n_stages - 1) * DK, dsm + ws * SB, w);
-    __builtin_amdgcn_sched_barrier(0);
-    const unsigned char* sb = dsm + rs * SB;
-    bf16x8 b[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) b[t] = *reinterpret_cast<const bf16x8*>(sb + boff + t * 16 * DROW);
-#pragma unroll
-    for (int i = 0; i < TI; ++i) {
-      const bf16x8 a = *reinterpret_cast<const bf16x8*>(sb + aoff + i * 16 * DROW);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[i][j], 0, 0, 0);
-    }
-    rs = rs + 1 == NS ? 0 : rs + 1;
-    ws = ws + 1 == NS ? 0 : ws + 1;
+  for (int s = 0; s < NS; ++s) dma_issue<P>(src, min(s, n_stages - 1) * DK, dsm + s * SB, w);
+  wait_vmcnt<(NS - 1) * P>();
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_sched_barrier(0);
+  bf16x8 a0[TI], b0[4], a1[TI], b1[4];
+  pf_read<TI>(dsm, aoff, boff, a0, b0);
+  int slot = 0, st = 0;
+  for (; st + 1 < n_stages; st += 2) {
+    pf_stage<TI, P, NS, SB>(acc, a0, b0, a1, b1, src, dsm, st, n_stages, slot, aoff, boff, w);
+    pf_stage<TI, P, NS, SB>(acc, a1, b1, a0, b0, src, dsm, st + 1, n_stages, slot, aoff, boff, w);
   }
-  // the loads issued past the last stage (in-bounds repeats) land before the workgroup's LDS is released
+  if (st < n_stages) pf_stage<TI, P, NS, SB>(acc, a0, b0, a1, b1, src, dsm, st, n_stages, slot, aoff, boff, w);
+  // the loads issued past the last stage land before the workgroup's LDS is released
   wait_vmcnt<0>();
   __builtin_amdgcn_sched_barrier(0);
   epilogue<EPI, OutT, float, f32x4, false, ZINT, TI, 4, TM / 2, 64, TM>(acc, m0, n0, tn, wm, wn, e, sE);
@@ -581,8 +614,9 @@ static EpiArgs gap_epi(const tgx_model_t* m) {
 
 // BF16 NT GEMM launch. Plain FP32 stores with at least 512 tiles of 256 x 128
 // (one round of two workgroups per CU) go to the kernel fed by direct LDS
-// loads (0.387 -> 0.359 ms on the 32,768 x 1,280 x 4,096 Qss product); with
-// the fused epilogues it measured slower than the register-staged kernels.
+// loads (0.389 -> 0.345 ms on the 32,768 x 1,280 x 4,096 Qss product, with the
+// grouped tile order); with the fused epilogues the 256-row wave tiles measured
+// slower than the register-staged kernels.
 // Otherwise 256 x 128 tiles when M is a multiple of 256 and there are at least
 // 1,024 of them (two rounds of two workgroups per CU; with fewer, the last
 // round's imbalance costs more than the wider tile saves, measured on the
@@ -595,7 +629,7 @@ static int launch_bf16(tgx_ctx* ctx, const char* name, int M, int N, int K, int 
               "gemm_bf16 %s: unpadded shape %d x %d x %d", name, M, N, K);
   if constexpr (EPI == EPI_STORE) {
     if (M % 256 == 0 && (long)(M / 256) * (N / BN) * batch >= 512) {
-      TGX_LAUNCH(ctx, name, (k_gemm_bf16_dma<EPI, OutT, ZINT, 256, 3>), dim3(N / BN, M / 256, batch), dim3(256),
+      TGX_LAUNCH(ctx, name, (k_gemm_bf16_pf<EPI, OutT, ZINT, 256, 3, 2>), dim3(N / BN, M / 256, batch), dim3(256),
                  dma_lds_bytes(256, 3), K, A, lda, B, ldb, e);
       return TGX_OK;
     }
