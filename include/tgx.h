@@ -439,9 +439,10 @@ int tgx_md_advance(tgx_ctx* ctx, double dt, int first_step, int remove_cm, doubl
  * on, the surrogate path's blocks run one after the other. */
 int tgx_set_profiling(tgx_ctx* ctx, int on);
 /* Execution option with no counterpart in the reference: tgx_compute runs blocks
- * of distinct central species concurrently, each on a stream of its own (on by
- * default); 0 runs them in turn on the context's stream. Results are the same
- * bits either way. */
+ * of distinct central species concurrently, each on a stream of its own.
+ * 1 (the default): on the surrogate and FP64 paths, the mixed path's blocks in
+ * turn (there the overlap measured slower); 2: on every path; 0: every path in
+ * turn on the context's stream. Results are the same bits either way. */
 int tgx_set_block_streams(tgx_ctx* ctx, int on);
 /* Launch count and total milliseconds of kernels whose name contains `name`. */
 int tgx_kernel_time(tgx_ctx* ctx, const char* name, int* count, double* total_ms);

@@ -244,9 +244,9 @@ def test_surrogate_block_streams_match_serial(ctx):
         assert E1 == E2 and np.array_equal(W1, W2) and np.array_equal(e1, e2) and np.array_equal(F1, F2)
         assert g1 == g2
     assert 0.0 < g1[0] < 1.0  # the second threshold gates part of the sites
-    for path in ("mixed", "fp64_adjoint", "fp64_plain"):  # the other paths
+    for path in ("mixed", "fp64_adjoint", "fp64_plain"):  # the other paths (mixed: forced)
         out = []
-        for on in (True, False):
+        for on in (2, 0):
             ctx.set_block_streams(on)
             E, W = ctx.compute(path)
             out.append((E, W) + ctx.results())

@@ -115,7 +115,8 @@ def test_block_streams_above_two_launch_scan_limit(ctx):
     """81,920 atoms (two central species): the block selection scans more values
     than the two-launch scan takes, so each block falls back to the three-launch
     scan with its own block totals. Running the blocks on streams of their own
-    gives bit-identical energies, forces and virial to the blocks in turn."""
+    (forced on the mixed path, which runs them in turn by default) gives
+    bit-identical energies, forces and virial to the blocks in turn."""
     _, models = H.subset(500, 1024)
     sysm = S.binary_system(81920, seed=2003)
     ctx.set_system(sysm.positions, sysm.species, sysm.box)
@@ -123,7 +124,7 @@ def test_block_streams_above_two_launch_scan_limit(ctx):
     H.load_models(ctx, models)
     for path in ("mixed", "fp64_adjoint"):
         out = []
-        for on in (True, False, True):
+        for on in (2, 0, 2):
             ctx.set_block_streams(on)
             E, W = ctx.compute(path)
             out.append((E, W) + ctx.results())

@@ -344,9 +344,11 @@ class Context:
     def set_profiling(self, on: bool):
         L.check(self._lib.tgx_set_profiling(self.h, int(on)))
 
-    def set_block_streams(self, on: bool):
-        """Run the surrogate path's blocks of distinct central species concurrently (default)."""
-        L.check(self._lib.tgx_set_block_streams(self.h, int(on)), "set_block_streams")
+    def set_block_streams(self, mode):
+        """Blocks of distinct central species on streams of their own: True / 1 (the
+        default) on the surrogate and FP64 paths with the mixed path's blocks in
+        turn, 2 on every path, False / 0 every path in turn."""
+        L.check(self._lib.tgx_set_block_streams(self.h, int(mode)), "set_block_streams")
 
     def reset_profile(self):
         L.check(self._lib.tgx_reset_profile(self.h))

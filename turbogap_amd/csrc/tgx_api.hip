@@ -44,7 +44,8 @@ extern "C" int tgx_set_profiling(tgx_ctx* ctx, int on) {
 
 extern "C" int tgx_set_block_streams(tgx_ctx* ctx, int on) {
   TGX_CTX_CHECK(ctx);
-  ctx->block_streams = on != 0;
+  TGX_REQUIRE(on >= 0 && on <= 2, "tgx_set_block_streams: mode %d (0 off, 1 default, 2 every path)", on);
+  ctx->block_streams = on;
   return TGX_OK;
 }
 
@@ -974,8 +975,13 @@ extern "C" int tgx_compute(tgx_ctx* ctx, int path, double* energy, double* viria
   // then runs on a stream of its own, so that one block's kernels fill the gaps
   // and tails of the other's. Blocks that share a central species (or take
   // every site) run in turn on the context stream, and so do all blocks while
-  // kernels are being timed (each kernel's events then time it alone).
-  bool concurrent = nm > 1 && ctx->block_streams && !ctx->profiling;
+  // kernels are being timed (each kernel's events then time it alone). The
+  // mixed path runs its blocks in turn too: its BF16 GEMMs and the FP32
+  // adjoint slow each other down more than the overlap gains (65,536 atoms:
+  // 3.86 ms per step in turn against 4.14 ms concurrent; the surrogate and
+  // FP64 adjoint paths gain 1-2% from the overlap).
+  bool concurrent = nm > 1 && ctx->block_streams > 0 && !ctx->profiling &&
+                    (path != TGX_PATH_MIXED || ctx->block_streams == 2);
   for (int a = 0; a < nm && concurrent; ++a) {
     const int ca = ctx->models[a]->central_species;
     if (ca == 0) concurrent = false;

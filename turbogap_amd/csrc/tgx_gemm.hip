@@ -615,8 +615,10 @@ static EpiArgs gap_epi(const tgx_model_t* m) {
 // BF16 NT GEMM launch. Plain FP32 stores with at least 512 tiles of 256 x 128
 // (one round of two workgroups per CU) go to the kernel fed by direct LDS
 // loads (0.389 -> 0.345 ms on the 32,768 x 1,280 x 4,096 Qss product, with the
-// grouped tile order); with the fused epilogues the 256-row wave tiles measured
-// slower than the register-staged kernels.
+// grouped tile order); the GAP and second-layer epilogues to its 128 x 128
+// form with three workgroups per CU (GAP kernel 0.495 -> 0.476 ms, second
+// layer 0.052 -> 0.047 ms). The SiLU and scaled epilogues measured faster on
+// the register-staged kernels.
 // Otherwise 256 x 128 tiles when M is a multiple of 256 and there are at least
 // 1,024 of them (two rounds of two workgroups per CU; with fewer, the last
 // round's imbalance costs more than the wider tile saves, measured on the
@@ -631,6 +633,13 @@ static int launch_bf16(tgx_ctx* ctx, const char* name, int M, int N, int K, int 
     if (M % 256 == 0 && (long)(M / 256) * (N / BN) * batch >= 512) {
       TGX_LAUNCH(ctx, name, (k_gemm_bf16_pf<EPI, OutT, ZINT, 256, 3, 2>), dim3(N / BN, M / 256, batch), dim3(256),
                  dma_lds_bytes(256, 3), K, A, lda, B, ldb, e);
+      return TGX_OK;
+    }
+  }
+  if constexpr (EPI == EPI_GAP || EPI == EPI_ACT2) {
+    if (K % DK == 0) {
+      TGX_LAUNCH(ctx, name, (k_gemm_bf16_pf<EPI, OutT, ZINT, 128, 3, 3>), dim3(N / BN, M / BM, batch), dim3(256),
+                 dma_lds_bytes(128, 3), K, A, lda, B, ldb, e);
       return TGX_OK;
     }
   }

@@ -242,7 +242,7 @@ struct tgx_ctx {
   int gate_host_n = 0;
   dbuf gate_mask;
   hipEvent_t ev_blocks = nullptr;  // recorded on the context stream before concurrent blocks start
-  bool block_streams = true;       // surrogate blocks of distinct central species run concurrently
+  int block_streams = 1;           // blocks of distinct central species on streams of their own: 0 off, 1 default, 2 every path
   // profiling
   bool profiling = false;
   std::vector<prof_rec> prof;
