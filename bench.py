@@ -79,7 +79,8 @@ def roofline_entries(stats, n_atoms, blocks, n_pairs, members=1, hidden=256):
       soap_site_der_f64          3 D 8 B/pair (dq/dr) + D 8 B/site + 40 B/pair
       plain_force                3 D 8 B/pair + D 8 B/site + 28 B/pair + 24 B/site
       soap_pair_basis            36 B/pair read + 4 ldb B/pair written
-      soap_desc_f32              4 ldb B/pair + 4 Dpad B/site
+      soap_desc_f32              4 ldb B/pair + 2 Dpad B/site (the BF16 copy; the FP32 one is
+                                 written only for the VALU adjoint)
       soap_adjoint_f32           4 ldb + 36 + 48 B/pair + 4 Dpad B/site (rows read once, q not
                                  read, FP64 force entry read and written, FP32 Qss)
       gemm_*                     2 n S D flops per GEMM
@@ -113,7 +114,7 @@ def roofline_entries(stats, n_atoms, blocks, n_pairs, members=1, hidden=256):
     hbm("soap_site_der_f64", np.mean([p * 3 * D * 8 + n * D * 8 + 40 * p for n, p in zip(Nb, Pb)]))
     hbm("plain_force", np.mean([p * 3 * D * 8 + n * D * 8 + 28 * p + 24 * n for n, p in zip(Nb, Pb)]))
     hbm("soap_pair_basis", np.mean([p * (36 + 4 * ldb) for p in Pb]))
-    hbm("soap_desc_f32", np.mean([p * 4 * ldb + n * 4 * Dpad for n, p in zip(Nb, Pb)]))
+    hbm("soap_desc_f32", np.mean([p * 4 * ldb + n * 2 * Dpad for n, p in zip(Nb, Pb)]))
     hbm("soap_adjoint_f32", np.mean([p * (4 * ldb + 36 + 48) + n * 4 * Dpad for n, p in zip(Nb, Pb)]))
     for name, peak in (("gemm_kernel_f64", "fp64_TF"), ("gemm_qss_f64", "fp64_TF"), ("gemm_kernel_bf16", "bf16_TF"),
                        ("gemm_qss_bf16", "bf16_TF")):
@@ -126,10 +127,16 @@ def roofline_entries(stats, n_atoms, blocks, n_pairs, members=1, hidden=256):
 
 
 # bench kernel label -> device kernel name in the rocprofv3 counter summary
-KERNEL_SYMBOL = {"soap_adjoint_f32": "k_site_adjoint_mfma", "soap_desc_f32": "k_site_desc", "soap_pair_basis": "k_pair_basis",
-                 "nbr_fill": "k_cell_neigh<1>", "soap_site_der_f64": "k_soap_der_l<8, 5>",
-                 "plain_force": "k_plain_force", "gemm_kernel_f64": "k_gemm_f64<1, true>",
-                 "gemm_qss_f64": "k_gemm_f64<0, true>"}
+# (the Qss product and the network's first backward layer share one kernel
+# symbol, so the summary's average over both is not either one's traffic)
+KERNEL_SYMBOL = {"soap_adjoint_f32": "k_site_adjoint_mfma", "soap_desc_f32": "k_site_desc",
+                 "soap_pair_basis": "k_pair_basis8", "nbr_fill": "k_cell_neigh<1>",
+                 "soap_site_der_f64": "k_soap_der_l<8, 5>", "plain_force": "k_plain_force",
+                 "gemm_kernel_f64": "k_gemm_f64<1, true>", "gemm_qss_f64": "k_gemm_f64<0, true>",
+                 "gemm_kernel_bf16": "k_gemm_bf16_pf<1, bool _Accum, bool, E, 128, 3, 3>",
+                 "mlp_l1": "k_gemm_bf16<2, bool _Accum, bool, E>",
+                 "mlp_l2": "k_gemm_bf16_pf<4, bool _Accum, bool, E, 128, 3, 3>",
+                 "mlp_back2": "k_gemm_bf16<3, bool _Accum, bool, E>"}
 
 
 def measured_traffic(name):
