@@ -162,7 +162,7 @@ extern "C" int tgx_finalize(tgx_ctx* ctx) {
                   &ctx->thetas, &ctx->phis, &ctx->nspec, &ctx->rev, &ctx->cell_of, &ctx->cell_cnt, &ctx->cell_start,
                   &ctx->cell_atoms, &ctx->scan_tmp, &ctx->do_list, &ctx->energies, &ctx->forces, &ctx->fglob, &ctx->fres,
                   &ctx->virial_part, &ctx->scratch, &ctx->red, &ctx->vel, &ctx->mass, &ctx->forces_prev,
-                  &ctx->pos_prev, &ctx->gate_mask, &ctx->cell_pos, &ctx->md_disp, &ctx->md_flag,
+                  &ctx->pos_prev, &ctx->gate_mask, &ctx->cell_pos, &ctx->nbr_hits, &ctx->md_disp, &ctx->md_flag,
                   &ctx->md_fix, &ctx->md_gd0, &ctx->md_cm};
   for (dbuf* b : bufs) b->release();
   if (ctx->gate_host) (void)hipHostFree(ctx->gate_host);

@@ -207,6 +207,7 @@ struct tgx_ctx {
   bool have_list = false;
   bool have_angles = false;  // thetas / phis of the resident list computed
   dbuf cell_pos;             // positions in cell order (cell-list build)
+  dbuf nbr_hits;             // cell slots of each atom's first hits (count pass, read by the fill)
   // models
   std::vector<tgx_model_t*> models;
   int current = -1;

@@ -187,6 +187,12 @@ constexpr int NBW = 4;  // wavefronts (central atoms) per workgroup
 // position from the cell-ordered arrays. Hits are compacted with a ballot, so
 // the output order is the reference's. mode 0 counts, mode 1 fills (index,
 // distance, displacement, species; the angles come from k_angles on demand).
+// The count also keeps the cell slots of an atom's first NB_HITS hits, in that
+// order, in hit_slots[i][.]: the fill of an atom with at most NB_HITS
+// neighbours then rewrites those hits (the same distance arithmetic on the same
+// operands, so the same bits) instead of visiting its 27 cells again.
+constexpr int NB_HITS = 64;
+
 template <int MODE>
 __global__ __launch_bounds__(64 * NBW) void k_cell_neigh(int n, const double* __restrict__ pos, BoxInfo bx, int mx,
                                                        int my, int mz, double rcut, const int* __restrict__ do_list,
@@ -198,7 +204,7 @@ __global__ __launch_bounds__(64 * NBW) void k_cell_neigh(int n, const double* __
                                                        const int* __restrict__ species, int* __restrict__ n_neigh,
                                                        const int64_t* __restrict__ off, int* __restrict__ nlist,
                                                        double* __restrict__ rjs, double* __restrict__ xyz,
-                                                       int* __restrict__ nspec) {
+                                                       int* __restrict__ nspec, int* __restrict__ hit_slots) {
   __shared__ int s_beg[NBW][29];  // running counts of the (up to 27) visited cells, then their number
   __shared__ int64_t s_st[NBW][27];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -209,6 +215,34 @@ __global__ __launch_bounds__(64 * NBW) void k_cell_neigh(int n, const double* __
   if (do_list && !do_list[i]) {
     if (MODE == 0 && lane == 0) n_neigh[i] = 0;
     return;
+  }
+  if (MODE == 1) {
+    const int nh = n_neigh[i] - 1;
+    if (nh <= NB_HITS) {
+      // the count's hits, in its order (lane h writes list entry off[i] + 1 + h)
+      const int64_t o = off[i];
+      if (lane == 0) {
+        nlist[o] = i + 1;
+        rjs[o] = 0.0;
+        xyz[3 * o] = 0.0; xyz[3 * o + 1] = 0.0; xyz[3 * o + 2] = 0.0;
+        nspec[o] = species[i];
+      }
+      if (lane < nh) {
+        const int slot = hit_slots[(size_t)i * NB_HITS + lane];
+        const int k = cell_atoms[slot];
+        const double pi[3] = {pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2]};
+        const double pk[3] = {cell_pos[3 * (size_t)slot], cell_pos[3 * (size_t)slot + 1],
+                              cell_pos[3 * (size_t)slot + 2]};
+        double dist[3], d;
+        get_distance_ortho(pi, pk, bx, dist, d);
+        const int64_t p = o + 1 + lane;
+        nlist[p] = k + 1;
+        rjs[p] = d;
+        xyz[3 * p] = dist[0]; xyz[3 * p + 1] = dist[1]; xyz[3 * p + 2] = dist[2];
+        nspec[p] = species[k];
+      }
+      return;
+    }
   }
   int* vbeg = s_beg[w];
   int64_t* vst = s_st[w];
@@ -275,6 +309,7 @@ __global__ __launch_bounds__(64 * NBW) void k_cell_neigh(int n, const double* __
     const int cidx = base + lane;
     bool hit = false;
     int k = -1;
+    int64_t slot = 0;
     double dist[3] = {0, 0, 0}, d = 0.0;
     if (cidx < total) {
       int lo = 0, hi = nv;  // vbeg[lo] <= cidx < vbeg[hi]
@@ -283,7 +318,7 @@ __global__ __launch_bounds__(64 * NBW) void k_cell_neigh(int n, const double* __
         if (vbeg[mid] <= cidx) lo = mid;
         else hi = mid;
       }
-      const int64_t slot = vst[lo] + (cidx - vbeg[lo]);
+      slot = vst[lo] + (cidx - vbeg[lo]);
       k = cell_atoms[slot];
       if (k != i) {
         const double pk[3] = {cell_pos[3 * slot], cell_pos[3 * slot + 1], cell_pos[3 * slot + 2]};
@@ -292,6 +327,10 @@ __global__ __launch_bounds__(64 * NBW) void k_cell_neigh(int n, const double* __
       }
     }
     const uint64_t mask = __ballot(hit);
+    if (MODE == 0 && hit) {
+      const int h = hits + __popcll(mask & ((1ull << lane) - 1ull));
+      if (h < NB_HITS) hit_slots[(size_t)i * NB_HITS + h] = (int)slot;
+    }
     if (MODE == 1 && hit) {
       const int64_t p = out + hits + __popcll(mask & ((1ull << lane) - 1ull));
       nlist[p] = k + 1;
@@ -377,16 +416,21 @@ __global__ __launch_bounds__(256) void k_brute_neigh(int n, int n_sc, const doub
 // For pair p = (i -> k), the index of the pair in j = fold(k)'s list that points
 // back at i with the opposite displacement; the self pair maps to itself. -1
 // when j has no list. Used by the deterministic force gather.
+// REV_LANES lanes per central atom (64 / REV_LANES atoms per wavefront): lists
+// average about 22 entries, so a whole wavefront per atom would leave most
+// lanes idle.
+constexpr int REV_LANES = 32;
+
 __global__ void k_reverse(int n, const int* __restrict__ n_neigh, const int64_t* __restrict__ off,
                           const int* __restrict__ nlist, const double* __restrict__ xyz, const int* __restrict__ order,
                           int64_t* __restrict__ rev) {
-  const int lane = threadIdx.x & 63;
-  const int w = xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63, sub = lane % REV_LANES;
+  const int w = (xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / REV_LANES) + lane / REV_LANES;
   if (w >= n) return;
   const int i = order ? order[w] : w;  // cell order when the cell list was built
   const int64_t b = off[i];
   const int nn = n_neigh[i];
-  for (int t = lane; t < nn; t += 64) {
+  for (int t = sub; t < nn; t += REV_LANES) {
     const int64_t p = b + t;
     if (t == 0) { rev[p] = p; continue; }
     int j = nlist[p] - 1;
@@ -534,10 +578,11 @@ int tgx_neighbors_run(tgx_ctx* ctx, double rcut_max, const int* indices, const i
     TGX_TRY(ctx->cell_pos.ensure(sizeof(double) * 3 * (size_t)n));
     TGX_LAUNCH(ctx, "nbr_cell_pos", k_cell_pos, dim3((n + 255) / 256), dim3(256), 0, n, ctx->cell_atoms.as<int>(),
                ctx->pos.as<double>(), ctx->cell_pos.as<double>());
+    TGX_TRY(ctx->nbr_hits.ensure(sizeof(int) * (size_t)n * NB_HITS));
     TGX_LAUNCH(ctx, "nbr_count", k_cell_neigh<0>, wgrid, wblk, 0, n, ctx->pos.as<double>(), bx, mx, my, mz, rcut_max,
                d_do, cell_xyz, ctx->cell_start.as<int64_t>(), cnt, ctx->cell_atoms.as<int>(),
                ctx->cell_pos.as<double>(), ctx->species_sc.as<int>(), ctx->n_neigh.as<int>(), nullptr, nullptr,
-               nullptr, nullptr, nullptr);
+               nullptr, nullptr, nullptr, ctx->nbr_hits.as<int>());
   } else {
     TGX_LAUNCH(ctx, "nbr_count_brute", k_brute_neigh<0>, wgrid, wblk, 0, n, n_sc, ctx->pos.as<double>(), bx, rcut_max,
                d_do, ctx->species_sc.as<int>(), ctx->n_neigh.as<int>(), nullptr, nullptr, nullptr, nullptr, nullptr);
@@ -559,13 +604,15 @@ int tgx_neighbors_run(tgx_ctx* ctx, double rcut_max, const int* indices, const i
                d_do, cell_xyz, ctx->cell_start.as<int64_t>(), ctx->cell_cnt.as<int>(), ctx->cell_atoms.as<int>(),
                ctx->cell_pos.as<double>(), ctx->species_sc.as<int>(), ctx->n_neigh.as<int>(),
                ctx->nl_off.as<int64_t>(), ctx->nlist.as<int>(), ctx->rjs.as<double>(), ctx->xyz.as<double>(),
-               ctx->nspec.as<int>());
+               ctx->nspec.as<int>(), ctx->nbr_hits.as<int>());
   } else {
     TGX_LAUNCH(ctx, "nbr_fill_brute", k_brute_neigh<1>, wgrid, wblk, 0, n, n_sc, ctx->pos.as<double>(), bx, rcut_max,
                d_do, ctx->species_sc.as<int>(), ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(),
                ctx->nlist.as<int>(), ctx->rjs.as<double>(), ctx->xyz.as<double>(), ctx->nspec.as<int>());
   }
-  TGX_LAUNCH(ctx, "nbr_reverse", k_reverse, wgrid, wblk, 0, n, ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(),
+  const int rev_waves = (n + 64 / REV_LANES - 1) / (64 / REV_LANES);
+  TGX_LAUNCH(ctx, "nbr_reverse", k_reverse, dim3((rev_waves + NBW - 1) / NBW), wblk, 0, n, ctx->n_neigh.as<int>(),
+             ctx->nl_off.as<int64_t>(),
              ctx->nlist.as<int>(), ctx->xyz.as<double>(), cells ? ctx->cell_atoms.as<int>() : (const int*)nullptr,
              ctx->rev.as<int64_t>());
   ctx->rcut_max = rcut_max;
