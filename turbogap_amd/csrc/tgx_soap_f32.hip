@@ -445,6 +445,7 @@ __device__ __forceinline__ void site_coefficients(const SoapParams& P, const F32
 }
 
 __host__ __device__ inline int desc_wave_floats(const SoapParams& P) { return P.N * P.NLM + P.D; }
+constexpr int DESC_NT = 8;  // lm tiles of 16 of k_site_desc (NLM <= 128)
 constexpr int APASS = 16;  // pair rows staged per pass of the adjoint
 // per wavefront: max(X [D], APASS rows) then the pass's pair forces [APASS][3]
 __host__ __device__ inline int adj_rows_floats(const SoapParams& P, int ldb) {
@@ -464,15 +465,46 @@ __global__ __launch_bounds__(64 * WAVES) void k_site_desc(SoapParams P, F32Args 
   float* X = C + P.N * P.NLM;
   const int64_t b0 = a.site_beg[site];
   const int nn = (int)(a.site_beg[site + 1] - b0);
-  float acc[2][NMAXF];
-  site_coefficients<false>(P, a, b0, nn, lane, acc);
+  // c(n, lm) = cen + sum_j R_n(j) W_lm(j) on the matrix cores, K over the
+  // neighbour rows (4 per step): lane (n = lane % 16, j = k0 + lane / 16) holds
+  // R_n(j) and, per lm tile t, W_(16 t + lane % 16)(j), both read straight from
+  // the rows; accumulator t holds c(4 (lane / 16) + r, 16 t + lane % 16)
+  const int li = lane & 15, lg = lane >> 4, NLM = P.NLM;
+  f32x4_t cacc[DESC_NT];
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int lm = lane + 64 * h;
-    if (lm < P.NLM)
+  for (int t = 0; t < DESC_NT; ++t) cacc[t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  const int np1 = nn - 1;
+  const float* rows = a.basis + (b0 + 1) * a.ldb;
+#pragma unroll 2
+  for (int k0 = 0; k0 < np1; k0 += 4) {
+    const int j = k0 + lg;
+    const bool ok = j < np1;
+    const float* row = rows + (size_t)(ok ? j : 0) * a.ldb;
+    const float av = ok ? row[a.roff + li] : 0.f;
+    float bv[DESC_NT];
 #pragma unroll
-      for (int n = 0; n < NMAXF; ++n)
-        if (n < P.N) C[n * P.NLM + lm] = acc[h][n];
+    for (int t = 0; t < DESC_NT; ++t) bv[t] = (ok && 16 * t + li < NLM) ? row[16 * t + li] : 0.f;
+#pragma unroll
+    for (int t = 0; t < DESC_NT; ++t)
+      if (16 * t < NLM) cacc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[t], cacc[t], 0, 0, 0);
+  }
+  {
+    const int s = a.pr_spec[b0];  // the central atom's term, lm = 0
+    if (s >= 0 && li == 0) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int nl = 4 * lg + r - P.sp[s].off;
+        if (nl >= 0 && nl < P.sp[s].nb) cacc[0][r] += (float)P.cen[s * TGX_MAX_NB + nl];
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < DESC_NT; ++t) {
+    const int lm = 16 * t + li;
+    if (lm < NLM)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (4 * lg + r < P.N) C[(4 * lg + r) * NLM + lm] = cacc[t][r];
   }
   wave_sync();
   const float nrm = power(P, C, X);
@@ -1379,6 +1411,8 @@ int tgx_desc_f32(tgx_ctx* ctx, tgx_model_t* m) {
   const int ns = m->n_sites_b;
   if (ns == 0) return TGX_OK;
   TGX_REQUIRE(m->P.N <= NMAXF, "FP32 descriptor kernel holds at most %d radial functions", NMAXF);
+  TGX_REQUIRE(m->P.NLM <= 16 * DESC_NT, "FP32 descriptor kernel holds at most %d harmonics (l_max %d)", 16 * DESC_NT,
+              m->l_max);
   const int64_t np = m->n_pairs_b;
   F32Args a = f32_args(m);
   TGX_TRY(m->pbasis.ensure(sizeof(float) * (size_t)(np > 0 ? np : 1) * a.ldb));
