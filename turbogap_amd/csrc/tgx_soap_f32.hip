@@ -820,7 +820,7 @@ constexpr int AM_NRES = 32;  // pair rows kept in LDS through the whole site
 constexpr int AM_WIN = 64;   // neighbour entries per list window
 constexpr int AM_KR = 4;     // raising terms per (lm', component)
 constexpr int AM_MF = 8;     // LDS floats per window entry: 1/r, k, k', u (3), pair index (int64)
-constexpr int AM_QV = 5;     // Qss values per thread (D <= 1280)
+constexpr int AM_QV = 2;     // Qss quads per thread (D <= 2048): floats 4 q .. 4 q + 3, q = thread + 256 j
 constexpr float AM_LOG2E = 1.4426950408889634f;
 
 constexpr int AM_NRT = 64 * 3 * AM_KR;  // raising-table entries (lm' < 64)
@@ -891,7 +891,7 @@ struct AmPrefetch {
   int64_t b0;
   int np1;
   float4 rv[4];
-  float qv[AM_QV];
+  float4 qv[AM_QV];
   AmRaw raw;
   int64_t gself;
   int sself;
@@ -908,11 +908,17 @@ __device__ __forceinline__ void am_prefetch(const SoapParams& P, const F32Args& 
     const int t = threadIdx.x + 256 * u;
     f.rv[u] = t < n4 ? src[t] : float4{0.f, 0.f, 0.f, 0.f};
   }
-  const float* srow = qss + (size_t)site * P.Dpad;
+  const float4* srow = reinterpret_cast<const float4*>(qss + (size_t)site * P.Dpad);
 #pragma unroll
   for (int j = 0; j < AM_QV; ++j) {
-    const int d = threadIdx.x + 256 * j;
-    f.qv[j] = d < P.D ? srow[d] : 0.f;
+    const int q = threadIdx.x + 256 * j;
+    float4 v = 4 * q < P.D ? srow[q] : float4{0.f, 0.f, 0.f, 0.f};
+    if (4 * q + 3 >= P.D) {  // the row's padding columns
+      if (4 * q + 1 >= P.D) v.y = 0.f;
+      if (4 * q + 2 >= P.D) v.z = 0.f;
+      if (4 * q + 3 >= P.D) v.w = 0.f;
+    }
+    f.qv[j] = v;
   }
   f.raw = AmRaw{-1, 0.0, 0.0, 0.0, 0.0, 0};
   f.gself = 0;
@@ -975,7 +981,7 @@ __global__ __launch_bounds__(256, 4) void k_site_adjoint_mfma(SoapParams P, F32A
   const int64_t p1 = b0 + 1;
   const bool resident = np1 <= AM_NRES;
   const int cnt0 = min(AM_NRES, max(np1, 0));
-  float qv[AM_QV];
+  float4 qv[AM_QV];
 #pragma unroll
   for (int j = 0; j < AM_QV; ++j) qv[j] = pf.qv[j];
   const int64_t gself = pf.gself;
@@ -1064,11 +1070,16 @@ __global__ __launch_bounds__(256, 4) void k_site_adjoint_mfma(SoapParams P, F32A
     float s1 = 0.f, s2 = 0.f;
 #pragma unroll
     for (int j = 0; j < AM_QV; ++j) {
-      const int d = threadIdx.x + 256 * j;
-      if (d < D) {
-        const float x = X[d];
-        s1 += qv[j] * x;
-        s2 += x * x;
+      const int q = threadIdx.x + 256 * j;
+      if (4 * q < D) {  // X holds (D + 3) / 4 whole quads; entries past D are not the spectrum's
+        float4 x = *reinterpret_cast<const float4*>(X + 4 * q);
+        if (4 * q + 3 >= D) {
+          if (4 * q + 1 >= D) x.y = 0.f;
+          if (4 * q + 2 >= D) x.z = 0.f;
+          if (4 * q + 3 >= D) x.w = 0.f;
+        }
+        s1 += (qv[j].x * x.x + qv[j].y * x.y) + (qv[j].z * x.z + qv[j].w * x.w);
+        s2 += (x.x * x.x + x.y * x.y) + (x.z * x.z + x.w * x.w);
       }
     }
     s1 = wave_total(s1);
@@ -1086,8 +1097,13 @@ __global__ __launch_bounds__(256, 4) void k_site_adjoint_mfma(SoapParams P, F32A
     const float cq = pp > 0.f ? qp / pp : 0.f;
 #pragma unroll
     for (int j = 0; j < AM_QV; ++j) {
-      const int d = threadIdx.x + 256 * j;
-      if (d < D) X[d] = (qv[j] - cq * X[d]) * inv_n;
+      const int q = threadIdx.x + 256 * j;
+      if (4 * q < D) {
+        float4* xp = reinterpret_cast<float4*>(X + 4 * q);
+        const float4 x = *xp;
+        *xp = float4{(qv[j].x - cq * x.x) * inv_n, (qv[j].y - cq * x.y) * inv_n, (qv[j].z - cq * x.z) * inv_n,
+                     (qv[j].w - cq * x.w) * inv_n};
+      }
     }
   }
   __syncthreads();
