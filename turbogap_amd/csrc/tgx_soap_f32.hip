@@ -969,9 +969,11 @@ __global__ __launch_bounds__(256, 4) void k_site_adjoint_mfma(SoapParams P, F32A
   int* rt_idx = gcnt + 4;                   // [AM_NRT] the raising table (row offsets in CH), loaded once
   float* rt_coef = reinterpret_cast<float*>(rt_idx + AM_NRT);
   const int tx = w, te = 4 + (w >> 1), tpart = w & 1, tu = w;
-  for (int t = threadIdx.x; t < AM_NRT; t += 256) {
-    rt_idx[t] = raise_idx[t] * AM_HP;
-    rt_coef[t] = raise_coef[t];
+  static_assert(AM_NRT % 4 == 0, "raising table in 16-byte pieces");
+  for (int t = threadIdx.x; t < AM_NRT / 4; t += 256) {  // (LDS offsets of rt_idx: multiples of 4 floats)
+    const int4 ix = reinterpret_cast<const int4*>(raise_idx)[t];
+    reinterpret_cast<int4*>(rt_idx)[t] = int4{ix.x * AM_HP, ix.y * AM_HP, ix.z * AM_HP, ix.w * AM_HP};
+    reinterpret_cast<float4*>(rt_coef)[t] = reinterpret_cast<const float4*>(raise_coef)[t];
   }
   const int site = blockIdx.x;
   AmPrefetch pf;
