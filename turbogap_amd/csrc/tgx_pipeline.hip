@@ -53,24 +53,27 @@ __global__ __launch_bounds__(256) void k_select_fill(int n, SpeciesMap sm, doubl
                                                    int64_t* __restrict__ pr_gidx, int* __restrict__ pr_nbr,
                                                    int* __restrict__ pr_spec, double* __restrict__ pr_r,
                                                    double* __restrict__ pr_xyz, int* __restrict__ seen) {
-  const int lane = threadIdx.x & 63;
-  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  // two atoms per wavefront, 32 lanes each (lists average about 22 entries);
+  // each half's ballot bits are masked to its own lanes
+  const int lane = threadIdx.x & 63, sub = lane & 31;
+  const uint64_t half = (lane < 32 ? 0xffffffffull : 0xffffffff00000000ull);
+  const int i = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 2 + (lane >> 5);
   if (i >= n || !flag[i]) return;
   const int64_t s = site_idx[i];
   const int64_t o = pair_off[i];
-  if (lane == 0) { site_atom[s] = i; site_beg[s] = o; }
+  if (sub == 0) { site_atom[s] = i; site_beg[s] = o; }
   const int64_t b = off[i];
   const int nn = n_neigh[i];
   int kept = 0;
-  for (int base = 0; base < nn; base += 64) {
-    const int k = base + lane;
+  for (int base = 0; base < nn; base += 32) {
+    const int k = base + sub;
     bool keep = false;
     int sp = -1;
     if (k < nn) {
       sp = sm.index(nspec[b + k]);
       keep = rjs[b + k] < rcut && sp >= 0;
     }
-    const uint64_t mask = __ballot(keep);
+    const uint64_t mask = __ballot(keep) & half;
     if (keep) {
       const int64_t p = o + kept + __popcll(mask & ((1ull << lane) - 1ull));
       const int64_t g = b + k;
@@ -93,23 +96,40 @@ __global__ void k_set_last(int64_t* __restrict__ site_beg, int ns, const int64_t
 }
 
 // F_j = sum over j's list entries q of fglob[rev[q]], in list order.
+// GATHER_LANES lanes per atom: lane t sums entries t, t + GATHER_LANES, ... in
+// order, then the lanes' sums combine by a fixed xor butterfly, so every run
+// adds in the same order.
+constexpr int GATHER_LANES = 16;
+
 __global__ void k_gather(int n, const int* __restrict__ n_neigh, const int64_t* __restrict__ off,
                          const int64_t* __restrict__ rev, const double* __restrict__ fglob, double* __restrict__ F,
                          int* __restrict__ bad) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
+  const int t = threadIdx.x % GATHER_LANES;
+  const int j = (blockIdx.x * blockDim.x + threadIdx.x) / GATHER_LANES;
+  const bool live = j < n;
   double f0 = 0, f1 = 0, f2 = 0;
-  const int64_t b = off[j];
-  for (int k = 0; k < n_neigh[j]; ++k) {
-    const int64_t r = rev[b + k];
-    if (r < 0) { atomicExch(bad, 1); continue; }
-    f0 += fglob[3 * r];
-    f1 += fglob[3 * r + 1];
-    f2 += fglob[3 * r + 2];
+  if (live) {
+    const int64_t b = off[j];
+    const int nn = n_neigh[j];
+    for (int k = t; k < nn; k += GATHER_LANES) {
+      const int64_t r = rev[b + k];
+      if (r < 0) { atomicExch(bad, 1); continue; }
+      f0 += fglob[3 * r];
+      f1 += fglob[3 * r + 1];
+      f2 += fglob[3 * r + 2];
+    }
   }
-  F[3 * j] = f0;
-  F[3 * j + 1] = f1;
-  F[3 * j + 2] = f2;
+#pragma unroll
+  for (int o = GATHER_LANES / 2; o > 0; o >>= 1) {  // every lane of the wavefront takes part
+    f0 += __shfl_xor(f0, o);
+    f1 += __shfl_xor(f1, o);
+    f2 += __shfl_xor(f2, o);
+  }
+  if (live && t == 0) {
+    F[3 * j] = f0;
+    F[3 * j + 1] = f1;
+    F[3 * j + 2] = f2;
+  }
 }
 
 // Virial partials: 0.5 (f x^T + x f^T) summed over a fixed slice of pairs per
@@ -220,7 +240,7 @@ int tgx_block_select_fill(tgx_ctx* ctx, tgx_model_t* m) {
   TGX_TRY(m->pr_spec.ensure(sizeof(int) * npz));
   TGX_TRY(m->pr_r.ensure(sizeof(double) * npz));
   TGX_TRY(m->pr_xyz.ensure(sizeof(double) * 3 * npz));
-  TGX_LAUNCH(ctx, "select_fill", k_select_fill, dim3((n + 3) / 4), dim3(256), 0, n, sm, m->rcut_max, v.flag,
+  TGX_LAUNCH(ctx, "select_fill", k_select_fill, dim3((n + 7) / 8), dim3(256), 0, n, sm, m->rcut_max, v.flag,
              v.site_idx, v.pair_off, ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(), ctx->nlist.as<int>(),
              ctx->rjs.as<double>(), ctx->xyz.as<double>(), ctx->nspec.as<int>(), m->site_atom.as<int>(),
              m->site_beg.as<int64_t>(), m->pr_gidx.as<int64_t>(), m->pr_nbr.as<int>(), m->pr_spec.as<int>(),
@@ -258,7 +278,8 @@ int tgx_gather_pair_forces(tgx_ctx* ctx, const double* fglob, double* F, double*
   double* epart = vpart + 9 * nbk;
   int* bad = reinterpret_cast<int*>(epart + TGX_SUM_PARTS);
   TGX_TRY(tgx_zero(ctx, bad, sizeof(double)));
-  TGX_LAUNCH(ctx, "force_gather", k_gather, dim3((n + 255) / 256), dim3(256), 0, n, ctx->n_neigh.as<int>(),
+  TGX_LAUNCH(ctx, "force_gather", k_gather, dim3((unsigned)(((int64_t)n * GATHER_LANES + 255) / 256)), dim3(256), 0,
+             n, ctx->n_neigh.as<int>(),
              ctx->nl_off.as<int64_t>(), ctx->rev.as<int64_t>(), fglob, F, bad);
   TGX_LAUNCH(ctx, "virial", k_virial, dim3(nbk), dim3(256), 0, P, fglob, ctx->xyz.as<double>(), vpart);
   const bool with_e = energy_out && d_energies;
