@@ -193,6 +193,14 @@ constexpr int NBW = 4;  // wavefronts (central atoms) per workgroup
 // operands, so the same bits) instead of visiting its 27 cells again.
 constexpr int NB_HITS = 64;
 
+// FP32 screen of the count pass: squared distances surely inside (in2) and
+// surely outside (out2) the cutoff, and the box sides in FP32.
+struct CountScreen {
+  int on;
+  float in2, out2;
+  float L[3];
+};
+
 template <int MODE>
 __global__ __launch_bounds__(64 * NBW) void k_cell_neigh(int n, const double* __restrict__ pos, BoxInfo bx, int mx,
                                                        int my, int mz, double rcut, const int* __restrict__ do_list,
@@ -204,7 +212,8 @@ __global__ __launch_bounds__(64 * NBW) void k_cell_neigh(int n, const double* __
                                                        const int* __restrict__ species, int* __restrict__ n_neigh,
                                                        const int64_t* __restrict__ off, int* __restrict__ nlist,
                                                        double* __restrict__ rjs, double* __restrict__ xyz,
-                                                       int* __restrict__ nspec, int* __restrict__ hit_slots) {
+                                                       int* __restrict__ nspec, int* __restrict__ hit_slots,
+                                                       CountScreen screen) {
   __shared__ int s_beg[NBW][29];  // running counts of the (up to 27) visited cells, then their number
   __shared__ int64_t s_st[NBW][27];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -322,8 +331,31 @@ __global__ __launch_bounds__(64 * NBW) void k_cell_neigh(int n, const double* __
       k = cell_atoms[slot];
       if (k != i) {
         const double pk[3] = {cell_pos[3 * slot], cell_pos[3 * slot + 1], cell_pos[3 * slot + 2]};
-        get_distance_ortho(pi, pk, bx, dist, d);  // the cell list is built for orthorhombic boxes only
-        hit = d < rcut;
+        // the count decides candidates far from the cutoff from an FP32 minimum-image
+        // estimate (screen.on: at least three cells a side, so a component near L/2
+        // is far outside, and sides <= 1000 A, so the estimate's error is a tenth of
+        // the margin or less); the rest take the reference's FP64 sequence
+        int verdict = -1;
+        if (MODE == 0 && screen.on) {
+          float d2f = 0.f;
+          bool wide = false;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const double a = pk[c] - pi[c];
+            wide = wide || fabs(a) >= bx.L[c];
+            float r = (float)a;
+            if (r < 0.f) r += screen.L[c];
+            if (r > 0.5f * screen.L[c]) r -= screen.L[c];
+            d2f += r * r;
+          }
+          if (!wide) verdict = d2f < screen.in2 ? 1 : (d2f > screen.out2 ? 0 : -1);
+        }
+        if (verdict >= 0) {
+          hit = verdict == 1;
+        } else {
+          get_distance_ortho(pi, pk, bx, dist, d);  // the cell list is built for orthorhombic boxes only
+          hit = d < rcut;
+        }
       }
     }
     const uint64_t mask = __ballot(hit);
@@ -579,10 +611,18 @@ int tgx_neighbors_run(tgx_ctx* ctx, double rcut_max, const int* indices, const i
     TGX_LAUNCH(ctx, "nbr_cell_pos", k_cell_pos, dim3((n + 255) / 256), dim3(256), 0, n, ctx->cell_atoms.as<int>(),
                ctx->pos.as<double>(), ctx->cell_pos.as<double>());
     TGX_TRY(ctx->nbr_hits.ensure(sizeof(int) * (size_t)n * NB_HITS));
+    // margin 1e-3 rcut (2e-3 rcut^2 in the squares): the FP32 estimate's error, a
+    // few ulp of the side per component, is a tenth of it or less up to 1000 A
+    CountScreen screen{0, 0.f, 0.f, {(float)bx.L[0], (float)bx.L[1], (float)bx.L[2]}};
+    if (mx >= 3 && my >= 3 && mz >= 3 && bx.L[0] <= 1000.0 && bx.L[1] <= 1000.0 && bx.L[2] <= 1000.0) {
+      screen.on = 1;
+      screen.in2 = (float)((rcut_max * (1.0 - 1e-3)) * (rcut_max * (1.0 - 1e-3)));
+      screen.out2 = (float)((rcut_max * (1.0 + 1e-3)) * (rcut_max * (1.0 + 1e-3)));
+    }
     TGX_LAUNCH(ctx, "nbr_count", k_cell_neigh<0>, wgrid, wblk, 0, n, ctx->pos.as<double>(), bx, mx, my, mz, rcut_max,
                d_do, cell_xyz, ctx->cell_start.as<int64_t>(), cnt, ctx->cell_atoms.as<int>(),
                ctx->cell_pos.as<double>(), ctx->species_sc.as<int>(), ctx->n_neigh.as<int>(), nullptr, nullptr,
-               nullptr, nullptr, nullptr, ctx->nbr_hits.as<int>());
+               nullptr, nullptr, nullptr, ctx->nbr_hits.as<int>(), screen);
   } else {
     TGX_LAUNCH(ctx, "nbr_count_brute", k_brute_neigh<0>, wgrid, wblk, 0, n, n_sc, ctx->pos.as<double>(), bx, rcut_max,
                d_do, ctx->species_sc.as<int>(), ctx->n_neigh.as<int>(), nullptr, nullptr, nullptr, nullptr, nullptr);
@@ -604,7 +644,7 @@ int tgx_neighbors_run(tgx_ctx* ctx, double rcut_max, const int* indices, const i
                d_do, cell_xyz, ctx->cell_start.as<int64_t>(), ctx->cell_cnt.as<int>(), ctx->cell_atoms.as<int>(),
                ctx->cell_pos.as<double>(), ctx->species_sc.as<int>(), ctx->n_neigh.as<int>(),
                ctx->nl_off.as<int64_t>(), ctx->nlist.as<int>(), ctx->rjs.as<double>(), ctx->xyz.as<double>(),
-               ctx->nspec.as<int>(), ctx->nbr_hits.as<int>());
+               ctx->nspec.as<int>(), ctx->nbr_hits.as<int>(), CountScreen{0, 0.f, 0.f, {0.f, 0.f, 0.f}});
   } else {
     TGX_LAUNCH(ctx, "nbr_fill_brute", k_brute_neigh<1>, wgrid, wblk, 0, n, n_sc, ctx->pos.as<double>(), bx, rcut_max,
                d_do, ctx->species_sc.as<int>(), ctx->n_neigh.as<int>(), ctx->nl_off.as<int64_t>(),
