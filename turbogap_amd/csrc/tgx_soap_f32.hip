@@ -930,6 +930,16 @@ __device__ __forceinline__ void am_prefetch(const SoapParams& P, const F32Args& 
   }
 }
 
+// Sum over the four 16-lane rows of a wavefront, (row 0 + row 1) + (row 2 + row 3)
+// in every lane (the order of xor-16 then xor-32 shuffles): v_permlane16_swap
+// pairs rows 0/1 and 2/3, v_permlane32_swap the two halves, both on the VALU.
+__device__ __forceinline__ float am_row_sum(float v) {
+  const auto s16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  const float h = __uint_as_float(s16[0]) + __uint_as_float(s16[1]);
+  const auto s32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(h), __float_as_uint(h), false, false);
+  return __uint_as_float(s32[0]) + __uint_as_float(s32[1]);
+}
+
 __device__ __forceinline__ void am_store_entry(const AmEntry& e, int lane, float* meta, int* lists, int* gcnt) {
   if (e.grp >= 0) {
     float* mt = meta + lane * AM_MF;
@@ -1248,15 +1258,11 @@ __global__ __launch_bounds__(256, 4) void k_site_adjoint_mfma(SoapParams P, F32A
             V2 += Wr * x2[r];
           }
         }
-        // the four lane rows hold the same pair
-        T += __shfl_xor(T, 16);
-        V0 += __shfl_xor(V0, 16);
-        V1 += __shfl_xor(V1, 16);
-        V2 += __shfl_xor(V2, 16);
-        T += __shfl_xor(T, 32);
-        V0 += __shfl_xor(V0, 32);
-        V1 += __shfl_xor(V1, 32);
-        V2 += __shfl_xor(V2, 32);
+        // the four lane rows hold the same pair: (row 0 + row 1) + (row 2 + row 3)
+        T = am_row_sum(T);
+        V0 = am_row_sum(V0);
+        V1 = am_row_sum(V1);
+        V2 = am_row_sum(V2);
         float* pb = part + (tcount & 1) * (4 * 16 * 4);
         if (lg == 0) *reinterpret_cast<f32x4_t*>(pb + (w * 16 + li) * 4) = f32x4_t{T, V0, V1, V2};
         __syncthreads();
