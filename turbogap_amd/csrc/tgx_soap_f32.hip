@@ -830,11 +830,13 @@ __host__ __device__ inline int am_lds_floats(const SoapParams& P, int ldb) {
          2 * AM_NRT;
 }
 
-__device__ __forceinline__ int am_degree(int lm) {
-  int l = (int)sqrtf((float)lm + 0.5f);
-  if (l * l > lm) --l;
-  return l;
-}
+// Degree l of lm = l^2 + l + m. sqrt(lm + 1/2) lies at least 1/(4 l + 4) above
+// l and below l + 1, far more than the error of v_sqrt_f32 (one ulp), so the
+// truncation of the hardware square root is exact for every lm below 2^20
+// (checked exhaustively with two ulp of error either way); the correctly
+// rounded sqrtf sequence and its integer correction cost five times the
+// instructions, and the adjoint evaluates twelve degrees per lane and site.
+__device__ __forceinline__ int am_degree(int lm) { return (int)__builtin_amdgcn_sqrtf((float)lm + 0.5f); }
 
 // One window of neighbour entries [win0, win0 + wcnt) (wave 0): the per-pair
 // data of the contributing pairs (the block's species, 0 < r < rcut of that
@@ -1061,7 +1063,8 @@ __global__ __launch_bounds__(256, 4) void k_site_adjoint_mfma(SoapParams P, F32A
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
       const int k = 4 * q + lg;
-      v[q] = (li < N && k < K) ? CH[(l * l + min(k, 2 * l)) * AM_HP + li] : 0.f;
+      // (24-bit products: full-rate on the VALU, where a 32-bit product is quarter-rate)
+      v[q] = (li < N && k < K) ? CH[__umul24(l * l + min(k, 2 * l), AM_HP) + li] : 0.f;
     }
 #pragma unroll
     for (int q = 0; q < 5; ++q)
@@ -1070,8 +1073,8 @@ __global__ __launch_bounds__(256, 4) void k_site_adjoint_mfma(SoapParams P, F32A
     for (int r = 0; r < 4; ++r) {
       const int n = 4 * lg + r, np = li;
       if (n <= np && np < N) {
-        const int pidx = n * N - n * (n - 1) / 2 + (np - n);
-        X[pidx * L1 + l] = n == np ? g[r] : (float)M_SQRT2 * g[r];
+        const int pidx = __umul24(n, N) - n * (n - 1) / 2 + (np - n);
+        X[__umul24(pidx, L1) + l] = n == np ? g[r] : (float)M_SQRT2 * g[r];
       }
     }
   }
@@ -1129,7 +1132,7 @@ __global__ __launch_bounds__(256, 4) void k_site_adjoint_mfma(SoapParams P, F32A
       const int n = li, np = 4 * s + lg;
       if (n < N && np < N) {
         const int u = min(n, np), v = max(n, np);
-        sidx[s] = (u * N - u * (u - 1) / 2 + (v - u)) * L1;
+        sidx[s] = __umul24(__umul24(u, N) - u * (u - 1) / 2 + (v - u), L1);
         swt[s] = n == np ? 2.f : (float)M_SQRT2;
       } else {
         sidx[s] = 0;
@@ -1181,13 +1184,13 @@ __global__ __launch_bounds__(256, 4) void k_site_adjoint_mfma(SoapParams P, F32A
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int l = am_degree(16 * tx + 4 * lg + r);
-    lx1[r] = (float)(l * (l + 1));
     lx2[r] = (float)l;
+    lx1[r] = lx2[r] * (lx2[r] + 1.f);  // exact in FP32, without a quarter-rate integer product
     const int lme = 16 * te + 4 * lg + r;
     ve[r] = lme < AM_NLM;
     const int le = am_degree(min(lme, AM_NLM - 1));
-    le1[r] = (float)(le * (le + 1));
     le2[r] = (float)le;
+    le1[r] = le2[r] * (le2[r] + 1.f);
     lr1[r] = -2.f * AM_LOG2E * (float)(am_degree(16 * tu + 4 * lg + r) + 1);
   }
 
