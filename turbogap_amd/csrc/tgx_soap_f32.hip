@@ -133,44 +133,6 @@ typedef float f32x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x16_t __attribute__((ext_vector_type(16), aligned(4)));
 typedef float f32x8_t __attribute__((ext_vector_type(8), aligned(4)));  // rows are only dword-aligned in general
 
-// Power spectrum p(n, n', l) = w sum_m c(n, lm) c(n', lm) (w = sqrt(2) for
-// n != n') and q = p/|p| into X[D]. For every l the 16 x 16 Gram matrix
-// C_l C_l^T is formed on the matrix cores by v_mfma_f32_16x16x4_f32 (k over m
-// in steps of 4): lane t supplies C[t % 16][l^2 + k + t / 16] as both operands,
-// and holds the result rows 4 (t / 16) + r, column t % 16. Needs N <= 16 and
-// the whole wavefront active.
-__device__ float power(const SoapParams& P, const float* C, float* X) {
-  const int lane = threadIdx.x & 63;
-  const int NLM = P.NLM, N = P.N, L1 = P.L + 1;
-  const int cn = lane & 15, kq = lane >> 4;
-  float acc = 0.f;
-  for (int l = 0; l < L1; ++l) {
-    const int K = 2 * l + 1;
-    f32x4_t g = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += 4) {
-      const int k = k0 + kq;
-      const float v = (cn < N && k < K) ? C[cn * NLM + l * l + k] : 0.f;
-      g = __builtin_amdgcn_mfma_f32_16x16x4f32(v, v, g, 0, 0, 0);
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int n = 4 * kq + r, np = cn;
-      if (n <= np && np < N) {
-        const int pidx = n * N - n * (n - 1) / 2 + (np - n);
-        const float s = n == np ? g[r] : (float)M_SQRT2 * g[r];
-        X[pidx * L1 + l] = s;
-        acc += s * s;
-      }
-    }
-  }
-  const float nrm = sqrtf(wsum(acc));
-  const float inv = nrm > 0.f ? 1.f / nrm : 0.f;
-  wave_sync();
-  for (int d = lane; d < P.D; d += 64) X[d] *= inv;
-  wave_sync();
-  return nrm;
-}
-
 // All damped harmonics of one direction: w[lm] = A[l] Y_lm(u), the recurrences
 // of tgx_sph_host.h at r = 1.
 __device__ __forceinline__ void harmonics_all(int L, const float* ynorm, float ux, float uy, float uz,
@@ -445,7 +407,7 @@ __device__ __forceinline__ void site_coefficients(const SoapParams& P, const F32
 }
 
 __host__ __device__ inline int desc_wave_floats(const SoapParams& P) { return P.N * P.NLM + P.D; }
-constexpr int DESC_NT = 8;  // lm tiles of 16 of k_site_desc (NLM <= 128)
+constexpr int DESC_NT = 8;  // most lm tiles of 16 of k_site_desc (NLM <= 128)
 constexpr int APASS = 16;  // pair rows staged per pass of the adjoint
 // per wavefront: max(X [D], APASS rows) then the pass's pair forces [APASS][3]
 __host__ __device__ inline int adj_rows_floats(const SoapParams& P, int ldb) {
@@ -454,72 +416,141 @@ __host__ __device__ inline int adj_rows_floats(const SoapParams& P, int ldb) {
 }
 __host__ __device__ inline int adj_wave_floats(const SoapParams& P, int ldb) { return adj_rows_floats(P, ldb) + 16 * APASS; }
 
+// One wavefront per site; l_max is a template argument (NLM = (LL + 1)^2 <=
+// 128), so the coefficient and power-spectrum stages are chains of matrix-core
+// steps without run-time guards: with a guarded MFMA the compiler copies every
+// accumulator out of and back into the accumulation registers on each step, and
+// each guarded load becomes its own branch with a full wait.
+template <int LL>
 __global__ __launch_bounds__(64 * WAVES) void k_site_desc(SoapParams P, F32Args a, int n_sites,
                                                           float* __restrict__ q_out, __bf16* __restrict__ qb_out,
                                                           float* __restrict__ pnorm_out) {
+  constexpr int L1 = LL + 1, NLM = L1 * L1, NT = (NLM + 15) / 16;
+  static_assert(NT <= DESC_NT, "at most 128 harmonics");
   extern __shared__ __align__(16) float smf[];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int site = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES + w);
   if (site >= n_sites) return;
+  const int N = P.N;
   float* C = smf + w * desc_wave_floats(P);
-  float* X = C + P.N * P.NLM;
+  float* X = C + N * NLM;
   const int64_t b0 = a.site_beg[site];
   const int nn = (int)(a.site_beg[site + 1] - b0);
+  const int li = lane & 15, lg = lane >> 4;
+  // the central atom's term, lm = 0 (read here, so that it overlaps the rows)
+  float cadd[4];
+  bool cuse[4];
+  {
+    const int s = a.pr_spec[b0];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int nl = s >= 0 ? 4 * lg + r - P.sp[s].off : -1;
+      cuse[r] = s >= 0 && li == 0 && nl >= 0 && nl < P.sp[s].nb;
+      cadd[r] = cuse[r] ? (float)P.cen[s * TGX_MAX_NB + nl] : 0.f;
+    }
+  }
   // c(n, lm) = cen + sum_j R_n(j) W_lm(j) on the matrix cores, K over the
   // neighbour rows (4 per step): lane (n = lane % 16, j = k0 + lane / 16) holds
   // R_n(j) and, per lm tile t, W_(16 t + lane % 16)(j), both read straight from
-  // the rows; accumulator t holds c(4 (lane / 16) + r, 16 t + lane % 16)
-  const int li = lane & 15, lg = lane >> 4, NLM = P.NLM;
-  f32x4_t cacc[DESC_NT];
+  // the rows; accumulator t holds c(4 (lane / 16) + r, 16 t + lane % 16).
+  // Every load is in bounds without a guard: rows past the site's last one
+  // read that last row and columns past NLM read column NLM - 1. A row past the
+  // end contributes R = 0 (its products are zero), and the columns past NLM
+  // are never stored, so c is what the guarded form gives, bit for bit.
+  f32x4_t cacc[NT];
 #pragma unroll
-  for (int t = 0; t < DESC_NT; ++t) cacc[t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  for (int t = 0; t < NT; ++t) cacc[t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
   const int np1 = nn - 1;
   const float* rows = a.basis + (b0 + 1) * a.ldb;
-#pragma unroll 2
-  for (int k0 = 0; k0 < np1; k0 += 4) {
-    const int j = k0 + lg;
-    const bool ok = j < np1;
-    const float* row = rows + (size_t)(ok ? j : 0) * a.ldb;
-    const float av = ok ? row[a.roff + li] : 0.f;
-    float bv[DESC_NT];
+  constexpr int KS = 8;  // k-steps per pass (32 rows): all of a pass's loads are issued before its MFMAs
+  for (int k0 = 0; k0 < np1; k0 += 4 * KS) {
+    float av[KS], bv[KS][NT];
 #pragma unroll
-    for (int t = 0; t < DESC_NT; ++t) bv[t] = (ok && 16 * t + li < NLM) ? row[16 * t + li] : 0.f;
+    for (int u = 0; u < KS; ++u) {
+      const float* row = rows + (size_t)min(k0 + 4 * u + lg, np1 - 1) * a.ldb;
+      av[u] = row[a.roff + li];
 #pragma unroll
-    for (int t = 0; t < DESC_NT; ++t)
-      if (16 * t < NLM) cacc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[t], cacc[t], 0, 0, 0);
-  }
-  {
-    const int s = a.pr_spec[b0];  // the central atom's term, lm = 0
-    if (s >= 0 && li == 0) {
+      for (int t = 0; t < NT; ++t) bv[u][t] = row[min(16 * t + li, NLM - 1)];
+    }
+    // every k-step of the pass, also those past the site's rows (R = 0 there):
+    // a guard, even a uniform one, brings back the accumulator copies
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int nl = 4 * lg + r - P.sp[s].off;
-        if (nl >= 0 && nl < P.sp[s].nb) cacc[0][r] += (float)P.cen[s * TGX_MAX_NB + nl];
-      }
+    for (int u = 0; u < KS; ++u) {
+      const float ar = k0 + 4 * u + lg < np1 ? av[u] : 0.f;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) cacc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bv[u][t], cacc[t], 0, 0, 0);
     }
   }
 #pragma unroll
-  for (int t = 0; t < DESC_NT; ++t) {
+  for (int r = 0; r < 4; ++r)
+    if (cuse[r]) cacc[0][r] += cadd[r];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
     const int lm = 16 * t + li;
     if (lm < NLM)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        if (4 * lg + r < P.N) C[(4 * lg + r) * NLM + lm] = cacc[t][r];
+        if (4 * lg + r < N) C[(4 * lg + r) * NLM + lm] = cacc[t][r];
   }
   wave_sync();
-  const float nrm = power(P, C, X);
-  __bf16* qbrow = qb_out + (size_t)site * P.Dpad;
-  if (q_out) {  // the FP32 copy and norm, for the adjoint kernel that reads them
-    float* qrow = q_out + (size_t)site * P.Dpad;
-    for (int d = lane; d < P.Dpad; d += 64) {
-      const float v = d < P.D ? X[d] : 0.f;
-      qrow[d] = v;
-      qbrow[d] = (__bf16)v;
-    }
-    if (lane == 0) pnorm_out[site] = nrm;
-  } else {
-    for (int d = lane; d < P.Dpad; d += 64) qbrow[d] = (__bf16)(d < P.D ? X[d] : 0.f);
+  // power spectrum p(n <= n', l) = w sum_m c(n, lm) c(n', lm), w = 1 or sqrt 2:
+  // lane (n' = lane % 16, k = 4 q + lane / 16) holds c(n', l^2 + k); the MFMA of
+  // that column with itself leaves g(4 (lane / 16) + r, n') in accumulator r.
+  // The degrees are independent chains (stored unnormalised; the norm is
+  // applied on the way out).
+  int xo[4];
+  float xw[4];
+  bool xv[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int n = 4 * lg + r, np = li;
+    xv[r] = n <= np && np < N;
+    xo[r] = (n * N - n * (n - 1) / 2 + (np - n)) * L1;
+    xw[r] = n == np ? 1.f : (float)M_SQRT2;
   }
+  const float* Cn = C + min(li, N - 1) * NLM;
+  float acc = 0.f;
+#pragma unroll
+  for (int l = 0; l <= LL; ++l) {
+    const int K = 2 * l + 1;
+    f32x4_t g = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int q = 0; 4 * q < K; ++q) {
+      const int k = 4 * q + lg;
+      const float v = Cn[l * l + min(k, 2 * l)];
+      g = __builtin_amdgcn_mfma_f32_16x16x4f32(li < N && k < K ? v : 0.f, li < N && k < K ? v : 0.f, g, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (xv[r]) {
+        const float sv = xw[r] * g[r];
+        X[xo[r] + l] = sv;
+        acc += sv * sv;
+      }
+  }
+  const float nrm = sqrtf(wsum(acc));
+  const float inv = nrm > 0.f ? 1.f / nrm : 0.f;
+  wave_sync();
+  // the BF16 row in 16-byte stores of eight components (Dpad is a multiple of 8)
+  const int D = P.D;
+  uint4* qbrow = reinterpret_cast<uint4*>(qb_out + (size_t)site * P.Dpad);
+  float* qrow = q_out ? q_out + (size_t)site * P.Dpad : nullptr;
+  for (int d0 = 8 * lane; d0 < P.Dpad; d0 += 512) {
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = d0 + i < D ? X[d0 + i] * inv : 0.f;
+    unsigned pk[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      pk[i] = (unsigned)__builtin_bit_cast(unsigned short, (__bf16)v[2 * i]) |
+              ((unsigned)__builtin_bit_cast(unsigned short, (__bf16)v[2 * i + 1]) << 16);
+    qbrow[d0 / 8] = uint4{pk[0], pk[1], pk[2], pk[3]};
+    if (qrow) {  // the FP32 copy, for the adjoint kernel that reads it
+      reinterpret_cast<float4*>(qrow + d0)[0] = float4{v[0], v[1], v[2], v[3]};
+      reinterpret_cast<float4*>(qrow + d0)[1] = float4{v[4], v[5], v[6], v[7]};
+    }
+  }
+  if (q_out && lane == 0) pnorm_out[site] = nrm;
 }
 
 // Sum over each 16-lane row, left in the row's last lane (lanes 15, 31, 47, 63):
@@ -1459,9 +1490,30 @@ int tgx_desc_f32(tgx_ctx* ctx, tgx_model_t* m) {
   // the matrix-core adjoint forms the power spectrum from the coefficients
   // again: the FP32 descriptors and norms are written only for the other one
   const bool fp32_copy = !adjoint_on_mfma(m, a);
-  TGX_LAUNCH(ctx, "soap_desc_f32", k_site_desc, dim3((ns + WAVES - 1) / WAVES), dim3(64 * WAVES), lds, m->P, a, ns,
-             fp32_copy ? m->q32.as<float>() : nullptr, m->qbf.as<__bf16>(),
-             fp32_copy ? m->pnorm.as<float>() : nullptr);
+  TGX_REQUIRE(m->Dpad % 8 == 0, "FP32 descriptor rows need a multiple of 8 columns (%d)", m->Dpad);
+  float* q32 = fp32_copy ? m->q32.as<float>() : nullptr;
+  float* pn = fp32_copy ? m->pnorm.as<float>() : nullptr;
+  const dim3 grid((ns + WAVES - 1) / WAVES), block(64 * WAVES);
+  switch (m->P.L) {
+#define TGX_DESC_L(ll) \
+  case ll:             \
+    TGX_LAUNCH(ctx, "soap_desc_f32", k_site_desc<ll>, grid, block, lds, m->P, a, ns, q32, m->qbf.as<__bf16>(), pn); \
+    break;
+    TGX_DESC_L(0)
+    TGX_DESC_L(1)
+    TGX_DESC_L(2)
+    TGX_DESC_L(3)
+    TGX_DESC_L(4)
+    TGX_DESC_L(5)
+    TGX_DESC_L(6)
+    TGX_DESC_L(7)
+    TGX_DESC_L(8)
+    TGX_DESC_L(9)
+    TGX_DESC_L(10)
+#undef TGX_DESC_L
+    default:
+      TGX_REQUIRE(false, "FP32 descriptor kernel: l_max %d", m->P.L);
+  }
   return TGX_OK;
 }
 
