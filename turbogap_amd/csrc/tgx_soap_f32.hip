@@ -309,14 +309,19 @@ __global__ __launch_bounds__(PB) void k_pair_basis8(SoapParams P, F32Args a, int
     damping(sp, r, A, dA);
     harmonics_fixed<8>(sh.ynorm, ux, uy, uz, A, W);
   }
+  // rows go out in 16-byte stores: ldb and the column offsets 28 pass and roff
+  // are multiples of 4, and the basis array is 16-byte aligned
+  constexpr int W4 = PB8_W / 4;
+  static_assert(PB8_W % 4 == 0 && (2 * NMAXF) % 4 == 0, "whole 16-byte pieces per pass");
 #pragma unroll
   for (int pass = 0; pass < 3; ++pass) {
 #pragma unroll
     for (int k = 0; k < PB8_W; ++k) row[k] = W[pass * PB8_W + k];
     __syncthreads();
-    for (int e = t; e < nrows * PB8_W; e += PB) {
-      const int rr = e / PB8_W, cc = e - rr * PB8_W;
-      dst[(size_t)rr * a.ldb + pass * PB8_W + cc] = st[rr * PB8_LS + cc];
+    for (int e = t; e < nrows * W4; e += PB) {
+      const int rr = e / W4, c4 = e - rr * W4;
+      const float* sr = st + rr * PB8_LS + 4 * c4;
+      *reinterpret_cast<float4*>(dst + (size_t)rr * a.ldb + pass * PB8_W + 4 * c4) = float4{sr[0], sr[1], sr[2], sr[3]};
     }
     __syncthreads();  // the staging rows are rewritten by the next pass
   }
@@ -332,9 +337,10 @@ __global__ __launch_bounds__(PB) void k_pair_basis8(SoapParams P, F32Args a, int
       }
   }
   __syncthreads();
-  for (int e = t; e < nrows * 2 * NMAXF; e += PB) {
-    const int rr = e / (2 * NMAXF), cc = e - rr * (2 * NMAXF);
-    dst[(size_t)rr * a.ldb + a.roff + cc] = st[rr * PB8_LS + cc];
+  for (int e = t; e < nrows * (NMAXF / 2); e += PB) {
+    const int rr = e / (NMAXF / 2), c4 = e - rr * (NMAXF / 2);
+    const float* sr = st + rr * PB8_LS + 4 * c4;
+    *reinterpret_cast<float4*>(dst + (size_t)rr * a.ldb + a.roff + 4 * c4) = float4{sr[0], sr[1], sr[2], sr[3]};
   }
 }
 
