@@ -311,23 +311,30 @@ __global__ void k_surrogate_out(int n, int K, const double* __restrict__ epart, 
                                 const double* __restrict__ scale, double* __restrict__ e_site, float threshold,
                                 int* __restrict__ stat, int* __restrict__ gated) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const bool in = i < n;
   double s = 0.0, lo = 1e300, hi = -1e300;
-  for (int k = 0; k < K; ++k) {
-    // member k's network output: its second layer's tile partials plus b3
-    const double* pk = epart + ((size_t)k * ldm + i) * nt;
-    double ek = (double)b3[k];
-    for (int t = 0; t < nt; ++t) ek += pk[t];
-    const double v = shift[k] + scale[k] * ek;
-    s += v;
-    lo = v < lo ? v : lo;
-    hi = v > hi ? v : hi;
+  if (in) {
+    for (int k = 0; k < K; ++k) {
+      // member k's network output: its second layer's tile partials plus b3
+      const double* pk = epart + ((size_t)k * ldm + i) * nt;
+      double ek = (double)b3[k];
+      for (int t = 0; t < nt; ++t) ek += pk[t];
+      const double v = shift[k] + scale[k] * ek;
+      s += v;
+      lo = v < lo ? v : lo;
+      hi = v > hi ? v : hi;
+    }
+    e_site[i] = s / K;
   }
-  e_site[i] = s / K;
   if (K > 1) {
-    const float sp = (float)(hi - lo);
-    atomicMax(&stat[1], __float_as_int(sp));
-    if (sp > threshold) gated[atomicAdd(&stat[0], 1)] = i;
+    // the largest spread as the largest bit pattern (spreads are >= 0, so the
+    // integer order is the float order, NaN above all), one atomic per
+    // wavefront instead of one per site on the same word
+    const float sp = in ? (float)(hi - lo) : 0.f;
+    int m = __float_as_int(sp);
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(&stat[1], m);
+    if (in && sp > threshold) gated[atomicAdd(&stat[0], 1)] = i;
   }
 }
 
