@@ -307,6 +307,20 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int K, const __bf16* __restri
   const __bf16* Ag = A + (size_t)(m0 + lr) * lda + lc;
   const __bf16* Bg = B + (size_t)(n0 + lr) * ldb + lc;
   const size_t a32 = (size_t)32 * lda, b32 = (size_t)32 * ldb;
+  // The scaled epilogue's factors are read before the main loop, so that their
+  // latency overlaps it (a short K leaves the epilogue most of the time; the
+  // 64 registers fit in the two waves per SIMD that the LDS allows)
+  float auxv[4][4][4];
+  if constexpr (EPI == EPI_SCALE_AUX) {
+    const int rb = m0 + wm * 64, cb = n0 + wn * 64 + (lane & 15);
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          auxv[ti][tj][r] = e.aux[(size_t)(rb + ti * 16 + tile_row<false>(lane, r)) * e.ldaux + cb + tj * 16];
+  }
   f32x4 acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -363,7 +377,20 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int K, const __bf16* __restri
     lstore(buf ^ 1);
     __syncthreads();
   }
-  epilogue<EPI, OutT, float, f32x4, false, ZINT>(acc, m0, n0, tn, wm, wn, e, sE);
+  if constexpr (EPI == EPI_SCALE_AUX) {
+    OutT* out = reinterpret_cast<OutT*>(e.out);
+    const int rb = m0 + wm * 64, cb = n0 + wn * 64 + (lane & 15);
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          out[(size_t)(rb + ti * 16 + tile_row<false>(lane, r)) * e.ldo + cb + tj * 16] =
+              (OutT)((float)acc[ti][tj][r] * auxv[ti][tj][r] * (float)e.pref);
+  } else {
+    epilogue<EPI, OutT, float, f32x4, false, ZINT>(acc, m0, n0, tn, wm, wn, e, sE);
+  }
 }
 
 // BF16 NT GEMM, workgroup tile 256 x 128: four wavefronts of 128 x 64 (8 x 4
