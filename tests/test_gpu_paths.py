@@ -77,6 +77,25 @@ def test_mixed_path_error_bounds(ctx, orc):
     assert np.abs(W - Wo).max() <= 0.05
 
 
+@pytest.mark.parametrize("n_max,l_max", [(6, 6), (5, 4)])
+def test_mixed_path_other_basis_sizes(ctx, orc, n_max, l_max):
+    """The FP32 path at l_max other than 8 (the descriptor kernel's other template
+    instances, the general adjoint) against the FP64 oracle, with the bounds above."""
+    from oracle import oracle as O
+    sysm = S.carbon216_system()
+    h = S.soap_hypers([S.CARBON], n_max=n_max, l_max=l_max)
+    Qs, al = S.sparse_set(h, 300, O.block_descriptors, S.carbon216_system, seed=5003)
+    ctx.set_system(sysm.positions, sysm.species, sysm.box)
+    ctx.build_neighbors_list(h.rcut_max, fetch=False)
+    ctx.soap_model(h, Qs, al)
+    E, W = ctx.compute("mixed")
+    en, F = ctx.results()
+    Eo, eno, Fo, Wo, _ = orc.compute(sysm, [(h, Qs, al)])
+    assert np.isfinite(F).all()
+    assert np.mean(np.abs(en - eno)) <= 1e-4
+    assert np.sqrt(np.mean((F - Fo) ** 2)) <= 1e-3
+
+
 def test_mixed_path_16k_atoms(ctx):
     """16,384 atoms, 4,000 sparse points per block (the contraction shapes of the
     benchmark's blocks at a quarter of the rows): the energy and force bounds
