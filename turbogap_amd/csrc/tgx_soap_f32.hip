@@ -412,7 +412,11 @@ __device__ __forceinline__ void site_coefficients(const SoapParams& P, const F32
   }
 }
 
-__host__ __device__ inline int desc_wave_floats(const SoapParams& P) { return P.N * P.NLM + P.D; }
+// per wavefront: c [N][NLM], then the spectrum X [D] over it (each a multiple of 4 floats)
+__host__ __device__ inline int desc_wave_floats(const SoapParams& P) {
+  const int c = (P.N * P.NLM + 3) & ~3, x = (P.D + 3) & ~3;
+  return c > x ? c : x;
+}
 constexpr int DESC_NT = 8;  // most lm tiles of 16 of k_site_desc (NLM <= 128)
 constexpr int APASS = 16;  // pair rows staged per pass of the adjoint
 // per wavefront: max(X [D], APASS rows) then the pass's pair forces [APASS][3]
@@ -428,7 +432,7 @@ __host__ __device__ inline int adj_wave_floats(const SoapParams& P, int ldb) { r
 // accumulator out of and back into the accumulation registers on each step, and
 // each guarded load becomes its own branch with a full wait.
 template <int LL>
-__global__ __launch_bounds__(64 * WAVES) void k_site_desc(SoapParams P, F32Args a, int n_sites,
+__global__ __launch_bounds__(64 * WAVES, 5) void k_site_desc(SoapParams P, F32Args a, int n_sites,
                                                           float* __restrict__ q_out, __bf16* __restrict__ qb_out,
                                                           float* __restrict__ pnorm_out) {
   constexpr int L1 = LL + 1, NLM = L1 * L1, NT = (NLM + 15) / 16;
@@ -439,7 +443,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_site_desc(SoapParams P, F32Args 
   if (site >= n_sites) return;
   const int N = P.N;
   float* C = smf + w * desc_wave_floats(P);
-  float* X = C + N * NLM;
+  float* X = C;  // the spectrum overwrites c once every read of c is in registers
   const int64_t b0 = a.site_beg[site];
   const int nn = (int)(a.site_beg[site + 1] - b0);
   const int li = lane & 15, lg = lane >> 4;
@@ -468,7 +472,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_site_desc(SoapParams P, F32Args 
   for (int t = 0; t < NT; ++t) cacc[t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
   const int np1 = nn - 1;
   const float* rows = a.basis + (b0 + 1) * a.ldb;
-  constexpr int KS = 8;  // k-steps per pass (32 rows): all of a pass's loads are issued before its MFMAs
+  constexpr int KS = 4;  // k-steps per pass (16 rows): all of a pass's loads are issued before its MFMAs
   for (int k0 = 0; k0 < np1; k0 += 4 * KS) {
     float av[KS], bv[KS][NT];
 #pragma unroll
@@ -515,17 +519,23 @@ __global__ __launch_bounds__(64 * WAVES) void k_site_desc(SoapParams P, F32Args 
     xw[r] = n == np ? 1.f : (float)M_SQRT2;
   }
   const float* Cn = C + min(li, N - 1) * NLM;
+  // every k-step's column of c, read before X overwrites c
+  float cv[LL + 1][(2 * LL + 4) / 4];
+#pragma unroll
+  for (int l = 0; l <= LL; ++l)
+#pragma unroll
+    for (int q = 0; 4 * q < 2 * l + 1; ++q) {
+      const int k = 4 * q + lg;
+      const float v = Cn[l * l + min(k, 2 * l)];
+      cv[l][q] = li < N && k < 2 * l + 1 ? v : 0.f;
+    }
+  wave_sync();
   float acc = 0.f;
 #pragma unroll
   for (int l = 0; l <= LL; ++l) {
-    const int K = 2 * l + 1;
     f32x4_t g = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int q = 0; 4 * q < K; ++q) {
-      const int k = 4 * q + lg;
-      const float v = Cn[l * l + min(k, 2 * l)];
-      g = __builtin_amdgcn_mfma_f32_16x16x4f32(li < N && k < K ? v : 0.f, li < N && k < K ? v : 0.f, g, 0, 0, 0);
-    }
+    for (int q = 0; 4 * q < 2 * l + 1; ++q) g = __builtin_amdgcn_mfma_f32_16x16x4f32(cv[l][q], cv[l][q], g, 0, 0, 0);
 #pragma unroll
     for (int r = 0; r < 4; ++r)
       if (xv[r]) {
